@@ -1,0 +1,83 @@
+"""Closed-form graphs and subspace metrics shared by the tests."""
+from __future__ import annotations
+
+import numpy as np
+
+
+def ring_graph(n: int):
+    """C_n adjacency, unit weights, sorted columns (tests/common/mod.rs:191-207)."""
+    indptr = np.arange(0, 2 * n + 1, 2, dtype=np.int64)
+    idx = np.empty(2 * n, np.int32)
+    for i in range(n):
+        a, b = (i - 1) % n, (i + 1) % n
+        idx[2 * i], idx[2 * i + 1] = min(a, b), max(a, b)
+    return indptr, idx, np.ones(2 * n, np.float32)
+
+
+def path_graph(n: int):
+    """P_n adjacency, unit weights (tests/common/mod.rs:211-228)."""
+    rows = []
+    for i in range(n):
+        nb = [j for j in (i - 1, i + 1) if 0 <= j < n]
+        rows.append(nb)
+    indptr = np.zeros(n + 1, np.int64)
+    indptr[1:] = np.cumsum([len(r) for r in rows])
+    idx = np.array([j for r in rows for j in r], np.int32)
+    return indptr, idx, np.ones(idx.size, np.float32)
+
+
+def ring_eigenvalues(n: int, k: int):
+    """Normalized Laplacian of C_n: 1 - cos(2 pi j / n), ascending (with multiplicity)."""
+    lam = np.sort(1.0 - np.cos(2.0 * np.pi * np.arange(n) / n))
+    return lam[:k]
+
+
+def random_symmetric_graph(n: int, avg_deg: int, seed: int):
+    """Random symmetric weighted graph with sorted CSR rows and no self loops."""
+    rng = np.random.default_rng(seed)
+    m = n * avg_deg // 2
+    r = rng.integers(0, n, m)
+    c = rng.integers(0, n, m)
+    keep = r != c
+    r, c = r[keep], c[keep]
+    w = rng.uniform(0.05, 1.0, r.size).astype(np.float32)
+    import scipy.sparse as sp
+    a = sp.coo_matrix((w, (r, c)), shape=(n, n)).tocsr()
+    a = a.maximum(a.T).tocsr()
+    # make sure the graph is connected: add a ring with small weight
+    ring = sp.coo_matrix((np.full(n, 0.01, np.float32), (np.arange(n), (np.arange(n) + 1) % n)),
+                         shape=(n, n))
+    a = (a + ring + ring.T).tocsr()
+    a.sort_indices()
+    a.data = a.data.astype(np.float32)
+    return a.indptr.astype(np.int64), a.indices.astype(np.int32), a.data
+
+
+def csr_to_dense(indptr, indices, data, n):
+    d = np.zeros((n, n))
+    for i in range(n):
+        for p in range(indptr[i], indptr[i + 1]):
+            d[i, indices[p]] += data[p]
+    return d
+
+
+def orthonormal(x):
+    q, _ = np.linalg.qr(x)
+    return q
+
+
+def principal_angles(a, b):
+    """Principal angles (radians, descending) between span(a) and span(b)."""
+    qa, qb = orthonormal(np.asarray(a, float)), orthonormal(np.asarray(b, float))
+    s = np.linalg.svd(qa.T @ qb, compute_uv=False)
+    s = np.clip(s, -1.0, 1.0)
+    return np.sort(np.arccos(s))[::-1]
+
+
+def max_subspace_angle(a, b):
+    return float(principal_angles(a, b)[0])
+
+
+def residuals_dense(lmat, evals, evecs):
+    r = lmat @ evecs - evecs * evals[None, :]
+    return np.linalg.norm(r, axis=0) / np.maximum(np.linalg.norm(evecs, axis=0), 1e-300)
