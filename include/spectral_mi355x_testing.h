@@ -1,0 +1,43 @@
+/*
+ * spectral_mi355x_testing.h -- kernel-level seams for parity tests.
+ *
+ * The reference exposes testing-only internals behind its `testing` cargo
+ * feature (src/lib.rs:27-162; e.g. spmv_csr as the fixed-signature SIMD
+ * replacement point, src/operator.rs:16-32).  These entry points are the
+ * device counterparts: they drive one kernel family at a time so the tests can
+ * compare it with the CPU oracle.  Not part of the stable ABI.
+ */
+#ifndef SPECTRAL_MI355X_TESTING_H
+#define SPECTRAL_MI355X_TESTING_H
+
+#include "spectral_mi355x.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* G = A^T B with FP64 accumulation; a, b device [n x lda] / [n x ldb]; g host [p x q]. */
+si_status si_test_gram(si_context* ctx, int32_t precision, int64_t n, const void* a, int32_t lda,
+                       int32_t p, const void* b, int32_t ldb, int32_t q, double* g_host);
+/* out = A C; a, out device; c host [p x q] FP64. */
+si_status si_test_block_mul(si_context* ctx, int32_t precision, int64_t n, const void* a,
+                            int32_t lda, int32_t p, const double* c_host, int32_t q, void* out,
+                            int32_t ldo);
+/* one-workgroup Jacobi eigen-decomposition (host in / out, n <= 64). */
+si_status si_test_small_sym_eig(si_context* ctx, int32_t n, const double* a_host, double* w_host,
+                                double* v_host);
+/* one-workgroup generalised eigenproblem A c = theta B c (host in / out). */
+si_status si_test_small_gen_eig(si_context* ctx, int32_t n, const double* a_host,
+                                const double* b_host, double* w_host, double* c_host);
+/* Philox4x32-10 / Box-Muller normals into a device block [n x k]. */
+si_status si_test_fill_normal(si_context* ctx, double* x, int64_t n, int32_t k, uint64_t seed,
+                              uint32_t stream);
+/* the FP32-bulk Chebyshev-filtered block solver alone (escalation level 1). */
+si_status si_test_fast_solve(si_context* ctx, const si_laplacian* lap, int32_t n_components,
+                             uint64_t seed, const si_config* cfg, double* evals, double* evecs,
+                             si_report* rep);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,583 @@
+// api.cpp -- the C ABI (include/spectral_mi355x.h) over the device solvers,
+// plus the spectral_init pipeline around them (src/lib.rs:198-258).
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <numeric>
+
+#include "si_abi.hpp"
+
+namespace {
+
+template <typename F>
+si_status guard(si_context* ctx, F&& f) {
+    try {
+        if (ctx) SI_HIP(hipSetDevice(ctx->c.device));
+        f();
+        if (ctx) ctx->c.last_error.clear();
+        return SI_OK;
+    } catch (const si::Error& e) {
+        if (ctx) ctx->c.last_error = e.what();
+        return (si_status)e.code;
+    } catch (const std::bad_alloc&) {
+        if (ctx) ctx->c.last_error = "host allocation failed";
+        return SI_ERR_INTERNAL;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->c.last_error = e.what();
+        return SI_ERR_INTERNAL;
+    }
+}
+
+void fill_report(si_report* rep, const si::SolveStats& st, const double* ev, int k, double ms) {
+    if (!rep) return;
+    rep->level = st.level;
+    rep->restarts = st.restarts;
+    rep->block_size = st.block_size;
+    rep->iterations = st.iterations;
+    rep->spmm_calls = st.spmm_calls;
+    rep->max_residual = st.max_residual;
+    for (int i = 0; i < 8; ++i) rep->eigenvalues[i] = i < k ? ev[i] : 0.0;
+    rep->solve_ms = ms;
+}
+
+si::ChainOptions chain_options(const si_config* cfg) {
+    si::ChainOptions o;
+    si_config d;
+    si_config_default(&d);
+    if (!cfg) cfg = &d;
+    o.solver = cfg->solver;
+    o.tol = cfg->tol > 0 ? cfg->tol : 1e-5;
+    o.dense_threshold = cfg->dense_threshold >= 0 ? cfg->dense_threshold : 2000;
+    o.fast.block_size = cfg->block_size;
+    o.fast.cheb_degree = cfg->cheb_degree > 0 ? cfg->cheb_degree : 8;
+    o.fast.max_outer = cfg->max_outer > 0 ? cfg->max_outer : 60;
+    o.fast.warm_start = cfg->warm_start;
+    o.fast.tol = o.tol;
+    return o;
+}
+
+// ---------------------------------------------------------------- host helpers
+// find_components (src/components.rs:7-44): BFS labelling in node order.
+int64_t find_components(int64_t n, const int64_t* ip, const int32_t* ix, std::vector<int64_t>& lab) {
+    lab.assign(n, -1);
+    int64_t nc = 0;
+    std::vector<int64_t> q;
+    q.reserve(n);
+    for (int64_t s0 = 0; s0 < n; ++s0) {
+        if (lab[s0] >= 0) continue;
+        q.clear();
+        lab[s0] = nc;
+        q.push_back(s0);
+        for (size_t h = 0; h < q.size(); ++h) {
+            const int64_t u = q[h];
+            for (int64_t p = ip[u]; p < ip[u + 1]; ++p) {
+                const int64_t v = ix[p];
+                if (lab[v] < 0) { lab[v] = nc; q.push_back(v); }
+            }
+        }
+        ++nc;
+    }
+    return nc;
+}
+
+// select_eigenvectors (src/selection.rs:11-29) + normalize_signs
+// (src/scaling.rs:62-74) on host copies of the eigenvectors.
+void select_and_sign(const std::vector<double>& ev, const std::vector<double>& vec, int64_t n,
+                     int k, int d, std::vector<double>& out) {
+    std::vector<int> idx(k);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return ev[a] < ev[b]; });
+    out.assign((size_t)n * d, 0.0);
+    for (int64_t r = 0; r < n; ++r)
+        for (int c = 0; c < d; ++c) out[(size_t)r * d + c] = vec[(size_t)r * k + idx[1 + c]];
+    for (int c = 0; c < d; ++c) {
+        double best = out[c];
+        for (int64_t r = 1; r < n; ++r) {
+            const double b = out[(size_t)r * d + c];
+            if (std::fabs(b) > std::fabs(best)) best = b;
+        }
+        if (std::signbit(best))
+            for (int64_t r = 0; r < n; ++r) out[(size_t)r * d + c] = -out[(size_t)r * d + c];
+    }
+}
+
+struct DeviceGraph {
+    si::DevBuf<int64_t> ip;
+    si::DevBuf<int32_t> ix;
+    si::DevBuf<float> w;
+};
+
+void upload_graph(DeviceGraph& g, int64_t n, const int64_t* ip, const int32_t* ix, const float* w,
+                  hipStream_t s) {
+    const int64_t nnz = ip[n];
+    g.ip.alloc(n + 1);
+    g.ix.alloc(nnz);
+    g.w.alloc(nnz);
+    SI_HIP(hipMemcpyAsync(g.ip.get(), ip, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, s));
+    if (nnz > 0) {
+        SI_HIP(hipMemcpyAsync(g.ix.get(), ix, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s));
+        SI_HIP(hipMemcpyAsync(g.w.get(), w, sizeof(float) * nnz, hipMemcpyHostToDevice, s));
+    }
+    SI_HIP(hipStreamSynchronize(s));
+}
+
+void check_host_csr(int64_t n, const int64_t* ip) {
+    SI_CHECK(ip != nullptr, si::kErrInvalidArgument, "indptr is null");
+    SI_CHECK(ip[0] == 0, si::kErrInvalidGraph, "invalid graph: indptr[0] must be 0");
+    for (int64_t i = 0; i < n; ++i)
+        SI_CHECK(ip[i + 1] >= ip[i], si::kErrInvalidGraph, "invalid graph: indptr must be non-decreasing");
+}
+
+// Solve one connected graph: build L, run the chain, return host eigenpairs.
+int solve_graph(si::Context& c, int64_t n, const int64_t* ip, const int32_t* ix, const float* w,
+                int n_components, uint64_t seed, int mode, const si::ChainOptions& opt,
+                std::vector<double>& ev, std::vector<double>& vec, si::SolveStats& st,
+                double* lap_ms, double* solve_ms) {
+    hipStream_t s = c.stream;
+    DeviceGraph g;
+    upload_graph(g, n, ip, ix, w, s);
+    si::DevLaplacian L;
+    {
+        si::EventTimer t(s);
+        si::laplacian_build_device(L, n, g.ip, g.ix, g.w, ip[n], mode, s);
+        const double ms = t.stop_ms();
+        if (lap_ms) *lap_ms += ms;
+    }
+    const int k = n_components + 1;
+    ev.assign(k, 0.0);
+    si::DevBuf<double> dv((size_t)n * k);
+    int level = -1;
+    try {
+        si::EventTimer t(s);
+        level = si::solve_eigenproblem_device(L, n_components, seed, opt, ev.data(), dv, st, s);
+        const double ms = t.stop_ms();
+        if (solve_ms) *solve_ms += ms;
+    } catch (...) {
+        si::laplacian_free(L);
+        throw;
+    }
+    vec.resize((size_t)n * k);
+    si::to_host(vec.data(), dv.get(), vec.size(), s);
+    si::laplacian_free(L);
+    return level;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* si_version(void) { return "spectral-mi355x 0.1 (gfx950)"; }
+
+void si_config_default(si_config* cfg) {
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->struct_size = sizeof(si_config);
+    cfg->compute_mode = SI_COMPUTE_PYTHON_COMPAT;
+    cfg->solver = SI_SOLVER_FAST;
+    cfg->block_size = 0;
+    cfg->cheb_degree = 8;
+    cfg->max_outer = 60;
+    cfg->tol = 1e-5;
+    cfg->dense_threshold = 2000;
+    cfg->warm_start = nullptr;
+}
+
+si_status si_context_create(int device, si_context** out) {
+    if (!out) return SI_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    si_context* ctx = nullptr;
+    try {
+        ctx = new si_context();
+    } catch (...) {
+        return SI_ERR_INTERNAL;
+    }
+    ctx->c.device = device;
+    si_status st = guard(nullptr, [&] {
+        int count = 0;
+        SI_HIP(hipGetDeviceCount(&count));
+        SI_CHECK(device >= 0 && device < count, si::kErrInvalidArgument, "no such HIP device");
+        SI_HIP(hipSetDevice(device));
+        SI_HIP(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
+        hipDeviceProp_t prop;
+        SI_HIP(hipGetDeviceProperties(&prop, device));
+        ctx->c.num_cus = prop.multiProcessorCount;
+    });
+    if (st != SI_OK) {
+        delete ctx;
+        return st;
+    }
+    *out = ctx;
+    return SI_OK;
+}
+
+void si_context_destroy(si_context* ctx) {
+    if (!ctx) return;
+    if (ctx->c.stream) {
+        hipSetDevice(ctx->c.device);
+        hipStreamSynchronize(ctx->c.stream);
+        hipStreamDestroy(ctx->c.stream);
+    }
+    delete ctx;
+}
+
+const char* si_last_error(const si_context* ctx) {
+    return ctx ? ctx->c.last_error.c_str() : "null context";
+}
+
+si_status si_laplacian_build(si_context* ctx, int64_t n, const int64_t* indptr,
+                             const int32_t* indices, const float* data, int input_on_device,
+                             int32_t compute_mode, si_laplacian** out) {
+    if (!ctx || !out) return SI_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    return guard(ctx, [&] {
+        SI_CHECK(n > 0, si::kErrInvalidGraph, "invalid graph: n must be > 0");
+        SI_CHECK(compute_mode == 0 || compute_mode == 1, si::kErrInvalidArgument, "unknown compute mode");
+        hipStream_t s = ctx->c.stream;
+        std::unique_ptr<si_laplacian> lap(new si_laplacian());
+        lap->device = ctx->c.device;
+        if (input_on_device) {
+            int64_t nnz = 0;
+            SI_HIP(hipMemcpyAsync(&nnz, indptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+            SI_HIP(hipStreamSynchronize(s));
+            si::laplacian_build_device(lap->L, n, indptr, indices, data, nnz, compute_mode, s);
+        } else {
+            check_host_csr(n, indptr);
+            DeviceGraph g;
+            upload_graph(g, n, indptr, indices, data, s);
+            si::laplacian_build_device(lap->L, n, g.ip, g.ix, g.w, indptr[n], compute_mode, s);
+        }
+        *out = lap.release();
+    });
+}
+
+void si_laplacian_destroy(si_laplacian* lap) {
+    if (!lap) return;
+    hipSetDevice(lap->device);
+    si::laplacian_free(lap->L);
+    delete lap;
+}
+
+si_status si_laplacian_info(const si_laplacian* lap, int64_t* n, int64_t* nnz, int64_t* n_long,
+                            int32_t* degree_exact) {
+    if (!lap) return SI_ERR_INVALID_ARGUMENT;
+    if (n) *n = lap->L.n;
+    if (nnz) *nnz = lap->L.nnz;
+    if (n_long) *n_long = lap->L.n_long;
+    if (degree_exact) *degree_exact = lap->L.degree_exact;
+    return SI_OK;
+}
+
+si_status si_laplacian_export(si_context* ctx, const si_laplacian* lap, double* degrees,
+                              double* sqrt_deg, int64_t* l_indptr, int32_t* l_indices,
+                              double* l_data, int64_t* nnz_l) {
+    if (!ctx || !lap) return SI_ERR_INVALID_ARGUMENT;
+    return guard(ctx, [&] {
+        hipStream_t s = ctx->c.stream;
+        const si::DevLaplacian& L = lap->L;
+        const int64_t n = L.n;
+        if (degrees) si::to_host(degrees, L.deg, n, s);
+        if (sqrt_deg) si::to_host(sqrt_deg, L.sqrt_deg, n, s);
+        std::vector<int32_t> rp(n + 1), col(L.nnz);
+        std::vector<double> val(L.nnz);
+        si::to_host(rp.data(), L.rowptr, n + 1, s);
+        if (L.nnz) {
+            si::to_host(col.data(), L.col, L.nnz, s);
+            si::to_host(val.data(), L.l64, L.nnz, s);
+        }
+        // reference layout: off-diagonal entries plus the unit diagonal, ascending columns
+        int64_t o = 0;
+        if (l_indptr) l_indptr[0] = 0;
+        for (int64_t i = 0; i < n; ++i) {
+            bool diag_done = false;
+            for (int32_t p = rp[i]; p < rp[i + 1]; ++p) {
+                const int32_t j = col[p];
+                if (j == i) continue;
+                if (!diag_done && j > i) {
+                    if (l_indices) l_indices[o] = (int32_t)i;
+                    if (l_data) l_data[o] = 1.0;
+                    ++o;
+                    diag_done = true;
+                }
+                if (l_indices) l_indices[o] = j;
+                if (l_data) l_data[o] = val[p];
+                ++o;
+            }
+            if (!diag_done) {
+                if (l_indices) l_indices[o] = (int32_t)i;
+                if (l_data) l_data[o] = 1.0;
+                ++o;
+            }
+            if (l_indptr) l_indptr[i + 1] = o;
+        }
+        if (nnz_l) *nnz_l = o;
+    });
+}
+
+si_status si_spmm(si_context* ctx, const si_laplacian* lap, int32_t precision, const void* x,
+                  void* y, int32_t k, int32_t ld) {
+    if (!ctx || !lap || !x || !y) return SI_ERR_INVALID_ARGUMENT;
+    return guard(ctx, [&] {
+        SI_CHECK(si::spmm_width_supported(k), si::kErrInvalidArgument, "spmm: unsupported width");
+        if (precision == SI_F32)
+            si::spmm_f32(lap->L, (const float*)x, (float*)y, k, ld, 1.f, 0.f, 0.f, nullptr, ctx->c.stream);
+        else
+            si::spmm_f64(lap->L, (const double*)x, (double*)y, k, ld, 1.0, 0.0, 0.0, nullptr, ctx->c.stream);
+        SI_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
+si_status si_chebyshev_filter(si_context* ctx, const si_laplacian* lap, const double* r,
+                              int32_t k, double a, double b, int32_t degree, double* out) {
+    if (!ctx || !lap || !r || !out) return SI_ERR_INVALID_ARGUMENT;
+    return guard(ctx, [&] {
+        SI_CHECK(si::spmm_width_supported(k), si::kErrInvalidArgument, "chebyshev: unsupported width");
+        SI_CHECK(b > a, si::kErrInvalidArgument, "chebyshev: need b > a");
+        si::chebyshev_filter_f64(lap->L, r, k, a, b, degree, out, ctx->c.stream);
+    });
+}
+
+si_status si_max_eigenpair_residual(si_context* ctx, const si_laplacian* lap,
+                                    const double* evals, const double* evecs, int32_t k,
+                                    double* out) {
+    if (!ctx || !lap || !evals || !evecs || !out) return SI_ERR_INVALID_ARGUMENT;
+    return guard(ctx, [&] {
+        SI_CHECK(si::spmm_width_supported(k), si::kErrInvalidArgument, "residual: unsupported width");
+        std::vector<double> res(k);
+        si::fp64_residuals(lap->L, evecs, k, evals, res.data(), ctx->c.stream);
+        double q = 0.0;
+        for (double r : res) q = (std::isnan(q) || std::isnan(r)) ? NAN : std::max(q, r);
+        *out = q;
+    });
+}
+
+si_status si_lobpcg_solve(si_context* ctx, const si_laplacian* lap, int32_t n_components,
+                          uint64_t seed, int32_t regularize, double tol, double* evals,
+                          double* evecs, si_report* rep) {
+    if (!ctx || !lap || !evals || !evecs) return SI_ERR_INVALID_ARGUMENT;
+    return guard(ctx, [&] {
+        si::SolveStats st;
+        si::EventTimer t(ctx->c.stream);
+        const bool ok = si::lobpcg_solve_plain(lap->L, n_components, seed, regularize != 0,
+                                               tol > 0 ? tol : 1e-5, evals, evecs, st,
+                                               ctx->c.stream);
+        const double ms = t.stop_ms();
+        if (ok) {
+            std::vector<double> res(n_components + 1);
+            si::fp64_residuals(lap->L, evecs, n_components + 1, evals, res.data(), ctx->c.stream);
+            st.max_residual = *std::max_element(res.begin(), res.end());
+        }
+        st.level = 1;
+        fill_report(rep, st, evals, n_components + 1, ms);
+        SI_CHECK(ok, si::kErrConvergence, "lobpcg_solve produced no result");
+    });
+}
+
+si_status si_solve_eigenproblem(si_context* ctx, const si_laplacian* lap, int32_t n_components,
+                                uint64_t seed, const si_config* cfg, double* evals,
+                                double* evecs, si_report* rep) {
+    if (!ctx || !lap || !evals || !evecs) return SI_ERR_INVALID_ARGUMENT;
+    return guard(ctx, [&] {
+        SI_CHECK(n_components > 0 && (int64_t)n_components < lap->L.n, si::kErrTooFewNodes,
+                 "graph has too few nodes for the requested embedding");
+        SI_CHECK(n_components + 1 <= 64, si::kErrUnsupported, "n_components above 63 is not supported");
+        si::ChainOptions o = chain_options(cfg);
+        si::SolveStats st;
+        si::EventTimer t(ctx->c.stream);
+        si::solve_eigenproblem_device(lap->L, n_components, seed, o, evals, evecs, st, ctx->c.stream);
+        const double ms = t.stop_ms();
+        fill_report(rep, st, evals, n_components + 1, ms);
+    });
+}
+
+si_status si_rsvd_solve(si_context* ctx, const si_laplacian* lap, int32_t n_components,
+                        uint64_t seed, int32_t oversampling, int32_t nbiter, double* evals,
+                        double* evecs, si_report* rep) {
+    if (!ctx || !lap || !evals || !evecs) return SI_ERR_INVALID_ARGUMENT;
+    return guard(ctx, [&] {
+        si::SolveStats st;
+        si::EventTimer t(ctx->c.stream);
+        si::rsvd_solve_device(lap->L, n_components, seed, oversampling, nbiter, evals, evecs, st,
+                              ctx->c.stream);
+        const double ms = t.stop_ms();
+        std::vector<double> res(n_components + 1);
+        si::fp64_residuals(lap->L, evecs, n_components + 1, evals, res.data(), ctx->c.stream);
+        st.max_residual = *std::max_element(res.begin(), res.end());
+        st.level = 4;
+        fill_report(rep, st, evals, n_components + 1, ms);
+    });
+}
+
+si_status si_dense_evd(si_context* ctx, const si_laplacian* lap, int32_t k, double* evals,
+                       double* evecs) {
+    if (!ctx || !lap || !evals || !evecs) return SI_ERR_INVALID_ARGUMENT;
+    return guard(ctx, [&] { si::dense_evd_small(lap->L, k, evals, evecs, ctx->c.stream); });
+}
+
+si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
+                           const int32_t* indices, const float* data, int32_t n_components,
+                           uint64_t seed, const float* raw_data, int64_t n_features,
+                           const si_config* cfg, float* out, si_report* rep) {
+    if (!ctx || !out) return SI_ERR_INVALID_ARGUMENT;
+    return guard(ctx, [&] {
+        // input validation (lib.rs:206-214)
+        if (n == 0 || n_components == 0)
+            throw si::Error(si::kErrInvalidGraph,
+                            "invalid graph: spectral_init requires n > 0 and n_components > 0; got n=" +
+                                std::to_string(n) + ", n_components=" + std::to_string(n_components));
+        SI_CHECK(n > 0 && n_components > 0, si::kErrInvalidArgument, "negative sizes");
+        if (n <= n_components)
+            throw si::Error(si::kErrTooFewNodes, "graph has too few nodes (" + std::to_string(n) +
+                                                     ") for " + std::to_string(n_components) +
+                                                     "-dimensional embedding");
+        SI_CHECK(n_components + 1 <= 64, si::kErrUnsupported, "n_components above 63 is not supported");
+        check_host_csr(n, indptr);
+        const int mode = cfg ? cfg->compute_mode : 0;
+        si::ChainOptions opt = chain_options(cfg);
+        opt.fast.warm_start = nullptr;  // warm starts are per-graph device blocks
+        const int d = n_components;
+        const int k = d + 1;
+        std::vector<int64_t> labels;
+        const int64_t nc = find_components(n, indptr, indices, labels);
+        std::vector<double> coords((size_t)n * d, 0.0);
+        si::SolveStats last;
+        double lap_ms = 0.0, solve_ms = 0.0;
+        std::vector<double> ev, vec, sel;
+        if (nc == 1) {
+            last.level = solve_graph(ctx->c, n, indptr, indices, data, d, seed, mode, opt, ev, vec,
+                                     last, &lap_ms, &solve_ms);
+            select_and_sign(ev, vec, n, k, d, coords);
+        } else {
+            // embed_disconnected (src/multi_component.rs:16-74)
+            std::vector<std::vector<int64_t>> members(nc);
+            for (int64_t i = 0; i < n; ++i) members[labels[i]].push_back(i);
+            std::vector<double> meta((size_t)nc * d, 0.0);
+            if (nc > 2 * d) {
+                if (!raw_data)
+                    throw si::Error(si::kErrInvalidGraph,
+                                    "invalid graph: data is required for spectral meta-embedding when "
+                                    "n_conn_components > 2 * n_embedding_dims");
+                // spectral_meta_embedding (multi_component.rs:143-220)
+                std::vector<double> cen((size_t)nc * n_features, 0.0);
+                for (int64_t c = 0; c < nc; ++c) {
+                    for (int64_t node : members[c])
+                        for (int64_t f = 0; f < n_features; ++f)
+                            cen[c * n_features + f] += (double)raw_data[node * n_features + f];
+                    const double inv = 1.0 / (double)members[c].size();
+                    for (int64_t f = 0; f < n_features; ++f) cen[c * n_features + f] *= inv;
+                }
+                // the centroid affinity graph as a dense f32 CSR input, zero diagonal
+                std::vector<int64_t> mip(nc + 1);
+                std::vector<int32_t> mix;
+                std::vector<float> mw;
+                std::vector<double> aff((size_t)nc * nc);
+                for (int64_t i = 0; i < nc; ++i)
+                    for (int64_t j = 0; j < nc; ++j) {
+                        double d2 = 0.0;
+                        for (int64_t f = 0; f < n_features; ++f) {
+                            const double df = cen[i * n_features + f] - cen[j * n_features + f];
+                            d2 += df * df;
+                        }
+                        aff[i * nc + j] = std::exp(-d2);
+                    }
+                // The reference builds this Laplacian in f64 directly; the device
+                // operator takes f32 weights, so the affinities are rounded to f32
+                // here (relative 6e-8) before the normalisation.
+                mip[0] = 0;
+                for (int64_t i = 0; i < nc; ++i) {
+                    for (int64_t j = 0; j < nc; ++j)
+                        if (j != i) { mix.push_back((int32_t)j); mw.push_back((float)aff[i * nc + j]); }
+                    mip[i + 1] = (int64_t)mix.size();
+                }
+                si::ChainOptions mo = opt;
+                si::SolveStats mst;
+                solve_graph(ctx->c, nc, mip.data(), mix.data(), mw.data(), d, seed, 1, mo, ev, vec,
+                            mst, nullptr, nullptr);
+                std::vector<int> idx(k);
+                std::iota(idx.begin(), idx.end(), 0);
+                std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return ev[a] < ev[b]; });
+                double mx = 0.0;
+                for (int64_t r = 0; r < nc; ++r)
+                    for (int c = 0; c < d; ++c) {
+                        meta[r * d + c] = vec[(size_t)r * k + idx[1 + c]];
+                        mx = std::max(mx, std::fabs(meta[r * d + c]));
+                    }
+                if (mx == 0.0)
+                    throw si::Error(si::kErrInvalidGraph, "invalid graph: spectral meta-embedding produced all-zero result");
+                for (double& v : meta) v /= mx;
+            } else {
+                // orthogonal_meta_embedding (multi_component.rs:127-139)
+                for (int64_t r = 0; r < nc; ++r) meta[r * d + r % d] = r < d ? 1.0 : -1.0;
+            }
+            // compute_data_range (multi_component.rs:224-255)
+            std::vector<double> range(nc, 1.0);
+            for (int64_t c = 0; c < nc && nc > 1; ++c) {
+                double md = INFINITY;
+                for (int64_t i = 0; i < nc; ++i) {
+                    if (i == c) continue;
+                    double s2 = 0.0;
+                    for (int q = 0; q < d; ++q) {
+                        const double df = meta[c * d + q] - meta[i * d + q];
+                        s2 += df * df;
+                    }
+                    const double dist = std::sqrt(s2);
+                    if (dist > 0.0 && dist < md) md = dist;
+                }
+                range[c] = std::isfinite(md) ? md / 2.0 : 1.0;
+            }
+            // embed_single_component (multi_component.rs:261-312)
+            std::vector<int64_t> lookup(n, -1);
+            for (int64_t c = 0; c < nc; ++c) {
+                const std::vector<int64_t>& mem = members[c];
+                const int64_t sz = (int64_t)mem.size();
+                if (sz < 2 * d) {
+                    for (int64_t node : mem)
+                        for (int q = 0; q < d; ++q) coords[node * d + q] = meta[c * d + q];
+                    continue;
+                }
+                for (int64_t q = 0; q < sz; ++q) lookup[mem[q]] = q;
+                std::vector<int64_t> sip(sz + 1, 0);
+                std::vector<int32_t> six;
+                std::vector<float> sw;
+                for (int64_t q = 0; q < sz; ++q) {
+                    const int64_t u = mem[q];
+                    for (int64_t p = indptr[u]; p < indptr[u + 1]; ++p)
+                        if (lookup[indices[p]] >= 0) {
+                            six.push_back((int32_t)lookup[indices[p]]);
+                            sw.push_back(data[p]);
+                        }
+                    sip[q + 1] = (int64_t)six.size();
+                }
+                si::SolveStats cst;
+                last.level = solve_graph(ctx->c, sz, sip.data(), six.data(), sw.data(), d, seed, mode,
+                                         opt, ev, vec, cst, &lap_ms, &solve_ms);
+                last.iterations += cst.iterations;
+                last.spmm_calls += cst.spmm_calls;
+                last.max_residual = std::max(last.max_residual, cst.max_residual);
+                select_and_sign(ev, vec, sz, k, d, sel);
+                double mx = 0.0;
+                for (double v : sel) mx = std::max(mx, std::fabs(v));
+                const double f = mx > 0.0 ? range[c] / mx : 1.0;
+                for (int64_t q = 0; q < sz; ++q)
+                    for (int e = 0; e < d; ++e)
+                        coords[mem[q] * d + e] = sel[(size_t)q * d + e] * f + meta[c * d + e];
+                for (int64_t node : mem) lookup[node] = -1;
+            }
+        }
+        // scale_and_add_noise (src/scaling.rs:78-89): max-abs 10, N(0, 1e-4) f32 noise
+        double mx = 0.0;
+        for (double v : coords) mx = std::max(mx, std::fabs(v));
+        if (!(mx > 0.0))
+            throw si::Error(si::kErrInvalidGraph, "invalid graph: scale_coords: embedding is all-zero; cannot scale to max=10");
+        const double ex = 10.0 / mx;
+        for (size_t i = 0; i < coords.size(); ++i) {
+            const float noise = (float)(1e-4 * si::normal_at(seed, si::kStreamNoise, (uint64_t)i));
+            out[i] = (float)(coords[i] * ex) + noise;
+        }
+        if (rep) {
+            fill_report(rep, last, ev.empty() ? nullptr : ev.data(), ev.empty() ? 0 : (int)ev.size(), solve_ms);
+            rep->laplacian_ms = lap_ms;
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,132 @@
+// api_testing.cpp -- kernel-level seams (include/spectral_mi355x_testing.h),
+// the counterpart of the reference's `testing`-feature re-exports
+// (src/lib.rs:27-162, src/operator.rs:16-32): they let the parity tests drive
+// single device kernels against the CPU oracle.
+#include <cmath>
+#include <cstring>
+
+#include "si_abi.hpp"
+#include "../../include/spectral_mi355x_testing.h"
+
+namespace {
+template <typename F>
+si_status tguard(si_context* ctx, F&& f) {
+    try {
+        if (ctx) SI_HIP(hipSetDevice(ctx->c.device));
+        f();
+        return SI_OK;
+    } catch (const si::Error& e) {
+        if (ctx) ctx->c.last_error = e.what();
+        return (si_status)e.code;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->c.last_error = e.what();
+        return SI_ERR_INTERNAL;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+si_status si_test_gram(si_context* ctx, int32_t precision, int64_t n, const void* a, int32_t lda,
+                       int32_t p, const void* b, int32_t ldb, int32_t q, double* g_host) {
+    if (!ctx || !a || !b || !g_host) return SI_ERR_INVALID_ARGUMENT;
+    return tguard(ctx, [&] {
+        hipStream_t s = ctx->c.stream;
+        si::DevBuf<double> g((size_t)p * q), ws(si::gram_workspace_doubles(n, p, q));
+        if (precision == SI_F32)
+            si::gram_f32(n, (const float*)a, lda, p, (const float*)b, ldb, q, g, ws, s);
+        else
+            si::gram_f64(n, (const double*)a, lda, p, (const double*)b, ldb, q, g, ws, s);
+        si::to_host(g_host, g.get(), (size_t)p * q, s);
+    });
+}
+
+si_status si_test_block_mul(si_context* ctx, int32_t precision, int64_t n, const void* a,
+                            int32_t lda, int32_t p, const double* c_host, int32_t q, void* out,
+                            int32_t ldo) {
+    if (!ctx || !a || !c_host || !out) return SI_ERR_INVALID_ARGUMENT;
+    return tguard(ctx, [&] {
+        hipStream_t s = ctx->c.stream;
+        si::DevBuf<double> c((size_t)p * q);
+        si::to_device(c.get(), c_host, (size_t)p * q, s);
+        if (precision == SI_F32)
+            si::block_mul_f32(n, (const float*)a, lda, p, c, q, nullptr, 0, 0, nullptr, (float*)out, ldo, s);
+        else
+            si::block_mul_f64(n, (const double*)a, lda, p, c, q, nullptr, 0, 0, nullptr, (double*)out, ldo, s);
+        SI_HIP(hipStreamSynchronize(s));
+    });
+}
+
+si_status si_test_small_sym_eig(si_context* ctx, int32_t n, const double* a_host, double* w_host,
+                                double* v_host) {
+    if (!ctx || !a_host || !w_host || !v_host) return SI_ERR_INVALID_ARGUMENT;
+    return tguard(ctx, [&] {
+        hipStream_t s = ctx->c.stream;
+        si::DevBuf<double> a((size_t)n * n), w(n), v((size_t)n * n);
+        si::to_device(a.get(), a_host, (size_t)n * n, s);
+        si::small_sym_eig(n, a, n, w, v, s);
+        si::to_host(w_host, w.get(), n, s);
+        si::to_host(v_host, v.get(), (size_t)n * n, s);
+    });
+}
+
+si_status si_test_small_gen_eig(si_context* ctx, int32_t n, const double* a_host,
+                                const double* b_host, double* w_host, double* c_host) {
+    if (!ctx || !a_host || !b_host || !w_host || !c_host) return SI_ERR_INVALID_ARGUMENT;
+    return tguard(ctx, [&] {
+        hipStream_t s = ctx->c.stream;
+        si::DevBuf<double> a((size_t)n * n), b((size_t)n * n), w(n), c((size_t)n * n);
+        si::DevBuf<int> st(1);
+        si::to_device(a.get(), a_host, (size_t)n * n, s);
+        si::to_device(b.get(), b_host, (size_t)n * n, s);
+        si::small_gen_eig(n, a, b, w, c, st, s);
+        int h = 0;
+        si::to_host(&h, st.get(), 1, s);
+        SI_CHECK(h == 0, si::kErrInternal, "small_gen_eig: B is not positive definite");
+        si::to_host(w_host, w.get(), n, s);
+        si::to_host(c_host, c.get(), (size_t)n * n, s);
+    });
+}
+
+si_status si_test_fill_normal(si_context* ctx, double* x, int64_t n, int32_t k, uint64_t seed,
+                              uint32_t stream) {
+    if (!ctx || !x) return SI_ERR_INVALID_ARGUMENT;
+    return tguard(ctx, [&] {
+        si::fill_normal_f64(x, n, k, k, seed, stream, ctx->c.stream);
+        SI_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
+si_status si_test_fast_solve(si_context* ctx, const si_laplacian* lap, int32_t n_components,
+                             uint64_t seed, const si_config* cfg, double* evals, double* evecs,
+                             si_report* rep) {
+    if (!ctx || !lap || !evals || !evecs) return SI_ERR_INVALID_ARGUMENT;
+    return tguard(ctx, [&] {
+        si_config d;
+        si_config_default(&d);
+        if (!cfg) cfg = &d;
+        si::FastOptions o;
+        o.block_size = cfg->block_size;
+        o.cheb_degree = cfg->cheb_degree > 0 ? cfg->cheb_degree : 8;
+        o.max_outer = cfg->max_outer > 0 ? cfg->max_outer : 60;
+        o.tol = cfg->tol > 0 ? cfg->tol : 1e-6;
+        o.warm_start = cfg->warm_start;
+        si::SolveStats st;
+        si::EventTimer t(ctx->c.stream);
+        const bool ok = si::fast_solve(lap->L, n_components, seed, o, evals, evecs, st, ctx->c.stream);
+        const double ms = t.stop_ms();
+        if (rep) {
+            rep->level = 1;
+            rep->restarts = 0;
+            rep->block_size = st.block_size;
+            rep->iterations = st.iterations;
+            rep->spmm_calls = st.spmm_calls;
+            rep->max_residual = st.max_residual;
+            for (int i = 0; i < 8; ++i) rep->eigenvalues[i] = i <= n_components ? evals[i] : 0.0;
+            rep->solve_ms = ms;
+        }
+        SI_CHECK(ok, si::kErrConvergence, "fast_solve produced no result");
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,313 @@
+// k_block.hip -- tall-skinny block kernels of the block eigensolvers:
+// FP64-accumulated Gram matrices on the FP64 matrix cores, block x small
+// matrix products (the fused AXPY updates), residual blocks, Philox normal
+// fills and conversions.
+//
+// Gram G = A^T B, A [n x p], B [n x q], p, q <= 64.  Each wavefront walks
+// 4 rows per step and feeds v_mfma_f64_16x16x4f64 with A^T (16 x 4) and
+// B (4 x 16) tiles; FP32 inputs are widened to FP64 in registers, so every
+// product and every sum is FP64.  The four wavefronts of a workgroup reduce in
+// LDS and write one partial per workgroup; a second kernel sums the partials
+// in a fixed order (deterministic).
+#include "si_device.hpp"
+
+namespace si {
+
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int kGramBlocks = 1024;
+
+template <typename T, int PT, int QT>
+__global__ __launch_bounds__(256) void k_gram(int64_t n, const T* __restrict__ a, int lda, int p,
+                                              const T* __restrict__ b, int ldb, int q,
+                                              double* __restrict__ partial,
+                                              int64_t rows_per_block) {
+    __shared__ double red[PT * 16][QT * 16 + 1];
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int li = lane & 15;
+    const int lk = lane >> 4;
+    v4d acc[PT][QT];
+#pragma unroll
+    for (int ti = 0; ti < PT; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < QT; ++tj) acc[ti][tj] = v4d{0.0, 0.0, 0.0, 0.0};
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+    const int64_t r1 = (r0 + rows_per_block) < n ? (r0 + rows_per_block) : n;
+    for (int64_t r = r0 + wave * 4; r < r1; r += 16) {
+        const int64_t row = r + lk;
+        const bool ok = row < r1;
+        double av[PT], bv[QT];
+#pragma unroll
+        for (int t = 0; t < PT; ++t) {
+            const int c = t * 16 + li;
+            av[t] = (ok && c < p) ? (double)a[row * lda + c] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < QT; ++t) {
+            const int c = t * 16 + li;
+            bv[t] = (ok && c < q) ? (double)b[row * ldb + c] : 0.0;
+        }
+#pragma unroll
+        for (int ti = 0; ti < PT; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < QT; ++tj)
+                acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ti], bv[tj], acc[ti][tj], 0, 0, 0);
+    }
+    // D[i][j] of a tile sits in lane (j + 16 * (i % 4)), element i / 4
+    // (measured on gfx950 with an integer-valued probe).
+    for (int w = 0; w < 4; ++w) {
+        if (wave == w) {
+#pragma unroll
+            for (int ti = 0; ti < PT; ++ti)
+#pragma unroll
+                for (int tj = 0; tj < QT; ++tj)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int i = ti * 16 + lk + 4 * v;
+                        const int j = tj * 16 + li;
+                        if (w == 0) red[i][j] = acc[ti][tj][v];
+                        else red[i][j] += acc[ti][tj][v];
+                    }
+        }
+        __syncthreads();
+    }
+    double* out = partial + (int64_t)blockIdx.x * p * q;
+    for (int e = threadIdx.x; e < p * q; e += 256) out[e] = red[e / q][e % q];
+}
+
+__global__ void k_reduce_partials(const double* __restrict__ partial, int nparts, int pq,
+                                  double* __restrict__ g) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= pq) return;
+    double s = 0.0;
+    for (int b = 0; b < nparts; ++b) s += partial[(int64_t)b * pq + e];
+    g[e] = s;
+}
+
+template <typename T>
+void gram_impl(int64_t n, const T* a, int lda, int p, const T* b, int ldb, int q, double* g,
+               double* ws, hipStream_t s) {
+    SI_CHECK(p >= 1 && q >= 1 && p <= 64 && q <= 64, kErrInvalidArgument, "gram: width out of range");
+    int64_t nblocks = ceil_div(n, 64);
+    if (nblocks > kGramBlocks) nblocks = kGramBlocks;
+    if (nblocks < 1) nblocks = 1;
+    int64_t rpb = ceil_div(n, nblocks);
+    rpb = ceil_div(rpb, 16) * 16;
+    nblocks = ceil_div(n, rpb);
+    if (nblocks < 1) nblocks = 1;
+    const int pt = (p + 15) / 16, qt = (q + 15) / 16;
+#define SI_GRAM_CASE(PT_, QT_)                                                                   \
+    if (pt == PT_ && qt == QT_) {                                                                \
+        hipLaunchKernelGGL((k_gram<T, PT_, QT_>), dim3(nblocks), dim3(256), 0, s, n, a, lda, p, b, \
+                           ldb, q, ws, rpb);                                                      \
+    } else
+    SI_GRAM_CASE(1, 1) SI_GRAM_CASE(1, 2) SI_GRAM_CASE(1, 3) SI_GRAM_CASE(1, 4)
+    SI_GRAM_CASE(2, 1) SI_GRAM_CASE(2, 2) SI_GRAM_CASE(2, 3) SI_GRAM_CASE(2, 4)
+    SI_GRAM_CASE(3, 1) SI_GRAM_CASE(3, 2) SI_GRAM_CASE(3, 3) SI_GRAM_CASE(3, 4)
+    SI_GRAM_CASE(4, 1) SI_GRAM_CASE(4, 2) SI_GRAM_CASE(4, 3) SI_GRAM_CASE(4, 4)
+    { throw Error(kErrInternal, "gram: no kernel for this shape"); }
+#undef SI_GRAM_CASE
+    SI_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_reduce_partials, dim3(ceil_div(p * q, 256)), dim3(256), 0, s, ws,
+                       (int)nblocks, p * q, g);
+    SI_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ block products
+template <typename T, int QMAX>
+__global__ __launch_bounds__(256) void k_block_mul(int64_t n, const T* __restrict__ a, int lda,
+                                                   int p, const double* __restrict__ c, int q,
+                                                   const T* __restrict__ b, int ldb, int p2,
+                                                   const double* __restrict__ d, T* out, int ldo) {
+    __shared__ T cs[64 * QMAX];
+    __shared__ T ds[64 * QMAX];
+    for (int e = threadIdx.x; e < p * q; e += blockDim.x) cs[(e / q) * QMAX + e % q] = (T)c[e];
+    if (b)
+        for (int e = threadIdx.x; e < p2 * q; e += blockDim.x) ds[(e / q) * QMAX + e % q] = (T)d[e];
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    T acc[QMAX];
+#pragma unroll
+    for (int j = 0; j < QMAX; ++j) acc[j] = T(0);
+    const T* ar = a + i * lda;
+    for (int k = 0; k < p; ++k) {
+        const T av = ar[k];
+#pragma unroll
+        for (int j = 0; j < QMAX; ++j) acc[j] += av * cs[k * QMAX + j];
+    }
+    if (b) {
+        const T* br = b + i * ldb;
+        for (int k = 0; k < p2; ++k) {
+            const T bv = br[k];
+#pragma unroll
+            for (int j = 0; j < QMAX; ++j) acc[j] += bv * ds[k * QMAX + j];
+        }
+    }
+    T* o = out + i * ldo;
+#pragma unroll
+    for (int j = 0; j < QMAX; ++j)
+        if (j < q) o[j] = acc[j];
+}
+
+template <typename T>
+void block_mul_impl(int64_t n, const T* a, int lda, int p, const double* c, int q, const T* b,
+                    int ldb, int p2, const double* d, T* out, int ldo, hipStream_t s) {
+    SI_CHECK(p <= 64 && q <= 64 && p2 <= 64, kErrInvalidArgument, "block_mul: width > 64");
+    SI_CHECK(out != b || b == nullptr, kErrInvalidArgument, "block_mul: out aliases the second term");
+    const int64_t blocks = ceil_div(n, 256);
+    if (q <= 4)
+        hipLaunchKernelGGL((k_block_mul<T, 4>), dim3(blocks), dim3(256), 0, s, n, a, lda, p, c, q, b, ldb, p2, d, out, ldo);
+    else if (q <= 8)
+        hipLaunchKernelGGL((k_block_mul<T, 8>), dim3(blocks), dim3(256), 0, s, n, a, lda, p, c, q, b, ldb, p2, d, out, ldo);
+    else if (q <= 16)
+        hipLaunchKernelGGL((k_block_mul<T, 16>), dim3(blocks), dim3(256), 0, s, n, a, lda, p, c, q, b, ldb, p2, d, out, ldo);
+    else if (q <= 32)
+        hipLaunchKernelGGL((k_block_mul<T, 32>), dim3(blocks), dim3(256), 0, s, n, a, lda, p, c, q, b, ldb, p2, d, out, ldo);
+    else
+        hipLaunchKernelGGL((k_block_mul<T, 64>), dim3(blocks), dim3(256), 0, s, n, a, lda, p, c, q, b, ldb, p2, d, out, ldo);
+    SI_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ elementwise
+template <typename T>
+__global__ void k_residual(int64_t n, const T* __restrict__ ax, const T* __restrict__ x, int ld,
+                           int k, const double* __restrict__ theta, T* __restrict__ r, int ldr) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * k) return;
+    const int64_t i = e / k;
+    const int c = (int)(e % k);
+    r[i * ldr + c] = ax[i * ld + c] - (T)theta[c] * x[i * ld + c];
+}
+
+__global__ void k_fill_normal(double* __restrict__ x, int64_t n, int k, int ld, uint64_t seed,
+                              uint32_t stream) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * k) return;
+    const int64_t i = e / k;
+    const int c = (int)(e % k);
+    x[i * ld + c] = normal_at(seed, stream, (uint64_t)e);
+}
+
+template <typename A, typename B>
+__global__ void k_convert(const A* __restrict__ a, B* __restrict__ b, int64_t count) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < count) b[e] = (B)a[e];
+}
+
+template <typename T>
+__global__ void k_copy_cols(const T* __restrict__ a, int lda, T* __restrict__ b, int ldb,
+                            int64_t n, int k) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * k) return;
+    const int64_t i = e / k;
+    const int c = (int)(e % k);
+    b[i * ldb + c] = a[i * lda + c];
+}
+
+template <typename T>
+__global__ void k_scale_cols(T* __restrict__ x, int ld, int64_t n, int k,
+                             const double* __restrict__ scale) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * k) return;
+    const int64_t i = e / k;
+    const int c = (int)(e % k);
+    x[i * ld + c] = (T)((double)x[i * ld + c] * scale[c]);
+}
+
+__global__ void k_average(double* __restrict__ x, const double* __restrict__ t, int64_t count) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < count) x[e] = (x[e] + t[e]) / 2.0;
+}
+
+__global__ void k_set_column(double* __restrict__ x, int ld, int64_t n, int col,
+                             const double* __restrict__ v, double scale) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i * ld + col] = v[i] * scale;
+}
+
+inline dim3 grid1(int64_t count) { return dim3((unsigned)ceil_div(count, 256)); }
+
+}  // namespace
+
+size_t gram_workspace_doubles(int64_t n, int p, int q) {
+    int64_t nb = ceil_div(n, 64);
+    if (nb > kGramBlocks) nb = kGramBlocks;
+    if (nb < 1) nb = 1;
+    return (size_t)nb * p * q;
+}
+
+void gram_f32(int64_t n, const float* a, int lda, int p, const float* b, int ldb, int q,
+              double* g, double* ws, hipStream_t s) {
+    gram_impl<float>(n, a, lda, p, b, ldb, q, g, ws, s);
+}
+void gram_f64(int64_t n, const double* a, int lda, int p, const double* b, int ldb, int q,
+              double* g, double* ws, hipStream_t s) {
+    gram_impl<double>(n, a, lda, p, b, ldb, q, g, ws, s);
+}
+
+void block_mul_f32(int64_t n, const float* a, int lda, int p, const double* c, int q,
+                   const float* b, int ldb, int p2, const double* d, float* out, int ldo,
+                   hipStream_t s) {
+    block_mul_impl<float>(n, a, lda, p, c, q, b, ldb, p2, d, out, ldo, s);
+}
+void block_mul_f64(int64_t n, const double* a, int lda, int p, const double* c, int q,
+                   const double* b, int ldb, int p2, const double* d, double* out, int ldo,
+                   hipStream_t s) {
+    block_mul_impl<double>(n, a, lda, p, c, q, b, ldb, p2, d, out, ldo, s);
+}
+
+void residual_f32(int64_t n, const float* ax, const float* x, int ld, int k, const double* theta,
+                  float* r, int ldr, hipStream_t s) {
+    hipLaunchKernelGGL((k_residual<float>), grid1(n * k), dim3(256), 0, s, n, ax, x, ld, k, theta, r, ldr);
+    SI_LAUNCH_CHECK();
+}
+void residual_f64(int64_t n, const double* ax, const double* x, int ld, int k,
+                  const double* theta, double* r, int ldr, hipStream_t s) {
+    hipLaunchKernelGGL((k_residual<double>), grid1(n * k), dim3(256), 0, s, n, ax, x, ld, k, theta, r, ldr);
+    SI_LAUNCH_CHECK();
+}
+
+void fill_normal_f64(double* x, int64_t n, int k, int ld, uint64_t seed, uint32_t stream,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_normal, grid1(n * k), dim3(256), 0, s, x, n, k, ld, seed, stream);
+    SI_LAUNCH_CHECK();
+}
+void convert_f64_to_f32(const double* a, float* b, int64_t count, hipStream_t s) {
+    hipLaunchKernelGGL((k_convert<double, float>), grid1(count), dim3(256), 0, s, a, b, count);
+    SI_LAUNCH_CHECK();
+}
+void convert_f32_to_f64(const float* a, double* b, int64_t count, hipStream_t s) {
+    hipLaunchKernelGGL((k_convert<float, double>), grid1(count), dim3(256), 0, s, a, b, count);
+    SI_LAUNCH_CHECK();
+}
+void copy_cols_f64(const double* a, int lda, double* b, int ldb, int64_t n, int k, hipStream_t s) {
+    hipLaunchKernelGGL((k_copy_cols<double>), grid1(n * k), dim3(256), 0, s, a, lda, b, ldb, n, k);
+    SI_LAUNCH_CHECK();
+}
+void copy_cols_f32(const float* a, int lda, float* b, int ldb, int64_t n, int k, hipStream_t s) {
+    hipLaunchKernelGGL((k_copy_cols<float>), grid1(n * k), dim3(256), 0, s, a, lda, b, ldb, n, k);
+    SI_LAUNCH_CHECK();
+}
+void scale_cols_f32(float* x, int ld, int64_t n, int k, const double* scale, hipStream_t s) {
+    hipLaunchKernelGGL((k_scale_cols<float>), grid1(n * k), dim3(256), 0, s, x, ld, n, k, scale);
+    SI_LAUNCH_CHECK();
+}
+void scale_cols_f64(double* x, int ld, int64_t n, int k, const double* scale, hipStream_t s) {
+    hipLaunchKernelGGL((k_scale_cols<double>), grid1(n * k), dim3(256), 0, s, x, ld, n, k, scale);
+    SI_LAUNCH_CHECK();
+}
+void average_f64(double* x, const double* t, int64_t count, hipStream_t s) {
+    hipLaunchKernelGGL(k_average, grid1(count), dim3(256), 0, s, x, t, count);
+    SI_LAUNCH_CHECK();
+}
+void set_column_f64(double* x, int ld, int64_t n, int col, const double* v, double scale,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(k_set_column, grid1(n), dim3(256), 0, s, x, ld, n, col, v, scale);
+    SI_LAUNCH_CHECK();
+}
+
+}  // namespace si

@@ -1,0 +1,225 @@
+// k_laplacian.hip -- device build of the symmetric normalised Laplacian
+// (reference: src/laplacian.rs:8-88).
+//
+// Degrees.  PythonCompat accumulates deg_f32[col] += w over the nonzeros in
+// CSR order (laplacian.rs:8-16).  For a bitwise-symmetric W with ascending
+// columns, column j's entries in row order are row j's entries in column
+// order, so one thread summing row j sequentially in f32 reproduces the
+// scatter bit for bit.  The build verifies symmetry on the device; an
+// asymmetric input falls back to an f32 atomic scatter (sum order not fixed).
+// RustNative sums each row in f64 (laplacian.rs:41-46).
+//
+// Values.  L_ij = -isd_i * w_ij * isd_j evaluated left to right in f64
+// (laplacian.rs:70-80); W's own diagonal is dropped and L's unit diagonal is
+// implied (laplacian.rs:83-85).
+#include "si_device.hpp"
+
+namespace si {
+
+namespace {
+
+enum Flag { kFlagIndex = 0, kFlagWeight = 1, kFlagUnsorted = 2, kFlagAsym = 3, kNumFlags = 4 };
+
+__global__ void k_convert_rowptr(const int64_t* __restrict__ ip64, int32_t* __restrict__ ip32,
+                                 int64_t n1) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n1) ip32[i] = (int32_t)ip64[i];
+}
+
+// One thread per row: bounds, finiteness / sign of weights, ascending columns.
+__global__ void k_validate(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                           const float* __restrict__ w, int64_t n, int* __restrict__ flags,
+                           int32_t* __restrict__ max_len) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t b = rp[i], e = rp[i + 1];
+    if (e < b) { atomicOr(&flags[kFlagIndex], 1); return; }
+    atomicMax(max_len, e - b);
+    int32_t prev = -1;
+    for (int32_t p = b; p < e; ++p) {
+        int32_t j = col[p];
+        if (j < 0 || j >= n) { atomicOr(&flags[kFlagIndex], 1); return; }
+        float v = w[p];
+        if (!(v >= 0.0f) || isinf(v)) atomicOr(&flags[kFlagWeight], 1);
+        if (j <= prev) atomicOr(&flags[kFlagUnsorted], 1);
+        prev = j;
+    }
+}
+
+// One thread per row: each entry (i, j, w) must have a bitwise-equal (j, i, w).
+__global__ void k_check_symmetric(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                  const float* __restrict__ w, int64_t n, int* __restrict__ flags) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int32_t p = rp[i]; p < rp[i + 1]; ++p) {
+        int32_t j = col[p];
+        int32_t lo = rp[j], hi = rp[j + 1] - 1;
+        bool found = false;
+        while (lo <= hi) {
+            int32_t mid = (lo + hi) >> 1;
+            int32_t c = col[mid];
+            if (c == i) { found = __float_as_uint(w[mid]) == __float_as_uint(w[p]); break; }
+            if (c < i) lo = mid + 1; else hi = mid - 1;
+        }
+        if (!found) { atomicOr(&flags[kFlagAsym], 1); return; }
+    }
+}
+
+__global__ void k_degree_rowsum_f32(const int32_t* __restrict__ rp, const float* __restrict__ w,
+                                    int64_t n, double* __restrict__ deg) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float acc = 0.0f;
+    for (int32_t p = rp[i]; p < rp[i + 1]; ++p) acc += w[p];
+    deg[i] = (double)acc;
+}
+
+__global__ void k_degree_colscatter_f32(const int32_t* __restrict__ rp,
+                                        const int32_t* __restrict__ col,
+                                        const float* __restrict__ w, int64_t n,
+                                        float* __restrict__ acc) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int32_t p = rp[i]; p < rp[i + 1]; ++p) atomicAdd(&acc[col[p]], w[p]);
+}
+
+__global__ void k_widen(const float* __restrict__ a, double* __restrict__ b, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
+}
+
+__global__ void k_degree_rowsum_f64(const int32_t* __restrict__ rp, const float* __restrict__ w,
+                                    int64_t n, double* __restrict__ deg) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int32_t p = rp[i]; p < rp[i + 1]; ++p) acc += (double)w[p];
+    deg[i] = acc;
+}
+
+__global__ void k_isd(const double* __restrict__ deg, int64_t n, double* __restrict__ sq,
+                      double* __restrict__ isd) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = sqrt(deg[i]);
+    sq[i] = s;
+    isd[i] = s > 0.0 ? 1.0 / s : 0.0;  // lib.rs:233-236
+}
+
+__global__ void k_values(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                         const float* __restrict__ w, const double* __restrict__ isd, int64_t n,
+                         double* __restrict__ l64, float* __restrict__ a32,
+                         int32_t* __restrict__ long_rows, int* __restrict__ n_long) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double di = isd[i];
+    int32_t b = rp[i], e = rp[i + 1];
+    for (int32_t p = b; p < e; ++p) {
+        int32_t j = col[p];
+        double v = (j == i) ? 0.0 : (-di * (double)w[p]) * isd[j];
+        l64[p] = v;
+        a32[p] = (float)(-v);
+    }
+    if (e - b > kLongRow) {
+        int slot = atomicAdd(n_long, 1);
+        long_rows[slot] = (int32_t)i;
+    }
+}
+
+}  // namespace
+
+void laplacian_free(DevLaplacian& L) {
+    hipFree(L.rowptr); hipFree(L.col); hipFree(L.a32); hipFree(L.l64);
+    hipFree(L.deg); hipFree(L.sqrt_deg); hipFree(L.isd); hipFree(L.long_rows);
+    L = DevLaplacian{};
+}
+
+void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
+                            const int32_t* d_indices, const float* d_w, int64_t nnz, int mode,
+                            hipStream_t s) {
+    SI_CHECK(n > 0, kErrInvalidGraph, "laplacian: n must be > 0");
+    SI_CHECK(nnz < (int64_t)INT32_MAX && n < (int64_t)INT32_MAX, kErrUnsupported,
+             "laplacian: more than 2^31-1 stored entries or nodes is not supported");
+    laplacian_free(L);
+    L.n = n;
+    L.nnz = nnz;
+    L.degree_mode = mode;
+    const int T = 256;
+    const int64_t gn = ceil_div(n, T);
+    SI_HIP(hipMalloc(&L.rowptr, sizeof(int32_t) * (n + 1)));
+    SI_HIP(hipMalloc(&L.col, sizeof(int32_t) * (nnz > 0 ? nnz : 1)));
+    SI_HIP(hipMalloc(&L.a32, sizeof(float) * (nnz > 0 ? nnz : 1)));
+    SI_HIP(hipMalloc(&L.l64, sizeof(double) * (nnz > 0 ? nnz : 1)));
+    SI_HIP(hipMalloc(&L.deg, sizeof(double) * n));
+    SI_HIP(hipMalloc(&L.sqrt_deg, sizeof(double) * n));
+    SI_HIP(hipMalloc(&L.isd, sizeof(double) * n));
+    SI_HIP(hipMalloc(&L.long_rows, sizeof(int32_t) * (n > 0 ? n : 1)));
+    int* d_flags = nullptr;
+    SI_HIP(hipMalloc(&d_flags, sizeof(int) * (kNumFlags + 2)));
+    SI_HIP(hipMemsetAsync(d_flags, 0, sizeof(int) * (kNumFlags + 2), s));
+    int32_t* d_maxlen = d_flags + kNumFlags;
+    int* d_nlong = d_flags + kNumFlags + 1;
+
+    hipLaunchKernelGGL(k_convert_rowptr, dim3(ceil_div(n + 1, T)), dim3(T), 0, s, d_indptr,
+                       L.rowptr, n + 1);
+    if (nnz > 0) SI_HIP(hipMemcpyAsync(L.col, d_indices, sizeof(int32_t) * nnz,
+                                       hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_validate, dim3(gn), dim3(T), 0, s, L.rowptr, L.col, d_w, n, d_flags,
+                       d_maxlen);
+    SI_LAUNCH_CHECK();
+    int flags[kNumFlags + 2];
+    SI_HIP(hipMemcpyAsync(flags, d_flags, sizeof(flags), hipMemcpyDeviceToHost, s));
+    SI_HIP(hipStreamSynchronize(s));
+    if (flags[kFlagIndex]) {
+        hipFree(d_flags);
+        laplacian_free(L);
+        throw Error(kErrInvalidGraph, "invalid graph: row pointers or column indices out of range");
+    }
+    if (flags[kFlagWeight]) {
+        hipFree(d_flags);
+        laplacian_free(L);
+        throw Error(kErrInvalidGraph, "invalid graph: weights must be finite and non-negative");
+    }
+    L.max_row_len = flags[kNumFlags];
+    int sorted = flags[kFlagUnsorted] == 0;
+    if (sorted) {
+        hipLaunchKernelGGL(k_check_symmetric, dim3(gn), dim3(T), 0, s, L.rowptr, L.col, d_w, n,
+                           d_flags);
+        SI_LAUNCH_CHECK();
+        SI_HIP(hipMemcpyAsync(flags, d_flags, sizeof(int) * kNumFlags, hipMemcpyDeviceToHost, s));
+        SI_HIP(hipStreamSynchronize(s));
+    }
+    L.symmetric = sorted && flags[kFlagAsym] == 0;
+
+    if (mode == 0) {
+        if (L.symmetric) {
+            hipLaunchKernelGGL(k_degree_rowsum_f32, dim3(gn), dim3(T), 0, s, L.rowptr, d_w, n,
+                               L.deg);
+            L.degree_exact = 1;
+        } else {
+            float* acc = nullptr;
+            SI_HIP(hipMalloc(&acc, sizeof(float) * n));
+            SI_HIP(hipMemsetAsync(acc, 0, sizeof(float) * n, s));
+            hipLaunchKernelGGL(k_degree_colscatter_f32, dim3(gn), dim3(T), 0, s, L.rowptr, L.col,
+                               d_w, n, acc);
+            hipLaunchKernelGGL(k_widen, dim3(gn), dim3(T), 0, s, acc, L.deg, n);
+            SI_HIP(hipStreamSynchronize(s));
+            hipFree(acc);
+            L.degree_exact = 0;
+        }
+    } else {
+        hipLaunchKernelGGL(k_degree_rowsum_f64, dim3(gn), dim3(T), 0, s, L.rowptr, d_w, n, L.deg);
+        L.degree_exact = 1;
+    }
+    hipLaunchKernelGGL(k_isd, dim3(gn), dim3(T), 0, s, L.deg, n, L.sqrt_deg, L.isd);
+    hipLaunchKernelGGL(k_values, dim3(gn), dim3(T), 0, s, L.rowptr, L.col, d_w, L.isd, n, L.l64,
+                       L.a32, L.long_rows, d_nlong);
+    SI_LAUNCH_CHECK();
+    int nlong = 0;
+    SI_HIP(hipMemcpyAsync(&nlong, d_nlong, sizeof(int), hipMemcpyDeviceToHost, s));
+    SI_HIP(hipStreamSynchronize(s));
+    L.n_long = nlong;
+    hipFree(d_flags);
+}
+
+}  // namespace si

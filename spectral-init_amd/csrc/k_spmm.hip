@@ -1,0 +1,282 @@
+// k_spmm.hip -- block SpMM with the Laplacian, the replacement of
+// CsrOperator::apply / spmv_csr / sprs::prod::csr_mulacc_dense_rowmaj
+// (reference: src/operator.rs:24-57, 176-228) and of the Chebyshev
+// recurrence step (src/solvers/lobpcg.rs:113-150) through a fused epilogue:
+//
+//     out = s_l * (L x) + s_x * x + s_z * z,   L x = x - A x  (unit diagonal folded)
+//
+// Three schedules, picked per width and row length:
+//   narrow (k <= 8)   G lanes per row, every lane gathers whole x rows;
+//                     G-lane butterfly reduction; 64/G rows per wavefront.
+//   wide (k >= 16)    16-byte chunks of the x row across C = k/VEC lanes, the
+//                     remaining lanes split the row's nonzeros (slots); rows
+//                     per wavefront = 64 / (C * slots); slot reduction by DPP.
+//   long rows         rows with > kLongRow entries (degree hubs) get a 256-thread
+//                     workgroup each, reduced through LDS in a fixed order.
+// Every reduction has a fixed order, so results are deterministic.
+// Blocks are remapped so that each of the 8 XCDs streams a contiguous range
+// of rows (its L2 then holds the x rows that neighbouring rows gather).
+#include "si_device.hpp"
+
+namespace si {
+
+namespace {
+
+template <typename T> struct Vec4;
+template <> struct Vec4<float> { using type = float4; static constexpr int n = 4; };
+template <> struct Vec4<double> { using type = double2; static constexpr int n = 2; };
+
+__device__ inline int64_t xcd_block(int64_t b, int64_t nb) {
+    // Workgroups are dispatched round-robin over 8 XCDs; give XCD (b % 8) the
+    // contiguous chunk [x*per, (x+1)*per) of the first 8*per blocks (a
+    // bijection); the remaining nb % 8 blocks keep their own index.
+    const int64_t per = nb / 8;
+    if (b >= 8 * per) return b;
+    return (b % 8) * per + b / 8;
+}
+
+// sign of the stored off-diagonal values: a32 holds +A (so L x = x - acc),
+// l64 holds the negative L entries (so L x = x + acc).
+template <typename T> __device__ inline T apply_diag(T xi, T acc);
+template <> __device__ inline float apply_diag<float>(float xi, float acc) { return xi - acc; }
+template <> __device__ inline double apply_diag<double>(double xi, double acc) { return xi + acc; }
+
+// -------------------------------------------------------------- narrow
+template <typename T, int K, int G>
+__global__ __launch_bounds__(256) void k_spmm_narrow(
+    const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const T* __restrict__ val,
+    const T* __restrict__ x, T* __restrict__ out, const T* __restrict__ z, int ld, int64_t n,
+    T s_l, T s_x, T s_z) {
+    const int64_t nb = gridDim.x;
+    const int64_t blk = xcd_block(blockIdx.x, nb);
+    const int lane = threadIdx.x % G;
+    const int64_t i = (blk * blockDim.x + threadIdx.x) / G;
+    if (i >= n) return;
+    const int32_t b = rp[i], e = rp[i + 1];
+    if (e - b > kLongRow) return;
+    T acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = T(0);
+    int32_t p = b + lane;
+    // two nonzeros in flight per lane
+    for (; p + G < e; p += 2 * G) {
+        const T v0 = val[p], v1 = val[p + G];
+        const T* x0 = x + (int64_t)col[p] * ld;
+        const T* x1 = x + (int64_t)col[p + G] * ld;
+        T a0[K], a1[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) { a0[k] = x0[k]; a1[k] = x1[k]; }
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] += v0 * a0[k];
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] += v1 * a1[k];
+    }
+    if (p < e) {
+        const T v0 = val[p];
+        const T* x0 = x + (int64_t)col[p] * ld;
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] += v0 * x0[k];
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] += __shfl_xor(acc[k], off, G);
+    // lanes share the epilogue: lane l writes columns l, l+G, ...
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (k % G != lane) continue;
+        const int64_t o = i * ld + k;
+        const T xi = x[o];
+        T r = s_l * apply_diag<T>(xi, acc[k]) + s_x * xi;
+        if (z) r += s_z * z[o];
+        out[o] = r;
+    }
+}
+
+// -------------------------------------------------------------- wide
+template <typename T, int K, int SLOTS>
+__global__ __launch_bounds__(256) void k_spmm_wide(
+    const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const T* __restrict__ val,
+    const T* __restrict__ x, T* __restrict__ out, const T* __restrict__ z, int ld, int64_t n,
+    T s_l, T s_x, T s_z) {
+    using V = typename Vec4<T>::type;
+    constexpr int VEC = Vec4<T>::n;
+    constexpr int C = K / VEC;                 // lanes across one x row
+    constexpr int RPW = 64 / (C * SLOTS);      // rows per wavefront
+    static_assert(C * SLOTS * RPW == 64, "bad wide SpMM shape");
+    const int64_t nb = gridDim.x;
+    const int64_t blk = xcd_block(blockIdx.x, nb);
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int chunk = lane % C;
+    const int slot = (lane / C) % SLOTS;
+    const int rsub = lane / (C * SLOTS);
+    const int64_t i = (blk * (blockDim.x >> 6) + wave) * RPW + rsub;
+    const bool valid = i < n;
+    int32_t b = 0, e = 0;
+    if (valid) { b = rp[i]; e = rp[i + 1]; }
+    const bool is_long = (e - b) > kLongRow;
+    if (!valid || is_long) { b = 0; e = 0; }
+    T acc[VEC];
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) acc[v] = T(0);
+    int32_t p = b + slot;
+    for (; p + SLOTS < e; p += 2 * SLOTS) {
+        const T v0 = val[p], v1 = val[p + SLOTS];
+        const V x0 = *reinterpret_cast<const V*>(x + (int64_t)col[p] * ld + chunk * VEC);
+        const V x1 = *reinterpret_cast<const V*>(x + (int64_t)col[p + SLOTS] * ld + chunk * VEC);
+        const T* a0 = reinterpret_cast<const T*>(&x0);
+        const T* a1 = reinterpret_cast<const T*>(&x1);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) acc[v] += v0 * a0[v];
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) acc[v] += v1 * a1[v];
+    }
+    if (p < e) {
+        const T v0 = val[p];
+        const V x0 = *reinterpret_cast<const V*>(x + (int64_t)col[p] * ld + chunk * VEC);
+        const T* a0 = reinterpret_cast<const T*>(&x0);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) acc[v] += v0 * a0[v];
+    }
+    // reduce over slots (lane bits between C and C*SLOTS)
+#pragma unroll
+    for (int off = C; off < C * SLOTS; off <<= 1)
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) acc[v] += __shfl_xor(acc[v], off, 64);
+    if (slot == 0 && valid && !is_long) {
+        const int64_t o = i * ld + chunk * VEC;
+        const V xv = *reinterpret_cast<const V*>(x + o);
+        const T* xi = reinterpret_cast<const T*>(&xv);
+        V r;
+        T* rr = reinterpret_cast<T*>(&r);
+        if (z) {
+            const V zv = *reinterpret_cast<const V*>(z + o);
+            const T* zi = reinterpret_cast<const T*>(&zv);
+#pragma unroll
+            for (int v = 0; v < VEC; ++v)
+                rr[v] = s_l * apply_diag<T>(xi[v], acc[v]) + s_x * xi[v] + s_z * zi[v];
+        } else {
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) rr[v] = s_l * apply_diag<T>(xi[v], acc[v]) + s_x * xi[v];
+        }
+        *reinterpret_cast<V*>(out + o) = r;
+    }
+}
+
+// -------------------------------------------------------------- long rows
+template <typename T>
+__global__ __launch_bounds__(256) void k_spmm_long(
+    const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const T* __restrict__ val,
+    const T* __restrict__ x, T* __restrict__ out, const T* __restrict__ z, int ld, int k,
+    const int32_t* __restrict__ rows, T s_l, T s_x, T s_z) {
+    constexpr int CH = 16;                 // columns per pass
+    __shared__ T part[256][CH + 1];
+    const int64_t i = rows[blockIdx.x];
+    const int32_t b = rp[i], e = rp[i + 1];
+    const int t = threadIdx.x;
+    for (int c0 = 0; c0 < k; c0 += CH) {
+        const int cw = (k - c0) < CH ? (k - c0) : CH;
+        T acc[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) acc[c] = T(0);
+        for (int32_t p = b + t; p < e; p += 256) {
+            const T v = val[p];
+            const T* xr = x + (int64_t)col[p] * ld + c0;
+#pragma unroll
+            for (int c = 0; c < CH; ++c)
+                if (c < cw) acc[c] += v * xr[c];
+        }
+#pragma unroll
+        for (int c = 0; c < CH; ++c) part[t][c] = acc[c];
+        __syncthreads();
+        if (t < cw) {
+            T sum = T(0);
+            for (int q = 0; q < 256; ++q) sum += part[q][t];
+            const int64_t o = i * ld + c0 + t;
+            const T xi = x[o];
+            T r = s_l * apply_diag<T>(xi, sum) + s_x * xi;
+            if (z) r += s_z * z[o];
+            out[o] = r;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int K, int G>
+void launch_narrow(const DevLaplacian& L, const T* val, const T* x, T* out, int ld, T s_l,
+                   T s_x, T s_z, const T* z, hipStream_t s) {
+    const int64_t threads = L.n * G;
+    const int64_t blocks = ceil_div(threads, 256);
+    hipLaunchKernelGGL((k_spmm_narrow<T, K, G>), dim3(blocks), dim3(256), 0, s, L.rowptr, L.col,
+                       val, x, out, z, ld, L.n, s_l, s_x, s_z);
+}
+
+template <typename T, int K, int SLOTS>
+void launch_wide(const DevLaplacian& L, const T* val, const T* x, T* out, int ld, T s_l, T s_x,
+                 T s_z, const T* z, hipStream_t s) {
+    constexpr int VEC = Vec4<T>::n;
+    constexpr int C = K / VEC;
+    constexpr int RPW = 64 / (C * SLOTS);
+    const int64_t waves = ceil_div(L.n, RPW);
+    const int64_t blocks = ceil_div(waves, 4);
+    hipLaunchKernelGGL((k_spmm_wide<T, K, SLOTS>), dim3(blocks), dim3(256), 0, s, L.rowptr, L.col,
+                       val, x, out, z, ld, L.n, s_l, s_x, s_z);
+}
+
+template <typename T>
+void spmm_impl(const DevLaplacian& L, const T* val, const T* x, T* out, int k, int ld, T s_l,
+               T s_x, T s_z, const T* z, hipStream_t s) {
+    SI_CHECK(ld >= k, kErrInvalidArgument, "spmm: ld < k");
+    SI_CHECK(x != out && (z == nullptr || z != out), kErrInvalidArgument,
+             "spmm: output must not alias an input");
+    constexpr bool F32 = sizeof(T) == 4;
+    if (k >= 16) {
+        SI_CHECK(ld % Vec4<T>::n == 0, kErrInvalidArgument, "spmm: ld must be a multiple of 16 bytes");
+    }
+    switch (k) {
+        case 1: launch_narrow<T, 1, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s); break;
+        case 2: launch_narrow<T, 2, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s); break;
+        case 3: launch_narrow<T, 3, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s); break;
+        case 4: launch_narrow<T, 4, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s); break;
+        case 8: launch_narrow<T, 8, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s); break;
+        case 16:
+            if constexpr (F32) launch_wide<T, 16, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s);   // C=4, RPW=2
+            else launch_wide<T, 16, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s);       // C=8, RPW=1
+            break;
+        case 32:
+            if constexpr (F32) launch_wide<T, 32, 4>(L, val, x, out, ld, s_l, s_x, s_z, z, s);   // C=8, RPW=2
+            else launch_wide<T, 32, 4>(L, val, x, out, ld, s_l, s_x, s_z, z, s);       // C=16, RPW=1
+            break;
+        case 64:
+            if constexpr (F32) launch_wide<T, 64, 4>(L, val, x, out, ld, s_l, s_x, s_z, z, s);   // C=16, RPW=1
+            else launch_wide<T, 64, 2>(L, val, x, out, ld, s_l, s_x, s_z, z, s);       // C=32, RPW=1
+            break;
+        default:
+            throw Error(kErrInvalidArgument, "spmm: unsupported block width " + std::to_string(k));
+    }
+    SI_LAUNCH_CHECK();
+    if (L.n_long > 0) {
+        hipLaunchKernelGGL((k_spmm_long<T>), dim3(L.n_long), dim3(256), 0, s, L.rowptr, L.col,
+                           val, x, out, z, ld, k, L.long_rows, s_l, s_x, s_z);
+        SI_LAUNCH_CHECK();
+    }
+}
+
+}  // namespace
+
+bool spmm_width_supported(int k) {
+    return k == 1 || k == 2 || k == 3 || k == 4 || k == 8 || k == 16 || k == 32 || k == 64;
+}
+
+void spmm_f32(const DevLaplacian& L, const float* x, float* out, int k, int ld, float s_l,
+              float s_x, float s_z, const float* z, hipStream_t s) {
+    spmm_impl<float>(L, L.a32, x, out, k, ld, s_l, s_x, s_z, z, s);
+}
+
+void spmm_f64(const DevLaplacian& L, const double* x, double* out, int k, int ld, double s_l,
+              double s_x, double s_z, const double* z, hipStream_t s) {
+    spmm_impl<double>(L, L.l64, x, out, k, ld, s_l, s_x, s_z, z, s);
+}
+
+}  // namespace si

@@ -1,0 +1,119 @@
+// si_device.hpp -- device-resident data structures and kernel launchers.
+//
+// HBM layout (one GPU):
+//   Laplacian L = I - A,  A = D^-1/2 W D^-1/2 with W's diagonal dropped
+//     rowptr  int32[n+1]   (nnz < 2^31)
+//     col     int32[nnz]   columns, ascending within a row
+//     a32     f32[nnz]     A's entries (positive), used by the FP32 bulk sweeps
+//     l64     f64[nnz]     L's off-diagonal entries, -isd_i * w_ij * isd_j in FP64,
+//                          the exact values of the reference's f64 Laplacian
+//     deg, sqrt_deg, isd  f64[n]
+//   The unit diagonal is never stored: every SpMM adds x_i in its epilogue.
+//   Dense blocks are row-major [n x ld] (ld >= width), FP32 or FP64.
+#pragma once
+
+#include "si_common.hpp"
+
+namespace si {
+
+constexpr int kLongRow = 256;  // rows above this length get a workgroup of their own
+
+struct DevLaplacian {
+    int64_t n = 0;
+    int64_t nnz = 0;
+    int32_t* rowptr = nullptr;
+    int32_t* col = nullptr;
+    float* a32 = nullptr;
+    double* l64 = nullptr;
+    double* deg = nullptr;
+    double* sqrt_deg = nullptr;
+    double* isd = nullptr;
+    int32_t* long_rows = nullptr;  // indices of rows with > kLongRow entries
+    int64_t n_long = 0;
+    int64_t max_row_len = 0;
+    int symmetric = 0;       // W bitwise symmetric with sorted rows
+    int degree_mode = 0;     // 0 PythonCompat, 1 RustNative
+    int degree_exact = 1;    // PythonCompat degrees reproduce the f32 column-sum order
+};
+
+// ----------------------------------------------------------------- laplacian
+// Builds L on the device from a CSR graph already resident on the device
+// (int64 row pointers, int32 columns, f32 weights).  Throws si::Error on
+// malformed input.
+void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
+                            const int32_t* d_indices, const float* d_w, int64_t nnz, int mode,
+                            hipStream_t s);
+void laplacian_free(DevLaplacian& L);
+
+// ----------------------------------------------------------------- SpMM
+// out = s_l * (L x) + s_x * x + s_z * z   for row-major blocks of width k.
+// x, z, out have leading dimension ld (>= k).  z may be null when s_z == 0.
+void spmm_f32(const DevLaplacian& L, const float* x, float* out, int k, int ld, float s_l,
+              float s_x, float s_z, const float* z, hipStream_t s);
+void spmm_f64(const DevLaplacian& L, const double* x, double* out, int k, int ld, double s_l,
+              double s_x, double s_z, const double* z, hipStream_t s);
+bool spmm_width_supported(int k);
+
+// ----------------------------------------------------------------- dense blocks
+// Partial Gram: G = A^T B (FP64 accumulation) for A [n x p], B [n x q].
+// Writes the p x q result (row-major, ldg = q) into g; work buffer ws must hold
+// gram_workspace_doubles(n, p, q) doubles.
+size_t gram_workspace_doubles(int64_t n, int p, int q);
+void gram_f32(int64_t n, const float* a, int lda, int p, const float* b, int ldb, int q,
+              double* g, double* ws, hipStream_t s);
+void gram_f64(int64_t n, const double* a, int lda, int p, const double* b, int ldb, int q,
+              double* g, double* ws, hipStream_t s);
+
+// out[:, 0:q] = A[:, 0:p] * C (+ B[:, 0:p2] * D), C, D FP64 row-major device matrices.
+void block_mul_f32(int64_t n, const float* a, int lda, int p, const double* c, int q,
+                   const float* b, int ldb, int p2, const double* d, float* out, int ldo,
+                   hipStream_t s);
+void block_mul_f64(int64_t n, const double* a, int lda, int p, const double* c, int q,
+                   const double* b, int ldb, int p2, const double* d, double* out, int ldo,
+                   hipStream_t s);
+
+// r = ax - x * diag(theta) for the first k columns.
+void residual_f32(int64_t n, const float* ax, const float* x, int ld, int k, const double* theta,
+                  float* r, int ldr, hipStream_t s);
+void residual_f64(int64_t n, const double* ax, const double* x, int ld, int k,
+                  const double* theta, double* r, int ldr, hipStream_t s);
+
+void fill_normal_f64(double* x, int64_t n, int k, int ld, uint64_t seed, uint32_t stream,
+                     hipStream_t s);
+void convert_f64_to_f32(const double* a, float* b, int64_t count, hipStream_t s);
+void convert_f32_to_f64(const float* a, double* b, int64_t count, hipStream_t s);
+// copy columns [0,k) of a [n x lda] block into b [n x ldb]
+void copy_cols_f64(const double* a, int lda, double* b, int ldb, int64_t n, int k, hipStream_t s);
+void copy_cols_f32(const float* a, int lda, float* b, int ldb, int64_t n, int k, hipStream_t s);
+void scale_cols_f32(float* x, int ld, int64_t n, int k, const double* scale, hipStream_t s);
+void scale_cols_f64(double* x, int ld, int64_t n, int k, const double* scale, hipStream_t s);
+// x[:,c] = (x[:,c] + t[:,c]) / 2
+void average_f64(double* x, const double* t, int64_t count, hipStream_t s);
+void set_column_f64(double* x, int ld, int64_t n, int col, const double* v, double scale,
+                    hipStream_t s);
+
+// ----------------------------------------------------------------- small dense (one workgroup)
+// All small matrices are FP64 row-major, n <= kSmallMax, resident on the device.
+constexpr int kSmallMax = 64;
+// Symmetric eigen-decomposition of a (lower triangle read): w ascending, v columns.
+void small_sym_eig(int n, const double* a, int lda, double* w, double* v, hipStream_t s);
+// Generalised problem A c = theta B c.  status[0] = 0 ok, 1 if B not SPD.
+void small_gen_eig(int n, const double* a, const double* b, double* w, double* c, int* status,
+                   hipStream_t s);
+// Given G = X^T X (n x n), writes T = L^-T (upper) so that X T is orthonormal;
+// status[0] = 1 when G is not positive definite.
+void small_chol_inv(int n, const double* g, double* t, int* status, hipStream_t s);
+// scale[c] = 1/sqrt(g[c][c]) (0 when g[c][c] <= 0)
+void small_inv_sqrt_diag(int n, const double* g, double* scale, hipStream_t s);
+// SVQB orthonormalising transform from the eigenpairs (w, u) of a Gram matrix.
+void small_svqb(int n, const double* w, const double* u, double* t, double rel, hipStream_t s);
+// Dense FP64 copy of L (n <= kSmallMax), unit diagonal included.
+void dense_from_csr(const DevLaplacian& L, double* d, hipStream_t s);
+// dst[r * ldd + c] = alpha * src[r * lds + c] for r < rows, c < cols
+void small_copy_sub(int rows, int cols, const double* src, int lds, double* dst, int ldd,
+                    double alpha, hipStream_t s);
+// res[c] = sqrt(g[c][c]) / max(sqrt(gx[c][c]), eps)
+void small_residual_norms(int n, const double* gr, int ldr, const double* gx, int ldx,
+                          double* res, hipStream_t s);
+
+}  // namespace si

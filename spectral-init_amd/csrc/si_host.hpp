@@ -1,0 +1,141 @@
+// si_host.hpp -- host-side runtime of the library: device buffers, the
+// context (stream + workspace), timers and the solver entry points shared by
+// the C ABI.
+#pragma once
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "si_device.hpp"
+
+namespace si {
+
+// Owning device buffer (hipMalloc'd, freed on destruction).
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        SI_HIP(hipMalloc(&p, sizeof(T) * count));
+        n = count;
+    }
+    void ensure(size_t count) { if (count > n || p == nullptr) alloc(count); }
+    void release() { if (p) hipFree(p); p = nullptr; n = 0; }
+    T* get() const { return p; }
+    operator T*() const { return p; }
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    int num_cus = 0;
+};
+
+// Stream-ordered timer (hipEvents on the context's stream).
+struct EventTimer {
+    hipEvent_t a = nullptr, b = nullptr;
+    hipStream_t s;
+    explicit EventTimer(hipStream_t st) : s(st) {
+        SI_HIP(hipEventCreate(&a));
+        SI_HIP(hipEventCreate(&b));
+        SI_HIP(hipEventRecord(a, s));
+    }
+    double stop_ms() {
+        SI_HIP(hipEventRecord(b, s));
+        SI_HIP(hipEventSynchronize(b));
+        float ms = 0.f;
+        SI_HIP(hipEventElapsedTime(&ms, a, b));
+        return ms;
+    }
+    ~EventTimer() {
+        if (a) hipEventDestroy(a);
+        if (b) hipEventDestroy(b);
+    }
+};
+
+template <typename T>
+inline void to_host(T* h, const T* d, size_t count, hipStream_t s) {
+    SI_HIP(hipMemcpyAsync(h, d, sizeof(T) * count, hipMemcpyDeviceToHost, s));
+    SI_HIP(hipStreamSynchronize(s));
+}
+template <typename T>
+inline void to_device(T* d, const T* h, size_t count, hipStream_t s) {
+    SI_HIP(hipMemcpyAsync(d, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
+    SI_HIP(hipStreamSynchronize(s));
+}
+
+// Result of an eigensolve: host eigenvalues, device eigenvectors [n x k] FP64.
+struct SolveStats {
+    int level = -1;
+    int restarts = 0;
+    int block_size = 0;
+    int64_t iterations = 0;
+    int64_t spmm_calls = 0;
+    double max_residual = 0.0;
+    bool converged = false;
+};
+
+struct FastOptions {
+    int block_size = 0;     // 0 = automatic
+    int cheb_degree = 8;
+    int max_outer = 60;
+    double tol = 1e-6;
+    const double* warm_start = nullptr;  // device [n x k]
+};
+
+// Per-vector FP64 residuals ||L v - lambda v|| / max(||v||, eps) of device
+// vectors (lobpcg.rs:193-223, metrics.rs:57-92).
+void fp64_residuals(const DevLaplacian& L, const double* v, int k, const double* evals_host,
+                    double* res_host, hipStream_t s);
+
+// Rayleigh-Ritz refinement in FP64 (lobpcg.rs:158-187): v <- v V, evals <- eig(v^T L v).
+bool rayleigh_ritz_f64(const DevLaplacian& L, double* v, int k, double* evals_host, hipStream_t s);
+
+// lobpcg_solve (lobpcg.rs:227-465) on the device in FP64 (the plain baseline).
+bool lobpcg_solve_plain(const DevLaplacian& L, int n_components, uint64_t seed, bool regularize,
+                        double tol, double* evals_host, double* evecs_dev, SolveStats& st,
+                        hipStream_t s);
+
+// FP32-bulk Chebyshev-filtered block solver; FP64 Rayleigh-Ritz and residual.
+bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const FastOptions& opt,
+                double* evals_host, double* evecs_dev, SolveStats& st, hipStream_t s);
+
+// chebyshev_precond (lobpcg.rs:113-150) in FP64 on device blocks.
+void chebyshev_filter_f64(const DevLaplacian& L, const double* r, int k, double a, double b,
+                          int degree, double* out, hipStream_t s);
+
+// rsvd_solve (rsvd.rs:135-213).
+void rsvd_solve_device(const DevLaplacian& L, int n_components, uint64_t seed, int oversampling,
+                       int nbiter, double* evals_host, double* evecs_dev, SolveStats& st,
+                       hipStream_t s);
+
+// dense_evd (dense.rs:9-46) for n <= kSmallMax.
+void dense_evd_small(const DevLaplacian& L, int k, double* evals_host, double* evecs_dev,
+                     hipStream_t s);
+
+// solve_eigenproblem (solvers/mod.rs:70-224).
+struct ChainOptions {
+    int solver = 0;              // si_solver_kind at level 1
+    double tol = 1e-5;
+    int64_t dense_threshold = 2000;
+    FastOptions fast;
+};
+int solve_eigenproblem_device(const DevLaplacian& L, int n_components, uint64_t seed,
+                              const ChainOptions& opt, double* evals_host, double* evecs_dev,
+                              SolveStats& st, hipStream_t s);
+
+}  // namespace si

@@ -1,0 +1,75 @@
+// solver_common.cpp -- FP64 building blocks shared by the solvers: residual
+// norms, Rayleigh-Ritz refinement and the Chebyshev filter.
+#include <cmath>
+
+#include "si_host.hpp"
+
+namespace si {
+
+// ||L v_j - lambda_j v_j|| / max(||v_j||, eps): one block SpMM, one residual
+// block, two FP64 Grams (per_vector_residuals, lobpcg.rs:193-223).
+void fp64_residuals(const DevLaplacian& L, const double* v, int k, const double* evals_host,
+                    double* res_host, hipStream_t s) {
+    const int64_t n = L.n;
+    DevBuf<double> ax((size_t)n * k), r((size_t)n * k), lam(k), g((size_t)k * k),
+        gx((size_t)k * k), ws(gram_workspace_doubles(n, k, k));
+    spmm_f64(L, v, ax, k, k, 1.0, 0.0, 0.0, nullptr, s);
+    to_device(lam.get(), evals_host, k, s);
+    residual_f64(n, ax, v, k, k, lam, r, k, s);
+    gram_f64(n, r, k, k, r, k, k, g, ws, s);
+    gram_f64(n, v, k, k, v, k, k, gx, ws, s);
+    std::vector<double> hg((size_t)k * k), hx((size_t)k * k);
+    to_host(hg.data(), g.get(), hg.size(), s);
+    to_host(hx.data(), gx.get(), hx.size(), s);
+    for (int c = 0; c < k; ++c) {
+        double vn = std::sqrt(std::fmax(hx[c * k + c], 0.0));
+        if (vn < 2.220446049250313e-16) vn = 2.220446049250313e-16;
+        res_host[c] = std::sqrt(std::fmax(hg[c * k + c], 0.0)) / vn;
+    }
+}
+
+// rayleigh_ritz_refine (lobpcg.rs:158-187): AX = L X, G = X^T AX, eigh (lower
+// triangle), X <- X V.
+bool rayleigh_ritz_f64(const DevLaplacian& L, double* v, int k, double* evals_host, hipStream_t s) {
+    const int64_t n = L.n;
+    DevBuf<double> ax((size_t)n * k), g((size_t)k * k), w(k), vv((size_t)k * k),
+        ws(gram_workspace_doubles(n, k, k));
+    spmm_f64(L, v, ax, k, k, 1.0, 0.0, 0.0, nullptr, s);
+    gram_f64(n, v, k, k, ax, k, k, g, ws, s);
+    std::vector<double> hg((size_t)k * k);
+    to_host(hg.data(), g.get(), hg.size(), s);
+    for (double x : hg)
+        if (!std::isfinite(x)) return false;
+    small_sym_eig(k, g, k, w, vv, s);
+    block_mul_f64(n, v, k, k, vv, k, nullptr, 0, 0, nullptr, v, k, s);
+    to_host(evals_host, w.get(), k, s);
+    return true;
+}
+
+// chebyshev_precond (lobpcg.rs:113-150) through the fused SpMM epilogue:
+//   y1 = (L r - c r) / e,   y_{j+1} = scale L y_j - shift y_j - y_{j-1}.
+void chebyshev_filter_f64(const DevLaplacian& L, const double* r, int k, double a, double b,
+                          int degree, double* out, hipStream_t s) {
+    const int64_t n = L.n;
+    const size_t cnt = (size_t)n * k;
+    const double c = (a + b) / 2.0, e = (b - a) / 2.0;
+    const double scale = 2.0 / e, shift = scale * c;
+    if (degree <= 0) {
+        SI_HIP(hipMemcpyAsync(out, r, sizeof(double) * cnt, hipMemcpyDeviceToDevice, s));
+        return;
+    }
+    DevBuf<double> b0(cnt), b1(cnt), b2(cnt);
+    SI_HIP(hipMemcpyAsync(b0.get(), r, sizeof(double) * cnt, hipMemcpyDeviceToDevice, s));
+    spmm_f64(L, b0, b1, k, k, 1.0 / e, -c / e, 0.0, nullptr, s);
+    double* yprev = b0.get();
+    double* y = b1.get();
+    double* ynew = b2.get();
+    for (int d = 2; d <= degree; ++d) {
+        spmm_f64(L, y, ynew, k, k, scale, -shift, -1.0, yprev, s);
+        double* t = yprev; yprev = y; y = ynew; ynew = t;
+    }
+    SI_HIP(hipMemcpyAsync(out, y, sizeof(double) * cnt, hipMemcpyDeviceToDevice, s));
+    SI_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace si

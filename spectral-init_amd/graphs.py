@@ -1,0 +1,183 @@
+"""Synthetic fuzzy k-NN graphs: the inputs of the eigensolver path.
+
+This is input preparation (the step before the reference's hot path), used by
+the tests, the benchmark and the smoke check to produce seeded workloads:
+
+  points   Gaussian mixtures (optionally with a uniform background fraction)
+  kNN      exact k nearest neighbours, self excluded, Euclidean
+  weights  UMAP fuzzy weights w_ij = exp(-(d_ij - rho_i)/sigma_i), rho_i the
+           nearest-neighbour distance, sigma_i by binary search so that
+           sum_j w_ij = log2(k)
+  union    W = w + w^T - w o w^T (float32, bitwise symmetric), CSR with sorted
+           columns, int64 row pointers, int32 columns
+
+The float32 union formula is the fuzzy union of the reference's fixture
+generator (tests/generate_fixtures.py:145-160).  The distance/top-k work runs
+through torch (on the GPU when one is present): it is data preparation, not the
+measured path.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+# ----------------------------------------------------------------- points
+def gmm_points(n, dim, sizes, seed, center_scale=10.0, background_frac=0.0):
+    """Gaussian mixture with unit-variance clusters; centers ~ N(0, center_scale^2 I).
+
+    sizes: cluster proportions (normalised) or absolute sizes summing to the
+    non-background count.  The background fraction is drawn uniformly from the
+    bounding box of the centers.  Points are shuffled (no locality for free).
+    """
+    rng = np.random.default_rng(seed)
+    n_bg = int(round(background_frac * n))
+    n_fg = n - n_bg
+    p = np.asarray(sizes, dtype=np.float64)
+    p = p / p.sum()
+    counts = np.floor(p * n_fg).astype(np.int64)
+    counts[: n_fg - counts.sum()] += 1
+    centers = rng.standard_normal((p.size, dim)) * center_scale
+    x = np.empty((n, dim), np.float32)
+    labels = np.empty(n, np.int64)
+    o = 0
+    for c, m in enumerate(counts):
+        x[o:o + m] = centers[c] + rng.standard_normal((m, dim))
+        labels[o:o + m] = c
+        o += m
+    if n_bg:
+        lo, hi = centers.min(axis=0), centers.max(axis=0)
+        x[o:] = lo + (hi - lo) * rng.random((n_bg, dim))
+        labels[o:] = -1
+    perm = rng.permutation(n)
+    return x[perm], labels[perm]
+
+
+# ----------------------------------------------------------------- kNN
+def knn_exact(x, k, device=None, chunk=None):
+    """Exact k-NN (self excluded).  Returns (indices int32 [n,k], dists float32 [n,k])."""
+    import torch
+
+    n = x.shape[0]
+    if device is None:
+        device = "cuda" if torch.cuda.is_available() else "cpu"
+    xt = torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(device)
+    sq = (xt * xt).sum(dim=1)
+    if chunk is None:
+        chunk = max(256, min(8192, int(2e9 // (4 * max(n, 1)))))
+    idx_out = np.empty((n, k), np.int32)
+    dst_out = np.empty((n, k), np.float32)
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        d2 = sq[s:e, None] - 2.0 * (xt[s:e] @ xt.T) + sq[None, :]
+        d2[torch.arange(e - s, device=device), torch.arange(s, e, device=device)] = float("inf")
+        vals, idx = torch.topk(d2, k, dim=1, largest=False, sorted=True)
+        # exact distances for the selected pairs (avoid cancellation in the expansion)
+        diff = xt[s:e, None, :] - xt[idx]
+        dd = torch.sqrt((diff * diff).sum(dim=2))
+        dd, order = torch.sort(dd, dim=1)
+        idx = torch.gather(idx, 1, order)
+        idx_out[s:e] = idx.to(torch.int32).cpu().numpy()
+        dst_out[s:e] = dd.to(torch.float32).cpu().numpy()
+    return idx_out, dst_out
+
+
+# ----------------------------------------------------------------- weights
+def smooth_knn(dists, k, n_iter=64, tol=1e-5):
+    """rho = nearest distance; sigma by bisection so that sum_j exp(-(d-rho)/sigma) = log2(k)."""
+    d = np.asarray(dists, np.float32)
+    n = d.shape[0]
+    target = np.float32(math.log2(k))
+    rho = d[:, 0].copy()
+    lo = np.zeros(n, np.float32)
+    hi = np.full(n, np.inf, np.float32)
+    mid = np.ones(n, np.float32)
+    done = np.zeros(n, bool)
+    gap = np.maximum(d - rho[:, None], 0.0)
+    for _ in range(n_iter):
+        psum = np.exp(-gap / mid[:, None]).sum(axis=1, dtype=np.float32)
+        hit = np.abs(psum - target) < tol
+        done |= hit
+        upd = ~done
+        over = upd & (psum > target)
+        under = upd & ~(psum > target)
+        hi = np.where(over, mid, hi)
+        lo = np.where(under, mid, lo)
+        mid = np.where(over, (lo + hi) / 2,
+                       np.where(under, np.where(np.isinf(hi), mid * 2, (lo + hi) / 2), mid))
+        if done.all():
+            break
+    mean_d = d.mean(axis=1)
+    sigma = np.maximum(mid, np.float32(1e-3) * mean_d).astype(np.float32)
+    return sigma, rho
+
+
+def fuzzy_union(knn_idx, knn_dst, sigma, rho):
+    """Symmetric fuzzy union W = w + w^T - w o w^T as sorted float32 CSR."""
+    import scipy.sparse as sp
+
+    n, k = knn_idx.shape
+    gap = np.maximum(knn_dst - rho[:, None], 0.0).astype(np.float32)
+    w = np.exp(-gap / sigma[:, None]).astype(np.float32)
+    rows = np.repeat(np.arange(n, dtype=np.int64), k)
+    a = sp.csr_matrix((w.ravel(), (rows, knn_idx.ravel().astype(np.int64))), shape=(n, n),
+                      dtype=np.float32)
+    a.sum_duplicates()
+    at = a.T.tocsr()
+    s = (a + at - a.multiply(at)).tocsr()
+    s.sum_duplicates()
+    s.eliminate_zeros()
+    s.sort_indices()
+    return (s.indptr.astype(np.int64), s.indices.astype(np.int32), s.data.astype(np.float32))
+
+
+def fuzzy_knn_graph(x, k=15, device=None):
+    idx, dst = knn_exact(x, k, device=device)
+    sigma, rho = smooth_knn(dst, k)
+    return fuzzy_union(idx, dst, sigma, rho)
+
+
+def n_connected_components(indptr, indices):
+    import scipy.sparse as sp
+    import scipy.sparse.csgraph as cg
+
+    n = indptr.size - 1
+    a = sp.csr_matrix((np.ones(indices.size, np.int8), indices, indptr), shape=(n, n))
+    return cg.connected_components(a, directed=False)[0]
+
+
+# ----------------------------------------------------------------- workloads
+def workload(name, seed=0, device=None, n=None):
+    """Seeded inputs of the named workloads.
+
+    blobs2k   : n=2000, 10 clusters in 50-d, equal sizes (CPU-runnable case)
+    gmm100k   : n=100 000, 20 clusters in 50-d, Dirichlet(2) sizes
+    gmm1m     : n=1 000 000, 40 clusters in 30-d, Zipf(1.2) sizes, 5 % background
+    n overrides the node count (same construction, for reduced-size runs).
+    """
+    rng = np.random.default_rng(10_000 + seed)
+    if name == "blobs2k":
+        n = n or 2000
+        sizes = np.ones(10)
+        x, lab = gmm_points(n, 50, sizes, seed, center_scale=CENTER_SCALE["blobs2k"])
+    elif name == "gmm100k":
+        n = n or 100_000
+        sizes = rng.dirichlet(np.full(20, 2.0))
+        x, lab = gmm_points(n, 50, sizes, seed, center_scale=CENTER_SCALE["gmm100k"])
+    elif name == "gmm1m":
+        n = n or 1_000_000
+        sizes = 1.0 / np.arange(1, 41) ** 1.2
+        x, lab = gmm_points(n, 30, sizes, seed, center_scale=CENTER_SCALE["gmm1m"],
+                            background_frac=0.05)
+    else:
+        raise ValueError(name)
+    indptr, indices, data = fuzzy_knn_graph(x, 15, device=device)
+    return dict(name=name, n=n, x=x, labels=lab, indptr=indptr, indices=indices, data=data)
+
+
+# Center spread per workload.  With N(0, 10^2 I) centers and unit-variance
+# clusters in 30-50 dimensions the clusters are farther apart than any k-NN
+# edge reaches, so the graph splits into one component per cluster; the
+# spreads below keep clear cluster structure while the graph stays connected.
+CENTER_SCALE = {"blobs2k": 0.6, "gmm100k": 0.6, "gmm1m": 0.8}

@@ -1,0 +1,443 @@
+"""Python mirror of spectral-init's public interface over the MI355X C ABI.
+
+Names follow the reference crate: ``spectral_init``, ``SpectralInitConfig``,
+``ComputeMode``, ``SpectralError`` (``ConvergenceFailure``, ``InvalidGraph``,
+``TooFewNodes``), plus the seams the reference exposes for testing
+(``compute_degrees``/``build_normalized_laplacian`` -> ``Laplacian``,
+``CsrOperator.apply`` -> ``Laplacian.spmm``, ``lobpcg_solve``,
+``solve_eigenproblem``, ``rsvd_solve``, ``dense_evd``,
+``max_eigenpair_residual``).
+
+The library is ``lib/libspectral_mi355x.so`` (built by ``build.py``).  Loading
+it does not need a GPU; every compute call does, and fails loudly without one.
+Device blocks are passed as torch CUDA tensors (torch is only the allocator
+here) or raw integer device addresses.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import os
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libspectral_mi355x.so")
+
+SI_OK, SI_ERR_CONVERGENCE, SI_ERR_INVALID_GRAPH, SI_ERR_TOO_FEW_NODES = 0, 1, 2, 3
+SI_ERR_HIP, SI_ERR_INTERNAL, SI_ERR_INVALID_ARGUMENT, SI_ERR_UNSUPPORTED = 4, 5, 6, 7
+SI_F32, SI_F64 = 0, 1
+
+# every symbol the public and testing headers declare
+EXPORTED = [
+    "si_version", "si_config_default", "si_context_create", "si_context_destroy",
+    "si_last_error", "si_laplacian_build", "si_laplacian_destroy", "si_laplacian_info",
+    "si_laplacian_export", "si_spmm", "si_chebyshev_filter", "si_max_eigenpair_residual",
+    "si_lobpcg_solve", "si_solve_eigenproblem", "si_rsvd_solve", "si_dense_evd",
+    "si_spectral_init",
+    "si_test_gram", "si_test_block_mul", "si_test_small_sym_eig", "si_test_small_gen_eig",
+    "si_test_fill_normal", "si_test_fast_solve",
+]
+
+
+class ComputeMode(enum.IntEnum):
+    """src/config.rs:3-14"""
+    PythonCompat = 0
+    RustNative = 1
+
+
+class Solver(enum.IntEnum):
+    Fast = 0
+    PlainLobpcg = 1
+
+
+class _Config(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("compute_mode", C.c_int32), ("solver", C.c_int32),
+                ("block_size", C.c_int32), ("cheb_degree", C.c_int32), ("max_outer", C.c_int32),
+                ("tol", C.c_double), ("dense_threshold", C.c_int64), ("warm_start", C.c_void_p)]
+
+
+class _Report(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("level", C.c_int32), ("restarts", C.c_int32),
+                ("block_size", C.c_int32), ("iterations", C.c_int64), ("spmm_calls", C.c_int64),
+                ("max_residual", C.c_double), ("eigenvalues", C.c_double * 8),
+                ("solve_ms", C.c_double), ("laplacian_ms", C.c_double)]
+
+
+@dataclass
+class SpectralInitConfig:
+    """SpectralInitConfig (src/config.rs:19-25) plus the device knobs."""
+    compute_mode: ComputeMode = ComputeMode.PythonCompat
+    solver: Solver = Solver.Fast
+    block_size: int = 0
+    cheb_degree: int = 8
+    max_outer: int = 60
+    tol: float = 1e-5
+    dense_threshold: int = 2000
+    warm_start: Optional[int] = None  # device address of an [n x (n_components+1)] f64 block
+
+    def _c(self) -> _Config:
+        c = _Config()
+        c.struct_size = C.sizeof(_Config)
+        c.compute_mode = int(self.compute_mode)
+        c.solver = int(self.solver)
+        c.block_size = int(self.block_size)
+        c.cheb_degree = int(self.cheb_degree)
+        c.max_outer = int(self.max_outer)
+        c.tol = float(self.tol)
+        c.dense_threshold = int(self.dense_threshold)
+        c.warm_start = self.warm_start
+        return c
+
+
+@dataclass
+class Report:
+    level: int = -1
+    restarts: int = 0
+    block_size: int = 0
+    iterations: int = 0
+    spmm_calls: int = 0
+    max_residual: float = float("nan")
+    eigenvalues: list = field(default_factory=list)
+    solve_ms: float = 0.0
+    laplacian_ms: float = 0.0
+
+    @staticmethod
+    def _from(r: _Report, k: int = 8) -> "Report":
+        return Report(r.level, r.restarts, r.block_size, r.iterations, r.spmm_calls,
+                      r.max_residual, list(r.eigenvalues)[:k], r.solve_ms, r.laplacian_ms)
+
+
+class SpectralError(Exception):
+    """SpectralError (src/lib.rs:167-182)."""
+    code = SI_ERR_INTERNAL
+
+
+class ConvergenceFailure(SpectralError):
+    code = SI_ERR_CONVERGENCE
+
+
+class InvalidGraph(SpectralError):
+    code = SI_ERR_INVALID_GRAPH
+
+
+class TooFewNodes(SpectralError):
+    code = SI_ERR_TOO_FEW_NODES
+
+
+class DeviceError(SpectralError):
+    code = SI_ERR_HIP
+
+
+class Unsupported(SpectralError):
+    code = SI_ERR_UNSUPPORTED
+
+
+_ERRORS = {SI_ERR_CONVERGENCE: ConvergenceFailure, SI_ERR_INVALID_GRAPH: InvalidGraph,
+           SI_ERR_TOO_FEW_NODES: TooFewNodes, SI_ERR_HIP: DeviceError,
+           SI_ERR_INVALID_ARGUMENT: ValueError, SI_ERR_UNSUPPORTED: Unsupported}
+
+_lib = None
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def lib():
+    """Load the native library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run spectral-init_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64, u64, dbl = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64, C.c_double
+        L.si_version.restype = C.c_char_p
+        L.si_last_error.restype = C.c_char_p
+        L.si_last_error.argtypes = [vp]
+        L.si_config_default.argtypes = [C.POINTER(_Config)]
+        L.si_context_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.si_context_destroy.argtypes = [vp]
+        L.si_laplacian_build.argtypes = [vp, i64, vp, vp, vp, C.c_int, i32, C.POINTER(vp)]
+        L.si_laplacian_destroy.argtypes = [vp]
+        L.si_laplacian_info.argtypes = [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64),
+                                        C.POINTER(i32)]
+        L.si_laplacian_export.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.POINTER(i64)]
+        L.si_spmm.argtypes = [vp, vp, i32, vp, vp, i32, i32]
+        L.si_chebyshev_filter.argtypes = [vp, vp, vp, i32, dbl, dbl, i32, vp]
+        L.si_max_eigenpair_residual.argtypes = [vp, vp, vp, vp, i32, C.POINTER(dbl)]
+        L.si_lobpcg_solve.argtypes = [vp, vp, i32, u64, i32, dbl, vp, vp, C.POINTER(_Report)]
+        L.si_solve_eigenproblem.argtypes = [vp, vp, i32, u64, C.POINTER(_Config), vp, vp,
+                                            C.POINTER(_Report)]
+        L.si_rsvd_solve.argtypes = [vp, vp, i32, u64, i32, i32, vp, vp, C.POINTER(_Report)]
+        L.si_dense_evd.argtypes = [vp, vp, i32, vp, vp]
+        L.si_spectral_init.argtypes = [vp, i64, vp, vp, vp, i32, u64, vp, i64, C.POINTER(_Config),
+                                       vp, C.POINTER(_Report)]
+        L.si_test_gram.argtypes = [vp, i32, i64, vp, i32, i32, vp, i32, i32, vp]
+        L.si_test_block_mul.argtypes = [vp, i32, i64, vp, i32, i32, vp, i32, vp, i32]
+        L.si_test_small_sym_eig.argtypes = [vp, i32, vp, vp, vp]
+        L.si_test_small_gen_eig.argtypes = [vp, i32, vp, vp, vp, vp]
+        L.si_test_fill_normal.argtypes = [vp, vp, i64, i32, u64, C.c_uint32]
+        L.si_test_fast_solve.argtypes = [vp, vp, i32, u64, C.POINTER(_Config), vp, vp,
+                                         C.POINTER(_Report)]
+        for name in EXPORTED:
+            if name not in ("si_version", "si_last_error", "si_config_default",
+                            "si_context_destroy", "si_laplacian_destroy"):
+                getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a) -> Optional[int]:
+    """Address of a numpy array (host) or a torch tensor (device or host)."""
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return a
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    return a.ctypes.data
+
+
+class Context:
+    """si_context: one HIP device and one stream."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        self._check(lib().si_context_create(device, C.byref(self._h)))
+
+    def _check(self, st: int):
+        if st != SI_OK:
+            msg = lib().si_last_error(self._h).decode() if self._h else ""
+            raise _ERRORS.get(st, SpectralError)(msg or f"status {st}")
+
+    def close(self):
+        if self._h:
+            lib().si_context_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+
+class Laplacian:
+    """Device-resident L = I - D^-1/2 W D^-1/2 (src/laplacian.rs:29-88)."""
+
+    def __init__(self, ctx: Context, indptr, indices, data, mode: int = 0, on_device=False):
+        self.ctx = ctx
+        self._h = C.c_void_p()
+        if on_device:
+            n = indptr.numel() - 1
+            self._keep = (indptr, indices, data)
+        else:
+            indptr = np.ascontiguousarray(indptr, np.int64)
+            indices = np.ascontiguousarray(indices, np.int32)
+            data = np.ascontiguousarray(data, np.float32)
+            n = indptr.size - 1
+            self._keep = (indptr, indices, data)
+        ctx._check(lib().si_laplacian_build(ctx.handle, n, _ptr(indptr), _ptr(indices), _ptr(data),
+                                            int(on_device), int(mode), C.byref(self._h)))
+        n_, nnz, nl, ex = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
+        lib().si_laplacian_info(self._h, C.byref(n_), C.byref(nnz), C.byref(nl), C.byref(ex))
+        self.n, self.nnz, self.n_long, self.degree_exact = n_.value, nnz.value, nl.value, bool(ex.value)
+
+    def close(self):
+        if self._h:
+            lib().si_laplacian_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def export(self):
+        """(degrees, sqrt_deg, (l_indptr, l_indices, l_data)) in the reference's CSR layout."""
+        n = self.n
+        deg = np.empty(n)
+        sq = np.empty(n)
+        ip = np.empty(n + 1, np.int64)
+        ix = np.empty(self.nnz + n, np.int32)
+        d = np.empty(self.nnz + n)
+        m = C.c_int64()
+        self.ctx._check(lib().si_laplacian_export(self.ctx.handle, self._h, _ptr(deg), _ptr(sq),
+                                                  _ptr(ip), _ptr(ix), _ptr(d), C.byref(m)))
+        return deg, sq, (ip, ix[:m.value].copy(), d[:m.value].copy())
+
+    def spmm(self, x, y, precision=SI_F64):
+        """y = L x for device blocks (CsrOperator::apply, src/operator.rs:176-228)."""
+        k = x.shape[1] if x.dim() == 2 else 1
+        ld = x.stride(0) if x.dim() == 2 else 1
+        self.ctx._check(lib().si_spmm(self.ctx.handle, self._h, precision, _ptr(x), _ptr(y), k, ld))
+        return y
+
+
+def chebyshev_filter(lap: Laplacian, r, a=0.1, b=2.0, degree=8, out=None):
+    import torch
+    out = torch.empty_like(r) if out is None else out
+    lap.ctx._check(lib().si_chebyshev_filter(lap.ctx.handle, lap.handle, _ptr(r), r.shape[1], a, b,
+                                             degree, _ptr(out)))
+    return out
+
+
+def max_eigenpair_residual(lap: Laplacian, evals, evecs) -> float:
+    ev = np.ascontiguousarray(evals, np.float64)
+    out = C.c_double()
+    lap.ctx._check(lib().si_max_eigenpair_residual(lap.ctx.handle, lap.handle, _ptr(ev),
+                                                   _ptr(evecs), evecs.shape[1], C.byref(out)))
+    return out.value
+
+
+def _evecs(lap, k):
+    import torch
+    return torch.empty((lap.n, k), dtype=torch.float64, device="cuda")
+
+
+def lobpcg_solve(lap: Laplacian, n_components: int, seed: int, regularize=False, tol=1e-5):
+    """lobpcg_solve (src/solvers/lobpcg.rs:227-465), device FP64."""
+    k = n_components + 1
+    ev = np.empty(k)
+    vec = _evecs(lap, k)
+    rep = _Report()
+    rep.struct_size = C.sizeof(_Report)
+    lap.ctx._check(lib().si_lobpcg_solve(lap.ctx.handle, lap.handle, n_components, seed,
+                                         int(regularize), tol, _ptr(ev), _ptr(vec), C.byref(rep)))
+    return ev, vec, Report._from(rep, k)
+
+
+def solve_eigenproblem(lap: Laplacian, n_components: int, seed: int,
+                       config: Optional[SpectralInitConfig] = None):
+    """solve_eigenproblem (src/solvers/mod.rs:70-224)."""
+    k = n_components + 1
+    cfg = (config or SpectralInitConfig())._c()
+    ev = np.empty(k)
+    vec = _evecs(lap, k)
+    rep = _Report()
+    rep.struct_size = C.sizeof(_Report)
+    lap.ctx._check(lib().si_solve_eigenproblem(lap.ctx.handle, lap.handle, n_components, seed,
+                                               C.byref(cfg), _ptr(ev), _ptr(vec), C.byref(rep)))
+    return ev, vec, Report._from(rep, k)
+
+
+def fast_solve(lap: Laplacian, n_components: int, seed: int,
+               config: Optional[SpectralInitConfig] = None):
+    """The FP32-bulk Chebyshev-filtered block solver alone (level 1)."""
+    k = n_components + 1
+    cfg = (config or SpectralInitConfig(tol=1e-6))._c()
+    ev = np.empty(k)
+    vec = _evecs(lap, k)
+    rep = _Report()
+    rep.struct_size = C.sizeof(_Report)
+    lap.ctx._check(lib().si_test_fast_solve(lap.ctx.handle, lap.handle, n_components, seed,
+                                            C.byref(cfg), _ptr(ev), _ptr(vec), C.byref(rep)))
+    return ev, vec, Report._from(rep, k)
+
+
+def rsvd_solve(lap: Laplacian, n_components: int, seed: int, oversampling=-1, nbiter=2):
+    """rsvd_solve (src/solvers/rsvd.rs:135-213)."""
+    k = n_components + 1
+    ev = np.empty(k)
+    vec = _evecs(lap, k)
+    rep = _Report()
+    rep.struct_size = C.sizeof(_Report)
+    lap.ctx._check(lib().si_rsvd_solve(lap.ctx.handle, lap.handle, n_components, seed,
+                                       oversampling, nbiter, _ptr(ev), _ptr(vec), C.byref(rep)))
+    return ev, vec, Report._from(rep, k)
+
+
+def dense_evd(lap: Laplacian, k: int):
+    """dense_evd (src/solvers/dense.rs:9-46), n <= 64 on the device."""
+    ev = np.empty(k)
+    vec = _evecs(lap, k)
+    lap.ctx._check(lib().si_dense_evd(lap.ctx.handle, lap.handle, k, _ptr(ev), _ptr(vec)))
+    return ev, vec
+
+
+def spectral_init(graph, n_components: int, seed: int, data=None,
+                  config: Optional[SpectralInitConfig] = None, ctx: Optional[Context] = None,
+                  return_report: bool = False):
+    """spectral_init (src/lib.rs:198-258).
+
+    graph: (indptr, indices, data) CSR triple or a scipy.sparse CSR matrix
+    (f32 weights, u32/i32 columns).  Returns an (n, n_components) float32 array.
+    """
+    if hasattr(graph, "indptr"):
+        indptr, indices, weights = graph.indptr, graph.indices, graph.data
+    else:
+        indptr, indices, weights = graph
+    indptr = np.ascontiguousarray(indptr, np.int64)
+    indices = np.ascontiguousarray(indices, np.int32)
+    weights = np.ascontiguousarray(weights, np.float32)
+    n = indptr.size - 1
+    own = ctx is None
+    ctx = ctx or Context(0)
+    cfg = (config or SpectralInitConfig())._c()
+    raw = None if data is None else np.ascontiguousarray(data, np.float32)
+    out = np.empty((max(n, 0), n_components), np.float32)
+    rep = _Report()
+    rep.struct_size = C.sizeof(_Report)
+    try:
+        ctx._check(lib().si_spectral_init(ctx.handle, n, _ptr(indptr), _ptr(indices), _ptr(weights),
+                                          n_components, seed, _ptr(raw),
+                                          0 if raw is None else raw.shape[1], C.byref(cfg),
+                                          _ptr(out), C.byref(rep)))
+    finally:
+        if own:
+            ctx.close()
+    return (out, Report._from(rep, n_components + 1)) if return_report else out
+
+
+# ---------------------------------------------------------------- kernel seams (tests)
+def gram(ctx: Context, a, b, precision=None) -> np.ndarray:
+    import torch
+    prec = SI_F32 if a.dtype == torch.float32 else SI_F64
+    p, q = a.shape[1], b.shape[1]
+    g = np.empty((p, q))
+    ctx._check(lib().si_test_gram(ctx.handle, prec, a.shape[0], _ptr(a), a.stride(0), p, _ptr(b),
+                                  b.stride(0), q, _ptr(g)))
+    return g
+
+
+def block_mul(ctx: Context, a, c: np.ndarray, out):
+    import torch
+    prec = SI_F32 if a.dtype == torch.float32 else SI_F64
+    c = np.ascontiguousarray(c, np.float64)
+    ctx._check(lib().si_test_block_mul(ctx.handle, prec, a.shape[0], _ptr(a), a.stride(0),
+                                       a.shape[1], _ptr(c), c.shape[1], _ptr(out), out.stride(0)))
+    return out
+
+
+def small_sym_eig(ctx: Context, a: np.ndarray):
+    a = np.ascontiguousarray(a, np.float64)
+    n = a.shape[0]
+    w, v = np.empty(n), np.empty((n, n))
+    ctx._check(lib().si_test_small_sym_eig(ctx.handle, n, _ptr(a), _ptr(w), _ptr(v)))
+    return w, v
+
+
+def small_gen_eig(ctx: Context, a: np.ndarray, b: np.ndarray):
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    n = a.shape[0]
+    w, c = np.empty(n), np.empty((n, n))
+    ctx._check(lib().si_test_small_gen_eig(ctx.handle, n, _ptr(a), _ptr(b), _ptr(w), _ptr(c)))
+    return w, c
+
+
+def fill_normal(ctx: Context, x, seed: int, stream: int):
+    ctx._check(lib().si_test_fill_normal(ctx.handle, _ptr(x), x.shape[0], x.shape[1], seed, stream))
+    return x

@@ -1,0 +1,66 @@
+"""CPU checks of the C-ABI library: it loads without a GPU and exports every
+entry point the public headers declare."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import spectral_mi355x as si
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", "spectral_mi355x.h"),
+           os.path.join(ROOT, "include", "spectral_mi355x_testing.h")]
+
+
+def _declared():
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(si_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+@pytest.fixture(scope="module")
+def library():
+    if not os.path.exists(si.library_path()):
+        pytest.fail("native library not built (run spectral-init_amd/build.py)")
+    return ctypes.CDLL(si.library_path())
+
+
+def test_headers_declare_the_bound_symbols():
+    assert _declared() == set(si.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol(library):
+    missing = [n for n in _declared() if not hasattr(library, n)]
+    assert not missing, missing
+
+
+def test_version_and_default_config_without_gpu(library):
+    lib = si.lib()
+    assert b"gfx950" in lib.si_version()
+    c = si._Config()
+    lib.si_config_default(ctypes.byref(c))
+    # the reference's defaults: PythonCompat, tol 1e-5, dense threshold 2000, degree 8
+    assert c.struct_size == ctypes.sizeof(si._Config)
+    assert c.compute_mode == 0 and c.tol == 1e-5 and c.dense_threshold == 2000
+    assert c.cheb_degree == 8
+
+
+def test_context_create_fails_cleanly_without_a_device():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    h = ctypes.c_void_p()
+    st = si.lib().si_context_create(0, ctypes.byref(h))
+    assert st != si.SI_OK and not h.value
+
+
+def test_product_does_not_link_the_oracle(library):
+    # the shared library must not depend on the CPU oracle
+    import subprocess
+    out = subprocess.run(["ldd", si.library_path()], capture_output=True, text=True).stdout
+    assert "oracle" not in out
+    assert not hasattr(library, "or_spmm")

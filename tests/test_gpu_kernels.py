@@ -1,0 +1,237 @@
+"""GPU parity of the device kernels against the FP64 CPU oracle (through the C ABI).
+
+Tolerances: integer/structure and FP64-exact paths are compared bit for bit;
+FP64 arithmetic paths to 1e-12 relative; FP32 SpMM to 4 ulp of the
+sum of |terms| (the FP32 accumulation bound).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import DATASETS, load_fixture
+from helpers import random_symmetric_graph, ring_graph
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+si = pytest.importorskip("spectral_mi355x")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    c = si.Context(0)
+    yield c
+    c.close()
+
+
+def _hub_graph(n=3000, hubs=4, seed=7):
+    """Random graph plus a few hub rows with > 256 entries (long-row kernel)."""
+    import scipy.sparse as sp
+    ip, ix, w = random_symmetric_graph(n, 10, seed)
+    a = sp.csr_matrix((w, ix, ip), shape=(n, n)).tolil()
+    rng = np.random.default_rng(seed)
+    for h in range(hubs):
+        nb = rng.choice(n, 600 + 150 * h, replace=False)
+        nb = nb[nb != h]
+        wv = rng.uniform(0.1, 1.0, nb.size).astype(np.float32)
+        a[h, nb] = wv
+        a[nb, h] = wv[:, None]
+    a = a.tocsr()
+    a.sort_indices()
+    return a.indptr.astype(np.int64), a.indices.astype(np.int32), a.data.astype(np.float32)
+
+
+# ------------------------------------------------------------------ a1 / a2
+@pytest.mark.parametrize("name", DATASETS)
+def test_laplacian_build_matches_oracle_bitwise(ctx, name):
+    z = load_fixture(name)
+    print(f"case laplacian {name}", flush=True)
+    lap = si.Laplacian(ctx, z["w_indptr"], z["w_indices"], z["w_data"], mode=0)
+    deg, sq, (ip, ix, d) = lap.export()
+    assert lap.degree_exact
+    # PythonCompat degrees reproduce the fixture bit for bit (test_comp_a_degrees.rs:41-47)
+    np.testing.assert_array_equal(deg, z["degrees"])
+    np.testing.assert_array_equal(sq, z["sqrt_deg"])
+    (oip, oix, od), _, _ = O.laplacian_from_graph(z["w_indptr"], z["w_indices"], z["w_data"])
+    np.testing.assert_array_equal(ip, oip)
+    np.testing.assert_array_equal(ix, oix)
+    np.testing.assert_array_equal(d, od)          # same f64 expression, same bits
+    np.testing.assert_allclose(d, z["l_data"], rtol=0, atol=1e-14)
+    lap.close()
+
+
+def test_laplacian_rust_native_degrees(ctx):
+    z = load_fixture("blobs_500")
+    lap = si.Laplacian(ctx, z["w_indptr"], z["w_indices"], z["w_data"], mode=1)
+    deg, _, _ = lap.export()
+    odeg, _ = O.compute_degrees(z["w_indptr"], z["w_indices"], z["w_data"], 1)
+    np.testing.assert_allclose(deg, odeg, rtol=1e-14)
+
+
+def test_laplacian_rejects_bad_graphs(ctx):
+    ip = np.array([0, 1, 2], np.int64)
+    with pytest.raises(si.InvalidGraph):
+        si.Laplacian(ctx, ip, np.array([1, 5], np.int32), np.ones(2, np.float32))
+    with pytest.raises(si.InvalidGraph):
+        si.Laplacian(ctx, ip, np.array([1, 0], np.int32), np.array([np.nan, 1.0], np.float32))
+
+
+def test_laplacian_asymmetric_input_falls_back(ctx):
+    ip = np.array([0, 1, 2, 3], np.int64)
+    ix = np.array([1, 2, 0], np.int32)
+    w = np.array([0.5, 0.25, 1.0], np.float32)
+    lap = si.Laplacian(ctx, ip, ix, w)
+    deg, _, _ = lap.export()
+    odeg, _ = O.compute_degrees(ip, ix, w, 0)
+    np.testing.assert_allclose(deg, odeg, rtol=1e-7)
+    assert not lap.degree_exact
+
+
+# ------------------------------------------------------------------ a3 / a4
+GRAPHS = {
+    "blobs_connected_2000": lambda: tuple(load_fixture("blobs_connected_2000")[k]
+                                          for k in ("w_indptr", "w_indices", "w_data")),
+    "random5000": lambda: random_symmetric_graph(5000, 20, 11),
+    "hubs3000": lambda: _hub_graph(),
+    "ring1000": lambda: ring_graph(1000),
+}
+
+
+@pytest.mark.parametrize("gname", list(GRAPHS))
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 8, 16, 32, 64])
+def test_spmm_fp64_matches_oracle(ctx, gname, k):
+    print(f"case spmm64 {gname} k={k}", flush=True)
+    ip, ix, w = GRAPHS[gname]()
+    lap = si.Laplacian(ctx, ip, ix, w)
+    L, _, _ = O.laplacian_from_graph(ip, ix, w)
+    if gname == "hubs3000":
+        assert lap.n_long > 0
+    x = np.random.default_rng(k).standard_normal((lap.n, k))
+    ref = O.spmm(L, x)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.empty_like(xd)
+    lap.spmm(xd, yd, si.SI_F64)
+    y = yd.cpu().numpy()
+    absd = O.spmm((L[0], L[1], np.abs(L[2])), np.abs(x))
+    assert np.all(np.abs(y - ref) <= 1e-14 * absd + 1e-300)
+
+
+@pytest.mark.parametrize("gname", list(GRAPHS))
+@pytest.mark.parametrize("k", [1, 3, 8, 16, 32, 64])
+def test_spmm_fp32_within_accumulation_bound(ctx, gname, k):
+    print(f"case spmm32 {gname} k={k}", flush=True)
+    ip, ix, w = GRAPHS[gname]()
+    lap = si.Laplacian(ctx, ip, ix, w)
+    L, _, _ = O.laplacian_from_graph(ip, ix, w)
+    x = np.random.default_rng(100 + k).standard_normal((lap.n, k)).astype(np.float32)
+    ref = O.spmm(L, x.astype(np.float64))
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.empty_like(xd)
+    lap.spmm(xd, yd, si.SI_F32)
+    y = yd.cpu().numpy().astype(np.float64)
+    absd = O.spmm((L[0], L[1], np.abs(L[2])), np.abs(x.astype(np.float64)))
+    rowlen = np.diff(L[0])[:, None].astype(np.float64)
+    bound = (rowlen + 2) * 2.0 ** -24 * absd * 4
+    assert np.all(np.abs(y - ref) <= bound)
+
+
+def test_spmm_respects_leading_dimension(ctx):
+    ip, ix, w = random_symmetric_graph(2000, 12, 3)
+    lap = si.Laplacian(ctx, ip, ix, w)
+    L, _, _ = O.laplacian_from_graph(ip, ix, w)
+    xfull = np.random.default_rng(0).standard_normal((2000, 9))
+    xd = torch.from_numpy(xfull).cuda()
+    yd = torch.zeros_like(xd)
+    lap.spmm(xd[:, :3], yd[:, :3], si.SI_F64)
+    y = yd.cpu().numpy()
+    np.testing.assert_allclose(y[:, :3], O.spmm(L, xfull[:, :3]), rtol=1e-12, atol=1e-13)
+    assert np.all(y[:, 3:] == 0)
+
+
+def test_spmm_is_deterministic(ctx):
+    ip, ix, w = _hub_graph()
+    lap = si.Laplacian(ctx, ip, ix, w)
+    x = torch.randn(lap.n, 32, dtype=torch.float32, device="cuda")
+    y1, y2 = torch.empty_like(x), torch.empty_like(x)
+    lap.spmm(x, y1, si.SI_F32)
+    lap.spmm(x, y2, si.SI_F32)
+    assert torch.equal(y1, y2)
+
+
+# ------------------------------------------------------------------ a5
+@pytest.mark.parametrize("degree", [1, 2, 8])
+def test_chebyshev_filter_matches_oracle(ctx, degree):
+    z = load_fixture("blobs_connected_2000")
+    lap = si.Laplacian(ctx, z["w_indptr"], z["w_indices"], z["w_data"])
+    L, _, _ = O.laplacian_from_graph(z["w_indptr"], z["w_indices"], z["w_data"])
+    r = np.random.default_rng(degree).standard_normal((lap.n, 3))
+    ref = O.chebyshev_filter(L, r, 0.1, 2.0, degree)
+    out = si.chebyshev_filter(lap, torch.from_numpy(r).cuda(), 0.1, 2.0, degree).cpu().numpy()
+    np.testing.assert_allclose(out, ref, rtol=1e-11, atol=1e-11 * np.abs(ref).max())
+
+
+# ------------------------------------------------------------------ dense blocks
+@pytest.mark.parametrize("p,q", [(1, 1), (3, 3), (8, 8), (9, 17), (16, 16), (24, 24), (32, 32),
+                                 (48, 7), (64, 64)])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_gram_matches_numpy(ctx, p, q, dtype):
+    print(f"case gram {p}x{q} {dtype}", flush=True)
+    n = 12345
+    rng = np.random.default_rng(p * 100 + q)
+    a = rng.standard_normal((n, p))
+    b = rng.standard_normal((n, q))
+    if dtype == "f32":
+        a, b = a.astype(np.float32), b.astype(np.float32)
+    g = si.gram(ctx, torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda())
+    ref = a.astype(np.float64).T @ b.astype(np.float64)
+    np.testing.assert_allclose(g, ref, rtol=1e-12, atol=1e-10)
+
+
+@pytest.mark.parametrize("p,q", [(3, 3), (8, 5), (16, 16), (32, 32), (64, 64), (24, 8)])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_block_mul_matches_numpy(ctx, p, q, dtype):
+    n = 7001
+    rng = np.random.default_rng(p + q)
+    a = rng.standard_normal((n, p))
+    c = rng.standard_normal((p, q))
+    if dtype == "f32":
+        a = a.astype(np.float32)
+    ad = torch.from_numpy(a).cuda()
+    out = torch.empty((n, q), dtype=ad.dtype, device="cuda")
+    si.block_mul(ctx, ad, c, out)
+    ref = a.astype(np.float64) @ c
+    tol = 1e-12 if dtype == "f64" else 1e-5
+    np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=tol, atol=tol * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 9, 16, 33, 48, 64])
+def test_small_sym_eig_matches_numpy(ctx, n):
+    a = np.random.default_rng(n).standard_normal((n, n))
+    a = a + a.T
+    w, v = si.small_sym_eig(ctx, a)
+    np.testing.assert_allclose(w, np.linalg.eigvalsh(a), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(a @ v, v * w, atol=1e-11)
+    np.testing.assert_allclose(v.T @ v, np.eye(n), atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [3, 9, 24, 48])
+def test_small_gen_eig_matches_scipy(ctx, n):
+    import scipy.linalg
+    rng = np.random.default_rng(n + 1)
+    a = rng.standard_normal((n, n)); a = a + a.T
+    b = rng.standard_normal((n, n)); b = b @ b.T + n * np.eye(n)
+    w, c = si.small_gen_eig(ctx, a, b)
+    np.testing.assert_allclose(w, scipy.linalg.eigh(a, b, eigvals_only=True), rtol=1e-11, atol=1e-11)
+    np.testing.assert_allclose(c.T @ b @ c, np.eye(n), atol=1e-10)
+    # and it equals the oracle's generalised solver
+    ow, _ = O.gen_sym_eig(a, b)
+    np.testing.assert_allclose(w, ow, rtol=1e-11, atol=1e-11)
+
+
+def test_fill_normal_matches_oracle_rng(ctx):
+    x = torch.empty((1000, 3), dtype=torch.float64, device="cuda")
+    si.fill_normal(ctx, x, 42, 1)
+    ref = O.normal_block(42, 1, 3000).reshape(1000, 3)
+    np.testing.assert_allclose(x.cpu().numpy(), ref, rtol=1e-13, atol=1e-13)
