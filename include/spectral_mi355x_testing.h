@@ -37,6 +37,20 @@ si_status si_test_fast_solve(si_context* ctx, const si_laplacian* lap, int32_t n
                              uint64_t seed, const si_config* cfg, double* evals, double* evecs,
                              si_report* rep);
 
+/* Live timing of every SpMM launch on the context's stream (hipEvents around
+ * each launch) between begin and end; bytes are the algorithmic bytes of
+ * each launch (nnz*(value+index) + row pointers + x read + y write). */
+typedef struct {
+    int64_t launches;
+    double bytes;
+    double ms;
+    int64_t launches_f32;
+    double bytes_f32;
+    double ms_f32;
+} si_spmm_profile;
+si_status si_profile_spmm_begin(si_context* ctx);
+si_status si_profile_spmm_end(si_context* ctx, si_spmm_profile* out);
+
 #ifdef __cplusplus
 }
 #endif

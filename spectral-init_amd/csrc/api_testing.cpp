@@ -129,4 +129,29 @@ si_status si_test_fast_solve(si_context* ctx, const si_laplacian* lap, int32_t n
     });
 }
 
+si_status si_profile_spmm_begin(si_context* ctx) {
+    if (!ctx) return SI_ERR_INVALID_ARGUMENT;
+    return tguard(ctx, [&] {
+        ctx->c.profile = si::SpmmProfile{};
+        ctx->c.profile.enabled = true;
+        si::spmm_profile_attach(&ctx->c.profile);
+    });
+}
+
+si_status si_profile_spmm_end(si_context* ctx, si_spmm_profile* out) {
+    if (!ctx || !out) return SI_ERR_INVALID_ARGUMENT;
+    return tguard(ctx, [&] {
+        si::spmm_profile_collect(&ctx->c.profile);
+        si::spmm_profile_attach(nullptr);
+        const si::SpmmProfile& p = ctx->c.profile;
+        out->launches = p.launches;
+        out->bytes = p.bytes;
+        out->ms = p.ms;
+        out->launches_f32 = p.launches_f32;
+        out->bytes_f32 = p.bytes_f32;
+        out->ms_f32 = p.ms_f32;
+        ctx->c.profile.enabled = false;
+    });
+}
+
 }  // extern "C"

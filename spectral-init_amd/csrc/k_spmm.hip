@@ -22,6 +22,8 @@ namespace si {
 
 namespace {
 
+SpmmProfile* g_profile = nullptr;  // set by the benchmark seam; host-side only
+
 template <typename T> struct Vec4;
 template <> struct Vec4<float> { using type = float4; static constexpr int n = 4; };
 template <> struct Vec4<double> { using type = double2; static constexpr int n = 2; };
@@ -224,16 +226,11 @@ void launch_wide(const DevLaplacian& L, const T* val, const T* x, T* out, int ld
                        val, x, out, z, ld, L.n, s_l, s_x, s_z);
 }
 
+// One launch of a directly supported width (1, 2, 3, 4, 8, 16, 32, 64).
 template <typename T>
-void spmm_impl(const DevLaplacian& L, const T* val, const T* x, T* out, int k, int ld, T s_l,
-               T s_x, T s_z, const T* z, hipStream_t s) {
-    SI_CHECK(ld >= k, kErrInvalidArgument, "spmm: ld < k");
-    SI_CHECK(x != out && (z == nullptr || z != out), kErrInvalidArgument,
-             "spmm: output must not alias an input");
+void spmm_chunk(const DevLaplacian& L, const T* val, const T* x, T* out, int k, int ld, T s_l,
+                T s_x, T s_z, const T* z, hipStream_t s) {
     constexpr bool F32 = sizeof(T) == 4;
-    if (k >= 16) {
-        SI_CHECK(ld % Vec4<T>::n == 0, kErrInvalidArgument, "spmm: ld must be a multiple of 16 bytes");
-    }
     switch (k) {
         case 1: launch_narrow<T, 1, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s); break;
         case 2: launch_narrow<T, 2, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s); break;
@@ -253,21 +250,87 @@ void spmm_impl(const DevLaplacian& L, const T* val, const T* x, T* out, int k, i
             else launch_wide<T, 64, 2>(L, val, x, out, ld, s_l, s_x, s_z, z, s);       // C=32, RPW=1
             break;
         default:
-            throw Error(kErrInvalidArgument, "spmm: unsupported block width " + std::to_string(k));
+            throw Error(kErrInternal, "spmm: no kernel for width " + std::to_string(k));
     }
     SI_LAUNCH_CHECK();
+}
+
+// Any width: split the columns into directly supported chunks, widest first
+// (the 16-byte vector paths need 16-byte aligned chunk offsets and leading
+// dimension; otherwise chunks of at most 8 columns are used).  Hub rows are
+// handled once for all columns.
+template <typename T>
+void spmm_impl(const DevLaplacian& L, const T* val, const T* x, T* out, int k, int ld, T s_l,
+               T s_x, T s_z, const T* z, hipStream_t s) {
+    SI_CHECK(k >= 1 && ld >= k, kErrInvalidArgument, "spmm: need 1 <= k <= ld");
+    SI_CHECK(x != out && (z == nullptr || z != out), kErrInvalidArgument,
+             "spmm: output must not alias an input");
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (g_profile && g_profile->enabled) {
+        SI_HIP(hipEventCreate(&ev0));
+        SI_HIP(hipEventCreate(&ev1));
+        SI_HIP(hipEventRecord(ev0, s));
+    }
+    const bool vec_ok = (ld % Vec4<T>::n) == 0 &&
+                        (reinterpret_cast<uintptr_t>(x) % 16) == 0 &&
+                        (reinterpret_cast<uintptr_t>(out) % 16) == 0 &&
+                        (z == nullptr || (reinterpret_cast<uintptr_t>(z) % 16) == 0);
+    const int widths_vec[] = {64, 32, 16, 8, 4, 3, 2, 1};
+    const int widths_narrow[] = {8, 4, 3, 2, 1};
+    int off = 0;
+    while (off < k) {
+        const int rem = k - off;
+        int w = 1;
+        if (vec_ok) {
+            for (int c : widths_vec)
+                if (c <= rem) { w = c; break; }
+        } else {
+            for (int c : widths_narrow)
+                if (c <= rem) { w = c; break; }
+        }
+        spmm_chunk<T>(L, val, x + off, out + off, w, ld, s_l, s_x, s_z, z ? z + off : nullptr, s);
+        off += w;
+    }
     if (L.n_long > 0) {
         hipLaunchKernelGGL((k_spmm_long<T>), dim3(L.n_long), dim3(256), 0, s, L.rowptr, L.col,
                            val, x, out, z, ld, k, L.long_rows, s_l, s_x, s_z);
         SI_LAUNCH_CHECK();
     }
+    if (ev0) {
+        SI_HIP(hipEventRecord(ev1, s));
+        const double bytes = (double)L.nnz * (sizeof(T) + 4) + 4.0 * (L.n + 1) +
+                             (double)sizeof(T) * L.n * k * (z ? 3.0 : 2.0);
+        g_profile->pending.emplace_back(ev0, ev1);
+        g_profile->pending_bytes.emplace_back(bytes, sizeof(T) == 4 ? 1 : 0);
+    }
 }
 
 }  // namespace
 
-bool spmm_width_supported(int k) {
-    return k == 1 || k == 2 || k == 3 || k == 4 || k == 8 || k == 16 || k == 32 || k == 64;
+void spmm_profile_attach(SpmmProfile* p) { g_profile = p; }
+
+void spmm_profile_collect(SpmmProfile* p) {
+    if (!p) return;
+    for (size_t i = 0; i < p->pending.size(); ++i) {
+        SI_HIP(hipEventSynchronize(p->pending[i].second));
+        float ms = 0.f;
+        SI_HIP(hipEventElapsedTime(&ms, p->pending[i].first, p->pending[i].second));
+        p->ms += ms;
+        p->bytes += p->pending_bytes[i].first;
+        p->launches += 1;
+        if (p->pending_bytes[i].second) {
+            p->ms_f32 += ms;
+            p->bytes_f32 += p->pending_bytes[i].first;
+            p->launches_f32 += 1;
+        }
+        hipEventDestroy(p->pending[i].first);
+        hipEventDestroy(p->pending[i].second);
+    }
+    p->pending.clear();
+    p->pending_bytes.clear();
 }
+
+bool spmm_width_supported(int k) { return k >= 1 && k <= 4096; }
 
 void spmm_f32(const DevLaplacian& L, const float* x, float* out, int k, int ld, float s_l,
               float s_x, float s_z, const float* z, hipStream_t s) {

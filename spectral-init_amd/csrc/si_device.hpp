@@ -12,6 +12,9 @@
 //   Dense blocks are row-major [n x ld] (ld >= width), FP32 or FP64.
 #pragma once
 
+#include <utility>
+#include <vector>
+
 #include "si_common.hpp"
 
 namespace si {
@@ -53,6 +56,23 @@ void spmm_f32(const DevLaplacian& L, const float* x, float* out, int k, int ld, 
 void spmm_f64(const DevLaplacian& L, const double* x, double* out, int k, int ld, double s_l,
               double s_x, double s_z, const double* z, hipStream_t s);
 bool spmm_width_supported(int k);
+
+// Optional live timing of every SpMM launch (hipEvents on the launching
+// stream), used by the benchmark to measure the dominant kernel inside a
+// timed solve.  Algorithmic bytes per launch: nnz*(val+4) + 4*(n+1) +
+// 2*sizeof(T)*n*k (+ sizeof(T)*n*k when the fused epilogue reads z).
+struct SpmmProfile {
+    bool enabled = false;
+    int64_t launches = 0;
+    double bytes = 0.0;       // algorithmic bytes, summed
+    double ms = 0.0;          // event time, summed
+    double bytes_f32 = 0.0, ms_f32 = 0.0;
+    int64_t launches_f32 = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    std::vector<std::pair<double, int>> pending_bytes;  // (bytes, is_f32)
+};
+void spmm_profile_attach(SpmmProfile* p);     // nullptr detaches
+void spmm_profile_collect(SpmmProfile* p);    // sync + fold pending events into the sums
 
 // ----------------------------------------------------------------- dense blocks
 // Partial Gram: G = A^T B (FP64 accumulation) for A [n x p], B [n x q].

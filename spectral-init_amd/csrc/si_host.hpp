@@ -43,6 +43,7 @@ struct Context {
     hipStream_t stream = nullptr;
     std::string last_error;
     int num_cus = 0;
+    SpmmProfile profile;
 };
 
 // Stream-ordered timer (hipEvents on the context's stream).

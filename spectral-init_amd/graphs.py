@@ -65,7 +65,8 @@ def knn_exact(x, k, device=None, chunk=None):
     xt = torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(device)
     sq = (xt * xt).sum(dim=1)
     if chunk is None:
-        chunk = max(256, min(8192, int(2e9 // (4 * max(n, 1)))))
+        budget = 16e9 if str(device).startswith("cuda") else 1e9  # bytes of one distance tile
+        chunk = max(256, min(8192, int(budget // (4 * max(n, 1)))))
     idx_out = np.empty((n, k), np.int32)
     dst_out = np.empty((n, k), np.float32)
     for s in range(0, n, chunk):

@@ -38,8 +38,13 @@ EXPORTED = [
     "si_lobpcg_solve", "si_solve_eigenproblem", "si_rsvd_solve", "si_dense_evd",
     "si_spectral_init",
     "si_test_gram", "si_test_block_mul", "si_test_small_sym_eig", "si_test_small_gen_eig",
-    "si_test_fill_normal", "si_test_fast_solve",
+    "si_test_fill_normal", "si_test_fast_solve", "si_profile_spmm_begin", "si_profile_spmm_end",
 ]
+
+
+class _SpmmProfile(C.Structure):
+    _fields_ = [("launches", C.c_int64), ("bytes", C.c_double), ("ms", C.c_double),
+                ("launches_f32", C.c_int64), ("bytes_f32", C.c_double), ("ms_f32", C.c_double)]
 
 
 class ComputeMode(enum.IntEnum):
@@ -182,6 +187,8 @@ def lib():
         L.si_test_fill_normal.argtypes = [vp, vp, i64, i32, u64, C.c_uint32]
         L.si_test_fast_solve.argtypes = [vp, vp, i32, u64, C.POINTER(_Config), vp, vp,
                                          C.POINTER(_Report)]
+        L.si_profile_spmm_begin.argtypes = [vp]
+        L.si_profile_spmm_end.argtypes = [vp, C.POINTER(_SpmmProfile)]
         for name in EXPORTED:
             if name not in ("si_version", "si_last_error", "si_config_default",
                             "si_context_destroy", "si_laplacian_destroy"):
@@ -441,3 +448,22 @@ def small_gen_eig(ctx: Context, a: np.ndarray, b: np.ndarray):
 def fill_normal(ctx: Context, x, seed: int, stream: int):
     ctx._check(lib().si_test_fill_normal(ctx.handle, _ptr(x), x.shape[0], x.shape[1], seed, stream))
     return x
+
+
+class SpmmProfiler:
+    """Event-timed SpMM launches on the context's stream (bytes are algorithmic)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.result = None
+
+    def __enter__(self):
+        self.ctx._check(lib().si_profile_spmm_begin(self.ctx.handle))
+        return self
+
+    def __exit__(self, *exc):
+        p = _SpmmProfile()
+        self.ctx._check(lib().si_profile_spmm_end(self.ctx.handle, C.byref(p)))
+        self.result = dict(launches=p.launches, bytes=p.bytes, ms=p.ms,
+                           launches_f32=p.launches_f32, bytes_f32=p.bytes_f32, ms_f32=p.ms_f32)
+        return False

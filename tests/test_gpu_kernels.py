@@ -100,7 +100,7 @@ GRAPHS = {
 
 
 @pytest.mark.parametrize("gname", list(GRAPHS))
-@pytest.mark.parametrize("k", [1, 2, 3, 4, 8, 16, 32, 64])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 8, 13, 16, 32, 33, 64])
 def test_spmm_fp64_matches_oracle(ctx, gname, k):
     print(f"case spmm64 {gname} k={k}", flush=True)
     ip, ix, w = GRAPHS[gname]()
@@ -119,7 +119,7 @@ def test_spmm_fp64_matches_oracle(ctx, gname, k):
 
 
 @pytest.mark.parametrize("gname", list(GRAPHS))
-@pytest.mark.parametrize("k", [1, 3, 8, 16, 32, 64])
+@pytest.mark.parametrize("k", [1, 3, 8, 13, 16, 32, 48, 64])
 def test_spmm_fp32_within_accumulation_bound(ctx, gname, k):
     print(f"case spmm32 {gname} k={k}", flush=True)
     ip, ix, w = GRAPHS[gname]()
