@@ -91,6 +91,10 @@ def main():
     ap.add_argument("--plain-runs", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--spmm-sweep", action="store_true",
+                    help="also time the SpMM alone for widths 1..64 (FP32 and FP64)")
+    ap.add_argument("--no-reorder", action="store_true",
+                    help="keep the input node order (no locality reordering)")
     ap.add_argument("--traffic-json", default=None,
                     help="JSON with measured HBM bytes per SpMM launch (rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -121,10 +125,28 @@ def main():
 
     ctx = si.Context(0)
     t0 = time.perf_counter()
-    lap = si.Laplacian(ctx, ip, ix, wt)
+    lap = si.Laplacian(ctx, ip, ix, wt, reorder=not args.no_reorder)
     torch.cuda.synchronize()
     lap_ms = 1e3 * (time.perf_counter() - t0)
     log(f"laplacian on device: nnz={lap.nnz} hub rows={lap.n_long} ({lap_ms:.1f} ms incl. upload)")
+
+    if args.spmm_sweep:
+        for prec, dt in ((si.SI_F32, torch.float32), (si.SI_F64, torch.float64)):
+            for kk in (1, 3, 8, 16, 32, 64):
+                x = torch.randn(n, kk, dtype=dt, device="cuda")
+                y = torch.empty_like(x)
+                lap.spmm(x, y, prec)
+                p = si.SpmmProfiler(ctx)
+                with p:
+                    for _ in range(10):
+                        lap.spmm(x, y, prec)
+                r = p.result
+                gbs = r["bytes"] / (r["ms"] * 1e-3) / 1e9
+                log(f"spmm sweep {'f32' if prec == si.SI_F32 else 'f64'} k={kk}: "
+                    f"{1e3 * r['ms'] / r['launches']:.1f} us/launch, "
+                    f"{r['bytes'] / r['launches'] / 1e6:.1f} MB -> {gbs:.0f} GB/s "
+                    f"({100 * gbs / HBM_PEAK_GBS:.1f}% of HBM peak)")
+                del x, y
 
     cfg = si.SpectralInitConfig(tol=args.tol, block_size=args.block, cheb_degree=args.degree,
                                 solver=si.Solver.Fast)

@@ -139,17 +139,19 @@ int solve_graph(si::Context& c, int64_t n, const int64_t* ip, const int32_t* ix,
     si::DevLaplacian L;
     {
         si::EventTimer t(s);
-        si::laplacian_build_device(L, n, g.ip, g.ix, g.w, ip[n], mode, s);
+        si::laplacian_build_device(L, n, g.ip, g.ix, g.w, ip[n], mode, s, /*reorder=*/n >= 4096,
+                                   ip, ix, w);
         const double ms = t.stop_ms();
         if (lap_ms) *lap_ms += ms;
     }
     const int k = n_components + 1;
     ev.assign(k, 0.0);
-    si::DevBuf<double> dv((size_t)n * k);
+    si::DevBuf<double> dv((size_t)n * k), dx((size_t)n * k);
     int level = -1;
     try {
         si::EventTimer t(s);
         level = si::solve_eigenproblem_device(L, n_components, seed, opt, ev.data(), dv, st, s);
+        si::rows_to_external(L, dv, k, dx, k, k, s);
         const double ms = t.stop_ms();
         if (solve_ms) *solve_ms += ms;
     } catch (...) {
@@ -157,7 +159,7 @@ int solve_graph(si::Context& c, int64_t n, const int64_t* ip, const int32_t* ix,
         throw;
     }
     vec.resize((size_t)n * k);
-    si::to_host(vec.data(), dv.get(), vec.size(), s);
+    si::to_host(vec.data(), dx.get(), vec.size(), s);
     si::laplacian_free(L);
     return level;
 }
@@ -224,14 +226,15 @@ const char* si_last_error(const si_context* ctx) {
     return ctx ? ctx->c.last_error.c_str() : "null context";
 }
 
-si_status si_laplacian_build(si_context* ctx, int64_t n, const int64_t* indptr,
-                             const int32_t* indices, const float* data, int input_on_device,
-                             int32_t compute_mode, si_laplacian** out) {
+si_status si_laplacian_build_ex(si_context* ctx, int64_t n, const int64_t* indptr,
+                                const int32_t* indices, const float* data, int input_on_device,
+                                int32_t compute_mode, uint32_t flags, si_laplacian** out) {
     if (!ctx || !out) return SI_ERR_INVALID_ARGUMENT;
     *out = nullptr;
     return guard(ctx, [&] {
         SI_CHECK(n > 0, si::kErrInvalidGraph, "invalid graph: n must be > 0");
         SI_CHECK(compute_mode == 0 || compute_mode == 1, si::kErrInvalidArgument, "unknown compute mode");
+        const bool reorder = (flags & SI_BUILD_LOCALITY_ORDER) != 0;
         hipStream_t s = ctx->c.stream;
         std::unique_ptr<si_laplacian> lap(new si_laplacian());
         lap->device = ctx->c.device;
@@ -239,15 +242,24 @@ si_status si_laplacian_build(si_context* ctx, int64_t n, const int64_t* indptr,
             int64_t nnz = 0;
             SI_HIP(hipMemcpyAsync(&nnz, indptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
             SI_HIP(hipStreamSynchronize(s));
-            si::laplacian_build_device(lap->L, n, indptr, indices, data, nnz, compute_mode, s);
+            si::laplacian_build_device(lap->L, n, indptr, indices, data, nnz, compute_mode, s, reorder);
         } else {
             check_host_csr(n, indptr);
             DeviceGraph g;
             upload_graph(g, n, indptr, indices, data, s);
-            si::laplacian_build_device(lap->L, n, g.ip, g.ix, g.w, indptr[n], compute_mode, s);
+            si::laplacian_build_device(lap->L, n, g.ip, g.ix, g.w, indptr[n], compute_mode, s,
+                                       reorder, indptr, indices, data);
         }
         *out = lap.release();
     });
+}
+
+si_status si_laplacian_build(si_context* ctx, int64_t n, const int64_t* indptr,
+                             const int32_t* indices, const float* data, int input_on_device,
+                             int32_t compute_mode, si_laplacian** out) {
+    const uint32_t flags = n >= 4096 ? SI_BUILD_LOCALITY_ORDER : 0u;
+    return si_laplacian_build_ex(ctx, n, indptr, indices, data, input_on_device, compute_mode,
+                                 flags, out);
 }
 
 void si_laplacian_destroy(si_laplacian* lap) {
@@ -275,22 +287,45 @@ si_status si_laplacian_export(si_context* ctx, const si_laplacian* lap, double* 
         hipStream_t s = ctx->c.stream;
         const si::DevLaplacian& L = lap->L;
         const int64_t n = L.n;
-        if (degrees) si::to_host(degrees, L.deg, n, s);
-        if (sqrt_deg) si::to_host(sqrt_deg, L.sqrt_deg, n, s);
-        std::vector<int32_t> rp(n + 1), col(L.nnz);
-        std::vector<double> val(L.nnz);
+        std::vector<int32_t> perm(n), iperm(n);
+        if (L.perm) {
+            si::to_host(perm.data(), L.perm, n, s);
+            si::to_host(iperm.data(), L.iperm, n, s);
+        } else {
+            for (int64_t i = 0; i < n; ++i) perm[i] = iperm[i] = (int32_t)i;
+        }
+        std::vector<double> tmp(n);
+        if (degrees) {
+            si::to_host(tmp.data(), L.deg, n, s);
+            for (int64_t v = 0; v < n; ++v) degrees[v] = tmp[iperm[v]];
+        }
+        if (sqrt_deg) {
+            si::to_host(tmp.data(), L.sqrt_deg, n, s);
+            for (int64_t v = 0; v < n; ++v) sqrt_deg[v] = tmp[iperm[v]];
+        }
+        std::vector<int32_t> rp(n + 1), colp(L.nnz);
+        std::vector<double> valp(L.nnz);
         si::to_host(rp.data(), L.rowptr, n + 1, s);
         if (L.nnz) {
-            si::to_host(col.data(), L.col, L.nnz, s);
-            si::to_host(val.data(), L.l64, L.nnz, s);
+            si::to_host(colp.data(), L.col, L.nnz, s);
+            si::to_host(valp.data(), L.l64, L.nnz, s);
         }
-        // reference layout: off-diagonal entries plus the unit diagonal, ascending columns
+        // reference layout in the caller's node order: off-diagonal entries plus
+        // the unit diagonal, ascending columns
+        std::vector<std::pair<int32_t, double>> row;
         int64_t o = 0;
         if (l_indptr) l_indptr[0] = 0;
         for (int64_t i = 0; i < n; ++i) {
+            const int32_t ii = iperm[i];
+            row.clear();
+            for (int32_t p = rp[ii]; p < rp[ii + 1]; ++p) row.emplace_back(perm[colp[p]], valp[p]);
+            std::sort(row.begin(), row.end(),
+                      [](const std::pair<int32_t, double>& x, const std::pair<int32_t, double>& y) {
+                          return x.first < y.first;
+                      });
             bool diag_done = false;
-            for (int32_t p = rp[i]; p < rp[i + 1]; ++p) {
-                const int32_t j = col[p];
+            for (const auto& e : row) {
+                const int32_t j = e.first;
                 if (j == i) continue;
                 if (!diag_done && j > i) {
                     if (l_indices) l_indices[o] = (int32_t)i;
@@ -299,7 +334,7 @@ si_status si_laplacian_export(si_context* ctx, const si_laplacian* lap, double* 
                     diag_done = true;
                 }
                 if (l_indices) l_indices[o] = j;
-                if (l_data) l_data[o] = val[p];
+                if (l_data) l_data[o] = e.second;
                 ++o;
             }
             if (!diag_done) {
@@ -318,11 +353,32 @@ si_status si_spmm(si_context* ctx, const si_laplacian* lap, int32_t precision, c
     if (!ctx || !lap || !x || !y) return SI_ERR_INVALID_ARGUMENT;
     return guard(ctx, [&] {
         SI_CHECK(si::spmm_width_supported(k), si::kErrInvalidArgument, "spmm: unsupported width");
-        if (precision == SI_F32)
-            si::spmm_f32(lap->L, (const float*)x, (float*)y, k, ld, 1.f, 0.f, 0.f, nullptr, ctx->c.stream);
-        else
-            si::spmm_f64(lap->L, (const double*)x, (double*)y, k, ld, 1.0, 0.0, 0.0, nullptr, ctx->c.stream);
-        SI_HIP(hipStreamSynchronize(ctx->c.stream));
+        SI_CHECK(ld >= k, si::kErrInvalidArgument, "spmm: ld < k");
+        const si::DevLaplacian& L = lap->L;
+        hipStream_t s = ctx->c.stream;
+        const size_t cnt = (size_t)L.n * k;
+        if (precision == SI_F32) {
+            if (!L.perm) {
+                si::spmm_f32(L, (const float*)x, (float*)y, k, ld, 1.f, 0.f, 0.f, nullptr, s);
+            } else {
+                si::DevBuf<float> xi(cnt), yi(cnt);
+                si::rows_to_internal(L, (const float*)x, ld, xi.get(), k, k, s);
+                si::spmm_f32(L, xi, yi, k, k, 1.f, 0.f, 0.f, nullptr, s);
+                si::rows_to_external(L, yi.get(), k, (float*)y, ld, k, s);
+                SI_HIP(hipStreamSynchronize(s));
+            }
+        } else {
+            if (!L.perm) {
+                si::spmm_f64(L, (const double*)x, (double*)y, k, ld, 1.0, 0.0, 0.0, nullptr, s);
+            } else {
+                si::DevBuf<double> xi(cnt), yi(cnt);
+                si::rows_to_internal(L, (const double*)x, ld, xi.get(), k, k, s);
+                si::spmm_f64(L, xi, yi, k, k, 1.0, 0.0, 0.0, nullptr, s);
+                si::rows_to_external(L, yi.get(), k, (double*)y, ld, k, s);
+                SI_HIP(hipStreamSynchronize(s));
+            }
+        }
+        SI_HIP(hipStreamSynchronize(s));
     });
 }
 
@@ -332,7 +388,13 @@ si_status si_chebyshev_filter(si_context* ctx, const si_laplacian* lap, const do
     return guard(ctx, [&] {
         SI_CHECK(si::spmm_width_supported(k), si::kErrInvalidArgument, "chebyshev: unsupported width");
         SI_CHECK(b > a, si::kErrInvalidArgument, "chebyshev: need b > a");
-        si::chebyshev_filter_f64(lap->L, r, k, a, b, degree, out, ctx->c.stream);
+        const si::DevLaplacian& L = lap->L;
+        hipStream_t s = ctx->c.stream;
+        si::DevBuf<double> ri((size_t)L.n * k), oi((size_t)L.n * k);
+        si::rows_to_internal(L, r, k, ri.get(), k, k, s);
+        si::chebyshev_filter_f64(L, ri, k, a, b, degree, oi, s);
+        si::rows_to_external(L, oi.get(), k, out, k, k, s);
+        SI_HIP(hipStreamSynchronize(s));
     });
 }
 
@@ -343,7 +405,9 @@ si_status si_max_eigenpair_residual(si_context* ctx, const si_laplacian* lap,
     return guard(ctx, [&] {
         SI_CHECK(si::spmm_width_supported(k), si::kErrInvalidArgument, "residual: unsupported width");
         std::vector<double> res(k);
-        si::fp64_residuals(lap->L, evecs, k, evals, res.data(), ctx->c.stream);
+        si::DevBuf<double> vi((size_t)lap->L.n * k);
+        si::rows_to_internal(lap->L, evecs, k, vi.get(), k, k, ctx->c.stream);
+        si::fp64_residuals(lap->L, vi, k, evals, res.data(), ctx->c.stream);
         double q = 0.0;
         for (double r : res) q = (std::isnan(q) || std::isnan(r)) ? NAN : std::max(q, r);
         *out = q;
@@ -356,16 +420,19 @@ si_status si_lobpcg_solve(si_context* ctx, const si_laplacian* lap, int32_t n_co
     if (!ctx || !lap || !evals || !evecs) return SI_ERR_INVALID_ARGUMENT;
     return guard(ctx, [&] {
         si::SolveStats st;
+        const int k = n_components + 1;
+        si::DevBuf<double> vi((size_t)lap->L.n * k);
         si::EventTimer t(ctx->c.stream);
         const bool ok = si::lobpcg_solve_plain(lap->L, n_components, seed, regularize != 0,
-                                               tol > 0 ? tol : 1e-5, evals, evecs, st,
+                                               tol > 0 ? tol : 1e-5, evals, vi, st,
                                                ctx->c.stream);
-        const double ms = t.stop_ms();
         if (ok) {
-            std::vector<double> res(n_components + 1);
-            si::fp64_residuals(lap->L, evecs, n_components + 1, evals, res.data(), ctx->c.stream);
+            std::vector<double> res(k);
+            si::fp64_residuals(lap->L, vi, k, evals, res.data(), ctx->c.stream);
             st.max_residual = *std::max_element(res.begin(), res.end());
+            si::rows_to_external(lap->L, vi.get(), k, evecs, k, k, ctx->c.stream);
         }
+        const double ms = t.stop_ms();
         st.level = 1;
         fill_report(rep, st, evals, n_components + 1, ms);
         SI_CHECK(ok, si::kErrConvergence, "lobpcg_solve produced no result");
@@ -382,10 +449,20 @@ si_status si_solve_eigenproblem(si_context* ctx, const si_laplacian* lap, int32_
         SI_CHECK(n_components + 1 <= 64, si::kErrUnsupported, "n_components above 63 is not supported");
         si::ChainOptions o = chain_options(cfg);
         si::SolveStats st;
-        si::EventTimer t(ctx->c.stream);
-        si::solve_eigenproblem_device(lap->L, n_components, seed, o, evals, evecs, st, ctx->c.stream);
+        const int k = n_components + 1;
+        const si::DevLaplacian& L = lap->L;
+        hipStream_t s = ctx->c.stream;
+        si::DevBuf<double> vi((size_t)L.n * k), wsi;
+        if (o.fast.warm_start) {
+            wsi.alloc((size_t)L.n * k);
+            si::rows_to_internal(L, o.fast.warm_start, k, wsi.get(), k, k, s);
+            o.fast.warm_start = wsi.get();
+        }
+        si::EventTimer t(s);
+        si::solve_eigenproblem_device(L, n_components, seed, o, evals, vi, st, s);
+        si::rows_to_external(L, vi.get(), k, evecs, k, k, s);
         const double ms = t.stop_ms();
-        fill_report(rep, st, evals, n_components + 1, ms);
+        fill_report(rep, st, evals, k, ms);
     });
 }
 
@@ -395,13 +472,17 @@ si_status si_rsvd_solve(si_context* ctx, const si_laplacian* lap, int32_t n_comp
     if (!ctx || !lap || !evals || !evecs) return SI_ERR_INVALID_ARGUMENT;
     return guard(ctx, [&] {
         si::SolveStats st;
+        const int k = n_components + 1;
+        si::DevBuf<double> vi((size_t)lap->L.n * k);
         si::EventTimer t(ctx->c.stream);
-        si::rsvd_solve_device(lap->L, n_components, seed, oversampling, nbiter, evals, evecs, st,
+        si::rsvd_solve_device(lap->L, n_components, seed, oversampling, nbiter, evals, vi, st,
                               ctx->c.stream);
         const double ms = t.stop_ms();
-        std::vector<double> res(n_components + 1);
-        si::fp64_residuals(lap->L, evecs, n_components + 1, evals, res.data(), ctx->c.stream);
+        std::vector<double> res(k);
+        si::fp64_residuals(lap->L, vi, k, evals, res.data(), ctx->c.stream);
         st.max_residual = *std::max_element(res.begin(), res.end());
+        si::rows_to_external(lap->L, vi.get(), k, evecs, k, k, ctx->c.stream);
+        SI_HIP(hipStreamSynchronize(ctx->c.stream));
         st.level = 4;
         fill_report(rep, st, evals, n_components + 1, ms);
     });
@@ -410,7 +491,12 @@ si_status si_rsvd_solve(si_context* ctx, const si_laplacian* lap, int32_t n_comp
 si_status si_dense_evd(si_context* ctx, const si_laplacian* lap, int32_t k, double* evals,
                        double* evecs) {
     if (!ctx || !lap || !evals || !evecs) return SI_ERR_INVALID_ARGUMENT;
-    return guard(ctx, [&] { si::dense_evd_small(lap->L, k, evals, evecs, ctx->c.stream); });
+    return guard(ctx, [&] {
+        si::DevBuf<double> vi((size_t)lap->L.n * (k > 0 ? k : 1));
+        si::dense_evd_small(lap->L, k, evals, vi, ctx->c.stream);
+        si::rows_to_external(lap->L, vi.get(), k, evecs, k, k, ctx->c.stream);
+        SI_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
 }
 
 si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,

@@ -112,8 +112,17 @@ si_status si_test_fast_solve(si_context* ctx, const si_laplacian* lap, int32_t n
         o.tol = cfg->tol > 0 ? cfg->tol : 1e-6;
         o.warm_start = cfg->warm_start;
         si::SolveStats st;
-        si::EventTimer t(ctx->c.stream);
-        const bool ok = si::fast_solve(lap->L, n_components, seed, o, evals, evecs, st, ctx->c.stream);
+        const int k = n_components + 1;
+        hipStream_t s = ctx->c.stream;
+        si::DevBuf<double> vi((size_t)lap->L.n * k), wsi;
+        if (o.warm_start) {
+            wsi.alloc((size_t)lap->L.n * k);
+            si::rows_to_internal(lap->L, o.warm_start, k, wsi.get(), k, k, s);
+            o.warm_start = wsi.get();
+        }
+        si::EventTimer t(s);
+        const bool ok = si::fast_solve(lap->L, n_components, seed, o, evals, vi, st, s);
+        if (ok) si::rows_to_external(lap->L, vi.get(), k, evecs, k, k, s);
         const double ms = t.stop_ms();
         if (rep) {
             rep->level = 1;

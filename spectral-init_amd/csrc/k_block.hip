@@ -17,7 +17,7 @@ namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-constexpr int kGramBlocks = 1024;
+constexpr int kGramBlocks = 512;
 
 template <typename T, int PT, int QT>
 __global__ __launch_bounds__(256) void k_gram(int64_t n, const T* __restrict__ a, int lda, int p,
@@ -36,6 +36,8 @@ __global__ __launch_bounds__(256) void k_gram(int64_t n, const T* __restrict__ a
         for (int tj = 0; tj < QT; ++tj) acc[ti][tj] = v4d{0.0, 0.0, 0.0, 0.0};
     const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
     const int64_t r1 = (r0 + rows_per_block) < n ? (r0 + rows_per_block) : n;
+    // A^T A: read the block once
+    const bool same = (PT == QT) && (a == b) && (lda == ldb) && (p == q);
     for (int64_t r = r0 + wave * 4; r < r1; r += 16) {
         const int64_t row = r + lk;
         const bool ok = row < r1;
@@ -45,10 +47,15 @@ __global__ __launch_bounds__(256) void k_gram(int64_t n, const T* __restrict__ a
             const int c = t * 16 + li;
             av[t] = (ok && c < p) ? (double)a[row * lda + c] : 0.0;
         }
+        if (same) {
 #pragma unroll
-        for (int t = 0; t < QT; ++t) {
-            const int c = t * 16 + li;
-            bv[t] = (ok && c < q) ? (double)b[row * ldb + c] : 0.0;
+            for (int t = 0; t < QT; ++t) bv[t] = av[t < PT ? t : 0];
+        } else {
+#pragma unroll
+            for (int t = 0; t < QT; ++t) {
+                const int c = t * 16 + li;
+                bv[t] = (ok && c < q) ? (double)b[row * ldb + c] : 0.0;
+            }
         }
 #pragma unroll
         for (int ti = 0; ti < PT; ++ti)
@@ -78,13 +85,25 @@ __global__ __launch_bounds__(256) void k_gram(int64_t n, const T* __restrict__ a
     for (int e = threadIdx.x; e < p * q; e += 256) out[e] = red[e / q][e % q];
 }
 
-__global__ void k_reduce_partials(const double* __restrict__ partial, int nparts, int pq,
-                                  double* __restrict__ g) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= pq) return;
+// Sum the per-workgroup partials: each workgroup owns 32 outputs; 8 groups of
+// 32 threads each sum a fixed stride of the partials, then one group adds the
+// 8 sums in order (fixed order: deterministic).
+__global__ __launch_bounds__(256) void k_reduce_partials(const double* __restrict__ partial,
+                                                         int nparts, int pq,
+                                                         double* __restrict__ g) {
+    __shared__ double red[8][33];
+    const int lane = threadIdx.x & 31, grp = threadIdx.x >> 5;
+    const int e = blockIdx.x * 32 + lane;
     double s = 0.0;
-    for (int b = 0; b < nparts; ++b) s += partial[(int64_t)b * pq + e];
-    g[e] = s;
+    if (e < pq)
+        for (int b = grp; b < nparts; b += 8) s += partial[(int64_t)b * pq + e];
+    red[grp][lane] = s;
+    __syncthreads();
+    if (grp == 0 && e < pq) {
+        double t = red[0][lane];
+        for (int q = 1; q < 8; ++q) t += red[q][lane];
+        g[e] = t;
+    }
 }
 
 template <typename T>
@@ -111,46 +130,67 @@ void gram_impl(int64_t n, const T* a, int lda, int p, const T* b, int ldb, int q
     { throw Error(kErrInternal, "gram: no kernel for this shape"); }
 #undef SI_GRAM_CASE
     SI_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce_partials, dim3(ceil_div(p * q, 256)), dim3(256), 0, s, ws,
+    hipLaunchKernelGGL(k_reduce_partials, dim3(ceil_div(p * q, 32)), dim3(256), 0, s, ws,
                        (int)nblocks, p * q, g);
     SI_LAUNCH_CHECK();
 }
 
 // ------------------------------------------------------------------ block products
+// Out[:, 0:q] = A[:, 0:p] C (+ B[:, 0:p2] D).  A workgroup owns R rows; their
+// A (then B) tile is staged through LDS with coalesced loads, the small matrix
+// (C, then D) sits in LDS in the data precision, and TPR threads per row each
+// produce CPT contiguous output columns (coalesced stores).  Rows are only
+// written after both tiles were read, so out may alias a (in place).
 template <typename T, int QMAX>
 __global__ __launch_bounds__(256) void k_block_mul(int64_t n, const T* __restrict__ a, int lda,
                                                    int p, const double* __restrict__ c, int q,
                                                    const T* __restrict__ b, int ldb, int p2,
                                                    const double* __restrict__ d, T* out, int ldo) {
-    __shared__ T cs[64 * QMAX];
-    __shared__ T ds[64 * QMAX];
-    for (int e = threadIdx.x; e < p * q; e += blockDim.x) cs[(e / q) * QMAX + e % q] = (T)c[e];
-    if (b)
-        for (int e = threadIdx.x; e < p2 * q; e += blockDim.x) ds[(e / q) * QMAX + e % q] = (T)d[e];
-    __syncthreads();
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    T acc[QMAX];
+    constexpr int R = sizeof(T) == 8 ? 32 : 64;
+    constexpr int TPR = 256 / R;
+    constexpr int CPT = (QMAX + TPR - 1) / TPR;
+    constexpr int LDT = 65;
+    __shared__ T cm[64 * QMAX];
+    __shared__ T tile[R * LDT];
+    const int t = threadIdx.x;
+    const int r = t / TPR, g = t % TPR;
+    const int64_t row0 = (int64_t)blockIdx.x * R;
+    const int rows = (n - row0) < R ? (int)(n - row0) : R;
+    T acc[CPT];
 #pragma unroll
-    for (int j = 0; j < QMAX; ++j) acc[j] = T(0);
-    const T* ar = a + i * lda;
-    for (int k = 0; k < p; ++k) {
-        const T av = ar[k];
+    for (int u = 0; u < CPT; ++u) acc[u] = T(0);
+    for (int term = 0; term < 2; ++term) {
+        const T* src = term == 0 ? a : b;
+        if (!src) break;
+        const int ld = term == 0 ? lda : ldb;
+        const int pp = term == 0 ? p : p2;
+        const double* cc = term == 0 ? c : d;
+        __syncthreads();
+        for (int e = t; e < pp * q; e += 256) cm[(e / q) * QMAX + e % q] = (T)cc[e];
+        for (int e = t; e < rows * pp; e += 256) {
+            const int rr = e / pp, kk = e % pp;
+            tile[rr * LDT + kk] = src[(row0 + rr) * ld + kk];
+        }
+        __syncthreads();
+        if (r < rows) {
+            for (int k = 0; k < pp; ++k) {
+                const T av = tile[r * LDT + k];
 #pragma unroll
-        for (int j = 0; j < QMAX; ++j) acc[j] += av * cs[k * QMAX + j];
-    }
-    if (b) {
-        const T* br = b + i * ldb;
-        for (int k = 0; k < p2; ++k) {
-            const T bv = br[k];
-#pragma unroll
-            for (int j = 0; j < QMAX; ++j) acc[j] += bv * ds[k * QMAX + j];
+                for (int u = 0; u < CPT; ++u) {
+                    const int j = g * CPT + u;
+                    if (j < QMAX) acc[u] += av * cm[k * QMAX + j];
+                }
+            }
         }
     }
-    T* o = out + i * ldo;
+    if (r < rows) {
+        T* o = out + (row0 + r) * ldo;
 #pragma unroll
-    for (int j = 0; j < QMAX; ++j)
-        if (j < q) o[j] = acc[j];
+        for (int u = 0; u < CPT; ++u) {
+            const int j = g * CPT + u;
+            if (j < q) o[j] = acc[u];
+        }
+    }
 }
 
 template <typename T>
@@ -158,7 +198,7 @@ void block_mul_impl(int64_t n, const T* a, int lda, int p, const double* c, int 
                     int ldb, int p2, const double* d, T* out, int ldo, hipStream_t s) {
     SI_CHECK(p <= 64 && q <= 64 && p2 <= 64, kErrInvalidArgument, "block_mul: width > 64");
     SI_CHECK(out != b || b == nullptr, kErrInvalidArgument, "block_mul: out aliases the second term");
-    const int64_t blocks = ceil_div(n, 256);
+    const int64_t blocks = ceil_div(n, sizeof(T) == 8 ? 32 : 64);
     if (q <= 4)
         hipLaunchKernelGGL((k_block_mul<T, 4>), dim3(blocks), dim3(256), 0, s, n, a, lda, p, c, q, b, ldb, p2, d, out, ldo);
     else if (q <= 8)
@@ -183,13 +223,27 @@ __global__ void k_residual(int64_t n, const T* __restrict__ ax, const T* __restr
     r[i * ldr + c] = ax[i * ld + c] - (T)theta[c] * x[i * ld + c];
 }
 
+// Row i draws the normals of original node perm[i], so a reordered solve
+// starts from the same block as an unordered one.
 __global__ void k_fill_normal(double* __restrict__ x, int64_t n, int k, int ld, uint64_t seed,
-                              uint32_t stream) {
+                              uint32_t stream, const int32_t* __restrict__ perm) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n * k) return;
     const int64_t i = e / k;
     const int c = (int)(e % k);
-    x[i * ld + c] = normal_at(seed, stream, (uint64_t)e);
+    const int64_t orig = perm ? (int64_t)perm[i] : i;
+    x[i * ld + c] = normal_at(seed, stream, (uint64_t)(orig * k + c));
+}
+
+template <typename T>
+__global__ void k_gather_rows(const T* __restrict__ src, int lds, T* __restrict__ dst, int ldd,
+                              int64_t n, int k, const int32_t* __restrict__ idx) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * k) return;
+    const int64_t i = e / k;
+    const int c = (int)(e % k);
+    const int64_t r = idx ? (int64_t)idx[i] : i;
+    dst[i * ldd + c] = src[r * lds + c];
 }
 
 template <typename A, typename B>
@@ -271,9 +325,20 @@ void residual_f64(int64_t n, const double* ax, const double* x, int ld, int k,
     SI_LAUNCH_CHECK();
 }
 
+void gather_rows_f64(const double* src, int lds, double* dst, int ldd, int64_t n, int k,
+                     const int32_t* idx, hipStream_t s) {
+    hipLaunchKernelGGL((k_gather_rows<double>), grid1(n * k), dim3(256), 0, s, src, lds, dst, ldd, n, k, idx);
+    SI_LAUNCH_CHECK();
+}
+void gather_rows_f32(const float* src, int lds, float* dst, int ldd, int64_t n, int k,
+                     const int32_t* idx, hipStream_t s) {
+    hipLaunchKernelGGL((k_gather_rows<float>), grid1(n * k), dim3(256), 0, s, src, lds, dst, ldd, n, k, idx);
+    SI_LAUNCH_CHECK();
+}
+
 void fill_normal_f64(double* x, int64_t n, int k, int ld, uint64_t seed, uint32_t stream,
-                     hipStream_t s) {
-    hipLaunchKernelGGL(k_fill_normal, grid1(n * k), dim3(256), 0, s, x, n, k, ld, seed, stream);
+                     hipStream_t s, const int32_t* perm) {
+    hipLaunchKernelGGL(k_fill_normal, grid1(n * k), dim3(256), 0, s, x, n, k, ld, seed, stream, perm);
     SI_LAUNCH_CHECK();
 }
 void convert_f64_to_f32(const double* a, float* b, int64_t count, hipStream_t s) {

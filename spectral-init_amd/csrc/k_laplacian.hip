@@ -12,6 +12,10 @@
 // Values.  L_ij = -isd_i * w_ij * isd_j evaluated left to right in f64
 // (laplacian.rs:70-80); W's own diagonal is dropped and L's unit diagonal is
 // implied (laplacian.rs:83-85).
+#include <algorithm>
+#include <utility>
+#include <vector>
+
 #include "si_device.hpp"
 
 namespace si {
@@ -131,12 +135,14 @@ __global__ void k_values(const int32_t* __restrict__ rp, const int32_t* __restri
 void laplacian_free(DevLaplacian& L) {
     hipFree(L.rowptr); hipFree(L.col); hipFree(L.a32); hipFree(L.l64);
     hipFree(L.deg); hipFree(L.sqrt_deg); hipFree(L.isd); hipFree(L.long_rows);
+    hipFree(L.perm); hipFree(L.iperm);
     L = DevLaplacian{};
 }
 
 void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
                             const int32_t* d_indices, const float* d_w, int64_t nnz, int mode,
-                            hipStream_t s) {
+                            hipStream_t s, bool reorder, const int64_t* h_indptr,
+                            const int32_t* h_indices, const float* h_w) {
     SI_CHECK(n > 0, kErrInvalidGraph, "laplacian: n must be > 0");
     SI_CHECK(nnz < (int64_t)INT32_MAX && n < (int64_t)INT32_MAX, kErrUnsupported,
              "laplacian: more than 2^31-1 stored entries or nodes is not supported");
@@ -212,7 +218,70 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
         L.degree_exact = 1;
     }
     hipLaunchKernelGGL(k_isd, dim3(gn), dim3(T), 0, s, L.deg, n, L.sqrt_deg, L.isd);
-    hipLaunchKernelGGL(k_values, dim3(gn), dim3(T), 0, s, L.rowptr, L.col, d_w, L.isd, n, L.l64,
+    SI_LAUNCH_CHECK();
+    const float* w_vals = d_w;
+    float* d_w2 = nullptr;
+    if (reorder && n > 1) {
+        // host copy of the pattern (downloaded only if the caller has none)
+        std::vector<int64_t> hp_buf;
+        std::vector<int32_t> hx_buf;
+        std::vector<float> hw_buf;
+        const int64_t* hp = h_indptr;
+        const int32_t* hx = h_indices;
+        const float* hw = h_w;
+        if (!hp || !hx || !hw) {
+            hp_buf.resize(n + 1);
+            hx_buf.resize(nnz);
+            hw_buf.resize(nnz);
+            SI_HIP(hipMemcpyAsync(hp_buf.data(), d_indptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, s));
+            if (nnz) {
+                SI_HIP(hipMemcpyAsync(hx_buf.data(), d_indices, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost, s));
+                SI_HIP(hipMemcpyAsync(hw_buf.data(), d_w, sizeof(float) * nnz, hipMemcpyDeviceToHost, s));
+            }
+            SI_HIP(hipStreamSynchronize(s));
+            hp = hp_buf.data(); hx = hx_buf.data(); hw = hw_buf.data();
+        }
+        std::vector<int32_t> perm = rcm_order(n, hp, hx);
+        std::vector<int32_t> iperm(n);
+        for (int64_t i = 0; i < n; ++i) iperm[perm[i]] = (int32_t)i;
+        std::vector<int32_t> rp2(n + 1), c2(nnz);
+        std::vector<float> w2(nnz);
+        std::vector<std::pair<int32_t, float>> row;
+        rp2[0] = 0;
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t v = perm[i];
+            row.clear();
+            for (int64_t p = hp[v]; p < hp[v + 1]; ++p) row.emplace_back(iperm[hx[p]], hw[p]);
+            std::sort(row.begin(), row.end(),
+                      [](const std::pair<int32_t, float>& x, const std::pair<int32_t, float>& y) {
+                          return x.first < y.first;
+                      });
+            int32_t o = rp2[i];
+            for (const auto& e : row) { c2[o] = e.first; w2[o] = e.second; ++o; }
+            rp2[i + 1] = o;
+        }
+        SI_HIP(hipMalloc(&d_w2, sizeof(float) * (nnz > 0 ? nnz : 1)));
+        SI_HIP(hipMalloc(&L.perm, sizeof(int32_t) * n));
+        SI_HIP(hipMalloc(&L.iperm, sizeof(int32_t) * n));
+        SI_HIP(hipMemcpyAsync(L.rowptr, rp2.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, s));
+        if (nnz) {
+            SI_HIP(hipMemcpyAsync(L.col, c2.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s));
+            SI_HIP(hipMemcpyAsync(d_w2, w2.data(), sizeof(float) * nnz, hipMemcpyHostToDevice, s));
+        }
+        SI_HIP(hipMemcpyAsync(L.perm, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+        SI_HIP(hipMemcpyAsync(L.iperm, iperm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+        // node vectors follow the permutation (computed in input order above)
+        double* tmp = nullptr;
+        SI_HIP(hipMalloc(&tmp, sizeof(double) * n));
+        for (double* vec : {L.deg, L.sqrt_deg, L.isd}) {
+            gather_rows_f64(vec, 1, tmp, 1, n, 1, L.perm, s);
+            SI_HIP(hipMemcpyAsync(vec, tmp, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        }
+        SI_HIP(hipStreamSynchronize(s));
+        hipFree(tmp);
+        w_vals = d_w2;
+    }
+    hipLaunchKernelGGL(k_values, dim3(gn), dim3(T), 0, s, L.rowptr, L.col, w_vals, L.isd, n, L.l64,
                        L.a32, L.long_rows, d_nlong);
     SI_LAUNCH_CHECK();
     int nlong = 0;
@@ -220,6 +289,7 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
     SI_HIP(hipStreamSynchronize(s));
     L.n_long = nlong;
     hipFree(d_flags);
+    if (d_w2) hipFree(d_w2);
 }
 
 }  // namespace si

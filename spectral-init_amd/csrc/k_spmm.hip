@@ -59,25 +59,34 @@ __global__ __launch_bounds__(256) void k_spmm_narrow(
     T acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = T(0);
-    int32_t p = b + lane;
-    // two nonzeros in flight per lane
-    for (; p + G < e; p += 2 * G) {
-        const T v0 = val[p], v1 = val[p + G];
-        const T* x0 = x + (int64_t)col[p] * ld;
-        const T* x1 = x + (int64_t)col[p + G] * ld;
-        T a0[K], a1[K];
+    // U nonzeros in flight per lane: indices first, then the gathers
+    constexpr int U = 4;
+    for (int32_t base = b + lane; base < e; base += G * U) {
+        T vv[U];
+        int32_t cc[U];
 #pragma unroll
-        for (int k = 0; k < K; ++k) { a0[k] = x0[k]; a1[k] = x1[k]; }
+        for (int u = 0; u < U; ++u) {
+            const int32_t p = base + u * G;
+            const bool in = p < e;
+            vv[u] = in ? val[p] : T(0);
+            cc[u] = in ? col[p] : -1;
+        }
+        T xs[U][K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) acc[k] += v0 * a0[k];
+        for (int u = 0; u < U; ++u) {
+            if (cc[u] >= 0) {
+                const T* xr = x + (int64_t)cc[u] * ld;
 #pragma unroll
-        for (int k = 0; k < K; ++k) acc[k] += v1 * a1[k];
-    }
-    if (p < e) {
-        const T v0 = val[p];
-        const T* x0 = x + (int64_t)col[p] * ld;
+                for (int k = 0; k < K; ++k) xs[u][k] = xr[k];
+            } else {
 #pragma unroll
-        for (int k = 0; k < K; ++k) acc[k] += v0 * x0[k];
+                for (int k = 0; k < K; ++k) xs[u][k] = T(0);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int k = 0; k < K; ++k) acc[k] += vv[u] * xs[u][k];
     }
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1)
@@ -122,24 +131,37 @@ __global__ __launch_bounds__(256) void k_spmm_wide(
     T acc[VEC];
 #pragma unroll
     for (int v = 0; v < VEC; ++v) acc[v] = T(0);
-    int32_t p = b + slot;
-    for (; p + SLOTS < e; p += 2 * SLOTS) {
-        const T v0 = val[p], v1 = val[p + SLOTS];
-        const V x0 = *reinterpret_cast<const V*>(x + (int64_t)col[p] * ld + chunk * VEC);
-        const V x1 = *reinterpret_cast<const V*>(x + (int64_t)col[p + SLOTS] * ld + chunk * VEC);
-        const T* a0 = reinterpret_cast<const T*>(&x0);
-        const T* a1 = reinterpret_cast<const T*>(&x1);
+    // Memory-level parallelism: each lane first issues U (value, column) loads,
+    // then U independent x-row gathers, then the FMAs -- one dependent
+    // index->gather round trip per U*SLOTS nonzeros of the row.
+    constexpr int U = 8;
+    for (int32_t base = b + slot; base < e; base += SLOTS * U) {
+        T vv[U];
+        int32_t cc[U];
 #pragma unroll
-        for (int v = 0; v < VEC; ++v) acc[v] += v0 * a0[v];
+        for (int u = 0; u < U; ++u) {
+            const int32_t p = base + u * SLOTS;
+            const bool in = p < e;
+            vv[u] = in ? val[p] : T(0);
+            cc[u] = in ? col[p] : -1;
+        }
+        V xs[U];
 #pragma unroll
-        for (int v = 0; v < VEC; ++v) acc[v] += v1 * a1[v];
-    }
-    if (p < e) {
-        const T v0 = val[p];
-        const V x0 = *reinterpret_cast<const V*>(x + (int64_t)col[p] * ld + chunk * VEC);
-        const T* a0 = reinterpret_cast<const T*>(&x0);
+        for (int u = 0; u < U; ++u) {
+            if (cc[u] >= 0) {
+                xs[u] = *reinterpret_cast<const V*>(x + (int64_t)cc[u] * ld + chunk * VEC);
+            } else {
+                T* zr = reinterpret_cast<T*>(&xs[u]);
 #pragma unroll
-        for (int v = 0; v < VEC; ++v) acc[v] += v0 * a0[v];
+                for (int v = 0; v < VEC; ++v) zr[v] = T(0);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const T* a = reinterpret_cast<const T*>(&xs[u]);
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) acc[v] += vv[u] * a[v];
+        }
     }
     // reduce over slots (lane bits between C and C*SLOTS)
 #pragma unroll

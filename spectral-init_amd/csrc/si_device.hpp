@@ -37,16 +37,30 @@ struct DevLaplacian {
     int symmetric = 0;       // W bitwise symmetric with sorted rows
     int degree_mode = 0;     // 0 PythonCompat, 1 RustNative
     int degree_exact = 1;    // PythonCompat degrees reproduce the f32 column-sum order
+    int32_t* perm = nullptr;   // perm[new] = old node id (nullptr: identity order)
+    int32_t* iperm = nullptr;  // iperm[old] = new node id
 };
 
 // ----------------------------------------------------------------- laplacian
 // Builds L on the device from a CSR graph already resident on the device
 // (int64 row pointers, int32 columns, f32 weights).  Throws si::Error on
 // malformed input.
+// reorder = true renumbers the nodes in reverse Cuthill-McKee order for gather
+// locality (degrees are computed in the input order first, so PythonCompat
+// stays bit-exact); h_* optionally give the same CSR on the host.
 void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
                             const int32_t* d_indices, const float* d_w, int64_t nnz, int mode,
-                            hipStream_t s);
+                            hipStream_t s, bool reorder = false,
+                            const int64_t* h_indptr = nullptr, const int32_t* h_indices = nullptr,
+                            const float* h_w = nullptr);
 void laplacian_free(DevLaplacian& L);
+// perm[new] = old of a symmetric CSR pattern (reverse Cuthill-McKee).
+std::vector<int32_t> rcm_order(int64_t n, const int64_t* ip, const int32_t* ix);
+// dst[i, :k] = src[idx[i], :k] (row gather); idx == nullptr copies.
+void gather_rows_f64(const double* src, int lds, double* dst, int ldd, int64_t n, int k,
+                     const int32_t* idx, hipStream_t s);
+void gather_rows_f32(const float* src, int lds, float* dst, int ldd, int64_t n, int k,
+                     const int32_t* idx, hipStream_t s);
 
 // ----------------------------------------------------------------- SpMM
 // out = s_l * (L x) + s_x * x + s_z * z   for row-major blocks of width k.
@@ -99,7 +113,7 @@ void residual_f64(int64_t n, const double* ax, const double* x, int ld, int k,
                   const double* theta, double* r, int ldr, hipStream_t s);
 
 void fill_normal_f64(double* x, int64_t n, int k, int ld, uint64_t seed, uint32_t stream,
-                     hipStream_t s);
+                     hipStream_t s, const int32_t* perm = nullptr);
 void convert_f64_to_f32(const double* a, float* b, int64_t count, hipStream_t s);
 void convert_f32_to_f64(const float* a, double* b, int64_t count, hipStream_t s);
 // copy columns [0,k) of a [n x lda] block into b [n x ldb]

@@ -202,7 +202,7 @@ bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const Fa
     Small sm(m);
 
     // start block: warm start (first k columns) or N(0,1) with the trivial hint
-    fill_normal_f64(x64, n, m, m, seed, kStreamLobpcgInit, s);
+    fill_normal_f64(x64, n, m, m, seed, kStreamLobpcgInit, s, L.perm);
     if (opt.warm_start) {
         copy_cols_f64(opt.warm_start, k, x64, m, n, k, s);
     } else {

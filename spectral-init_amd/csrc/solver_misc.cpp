@@ -54,7 +54,7 @@ void rsvd_solve_device(const DevLaplacian& L, int n_components, uint64_t seed, i
         spmm_f64(L, in, out, k, k, -1.0, 2.0, 0.0, nullptr, s);  // (2I - L) x
         ++st.spmm_calls;
     };
-    fill_normal_f64(om, n, k, k, seed, kStreamRsvdOmega, s);
+    fill_normal_f64(om, n, k, k, seed, kStreamRsvdOmega, s, L.perm);
     for (int it = 0; it < nbiter; ++it) {
         apply_m(om, y);
         svqb_orth(n, y, k, g, w, u, t, ws, s);

@@ -186,7 +186,7 @@ bool lobpcg_solve_plain(const DevLaplacian& L, int n_components, uint64_t seed, 
     DevBuf<double> x(nk), t(nk), g(1), ws(gram_workspace_doubles(n, k, k)), gk((size_t)k * k),
         sc(k);
     // N(0,1) start (lobpcg.rs:244-246) and the trivial-vector hint (:249-252)
-    fill_normal_f64(x, n, k, k, seed, kStreamLobpcgInit, s);
+    fill_normal_f64(x, n, k, k, seed, kStreamLobpcgInit, s, L.perm);
     gram_f64(n, L.sqrt_deg, 1, 1, L.sqrt_deg, 1, 1, g, ws, s);
     double sn2 = 0.0;
     to_host(&sn2, g.get(), 1, s);

@@ -33,7 +33,8 @@ SI_F32, SI_F64 = 0, 1
 # every symbol the public and testing headers declare
 EXPORTED = [
     "si_version", "si_config_default", "si_context_create", "si_context_destroy",
-    "si_last_error", "si_laplacian_build", "si_laplacian_destroy", "si_laplacian_info",
+    "si_last_error", "si_laplacian_build", "si_laplacian_build_ex", "si_laplacian_destroy",
+    "si_laplacian_info",
     "si_laplacian_export", "si_spmm", "si_chebyshev_filter", "si_max_eigenpair_residual",
     "si_lobpcg_solve", "si_solve_eigenproblem", "si_rsvd_solve", "si_dense_evd",
     "si_spectral_init",
@@ -166,6 +167,8 @@ def lib():
         L.si_context_create.argtypes = [C.c_int, C.POINTER(vp)]
         L.si_context_destroy.argtypes = [vp]
         L.si_laplacian_build.argtypes = [vp, i64, vp, vp, vp, C.c_int, i32, C.POINTER(vp)]
+        L.si_laplacian_build_ex.argtypes = [vp, i64, vp, vp, vp, C.c_int, i32, C.c_uint32,
+                                            C.POINTER(vp)]
         L.si_laplacian_destroy.argtypes = [vp]
         L.si_laplacian_info.argtypes = [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64),
                                         C.POINTER(i32)]
@@ -239,7 +242,10 @@ class Context:
 class Laplacian:
     """Device-resident L = I - D^-1/2 W D^-1/2 (src/laplacian.rs:29-88)."""
 
-    def __init__(self, ctx: Context, indptr, indices, data, mode: int = 0, on_device=False):
+    def __init__(self, ctx: Context, indptr, indices, data, mode: int = 0, on_device=False,
+                 reorder: Optional[bool] = None):
+        """reorder: store in locality (reverse Cuthill-McKee) order; None = the
+        library default (on for n >= 4096).  Vectors always use the caller's order."""
         self.ctx = ctx
         self._h = C.c_void_p()
         if on_device:
@@ -251,8 +257,14 @@ class Laplacian:
             data = np.ascontiguousarray(data, np.float32)
             n = indptr.size - 1
             self._keep = (indptr, indices, data)
-        ctx._check(lib().si_laplacian_build(ctx.handle, n, _ptr(indptr), _ptr(indices), _ptr(data),
-                                            int(on_device), int(mode), C.byref(self._h)))
+        if reorder is None:
+            ctx._check(lib().si_laplacian_build(ctx.handle, n, _ptr(indptr), _ptr(indices),
+                                                _ptr(data), int(on_device), int(mode),
+                                                C.byref(self._h)))
+        else:
+            ctx._check(lib().si_laplacian_build_ex(ctx.handle, n, _ptr(indptr), _ptr(indices),
+                                                   _ptr(data), int(on_device), int(mode),
+                                                   1 if reorder else 0, C.byref(self._h)))
         n_, nnz, nl, ex = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
         lib().si_laplacian_info(self._h, C.byref(n_), C.byref(nnz), C.byref(nl), C.byref(ex))
         self.n, self.nnz, self.n_long, self.degree_exact = n_.value, nnz.value, nl.value, bool(ex.value)

@@ -235,3 +235,30 @@ def test_fill_normal_matches_oracle_rng(ctx):
     si.fill_normal(ctx, x, 42, 1)
     ref = O.normal_block(42, 1, 3000).reshape(1000, 3)
     np.testing.assert_allclose(x.cpu().numpy(), ref, rtol=1e-13, atol=1e-13)
+
+
+# ------------------------------------------------------------------ locality order
+@pytest.mark.parametrize("name", ["blobs_connected_2000", "blobs_5000", "disconnected_200"])
+def test_reordered_laplacian_exports_reference_layout(ctx, name):
+    # the locality order is internal: degrees stay bit-exact and the exported
+    # L is the reference's CSR in the caller's node order
+    z = load_fixture(name)
+    lap = si.Laplacian(ctx, z["w_indptr"], z["w_indices"], z["w_data"], reorder=True)
+    deg, sq, (ip, ix, d) = lap.export()
+    np.testing.assert_array_equal(deg, z["degrees"])
+    (oip, oix, od), _, _ = O.laplacian_from_graph(z["w_indptr"], z["w_indices"], z["w_data"])
+    np.testing.assert_array_equal(ip, oip)
+    np.testing.assert_array_equal(ix, oix)
+    np.testing.assert_array_equal(d, od)
+
+
+@pytest.mark.parametrize("k", [3, 32])
+def test_reordered_spmm_matches_oracle(ctx, k):
+    ip, ix, w = _hub_graph()
+    lap = si.Laplacian(ctx, ip, ix, w, reorder=True)
+    L, _, _ = O.laplacian_from_graph(ip, ix, w)
+    x = np.random.default_rng(k).standard_normal((lap.n, k))
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.empty_like(xd)
+    lap.spmm(xd, yd, si.SI_F64)
+    np.testing.assert_allclose(yd.cpu().numpy(), O.spmm(L, x), rtol=1e-12, atol=1e-12)
