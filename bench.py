@@ -81,7 +81,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="gmm1m", choices=["gmm1m", "gmm100k", "blobs2k"])
+    ap.add_argument("--workload", default="gmm1m", choices=["gmm1m", "gmm100k", "blobs2k", "band1m"])
     ap.add_argument("--n", type=int, default=None, help="override the node count")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--n-components", type=int, default=2)
@@ -91,6 +91,8 @@ def main():
     ap.add_argument("--plain-runs", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--spmm-only", action="store_true",
+                    help="time the SpMM alone for widths 1..64 and exit")
     ap.add_argument("--spmm-sweep", action="store_true",
                     help="also time the SpMM alone for widths 1..64 (FP32 and FP64)")
     ap.add_argument("--no-reorder", action="store_true",
@@ -130,7 +132,8 @@ def main():
     lap_ms = 1e3 * (time.perf_counter() - t0)
     log(f"laplacian on device: nnz={lap.nnz} hub rows={lap.n_long} ({lap_ms:.1f} ms incl. upload)")
 
-    if args.spmm_sweep:
+    sweep = []
+    if args.spmm_sweep or args.spmm_only:
         for prec, dt in ((si.SI_F32, torch.float32), (si.SI_F64, torch.float64)):
             for kk in (1, 3, 8, 16, 32, 64):
                 x = torch.randn(n, kk, dtype=dt, device="cuda")
@@ -146,7 +149,15 @@ def main():
                     f"{1e3 * r['ms'] / r['launches']:.1f} us/launch, "
                     f"{r['bytes'] / r['launches'] / 1e6:.1f} MB -> {gbs:.0f} GB/s "
                     f"({100 * gbs / HBM_PEAK_GBS:.1f}% of HBM peak)")
+                sweep.append(dict(precision="f32" if prec == si.SI_F32 else "f64", k=kk,
+                                  us=round(1e3 * r["ms"] / r["launches"], 2),
+                                  mb=round(r["bytes"] / r["launches"] / 1e6, 2),
+                                  gbs=round(gbs, 1)))
                 del x, y
+        if args.spmm_only:
+            print(json.dumps({"workload": args.workload, "n": n, "nnz": nnz,
+                              "hub_rows": lap.n_long, "spmm": sweep}), flush=True)
+            return
 
     cfg = si.SpectralInitConfig(tol=args.tol, block_size=args.block, cheb_degree=args.degree,
                                 solver=si.Solver.Fast)

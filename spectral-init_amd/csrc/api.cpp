@@ -421,7 +421,7 @@ si_status si_lobpcg_solve(si_context* ctx, const si_laplacian* lap, int32_t n_co
     return guard(ctx, [&] {
         si::SolveStats st;
         const int k = n_components + 1;
-        si::DevBuf<double> vi((size_t)lap->L.n * k);
+        double* vi = si::ws_get<double>(si::kWsChainVec, (size_t)lap->L.n * k);
         si::EventTimer t(ctx->c.stream);
         const bool ok = si::lobpcg_solve_plain(lap->L, n_components, seed, regularize != 0,
                                                tol > 0 ? tol : 1e-5, evals, vi, st,
@@ -430,7 +430,7 @@ si_status si_lobpcg_solve(si_context* ctx, const si_laplacian* lap, int32_t n_co
             std::vector<double> res(k);
             si::fp64_residuals(lap->L, vi, k, evals, res.data(), ctx->c.stream);
             st.max_residual = *std::max_element(res.begin(), res.end());
-            si::rows_to_external(lap->L, vi.get(), k, evecs, k, k, ctx->c.stream);
+            si::rows_to_external(lap->L, vi, k, evecs, k, k, ctx->c.stream);
         }
         const double ms = t.stop_ms();
         st.level = 1;
@@ -452,15 +452,15 @@ si_status si_solve_eigenproblem(si_context* ctx, const si_laplacian* lap, int32_
         const int k = n_components + 1;
         const si::DevLaplacian& L = lap->L;
         hipStream_t s = ctx->c.stream;
-        si::DevBuf<double> vi((size_t)L.n * k), wsi;
+        double* vi = si::ws_get<double>(si::kWsChainVec, (size_t)L.n * k);
         if (o.fast.warm_start) {
-            wsi.alloc((size_t)L.n * k);
-            si::rows_to_internal(L, o.fast.warm_start, k, wsi.get(), k, k, s);
-            o.fast.warm_start = wsi.get();
+            double* wsi = si::ws_get<double>(si::kWsAbiIn, (size_t)L.n * k);
+            si::rows_to_internal(L, o.fast.warm_start, k, wsi, k, k, s);
+            o.fast.warm_start = wsi;
         }
         si::EventTimer t(s);
         si::solve_eigenproblem_device(L, n_components, seed, o, evals, vi, st, s);
-        si::rows_to_external(L, vi.get(), k, evecs, k, k, s);
+        si::rows_to_external(L, vi, k, evecs, k, k, s);
         const double ms = t.stop_ms();
         fill_report(rep, st, evals, k, ms);
     });

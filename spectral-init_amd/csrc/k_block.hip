@@ -19,9 +19,12 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 
 constexpr int kGramBlocks = 512;
 
+// B may be split in two column blocks: columns c < q1 come from b, columns
+// q1 <= c < q from b2 (column c - q1), so [Y^T Y | Y^T LY] is one pass.
 template <typename T, int PT, int QT>
 __global__ __launch_bounds__(256) void k_gram(int64_t n, const T* __restrict__ a, int lda, int p,
-                                              const T* __restrict__ b, int ldb, int q,
+                                              const T* __restrict__ b, int ldb, int q1,
+                                              const T* __restrict__ b2, int ldb2, int q,
                                               double* __restrict__ partial,
                                               int64_t rows_per_block) {
     __shared__ double red[PT * 16][QT * 16 + 1];
@@ -36,32 +39,41 @@ __global__ __launch_bounds__(256) void k_gram(int64_t n, const T* __restrict__ a
         for (int tj = 0; tj < QT; ++tj) acc[ti][tj] = v4d{0.0, 0.0, 0.0, 0.0};
     const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
     const int64_t r1 = (r0 + rows_per_block) < n ? (r0 + rows_per_block) : n;
-    // A^T A: read the block once
-    const bool same = (PT == QT) && (a == b) && (lda == ldb) && (p == q);
-    for (int64_t r = r0 + wave * 4; r < r1; r += 16) {
-        const int64_t row = r + lk;
-        const bool ok = row < r1;
-        double av[PT], bv[QT];
+    // the first B block equal to A: read it once
+    const bool same = (a == b) && (lda == ldb) && (q1 == p) && (q1 % 16 == 0 || q1 == q);
+    // UR row groups of 4 rows per wave in flight: all loads first, then MFMAs
+    constexpr int UR = 4;
+    for (int64_t rb = r0 + wave * 4; rb < r1; rb += 16 * UR) {
+        double av[UR][PT], bv[UR][QT];
 #pragma unroll
-        for (int t = 0; t < PT; ++t) {
-            const int c = t * 16 + li;
-            av[t] = (ok && c < p) ? (double)a[row * lda + c] : 0.0;
-        }
-        if (same) {
+        for (int g = 0; g < UR; ++g) {
+            const int64_t row = rb + 16 * g + lk;
+            const bool ok = row < r1;
 #pragma unroll
-            for (int t = 0; t < QT; ++t) bv[t] = av[t < PT ? t : 0];
-        } else {
+            for (int t = 0; t < PT; ++t) {
+                const int c = t * 16 + li;
+                av[g][t] = (ok && c < p) ? (double)a[row * lda + c] : 0.0;
+            }
 #pragma unroll
             for (int t = 0; t < QT; ++t) {
                 const int c = t * 16 + li;
-                bv[t] = (ok && c < q) ? (double)b[row * ldb + c] : 0.0;
+                if (same && c < q1 && t < PT) {
+                    bv[g][t] = av[g][t < PT ? t : 0];
+                } else if (c < q1) {
+                    bv[g][t] = ok ? (double)b[row * ldb + c] : 0.0;
+                } else {
+                    bv[g][t] = (ok && c < q) ? (double)b2[row * ldb2 + (c - q1)] : 0.0;
+                }
             }
         }
 #pragma unroll
-        for (int ti = 0; ti < PT; ++ti)
+        for (int g = 0; g < UR; ++g)
 #pragma unroll
-            for (int tj = 0; tj < QT; ++tj)
-                acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ti], bv[tj], acc[ti][tj], 0, 0, 0);
+            for (int ti = 0; ti < PT; ++ti)
+#pragma unroll
+                for (int tj = 0; tj < QT; ++tj)
+                    acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[g][ti], bv[g][tj],
+                                                                       acc[ti][tj], 0, 0, 0);
     }
     // D[i][j] of a tile sits in lane (j + 16 * (i % 4)), element i / 4
     // (measured on gfx950 with an integer-valued probe).
@@ -107,9 +119,11 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const double* __restric
 }
 
 template <typename T>
-void gram_impl(int64_t n, const T* a, int lda, int p, const T* b, int ldb, int q, double* g,
-               double* ws, hipStream_t s) {
-    SI_CHECK(p >= 1 && q >= 1 && p <= 64 && q <= 64, kErrInvalidArgument, "gram: width out of range");
+void gram_impl(int64_t n, const T* a, int lda, int p, const T* b, int ldb, int q1, const T* b2,
+               int ldb2, int q, double* g, double* ws, hipStream_t s) {
+    SI_CHECK(p >= 1 && q >= 1 && p <= 64 && q <= 64 && q1 >= 1 && q1 <= q, kErrInvalidArgument,
+             "gram: width out of range");
+    SI_CHECK(q1 == q || b2 != nullptr, kErrInvalidArgument, "gram: second block missing");
     int64_t nblocks = ceil_div(n, 64);
     if (nblocks > kGramBlocks) nblocks = kGramBlocks;
     if (nblocks < 1) nblocks = 1;
@@ -121,7 +135,7 @@ void gram_impl(int64_t n, const T* a, int lda, int p, const T* b, int ldb, int q
 #define SI_GRAM_CASE(PT_, QT_)                                                                   \
     if (pt == PT_ && qt == QT_) {                                                                \
         hipLaunchKernelGGL((k_gram<T, PT_, QT_>), dim3(nblocks), dim3(256), 0, s, n, a, lda, p, b, \
-                           ldb, q, ws, rpb);                                                      \
+                           ldb, q1, b2, ldb2, q, ws, rpb);                                        \
     } else
     SI_GRAM_CASE(1, 1) SI_GRAM_CASE(1, 2) SI_GRAM_CASE(1, 3) SI_GRAM_CASE(1, 4)
     SI_GRAM_CASE(2, 1) SI_GRAM_CASE(2, 2) SI_GRAM_CASE(2, 3) SI_GRAM_CASE(2, 4)
@@ -212,6 +226,108 @@ void block_mul_impl(int64_t n, const T* a, int lda, int p, const double* c, int 
     SI_LAUNCH_CHECK();
 }
 
+// ------------------------------------------------------------------ fused RR update
+// x = y C, ax = ly C (same C), and for the first k columns the partial sums
+// ||ax_j - theta_j x_j||^2 and ||x_j||^2 of this workgroup's rows
+// (partial[block][j], partial[block][k + j]); k <= CPT so one thread per row
+// owns all wanted columns.
+template <typename T, int QMAX>
+__global__ __launch_bounds__(256) void k_rr_update(int64_t n, const T* __restrict__ y,
+                                                   const T* __restrict__ ly, int ld, int m,
+                                                   const double* __restrict__ c,
+                                                   const double* __restrict__ theta, int k,
+                                                   T* __restrict__ x, T* __restrict__ ax,
+                                                   double* __restrict__ partial) {
+    constexpr int R = sizeof(T) == 8 ? 32 : 64;
+    constexpr int TPR = 256 / R;
+    constexpr int CPT = (QMAX + TPR - 1) / TPR;
+    constexpr int LDT = 65;
+    __shared__ T cm[64 * QMAX];
+    __shared__ T tile[R * LDT];
+    __shared__ double red[R][2 * 8 + 1];
+    const int t = threadIdx.x;
+    const int r = t / TPR, g = t % TPR;
+    const int64_t row0 = (int64_t)blockIdx.x * R;
+    const int rows = (n - row0) < R ? (int)(n - row0) : R;
+    for (int e = t; e < m * m; e += 256) cm[(e / m) * QMAX + e % m] = (T)c[e];
+    T accx[CPT], acca[CPT];
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) { accx[u] = T(0); acca[u] = T(0); }
+    for (int pass = 0; pass < 2; ++pass) {
+        const T* src = pass == 0 ? y : ly;
+        __syncthreads();
+        for (int e = t; e < rows * m; e += 256) {
+            const int rr = e / m, kk = e % m;
+            tile[rr * LDT + kk] = src[(row0 + rr) * ld + kk];
+        }
+        __syncthreads();
+        if (r < rows) {
+            for (int kk = 0; kk < m; ++kk) {
+                const T av = tile[r * LDT + kk];
+#pragma unroll
+                for (int u = 0; u < CPT; ++u) {
+                    const int j = g * CPT + u;
+                    if (j < QMAX) {
+                        if (pass == 0) accx[u] += av * cm[kk * QMAX + j];
+                        else acca[u] += av * cm[kk * QMAX + j];
+                    }
+                }
+            }
+        }
+    }
+    double rr2[8], xx2[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) { rr2[j] = 0.0; xx2[j] = 0.0; }
+    if (r < rows) {
+        T* xo = x + (row0 + r) * ld;
+        T* ao = ax + (row0 + r) * ld;
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) {
+            const int j = g * CPT + u;
+            if (j < m) {
+                xo[j] = accx[u];
+                ao[j] = acca[u];
+                if (j < k && j < 8) {
+                    const double xv = (double)accx[u];
+                    const double rv = (double)acca[u] - theta[j] * xv;
+                    rr2[j] = rv * rv;
+                    xx2[j] = xv * xv;
+                }
+            }
+        }
+    }
+    if (g == 0) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) { red[r][j] = rr2[j]; red[r][8 + j] = xx2[j]; }
+    }
+    __syncthreads();
+    if (t < 2 * k && t < 16) {
+        const int j = t < k ? t : 8 + (t - k);
+        double sacc = 0.0;
+        for (int q = 0; q < R; ++q) sacc += red[q][j];
+        partial[(int64_t)blockIdx.x * 2 * k + t] = sacc;
+    }
+}
+
+template <typename T>
+void rr_update_impl(int64_t n, const T* y, const T* ly, int ld, int m, const double* c,
+                    const double* theta, int k, T* x, T* ax, double* norms2, double* ws,
+                    hipStream_t s) {
+    SI_CHECK(m >= 1 && m <= 64 && k >= 1 && k <= 8 && k <= m, kErrInvalidArgument,
+             "rr_update: need k <= 8 <= m <= 64");
+    constexpr int R = sizeof(T) == 8 ? 32 : 64;
+    const int64_t blocks = ceil_div(n, R);
+    // QMAX = 32 (FP32, m <= 32) or 64 gives CPT = 8 or 16 columns per thread,
+    // so the thread that owns column 0 owns every wanted column (k <= 8)
+    if (m <= 32 && sizeof(T) == 4)
+        hipLaunchKernelGGL((k_rr_update<T, 32>), dim3(blocks), dim3(256), 0, s, n, y, ly, ld, m, c, theta, k, x, ax, ws);
+    else
+        hipLaunchKernelGGL((k_rr_update<T, 64>), dim3(blocks), dim3(256), 0, s, n, y, ly, ld, m, c, theta, k, x, ax, ws);
+    SI_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, ws, (int)blocks, 2 * k, norms2);
+    SI_LAUNCH_CHECK();
+}
+
 // ------------------------------------------------------------------ elementwise
 template <typename T>
 __global__ void k_residual(int64_t n, const T* __restrict__ ax, const T* __restrict__ x, int ld,
@@ -296,11 +412,31 @@ size_t gram_workspace_doubles(int64_t n, int p, int q) {
 
 void gram_f32(int64_t n, const float* a, int lda, int p, const float* b, int ldb, int q,
               double* g, double* ws, hipStream_t s) {
-    gram_impl<float>(n, a, lda, p, b, ldb, q, g, ws, s);
+    gram_impl<float>(n, a, lda, p, b, ldb, q, nullptr, 0, q, g, ws, s);
 }
 void gram_f64(int64_t n, const double* a, int lda, int p, const double* b, int ldb, int q,
               double* g, double* ws, hipStream_t s) {
-    gram_impl<double>(n, a, lda, p, b, ldb, q, g, ws, s);
+    gram_impl<double>(n, a, lda, p, b, ldb, q, nullptr, 0, q, g, ws, s);
+}
+void rr_update_f32(int64_t n, const float* y, const float* ly, int ld, int m, const double* c,
+                   const double* theta, int k, float* x, float* ax, double* norms2, double* ws,
+                   hipStream_t s) {
+    rr_update_impl<float>(n, y, ly, ld, m, c, theta, k, x, ax, norms2, ws, s);
+}
+void rr_update_f64(int64_t n, const double* y, const double* ly, int ld, int m, const double* c,
+                   const double* theta, int k, double* x, double* ax, double* norms2, double* ws,
+                   hipStream_t s) {
+    rr_update_impl<double>(n, y, ly, ld, m, c, theta, k, x, ax, norms2, ws, s);
+}
+size_t rr_update_workspace_doubles(int64_t n, int k) { return (size_t)ceil_div(n, 32) * 2 * k; }
+
+void gram2_f32(int64_t n, const float* a, int lda, int p, const float* b, int ldb, int q1,
+               const float* b2, int ldb2, int q2, double* g, double* ws, hipStream_t s) {
+    gram_impl<float>(n, a, lda, p, b, ldb, q1, b2, ldb2, q1 + q2, g, ws, s);
+}
+void gram2_f64(int64_t n, const double* a, int lda, int p, const double* b, int ldb, int q1,
+               const double* b2, int ldb2, int q2, double* g, double* ws, hipStream_t s) {
+    gram_impl<double>(n, a, lda, p, b, ldb, q1, b2, ldb2, q1 + q2, g, ws, s);
 }
 
 void block_mul_f32(int64_t n, const float* a, int lda, int p, const double* c, int q,

@@ -182,8 +182,13 @@ __global__ __launch_bounds__(kThreads) void k_sym_eig(int n, const double* __res
     sort_and_store(a, v, n, order, w, out);
 }
 
+// A c = theta B c for strided inputs (lower triangles read).  With scale != 0
+// the problem is first scaled by D = diag(B)^-1/2 on both sides (column
+// normalisation of the underlying block, done on the small matrices) and the
+// returned vectors are D c'.
 __global__ __launch_bounds__(kThreads) void k_gen_eig(int n, const double* __restrict__ ain,
-                                                       const double* __restrict__ bin,
+                                                       int lda, const double* __restrict__ bin,
+                                                       int ldb, int scale,
                                                        double* __restrict__ w,
                                                        double* __restrict__ cout,
                                                        int* __restrict__ status) {
@@ -193,12 +198,19 @@ __global__ __launch_bounds__(kThreads) void k_gen_eig(int n, const double* __res
     __shared__ double cs[N / 2], sn[N / 2];
     __shared__ int pp[N / 2], qq[N / 2], order[N];
     __shared__ double red[2 * kThreads];
+    __shared__ double dsc[N];
     __shared__ int flag;
     const int t = threadIdx.x;
+    if (t < n) {
+        const double bd = bin[t * ldb + t];
+        dsc[t] = scale ? (bd > 0.0 ? 1.0 / sqrt(bd) : 0.0) : 1.0;
+    }
+    __syncthreads();
     for (int e = t; e < n * n; e += kThreads) {
         const int i = e / n, j = e % n;
-        a[i][j] = (j <= i) ? ain[i * n + j] : ain[j * n + i];
-        l[i][j] = (j <= i) ? bin[i * n + j] : bin[j * n + i];
+        const double s = dsc[i] * dsc[j];
+        a[i][j] = s * ((j <= i) ? ain[i * lda + j] : ain[j * lda + i]);
+        l[i][j] = s * ((j <= i) ? bin[i * ldb + j] : bin[j * ldb + i]);
     }
     __syncthreads();
     if (!cholesky_lds(l, n, &flag)) {
@@ -238,6 +250,8 @@ __global__ __launch_bounds__(kThreads) void k_gen_eig(int n, const double* __res
             for (int k = i + 1; k < n; ++k) s -= l[k][i] * v[k][c];
             v[i][c] = s / l[i][i];
         }
+    __syncthreads();
+    for (int e = t; e < n * n; e += kThreads) v[e / n][e % n] *= dsc[e / n];
     __syncthreads();
     sort_and_store(a, v, n, order, w, cout);
     if (t == 0) status[0] = 0;
@@ -348,7 +362,14 @@ void small_sym_eig(int n, const double* a, int lda, double* w, double* v, hipStr
 void small_gen_eig(int n, const double* a, const double* b, double* w, double* c, int* status,
                    hipStream_t s) {
     SI_CHECK(n >= 1 && n <= kSmallMax, kErrInvalidArgument, "small_gen_eig: n out of range");
-    hipLaunchKernelGGL(k_gen_eig, dim3(1), dim3(kThreads), 0, s, n, a, b, w, c, status);
+    hipLaunchKernelGGL(k_gen_eig, dim3(1), dim3(kThreads), 0, s, n, a, n, b, n, 0, w, c, status);
+    SI_LAUNCH_CHECK();
+}
+
+void small_gen_eig_scaled(int n, const double* a, int lda, const double* b, int ldb, double* w,
+                          double* c, int* status, hipStream_t s) {
+    SI_CHECK(n >= 1 && n <= kSmallMax, kErrInvalidArgument, "small_gen_eig: n out of range");
+    hipLaunchKernelGGL(k_gen_eig, dim3(1), dim3(kThreads), 0, s, n, a, lda, b, ldb, 1, w, c, status);
     SI_LAUNCH_CHECK();
 }
 

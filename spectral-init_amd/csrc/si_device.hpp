@@ -97,6 +97,21 @@ void gram_f32(int64_t n, const float* a, int lda, int p, const float* b, int ldb
               double* g, double* ws, hipStream_t s);
 void gram_f64(int64_t n, const double* a, int lda, int p, const double* b, int ldb, int q,
               double* g, double* ws, hipStream_t s);
+// Fused Rayleigh-Ritz update: x = y C, ax = ly C (C m x m), and the sums
+// norms2[j] = ||ax_j - theta_j x_j||^2, norms2[k + j] = ||x_j||^2 for j < k <= 8.
+// ws: rr_update_workspace_doubles(n, k).
+void rr_update_f32(int64_t n, const float* y, const float* ly, int ld, int m, const double* c,
+                   const double* theta, int k, float* x, float* ax, double* norms2, double* ws,
+                   hipStream_t s);
+void rr_update_f64(int64_t n, const double* y, const double* ly, int ld, int m, const double* c,
+                   const double* theta, int k, double* x, double* ax, double* norms2, double* ws,
+                   hipStream_t s);
+size_t rr_update_workspace_doubles(int64_t n, int k);
+// G = A^T [B1 | B2]  (p x (q1+q2), q1 + q2 <= 64) in one pass over the rows.
+void gram2_f32(int64_t n, const float* a, int lda, int p, const float* b, int ldb, int q1,
+               const float* b2, int ldb2, int q2, double* g, double* ws, hipStream_t s);
+void gram2_f64(int64_t n, const double* a, int lda, int p, const double* b, int ldb, int q1,
+               const double* b2, int ldb2, int q2, double* g, double* ws, hipStream_t s);
 
 // out[:, 0:q] = A[:, 0:p] * C (+ B[:, 0:p2] * D), C, D FP64 row-major device matrices.
 void block_mul_f32(int64_t n, const float* a, int lda, int p, const double* c, int q,
@@ -134,6 +149,10 @@ void small_sym_eig(int n, const double* a, int lda, double* w, double* v, hipStr
 // Generalised problem A c = theta B c.  status[0] = 0 ok, 1 if B not SPD.
 void small_gen_eig(int n, const double* a, const double* b, double* w, double* c, int* status,
                    hipStream_t s);
+// Same for strided A, B, with both sides scaled by diag(B)^-1/2 first; the
+// returned vectors apply to the unscaled problem.
+void small_gen_eig_scaled(int n, const double* a, int lda, const double* b, int ldb, double* w,
+                          double* c, int* status, hipStream_t s);
 // Given G = X^T X (n x n), writes T = L^-T (upper) so that X T is orthonormal;
 // status[0] = 1 when G is not positive definite.
 void small_chol_inv(int n, const double* g, double* t, int* status, hipStream_t s);

@@ -38,6 +38,26 @@ struct DevBuf {
     operator T*() const { return p; }
 };
 
+// Grow-only device workspace, one per host thread (a context is used by one
+// host thread).  Solvers take their blocks from named slots, so repeated
+// solves allocate nothing after the first one of a given size.  Released
+// when the thread exits or by workspace_release().
+enum WsSlot : int {
+    kWsFastX64, kWsFastAx64, kWsFastR64, kWsFastF64a, kWsFastF64b, kWsFastF64c,
+    kWsFastX32, kWsFastAx32, kWsFastR32, kWsFastF32a, kWsFastF32b, kWsFastF32c,
+    kWsGram, kWsResAx, kWsResR, kWsRrAx, kWsChainVec, kWsAbiIn, kWsAbiOut,
+    kWsSmall0, kWsSmall1, kWsSmall2, kWsSmall3, kWsSmall4, kWsSmall5, kWsSmall6, kWsSmall7,
+    kWsPlainS, kWsPlainAS, kWsPlainR, kWsPlainP, kWsPlainAP, kWsPlainPn, kWsPlainAPn,
+    kWsPlainX, kWsPlainT, kWsPlainXr, kWsPlainBest, kWsStatus,
+    kWsNumSlots
+};
+void* workspace_get(int slot, size_t bytes);
+void workspace_release();
+template <typename T>
+inline T* ws_get(int slot, size_t count) {
+    return static_cast<T*>(workspace_get(slot, sizeof(T) * (count ? count : 1)));
+}
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -88,6 +108,7 @@ struct SolveStats {
     int64_t spmm_calls = 0;
     double max_residual = 0.0;
     bool converged = false;
+    bool residual_is_fp64 = false;  // max_residual is the FP64 residual of the returned vectors
 };
 
 struct FastOptions {

@@ -6,21 +6,48 @@
 
 namespace si {
 
+namespace {
+struct Workspace {
+    std::vector<DevBuf<char>> slots;
+    Workspace() : slots(kWsNumSlots) {}
+};
+thread_local Workspace* g_ws = nullptr;
+}  // namespace
+
+void* workspace_get(int slot, size_t bytes) {
+    if (!g_ws) g_ws = new Workspace();
+    DevBuf<char>& b = g_ws->slots[slot];
+    if (b.p == nullptr || b.n < bytes) {
+        // grow by 25 % beyond the request so nearby sizes reuse the block
+        b.alloc(bytes + bytes / 4);
+    }
+    return b.p;
+}
+
+void workspace_release() {
+    delete g_ws;
+    g_ws = nullptr;
+}
+
 // ||L v_j - lambda_j v_j|| / max(||v_j||, eps): one block SpMM, one residual
 // block, two FP64 Grams (per_vector_residuals, lobpcg.rs:193-223).
 void fp64_residuals(const DevLaplacian& L, const double* v, int k, const double* evals_host,
                     double* res_host, hipStream_t s) {
     const int64_t n = L.n;
-    DevBuf<double> ax((size_t)n * k), r((size_t)n * k), lam(k), g((size_t)k * k),
-        gx((size_t)k * k), ws(gram_workspace_doubles(n, k, k));
+    double* ax = ws_get<double>(kWsResAx, (size_t)n * k);
+    double* r = ws_get<double>(kWsResR, (size_t)n * k);
+    double* lam = ws_get<double>(kWsSmall0, k);
+    double* g = ws_get<double>(kWsSmall1, (size_t)k * k);
+    double* gx = ws_get<double>(kWsSmall2, (size_t)k * k);
+    double* ws = ws_get<double>(kWsGram, gram_workspace_doubles(n, k, k));
     spmm_f64(L, v, ax, k, k, 1.0, 0.0, 0.0, nullptr, s);
-    to_device(lam.get(), evals_host, k, s);
+    to_device(lam, evals_host, k, s);
     residual_f64(n, ax, v, k, k, lam, r, k, s);
     gram_f64(n, r, k, k, r, k, k, g, ws, s);
     gram_f64(n, v, k, k, v, k, k, gx, ws, s);
     std::vector<double> hg((size_t)k * k), hx((size_t)k * k);
-    to_host(hg.data(), g.get(), hg.size(), s);
-    to_host(hx.data(), gx.get(), hx.size(), s);
+    to_host(hg.data(), g, hg.size(), s);
+    to_host(hx.data(), gx, hx.size(), s);
     for (int c = 0; c < k; ++c) {
         double vn = std::sqrt(std::fmax(hx[c * k + c], 0.0));
         if (vn < 2.220446049250313e-16) vn = 2.220446049250313e-16;
@@ -32,17 +59,20 @@ void fp64_residuals(const DevLaplacian& L, const double* v, int k, const double*
 // triangle), X <- X V.
 bool rayleigh_ritz_f64(const DevLaplacian& L, double* v, int k, double* evals_host, hipStream_t s) {
     const int64_t n = L.n;
-    DevBuf<double> ax((size_t)n * k), g((size_t)k * k), w(k), vv((size_t)k * k),
-        ws(gram_workspace_doubles(n, k, k));
+    double* ax = ws_get<double>(kWsRrAx, (size_t)n * k);
+    double* g = ws_get<double>(kWsSmall3, (size_t)k * k);
+    double* w = ws_get<double>(kWsSmall4, k);
+    double* vv = ws_get<double>(kWsSmall5, (size_t)k * k);
+    double* ws = ws_get<double>(kWsGram, gram_workspace_doubles(n, k, k));
     spmm_f64(L, v, ax, k, k, 1.0, 0.0, 0.0, nullptr, s);
     gram_f64(n, v, k, k, ax, k, k, g, ws, s);
     std::vector<double> hg((size_t)k * k);
-    to_host(hg.data(), g.get(), hg.size(), s);
+    to_host(hg.data(), g, hg.size(), s);
     for (double x : hg)
         if (!std::isfinite(x)) return false;
     small_sym_eig(k, g, k, w, vv, s);
     block_mul_f64(n, v, k, k, vv, k, nullptr, 0, 0, nullptr, v, k, s);
-    to_host(evals_host, w.get(), k, s);
+    to_host(evals_host, w, k, s);
     return true;
 }
 
@@ -58,12 +88,14 @@ void chebyshev_filter_f64(const DevLaplacian& L, const double* r, int k, double 
         SI_HIP(hipMemcpyAsync(out, r, sizeof(double) * cnt, hipMemcpyDeviceToDevice, s));
         return;
     }
-    DevBuf<double> b0(cnt), b1(cnt), b2(cnt);
-    SI_HIP(hipMemcpyAsync(b0.get(), r, sizeof(double) * cnt, hipMemcpyDeviceToDevice, s));
+    double* b0 = ws_get<double>(kWsFastF64a, cnt);
+    double* b1 = ws_get<double>(kWsFastF64b, cnt);
+    double* b2 = ws_get<double>(kWsFastF64c, cnt);
+    SI_HIP(hipMemcpyAsync(b0, r, sizeof(double) * cnt, hipMemcpyDeviceToDevice, s));
     spmm_f64(L, b0, b1, k, k, 1.0 / e, -c / e, 0.0, nullptr, s);
-    double* yprev = b0.get();
-    double* y = b1.get();
-    double* ynew = b2.get();
+    double* yprev = b0;
+    double* y = b1;
+    double* ynew = b2;
     for (int d = 2; d <= degree; ++d) {
         spmm_f64(L, y, ynew, k, k, scale, -shift, -1.0, yprev, s);
         double* t = yprev; yprev = y; y = ynew; ynew = t;

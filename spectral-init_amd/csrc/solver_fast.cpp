@@ -43,49 +43,35 @@ int choose_block(int64_t n, int k, int requested) {
 
 template <typename T>
 struct FilterBufs {
-    DevBuf<T> b0, b1, b2;
+    T* b0 = nullptr;
+    T* b1 = nullptr;
+    T* b2 = nullptr;
 };
 
-// Y = T_d((L - cI)/e) X in precision T, returned in one of the buffers.
+// Y = T_d((L - cI)/e) X on the first w columns (leading dimension ld), in
+// precision T, returned in one of the buffers.  The recurrence runs in the
+// SpMM epilogue: y1 = (L x - c x)/e, y_{j+1} = (2/e)(L y_j - c y_j) - y_{j-1}.
 template <typename T>
-T* cheb_filter(const DevLaplacian& L, FilterBufs<T>& fb, const T* x, int m, double a, double b,
-               int degree, int64_t& spmm_calls, hipStream_t s);
-
-template <>
-float* cheb_filter<float>(const DevLaplacian& L, FilterBufs<float>& fb, const float* x, int m,
-                          double a, double b, int degree, int64_t& spmm_calls, hipStream_t s) {
+T* cheb_filter(const DevLaplacian& L, FilterBufs<T>& fb, const T* x, int w, int ld, double a,
+               double b, int degree, int64_t& spmm_calls, hipStream_t s) {
     const double c = (a + b) / 2.0, e = (b - a) / 2.0;
-    const float scale = (float)(2.0 / e), shift = (float)(2.0 * c / e);
-    float* yprev = fb.b0.get();
-    float* y = fb.b1.get();
-    float* ynew = fb.b2.get();
-    spmm_f32(L, x, y, m, m, (float)(1.0 / e), (float)(-c / e), 0.f, nullptr, s);
+    const T scale = (T)(2.0 / e), shift = (T)(2.0 * c / e);
+    T* yprev = fb.b0;
+    T* y = fb.b1;
+    T* ynew = fb.b2;
+    if constexpr (sizeof(T) == 4)
+        spmm_f32(L, x, y, w, ld, (T)(1.0 / e), (T)(-c / e), T(0), nullptr, s);
+    else
+        spmm_f64(L, x, y, w, ld, (T)(1.0 / e), (T)(-c / e), T(0), nullptr, s);
     ++spmm_calls;
-    const float* prev = x;
+    const T* prev = x;
     for (int d = 2; d <= degree; ++d) {
-        spmm_f32(L, y, ynew, m, m, scale, -shift, -1.f, prev, s);
+        if constexpr (sizeof(T) == 4)
+            spmm_f32(L, y, ynew, w, ld, scale, -shift, T(-1), prev, s);
+        else
+            spmm_f64(L, y, ynew, w, ld, scale, -shift, T(-1), prev, s);
         ++spmm_calls;
-        float* t = yprev; yprev = y; y = ynew; ynew = t;
-        prev = yprev;
-    }
-    return y;
-}
-
-template <>
-double* cheb_filter<double>(const DevLaplacian& L, FilterBufs<double>& fb, const double* x, int m,
-                            double a, double b, int degree, int64_t& spmm_calls, hipStream_t s) {
-    const double c = (a + b) / 2.0, e = (b - a) / 2.0;
-    const double scale = 2.0 / e, shift = 2.0 * c / e;
-    double* yprev = fb.b0.get();
-    double* y = fb.b1.get();
-    double* ynew = fb.b2.get();
-    spmm_f64(L, x, y, m, m, 1.0 / e, -c / e, 0.0, nullptr, s);
-    ++spmm_calls;
-    const double* prev = x;
-    for (int d = 2; d <= degree; ++d) {
-        spmm_f64(L, y, ynew, m, m, scale, -shift, -1.0, prev, s);
-        ++spmm_calls;
-        double* t = yprev; yprev = y; y = ynew; ynew = t;
+        T* t = yprev; yprev = y; y = ynew; ynew = t;
         prev = yprev;
     }
     return y;
@@ -116,6 +102,14 @@ inline void scale_cols_t(double* x, int ld, int64_t n, int k, const double* sc, 
 inline void spmm_t(const DevLaplacian& L, const float* x, float* y, int m, hipStream_t s) {
     spmm_f32(L, x, y, m, m, 1.f, 0.f, 0.f, nullptr, s);
 }
+inline void spmm_t_ld(const DevLaplacian& L, const float* x, float* y, int w, int ld,
+                      hipStream_t s) {
+    spmm_f32(L, x, y, w, ld, 1.f, 0.f, 0.f, nullptr, s);
+}
+inline void spmm_t_ld(const DevLaplacian& L, const double* x, double* y, int w, int ld,
+                      hipStream_t s) {
+    spmm_f64(L, x, y, w, ld, 1.0, 0.0, 0.0, nullptr, s);
+}
 inline void spmm_t(const DevLaplacian& L, const double* x, double* y, int m, hipStream_t s) {
     spmm_f64(L, x, y, m, m, 1.0, 0.0, 0.0, nullptr, s);
 }
@@ -128,13 +122,15 @@ inline void residual_t(int64_t n, const double* ax, const double* x, int ld, int
     residual_f64(n, ax, x, ld, k, th, r, ldr, s);
 }
 
+// Small FP64 matrices of one solve, carved from one workspace slot.
 struct Small {
-    DevBuf<double> g, g2, t, w, v, sc, res;
-    DevBuf<int> status;
+    double *g, *g2, *t, *w, *v, *sc, *res;
+    int* status;
     explicit Small(int m) {
-        g.alloc((size_t)m * m); g2.alloc((size_t)m * m); t.alloc((size_t)m * m);
-        w.alloc(m); v.alloc((size_t)m * m); sc.alloc(m); res.alloc(m);
-        status.alloc(1);
+        const size_t mm = (size_t)m * m;
+        double* base = ws_get<double>(kWsSmall6, 4 * mm + 4 * (size_t)m + 32);
+        g = base; g2 = g + mm; t = g2 + mm; v = t + mm; w = v + mm; sc = w + m; res = sc + m;
+        status = ws_get<int>(kWsStatus, 1);
     }
 };
 
@@ -149,7 +145,7 @@ bool orthonormalise(int64_t n, T* y, int m, Small& sm, double* ws, hipStream_t s
         gram_t(n, y, m, m, y, m, m, sm.g, ws, s);
         small_chol_inv(m, sm.g, sm.t, sm.status, s);
         int st = 0;
-        to_host(&st, sm.status.get(), 1, s);
+        to_host(&st, sm.status, 1, s);
         if (st != 0) return false;
         block_mul_t(n, y, m, m, sm.t, m, y, m, s);
     }
@@ -167,10 +163,60 @@ bool rayleigh_ritz(const DevLaplacian& L, int64_t n, T* y, T* ly, int m, T* x, T
     gram_t(n, y, m, m, y, m, m, sm.g2, ws, s);
     small_gen_eig(m, sm.g, sm.g2, sm.w, sm.v, sm.status, s);
     int st = 0;
-    to_host(&st, sm.status.get(), 1, s);
+    to_host(&st, sm.status, 1, s);
     if (st != 0) return false;
     block_mul_t(n, y, m, m, sm.v, m, x, m, s);
     block_mul_t(n, ly, m, m, sm.v, m, ax, m, s);
+    return true;
+}
+
+inline void gram2_t(int64_t n, const float* a, int ld, int p, const float* b2, double* g,
+                    double* ws, hipStream_t s) {
+    gram2_f32(n, a, ld, p, a, ld, p, b2, ld, p, g, ws, s);
+}
+inline void gram2_t(int64_t n, const double* a, int ld, int p, const double* b2, double* g,
+                    double* ws, hipStream_t s) {
+    gram2_f64(n, a, ld, p, a, ld, p, b2, ld, p, g, ws, s);
+}
+inline void rr_update_t(int64_t n, const float* y, const float* ly, int ld, int m,
+                        const double* c, const double* th, int k, float* x, float* ax,
+                        double* norms2, double* ws, hipStream_t s) {
+    rr_update_f32(n, y, ly, ld, m, c, th, k, x, ax, norms2, ws, s);
+}
+inline void rr_update_t(int64_t n, const double* y, const double* ly, int ld, int m,
+                        const double* c, const double* th, int k, double* x, double* ax,
+                        double* norms2, double* ws, hipStream_t s) {
+    rr_update_f64(n, y, ly, ld, m, c, th, k, x, ax, norms2, ws, s);
+}
+
+// Fused Rayleigh-Ritz of the first w columns of y (leading dimension ld):
+//   ly = L y;  G = y^T [y | ly] (one pass);  (y^T ly) c = theta (y^T y) c with
+//   diagonal scaling (small, on the device);  x = y C, ax = ly C and the
+//   wanted residual norms in one pass;  one host synchronisation.
+// Returns false when y^T y is not numerically positive definite.
+template <typename T>
+bool fused_rr(const DevLaplacian& L, int64_t n, T* y, T* ly, int w, int ld, int k, T* x, T* ax,
+              Small& sm, double* gws, double* rr_ws, double* theta_host, double* res_host,
+              int64_t& spmm_calls, hipStream_t s) {
+    if (2 * w > 64 || k > 8) return false;  // the one-pass Gram covers up to 64 columns
+    spmm_t_ld(L, y, ly, w, ld, s);
+    ++spmm_calls;
+    double* g = sm.g;  // w x 2w, row stride 2w (fits: g, g2 are contiguous m*m blocks)
+    gram2_t(n, y, ld, w, ly, g, gws, s);
+    small_gen_eig_scaled(w, g + w, 2 * w, g, 2 * w, sm.w, sm.v, sm.status, s);
+    rr_update_t(n, y, ly, ld, w, sm.v, sm.w, k, x, ax, sm.res, rr_ws, s);
+    int st = 0;
+    std::vector<double> nrm(2 * k);
+    SI_HIP(hipMemcpyAsync(&st, sm.status, sizeof(int), hipMemcpyDeviceToHost, s));
+    SI_HIP(hipMemcpyAsync(theta_host, sm.w, sizeof(double) * w, hipMemcpyDeviceToHost, s));
+    SI_HIP(hipMemcpyAsync(nrm.data(), sm.res, sizeof(double) * 2 * k, hipMemcpyDeviceToHost, s));
+    SI_HIP(hipStreamSynchronize(s));
+    if (st != 0) return false;
+    for (int j = 0; j < k; ++j) {
+        double xn = std::sqrt(std::fmax(nrm[k + j], 0.0));
+        if (xn < 2.220446049250313e-16) xn = 2.220446049250313e-16;
+        res_host[j] = std::sqrt(std::fmax(nrm[j], 0.0)) / xn;
+    }
     return true;
 }
 
@@ -182,7 +228,7 @@ void residual_norms(int64_t n, const T* ax, const T* x, int m, int k, T* r, Smal
     gram_t(n, r, k, k, r, k, k, sm.g, ws, s);
     gram_t(n, x, m, k, x, m, k, sm.g2, ws, s);
     small_residual_norms(k, sm.g, k, sm.g2, k, sm.res, s);
-    to_host(res_host, sm.res.get(), k, s);
+    to_host(res_host, sm.res, k, s);
 }
 
 }  // namespace
@@ -198,7 +244,9 @@ bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const Fa
     st.block_size = m;
     const int degree = opt.cheb_degree > 0 ? opt.cheb_degree : 8;
     const size_t nm = (size_t)n * m;
-    DevBuf<double> x64(nm), ws(gram_workspace_doubles(n, m, m)), g(1);
+    double* x64 = ws_get<double>(kWsFastX64, nm);
+    double* ws = ws_get<double>(kWsGram, gram_workspace_doubles(n, m, m));
+    double* g = ws_get<double>(kWsSmall7, 1);
     Small sm(m);
 
     // start block: warm start (first k columns) or N(0,1) with the trivial hint
@@ -208,14 +256,20 @@ bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const Fa
     } else {
         gram_f64(n, L.sqrt_deg, 1, 1, L.sqrt_deg, 1, 1, g, ws, s);
         double sn2 = 0.0;
-        to_host(&sn2, g.get(), 1, s);
+        to_host(&sn2, g, 1, s);
         if (sn2 > 0.0) set_column_f64(x64, m, n, 0, L.sqrt_deg, 1.0 / std::sqrt(sn2), s);
     }
 
     // ---------------- FP32 rounds
-    DevBuf<float> x32(nm), ax32(nm), r32((size_t)n * k);
+    float* x32 = ws_get<float>(kWsFastX32, nm);
+    float* ax32 = ws_get<float>(kWsFastAx32, nm);
+    float* r32 = ws_get<float>(kWsFastR32, (size_t)n * k);
+    double* rr_ws = ws_get<double>(kWsSmall5, rr_update_workspace_doubles(n, k));
+    double* g2w = ws_get<double>(kWsSmall4, gram_workspace_doubles(n, m, 2 * m));
     FilterBufs<float> fb32;
-    fb32.b0.alloc(nm); fb32.b1.alloc(nm); fb32.b2.alloc(nm);
+    fb32.b0 = ws_get<float>(kWsFastF32a, nm);
+    fb32.b1 = ws_get<float>(kWsFastF32b, nm);
+    fb32.b2 = ws_get<float>(kWsFastF32c, nm);
     convert_f64_to_f32(x64, x32, (int64_t)nm, s);
     std::vector<double> theta(m), res(k);
     double a = 0.1;               // first round: the reference's ChFSI interval
@@ -234,17 +288,21 @@ bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const Fa
         return std::max(2, std::min(degree, dmax));
     };
     for (; round < opt.max_outer; ++round) {
-        float* y = cheb_filter<float>(L, fb32, x32, m, a, b, degree_for(a), st.spmm_calls, s);
-        if (!orthonormalise<float>(n, y, m, sm, ws, s)) {
-            // fall back to a normalised (not orthonormalised) block
-            gram_t(n, y, m, m, y, m, m, sm.g, ws, s);
-            small_inv_sqrt_diag(m, sm.g, sm.sc, s);
-            scale_cols_t(y, m, n, m, sm.sc, s);
+        float* y = cheb_filter<float>(L, fb32, x32, m, m, a, b, degree_for(a), st.spmm_calls, s);
+        float* ly = (y == fb32.b0) ? fb32.b1 : fb32.b0;
+        // one fused Rayleigh-Ritz on the filtered block; the explicit
+        // orthonormalisation path is the fallback when Y^T Y is too ill-conditioned
+        if (!fused_rr<float>(L, n, y, ly, m, m, k, x32, ax32, sm, g2w, rr_ws, theta.data(),
+                             res.data(), st.spmm_calls, s)) {
+            if (!orthonormalise<float>(n, y, m, sm, ws, s)) {
+                gram_t(n, y, m, m, y, m, m, sm.g, ws, s);
+                small_inv_sqrt_diag(m, sm.g, sm.sc, s);
+                scale_cols_t(y, m, n, m, sm.sc, s);
+            }
+            if (!rayleigh_ritz<float>(L, n, y, ly, m, x32, ax32, sm, ws, st.spmm_calls, s)) break;
+            to_host(theta.data(), sm.w, m, s);
+            residual_norms<float>(n, ax32, x32, m, k, r32, sm, ws, res.data(), s);
         }
-        float* ly = (y == fb32.b0.get()) ? fb32.b1.get() : fb32.b0.get();
-        if (!rayleigh_ritz<float>(L, n, y, ly, m, x32, ax32, sm, ws, st.spmm_calls, s)) break;
-        to_host(theta.data(), sm.w.get(), m, s);
-        residual_norms<float>(n, ax32, x32, m, k, r32, sm, ws, res.data(), s);
         ++st.iterations;
         const double worst = *std::max_element(res.begin(), res.end());
         if (worst <= opt.tol) { fp32_ok = true; break; }
@@ -255,24 +313,30 @@ bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const Fa
     }
     (void)fp32_ok;
 
-    // ---------------- FP64 Rayleigh-Ritz, residual and (if needed) FP64 rounds
-    DevBuf<double> ax64(nm), r64((size_t)n * k);
+    // ---------------- FP64 Rayleigh-Ritz on the lowest mf Ritz vectors, FP64
+    // residual against the exact Laplacian, FP64 rounds only if still needed.
+    // The FP32 subspace is accurate to ~1e-7/gap; an FP64 Rayleigh-Ritz on it
+    // recovers eigenvalues to (1e-7/gap)^2, so the wanted columns plus a few
+    // guards suffice.
+    const int mf = std::min(m, std::max(8, k + 5));
+    double* ax64 = ws_get<double>(kWsFastAx64, nm);
+    (void)r32;
     FilterBufs<double> fb64;
-    fb64.b0.alloc(nm); fb64.b1.alloc(nm); fb64.b2.alloc(nm);
+    fb64.b0 = ws_get<double>(kWsFastF64a, nm);
+    fb64.b1 = ws_get<double>(kWsFastF64b, nm);
+    fb64.b2 = ws_get<double>(kWsFastF64c, nm);
     convert_f32_to_f64(x32, fb64.b0, (int64_t)nm, s);
-    bool ok = rayleigh_ritz<double>(L, n, fb64.b0, fb64.b1, m, x64, ax64, sm, ws, st.spmm_calls, s);
-    if (!ok) return false;
-    to_host(theta.data(), sm.w.get(), m, s);
-    residual_norms<double>(n, ax64, x64, m, k, r64, sm, ws, res.data(), s);
+    if (!fused_rr<double>(L, n, fb64.b0, fb64.b1, mf, m, k, x64, ax64, sm, g2w, rr_ws,
+                          theta.data(), res.data(), st.spmm_calls, s))
+        return false;
     double worst = *std::max_element(res.begin(), res.end());
     for (int extra = 0; worst > opt.tol && extra < 20; ++extra) {
-        a = std::min(std::max(theta[m - 1], theta[k - 1] * 1.5 + 1e-12), 0.6);
-        double* y = cheb_filter<double>(L, fb64, x64, m, a, b, degree_for(a), st.spmm_calls, s);
-        if (!orthonormalise<double>(n, y, m, sm, ws, s)) break;
-        double* ly = (y == fb64.b0.get()) ? fb64.b1.get() : fb64.b0.get();
-        if (!rayleigh_ritz<double>(L, n, y, ly, m, x64, ax64, sm, ws, st.spmm_calls, s)) break;
-        to_host(theta.data(), sm.w.get(), m, s);
-        residual_norms<double>(n, ax64, x64, m, k, r64, sm, ws, res.data(), s);
+        a = std::min(std::max(theta[mf - 1], theta[k - 1] * 1.5 + 1e-12), 0.6);
+        double* y = cheb_filter<double>(L, fb64, x64, mf, m, a, b, degree_for(a), st.spmm_calls, s);
+        double* ly = (y == fb64.b0) ? fb64.b1 : fb64.b0;
+        if (!fused_rr<double>(L, n, y, ly, mf, m, k, x64, ax64, sm, g2w, rr_ws, theta.data(),
+                              res.data(), st.spmm_calls, s))
+            break;
         ++st.iterations;
         worst = *std::max_element(res.begin(), res.end());
     }
@@ -281,6 +345,7 @@ bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const Fa
     for (int c = 0; c < k; ++c) evals_host[c] = theta[c];
     st.max_residual = worst;
     st.converged = worst <= opt.tol;
+    st.residual_is_fp64 = true;
     return true;
 }
 

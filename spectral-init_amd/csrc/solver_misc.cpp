@@ -141,7 +141,9 @@ int solve_eigenproblem_device(const DevLaplacian& L, int n_components, uint64_t 
         have = lobpcg_solve_plain(L, n_components, seed, false, opt.tol, evals_host, evecs_dev, st, s);
     }
     if (have) {
-        const double q = quality(evals_host);
+        // the FP32-bulk solver already measured the FP64 residual of exactly
+        // these vectors against the exact Laplacian (metrics.rs:57-92 semantics)
+        const double q = st.residual_is_fp64 ? st.max_residual : quality(evals_host);
         if (q < gate_l1) { st.level = 1; st.max_residual = q; return 1; }
     }
     SolveStats st3;

@@ -21,19 +21,20 @@ namespace {
 struct PlainWork {
     int64_t n = 0;
     int m = 0;
-    DevBuf<double> S, AS, R, P, AP, Pn, APn, ws;
+    // n-sized blocks come from the thread's workspace pool
+    double *S, *AS, *R, *P, *AP, *Pn, *APn, *ws;
     DevBuf<double> g1, g2, t1, w, c, csub, sel, eye, lam;
     DevBuf<int> status;
     PlainWork(int64_t n_, int m_) : n(n_), m(m_) {
         const int s3 = 3 * m;
-        S.alloc((size_t)n * s3);
-        AS.alloc((size_t)n * s3);
-        R.alloc((size_t)n * m);
-        P.alloc((size_t)n * m);
-        AP.alloc((size_t)n * m);
-        Pn.alloc((size_t)n * m);
-        APn.alloc((size_t)n * m);
-        ws.alloc(gram_workspace_doubles(n, s3, s3));
+        S = ws_get<double>(kWsPlainS, (size_t)n * s3);
+        AS = ws_get<double>(kWsPlainAS, (size_t)n * s3);
+        R = ws_get<double>(kWsPlainR, (size_t)n * m);
+        P = ws_get<double>(kWsPlainP, (size_t)n * m);
+        AP = ws_get<double>(kWsPlainAP, (size_t)n * m);
+        Pn = ws_get<double>(kWsPlainPn, (size_t)n * m);
+        APn = ws_get<double>(kWsPlainAPn, (size_t)n * m);
+        ws = ws_get<double>(kWsGram, gram_workspace_doubles(n, s3, s3));
         g1.alloc((size_t)s3 * s3);
         g2.alloc((size_t)s3 * s3);
         t1.alloc((size_t)s3 * s3);
@@ -68,8 +69,8 @@ bool lobpcg_core_plain(const DevLaplacian& L, double shift, double* x, int m, do
     const int s3 = 3 * m;
     PlainWork W(n, m);
     upload_eye(W.eye, m, s);
-    double* X = W.S.get();
-    double* AX = W.AS.get();
+    double* X = W.S;
+    double* AX = W.AS;
     auto apply = [&](const double* in, double* out, int k) {
         spmm_f64(L, in, out, k, s3, 1.0, shift, 0.0, nullptr, s);
         ++spmm_calls;

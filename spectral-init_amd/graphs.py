@@ -148,6 +148,23 @@ def n_connected_components(indptr, indices):
     return cg.connected_components(a, directed=False)[0]
 
 
+def band_graph(n, per_row, half_band, seed):
+    """Symmetric random band graph: per_row random neighbours within +-half_band."""
+    import scipy.sparse as sp
+
+    rng = np.random.default_rng(seed)
+    rows = np.repeat(np.arange(n, dtype=np.int64), per_row)
+    off = rng.integers(1, half_band + 1, rows.size) * rng.choice([-1, 1], rows.size)
+    cols = np.clip(rows + off, 0, n - 1)
+    keep = cols != rows
+    w = rng.uniform(0.05, 1.0, keep.sum()).astype(np.float32)
+    a = sp.csr_matrix((w, (rows[keep], cols[keep])), shape=(n, n))
+    a.sum_duplicates()
+    a = a.maximum(a.T).tocsr()
+    a.sort_indices()
+    return a.indptr.astype(np.int64), a.indices.astype(np.int32), a.data.astype(np.float32)
+
+
 # ----------------------------------------------------------------- workloads
 def workload(name, seed=0, device=None, n=None):
     """Seeded inputs of the named workloads.
@@ -171,6 +188,13 @@ def workload(name, seed=0, device=None, n=None):
         sizes = 1.0 / np.arange(1, 41) ** 1.2
         x, lab = gmm_points(n, 30, sizes, seed, center_scale=CENTER_SCALE["gmm1m"],
                             background_frac=0.05)
+    elif name == "band1m":
+        # SpMM locality reference (not a solver workload): symmetric random band
+        # graph, 12 neighbours per row within +-512 rows (~24 stored per row)
+        n = n or 1_000_000
+        indptr, indices, data = band_graph(n, 12, 512, seed)
+        return dict(name=name, n=n, x=None, labels=None, indptr=indptr, indices=indices,
+                    data=data)
     else:
         raise ValueError(name)
     indptr, indices, data = fuzzy_knn_graph(x, 15, device=device)
