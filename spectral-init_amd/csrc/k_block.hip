@@ -17,7 +17,7 @@ namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-constexpr int kGramBlocks = 512;
+constexpr int kGramBlocks = 1024;
 
 // B may be split in two column blocks: columns c < q1 come from b, columns
 // q1 <= c < q from b2 (column c - q1), so [Y^T Y | Y^T LY] is one pass.
@@ -97,25 +97,45 @@ __global__ __launch_bounds__(256) void k_gram(int64_t n, const T* __restrict__ a
     for (int e = threadIdx.x; e < p * q; e += 256) out[e] = red[e / q][e % q];
 }
 
-// Sum the per-workgroup partials: each workgroup owns 32 outputs; 8 groups of
-// 32 threads each sum a fixed stride of the partials, then one group adds the
-// 8 sums in order (fixed order: deterministic).
+// Sum the per-workgroup partials, g[e] = sum_b partial[b][e], in a fixed order
+// (deterministic).  A workgroup owns 8 outputs; 32 groups of 8 lanes stride the
+// partials, each lane keeping 8 independent loads in flight, and the group sums
+// are added by a fixed tree in LDS.
 __global__ __launch_bounds__(256) void k_reduce_partials(const double* __restrict__ partial,
                                                          int nparts, int pq,
                                                          double* __restrict__ g) {
-    __shared__ double red[8][33];
-    const int lane = threadIdx.x & 31, grp = threadIdx.x >> 5;
-    const int e = blockIdx.x * 32 + lane;
-    double s = 0.0;
-    if (e < pq)
-        for (int b = grp; b < nparts; b += 8) s += partial[(int64_t)b * pq + e];
-    red[grp][lane] = s;
-    __syncthreads();
-    if (grp == 0 && e < pq) {
-        double t = red[0][lane];
-        for (int q = 1; q < 8; ++q) t += red[q][lane];
-        g[e] = t;
+    constexpr int E = 8, G = 256 / E, D = 8;
+    __shared__ double red[G][E + 1];
+    const int lane = threadIdx.x % E, grp = threadIdx.x / E;
+    const int e = blockIdx.x * E + lane;
+    double acc[D];
+#pragma unroll
+    for (int u = 0; u < D; ++u) acc[u] = 0.0;
+    if (e < pq) {
+        for (int b = grp; b < nparts; b += D * G) {
+#pragma unroll
+            for (int u = 0; u < D; ++u) {
+                const int bb = b + u * G;
+                if (bb < nparts) acc[u] += partial[(int64_t)bb * pq + e];
+            }
+        }
     }
+    double t = 0.0;
+#pragma unroll
+    for (int u = 0; u < D; ++u) t += acc[u];
+    red[grp][lane] = t;
+    __syncthreads();
+    for (int h = G / 2; h > 0; h >>= 1) {
+        if (grp < h) red[grp][lane] += red[grp + h][lane];
+        __syncthreads();
+    }
+    if (grp == 0 && e < pq) g[e] = red[0][lane];
+}
+
+void reduce_partials(const double* partial, int nparts, int pq, double* g, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(ceil_div(pq, 8)), dim3(256), 0, s, partial, nparts,
+                       pq, g);
+    SI_LAUNCH_CHECK();
 }
 
 template <typename T>
@@ -144,9 +164,7 @@ void gram_impl(int64_t n, const T* a, int lda, int p, const T* b, int ldb, int q
     { throw Error(kErrInternal, "gram: no kernel for this shape"); }
 #undef SI_GRAM_CASE
     SI_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce_partials, dim3(ceil_div(p * q, 32)), dim3(256), 0, s, ws,
-                       (int)nblocks, p * q, g);
-    SI_LAUNCH_CHECK();
+    reduce_partials(ws, (int)nblocks, p * q, g, s);
 }
 
 // ------------------------------------------------------------------ block products
@@ -228,9 +246,12 @@ void block_mul_impl(int64_t n, const T* a, int lda, int p, const double* c, int 
 
 // ------------------------------------------------------------------ fused RR update
 // x = y C, ax = ly C (same C), and for the first k columns the partial sums
-// ||ax_j - theta_j x_j||^2 and ||x_j||^2 of this workgroup's rows
-// (partial[block][j], partial[block][k + j]); k <= CPT so one thread per row
-// owns all wanted columns.
+// ||ax_j - theta_j x_j||^2 and ||x_j||^2 (partial[block][j], partial[block][k + j]).
+// A workgroup walks R-row tiles with a grid stride (at most kRrBlocks
+// workgroups, so few partials); TPR threads per row own CPT contiguous columns
+// each and accumulate the norms of the wanted columns they own.
+constexpr int kRrBlocks = 2048;
+
 template <typename T, int QMAX>
 __global__ __launch_bounds__(256) void k_rr_update(int64_t n, const T* __restrict__ y,
                                                    const T* __restrict__ ly, int ld, int m,
@@ -240,68 +261,77 @@ __global__ __launch_bounds__(256) void k_rr_update(int64_t n, const T* __restric
                                                    double* __restrict__ partial) {
     constexpr int R = sizeof(T) == 8 ? 32 : 64;
     constexpr int TPR = 256 / R;
-    constexpr int CPT = (QMAX + TPR - 1) / TPR;
+    constexpr int CPT = QMAX / TPR;
+    static_assert(CPT * TPR == QMAX, "rr_update: QMAX must be a multiple of TPR");
     constexpr int LDT = 65;
     __shared__ T cm[64 * QMAX];
     __shared__ T tile[R * LDT];
     __shared__ double red[R][2 * 8 + 1];
     const int t = threadIdx.x;
     const int r = t / TPR, g = t % TPR;
-    const int64_t row0 = (int64_t)blockIdx.x * R;
-    const int rows = (n - row0) < R ? (int)(n - row0) : R;
     for (int e = t; e < m * m; e += 256) cm[(e / m) * QMAX + e % m] = (T)c[e];
-    T accx[CPT], acca[CPT];
+    double rr2[CPT], xx2[CPT], th[CPT];
 #pragma unroll
-    for (int u = 0; u < CPT; ++u) { accx[u] = T(0); acca[u] = T(0); }
-    for (int pass = 0; pass < 2; ++pass) {
-        const T* src = pass == 0 ? y : ly;
-        __syncthreads();
-        for (int e = t; e < rows * m; e += 256) {
-            const int rr = e / m, kk = e % m;
-            tile[rr * LDT + kk] = src[(row0 + rr) * ld + kk];
+    for (int u = 0; u < CPT; ++u) {
+        const int j = g * CPT + u;
+        rr2[u] = 0.0;
+        xx2[u] = 0.0;
+        th[u] = j < k ? theta[j] : 0.0;
+    }
+    const int64_t tiles = (n + R - 1) / R;
+    for (int64_t tl = blockIdx.x; tl < tiles; tl += gridDim.x) {
+        const int64_t row0 = tl * R;
+        const int rows = (n - row0) < R ? (int)(n - row0) : R;
+        T accx[CPT], acca[CPT];
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) { accx[u] = T(0); acca[u] = T(0); }
+        for (int pass = 0; pass < 2; ++pass) {
+            const T* src = pass == 0 ? y : ly;
+            __syncthreads();
+            for (int e = t; e < rows * m; e += 256) {
+                const int rr = e / m, kk = e % m;
+                tile[rr * LDT + kk] = src[(row0 + rr) * ld + kk];
+            }
+            __syncthreads();
+            if (r < rows) {
+                for (int kk = 0; kk < m; ++kk) {
+                    const T av = tile[r * LDT + kk];
+#pragma unroll
+                    for (int u = 0; u < CPT; ++u) {
+                        const T cv = cm[kk * QMAX + g * CPT + u];
+                        if (pass == 0) accx[u] += av * cv;
+                        else acca[u] += av * cv;
+                    }
+                }
+            }
         }
-        __syncthreads();
         if (r < rows) {
-            for (int kk = 0; kk < m; ++kk) {
-                const T av = tile[r * LDT + kk];
+            T* xo = x + (row0 + r) * ld;
+            T* ao = ax + (row0 + r) * ld;
 #pragma unroll
-                for (int u = 0; u < CPT; ++u) {
-                    const int j = g * CPT + u;
-                    if (j < QMAX) {
-                        if (pass == 0) accx[u] += av * cm[kk * QMAX + j];
-                        else acca[u] += av * cm[kk * QMAX + j];
+            for (int u = 0; u < CPT; ++u) {
+                const int j = g * CPT + u;
+                if (j < m) {
+                    xo[j] = accx[u];
+                    ao[j] = acca[u];
+                    if (j < k) {
+                        const double xv = (double)accx[u];
+                        const double rv = (double)acca[u] - th[u] * xv;
+                        rr2[u] += rv * rv;
+                        xx2[u] += xv * xv;
                     }
                 }
             }
         }
     }
-    double rr2[8], xx2[8];
+    // every (row slot, wanted column) has exactly one owner thread
 #pragma unroll
-    for (int j = 0; j < 8; ++j) { rr2[j] = 0.0; xx2[j] = 0.0; }
-    if (r < rows) {
-        T* xo = x + (row0 + r) * ld;
-        T* ao = ax + (row0 + r) * ld;
-#pragma unroll
-        for (int u = 0; u < CPT; ++u) {
-            const int j = g * CPT + u;
-            if (j < m) {
-                xo[j] = accx[u];
-                ao[j] = acca[u];
-                if (j < k && j < 8) {
-                    const double xv = (double)accx[u];
-                    const double rv = (double)acca[u] - theta[j] * xv;
-                    rr2[j] = rv * rv;
-                    xx2[j] = xv * xv;
-                }
-            }
-        }
-    }
-    if (g == 0) {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) { red[r][j] = rr2[j]; red[r][8 + j] = xx2[j]; }
+    for (int u = 0; u < CPT; ++u) {
+        const int j = g * CPT + u;
+        if (j < k) { red[r][j] = rr2[u]; red[r][8 + j] = xx2[u]; }
     }
     __syncthreads();
-    if (t < 2 * k && t < 16) {
+    if (t < 2 * k) {
         const int j = t < k ? t : 8 + (t - k);
         double sacc = 0.0;
         for (int q = 0; q < R; ++q) sacc += red[q][j];
@@ -314,18 +344,18 @@ void rr_update_impl(int64_t n, const T* y, const T* ly, int ld, int m, const dou
                     const double* theta, int k, T* x, T* ax, double* norms2, double* ws,
                     hipStream_t s) {
     SI_CHECK(m >= 1 && m <= 64 && k >= 1 && k <= 8 && k <= m, kErrInvalidArgument,
-             "rr_update: need k <= 8 <= m <= 64");
+             "rr_update: need k <= 8, k <= m <= 64");
     constexpr int R = sizeof(T) == 8 ? 32 : 64;
-    const int64_t blocks = ceil_div(n, R);
-    // QMAX = 32 (FP32, m <= 32) or 64 gives CPT = 8 or 16 columns per thread,
-    // so the thread that owns column 0 owns every wanted column (k <= 8)
-    if (m <= 32 && sizeof(T) == 4)
+    int64_t blocks = ceil_div(n, (int64_t)R);
+    if (blocks > kRrBlocks) blocks = kRrBlocks;
+    if (m <= 16)
+        hipLaunchKernelGGL((k_rr_update<T, 16>), dim3(blocks), dim3(256), 0, s, n, y, ly, ld, m, c, theta, k, x, ax, ws);
+    else if (m <= 32)
         hipLaunchKernelGGL((k_rr_update<T, 32>), dim3(blocks), dim3(256), 0, s, n, y, ly, ld, m, c, theta, k, x, ax, ws);
     else
         hipLaunchKernelGGL((k_rr_update<T, 64>), dim3(blocks), dim3(256), 0, s, n, y, ly, ld, m, c, theta, k, x, ax, ws);
     SI_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, ws, (int)blocks, 2 * k, norms2);
-    SI_LAUNCH_CHECK();
+    reduce_partials(ws, (int)blocks, 2 * k, norms2, s);
 }
 
 // ------------------------------------------------------------------ elementwise
@@ -339,16 +369,28 @@ __global__ void k_residual(int64_t n, const T* __restrict__ ax, const T* __restr
     r[i * ldr + c] = ax[i * ld + c] - (T)theta[c] * x[i * ld + c];
 }
 
+// out = alpha * a + beta * b on the first w columns (common leading dimension)
+template <typename T>
+__global__ void k_axpby(int64_t n, int w, int ld, T alpha, const T* __restrict__ a, T beta,
+                        const T* __restrict__ b, T* __restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * w) return;
+    const int64_t o = (e / w) * ld + e % w;
+    out[o] = alpha * a[o] + beta * b[o];
+}
+
 // Row i draws the normals of original node perm[i], so a reordered solve
-// starts from the same block as an unordered one.
-__global__ void k_fill_normal(double* __restrict__ x, int64_t n, int k, int ld, uint64_t seed,
+// starts from the same block as an unordered one.  The draws are FP64; an FP32
+// block holds them rounded.
+template <typename T>
+__global__ void k_fill_normal(T* __restrict__ x, int64_t n, int k, int ld, uint64_t seed,
                               uint32_t stream, const int32_t* __restrict__ perm) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n * k) return;
     const int64_t i = e / k;
     const int c = (int)(e % k);
     const int64_t orig = perm ? (int64_t)perm[i] : i;
-    x[i * ld + c] = normal_at(seed, stream, (uint64_t)(orig * k + c));
+    x[i * ld + c] = (T)normal_at(seed, stream, (uint64_t)(orig * k + c));
 }
 
 template <typename T>
@@ -368,14 +410,14 @@ __global__ void k_convert(const A* __restrict__ a, B* __restrict__ b, int64_t co
     if (e < count) b[e] = (B)a[e];
 }
 
-template <typename T>
-__global__ void k_copy_cols(const T* __restrict__ a, int lda, T* __restrict__ b, int ldb,
+template <typename A, typename B>
+__global__ void k_copy_cols(const A* __restrict__ a, int lda, B* __restrict__ b, int ldb,
                             int64_t n, int k) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n * k) return;
     const int64_t i = e / k;
     const int c = (int)(e % k);
-    b[i * ldb + c] = a[i * lda + c];
+    b[i * ldb + c] = (B)a[i * lda + c];
 }
 
 template <typename T>
@@ -393,10 +435,11 @@ __global__ void k_average(double* __restrict__ x, const double* __restrict__ t, 
     if (e < count) x[e] = (x[e] + t[e]) / 2.0;
 }
 
-__global__ void k_set_column(double* __restrict__ x, int ld, int64_t n, int col,
+template <typename T>
+__global__ void k_set_column(T* __restrict__ x, int ld, int64_t n, int col,
                              const double* __restrict__ v, double scale) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i * ld + col] = v[i] * scale;
+    if (i < n) x[i * ld + col] = (T)(v[i] * scale);
 }
 
 inline dim3 grid1(int64_t count) { return dim3((unsigned)ceil_div(count, 256)); }
@@ -428,7 +471,11 @@ void rr_update_f64(int64_t n, const double* y, const double* ly, int ld, int m, 
                    hipStream_t s) {
     rr_update_impl<double>(n, y, ly, ld, m, c, theta, k, x, ax, norms2, ws, s);
 }
-size_t rr_update_workspace_doubles(int64_t n, int k) { return (size_t)ceil_div(n, 32) * 2 * k; }
+size_t rr_update_workspace_doubles(int64_t n, int k) {
+    int64_t nb = ceil_div(n, (int64_t)32);
+    if (nb > kRrBlocks) nb = kRrBlocks;
+    return (size_t)nb * 2 * k;
+}
 
 void gram2_f32(int64_t n, const float* a, int lda, int p, const float* b, int ldb, int q1,
                const float* b2, int ldb2, int q2, double* g, double* ws, hipStream_t s) {
@@ -474,7 +521,26 @@ void gather_rows_f32(const float* src, int lds, float* dst, int ldd, int64_t n, 
 
 void fill_normal_f64(double* x, int64_t n, int k, int ld, uint64_t seed, uint32_t stream,
                      hipStream_t s, const int32_t* perm) {
-    hipLaunchKernelGGL(k_fill_normal, grid1(n * k), dim3(256), 0, s, x, n, k, ld, seed, stream, perm);
+    hipLaunchKernelGGL((k_fill_normal<double>), grid1(n * k), dim3(256), 0, s, x, n, k, ld, seed,
+                       stream, perm);
+    SI_LAUNCH_CHECK();
+}
+void fill_normal_f32(float* x, int64_t n, int k, int ld, uint64_t seed, uint32_t stream,
+                     hipStream_t s, const int32_t* perm) {
+    hipLaunchKernelGGL((k_fill_normal<float>), grid1(n * k), dim3(256), 0, s, x, n, k, ld, seed,
+                       stream, perm);
+    SI_LAUNCH_CHECK();
+}
+void axpby_cols_f32(int64_t n, int w, int ld, float alpha, const float* a, float beta,
+                    const float* b, float* out, hipStream_t s) {
+    hipLaunchKernelGGL((k_axpby<float>), grid1(n * w), dim3(256), 0, s, n, w, ld, alpha, a, beta,
+                       b, out);
+    SI_LAUNCH_CHECK();
+}
+void axpby_cols_f64(int64_t n, int w, int ld, double alpha, const double* a, double beta,
+                    const double* b, double* out, hipStream_t s) {
+    hipLaunchKernelGGL((k_axpby<double>), grid1(n * w), dim3(256), 0, s, n, w, ld, alpha, a, beta,
+                       b, out);
     SI_LAUNCH_CHECK();
 }
 void convert_f64_to_f32(const double* a, float* b, int64_t count, hipStream_t s) {
@@ -486,11 +552,16 @@ void convert_f32_to_f64(const float* a, double* b, int64_t count, hipStream_t s)
     SI_LAUNCH_CHECK();
 }
 void copy_cols_f64(const double* a, int lda, double* b, int ldb, int64_t n, int k, hipStream_t s) {
-    hipLaunchKernelGGL((k_copy_cols<double>), grid1(n * k), dim3(256), 0, s, a, lda, b, ldb, n, k);
+    hipLaunchKernelGGL((k_copy_cols<double, double>), grid1(n * k), dim3(256), 0, s, a, lda, b, ldb, n, k);
+    SI_LAUNCH_CHECK();
+}
+void copy_cols_f64_to_f32(const double* a, int lda, float* b, int ldb, int64_t n, int k,
+                          hipStream_t s) {
+    hipLaunchKernelGGL((k_copy_cols<double, float>), grid1(n * k), dim3(256), 0, s, a, lda, b, ldb, n, k);
     SI_LAUNCH_CHECK();
 }
 void copy_cols_f32(const float* a, int lda, float* b, int ldb, int64_t n, int k, hipStream_t s) {
-    hipLaunchKernelGGL((k_copy_cols<float>), grid1(n * k), dim3(256), 0, s, a, lda, b, ldb, n, k);
+    hipLaunchKernelGGL((k_copy_cols<float, float>), grid1(n * k), dim3(256), 0, s, a, lda, b, ldb, n, k);
     SI_LAUNCH_CHECK();
 }
 void scale_cols_f32(float* x, int ld, int64_t n, int k, const double* scale, hipStream_t s) {
@@ -507,7 +578,12 @@ void average_f64(double* x, const double* t, int64_t count, hipStream_t s) {
 }
 void set_column_f64(double* x, int ld, int64_t n, int col, const double* v, double scale,
                     hipStream_t s) {
-    hipLaunchKernelGGL(k_set_column, grid1(n), dim3(256), 0, s, x, ld, n, col, v, scale);
+    hipLaunchKernelGGL((k_set_column<double>), grid1(n), dim3(256), 0, s, x, ld, n, col, v, scale);
+    SI_LAUNCH_CHECK();
+}
+void set_column_f32(float* x, int ld, int64_t n, int col, const double* v, double scale,
+                    hipStream_t s) {
+    hipLaunchKernelGGL((k_set_column<float>), grid1(n), dim3(256), 0, s, x, ld, n, col, v, scale);
     SI_LAUNCH_CHECK();
 }
 

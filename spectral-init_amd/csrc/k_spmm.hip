@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void k_spmm_narrow(
 }
 
 // -------------------------------------------------------------- wide
-template <typename T, int K, int SLOTS>
+template <typename T, int K, int SLOTS, int U>
 __global__ __launch_bounds__(256) void k_spmm_wide(
     const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const T* __restrict__ val,
     const T* __restrict__ x, T* __restrict__ out, const T* __restrict__ z, int ld, int64_t n,
@@ -134,7 +134,6 @@ __global__ __launch_bounds__(256) void k_spmm_wide(
     // Memory-level parallelism: each lane first issues U (value, column) loads,
     // then U independent x-row gathers, then the FMAs -- one dependent
     // index->gather round trip per U*SLOTS nonzeros of the row.
-    constexpr int U = 8;
     for (int32_t base = b + slot; base < e; base += SLOTS * U) {
         T vv[U];
         int32_t cc[U];
@@ -195,7 +194,9 @@ __global__ __launch_bounds__(256) void k_spmm_long(
     const T* __restrict__ x, T* __restrict__ out, const T* __restrict__ z, int ld, int k,
     const int32_t* __restrict__ rows, T s_l, T s_x, T s_z) {
     constexpr int CH = 16;                 // columns per pass
+    static_assert(CH * 16 == 256, "long-row reduction assumes 16 columns per pass");
     __shared__ T part[256][CH + 1];
+    __shared__ T segs[16][CH + 1];
     const int64_t i = rows[blockIdx.x];
     const int32_t b = rp[i], e = rp[i + 1];
     const int t = threadIdx.x;
@@ -214,9 +215,17 @@ __global__ __launch_bounds__(256) void k_spmm_long(
 #pragma unroll
         for (int c = 0; c < CH; ++c) part[t][c] = acc[c];
         __syncthreads();
+        // fixed-order two-level sum: 16 segments of 16 threads per column
+        {
+            const int c = t % CH, q = t / CH;
+            T seg = T(0);
+            for (int r = 0; r < 16; ++r) seg += part[q * 16 + r][c];
+            segs[q][c] = seg;
+        }
+        __syncthreads();
         if (t < cw) {
             T sum = T(0);
-            for (int q = 0; q < 256; ++q) sum += part[q][t];
+            for (int q = 0; q < 16; ++q) sum += segs[q][t];
             const int64_t o = i * ld + c0 + t;
             const T xi = x[o];
             T r = s_l * apply_diag<T>(xi, sum) + s_x * xi;
@@ -236,7 +245,7 @@ void launch_narrow(const DevLaplacian& L, const T* val, const T* x, T* out, int 
                        val, x, out, z, ld, L.n, s_l, s_x, s_z);
 }
 
-template <typename T, int K, int SLOTS>
+template <typename T, int K, int SLOTS, int U>
 void launch_wide(const DevLaplacian& L, const T* val, const T* x, T* out, int ld, T s_l, T s_x,
                  T s_z, const T* z, hipStream_t s) {
     constexpr int VEC = Vec4<T>::n;
@@ -244,7 +253,7 @@ void launch_wide(const DevLaplacian& L, const T* val, const T* x, T* out, int ld
     constexpr int RPW = 64 / (C * SLOTS);
     const int64_t waves = ceil_div(L.n, RPW);
     const int64_t blocks = ceil_div(waves, 4);
-    hipLaunchKernelGGL((k_spmm_wide<T, K, SLOTS>), dim3(blocks), dim3(256), 0, s, L.rowptr, L.col,
+    hipLaunchKernelGGL((k_spmm_wide<T, K, SLOTS, U>), dim3(blocks), dim3(256), 0, s, L.rowptr, L.col,
                        val, x, out, z, ld, L.n, s_l, s_x, s_z);
 }
 
@@ -259,17 +268,19 @@ void spmm_chunk(const DevLaplacian& L, const T* val, const T* x, T* out, int k, 
         case 3: launch_narrow<T, 3, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s); break;
         case 4: launch_narrow<T, 4, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s); break;
         case 8: launch_narrow<T, 8, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s); break;
+        // (SLOTS, U): SLOTS*U nonzero positions per row per pass; ~32 fits the
+        // 15-30 entries of k-NN rows with few idle lanes (measured on MI355X)
         case 16:
-            if constexpr (F32) launch_wide<T, 16, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s);   // C=4, RPW=2
-            else launch_wide<T, 16, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s);       // C=8, RPW=1
+            if constexpr (F32) launch_wide<T, 16, 4, 4>(L, val, x, out, ld, s_l, s_x, s_z, z, s);   // C=4, RPW=4
+            else launch_wide<T, 16, 4, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s);       // C=8, RPW=2
             break;
         case 32:
-            if constexpr (F32) launch_wide<T, 32, 4>(L, val, x, out, ld, s_l, s_x, s_z, z, s);   // C=8, RPW=2
-            else launch_wide<T, 32, 4>(L, val, x, out, ld, s_l, s_x, s_z, z, s);       // C=16, RPW=1
+            if constexpr (F32) launch_wide<T, 32, 4, 4>(L, val, x, out, ld, s_l, s_x, s_z, z, s);   // C=8, RPW=2
+            else launch_wide<T, 32, 4, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s);       // C=16, RPW=1
             break;
         case 64:
-            if constexpr (F32) launch_wide<T, 64, 4>(L, val, x, out, ld, s_l, s_x, s_z, z, s);   // C=16, RPW=1
-            else launch_wide<T, 64, 2>(L, val, x, out, ld, s_l, s_x, s_z, z, s);       // C=32, RPW=1
+            if constexpr (F32) launch_wide<T, 64, 4, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s);   // C=16, RPW=1
+            else launch_wide<T, 64, 2, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s);       // C=32, RPW=1
             break;
         default:
             throw Error(kErrInternal, "spmm: no kernel for width " + std::to_string(k));

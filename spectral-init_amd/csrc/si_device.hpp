@@ -129,16 +129,28 @@ void residual_f64(int64_t n, const double* ax, const double* x, int ld, int k,
 
 void fill_normal_f64(double* x, int64_t n, int k, int ld, uint64_t seed, uint32_t stream,
                      hipStream_t s, const int32_t* perm = nullptr);
+// the same FP64 draws, rounded to FP32
+void fill_normal_f32(float* x, int64_t n, int k, int ld, uint64_t seed, uint32_t stream,
+                     hipStream_t s, const int32_t* perm = nullptr);
+// out = alpha a + beta b on columns [0,w) (all three with leading dimension ld)
+void axpby_cols_f32(int64_t n, int w, int ld, float alpha, const float* a, float beta,
+                    const float* b, float* out, hipStream_t s);
+void axpby_cols_f64(int64_t n, int w, int ld, double alpha, const double* a, double beta,
+                    const double* b, double* out, hipStream_t s);
 void convert_f64_to_f32(const double* a, float* b, int64_t count, hipStream_t s);
 void convert_f32_to_f64(const float* a, double* b, int64_t count, hipStream_t s);
 // copy columns [0,k) of a [n x lda] block into b [n x ldb]
 void copy_cols_f64(const double* a, int lda, double* b, int ldb, int64_t n, int k, hipStream_t s);
 void copy_cols_f32(const float* a, int lda, float* b, int ldb, int64_t n, int k, hipStream_t s);
+void copy_cols_f64_to_f32(const double* a, int lda, float* b, int ldb, int64_t n, int k,
+                          hipStream_t s);
 void scale_cols_f32(float* x, int ld, int64_t n, int k, const double* scale, hipStream_t s);
 void scale_cols_f64(double* x, int ld, int64_t n, int k, const double* scale, hipStream_t s);
 // x[:,c] = (x[:,c] + t[:,c]) / 2
 void average_f64(double* x, const double* t, int64_t count, hipStream_t s);
 void set_column_f64(double* x, int ld, int64_t n, int col, const double* v, double scale,
+                    hipStream_t s);
+void set_column_f32(float* x, int ld, int64_t n, int col, const double* v, double scale,
                     hipStream_t s);
 
 // ----------------------------------------------------------------- small dense (one workgroup)

@@ -22,12 +22,23 @@
 // Convergence is judged on the wanted k = n_components + 1 columns only.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 
 #include "si_host.hpp"
 
 namespace si {
 
 namespace {
+
+// SI_TRACE=1 prints one line per filter round to stderr (diagnostics only).
+bool trace_on() {
+    static const bool on = [] {
+        const char* v = std::getenv("SI_TRACE");
+        return v && v[0] == '1';
+    }();
+    return on;
+}
 
 int choose_block(int64_t n, int k, int requested) {
     int m = requested;
@@ -51,19 +62,28 @@ struct FilterBufs {
 // Y = T_d((L - cI)/e) X on the first w columns (leading dimension ld), in
 // precision T, returned in one of the buffers.  The recurrence runs in the
 // SpMM epilogue: y1 = (L x - c x)/e, y_{j+1} = (2/e)(L y_j - c y_j) - y_{j-1}.
+// When lx = L x is already known (the Rayleigh-Ritz update produces it for the
+// Ritz vectors), the first step is an elementwise update instead of an SpMM.
 template <typename T>
 T* cheb_filter(const DevLaplacian& L, FilterBufs<T>& fb, const T* x, int w, int ld, double a,
-               double b, int degree, int64_t& spmm_calls, hipStream_t s) {
+               double b, int degree, int64_t& spmm_calls, hipStream_t s, const T* lx = nullptr) {
     const double c = (a + b) / 2.0, e = (b - a) / 2.0;
     const T scale = (T)(2.0 / e), shift = (T)(2.0 * c / e);
     T* yprev = fb.b0;
     T* y = fb.b1;
     T* ynew = fb.b2;
-    if constexpr (sizeof(T) == 4)
-        spmm_f32(L, x, y, w, ld, (T)(1.0 / e), (T)(-c / e), T(0), nullptr, s);
-    else
-        spmm_f64(L, x, y, w, ld, (T)(1.0 / e), (T)(-c / e), T(0), nullptr, s);
-    ++spmm_calls;
+    if (lx) {
+        if constexpr (sizeof(T) == 4)
+            axpby_cols_f32(L.n, w, ld, (T)(1.0 / e), lx, (T)(-c / e), x, y, s);
+        else
+            axpby_cols_f64(L.n, w, ld, (T)(1.0 / e), lx, (T)(-c / e), x, y, s);
+    } else {
+        if constexpr (sizeof(T) == 4)
+            spmm_f32(L, x, y, w, ld, (T)(1.0 / e), (T)(-c / e), T(0), nullptr, s);
+        else
+            spmm_f64(L, x, y, w, ld, (T)(1.0 / e), (T)(-c / e), T(0), nullptr, s);
+        ++spmm_calls;
+    }
     const T* prev = x;
     for (int d = 2; d <= degree; ++d) {
         if constexpr (sizeof(T) == 4)
@@ -246,22 +266,19 @@ bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const Fa
     const size_t nm = (size_t)n * m;
     double* x64 = ws_get<double>(kWsFastX64, nm);
     double* ws = ws_get<double>(kWsGram, gram_workspace_doubles(n, m, m));
-    double* g = ws_get<double>(kWsSmall7, 1);
     Small sm(m);
 
-    // start block: warm start (first k columns) or N(0,1) with the trivial hint
-    fill_normal_f64(x64, n, m, m, seed, kStreamLobpcgInit, s, L.perm);
-    if (opt.warm_start) {
-        copy_cols_f64(opt.warm_start, k, x64, m, n, k, s);
-    } else {
-        gram_f64(n, L.sqrt_deg, 1, 1, L.sqrt_deg, 1, 1, g, ws, s);
-        double sn2 = 0.0;
-        to_host(&sn2, g, 1, s);
-        if (sn2 > 0.0) set_column_f64(x64, m, n, 0, L.sqrt_deg, 1.0 / std::sqrt(sn2), s);
-    }
+    // start block: warm start (first k columns) or N(0,1) with the trivial
+    // hint D^{1/2} 1 in column 0 (its scale is irrelevant: every Rayleigh-Ritz
+    // below is invariant to column scaling)
+    float* x32 = ws_get<float>(kWsFastX32, nm);
+    fill_normal_f32(x32, n, m, m, seed, kStreamLobpcgInit, s, L.perm);
+    if (opt.warm_start)
+        copy_cols_f64_to_f32(opt.warm_start, k, x32, m, n, k, s);
+    else
+        set_column_f32(x32, m, n, 0, L.sqrt_deg, 1.0, s);
 
     // ---------------- FP32 rounds
-    float* x32 = ws_get<float>(kWsFastX32, nm);
     float* ax32 = ws_get<float>(kWsFastAx32, nm);
     float* r32 = ws_get<float>(kWsFastR32, (size_t)n * k);
     double* rr_ws = ws_get<double>(kWsSmall5, rr_update_workspace_doubles(n, k));
@@ -270,7 +287,6 @@ bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const Fa
     fb32.b0 = ws_get<float>(kWsFastF32a, nm);
     fb32.b1 = ws_get<float>(kWsFastF32b, nm);
     fb32.b2 = ws_get<float>(kWsFastF32c, nm);
-    convert_f64_to_f32(x64, x32, (int64_t)nm, s);
     std::vector<double> theta(m), res(k);
     double a = 0.1;               // first round: the reference's ChFSI interval
     const double b = 2.0;          // spectral radius bound of the normalised Laplacian
@@ -288,7 +304,8 @@ bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const Fa
         return std::max(2, std::min(degree, dmax));
     };
     for (; round < opt.max_outer; ++round) {
-        float* y = cheb_filter<float>(L, fb32, x32, m, m, a, b, degree_for(a), st.spmm_calls, s);
+        float* y = cheb_filter<float>(L, fb32, x32, m, m, a, b, degree_for(a), st.spmm_calls, s,
+                                      round > 0 ? ax32 : nullptr);
         float* ly = (y == fb32.b0) ? fb32.b1 : fb32.b0;
         // one fused Rayleigh-Ritz on the filtered block; the explicit
         // orthonormalisation path is the fallback when Y^T Y is too ill-conditioned
@@ -305,6 +322,10 @@ bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const Fa
         }
         ++st.iterations;
         const double worst = *std::max_element(res.begin(), res.end());
+        if (trace_on())
+            std::fprintf(stderr, "[fast] fp32 round %d m=%d a=%.4g deg=%d worst=%.3e theta_k=%.8g theta_m=%.6g spmm=%lld\n",
+                         round, m, a, degree_for(a), worst, theta[k - 1], theta[m - 1],
+                         (long long)st.spmm_calls);
         if (worst <= opt.tol) { fp32_ok = true; break; }
         if (worst < 0.7 * best) { best = worst; stall = 0; }
         else if (++stall >= 3 && worst < 1e-4) break;  // FP32 floor reached
@@ -330,15 +351,22 @@ bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const Fa
                           theta.data(), res.data(), st.spmm_calls, s))
         return false;
     double worst = *std::max_element(res.begin(), res.end());
+    if (trace_on())
+        std::fprintf(stderr, "[fast] fp64 rr mf=%d worst=%.3e spmm=%lld\n", mf, worst,
+                     (long long)st.spmm_calls);
     for (int extra = 0; worst > opt.tol && extra < 20; ++extra) {
         a = std::min(std::max(theta[mf - 1], theta[k - 1] * 1.5 + 1e-12), 0.6);
-        double* y = cheb_filter<double>(L, fb64, x64, mf, m, a, b, degree_for(a), st.spmm_calls, s);
+        double* y = cheb_filter<double>(L, fb64, x64, mf, m, a, b, degree_for(a), st.spmm_calls, s,
+                                        ax64);
         double* ly = (y == fb64.b0) ? fb64.b1 : fb64.b0;
         if (!fused_rr<double>(L, n, y, ly, mf, m, k, x64, ax64, sm, g2w, rr_ws, theta.data(),
                               res.data(), st.spmm_calls, s))
             break;
         ++st.iterations;
         worst = *std::max_element(res.begin(), res.end());
+        if (trace_on())
+            std::fprintf(stderr, "[fast] fp64 round %d mf=%d worst=%.3e spmm=%lld\n", extra, mf,
+                         worst, (long long)st.spmm_calls);
     }
     copy_cols_f64(x64, m, evecs_dev, k, n, k, s);
     SI_HIP(hipStreamSynchronize(s));
