@@ -76,6 +76,26 @@ def cpu_baseline(lap, si, n_components, plain_iters, budget_s):
                 projected_ms_to_eigvecs=round(ms_it * plain_iters, 1) if plain_iters else None)
 
 
+def read_traffic(path, workload, block):
+    """HBM bytes per launch of the dominant SpMM, from a committed rocprofv3
+    --pmc summary (lines "workload: ...", "block: ...", "hbm_bytes_per_launch: ...").
+    None when the summary is absent or measured another workload or width."""
+    if not path or not os.path.exists(path):
+        return None
+    kv = {}
+    with open(path) as fh:
+        for line in fh:
+            if ":" in line and not line.startswith(("#", " ", "|")):
+                key, _, val = line.partition(":")
+                kv[key.strip()] = val.strip()
+    if kv.get("workload") != workload or kv.get("block") != str(block):
+        return None
+    try:
+        return int(float(kv["hbm_bytes_per_launch"]))
+    except (KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,7 +107,7 @@ def main():
     ap.add_argument("--n-components", type=int, default=2)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--block", type=int, default=0)
-    ap.add_argument("--degree", type=int, default=8)
+    ap.add_argument("--degree", type=int, default=0, help="max filter degree per round, 0 = automatic")
     ap.add_argument("--plain-runs", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -97,8 +117,14 @@ def main():
                     help="also time the SpMM alone for widths 1..64 (FP32 and FP64)")
     ap.add_argument("--no-reorder", action="store_true",
                     help="keep the input node order (no locality reordering)")
-    ap.add_argument("--traffic-json", default=None,
-                    help="JSON with measured HBM bytes per SpMM launch (rocprofv3 --pmc)")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "spmm_wide_f32_gmm1m_traffic.md"),
+                    help="rocprofv3 --pmc summary holding the measured HBM bytes per SpMM launch")
+    ap.add_argument("--rsvd-runs", type=int, default=1,
+                    help="randomized SVD runs beside the solver (oversampling 10, 4 power iterations)")
+    ap.add_argument("--graph-cache", default=None,
+                    help="load the workload graph from this .npz (written on first use)")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="one solve plus a 1 GiB device copy (scale reference for HBM counters), then exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -116,10 +142,18 @@ def main():
         sys.exit(1)
     si.lib()  # fails loudly if the native library is missing
 
-    log(f"building workload {args.workload} (n={args.n or 'default'}) ...")
     t0 = time.perf_counter()
-    w = graphs.workload(args.workload, seed=args.seed, n=args.n, device="cuda")
-    ip, ix, wt, n_gen, ncomp = largest_component(w["indptr"], w["indices"], w["data"])
+    if args.graph_cache and os.path.exists(args.graph_cache):
+        log(f"loading workload graph from {args.graph_cache}")
+        z = np.load(args.graph_cache)
+        ip, ix, wt = z["indptr"], z["indices"], z["data"]
+        n_gen, ncomp = int(z["n_gen"]), int(z["ncomp"])
+    else:
+        log(f"building workload {args.workload} (n={args.n or 'default'}) ...")
+        w = graphs.workload(args.workload, seed=args.seed, n=args.n, device="cuda")
+        ip, ix, wt, n_gen, ncomp = largest_component(w["indptr"], w["indices"], w["data"])
+        if args.graph_cache:
+            np.savez(args.graph_cache, indptr=ip, indices=ix, data=wt, n_gen=n_gen, ncomp=ncomp)
     n = ip.size - 1
     nnz = int(ip[-1])
     log(f"graph: n={n} (generated {n_gen}, components {ncomp}) nnz={nnz} "
@@ -162,6 +196,18 @@ def main():
     cfg = si.SpectralInitConfig(tol=args.tol, block_size=args.block, cheb_degree=args.degree,
                                 solver=si.Solver.Fast)
     k = args.n_components + 1
+    if args.pmc_probe:
+        ev, vec, rep = si.solve_eigenproblem(lap, args.n_components, 42, cfg)
+        torch.cuda.synchronize()
+        a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc_probe": args.workload, "n": n, "nnz": nnz, "spmm_calls": rep.spmm_calls,
+                          "block": rep.block_size, "copy_bytes_read": a.numel() * 4,
+                          "copy_bytes_written": b.numel() * 4}), flush=True)
+        return
     for i in range(args.warmup):
         ev, vec, rep = si.solve_eigenproblem(lap, args.n_components, 42, cfg)
         log(f"warmup {i}: {rep.solve_ms:.2f} ms level={rep.level} res={rep.max_residual:.2e} "
@@ -216,16 +262,25 @@ def main():
             f"res={plain[-1]['max_residual']}")
     plain_ms = float(np.median([p["ms"] for p in plain])) if plain else None
 
+    rsvd = []
+    for i in range(args.rsvd_runs):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        rev, rvec, rrep = si.rsvd_solve(lap, args.n_components, 42, oversampling=10, nbiter=4)
+        torch.cuda.synchronize()
+        dt = 1e3 * (time.perf_counter() - t)
+        rres = si.max_eigenpair_residual(lap, rev, rvec)
+        rsvd.append(dict(ms=round(dt, 2), max_residual=rres, eigenvalues=[float(v) for v in rev]))
+        log(f"rSVD run {i}: {dt:.1f} ms residual={rres:.3e}")
+        del rvec
+
     cpu = None
     if not args.no_cpu:
         log("CPU oracle sample ...")
         cpu = cpu_baseline(lap, si, args.n_components, plain_iters, args.cpu_seconds)
         log(f"CPU oracle: {cpu['value']} {cpu['unit']}")
 
-    traffic = None
-    if args.traffic_json and os.path.exists(args.traffic_json):
-        with open(args.traffic_json) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    traffic = read_traffic(args.traffic, args.workload, last.block_size)
 
     out = {
         "metric": "ms to eigvecs @ residual<=%g (FP32-bulk Chebyshev block solver)" % args.tol,
@@ -241,9 +296,10 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic (seeded Gaussian-mixture exact 15-NN fuzzy graph)",
         "config": {"workload": args.workload, "n": n, "nnz": nnz, "n_components": args.n_components,
-                   "block": last.block_size, "cheb_degree": args.degree, "tol": args.tol,
+                   "block": last.block_size, "cheb_degree": args.degree or "auto", "tol": args.tol,
                    "components_generated": ncomp},
         "plain_gpu_lobpcg": {"ms": plain_ms, "runs": plain},
+        "rsvd_oversampling10_4iter": rsvd or None,
         "speedup_vs_plain_gpu_lobpcg": round(plain_ms / fast_ms, 2) if plain_ms else None,
         "fast_solver": {"rounds": last.iterations, "spmm_calls": last.spmm_calls,
                         "max_residual": last.max_residual, "eigenvalues": last.eigenvalues[:k],

@@ -58,7 +58,8 @@ typedef struct {
     int32_t compute_mode;      /* si_compute_mode */
     int32_t solver;            /* si_solver_kind */
     int32_t block_size;        /* fast solver block width: 0 = automatic, else 8/16/32/64 */
-    int32_t cheb_degree;       /* Chebyshev filter degree (reference: 8, lobpcg.rs:72) */
+    int32_t cheb_degree;       /* fast solver: max Chebyshev degree per round, 0 = automatic (<= 24);
+                                  the plain LOBPCG keeps the reference prefilter degree 8 (lobpcg.rs:72) */
     int32_t max_outer;         /* fast solver: cap on filter/Rayleigh-Ritz rounds */
     double tol;                /* residual tolerance (reference LOBPCG_ACCEPT_TOL 1e-5) */
     int64_t dense_threshold;   /* level-0 dense EVD below this n (reference 2000, solvers/mod.rs:32) */
@@ -115,8 +116,9 @@ si_status si_laplacian_export(si_context* ctx, const si_laplacian* lap, double* 
                               double* l_data, int64_t* nnz_l);
 
 /* LinearOperator::apply / CsrOperator::apply (src/operator.rs:6-12, 176-228):
- * y = L x for device blocks of width k (1, 2, 3, 4, 8, 16, 32, 64) and leading
- * dimension ld >= k.  SI_F32 uses FP32 values (FP32 accumulation); SI_F64 the
+ * y = L x for device blocks of width 1 <= k <= 4096 and leading dimension
+ * ld >= k (widths 16/32/64 take the vector path when ld and the pointers are
+ * 16-byte aligned; other widths are split into supported column chunks).  SI_F32 uses FP32 values (FP32 accumulation); SI_F64 the
  * exact FP64 Laplacian. */
 si_status si_spmm(si_context* ctx, const si_laplacian* lap, int32_t precision, const void* x,
                   void* y, int32_t k, int32_t ld);

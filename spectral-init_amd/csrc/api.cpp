@@ -49,7 +49,7 @@ si::ChainOptions chain_options(const si_config* cfg) {
     o.tol = cfg->tol > 0 ? cfg->tol : 1e-5;
     o.dense_threshold = cfg->dense_threshold >= 0 ? cfg->dense_threshold : 2000;
     o.fast.block_size = cfg->block_size;
-    o.fast.cheb_degree = cfg->cheb_degree > 0 ? cfg->cheb_degree : 8;
+    o.fast.cheb_degree = cfg->cheb_degree > 0 ? cfg->cheb_degree : 0;
     o.fast.max_outer = cfg->max_outer > 0 ? cfg->max_outer : 60;
     o.fast.warm_start = cfg->warm_start;
     o.fast.tol = o.tol;
@@ -177,7 +177,7 @@ void si_config_default(si_config* cfg) {
     cfg->compute_mode = SI_COMPUTE_PYTHON_COMPAT;
     cfg->solver = SI_SOLVER_FAST;
     cfg->block_size = 0;
-    cfg->cheb_degree = 8;
+    cfg->cheb_degree = 0;
     cfg->max_outer = 60;
     cfg->tol = 1e-5;
     cfg->dense_threshold = 2000;

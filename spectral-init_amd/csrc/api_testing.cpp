@@ -107,7 +107,7 @@ si_status si_test_fast_solve(si_context* ctx, const si_laplacian* lap, int32_t n
         if (!cfg) cfg = &d;
         si::FastOptions o;
         o.block_size = cfg->block_size;
-        o.cheb_degree = cfg->cheb_degree > 0 ? cfg->cheb_degree : 8;
+        o.cheb_degree = cfg->cheb_degree > 0 ? cfg->cheb_degree : 0;
         o.max_outer = cfg->max_outer > 0 ? cfg->max_outer : 60;
         o.tol = cfg->tol > 0 ? cfg->tol : 1e-6;
         o.warm_start = cfg->warm_start;

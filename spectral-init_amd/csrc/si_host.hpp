@@ -113,7 +113,7 @@ struct SolveStats {
 
 struct FastOptions {
     int block_size = 0;     // 0 = automatic
-    int cheb_degree = 8;
+    int cheb_degree = 0;    // 0 = automatic: amplification-capped, at most 24
     int max_outer = 60;
     double tol = 1e-6;
     const double* warm_start = nullptr;  // device [n x k]

@@ -78,7 +78,7 @@ class SpectralInitConfig:
     compute_mode: ComputeMode = ComputeMode.PythonCompat
     solver: Solver = Solver.Fast
     block_size: int = 0
-    cheb_degree: int = 8
+    cheb_degree: int = 0
     max_outer: int = 60
     tol: float = 1e-5
     dense_threshold: int = 2000

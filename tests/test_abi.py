@@ -43,10 +43,10 @@ def test_version_and_default_config_without_gpu(library):
     assert b"gfx950" in lib.si_version()
     c = si._Config()
     lib.si_config_default(ctypes.byref(c))
-    # the reference's defaults: PythonCompat, tol 1e-5, dense threshold 2000, degree 8
+    # the reference defaults (PythonCompat, tol 1e-5, dense threshold 2000); degree 0 = automatic
     assert c.struct_size == ctypes.sizeof(si._Config)
     assert c.compute_mode == 0 and c.tol == 1e-5 and c.dense_threshold == 2000
-    assert c.cheb_degree == 8
+    assert c.cheb_degree == 0
 
 
 def test_context_create_fails_cleanly_without_a_device():
