@@ -64,6 +64,9 @@ typedef struct {
     double tol;                /* residual tolerance (reference LOBPCG_ACCEPT_TOL 1e-5) */
     int64_t dense_threshold;   /* level-0 dense EVD below this n (reference 2000, solvers/mod.rs:32) */
     const double* warm_start;  /* optional device block [n x (n_components+1)] to start from */
+    int32_t batch_components;  /* disconnected graphs: 1 = all components in one batched device
+                                  solve (default), 0 = one solve per component as the reference
+                                  loops (src/multi_component.rs:261-312) */
 } si_config;
 
 /* Per-call report (timing and quality, the reference's [timing:level_N]

@@ -37,6 +37,12 @@ si_status si_test_fast_solve(si_context* ctx, const si_laplacian* lap, int32_t n
                              uint64_t seed, const si_config* cfg, double* evals, double* evecs,
                              si_report* rep);
 
+/* Batched component solves (si_spectral_init with batch_components): enable
+ * >= 0 switches hipGraph capture of the FP32 rounds on (1) or off (0), -1
+ * leaves it; captured / launches (may be NULL) receive the graph counts of
+ * the last batched solve. */
+si_status si_test_batch_graphs(int32_t enable, int64_t* captured, int64_t* launches);
+
 /* Live timing of every SpMM launch on the context's stream (hipEvents around
  * each launch) between begin and end; bytes are the algorithmic bytes of
  * each launch (nnz*(value+index) + row pointers + x read + y write). */

@@ -2,6 +2,7 @@
 // plus the spectral_init pipeline around them (src/lib.rs:198-258).
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <memory>
 #include <numeric>
 
@@ -164,6 +165,146 @@ int solve_graph(si::Context& c, int64_t n, const int64_t* ip, const int32_t* ix,
     return level;
 }
 
+// Solve a Laplacian given by FP64 off-diagonal values and degrees (the
+// centroid graph of the spectral meta-embedding); host eigenpairs out.
+void solve_values(si::Context& c, int64_t n, const int64_t* ip, const int32_t* ix,
+                  const double* lval, const double* deg, int n_components, uint64_t seed,
+                  const si::ChainOptions& opt, std::vector<double>& ev, std::vector<double>& vec,
+                  si::SolveStats& st) {
+    hipStream_t s = c.stream;
+    si::DevLaplacian L;
+    si::laplacian_from_values_device(L, n, ip, ix, lval, deg, s);
+    const int k = n_components + 1;
+    ev.assign(k, 0.0);
+    try {
+        si::DevBuf<double> dv((size_t)n * k);
+        si::solve_eigenproblem_device(L, n_components, seed, opt, ev.data(), dv, st, s);
+        vec.resize((size_t)n * k);
+        si::to_host(vec.data(), dv.get(), vec.size(), s);
+    } catch (...) {
+        si::laplacian_free(L);
+        throw;
+    }
+    si::laplacian_free(L);
+}
+
+}  // namespace
+
+namespace si {
+// batched-component solves: capture FP32 rounds as hipGraphs (a test seam can
+// turn it off) and the statistics of the last batched solve
+bool g_batch_graphs = true;
+BatchStats g_last_batch;
+}  // namespace si
+
+namespace {
+
+// All components of at least 2d nodes solved in one device pass: one Laplacian
+// of the whole (block-diagonal) graph in locality order, where every component
+// is a contiguous row range; components of <= 64 nodes by a batched dense EVD
+// (one workgroup each, the reference's level 0), the others by the batched
+// FP32-bulk solver.  A component the batch leaves above tol stays unsolved here
+// and goes through the per-component escalation chain.
+void solve_components_batched(si::Context& cx, int64_t n, const int64_t* ip, const int32_t* ix,
+                              const float* w, const std::vector<int64_t>& labels,
+                              const std::vector<std::vector<int64_t>>& members, int d,
+                              uint64_t seed, int mode, const si::ChainOptions& opt,
+                              std::vector<std::vector<double>>& cev,
+                              std::vector<std::vector<double>>& cvec, std::vector<int>& csolved,
+                              si::SolveStats& last, double* lap_ms, double* solve_ms) {
+    hipStream_t s = cx.stream;
+    const int k = d + 1;
+    const int64_t nc = (int64_t)members.size();
+    DeviceGraph g;
+    upload_graph(g, n, ip, ix, w, s);
+    si::DevLaplacian L;
+    {
+        si::EventTimer t(s);
+        si::laplacian_build_device(L, n, g.ip, g.ix, g.w, ip[n], mode, s, /*reorder=*/true, ip, ix, w);
+        *lap_ms += t.stop_ms();
+    }
+    try {
+        std::vector<int32_t> perm(n), iperm(n);
+        if (L.perm) si::to_host(perm.data(), L.perm, n, s);
+        else std::iota(perm.begin(), perm.end(), 0);
+        for (int64_t r = 0; r < n; ++r) iperm[perm[r]] = (int32_t)r;
+        // row range of every component (the locality order keeps them contiguous)
+        std::vector<int64_t> first(nc, -1), last_row(nc, -1);
+        for (int64_t r = 0; r < n; ++r) {
+            const int64_t c = labels[perm[r]];
+            if (first[c] < 0) first[c] = r;
+            last_row[c] = r;
+        }
+        for (int64_t c = 0; c < nc; ++c)
+            SI_CHECK(last_row[c] - first[c] + 1 == (int64_t)members[c].size(), si::kErrInternal,
+                     "batched components: a component is not a contiguous row range");
+        std::vector<int64_t> dcomp, drow0, icomp, irow0, in;
+        std::vector<int32_t> dn;
+        for (int64_t c = 0; c < nc; ++c) {
+            const int64_t sz = (int64_t)members[c].size();
+            if (sz < 2 * d) continue;
+            if (sz <= si::kSmallMax) {
+                dcomp.push_back(c); drow0.push_back(first[c]); dn.push_back((int32_t)sz);
+            } else {
+                icomp.push_back(c); irow0.push_back(first[c]); in.push_back(sz);
+            }
+        }
+        si::DevBuf<double> block((size_t)n * k);
+        std::vector<double> dev_h(dcomp.size() * k), iev(icomp.size() * k), resid;
+        std::vector<int> ok;
+        si::BatchStats bst;
+        {
+            si::EventTimer t(s);
+            // the iterative solve writes every row of the block, so it goes first
+            if (!icomp.empty()) {
+                si::FastOptions fo = opt.fast;
+                fo.tol = opt.tol;
+                si::fast_solve_segments(L, irow0, in, d, seed, fo, si::g_batch_graphs, iev.data(),
+                                        block, resid, ok, bst, s);
+                si::g_last_batch = bst;
+            }
+            if (!dcomp.empty()) {
+                si::DevBuf<int64_t> r0(drow0.size());
+                si::DevBuf<int32_t> nn(dn.size());
+                si::DevBuf<double> ev(dcomp.size() * k);
+                si::to_device(r0.get(), drow0.data(), drow0.size(), s);
+                si::to_device(nn.get(), dn.data(), dn.size(), s);
+                si::dense_seg_eig(L, (int)dcomp.size(), r0, nn, k, ev, block, k, s);
+                si::to_host(dev_h.data(), ev.get(), dev_h.size(), s);
+            }
+            *solve_ms += t.stop_ms();
+        }
+        std::vector<double> hb((size_t)n * k);
+        si::to_host(hb.data(), block.get(), hb.size(), s);
+        auto take = [&](int64_t c, const double* evc) {
+            const std::vector<int64_t>& mem = members[c];
+            cev[c].assign(evc, evc + k);
+            cvec[c].resize(mem.size() * k);
+            for (size_t q = 0; q < mem.size(); ++q) {
+                const int64_t row = iperm[mem[q]];
+                for (int j = 0; j < k; ++j) cvec[c][q * k + j] = hb[(size_t)row * k + j];
+            }
+            csolved[c] = 1;
+        };
+        for (size_t i = 0; i < dcomp.size(); ++i) {
+            take(dcomp[i], dev_h.data() + i * k);
+            last.level = std::max(last.level, 0);
+        }
+        for (size_t i = 0; i < icomp.size(); ++i) {
+            if (!ok[i]) continue;
+            take(icomp[i], iev.data() + i * k);
+            last.level = 1;
+            last.max_residual = std::max(last.max_residual, resid[i]);
+        }
+        last.iterations += bst.rounds;
+        last.spmm_calls += bst.spmm_calls;
+    } catch (...) {
+        si::laplacian_free(L);
+        throw;
+    }
+    si::laplacian_free(L);
+}
+
 }  // namespace
 
 extern "C" {
@@ -182,6 +323,7 @@ void si_config_default(si_config* cfg) {
     cfg->tol = 1e-5;
     cfg->dense_threshold = 2000;
     cfg->warm_start = nullptr;
+    cfg->batch_components = 1;
 }
 
 si_status si_context_create(int device, si_context** out) {
@@ -520,6 +662,10 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
         const int mode = cfg ? cfg->compute_mode : 0;
         si::ChainOptions opt = chain_options(cfg);
         opt.fast.warm_start = nullptr;  // warm starts are per-graph device blocks
+        // batch_components is read only from callers whose struct has the field
+        const bool batch = !cfg || cfg->struct_size < offsetof(si_config, batch_components) +
+                                                         sizeof(cfg->batch_components) ||
+                           cfg->batch_components != 0;
         const int d = n_components;
         const int k = d + 1;
         std::vector<int64_t> labels;
@@ -551,10 +697,7 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
                     const double inv = 1.0 / (double)members[c].size();
                     for (int64_t f = 0; f < n_features; ++f) cen[c * n_features + f] *= inv;
                 }
-                // the centroid affinity graph as a dense f32 CSR input, zero diagonal
-                std::vector<int64_t> mip(nc + 1);
-                std::vector<int32_t> mix;
-                std::vector<float> mw;
+                // Gaussian affinity exp(-||c_i - c_j||^2) in FP64
                 std::vector<double> aff((size_t)nc * nc);
                 for (int64_t i = 0; i < nc; ++i)
                     for (int64_t j = 0; j < nc; ++j) {
@@ -565,19 +708,34 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
                         }
                         aff[i * nc + j] = std::exp(-d2);
                     }
-                // The reference builds this Laplacian in f64 directly; the device
-                // operator takes f32 weights, so the affinities are rounded to f32
-                // here (relative 6e-8) before the normalisation.
+                // normalized Laplacian of the centroid affinity graph, built in FP64
+                // as the reference does (degrees over off-diagonal entries,
+                // L_ij = -isd_i * aff_ij * isd_j): the affinities are often far below
+                // the FP32 range, the normalised entries are O(1)
+                std::vector<double> mdeg(nc, 0.0), misd(nc, 0.0);
+                for (int64_t i = 0; i < nc; ++i) {
+                    double sum = 0.0;
+                    for (int64_t j = 0; j < nc; ++j)
+                        if (j != i) sum += aff[i * nc + j];
+                    mdeg[i] = sum;
+                    misd[i] = sum > 0.0 ? 1.0 / std::sqrt(sum) : 0.0;
+                }
+                std::vector<int64_t> mip(nc + 1);
+                std::vector<int32_t> mix;
+                std::vector<double> mval;
                 mip[0] = 0;
                 for (int64_t i = 0; i < nc; ++i) {
                     for (int64_t j = 0; j < nc; ++j)
-                        if (j != i) { mix.push_back((int32_t)j); mw.push_back((float)aff[i * nc + j]); }
+                        if (j != i) {
+                            mix.push_back((int32_t)j);
+                            mval.push_back(-misd[i] * aff[i * nc + j] * misd[j]);
+                        }
                     mip[i + 1] = (int64_t)mix.size();
                 }
                 si::ChainOptions mo = opt;
                 si::SolveStats mst;
-                solve_graph(ctx->c, nc, mip.data(), mix.data(), mw.data(), d, seed, 1, mo, ev, vec,
-                            mst, nullptr, nullptr);
+                solve_values(ctx->c, nc, mip.data(), mix.data(), mval.data(), mdeg.data(), d, seed,
+                             mo, ev, vec, mst);
                 std::vector<int> idx(k);
                 std::iota(idx.begin(), idx.end(), 0);
                 std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return ev[a] < ev[b]; });
@@ -610,16 +768,14 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
                 }
                 range[c] = std::isfinite(md) ? md / 2.0 : 1.0;
             }
-            // embed_single_component (multi_component.rs:261-312)
+            // Eigenpairs of every component that gets its own embedding (size >= 2d):
+            // all at once on the device (batched), or one solve per component.
+            std::vector<std::vector<double>> cev(nc), cvec(nc);
+            std::vector<int> csolved(nc, 0);
             std::vector<int64_t> lookup(n, -1);
-            for (int64_t c = 0; c < nc; ++c) {
+            auto solve_one = [&](int64_t c) {
                 const std::vector<int64_t>& mem = members[c];
                 const int64_t sz = (int64_t)mem.size();
-                if (sz < 2 * d) {
-                    for (int64_t node : mem)
-                        for (int q = 0; q < d; ++q) coords[node * d + q] = meta[c * d + q];
-                    continue;
-                }
                 for (int64_t q = 0; q < sz; ++q) lookup[mem[q]] = q;
                 std::vector<int64_t> sip(sz + 1, 0);
                 std::vector<int32_t> six;
@@ -635,19 +791,38 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
                 }
                 si::SolveStats cst;
                 last.level = solve_graph(ctx->c, sz, sip.data(), six.data(), sw.data(), d, seed, mode,
-                                         opt, ev, vec, cst, &lap_ms, &solve_ms);
+                                         opt, cev[c], cvec[c], cst, &lap_ms, &solve_ms);
                 last.iterations += cst.iterations;
                 last.spmm_calls += cst.spmm_calls;
                 last.max_residual = std::max(last.max_residual, cst.max_residual);
-                select_and_sign(ev, vec, sz, k, d, sel);
+                for (int64_t node : mem) lookup[node] = -1;
+                csolved[c] = 1;
+            };
+            if (batch) {
+                solve_components_batched(ctx->c, n, indptr, indices, data, labels, members, d, seed,
+                                         mode, opt, cev, cvec, csolved, last, &lap_ms, &solve_ms);
+            }
+            for (int64_t c = 0; c < nc; ++c)
+                if (!csolved[c] && (int64_t)members[c].size() >= 2 * d) solve_one(c);
+            // embed_single_component (multi_component.rs:261-312): place each component
+            for (int64_t c = 0; c < nc; ++c) {
+                const std::vector<int64_t>& mem = members[c];
+                const int64_t sz = (int64_t)mem.size();
+                if (sz < 2 * d) {
+                    for (int64_t node : mem)
+                        for (int q = 0; q < d; ++q) coords[node * d + q] = meta[c * d + q];
+                    continue;
+                }
+                select_and_sign(cev[c], cvec[c], sz, k, d, sel);
                 double mx = 0.0;
                 for (double v : sel) mx = std::max(mx, std::fabs(v));
                 const double f = mx > 0.0 ? range[c] / mx : 1.0;
                 for (int64_t q = 0; q < sz; ++q)
                     for (int e = 0; e < d; ++e)
                         coords[mem[q] * d + e] = sel[(size_t)q * d + e] * f + meta[c * d + e];
-                for (int64_t node : mem) lookup[node] = -1;
             }
+            for (int64_t c = 0; c < nc; ++c)
+                if (!cev[c].empty()) ev = cev[c];
         }
         // scale_and_add_noise (src/scaling.rs:78-89): max-abs 10, N(0, 1e-4) f32 noise
         double mx = 0.0;

@@ -25,6 +25,11 @@ si_status tguard(si_context* ctx, F&& f) {
 }
 }  // namespace
 
+namespace si {
+extern bool g_batch_graphs;
+extern BatchStats g_last_batch;
+}  // namespace si
+
 extern "C" {
 
 si_status si_test_gram(si_context* ctx, int32_t precision, int64_t n, const void* a, int32_t lda,
@@ -161,6 +166,13 @@ si_status si_profile_spmm_end(si_context* ctx, si_spmm_profile* out) {
         out->ms_f32 = p.ms_f32;
         ctx->c.profile.enabled = false;
     });
+}
+
+si_status si_test_batch_graphs(int32_t enable, int64_t* captured, int64_t* launches) {
+    if (captured) *captured = si::g_last_batch.graphs_captured;
+    if (launches) *launches = si::g_last_batch.graph_launches;
+    if (enable >= 0) si::g_batch_graphs = enable != 0;
+    return SI_OK;
 }
 
 }  // extern "C"

@@ -26,7 +26,8 @@ __global__ __launch_bounds__(256) void k_gram(int64_t n, const T* __restrict__ a
                                               const T* __restrict__ b, int ldb, int q1,
                                               const T* __restrict__ b2, int ldb2, int q,
                                               double* __restrict__ partial,
-                                              int64_t rows_per_block) {
+                                              int64_t rows_per_block,
+                                              const int64_t* __restrict__ blk_rows) {
     __shared__ double red[PT * 16][QT * 16 + 1];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
@@ -37,8 +38,14 @@ __global__ __launch_bounds__(256) void k_gram(int64_t n, const T* __restrict__ a
     for (int ti = 0; ti < PT; ++ti)
 #pragma unroll
         for (int tj = 0; tj < QT; ++tj) acc[ti][tj] = v4d{0.0, 0.0, 0.0, 0.0};
-    const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
-    const int64_t r1 = (r0 + rows_per_block) < n ? (r0 + rows_per_block) : n;
+    int64_t r0, r1;
+    if (blk_rows) {  // segmented: rows [r0, r1) of one segment
+        r0 = blk_rows[2 * blockIdx.x];
+        r1 = blk_rows[2 * blockIdx.x + 1];
+    } else {
+        r0 = (int64_t)blockIdx.x * rows_per_block;
+        r1 = (r0 + rows_per_block) < n ? (r0 + rows_per_block) : n;
+    }
     // the first B block equal to A: read it once
     const bool same = (a == b) && (lda == ldb) && (q1 == p) && (q1 % 16 == 0 || q1 == q);
     // UR row groups of 4 rows per wave in flight: all loads first, then MFMAs
@@ -132,6 +139,47 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const double* __restric
     if (grp == 0 && e < pq) g[e] = red[0][lane];
 }
 
+// Segmented: g[c][e] = sum of partial[b][e] over b in [seg_blk[c], seg_blk[c+1]),
+// segment c = blockIdx.y; same fixed order as above.
+__global__ __launch_bounds__(256) void k_reduce_partials_seg(const double* __restrict__ partial,
+                                                             const int32_t* __restrict__ seg_blk,
+                                                             int pq, double* __restrict__ g) {
+    constexpr int E = 8, G = 256 / E, D = 8;
+    __shared__ double red[G][E + 1];
+    const int lane = threadIdx.x % E, grp = threadIdx.x / E;
+    const int e = blockIdx.x * E + lane;
+    const int b0 = seg_blk[blockIdx.y], b1 = seg_blk[blockIdx.y + 1];
+    double acc[D];
+#pragma unroll
+    for (int u = 0; u < D; ++u) acc[u] = 0.0;
+    if (e < pq) {
+        for (int b = b0 + grp; b < b1; b += D * G) {
+#pragma unroll
+            for (int u = 0; u < D; ++u) {
+                const int bb = b + u * G;
+                if (bb < b1) acc[u] += partial[(int64_t)bb * pq + e];
+            }
+        }
+    }
+    double t = 0.0;
+#pragma unroll
+    for (int u = 0; u < D; ++u) t += acc[u];
+    red[grp][lane] = t;
+    __syncthreads();
+    for (int h = G / 2; h > 0; h >>= 1) {
+        if (grp < h) red[grp][lane] += red[grp + h][lane];
+        __syncthreads();
+    }
+    if (grp == 0 && e < pq) g[(int64_t)blockIdx.y * pq + e] = red[0][lane];
+}
+
+void reduce_partials_seg(const double* partial, const int32_t* seg_blk, int nseg, int pq,
+                         double* g, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_partials_seg, dim3(ceil_div(pq, 8), nseg), dim3(256), 0, s,
+                       partial, seg_blk, pq, g);
+    SI_LAUNCH_CHECK();
+}
+
 void reduce_partials(const double* partial, int nparts, int pq, double* g, hipStream_t s) {
     hipLaunchKernelGGL(k_reduce_partials, dim3(ceil_div(pq, 8)), dim3(256), 0, s, partial, nparts,
                        pq, g);
@@ -140,22 +188,28 @@ void reduce_partials(const double* partial, int nparts, int pq, double* g, hipSt
 
 template <typename T>
 void gram_impl(int64_t n, const T* a, int lda, int p, const T* b, int ldb, int q1, const T* b2,
-               int ldb2, int q, double* g, double* ws, hipStream_t s) {
+               int ldb2, int q, double* g, double* ws, hipStream_t s,
+               const SegWork* seg = nullptr) {
     SI_CHECK(p >= 1 && q >= 1 && p <= 64 && q <= 64 && q1 >= 1 && q1 <= q, kErrInvalidArgument,
              "gram: width out of range");
     SI_CHECK(q1 == q || b2 != nullptr, kErrInvalidArgument, "gram: second block missing");
-    int64_t nblocks = ceil_div(n, 64);
-    if (nblocks > kGramBlocks) nblocks = kGramBlocks;
-    if (nblocks < 1) nblocks = 1;
-    int64_t rpb = ceil_div(n, nblocks);
-    rpb = ceil_div(rpb, 16) * 16;
-    nblocks = ceil_div(n, rpb);
-    if (nblocks < 1) nblocks = 1;
+    int64_t nblocks = 1, rpb = 16;
+    const int64_t* blk_rows = nullptr;
+    if (seg) {  // the work list gives every block's rows
+        nblocks = seg->nblk;
+        blk_rows = seg->blk_rows;
+        SI_CHECK(nblocks >= 1, kErrInvalidArgument, "gram: empty segment work list");
+    } else if (n > 0) {
+        nblocks = ceil_div(n, 64);
+        if (nblocks > kGramBlocks) nblocks = kGramBlocks;
+        rpb = ceil_div(ceil_div(n, nblocks), 16) * 16;
+        nblocks = ceil_div(n, rpb);
+    }
     const int pt = (p + 15) / 16, qt = (q + 15) / 16;
 #define SI_GRAM_CASE(PT_, QT_)                                                                   \
     if (pt == PT_ && qt == QT_) {                                                                \
         hipLaunchKernelGGL((k_gram<T, PT_, QT_>), dim3(nblocks), dim3(256), 0, s, n, a, lda, p, b, \
-                           ldb, q1, b2, ldb2, q, ws, rpb);                                        \
+                           ldb, q1, b2, ldb2, q, ws, rpb, blk_rows);                              \
     } else
     SI_GRAM_CASE(1, 1) SI_GRAM_CASE(1, 2) SI_GRAM_CASE(1, 3) SI_GRAM_CASE(1, 4)
     SI_GRAM_CASE(2, 1) SI_GRAM_CASE(2, 2) SI_GRAM_CASE(2, 3) SI_GRAM_CASE(2, 4)
@@ -164,7 +218,10 @@ void gram_impl(int64_t n, const T* a, int lda, int p, const T* b, int ldb, int q
     { throw Error(kErrInternal, "gram: no kernel for this shape"); }
 #undef SI_GRAM_CASE
     SI_LAUNCH_CHECK();
-    reduce_partials(ws, (int)nblocks, p * q, g, s);
+    if (seg)
+        reduce_partials_seg(ws, seg->seg_blk, seg->nseg, p * q, g, s);
+    else
+        reduce_partials(ws, (int)nblocks, p * q, g, s);
 }
 
 // ------------------------------------------------------------------ block products
@@ -252,6 +309,89 @@ void block_mul_impl(int64_t n, const T* a, int lda, int p, const double* c, int 
 // each and accumulate the norms of the wanted columns they own.
 constexpr int kRrBlocks = 2048;
 
+// One R-row tile of the update; accumulates the wanted-column norms of the
+// tile's rows into rr2/xx2 (per owned column).  cm and tile are LDS arrays.
+template <typename T, int QMAX>
+__device__ inline void rr_tile(int64_t row0, int rows, const T* __restrict__ y,
+                               const T* __restrict__ ly, int ld, int m, int k, const T* cm,
+                               T* tile, const double* th, T* __restrict__ x, T* __restrict__ ax,
+                               double* rr2, double* xx2) {
+    constexpr int R = sizeof(T) == 8 ? 32 : 64;
+    constexpr int TPR = 256 / R;
+    constexpr int CPT = QMAX / TPR;
+    constexpr int LDT = 65;
+    const int t = threadIdx.x;
+    const int r = t / TPR, g = t % TPR;
+    T accx[CPT], acca[CPT];
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) { accx[u] = T(0); acca[u] = T(0); }
+    for (int pass = 0; pass < 2; ++pass) {
+        const T* src = pass == 0 ? y : ly;
+        __syncthreads();
+        for (int e = t; e < rows * m; e += 256) {
+            const int rr = e / m, kk = e % m;
+            tile[rr * LDT + kk] = src[(row0 + rr) * ld + kk];
+        }
+        __syncthreads();
+        if (r < rows) {
+            for (int kk = 0; kk < m; ++kk) {
+                const T av = tile[r * LDT + kk];
+#pragma unroll
+                for (int u = 0; u < CPT; ++u) {
+                    const T cv = cm[kk * QMAX + g * CPT + u];
+                    if (pass == 0) accx[u] += av * cv;
+                    else acca[u] += av * cv;
+                }
+            }
+        }
+    }
+    if (r < rows) {
+        T* xo = x + (row0 + r) * ld;
+        T* ao = ax + (row0 + r) * ld;
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) {
+            const int j = g * CPT + u;
+            if (j < m) {
+                xo[j] = accx[u];
+                ao[j] = acca[u];
+                if (j < k) {
+                    const double xv = (double)accx[u];
+                    const double rv = (double)acca[u] - th[u] * xv;
+                    rr2[u] += rv * rv;
+                    xx2[u] += xv * xv;
+                }
+            }
+        }
+    }
+}
+
+// Block-level fixed-order sum of the owned wanted-column norms into
+// partial[pidx][0..2k).
+template <typename T, int QMAX>
+__device__ inline void rr_norms_out(int k, const double* rr2, const double* xx2,
+                                    double (*red)[2 * 8 + 1], double* __restrict__ partial,
+                                    int64_t pidx) {
+    constexpr int R = sizeof(T) == 8 ? 32 : 64;
+    constexpr int TPR = 256 / R;
+    constexpr int CPT = QMAX / TPR;
+    const int t = threadIdx.x;
+    const int r = t / TPR, g = t % TPR;
+    __syncthreads();
+    // every (row slot, wanted column) has exactly one owner thread
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+        const int j = g * CPT + u;
+        if (j < k) { red[r][j] = rr2[u]; red[r][8 + j] = xx2[u]; }
+    }
+    __syncthreads();
+    if (t < 2 * k) {
+        const int j = t < k ? t : 8 + (t - k);
+        double sacc = 0.0;
+        for (int q = 0; q < R; ++q) sacc += red[q][j];
+        partial[pidx * 2 * k + t] = sacc;
+    }
+}
+
 template <typename T, int QMAX>
 __global__ __launch_bounds__(256) void k_rr_update(int64_t n, const T* __restrict__ y,
                                                    const T* __restrict__ ly, int ld, int m,
@@ -268,7 +408,7 @@ __global__ __launch_bounds__(256) void k_rr_update(int64_t n, const T* __restric
     __shared__ T tile[R * LDT];
     __shared__ double red[R][2 * 8 + 1];
     const int t = threadIdx.x;
-    const int r = t / TPR, g = t % TPR;
+    const int g = t % TPR;
     for (int e = t; e < m * m; e += 256) cm[(e / m) * QMAX + e % m] = (T)c[e];
     double rr2[CPT], xx2[CPT], th[CPT];
 #pragma unroll
@@ -282,61 +422,49 @@ __global__ __launch_bounds__(256) void k_rr_update(int64_t n, const T* __restric
     for (int64_t tl = blockIdx.x; tl < tiles; tl += gridDim.x) {
         const int64_t row0 = tl * R;
         const int rows = (n - row0) < R ? (int)(n - row0) : R;
-        T accx[CPT], acca[CPT];
-#pragma unroll
-        for (int u = 0; u < CPT; ++u) { accx[u] = T(0); acca[u] = T(0); }
-        for (int pass = 0; pass < 2; ++pass) {
-            const T* src = pass == 0 ? y : ly;
-            __syncthreads();
-            for (int e = t; e < rows * m; e += 256) {
-                const int rr = e / m, kk = e % m;
-                tile[rr * LDT + kk] = src[(row0 + rr) * ld + kk];
-            }
-            __syncthreads();
-            if (r < rows) {
-                for (int kk = 0; kk < m; ++kk) {
-                    const T av = tile[r * LDT + kk];
-#pragma unroll
-                    for (int u = 0; u < CPT; ++u) {
-                        const T cv = cm[kk * QMAX + g * CPT + u];
-                        if (pass == 0) accx[u] += av * cv;
-                        else acca[u] += av * cv;
-                    }
-                }
-            }
-        }
-        if (r < rows) {
-            T* xo = x + (row0 + r) * ld;
-            T* ao = ax + (row0 + r) * ld;
-#pragma unroll
-            for (int u = 0; u < CPT; ++u) {
-                const int j = g * CPT + u;
-                if (j < m) {
-                    xo[j] = accx[u];
-                    ao[j] = acca[u];
-                    if (j < k) {
-                        const double xv = (double)accx[u];
-                        const double rv = (double)acca[u] - th[u] * xv;
-                        rr2[u] += rv * rv;
-                        xx2[u] += xv * xv;
-                    }
-                }
-            }
-        }
+        rr_tile<T, QMAX>(row0, rows, y, ly, ld, m, k, cm, tile, th, x, ax, rr2, xx2);
     }
-    // every (row slot, wanted column) has exactly one owner thread
+    rr_norms_out<T, QMAX>(k, rr2, xx2, red, partial, blockIdx.x);
+}
+
+// Segmented: workgroup b updates tile b (rows blk_rows[2b] .. blk_rows[2b+1],
+// at most R, inside segment blk_seg[b]) with that segment's C (c + seg*m*m)
+// and Ritz values (theta + seg*m); one partial per tile.
+template <typename T, int QMAX>
+__global__ __launch_bounds__(256) void k_rr_update_seg(const int64_t* __restrict__ blk_rows,
+                                                       const int32_t* __restrict__ blk_seg,
+                                                       const T* __restrict__ y,
+                                                       const T* __restrict__ ly, int ld, int m,
+                                                       const double* __restrict__ c,
+                                                       const double* __restrict__ theta, int k,
+                                                       T* __restrict__ x, T* __restrict__ ax,
+                                                       double* __restrict__ partial) {
+    constexpr int R = sizeof(T) == 8 ? 32 : 64;
+    constexpr int TPR = 256 / R;
+    constexpr int CPT = QMAX / TPR;
+    static_assert(CPT * TPR == QMAX, "rr_update: QMAX must be a multiple of TPR");
+    constexpr int LDT = 65;
+    __shared__ T cm[64 * QMAX];
+    __shared__ T tile[R * LDT];
+    __shared__ double red[R][2 * 8 + 1];
+    const int t = threadIdx.x;
+    const int g = t % TPR;
+    const int sg = blk_seg[blockIdx.x];
+    const double* cs = c + (int64_t)sg * m * m;
+    const double* ts = theta + (int64_t)sg * m;
+    for (int e = t; e < m * m; e += 256) cm[(e / m) * QMAX + e % m] = (T)cs[e];
+    double rr2[CPT], xx2[CPT], th[CPT];
 #pragma unroll
     for (int u = 0; u < CPT; ++u) {
         const int j = g * CPT + u;
-        if (j < k) { red[r][j] = rr2[u]; red[r][8 + j] = xx2[u]; }
+        rr2[u] = 0.0;
+        xx2[u] = 0.0;
+        th[u] = j < k ? ts[j] : 0.0;
     }
-    __syncthreads();
-    if (t < 2 * k) {
-        const int j = t < k ? t : 8 + (t - k);
-        double sacc = 0.0;
-        for (int q = 0; q < R; ++q) sacc += red[q][j];
-        partial[(int64_t)blockIdx.x * 2 * k + t] = sacc;
-    }
+    const int64_t row0 = blk_rows[2 * blockIdx.x];
+    const int rows = (int)(blk_rows[2 * blockIdx.x + 1] - row0);
+    rr_tile<T, QMAX>(row0, rows, y, ly, ld, m, k, cm, tile, th, x, ax, rr2, xx2);
+    rr_norms_out<T, QMAX>(k, rr2, xx2, red, partial, blockIdx.x);
 }
 
 template <typename T>
@@ -356,6 +484,23 @@ void rr_update_impl(int64_t n, const T* y, const T* ly, int ld, int m, const dou
         hipLaunchKernelGGL((k_rr_update<T, 64>), dim3(blocks), dim3(256), 0, s, n, y, ly, ld, m, c, theta, k, x, ax, ws);
     SI_LAUNCH_CHECK();
     reduce_partials(ws, (int)blocks, 2 * k, norms2, s);
+}
+
+template <typename T>
+void rr_update_seg_impl(const SegWork& tiles, const T* y, const T* ly, int ld, int m,
+                        const double* c, const double* theta, int k, T* x, T* ax,
+                        double* norms2, double* ws, hipStream_t s) {
+    SI_CHECK(m >= 1 && m <= 64 && k >= 1 && k <= 8 && k <= m, kErrInvalidArgument,
+             "rr_update: need k <= 8, k <= m <= 64");
+    const dim3 grid((unsigned)tiles.nblk);
+    if (m <= 16)
+        hipLaunchKernelGGL((k_rr_update_seg<T, 16>), grid, dim3(256), 0, s, tiles.blk_rows, tiles.blk_seg, y, ly, ld, m, c, theta, k, x, ax, ws);
+    else if (m <= 32)
+        hipLaunchKernelGGL((k_rr_update_seg<T, 32>), grid, dim3(256), 0, s, tiles.blk_rows, tiles.blk_seg, y, ly, ld, m, c, theta, k, x, ax, ws);
+    else
+        hipLaunchKernelGGL((k_rr_update_seg<T, 64>), grid, dim3(256), 0, s, tiles.blk_rows, tiles.blk_seg, y, ly, ld, m, c, theta, k, x, ax, ws);
+    SI_LAUNCH_CHECK();
+    reduce_partials_seg(ws, tiles.seg_blk, tiles.nseg, 2 * k, norms2, s);
 }
 
 // ------------------------------------------------------------------ elementwise
@@ -477,6 +622,24 @@ size_t rr_update_workspace_doubles(int64_t n, int k) {
     return (size_t)nb * 2 * k;
 }
 
+void gram2_seg_f32(const SegWork& blocks, const float* a, int ld, int p, const float* b2, int ld2,
+                   double* g, double* ws, hipStream_t s) {
+    gram_impl<float>(0, a, ld, p, a, ld, p, b2, ld2, 2 * p, g, ws, s, &blocks);
+}
+void gram2_seg_f64(const SegWork& blocks, const double* a, int ld, int p, const double* b2,
+                   int ld2, double* g, double* ws, hipStream_t s) {
+    gram_impl<double>(0, a, ld, p, a, ld, p, b2, ld2, 2 * p, g, ws, s, &blocks);
+}
+void rr_update_seg_f32(const SegWork& tiles, const float* y, const float* ly, int ld, int m,
+                       const double* c, const double* theta, int k, float* x, float* ax,
+                       double* norms2, double* ws, hipStream_t s) {
+    rr_update_seg_impl<float>(tiles, y, ly, ld, m, c, theta, k, x, ax, norms2, ws, s);
+}
+void rr_update_seg_f64(const SegWork& tiles, const double* y, const double* ly, int ld, int m,
+                       const double* c, const double* theta, int k, double* x, double* ax,
+                       double* norms2, double* ws, hipStream_t s) {
+    rr_update_seg_impl<double>(tiles, y, ly, ld, m, c, theta, k, x, ax, norms2, ws, s);
+}
 void gram2_f32(int64_t n, const float* a, int lda, int p, const float* b, int ldb, int q1,
                const float* b2, int ldb2, int q2, double* g, double* ws, hipStream_t s) {
     gram_impl<float>(n, a, lda, p, b, ldb, q1, b2, ldb2, q1 + q2, g, ws, s);

@@ -13,6 +13,7 @@
 // (laplacian.rs:70-80); W's own diagonal is dropped and L's unit diagonal is
 // implied (laplacian.rs:83-85).
 #include <algorithm>
+#include <cmath>
 #include <utility>
 #include <vector>
 
@@ -290,6 +291,76 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
     L.n_long = nlong;
     hipFree(d_flags);
     if (d_w2) hipFree(d_w2);
+}
+
+// A Laplacian given by its FP64 off-diagonal values (unit diagonal implied),
+// e.g. the centroid graph of the spectral meta-embedding, whose Gaussian
+// affinities (often ~1e-80) exist only in FP64 before normalisation
+// (src/multi_component.rs:170-205).  The FP32 sweep values are the rounded
+// normalised entries; sqrt_deg (the trivial-vector hint) is stored scaled to
+// max 1 so it survives FP32.
+void laplacian_from_values_device(DevLaplacian& L, int64_t n, const int64_t* h_ip,
+                                  const int32_t* h_ix, const double* h_l, const double* h_deg,
+                                  hipStream_t s) {
+    SI_CHECK(n > 0 && h_ip && h_ip[0] == 0, kErrInvalidArgument, "laplacian_from_values: bad CSR");
+    const int64_t nnz = h_ip[n];
+    SI_CHECK(nnz < (int64_t)INT32_MAX, kErrUnsupported, "laplacian_from_values: nnz >= 2^31");
+    L = DevLaplacian{};
+    L.n = n;
+    L.nnz = nnz;
+    L.degree_mode = 1;
+    L.symmetric = 1;
+    L.degree_exact = 1;
+    std::vector<int32_t> rp(n + 1), lr;
+    std::vector<float> a32(nnz > 0 ? nnz : 1);
+    std::vector<double> sq(n), isd(n);
+    int32_t maxlen = 0;
+    double sqmax = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        rp[i] = (int32_t)h_ip[i];
+        const int32_t len = (int32_t)(h_ip[i + 1] - h_ip[i]);
+        maxlen = std::max(maxlen, len);
+        if (len > kLongRow) lr.push_back((int32_t)i);
+        for (int64_t p = h_ip[i]; p < h_ip[i + 1]; ++p) {
+            SI_CHECK(h_ix[p] >= 0 && h_ix[p] < n && h_ix[p] != i, kErrInvalidArgument,
+                     "laplacian_from_values: column out of range or on the diagonal");
+            a32[p] = (float)(-h_l[p]);
+        }
+        sq[i] = std::sqrt(std::max(h_deg[i], 0.0));
+        isd[i] = h_deg[i] > 0.0 ? 1.0 / sq[i] : 0.0;
+        sqmax = std::max(sqmax, sq[i]);
+    }
+    rp[n] = (int32_t)nnz;
+    if (sqmax > 0.0)
+        for (double& v : sq) v /= sqmax;
+    L.max_row_len = maxlen;
+    L.n_long = (int64_t)lr.size();
+    try {
+        SI_HIP(hipMalloc(&L.rowptr, sizeof(int32_t) * (n + 1)));
+        SI_HIP(hipMalloc(&L.col, sizeof(int32_t) * (nnz > 0 ? nnz : 1)));
+        SI_HIP(hipMalloc(&L.a32, sizeof(float) * (nnz > 0 ? nnz : 1)));
+        SI_HIP(hipMalloc(&L.l64, sizeof(double) * (nnz > 0 ? nnz : 1)));
+        SI_HIP(hipMalloc(&L.deg, sizeof(double) * n));
+        SI_HIP(hipMalloc(&L.sqrt_deg, sizeof(double) * n));
+        SI_HIP(hipMalloc(&L.isd, sizeof(double) * n));
+        SI_HIP(hipMalloc(&L.long_rows, sizeof(int32_t) * (lr.empty() ? 1 : lr.size())));
+        SI_HIP(hipMemcpyAsync(L.rowptr, rp.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, s));
+        if (nnz > 0) {
+            SI_HIP(hipMemcpyAsync(L.col, h_ix, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s));
+            SI_HIP(hipMemcpyAsync(L.a32, a32.data(), sizeof(float) * nnz, hipMemcpyHostToDevice, s));
+            SI_HIP(hipMemcpyAsync(L.l64, h_l, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
+        }
+        SI_HIP(hipMemcpyAsync(L.deg, h_deg, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        SI_HIP(hipMemcpyAsync(L.sqrt_deg, sq.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+        SI_HIP(hipMemcpyAsync(L.isd, isd.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+        if (!lr.empty())
+            SI_HIP(hipMemcpyAsync(L.long_rows, lr.data(), sizeof(int32_t) * lr.size(),
+                                  hipMemcpyHostToDevice, s));
+        SI_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        laplacian_free(L);
+        throw;
+    }
 }
 
 }  // namespace si

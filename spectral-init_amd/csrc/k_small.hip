@@ -186,12 +186,20 @@ __global__ __launch_bounds__(kThreads) void k_sym_eig(int n, const double* __res
 // the problem is first scaled by D = diag(B)^-1/2 on both sides (column
 // normalisation of the underlying block, done on the small matrices) and the
 // returned vectors are D c'.
+// Batched over blockIdx.x with strides (in doubles) between problems: problem
+// p reads ain + p*sa, bin + p*sb and writes w + p*n, cout + p*n*n, status[p].
 __global__ __launch_bounds__(kThreads) void k_gen_eig(int n, const double* __restrict__ ain,
                                                        int lda, const double* __restrict__ bin,
                                                        int ldb, int scale,
                                                        double* __restrict__ w,
                                                        double* __restrict__ cout,
-                                                       int* __restrict__ status) {
+                                                       int* __restrict__ status, int64_t sa,
+                                                       int64_t sb) {
+    ain += blockIdx.x * sa;
+    bin += blockIdx.x * sb;
+    w += (int64_t)blockIdx.x * n;
+    cout += (int64_t)blockIdx.x * n * n;
+    status += blockIdx.x;
     __shared__ double a[N][LDA];
     __shared__ double l[N][LDA];
     __shared__ double v[N][LDA];
@@ -325,6 +333,50 @@ __global__ void k_svqb(int n, const double* __restrict__ w, const double* __rest
     }
 }
 
+// Dense EVD of the diagonal blocks of a block-diagonal L, one workgroup per
+// block: block b covers rows [seg_row0[b], seg_row0[b] + seg_n[b]) (<= 64 rows,
+// columns inside the same range).  The k smallest eigenvalues go to
+// evals[b*k + j], their vectors to rows of evecs (leading dimension ldv).
+__global__ __launch_bounds__(kThreads) void k_dense_seg_eig(
+    const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+    const double* __restrict__ l64, const int64_t* __restrict__ seg_row0,
+    const int32_t* __restrict__ seg_n, int k, double* __restrict__ evals,
+    double* __restrict__ evecs, int ldv) {
+    __shared__ double a[N][LDA];
+    __shared__ double v[N][LDA];
+    __shared__ double cs[N / 2], sn[N / 2];
+    __shared__ int pp[N / 2], qq[N / 2], order[N];
+    __shared__ double red[2 * kThreads];
+    const int t = threadIdx.x;
+    const int64_t r0 = seg_row0[blockIdx.x];
+    const int n = seg_n[blockIdx.x];
+    for (int e = t; e < n * n; e += kThreads) a[e / n][e % n] = (e / n == e % n) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int i = t; i < n; i += kThreads)
+        for (int32_t q = rp[r0 + i]; q < rp[r0 + i + 1]; ++q) {
+            const int64_t j = (int64_t)col[q] - r0;
+            if (j != i && j >= 0 && j < n) a[i][j] += l64[q];
+        }
+    __syncthreads();
+    jacobi_lds(a, v, n, cs, sn, pp, qq, red);
+    if (t == 0) {
+        for (int i = 0; i < n; ++i) order[i] = i;
+        for (int i = 1; i < n; ++i) {
+            const int oi = order[i];
+            int j = i - 1;
+            while (j >= 0 && a[order[j]][order[j]] > a[oi][oi]) { order[j + 1] = order[j]; --j; }
+            order[j + 1] = oi;
+        }
+    }
+    __syncthreads();
+    const int kk = k < n ? k : n;
+    for (int j = t; j < kk; j += kThreads) evals[(int64_t)blockIdx.x * k + j] = a[order[j]][order[j]];
+    for (int e = t; e < n * kk; e += kThreads) {
+        const int r = e / kk, c = e % kk;
+        evecs[(r0 + r) * ldv + c] = v[r][order[c]];
+    }
+}
+
 __global__ void k_dense_from_csr(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                  const double* __restrict__ l64, int n, double* __restrict__ d) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -362,14 +414,32 @@ void small_sym_eig(int n, const double* a, int lda, double* w, double* v, hipStr
 void small_gen_eig(int n, const double* a, const double* b, double* w, double* c, int* status,
                    hipStream_t s) {
     SI_CHECK(n >= 1 && n <= kSmallMax, kErrInvalidArgument, "small_gen_eig: n out of range");
-    hipLaunchKernelGGL(k_gen_eig, dim3(1), dim3(kThreads), 0, s, n, a, n, b, n, 0, w, c, status);
+    hipLaunchKernelGGL(k_gen_eig, dim3(1), dim3(kThreads), 0, s, n, a, n, b, n, 0, w, c, status,
+                       (int64_t)0, (int64_t)0);
     SI_LAUNCH_CHECK();
 }
 
 void small_gen_eig_scaled(int n, const double* a, int lda, const double* b, int ldb, double* w,
                           double* c, int* status, hipStream_t s) {
     SI_CHECK(n >= 1 && n <= kSmallMax, kErrInvalidArgument, "small_gen_eig: n out of range");
-    hipLaunchKernelGGL(k_gen_eig, dim3(1), dim3(kThreads), 0, s, n, a, lda, b, ldb, 1, w, c, status);
+    hipLaunchKernelGGL(k_gen_eig, dim3(1), dim3(kThreads), 0, s, n, a, lda, b, ldb, 1, w, c, status,
+                       (int64_t)0, (int64_t)0);
+    SI_LAUNCH_CHECK();
+}
+void small_gen_eig_scaled_batched(int count, int n, const double* a, int lda, int64_t sa,
+                                  const double* b, int ldb, int64_t sb, double* w, double* c,
+                                  int* status, hipStream_t s) {
+    SI_CHECK(n >= 1 && n <= kSmallMax && count >= 1, kErrInvalidArgument,
+             "small_gen_eig_batched: size out of range");
+    hipLaunchKernelGGL(k_gen_eig, dim3(count), dim3(kThreads), 0, s, n, a, lda, b, ldb, 1, w, c,
+                       status, sa, sb);
+    SI_LAUNCH_CHECK();
+}
+void dense_seg_eig(const DevLaplacian& L, int count, const int64_t* seg_row0, const int32_t* seg_n,
+                   int k, double* evals, double* evecs, int ldv, hipStream_t s) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_dense_seg_eig, dim3(count), dim3(kThreads), 0, s, L.rowptr, L.col, L.l64,
+                       seg_row0, seg_n, k, evals, evecs, ldv);
     SI_LAUNCH_CHECK();
 }
 

@@ -44,18 +44,34 @@ template <> __device__ inline float apply_diag<float>(float xi, float acc) { ret
 template <> __device__ inline double apply_diag<double>(double xi, double acc) { return xi + acc; }
 
 // -------------------------------------------------------------- narrow
+// Per-segment epilogue scalars: with rseg set, row i uses
+// (s_l, s_x, s_z) = coef[3 * rseg[i] + {0, 1, 2}] instead of the arguments; a
+// segment with s_l == 0 is a copy-through (no gathers).
+template <typename T>
+__device__ inline void seg_scalars(const int32_t* __restrict__ rseg, const T* __restrict__ coef,
+                                   int64_t i, T& s_l, T& s_x, T& s_z) {
+    if (rseg) {
+        const int g = rseg[i];
+        s_l = coef[3 * g];
+        s_x = coef[3 * g + 1];
+        s_z = coef[3 * g + 2];
+    }
+}
+
 template <typename T, int K, int G>
 __global__ __launch_bounds__(256) void k_spmm_narrow(
     const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const T* __restrict__ val,
     const T* __restrict__ x, T* __restrict__ out, const T* __restrict__ z, int ld, int64_t n,
-    T s_l, T s_x, T s_z) {
+    T s_l, T s_x, T s_z, const int32_t* __restrict__ rseg, const T* __restrict__ coef) {
     const int64_t nb = gridDim.x;
     const int64_t blk = xcd_block(blockIdx.x, nb);
     const int lane = threadIdx.x % G;
     const int64_t i = (blk * blockDim.x + threadIdx.x) / G;
     if (i >= n) return;
-    const int32_t b = rp[i], e = rp[i + 1];
+    int32_t b = rp[i], e = rp[i + 1];
     if (e - b > kLongRow) return;
+    seg_scalars<T>(rseg, coef, i, s_l, s_x, s_z);
+    if (s_l == T(0)) e = b;
     T acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = T(0);
@@ -109,7 +125,7 @@ template <typename T, int K, int SLOTS, int U>
 __global__ __launch_bounds__(256) void k_spmm_wide(
     const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const T* __restrict__ val,
     const T* __restrict__ x, T* __restrict__ out, const T* __restrict__ z, int ld, int64_t n,
-    T s_l, T s_x, T s_z) {
+    T s_l, T s_x, T s_z, const int32_t* __restrict__ rseg, const T* __restrict__ coef) {
     using V = typename Vec4<T>::type;
     constexpr int VEC = Vec4<T>::n;
     constexpr int C = K / VEC;                 // lanes across one x row
@@ -127,7 +143,8 @@ __global__ __launch_bounds__(256) void k_spmm_wide(
     int32_t b = 0, e = 0;
     if (valid) { b = rp[i]; e = rp[i + 1]; }
     const bool is_long = (e - b) > kLongRow;
-    if (!valid || is_long) { b = 0; e = 0; }
+    if (valid) seg_scalars<T>(rseg, coef, i, s_l, s_x, s_z);
+    if (!valid || is_long || s_l == T(0)) { b = 0; e = 0; }
     T acc[VEC];
 #pragma unroll
     for (int v = 0; v < VEC; ++v) acc[v] = T(0);
@@ -192,13 +209,16 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_spmm_long(
     const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const T* __restrict__ val,
     const T* __restrict__ x, T* __restrict__ out, const T* __restrict__ z, int ld, int k,
-    const int32_t* __restrict__ rows, T s_l, T s_x, T s_z) {
+    const int32_t* __restrict__ rows, T s_l, T s_x, T s_z, const int32_t* __restrict__ rseg,
+    const T* __restrict__ coef) {
     constexpr int CH = 16;                 // columns per pass
     static_assert(CH * 16 == 256, "long-row reduction assumes 16 columns per pass");
     __shared__ T part[256][CH + 1];
     __shared__ T segs[16][CH + 1];
     const int64_t i = rows[blockIdx.x];
-    const int32_t b = rp[i], e = rp[i + 1];
+    int32_t b = rp[i], e = rp[i + 1];
+    seg_scalars<T>(rseg, coef, i, s_l, s_x, s_z);
+    if (s_l == T(0)) e = b;
     const int t = threadIdx.x;
     for (int c0 = 0; c0 < k; c0 += CH) {
         const int cw = (k - c0) < CH ? (k - c0) : CH;
@@ -236,51 +256,60 @@ __global__ __launch_bounds__(256) void k_spmm_long(
     }
 }
 
+// Epilogue scalars of one launch: uniform, or per row segment (rseg, coef).
+template <typename T>
+struct Epi {
+    T s_l, s_x, s_z;
+    const T* z;
+    const int32_t* rseg;
+    const T* coef;
+};
+
 template <typename T, int K, int G>
-void launch_narrow(const DevLaplacian& L, const T* val, const T* x, T* out, int ld, T s_l,
-                   T s_x, T s_z, const T* z, hipStream_t s) {
+void launch_narrow(const DevLaplacian& L, const T* val, const T* x, T* out, int ld,
+                   const Epi<T>& ep, hipStream_t s) {
     const int64_t threads = L.n * G;
     const int64_t blocks = ceil_div(threads, 256);
     hipLaunchKernelGGL((k_spmm_narrow<T, K, G>), dim3(blocks), dim3(256), 0, s, L.rowptr, L.col,
-                       val, x, out, z, ld, L.n, s_l, s_x, s_z);
+                       val, x, out, ep.z, ld, L.n, ep.s_l, ep.s_x, ep.s_z, ep.rseg, ep.coef);
 }
 
 template <typename T, int K, int SLOTS, int U>
-void launch_wide(const DevLaplacian& L, const T* val, const T* x, T* out, int ld, T s_l, T s_x,
-                 T s_z, const T* z, hipStream_t s) {
+void launch_wide(const DevLaplacian& L, const T* val, const T* x, T* out, int ld,
+                 const Epi<T>& ep, hipStream_t s) {
     constexpr int VEC = Vec4<T>::n;
     constexpr int C = K / VEC;
     constexpr int RPW = 64 / (C * SLOTS);
     const int64_t waves = ceil_div(L.n, RPW);
     const int64_t blocks = ceil_div(waves, 4);
     hipLaunchKernelGGL((k_spmm_wide<T, K, SLOTS, U>), dim3(blocks), dim3(256), 0, s, L.rowptr, L.col,
-                       val, x, out, z, ld, L.n, s_l, s_x, s_z);
+                       val, x, out, ep.z, ld, L.n, ep.s_l, ep.s_x, ep.s_z, ep.rseg, ep.coef);
 }
 
 // One launch of a directly supported width (1, 2, 3, 4, 8, 16, 32, 64).
 template <typename T>
-void spmm_chunk(const DevLaplacian& L, const T* val, const T* x, T* out, int k, int ld, T s_l,
-                T s_x, T s_z, const T* z, hipStream_t s) {
+void spmm_chunk(const DevLaplacian& L, const T* val, const T* x, T* out, int k, int ld,
+                const Epi<T>& ep, hipStream_t s) {
     constexpr bool F32 = sizeof(T) == 4;
     switch (k) {
-        case 1: launch_narrow<T, 1, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s); break;
-        case 2: launch_narrow<T, 2, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s); break;
-        case 3: launch_narrow<T, 3, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s); break;
-        case 4: launch_narrow<T, 4, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s); break;
-        case 8: launch_narrow<T, 8, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s); break;
+        case 1: launch_narrow<T, 1, 8>(L, val, x, out, ld, ep, s); break;
+        case 2: launch_narrow<T, 2, 8>(L, val, x, out, ld, ep, s); break;
+        case 3: launch_narrow<T, 3, 8>(L, val, x, out, ld, ep, s); break;
+        case 4: launch_narrow<T, 4, 8>(L, val, x, out, ld, ep, s); break;
+        case 8: launch_narrow<T, 8, 8>(L, val, x, out, ld, ep, s); break;
         // (SLOTS, U): SLOTS*U nonzero positions per row per pass; ~32 fits the
         // 15-30 entries of k-NN rows with few idle lanes (measured on MI355X)
         case 16:
-            if constexpr (F32) launch_wide<T, 16, 4, 4>(L, val, x, out, ld, s_l, s_x, s_z, z, s);   // C=4, RPW=4
-            else launch_wide<T, 16, 4, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s);       // C=8, RPW=2
+            if constexpr (F32) launch_wide<T, 16, 4, 4>(L, val, x, out, ld, ep, s);   // C=4, RPW=4
+            else launch_wide<T, 16, 4, 8>(L, val, x, out, ld, ep, s);       // C=8, RPW=2
             break;
         case 32:
-            if constexpr (F32) launch_wide<T, 32, 4, 4>(L, val, x, out, ld, s_l, s_x, s_z, z, s);   // C=8, RPW=2
-            else launch_wide<T, 32, 4, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s);       // C=16, RPW=1
+            if constexpr (F32) launch_wide<T, 32, 4, 4>(L, val, x, out, ld, ep, s);   // C=8, RPW=2
+            else launch_wide<T, 32, 4, 8>(L, val, x, out, ld, ep, s);       // C=16, RPW=1
             break;
         case 64:
-            if constexpr (F32) launch_wide<T, 64, 4, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s);   // C=16, RPW=1
-            else launch_wide<T, 64, 2, 8>(L, val, x, out, ld, s_l, s_x, s_z, z, s);       // C=32, RPW=1
+            if constexpr (F32) launch_wide<T, 64, 4, 8>(L, val, x, out, ld, ep, s);   // C=16, RPW=1
+            else launch_wide<T, 64, 2, 8>(L, val, x, out, ld, ep, s);       // C=32, RPW=1
             break;
         default:
             throw Error(kErrInternal, "spmm: no kernel for width " + std::to_string(k));
@@ -294,7 +323,8 @@ void spmm_chunk(const DevLaplacian& L, const T* val, const T* x, T* out, int k, 
 // handled once for all columns.
 template <typename T>
 void spmm_impl(const DevLaplacian& L, const T* val, const T* x, T* out, int k, int ld, T s_l,
-               T s_x, T s_z, const T* z, hipStream_t s) {
+               T s_x, T s_z, const T* z, hipStream_t s, const int32_t* rseg = nullptr,
+               const T* coef = nullptr) {
     SI_CHECK(k >= 1 && ld >= k, kErrInvalidArgument, "spmm: need 1 <= k <= ld");
     SI_CHECK(x != out && (z == nullptr || z != out), kErrInvalidArgument,
              "spmm: output must not alias an input");
@@ -321,12 +351,13 @@ void spmm_impl(const DevLaplacian& L, const T* val, const T* x, T* out, int k, i
             for (int c : widths_narrow)
                 if (c <= rem) { w = c; break; }
         }
-        spmm_chunk<T>(L, val, x + off, out + off, w, ld, s_l, s_x, s_z, z ? z + off : nullptr, s);
+        const Epi<T> ep{s_l, s_x, s_z, z ? z + off : nullptr, rseg, coef};
+        spmm_chunk<T>(L, val, x + off, out + off, w, ld, ep, s);
         off += w;
     }
     if (L.n_long > 0) {
         hipLaunchKernelGGL((k_spmm_long<T>), dim3(L.n_long), dim3(256), 0, s, L.rowptr, L.col,
-                           val, x, out, z, ld, k, L.long_rows, s_l, s_x, s_z);
+                           val, x, out, z, ld, k, L.long_rows, s_l, s_x, s_z, rseg, coef);
         SI_LAUNCH_CHECK();
     }
     if (ev0) {
@@ -341,6 +372,8 @@ void spmm_impl(const DevLaplacian& L, const T* val, const T* x, T* out, int k, i
 }  // namespace
 
 void spmm_profile_attach(SpmmProfile* p) { g_profile = p; }
+
+bool spmm_profiling() { return g_profile && g_profile->enabled; }
 
 void spmm_profile_collect(SpmmProfile* p) {
     if (!p) return;
@@ -373,6 +406,18 @@ void spmm_f32(const DevLaplacian& L, const float* x, float* out, int k, int ld, 
 void spmm_f64(const DevLaplacian& L, const double* x, double* out, int k, int ld, double s_l,
               double s_x, double s_z, const double* z, hipStream_t s) {
     spmm_impl<double>(L, L.l64, x, out, k, ld, s_l, s_x, s_z, z, s);
+}
+
+void spmm_seg_f32(const DevLaplacian& L, const float* x, float* out, int k, int ld,
+                  const float* z, const int32_t* rseg, const float* coef, hipStream_t s) {
+    SI_CHECK(rseg && coef, kErrInvalidArgument, "spmm_seg: segment map and scalars required");
+    spmm_impl<float>(L, L.a32, x, out, k, ld, 1.f, 0.f, 0.f, z, s, rseg, coef);
+}
+
+void spmm_seg_f64(const DevLaplacian& L, const double* x, double* out, int k, int ld,
+                  const double* z, const int32_t* rseg, const double* coef, hipStream_t s) {
+    SI_CHECK(rseg && coef, kErrInvalidArgument, "spmm_seg: segment map and scalars required");
+    spmm_impl<double>(L, L.l64, x, out, k, ld, 1.0, 0.0, 0.0, z, s, rseg, coef);
 }
 
 }  // namespace si

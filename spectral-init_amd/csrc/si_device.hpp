@@ -53,6 +53,11 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
                             hipStream_t s, bool reorder = false,
                             const int64_t* h_indptr = nullptr, const int32_t* h_indices = nullptr,
                             const float* h_w = nullptr);
+// Device Laplacian from host FP64 off-diagonal values (CSR without the
+// diagonal; unit diagonal implied) and FP64 degrees; input order is kept.
+void laplacian_from_values_device(DevLaplacian& L, int64_t n, const int64_t* h_ip,
+                                  const int32_t* h_ix, const double* h_l, const double* h_deg,
+                                  hipStream_t s);
 void laplacian_free(DevLaplacian& L);
 // perm[new] = old of a symmetric CSR pattern (reverse Cuthill-McKee).
 std::vector<int32_t> rcm_order(int64_t n, const int64_t* ip, const int32_t* ix);
@@ -69,7 +74,16 @@ void spmm_f32(const DevLaplacian& L, const float* x, float* out, int k, int ld, 
               float s_x, float s_z, const float* z, hipStream_t s);
 void spmm_f64(const DevLaplacian& L, const double* x, double* out, int k, int ld, double s_l,
               double s_x, double s_z, const double* z, hipStream_t s);
+// Segmented form (a block-diagonal L whose diagonal blocks are row segments):
+// row i uses (s_l, s_x, s_z) = coef[3 * rseg[i] + {0, 1, 2}] (device arrays);
+// a segment with s_l == 0 copies s_x * x + s_z * z without gathering.
+void spmm_seg_f32(const DevLaplacian& L, const float* x, float* out, int k, int ld,
+                  const float* z, const int32_t* rseg, const float* coef, hipStream_t s);
+void spmm_seg_f64(const DevLaplacian& L, const double* x, double* out, int k, int ld,
+                  const double* z, const int32_t* rseg, const double* coef, hipStream_t s);
 bool spmm_width_supported(int k);
+// true while an SpmmProfile is collecting (SpMM calls then record events)
+bool spmm_profiling();
 
 // Optional live timing of every SpMM launch (hipEvents on the launching
 // stream), used by the benchmark to measure the dominant kernel inside a
@@ -107,6 +121,34 @@ void rr_update_f64(int64_t n, const double* y, const double* ly, int ld, int m, 
                    const double* theta, int k, double* x, double* ax, double* norms2, double* ws,
                    hipStream_t s);
 size_t rr_update_workspace_doubles(int64_t n, int k);
+
+// ----------------------------------------------------------------- segmented block kernels
+// A block-diagonal operator's rows split into segments; a work list of nblk row
+// ranges [blk_rows[2b], blk_rows[2b+1]) each inside segment blk_seg[b], grouped
+// by segment: segment c owns work items [seg_blk[c], seg_blk[c+1]).  All arrays
+// are device arrays.  Gram work items may hold any number of rows; update tiles
+// hold at most 64 (FP32) or 32 (FP64) rows.
+struct SegWork {
+    int nseg = 0;
+    int64_t nblk = 0;
+    const int64_t* blk_rows = nullptr;
+    const int32_t* blk_seg = nullptr;
+    const int32_t* seg_blk = nullptr;
+};
+// per segment c: G_c = Y_c^T [Y_c | B2_c]  (p x 2p each, g = [nseg x p x 2p]);
+// workspace: nblk * 2 p^2 doubles
+void gram2_seg_f32(const SegWork& blocks, const float* a, int ld, int p, const float* b2, int ld2,
+                   double* g, double* ws, hipStream_t s);
+void gram2_seg_f64(const SegWork& blocks, const double* a, int ld, int p, const double* b2,
+                   int ld2, double* g, double* ws, hipStream_t s);
+// per segment c: x_c = y_c C_c, ax_c = ly_c C_c (C_c = c + c*m*m, theta + c*m),
+// norms2 [nseg x 2k]; workspace: nblk * 2k doubles
+void rr_update_seg_f32(const SegWork& tiles, const float* y, const float* ly, int ld, int m,
+                       const double* c, const double* theta, int k, float* x, float* ax,
+                       double* norms2, double* ws, hipStream_t s);
+void rr_update_seg_f64(const SegWork& tiles, const double* y, const double* ly, int ld, int m,
+                       const double* c, const double* theta, int k, double* x, double* ax,
+                       double* norms2, double* ws, hipStream_t s);
 // G = A^T [B1 | B2]  (p x (q1+q2), q1 + q2 <= 64) in one pass over the rows.
 void gram2_f32(int64_t n, const float* a, int lda, int p, const float* b, int ldb, int q1,
                const float* b2, int ldb2, int q2, double* g, double* ws, hipStream_t s);
@@ -165,6 +207,15 @@ void small_gen_eig(int n, const double* a, const double* b, double* w, double* c
 // returned vectors apply to the unscaled problem.
 void small_gen_eig_scaled(int n, const double* a, int lda, const double* b, int ldb, double* w,
                           double* c, int* status, hipStream_t s);
+// count problems of size n at strides sa, sb (doubles); w [count x n],
+// c [count x n x n], status [count]
+void small_gen_eig_scaled_batched(int count, int n, const double* a, int lda, int64_t sa,
+                                  const double* b, int ldb, int64_t sb, double* w, double* c,
+                                  int* status, hipStream_t s);
+// dense EVD of count diagonal blocks (<= 64 rows each) of L: device arrays
+// seg_row0 [count], seg_n [count]; evals [count x k]; vectors into rows of evecs
+void dense_seg_eig(const DevLaplacian& L, int count, const int64_t* seg_row0, const int32_t* seg_n,
+                   int k, double* evals, double* evecs, int ldv, hipStream_t s);
 // Given G = X^T X (n x n), writes T = L^-T (upper) so that X T is orthonormal;
 // status[0] = 1 when G is not positive definite.
 void small_chol_inv(int n, const double* g, double* t, int* status, hipStream_t s);

@@ -49,6 +49,7 @@ enum WsSlot : int {
     kWsSmall0, kWsSmall1, kWsSmall2, kWsSmall3, kWsSmall4, kWsSmall5, kWsSmall6, kWsSmall7,
     kWsPlainS, kWsPlainAS, kWsPlainR, kWsPlainP, kWsPlainAP, kWsPlainPn, kWsPlainAPn,
     kWsPlainX, kWsPlainT, kWsPlainXr, kWsPlainBest, kWsStatus,
+    kWsBatchSmall, kWsBatchCoef, kWsBatchRr,
     kWsNumSlots
 };
 void* workspace_get(int slot, size_t bytes);
@@ -135,6 +136,26 @@ bool lobpcg_solve_plain(const DevLaplacian& L, int n_components, uint64_t seed, 
 // FP32-bulk Chebyshev-filtered block solver; FP64 Rayleigh-Ritz and residual.
 bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const FastOptions& opt,
                 double* evals_host, double* evecs_dev, SolveStats& st, hipStream_t s);
+
+// Batched FP32-bulk solve of the diagonal blocks of a block-diagonal L (the
+// connected components of a graph, stored as contiguous row segments
+// [row0[c], row0[c] + rown[c]), each with more than 64 rows).  One launch per
+// filter step / Gram / Rayleigh-Ritz update serves every segment; each segment
+// has its own filter interval and degree.  evals_host [nseg x k]; evecs_dev
+// [L.n x k] receives each segment's vectors in its rows (rows outside every
+// segment are overwritten with unspecified values); resid [nseg] the FP64
+// residual of each segment's pairs; ok[c] = 0 if segment c did not reach tol.
+struct BatchStats {
+    int rounds = 0;
+    int64_t spmm_calls = 0;
+    int graphs_captured = 0;
+    int graph_launches = 0;
+};
+void fast_solve_segments(const DevLaplacian& L, const std::vector<int64_t>& row0,
+                         const std::vector<int64_t>& rown, int n_components, uint64_t seed,
+                         const FastOptions& opt, bool use_graphs, double* evals_host,
+                         double* evecs_dev, std::vector<double>& resid, std::vector<int>& ok,
+                         BatchStats& st, hipStream_t s);
 
 // chebyshev_precond (lobpcg.rs:113-150) in FP64 on device blocks.
 void chebyshev_filter_f64(const DevLaplacian& L, const double* r, int k, double a, double b,

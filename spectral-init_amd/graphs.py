@@ -165,6 +165,68 @@ def band_graph(n, per_row, half_band, seed):
     return a.indptr.astype(np.int64), a.indices.astype(np.int32), a.data.astype(np.float32)
 
 
+def largest_connected(indptr, indices, data):
+    """Restrict a CSR graph to its largest connected component (node order kept)."""
+    import scipy.sparse as sp
+    import scipy.sparse.csgraph as cg
+
+    n = indptr.size - 1
+    a = sp.csr_matrix((data, indices, indptr), shape=(n, n))
+    nc, lab = cg.connected_components(a, directed=False)
+    if nc == 1:
+        return indptr, indices, data, np.arange(n)
+    keep = np.flatnonzero(lab == np.bincount(lab).argmax())
+    sub = a[keep][:, keep].tocsr()
+    sub.sort_indices()
+    return (sub.indptr.astype(np.int64), sub.indices.astype(np.int32),
+            sub.data.astype(np.float32), keep)
+
+
+def components_graph(n_comp, lo, hi, dim=30, seed=0, device=None, clusters=3,
+                     center_scale=0.8, spread=10.0, tiny=()):
+    """Block-diagonal graph of independent components: component c has a size
+    drawn log-uniformly from [lo, hi], Gaussian-mixture points in `dim`-d
+    (`clusters` clusters, centres ~ N(0, center_scale^2 I)) around an offset
+    ~ N(0, spread^2 I), and its own exact 15-NN fuzzy graph, restricted to its
+    largest connected component.  `tiny` adds components of the given sizes
+    (2..15 nodes) as small cliques.  Returns the CSR triple, the points
+    (n x dim, for the meta-embedding) and the component sizes."""
+    rng = np.random.default_rng(20_000 + seed)
+    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), n_comp)).astype(np.int64)
+    ips, ixs, ws, pts, out_sizes = [], [], [], [], []
+    off = 0    # node offset
+    noff = 0   # stored-entry offset
+    for c, sz in enumerate(sizes):
+        x, _ = gmm_points(int(sz), dim, np.ones(clusters), seed * 1000 + c, center_scale=center_scale)
+        x = x + rng.standard_normal(dim).astype(np.float32) * spread
+        ip, ix, w = fuzzy_knn_graph(x, 15, device=device)
+        ip, ix, w, keep = largest_connected(ip, ix, w)
+        ips.append(ip[1:] + noff if ips else ip + noff)
+        ixs.append(ix + off)
+        ws.append(w)
+        pts.append(x[keep])
+        off += keep.size
+        noff += int(ip[-1])
+        out_sizes.append(int(keep.size))
+    for t in tiny:
+        t = int(t)
+        x = rng.standard_normal((t, dim)).astype(np.float32) * 0.1 + rng.standard_normal(dim).astype(np.float32) * spread
+        rows, cols = np.nonzero(~np.eye(t, dtype=bool))
+        ip = np.zeros(t + 1, np.int64)
+        ip[1:] = np.cumsum(np.bincount(rows, minlength=t))
+        ips.append(ip[1:] + noff if ips else ip + noff)
+        ixs.append((cols + off).astype(np.int32))
+        ws.append(np.full(rows.size, 0.5, np.float32))
+        pts.append(x)
+        off += t
+        noff += int(rows.size)
+        out_sizes.append(t)
+    indptr = np.concatenate(ips).astype(np.int64)
+    indices = np.concatenate(ixs).astype(np.int32)
+    data = np.concatenate(ws).astype(np.float32)
+    return indptr, indices, data, np.concatenate(pts).astype(np.float32), out_sizes
+
+
 # ----------------------------------------------------------------- workloads
 def workload(name, seed=0, device=None, n=None):
     """Seeded inputs of the named workloads.
@@ -172,6 +234,7 @@ def workload(name, seed=0, device=None, n=None):
     blobs2k   : n=2000, 10 clusters in 50-d, equal sizes (CPU-runnable case)
     gmm100k   : n=100 000, 20 clusters in 50-d, Dirichlet(2) sizes
     gmm1m     : n=1 000 000, 40 clusters in 30-d, Zipf(1.2) sizes, 5 % background
+    comp64    : 64 independent 30-d GMM 15-NN components, sizes log-uniform in [50, 200 000]
     n overrides the node count (same construction, for reduced-size runs).
     """
     rng = np.random.default_rng(10_000 + seed)
@@ -188,6 +251,14 @@ def workload(name, seed=0, device=None, n=None):
         sizes = 1.0 / np.arange(1, 41) ** 1.2
         x, lab = gmm_points(n, 30, sizes, seed, center_scale=CENTER_SCALE["gmm1m"],
                             background_frac=0.05)
+    elif name == "comp64":
+        # 64 disconnected components, sizes log-uniform in [50, 200 000] (~2M nodes)
+        # component offsets ~N(0, 0.5^2 I): centroid affinities exp(-d^2) stay well
+        # inside the FP64 range, so the spectral meta-embedding is well posed
+        indptr, indices, data, x, sizes = components_graph(64, 50, 200_000, dim=30, seed=seed,
+                                                           device=device, spread=0.5)
+        return dict(name=name, n=indptr.size - 1, x=x, labels=None, indptr=indptr,
+                    indices=indices, data=data, sizes=sizes)
     elif name == "band1m":
         # SpMM locality reference (not a solver workload): symmetric random band
         # graph, 12 neighbours per row within +-512 rows (~24 stored per row)

@@ -39,7 +39,8 @@ EXPORTED = [
     "si_lobpcg_solve", "si_solve_eigenproblem", "si_rsvd_solve", "si_dense_evd",
     "si_spectral_init",
     "si_test_gram", "si_test_block_mul", "si_test_small_sym_eig", "si_test_small_gen_eig",
-    "si_test_fill_normal", "si_test_fast_solve", "si_profile_spmm_begin", "si_profile_spmm_end",
+    "si_test_fill_normal", "si_test_fast_solve", "si_test_batch_graphs",
+    "si_profile_spmm_begin", "si_profile_spmm_end",
 ]
 
 
@@ -62,7 +63,8 @@ class Solver(enum.IntEnum):
 class _Config(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("compute_mode", C.c_int32), ("solver", C.c_int32),
                 ("block_size", C.c_int32), ("cheb_degree", C.c_int32), ("max_outer", C.c_int32),
-                ("tol", C.c_double), ("dense_threshold", C.c_int64), ("warm_start", C.c_void_p)]
+                ("tol", C.c_double), ("dense_threshold", C.c_int64), ("warm_start", C.c_void_p),
+                ("batch_components", C.c_int32)]
 
 
 class _Report(C.Structure):
@@ -83,6 +85,7 @@ class SpectralInitConfig:
     tol: float = 1e-5
     dense_threshold: int = 2000
     warm_start: Optional[int] = None  # device address of an [n x (n_components+1)] f64 block
+    batch_components: bool = True     # disconnected graphs: one batched device solve
 
     def _c(self) -> _Config:
         c = _Config()
@@ -95,6 +98,7 @@ class SpectralInitConfig:
         c.tol = float(self.tol)
         c.dense_threshold = int(self.dense_threshold)
         c.warm_start = self.warm_start
+        c.batch_components = 1 if self.batch_components else 0
         return c
 
 
@@ -479,3 +483,13 @@ class SpmmProfiler:
         self.result = dict(launches=p.launches, bytes=p.bytes, ms=p.ms,
                            launches_f32=p.launches_f32, bytes_f32=p.bytes_f32, ms_f32=p.ms_f32)
         return False
+
+
+def batch_graphs(enable: Optional[bool] = None):
+    """Testing seam: switch hipGraph capture of batched component solves
+    (None leaves it) and return (graphs captured, graph launches) of the last
+    batched solve."""
+    cap, lau = C.c_int64(0), C.c_int64(0)
+    lib().si_test_batch_graphs(-1 if enable is None else int(bool(enable)), C.byref(cap),
+                               C.byref(lau))
+    return cap.value, lau.value

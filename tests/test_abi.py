@@ -47,6 +47,7 @@ def test_version_and_default_config_without_gpu(library):
     assert c.struct_size == ctypes.sizeof(si._Config)
     assert c.compute_mode == 0 and c.tol == 1e-5 and c.dense_threshold == 2000
     assert c.cheb_degree == 0
+    assert c.batch_components == 1
 
 
 def test_context_create_fails_cleanly_without_a_device():
