@@ -76,6 +76,70 @@ def cpu_baseline(lap, si, n_components, plain_iters, budget_s):
                 projected_ms_to_eigvecs=round(ms_it * plain_iters, 1) if plain_iters else None)
 
 
+def bench_components(args, torch, graphs, si):
+    """64 disconnected components (sizes log-uniform in [50, 200 000], each an
+    independent 30-d GMM 15-NN graph): spectral_init with all components in one
+    batched device solve vs one solve per component (the reference's loop)."""
+    t0 = time.perf_counter()
+    if args.graph_cache and os.path.exists(args.graph_cache):
+        z = np.load(args.graph_cache)
+        ip, ix, wt, x, sizes = z["indptr"], z["indices"], z["data"], z["x"], list(z["sizes"])
+    else:
+        w = graphs.workload("comp64", seed=args.seed, device="cuda")
+        ip, ix, wt, x, sizes = w["indptr"], w["indices"], w["data"], w["x"], w["sizes"]
+        if args.graph_cache:
+            np.savez(args.graph_cache, indptr=ip, indices=ix, data=wt, x=x, sizes=np.array(sizes))
+    n, nnz = ip.size - 1, int(ip[-1])
+    log(f"graph: n={n} nnz={nnz} components={len(sizes)} sizes {min(sizes)}..{max(sizes)} "
+        f"in {time.perf_counter() - t0:.1f}s")
+    ctx = si.Context(0)
+    res = {}
+    for batch in (True, False):
+        cfg = si.SpectralInitConfig(tol=args.tol, batch_components=batch)
+        runs = []
+        for i in range(args.warmup + args.steps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            out, rep = si.spectral_init((ip, ix, wt), args.n_components, 42, data=x, config=cfg,
+                                        ctx=ctx, return_report=True)
+            dt = 1e3 * (time.perf_counter() - t)
+            if i >= args.warmup:
+                runs.append(dict(end_to_end_ms=dt, solve_ms=rep.solve_ms, laplacian_ms=rep.laplacian_ms,
+                                 rounds=rep.iterations, spmm_calls=rep.spmm_calls,
+                                 max_residual=rep.max_residual))
+            log(f"batch={batch} run {i}: {dt:.1f} ms end-to-end, solve {rep.solve_ms:.2f} ms, "
+                f"laplacian {rep.laplacian_ms:.1f} ms, residual {rep.max_residual:.2e}")
+        res[batch] = (runs, out)
+    cap, launches = si.batch_graphs()
+    med = lambda runs, key: float(np.median([r[key] for r in runs]))
+    bs, ls = med(res[True][0], "solve_ms"), med(res[False][0], "solve_ms")
+    out = {
+        "metric": "ms to eigvecs of 64 components @ residual<=%g (batched device solve)" % args.tol,
+        "value": round(bs, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(bs, 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (64 independent seeded 30-d GMM exact 15-NN fuzzy graphs)",
+        "config": {"workload": "comp64", "n": n, "nnz": nnz, "components": len(sizes),
+                   "n_components": args.n_components, "tol": args.tol},
+        "batched": {"solve_ms": bs, "end_to_end_ms": med(res[True][0], "end_to_end_ms"),
+                    "laplacian_ms": med(res[True][0], "laplacian_ms"),
+                    "graphs_captured": cap, "graph_launches": launches, "runs": res[True][0]},
+        "per_component_loop": {"solve_ms": ls, "end_to_end_ms": med(res[False][0], "end_to_end_ms"),
+                               "laplacian_ms": med(res[False][0], "laplacian_ms"),
+                               "runs": res[False][0]},
+        "speedup_solve_vs_loop": round(ls / bs, 2),
+        "max_abs_coord_diff_vs_loop": float(np.abs(res[True][1] - res[False][1]).max()),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def read_traffic(path, workload, block):
     """HBM bytes per launch of the dominant SpMM, from a committed rocprofv3
     --pmc summary (lines "workload: ...", "block: ...", "hbm_bytes_per_launch: ...").
@@ -101,7 +165,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="gmm1m", choices=["gmm1m", "gmm100k", "blobs2k", "band1m"])
+    ap.add_argument("--workload", default="gmm1m",
+                    choices=["gmm1m", "gmm100k", "blobs2k", "band1m", "comp64"])
     ap.add_argument("--n", type=int, default=None, help="override the node count")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--n-components", type=int, default=2)
@@ -141,6 +206,9 @@ def main():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
     si.lib()  # fails loudly if the native library is missing
+
+    if args.workload == "comp64":
+        return bench_components(args, torch, graphs, si)
 
     t0 = time.perf_counter()
     if args.graph_cache and os.path.exists(args.graph_cache):

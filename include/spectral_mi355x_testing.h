@@ -43,6 +43,12 @@ si_status si_test_fast_solve(si_context* ctx, const si_laplacian* lap, int32_t n
  * the last batched solve. */
 si_status si_test_batch_graphs(int32_t enable, int64_t* captured, int64_t* launches);
 
+/* FP32 SpMM shape for 16 / 32 columns (0 .. 3, see k_spmm.hip): autotune != 0
+ * times every candidate on this graph and keeps the fastest, otherwise the
+ * given shapes are set; out16 / out32 (may be NULL) receive the shapes used. */
+si_status si_test_spmm_shapes(si_context* ctx, si_laplacian* lap, int32_t shape16,
+                              int32_t shape32, int32_t autotune, int32_t* out16, int32_t* out32);
+
 /* Live timing of every SpMM launch on the context's stream (hipEvents around
  * each launch) between begin and end; bytes are the algorithmic bytes of
  * each launch (nnz*(value+index) + row pointers + x read + y write). */

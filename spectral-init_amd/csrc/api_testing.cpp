@@ -119,6 +119,8 @@ si_status si_test_fast_solve(si_context* ctx, const si_laplacian* lap, int32_t n
         si::SolveStats st;
         const int k = n_components + 1;
         hipStream_t s = ctx->c.stream;
+        si::DevLaplacian& ml = const_cast<si::DevLaplacian&>(lap->L);
+        if (!ml.tuned && ml.nnz >= (int64_t)1 << 20) si::spmm_autotune(ml, s);
         si::DevBuf<double> vi((size_t)lap->L.n * k), wsi;
         if (o.warm_start) {
             wsi.alloc((size_t)lap->L.n * k);
@@ -173,6 +175,25 @@ si_status si_test_batch_graphs(int32_t enable, int64_t* captured, int64_t* launc
     if (launches) *launches = si::g_last_batch.graph_launches;
     if (enable >= 0) si::g_batch_graphs = enable != 0;
     return SI_OK;
+}
+
+si_status si_test_spmm_shapes(si_context* ctx, si_laplacian* lap, int32_t shape16,
+                              int32_t shape32, int32_t autotune, int32_t* out16, int32_t* out32) {
+    if (!ctx || !lap) return SI_ERR_INVALID_ARGUMENT;
+    return tguard(ctx, [&] {
+        const int nc = si::spmm_shape_count();
+        if (autotune) {
+            si::spmm_autotune(lap->L, ctx->c.stream);
+        } else {
+            SI_CHECK(shape16 >= 0 && shape16 < nc && shape32 >= 0 && shape32 < nc,
+                     si::kErrInvalidArgument, "spmm shape out of range");
+            lap->L.shape16 = shape16;
+            lap->L.shape32 = shape32;
+            lap->L.tuned = 1;
+        }
+        if (out16) *out16 = lap->L.shape16;
+        if (out32) *out32 = lap->L.shape32;
+    });
 }
 
 }  // extern "C"

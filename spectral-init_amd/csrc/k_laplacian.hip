@@ -13,7 +13,10 @@
 // (laplacian.rs:70-80); W's own diagonal is dropped and L's unit diagonal is
 // implied (laplacian.rs:83-85).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -151,6 +154,15 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
     L.n = n;
     L.nnz = nnz;
     L.degree_mode = mode;
+    auto tp = std::chrono::steady_clock::now();
+    auto phase = [&](const char* what) {  // SI_TRACE=1: build phase timings
+        if (!trace_on()) return;
+        SI_HIP(hipStreamSynchronize(s));
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[build] %-24s %8.1f ms\n", what,
+                     std::chrono::duration<double, std::milli>(now - tp).count());
+        tp = now;
+    };
     const int T = 256;
     const int64_t gn = ceil_div(n, T);
     SI_HIP(hipMalloc(&L.rowptr, sizeof(int32_t) * (n + 1)));
@@ -188,6 +200,7 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
         throw Error(kErrInvalidGraph, "invalid graph: weights must be finite and non-negative");
     }
     L.max_row_len = flags[kNumFlags];
+    phase("validate");
     int sorted = flags[kFlagUnsorted] == 0;
     if (sorted) {
         hipLaunchKernelGGL(k_check_symmetric, dim3(gn), dim3(T), 0, s, L.rowptr, L.col, d_w, n,
@@ -197,6 +210,7 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
         SI_HIP(hipStreamSynchronize(s));
     }
     L.symmetric = sorted && flags[kFlagAsym] == 0;
+    phase("symmetry check");
 
     if (mode == 0) {
         if (L.symmetric) {
@@ -220,6 +234,7 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
     }
     hipLaunchKernelGGL(k_isd, dim3(gn), dim3(T), 0, s, L.deg, n, L.sqrt_deg, L.isd);
     SI_LAUNCH_CHECK();
+    phase("degrees");
     const float* w_vals = d_w;
     float* d_w2 = nullptr;
     if (reorder && n > 1) {
@@ -242,25 +257,39 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
             SI_HIP(hipStreamSynchronize(s));
             hp = hp_buf.data(); hx = hx_buf.data(); hw = hw_buf.data();
         }
+        phase("host copy");
         std::vector<int32_t> perm = rcm_order(n, hp, hx);
+        phase("rcm order");
         std::vector<int32_t> iperm(n);
         for (int64_t i = 0; i < n; ++i) iperm[perm[i]] = (int32_t)i;
         std::vector<int32_t> rp2(n + 1), c2(nnz);
         std::vector<float> w2(nnz);
-        std::vector<std::pair<int32_t, float>> row;
         rp2[0] = 0;
-        for (int64_t i = 0; i < n; ++i) {
-            const int64_t v = perm[i];
-            row.clear();
-            for (int64_t p = hp[v]; p < hp[v + 1]; ++p) row.emplace_back(iperm[hx[p]], hw[p]);
-            std::sort(row.begin(), row.end(),
-                      [](const std::pair<int32_t, float>& x, const std::pair<int32_t, float>& y) {
-                          return x.first < y.first;
-                      });
-            int32_t o = rp2[i];
-            for (const auto& e : row) { c2[o] = e.first; w2[o] = e.second; ++o; }
-            rp2[i + 1] = o;
-        }
+        for (int64_t i = 0; i < n; ++i)
+            rp2[i + 1] = rp2[i] + (int32_t)(hp[perm[i] + 1] - hp[perm[i]]);
+        // rows are independent: renumber and sort them on host threads
+        auto fill = [&](int64_t lo, int64_t hi) {
+            std::vector<std::pair<int32_t, float>> row;
+            for (int64_t i = lo; i < hi; ++i) {
+                const int64_t v = perm[i];
+                row.clear();
+                for (int64_t p = hp[v]; p < hp[v + 1]; ++p) row.emplace_back(iperm[hx[p]], hw[p]);
+                std::sort(row.begin(), row.end(),
+                          [](const std::pair<int32_t, float>& x, const std::pair<int32_t, float>& y) {
+                              return x.first < y.first;
+                          });
+                int32_t o = rp2[i];
+                for (const auto& e : row) { c2[o] = e.first; w2[o] = e.second; ++o; }
+            }
+        };
+        const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(
+            std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency())), n / 4096));
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nth; ++t)
+            pool.emplace_back(fill, n * t / nth, n * (t + 1) / nth);
+        fill(0, n / nth);
+        for (auto& th : pool) th.join();
+        phase("permuted csr (host)");
         SI_HIP(hipMalloc(&d_w2, sizeof(float) * (nnz > 0 ? nnz : 1)));
         SI_HIP(hipMalloc(&L.perm, sizeof(int32_t) * n));
         SI_HIP(hipMalloc(&L.iperm, sizeof(int32_t) * n));
@@ -281,6 +310,7 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
         SI_HIP(hipStreamSynchronize(s));
         hipFree(tmp);
         w_vals = d_w2;
+        phase("upload permuted");
     }
     hipLaunchKernelGGL(k_values, dim3(gn), dim3(T), 0, s, L.rowptr, L.col, w_vals, L.isd, n, L.l64,
                        L.a32, L.long_rows, d_nlong);
@@ -289,6 +319,7 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
     SI_HIP(hipMemcpyAsync(&nlong, d_nlong, sizeof(int), hipMemcpyDeviceToHost, s));
     SI_HIP(hipStreamSynchronize(s));
     L.n_long = nlong;
+    phase("values");
     hipFree(d_flags);
     if (d_w2) hipFree(d_w2);
 }

@@ -299,13 +299,30 @@ void spmm_chunk(const DevLaplacian& L, const T* val, const T* x, T* out, int k, 
         case 8: launch_narrow<T, 8, 8>(L, val, x, out, ld, ep, s); break;
         // (SLOTS, U): SLOTS*U nonzero positions per row per pass; ~32 fits the
         // 15-30 entries of k-NN rows with few idle lanes (measured on MI355X)
+        // FP32 16/32 columns: the shape chosen for this graph by spmm_autotune
         case 16:
-            if constexpr (F32) launch_wide<T, 16, 4, 4>(L, val, x, out, ld, ep, s);   // C=4, RPW=4
-            else launch_wide<T, 16, 4, 8>(L, val, x, out, ld, ep, s);       // C=8, RPW=2
+            if constexpr (F32) {
+                switch (L.shape16) {
+                    case 1: launch_wide<T, 16, 8, 4>(L, val, x, out, ld, ep, s); break;   // RPW=2
+                    case 2: launch_wide<T, 16, 4, 8>(L, val, x, out, ld, ep, s); break;   // RPW=4
+                    case 3: launch_wide<T, 16, 2, 8>(L, val, x, out, ld, ep, s); break;   // RPW=8
+                    default: launch_wide<T, 16, 4, 4>(L, val, x, out, ld, ep, s); break;  // C=4, RPW=4
+                }
+            } else {
+                launch_wide<T, 16, 4, 8>(L, val, x, out, ld, ep, s);       // C=8, RPW=2
+            }
             break;
         case 32:
-            if constexpr (F32) launch_wide<T, 32, 4, 4>(L, val, x, out, ld, ep, s);   // C=8, RPW=2
-            else launch_wide<T, 32, 4, 8>(L, val, x, out, ld, ep, s);       // C=16, RPW=1
+            if constexpr (F32) {
+                switch (L.shape32) {
+                    case 1: launch_wide<T, 32, 2, 8>(L, val, x, out, ld, ep, s); break;   // RPW=4
+                    case 2: launch_wide<T, 32, 4, 8>(L, val, x, out, ld, ep, s); break;   // RPW=2
+                    case 3: launch_wide<T, 32, 8, 4>(L, val, x, out, ld, ep, s); break;   // RPW=1
+                    default: launch_wide<T, 32, 4, 4>(L, val, x, out, ld, ep, s); break;  // C=8, RPW=2
+                }
+            } else {
+                launch_wide<T, 32, 4, 8>(L, val, x, out, ld, ep, s);       // C=16, RPW=1
+            }
             break;
         case 64:
             if constexpr (F32) launch_wide<T, 64, 4, 8>(L, val, x, out, ld, ep, s);   // C=16, RPW=1
@@ -397,6 +414,56 @@ void spmm_profile_collect(SpmmProfile* p) {
 }
 
 bool spmm_width_supported(int k) { return k >= 1 && k <= 4096; }
+
+int spmm_shape_count() { return 4; }
+
+void spmm_autotune(DevLaplacian& L, hipStream_t s) {
+    const int64_t n = L.n;
+    const int reps = 3;
+    float* x = nullptr;
+    float* y = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    SpmmProfile* saved = g_profile;
+    g_profile = nullptr;  // tuning launches are not part of a profiled region
+    try {
+        SI_HIP(hipMalloc(&x, sizeof(float) * n * 32));
+        SI_HIP(hipMalloc(&y, sizeof(float) * n * 32));
+        // any finite block: the x row values do not change the access pattern
+        SI_HIP(hipMemsetAsync(x, 0, sizeof(float) * n * 32, s));
+        SI_HIP(hipEventCreate(&e0));
+        SI_HIP(hipEventCreate(&e1));
+        for (int width : {16, 32}) {
+            float best = 0.f;
+            int pick = 0;
+            for (int c = 0; c < spmm_shape_count(); ++c) {
+                (width == 16 ? L.shape16 : L.shape32) = c;
+                spmm_f32(L, x, y, width, width, 1.f, 0.f, 0.f, nullptr, s);  // warm
+                SI_HIP(hipEventRecord(e0, s));
+                for (int r = 0; r < reps; ++r) spmm_f32(L, x, y, width, width, 1.f, 0.f, 0.f, nullptr, s);
+                SI_HIP(hipEventRecord(e1, s));
+                SI_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                SI_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (c == 0 || ms < best) { best = ms; pick = c; }
+            }
+            (width == 16 ? L.shape16 : L.shape32) = pick;
+        }
+        L.tuned = 1;
+    } catch (...) {
+        g_profile = saved;
+        if (e0) hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+        if (x) hipFree(x);
+        if (y) hipFree(y);
+        L.shape16 = L.shape32 = 0;
+        throw;
+    }
+    g_profile = saved;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(x);
+    hipFree(y);
+}
 
 void spmm_f32(const DevLaplacian& L, const float* x, float* out, int k, int ld, float s_l,
               float s_x, float s_z, const float* z, hipStream_t s) {

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -78,5 +79,14 @@ constexpr uint32_t kStreamRsvdOmega = 2u;
 constexpr uint32_t kStreamNoise = 7u;
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// SI_TRACE=1 prints diagnostics to stderr (solver rounds, build phases).
+inline bool trace_on() {
+    static const bool on = [] {
+        const char* v = std::getenv("SI_TRACE");
+        return v && v[0] == '1';
+    }();
+    return on;
+}
 
 }  // namespace si

@@ -39,6 +39,11 @@ struct DevLaplacian {
     int degree_exact = 1;    // PythonCompat degrees reproduce the f32 column-sum order
     int32_t* perm = nullptr;   // perm[new] = old node id (nullptr: identity order)
     int32_t* iperm = nullptr;  // iperm[old] = new node id
+    // FP32 wide-SpMM shapes for 16 and 32 columns (index into the candidate
+    // list of k_spmm.hip; 0 = default), set by spmm_autotune
+    int shape16 = 0;
+    int shape32 = 0;
+    int tuned = 0;
 };
 
 // ----------------------------------------------------------------- laplacian
@@ -84,6 +89,10 @@ void spmm_seg_f64(const DevLaplacian& L, const double* x, double* out, int k, in
 bool spmm_width_supported(int k);
 // true while an SpmmProfile is collecting (SpMM calls then record events)
 bool spmm_profiling();
+// Pick the FP32 16- and 32-column SpMM shapes for this graph by timing every
+// candidate on it (hipEvents, a few launches each); sets L.shape16/32, L.tuned.
+void spmm_autotune(DevLaplacian& L, hipStream_t s);
+int spmm_shape_count();  // candidates per width
 
 // Optional live timing of every SpMM launch (hipEvents on the launching
 // stream), used by the benchmark to measure the dominant kernel inside a

@@ -31,15 +31,6 @@ namespace si {
 
 namespace {
 
-// SI_TRACE=1 prints one line per filter round to stderr (diagnostics only).
-bool trace_on() {
-    static const bool on = [] {
-        const char* v = std::getenv("SI_TRACE");
-        return v && v[0] == '1';
-    }();
-    return on;
-}
-
 int choose_block(int64_t n, int k, int requested) {
     int m = requested;
     // 16 columns: on clustered k-NN graphs (tens of clusters) a 32-wide block

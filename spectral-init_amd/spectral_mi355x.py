@@ -39,7 +39,7 @@ EXPORTED = [
     "si_lobpcg_solve", "si_solve_eigenproblem", "si_rsvd_solve", "si_dense_evd",
     "si_spectral_init",
     "si_test_gram", "si_test_block_mul", "si_test_small_sym_eig", "si_test_small_gen_eig",
-    "si_test_fill_normal", "si_test_fast_solve", "si_test_batch_graphs",
+    "si_test_fill_normal", "si_test_fast_solve", "si_test_batch_graphs", "si_test_spmm_shapes",
     "si_profile_spmm_begin", "si_profile_spmm_end",
 ]
 
@@ -493,3 +493,13 @@ def batch_graphs(enable: Optional[bool] = None):
     lib().si_test_batch_graphs(-1 if enable is None else int(bool(enable)), C.byref(cap),
                                C.byref(lau))
     return cap.value, lau.value
+
+
+def spmm_shapes(lap: "Laplacian", shape16: int = 0, shape32: int = 0, autotune: bool = False):
+    """Testing seam: set (or autotune) the FP32 16/32-column SpMM shapes of a
+    Laplacian; returns the shapes in use."""
+    a, b = C.c_int32(0), C.c_int32(0)
+    lap.ctx._check(lib().si_test_spmm_shapes(lap.ctx.handle, lap.handle, int(shape16),
+                                             int(shape32), int(bool(autotune)), C.byref(a),
+                                             C.byref(b)))
+    return a.value, b.value

@@ -137,6 +137,35 @@ def test_spmm_fp32_within_accumulation_bound(ctx, gname, k):
     assert np.all(np.abs(y - ref) <= bound)
 
 
+@pytest.mark.parametrize("gname", ["random5000", "hubs3000"])
+@pytest.mark.parametrize("k", [16, 32])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+def test_spmm_fp32_every_tuned_shape(ctx, gname, k, shape):
+    """Each FP32 SpMM shape the autotuner can pick meets the same bound."""
+    print(f"case spmm32 shape {shape} {gname} k={k}", flush=True)
+    ip, ix, w = GRAPHS[gname]()
+    lap = si.Laplacian(ctx, ip, ix, w)
+    assert si.spmm_shapes(lap, shape, shape) == (shape, shape)
+    L, _, _ = O.laplacian_from_graph(ip, ix, w)
+    x = np.random.default_rng(200 + k).standard_normal((lap.n, k)).astype(np.float32)
+    ref = O.spmm(L, x.astype(np.float64))
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.empty_like(xd)
+    lap.spmm(xd, yd, si.SI_F32)
+    y = yd.cpu().numpy().astype(np.float64)
+    absd = O.spmm((L[0], L[1], np.abs(L[2])), np.abs(x.astype(np.float64)))
+    rowlen = np.diff(L[0])[:, None].astype(np.float64)
+    bound = (rowlen + 2) * 2.0 ** -24 * absd * 4
+    assert np.all(np.abs(y - ref) <= bound)
+
+
+def test_spmm_autotune_picks_a_candidate(ctx):
+    ip, ix, w = GRAPHS["random5000"]()
+    lap = si.Laplacian(ctx, ip, ix, w)
+    s16, s32 = si.spmm_shapes(lap, autotune=True)
+    assert 0 <= s16 < 4 and 0 <= s32 < 4
+
+
 def test_spmm_respects_leading_dimension(ctx):
     ip, ix, w = random_symmetric_graph(2000, 12, 3)
     lap = si.Laplacian(ctx, ip, ix, w)
