@@ -139,6 +139,17 @@ int or_spectral_init(int64_t n, const int64_t* indptr, const int32_t* indices,
                      int64_t dense_threshold, double lobpcg_tol, int add_noise,
                      float* out);
 
+/* ---- learned warm start (no reference counterpart; oracle_mpnn.c) ---- */
+int64_t or_mpnn_param_count(void);
+void or_mpnn_features(int64_t n, const int64_t* ip, const int32_t* ix, const double* a,
+                      const double* deg, uint64_t seed, double* f);
+void or_mpnn_forward(int64_t n, const int64_t* ip, const int32_t* ix, const double* a,
+                     const double* f, const double* params, double* h, double* av, double* x);
+double or_mpnn_loss(int64_t n, const double* x, const double* v, double* dx);
+void or_mpnn_backward(int64_t n, const int64_t* ip, const int32_t* ix, const double* a,
+                      const double* f, const double* params, const double* h,
+                      const double* av, const double* dx, double* grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -97,11 +97,15 @@ def lib():
 
 
 def _bind_mpnn(L):
-    L.or_mpnn_forward.restype = C.c_double
-    L.or_mpnn_forward.argtypes = [C.c_int64, _i64p, _i32p, _f32p, _f64p, C.c_int, C.c_int,
-                                  C.c_int, C.c_int, _f64p, _f64p, _f64p, C.c_void_p]
-    L.or_mpnn_backward.argtypes = [C.c_int64, _i64p, _i32p, _f32p, _f64p, C.c_int, C.c_int,
-                                   C.c_int, C.c_int, _f64p, _f64p, _f64p, _f64p]
+    L.or_mpnn_param_count.restype = C.c_int64
+    L.or_mpnn_param_count.argtypes = []
+    L.or_mpnn_features.argtypes = [C.c_int64, _i64p, _i32p, _f64p, _f64p, C.c_uint64, _f64p]
+    L.or_mpnn_forward.argtypes = [C.c_int64, _i64p, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p,
+                                  _f64p]
+    L.or_mpnn_loss.restype = C.c_double
+    L.or_mpnn_loss.argtypes = [C.c_int64, _f64p, _f64p, C.c_void_p]
+    L.or_mpnn_backward.argtypes = [C.c_int64, _i64p, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p,
+                                   _f64p, _f64p]
 
 
 def set_threads(n: int) -> None:
@@ -331,3 +335,60 @@ def spectral_init(indptr, indices, data, n_components, seed, raw_data=None, mode
     st = lib().or_spectral_init(n, ip, ix, w, n_components, seed, rawp, nf, mode,
                                 dense_threshold, tol, int(add_noise), out)
     return st, out
+
+
+# ---------------------------------------------------------------- warm-start network
+MPNN_HID, MPNN_NF, MPNN_NOUT = 64, 16, 2
+
+
+def adjacency_from_graph(indptr, indices, data, mode: int = 0):
+    """Off-diagonal normalised adjacency A_hat = I - L as CSR (ip, ix, a) and degrees."""
+    (lip, lix, ld), deg, _ = laplacian_from_graph(indptr, indices, data, mode)
+    keep = lix != np.repeat(np.arange(lip.size - 1), np.diff(lip))
+    a = -ld[keep]
+    counts = np.add.reduceat(keep.astype(np.int64), lip[:-1]) if lix.size else np.zeros(lip.size - 1, np.int64)
+    counts[np.diff(lip) == 0] = 0
+    ip = np.zeros(lip.size, np.int64)
+    ip[1:] = np.cumsum(counts)
+    return (ip, np.ascontiguousarray(lix[keep], np.int32), np.ascontiguousarray(a)), deg
+
+
+def mpnn_param_count() -> int:
+    return int(lib().or_mpnn_param_count())
+
+
+def mpnn_features(adj, deg, seed: int) -> np.ndarray:
+    ip, ix, a = adj
+    n = ip.size - 1
+    f = np.empty((n, MPNN_NF))
+    lib().or_mpnn_features(n, ip, ix, a, np.ascontiguousarray(deg, np.float64), seed, f)
+    return f
+
+
+def mpnn_forward(adj, f, params):
+    ip, ix, a = adj
+    n = ip.size - 1
+    h = np.zeros((4, n, MPNN_HID))
+    av = np.zeros((3, n, MPNN_HID))
+    x = np.empty((n, MPNN_NOUT))
+    lib().or_mpnn_forward(n, ip, ix, a, np.ascontiguousarray(f, np.float64),
+                          np.ascontiguousarray(params, np.float64), h, av, x)
+    return x, h, av
+
+
+def mpnn_loss(x, v, grad: bool = True):
+    x = np.ascontiguousarray(x, np.float64)
+    v = np.ascontiguousarray(v, np.float64)
+    dx = np.empty_like(x) if grad else None
+    loss = lib().or_mpnn_loss(x.shape[0], x, v, dx.ctypes.data if grad else None)
+    return (loss, dx) if grad else loss
+
+
+def mpnn_backward(adj, f, params, h, av, dx):
+    ip, ix, a = adj
+    n = ip.size - 1
+    g = np.empty(mpnn_param_count())
+    lib().or_mpnn_backward(n, ip, ix, a, np.ascontiguousarray(f, np.float64),
+                           np.ascontiguousarray(params, np.float64), h, av,
+                           np.ascontiguousarray(dx, np.float64), g)
+    return g

@@ -77,6 +77,8 @@ __host__ __device__ inline double normal_at(uint64_t seed, uint32_t stream, uint
 constexpr uint32_t kStreamLobpcgInit = 1u;
 constexpr uint32_t kStreamRsvdOmega = 2u;
 constexpr uint32_t kStreamNoise = 7u;
+constexpr uint32_t kStreamProbe = 9u;     // warm-start network: random-walk probes
+constexpr uint32_t kStreamNetInit = 11u;  // warm-start network: initial weights
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 

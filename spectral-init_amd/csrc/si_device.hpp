@@ -204,6 +204,23 @@ void set_column_f64(double* x, int ld, int64_t n, int col, const double* v, doub
 void set_column_f32(float* x, int ld, int64_t n, int col, const double* v, double scale,
                     hipStream_t s);
 
+// ----------------------------------------------------------------- warm-start network
+// out[n x q] = epi(A[n x k] Wa + B[n x k] Wb + bias + C) on the bf16 matrix cores
+// (FP32 accumulation); act 0 none, 1 tanh, 2 multiply by (1 - G^2).  (k, q) in
+// {(16, 64), (64, 64), (64, 16)}; Wa, Wb bf16 [k x q].
+void mpnn_gemm(int64_t n, int k, int q, const float* a, const short* wa, const float* b,
+               const short* wb, const float* bias, const float* c, int act, const float* g,
+               float* out, hipStream_t s);
+void mpnn_dtanh(int64_t count, const float* x, const float* h, float* y, hipStream_t s);
+void mpnn_rowfeat(const DevLaplacian& L, double inv_max_sq, float* f, hipStream_t s);
+void mpnn_setcol(int64_t n, int j, const float* p, float scale, float* f, hipStream_t s);
+void mpnn_deflate(int64_t n, double* p, const double* v, double alpha, float* pf, hipStream_t s);
+void mpnn_dx(int64_t n, const float* x, const double* v, const double cx[4], const double cv[4],
+             float* dx, hipStream_t s);
+void mpnn_pack(const float* w, int k, int q, int transpose, short* out, hipStream_t s);
+void mpnn_adam(int64_t count, float* p, const double* g, float* m, float* v, float lr, float b1,
+               float b2, float eps, int step, hipStream_t s);
+
 // ----------------------------------------------------------------- small dense (one workgroup)
 // All small matrices are FP64 row-major, n <= kSmallMax, resident on the device.
 constexpr int kSmallMax = 64;

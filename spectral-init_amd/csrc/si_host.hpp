@@ -131,7 +131,8 @@ bool rayleigh_ritz_f64(const DevLaplacian& L, double* v, int k, double* evals_ho
 // lobpcg_solve (lobpcg.rs:227-465) on the device in FP64 (the plain baseline).
 bool lobpcg_solve_plain(const DevLaplacian& L, int n_components, uint64_t seed, bool regularize,
                         double tol, double* evals_host, double* evecs_dev, SolveStats& st,
-                        hipStream_t s);
+                        hipStream_t s,
+                        const double* warm_start = nullptr);
 
 // FP32-bulk Chebyshev-filtered block solver; FP64 Rayleigh-Ritz and residual.
 bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const FastOptions& opt,

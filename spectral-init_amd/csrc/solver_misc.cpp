@@ -138,7 +138,8 @@ int solve_eigenproblem_device(const DevLaplacian& L, int n_components, uint64_t 
         fo.tol = opt.tol;
         have = fast_solve(L, n_components, seed, fo, evals_host, evecs_dev, st, s);
     } else {
-        have = lobpcg_solve_plain(L, n_components, seed, false, opt.tol, evals_host, evecs_dev, st, s);
+        have = lobpcg_solve_plain(L, n_components, seed, false, opt.tol, evals_host, evecs_dev, st, s,
+                                  opt.fast.warm_start);
     }
     if (have) {
         // the FP32-bulk solver already measured the FP64 residual of exactly

@@ -177,7 +177,7 @@ bool lobpcg_core_plain(const DevLaplacian& L, double shift, double* x, int m, do
 
 bool lobpcg_solve_plain(const DevLaplacian& L, int n_components, uint64_t seed, bool regularize,
                         double tol, double* evals_host, double* evecs_dev, SolveStats& st,
-                        hipStream_t s) {
+                        hipStream_t s, const double* warm_start) {
     const int64_t n = L.n;
     const int k = n_components + 1;
     st = SolveStats{};
@@ -186,13 +186,18 @@ bool lobpcg_solve_plain(const DevLaplacian& L, int n_components, uint64_t seed, 
     const size_t nk = (size_t)n * k;
     DevBuf<double> x(nk), t(nk), g(1), ws(gram_workspace_doubles(n, k, k)), gk((size_t)k * k),
         sc(k);
-    // N(0,1) start (lobpcg.rs:244-246) and the trivial-vector hint (:249-252)
-    fill_normal_f64(x, n, k, k, seed, kStreamLobpcgInit, s, L.perm);
-    gram_f64(n, L.sqrt_deg, 1, 1, L.sqrt_deg, 1, 1, g, ws, s);
-    double sn2 = 0.0;
-    to_host(&sn2, g.get(), 1, s);
-    const double sn = std::sqrt(sn2);
-    if (sn > 0.0) set_column_f64(x, k, n, 0, L.sqrt_deg, 1.0 / sn, s);
+    if (warm_start) {
+        // a caller's start block takes the place of the random one
+        SI_HIP(hipMemcpyAsync(x.get(), warm_start, sizeof(double) * nk, hipMemcpyDeviceToDevice, s));
+    } else {
+        // N(0,1) start (lobpcg.rs:244-246) and the trivial-vector hint (:249-252)
+        fill_normal_f64(x, n, k, k, seed, kStreamLobpcgInit, s, L.perm);
+        gram_f64(n, L.sqrt_deg, 1, 1, L.sqrt_deg, 1, 1, g, ws, s);
+        double sn2 = 0.0;
+        to_host(&sn2, g.get(), 1, s);
+        const double sn = std::sqrt(sn2);
+        if (sn > 0.0) set_column_f64(x, k, n, 0, L.sqrt_deg, 1.0 / sn, s);
+    }
     if (n >= 1000) {  // ChFSI (lobpcg.rs:270-291)
         chebyshev_filter_f64(L, x, k, 0.1, 2.0, 8, t, s);
         st.spmm_calls += 8;

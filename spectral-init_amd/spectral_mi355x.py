@@ -41,6 +41,9 @@ EXPORTED = [
     "si_test_gram", "si_test_block_mul", "si_test_small_sym_eig", "si_test_small_gen_eig",
     "si_test_fill_normal", "si_test_fast_solve", "si_test_batch_graphs", "si_test_spmm_shapes",
     "si_profile_spmm_begin", "si_profile_spmm_end",
+    "si_mpnn_param_count", "si_mpnn_create", "si_mpnn_destroy", "si_mpnn_get_params",
+    "si_mpnn_set_params", "si_mpnn_features", "si_mpnn_forward", "si_mpnn_gradient",
+    "si_mpnn_train_step", "si_mpnn_warm_start",
 ]
 
 
@@ -198,8 +201,21 @@ def lib():
         L.si_profile_spmm_end.argtypes = [vp, C.POINTER(_SpmmProfile)]
         for name in EXPORTED:
             if name not in ("si_version", "si_last_error", "si_config_default",
-                            "si_context_destroy", "si_laplacian_destroy"):
+                            "si_context_destroy", "si_laplacian_destroy", "si_mpnn_destroy",
+                            "si_mpnn_param_count"):
                 getattr(L, name).restype = C.c_int
+        L.si_mpnn_param_count.restype = C.c_int64
+        L.si_mpnn_destroy.restype = None
+        L.si_mpnn_destroy.argtypes = [C.c_void_p]
+        L.si_mpnn_create.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p]
+        L.si_mpnn_get_params.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+        L.si_mpnn_set_params.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+        L.si_mpnn_features.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
+        L.si_mpnn_forward.argtypes = [C.c_void_p] * 5
+        L.si_mpnn_gradient.argtypes = [C.c_void_p] * 7
+        L.si_mpnn_train_step.argtypes = [C.c_void_p] * 5 + [C.c_float, C.c_void_p]
+        L.si_mpnn_warm_start.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int32,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -503,3 +519,77 @@ def spmm_shapes(lap: "Laplacian", shape16: int = 0, shape32: int = 0, autotune: 
                                              int(shape32), int(bool(autotune)), C.byref(a),
                                              C.byref(b)))
     return a.value, b.value
+
+
+# ---------------------------------------------------------------- learned warm start
+class WarmStartNet:
+    """The learned warm-start network (include/spectral_mi355x_warmstart.h)."""
+
+    def __init__(self, ctx: Context, seed: int = 0, params: Optional[np.ndarray] = None):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx._check(lib().si_mpnn_create(ctx.handle, seed, C.byref(h)))
+        self._h = h
+        if params is not None:
+            self.set_params(params)
+
+    @staticmethod
+    def param_count() -> int:
+        return int(lib().si_mpnn_param_count())
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.param_count(), np.float32)
+        self.ctx._check(lib().si_mpnn_get_params(self.ctx.handle, self._h, out.ctypes.data, out.size))
+        return out
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        self.ctx._check(lib().si_mpnn_set_params(self.ctx.handle, self._h, p.ctypes.data, p.size))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().si_mpnn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def features(lap: "Laplacian", seed: int = 0):
+        import torch
+        f = torch.empty((lap.n, 16), dtype=torch.float32, device="cuda")
+        lap.ctx._check(lib().si_mpnn_features(lap.ctx.handle, lap.handle, seed, f.data_ptr()))
+        return f
+
+    def forward(self, lap: "Laplacian", f):
+        import torch
+        x = torch.empty((lap.n, 2), dtype=torch.float32, device="cuda")
+        self.ctx._check(lib().si_mpnn_forward(self.ctx.handle, self._h, lap.handle, f.data_ptr(),
+                                              x.data_ptr()))
+        return x
+
+    def gradient(self, lap: "Laplacian", f, v):
+        loss = C.c_double(0.0)
+        g = np.empty(self.param_count(), np.float64)
+        self.ctx._check(lib().si_mpnn_gradient(self.ctx.handle, self._h, lap.handle, f.data_ptr(),
+                                               v.data_ptr(), C.byref(loss), g.ctypes.data))
+        return loss.value, g
+
+    def train_step(self, lap: "Laplacian", f, v, lr: float) -> float:
+        loss = C.c_double(0.0)
+        self.ctx._check(lib().si_mpnn_train_step(self.ctx.handle, self._h, lap.handle, f.data_ptr(),
+                                                 v.data_ptr(), float(lr), C.byref(loss)))
+        return loss.value
+
+    def warm_start(self, lap: "Laplacian", seed: int = 0, n_components: int = 2):
+        import torch
+        blk = torch.empty((lap.n, n_components + 1), dtype=torch.float64, device="cuda")
+        used = C.c_int32(0)
+        rn, rr = C.c_double(0.0), C.c_double(0.0)
+        self.ctx._check(lib().si_mpnn_warm_start(self.ctx.handle, self._h, lap.handle, seed,
+                                                 n_components, blk.data_ptr(), C.byref(used),
+                                                 C.byref(rn), C.byref(rr)))
+        return blk, bool(used.value), rn.value, rr.value

@@ -1,0 +1,143 @@
+"""The learned warm start on the GPU against the FP64 oracle
+(oracle/oracle_mpnn.c): features, the bf16 forward pass, the gradients of the
+hand-written backward pass, Adam training and the residual-judged choice
+between the network's block and a random start."""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import random_symmetric_graph
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+si = pytest.importorskip("spectral_mi355x")
+
+BLOCKS = [("Ws1", 16 * 64), ("Wn1", 16 * 64), ("b1", 64), ("Ws2", 64 * 64), ("Wn2", 64 * 64),
+          ("b2", 64), ("Ws3", 64 * 64), ("Wn3", 64 * 64), ("b3", 64), ("Wo", 128)]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    c = si.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def g5000(ctx):
+    ip, ix, w = random_symmetric_graph(5000, 20, 11)   # n >= 4096: locality order inside
+    lap = si.Laplacian(ctx, ip, ix, w)
+    adj, deg = O.adjacency_from_graph(ip, ix, w)
+    return ip, ix, w, lap, adj, deg
+
+
+def test_features_match_oracle(ctx, g5000):
+    print("case warmstart features", flush=True)
+    _, _, _, lap, adj, deg = g5000
+    f = si.WarmStartNet.features(lap, seed=5).cpu().numpy().astype(np.float64)
+    ref = O.mpnn_features(adj, deg, 5)
+    cols = [0, 1, 2, 3, 4, 5, 14, 15]
+    np.testing.assert_allclose(f[:, cols], ref[:, cols], rtol=1e-5, atol=1e-6)
+    # probes: 8 FP32 SpMM steps against FP64
+    np.testing.assert_allclose(f[:, 6:14], ref[:, 6:14], atol=1e-4)
+
+
+def test_forward_matches_oracle(ctx, g5000):
+    print("case warmstart forward", flush=True)
+    _, _, _, lap, adj, deg = g5000
+    net = si.WarmStartNet(ctx, seed=3)
+    p = net.params().astype(np.float64)
+    fo = O.mpnn_features(adj, deg, 5)
+    x = net.forward(lap, torch.from_numpy(fo.astype(np.float32)).cuda()).cpu().numpy()
+    xo, _, _ = O.mpnn_forward(adj, fo.astype(np.float32).astype(np.float64), p)
+    # bf16 operands (8-bit mantissa), FP32 accumulation, three layers
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 2e-2
+
+
+def test_gradient_matches_oracle(ctx, g5000):
+    print("case warmstart gradient", flush=True)
+    _, _, _, lap, adj, deg = g5000
+    net = si.WarmStartNet(ctx, seed=4)
+    p = net.params().astype(np.float64)
+    fo = O.mpnn_features(adj, deg, 5).astype(np.float32)
+    rng = np.random.default_rng(0)
+    v, _ = np.linalg.qr(rng.standard_normal((lap.n, 2)))
+    loss, g = net.gradient(lap, torch.from_numpy(fo).cuda(), torch.from_numpy(v).cuda())
+    xo, h, av = O.mpnn_forward(adj, fo.astype(np.float64), p)
+    lo, dxo = O.mpnn_loss(xo, v)
+    go = O.mpnn_backward(adj, fo.astype(np.float64), p, h, av, dxo)
+    assert abs(loss - lo) <= 2e-2 * abs(lo)
+    o = 0
+    for name, size in BLOCKS:
+        a, b = g[o:o + size], go[o:o + size]
+        o += size
+        cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-300)
+        print(f"  {name}: cos {cos:.5f} rel {np.linalg.norm(a - b) / np.linalg.norm(b):.3e}")
+        assert cos > 0.995, name
+        assert np.linalg.norm(a - b) <= 6e-2 * np.linalg.norm(b), name
+
+
+@pytest.fixture(scope="module")
+def gmm20k(ctx):
+    import graphs
+    w = graphs.workload("gmm100k", seed=5, n=20000)
+    lap = si.Laplacian(ctx, w["indptr"], w["indices"], w["data"])
+    ev, vec, rep = si.solve_eigenproblem(lap, 2, 42, si.SpectralInitConfig(tol=1e-8))
+    assert rep.max_residual <= 1e-8
+    return lap, vec[:, 1:3].contiguous()
+
+
+def test_training_reduces_the_loss(ctx, gmm20k):
+    print("case warmstart training", flush=True)
+    lap, v = gmm20k
+    net = si.WarmStartNet(ctx, seed=1)
+    f = si.WarmStartNet.features(lap, seed=0)
+    losses = [net.train_step(lap, f, v, 3e-3) for _ in range(60)]
+    assert all(np.isfinite(losses))
+    assert np.mean(losses[-5:]) < 0.7 * losses[0], losses[::10]
+
+
+def test_warm_start_is_judged_by_residual(ctx, gmm20k):
+    print("case warmstart judge", flush=True)
+    lap, _ = gmm20k
+    net = si.WarmStartNet(ctx, seed=2)
+    blk, used, rn, rr = net.warm_start(lap, seed=0)
+    assert blk.shape == (lap.n, 3) and torch.isfinite(blk).all()
+    assert used == (rn < rr)
+    # the block is a valid solver start
+    cfg = si.SpectralInitConfig(tol=1e-6, warm_start=blk.data_ptr())
+    ev, vec, rep = si.solve_eigenproblem(lap, 2, 42, cfg)
+    assert rep.max_residual <= 1e-6
+
+
+def test_trained_network_gains_on_held_out_graphs(ctx):
+    """The committed network (spectral-init_amd/warmstart_mpnn.npy, trained on
+    50K-200K-node graphs of seeds 1000-1119) against N(0,1) starts on held-out
+    seeds 5004-5007 at n = 200 000: plain LOBPCG iterations fall on average, the
+    network's block is taken only when its Rayleigh-Ritz residual is lower, and
+    every warm-started solve reaches the same residual gate."""
+    import os
+    import graphs
+    path = os.path.join(os.path.dirname(si.__file__), "warmstart_mpnn.npy")
+    net = si.WarmStartNet(ctx, params=np.load(path))
+    it_rand, it_warm = [], []
+    for seed in range(5004, 5008):
+        print(f"case warmstart gain seed {seed}", flush=True)
+        w = graphs.workload("gmm100k", seed=seed, n=200000, device="cuda")
+        ip, ix, wt, _ = graphs.largest_connected(w["indptr"], w["indices"], w["data"])
+        lap = si.Laplacian(ctx, ip, ix, wt)
+        cfg = si.SpectralInitConfig(tol=1e-6, solver=si.Solver.PlainLobpcg)
+        _, _, rr = si.solve_eigenproblem(lap, 2, 42, cfg)
+        blk, used, res_net, res_rand = net.warm_start(lap, seed=42)
+        assert used == (res_net < res_rand)
+        cfg_w = si.SpectralInitConfig(tol=1e-6, solver=si.Solver.PlainLobpcg,
+                                      warm_start=blk.data_ptr())
+        _, _, rw = si.solve_eigenproblem(lap, 2, 42, cfg_w)
+        assert rr.max_residual <= 1e-6 and rw.max_residual <= 1e-6
+        it_rand.append(rr.iterations)
+        it_warm.append(rw.iterations)
+        print(f"  iterations random {rr.iterations} warm {rw.iterations} (net used {used})", flush=True)
+        del blk, lap
+    assert np.mean(it_warm) < 0.9 * np.mean(it_rand), (it_rand, it_warm)
