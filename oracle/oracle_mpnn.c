@@ -6,7 +6,7 @@
  * (src/solvers/lobpcg.rs:243-252) and this network predicts that block
  * instead.  The functions below define the network's mathematics; the device
  * kernels (bf16 matrix cores, FP32 accumulation) are checked against them, and
- * they are pinned by central finite differences (tests/test_oracle.py).
+ * they are pinned by central finite differences (tests/test_oracle_mpnn.py).
  *
  * Graph operator: A_hat = D^-1/2 W D^-1/2 without its diagonal, i.e. I - L,
  * given as CSR (a_indptr, a_indices, a_data) of the off-diagonal entries.
