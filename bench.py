@@ -190,6 +190,9 @@ def main():
                     help="load the workload graph from this .npz (written on first use)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one solve plus a 1 GiB device copy (scale reference for HBM counters), then exit")
+    ap.add_argument("--spmm-shape16", type=int, default=-1,
+                    help="pin the FP32 16-column SpMM shape (0-3) instead of autotuning; "
+                         "counter runs pin the shape the autotuner picks unprofiled")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -233,6 +236,8 @@ def main():
     torch.cuda.synchronize()
     lap_ms = 1e3 * (time.perf_counter() - t0)
     log(f"laplacian on device: nnz={lap.nnz} hub rows={lap.n_long} ({lap_ms:.1f} ms incl. upload)")
+    if args.spmm_shape16 >= 0:
+        log(f"FP32 16-column SpMM shape pinned to {si.spmm_shapes(lap, args.spmm_shape16, 0)[0]}")
 
     sweep = []
     if args.spmm_sweep or args.spmm_only:

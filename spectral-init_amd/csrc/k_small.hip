@@ -17,6 +17,12 @@ constexpr int N = kSmallMax;
 constexpr int LDA = N + 1;
 constexpr int kThreads = 256;
 
+// Block size of the one-workgroup solvers.  The device code loops over
+// blockDim.x, so any power of two up to kThreads works; four wavefronts
+// measured faster than one even at 16 rows (the column-serial triangular
+// solves dominate, not the barriers).
+inline int small_threads(int /*n*/) { return kThreads; }
+
 // Pair (p, q) of round r for an even number of players m2 (circle method).
 __device__ inline void round_pair(int r, int i, int m2, int& p, int& q) {
     const int m = m2 - 1;
@@ -38,25 +44,25 @@ __device__ void jacobi_lds(double (*a)[LDA], double (*v)[LDA], int n, double* cs
     const int t = threadIdx.x;
     const int m2 = n + (n & 1);
     const int half = m2 / 2;
-    for (int e = t; e < n * n; e += kThreads) v[e / n][e % n] = (e / n == e % n) ? 1.0 : 0.0;
+    for (int e = t; e < n * n; e += (int)blockDim.x) v[e / n][e % n] = (e / n == e % n) ? 1.0 : 0.0;
     __syncthreads();
     for (int sweep = 0; sweep < 60; ++sweep) {
         // convergence test: off-diagonal mass relative to total
         double off = 0.0, tot = 0.0;
-        for (int e = t; e < n * n; e += kThreads) {
+        for (int e = t; e < n * n; e += (int)blockDim.x) {
             const int i = e / n, j = e % n;
             const double q2 = a[i][j] * a[i][j];
             tot += q2;
             if (i != j) off += q2;
         }
         red[t] = off;
-        red[kThreads + t] = tot;
+        red[blockDim.x + t] = tot;
         __syncthreads();
-        for (int s = kThreads / 2; s > 0; s >>= 1) {
-            if (t < s) { red[t] += red[t + s]; red[kThreads + t] += red[kThreads + t + s]; }
+        for (int s = (int)blockDim.x / 2; s > 0; s >>= 1) {
+            if (t < s) { red[t] += red[t + s]; red[blockDim.x + t] += red[blockDim.x + t + s]; }
             __syncthreads();
         }
-        const double offs = red[0], tots = red[kThreads];
+        const double offs = red[0], tots = red[blockDim.x];
         __syncthreads();
         if (offs <= 1e-30 * tots || offs == 0.0 || n < 2) break;
         for (int r = 0; r < m2 - 1; ++r) {
@@ -81,7 +87,7 @@ __device__ void jacobi_lds(double (*a)[LDA], double (*v)[LDA], int n, double* cs
             }
             __syncthreads();
             // rows: A <- J^T A
-            for (int e = t; e < half * n; e += kThreads) {
+            for (int e = t; e < half * n; e += (int)blockDim.x) {
                 const int pi = e / n, j = e % n;
                 const int p = pp[pi], q = qq[pi];
                 if (p == q) continue;
@@ -92,7 +98,7 @@ __device__ void jacobi_lds(double (*a)[LDA], double (*v)[LDA], int n, double* cs
             }
             __syncthreads();
             // columns: A <- A J, V <- V J
-            for (int e = t; e < half * n; e += kThreads) {
+            for (int e = t; e < half * n; e += (int)blockDim.x) {
                 const int pi = e / n, j = e % n;
                 const int p = pp[pi], q = qq[pi];
                 if (p == q) continue;
@@ -129,8 +135,8 @@ __device__ void sort_and_store(double (*a)[LDA], double (*v)[LDA], int n, int* o
         }
     }
     __syncthreads();
-    for (int i = t; i < n; i += kThreads) w[i] = a[order[i]][order[i]];
-    for (int e = t; e < n * n; e += kThreads) {
+    for (int i = t; i < n; i += (int)blockDim.x) w[i] = a[order[i]][order[i]];
+    for (int e = t; e < n * n; e += (int)blockDim.x) {
         const int r = e / n, c = e % n;
         out[r * n + c] = v[r][order[c]];
     }
@@ -150,14 +156,14 @@ __device__ bool cholesky_lds(double (*l)[LDA], int n, int* flag) {
         }
         __syncthreads();
         const double d = l[j][j];
-        for (int i = j + 1 + t; i < n; i += kThreads) {
+        for (int i = j + 1 + t; i < n; i += (int)blockDim.x) {
             double s = l[i][j];
             for (int k = 0; k < j; ++k) s -= l[i][k] * l[j][k];
             l[i][j] = s / d;
         }
         __syncthreads();
     }
-    for (int e = t; e < n * n; e += kThreads)
+    for (int e = t; e < n * n; e += (int)blockDim.x)
         if (e % n > e / n) l[e / n][e % n] = 0.0;
     __syncthreads();
     return *flag == 0;
@@ -172,7 +178,7 @@ __global__ __launch_bounds__(kThreads) void k_sym_eig(int n, const double* __res
     __shared__ int pp[N / 2], qq[N / 2], order[N];
     __shared__ double red[2 * kThreads];
     const int t = threadIdx.x;
-    for (int e = t; e < n * n; e += kThreads) {
+    for (int e = t; e < n * n; e += (int)blockDim.x) {
         const int i = e / n, j = e % n;
         // lower triangle defines the matrix (SelfAdjointEigen, Side::Lower)
         a[i][j] = (j <= i) ? ain[i * lda + j] : ain[j * lda + i];
@@ -214,7 +220,7 @@ __global__ __launch_bounds__(kThreads) void k_gen_eig(int n, const double* __res
         dsc[t] = scale ? (bd > 0.0 ? 1.0 / sqrt(bd) : 0.0) : 1.0;
     }
     __syncthreads();
-    for (int e = t; e < n * n; e += kThreads) {
+    for (int e = t; e < n * n; e += (int)blockDim.x) {
         const int i = e / n, j = e % n;
         const double s = dsc[i] * dsc[j];
         a[i][j] = s * ((j <= i) ? ain[i * lda + j] : ain[j * lda + i]);
@@ -226,7 +232,7 @@ __global__ __launch_bounds__(kThreads) void k_gen_eig(int n, const double* __res
         return;
     }
     // v = L^-1 A  (thread per column, forward substitution)
-    for (int c = t; c < n; c += kThreads)
+    for (int c = t; c < n; c += (int)blockDim.x)
         for (int i = 0; i < n; ++i) {
             double s = a[i][c];
             for (int k = 0; k < i; ++k) s -= l[i][k] * v[k][c];
@@ -234,14 +240,14 @@ __global__ __launch_bounds__(kThreads) void k_gen_eig(int n, const double* __res
         }
     __syncthreads();
     // a = L^-1 v^T  (= L^-1 A L^-T), then symmetrise
-    for (int c = t; c < n; c += kThreads)
+    for (int c = t; c < n; c += (int)blockDim.x)
         for (int i = 0; i < n; ++i) {
             double s = v[c][i];
             for (int k = 0; k < i; ++k) s -= l[i][k] * a[k][c];
             a[i][c] = s / l[i][i];
         }
     __syncthreads();
-    for (int e = t; e < n * n; e += kThreads) {
+    for (int e = t; e < n * n; e += (int)blockDim.x) {
         const int i = e / n, j = e % n;
         if (j < i) {
             const double m = 0.5 * (a[i][j] + a[j][i]);
@@ -252,14 +258,14 @@ __global__ __launch_bounds__(kThreads) void k_gen_eig(int n, const double* __res
     __syncthreads();
     jacobi_lds(a, v, n, cs, sn, pp, qq, red);
     // eigenvalues on the diagonal of a; c = L^-T v, written back into v
-    for (int c = t; c < n; c += kThreads)
+    for (int c = t; c < n; c += (int)blockDim.x)
         for (int i = n - 1; i >= 0; --i) {
             double s = v[i][c];
             for (int k = i + 1; k < n; ++k) s -= l[k][i] * v[k][c];
             v[i][c] = s / l[i][i];
         }
     __syncthreads();
-    for (int e = t; e < n * n; e += kThreads) v[e / n][e % n] *= dsc[e / n];
+    for (int e = t; e < n * n; e += (int)blockDim.x) v[e / n][e % n] *= dsc[e / n];
     __syncthreads();
     sort_and_store(a, v, n, order, w, cout);
     if (t == 0) status[0] = 0;
@@ -272,7 +278,7 @@ __global__ __launch_bounds__(kThreads) void k_chol_inv(int n, const double* __re
     __shared__ double ti[N][LDA];
     __shared__ int flag;
     const int t = threadIdx.x;
-    for (int e = t; e < n * n; e += kThreads) {
+    for (int e = t; e < n * n; e += (int)blockDim.x) {
         const int i = e / n, j = e % n;
         l[i][j] = (j <= i) ? g[i * n + j] : g[j * n + i];
     }
@@ -282,14 +288,14 @@ __global__ __launch_bounds__(kThreads) void k_chol_inv(int n, const double* __re
         return;
     }
     // T = L^-T: solve L^T T[:, c] = e_c (back substitution, thread per column)
-    for (int c = t; c < n; c += kThreads)
+    for (int c = t; c < n; c += (int)blockDim.x)
         for (int i = n - 1; i >= 0; --i) {
             double s = (i == c) ? 1.0 : 0.0;
             for (int k = i + 1; k < n; ++k) s -= l[k][i] * ti[k][c];
             ti[i][c] = s / l[i][i];
         }
     __syncthreads();
-    for (int e = t; e < n * n; e += kThreads) tout[e] = ti[e / n][e % n];
+    for (int e = t; e < n * n; e += (int)blockDim.x) tout[e] = ti[e / n][e % n];
     if (t == 0) status[0] = 0;
 }
 
@@ -350,9 +356,9 @@ __global__ __launch_bounds__(kThreads) void k_dense_seg_eig(
     const int t = threadIdx.x;
     const int64_t r0 = seg_row0[blockIdx.x];
     const int n = seg_n[blockIdx.x];
-    for (int e = t; e < n * n; e += kThreads) a[e / n][e % n] = (e / n == e % n) ? 1.0 : 0.0;
+    for (int e = t; e < n * n; e += (int)blockDim.x) a[e / n][e % n] = (e / n == e % n) ? 1.0 : 0.0;
     __syncthreads();
-    for (int i = t; i < n; i += kThreads)
+    for (int i = t; i < n; i += (int)blockDim.x)
         for (int32_t q = rp[r0 + i]; q < rp[r0 + i + 1]; ++q) {
             const int64_t j = (int64_t)col[q] - r0;
             if (j != i && j >= 0 && j < n) a[i][j] += l64[q];
@@ -370,8 +376,8 @@ __global__ __launch_bounds__(kThreads) void k_dense_seg_eig(
     }
     __syncthreads();
     const int kk = k < n ? k : n;
-    for (int j = t; j < kk; j += kThreads) evals[(int64_t)blockIdx.x * k + j] = a[order[j]][order[j]];
-    for (int e = t; e < n * kk; e += kThreads) {
+    for (int j = t; j < kk; j += (int)blockDim.x) evals[(int64_t)blockIdx.x * k + j] = a[order[j]][order[j]];
+    for (int e = t; e < n * kk; e += (int)blockDim.x) {
         const int r = e / kk, c = e % kk;
         evecs[(r0 + r) * ldv + c] = v[r][order[c]];
     }
@@ -407,14 +413,14 @@ void small_copy_sub(int rows, int cols, const double* src, int lds, double* dst,
 
 void small_sym_eig(int n, const double* a, int lda, double* w, double* v, hipStream_t s) {
     SI_CHECK(n >= 1 && n <= kSmallMax, kErrInvalidArgument, "small_sym_eig: n out of range");
-    hipLaunchKernelGGL(k_sym_eig, dim3(1), dim3(kThreads), 0, s, n, a, lda, w, v);
+    hipLaunchKernelGGL(k_sym_eig, dim3(1), dim3(small_threads(n)), 0, s, n, a, lda, w, v);
     SI_LAUNCH_CHECK();
 }
 
 void small_gen_eig(int n, const double* a, const double* b, double* w, double* c, int* status,
                    hipStream_t s) {
     SI_CHECK(n >= 1 && n <= kSmallMax, kErrInvalidArgument, "small_gen_eig: n out of range");
-    hipLaunchKernelGGL(k_gen_eig, dim3(1), dim3(kThreads), 0, s, n, a, n, b, n, 0, w, c, status,
+    hipLaunchKernelGGL(k_gen_eig, dim3(1), dim3(small_threads(n)), 0, s, n, a, n, b, n, 0, w, c, status,
                        (int64_t)0, (int64_t)0);
     SI_LAUNCH_CHECK();
 }
@@ -422,7 +428,7 @@ void small_gen_eig(int n, const double* a, const double* b, double* w, double* c
 void small_gen_eig_scaled(int n, const double* a, int lda, const double* b, int ldb, double* w,
                           double* c, int* status, hipStream_t s) {
     SI_CHECK(n >= 1 && n <= kSmallMax, kErrInvalidArgument, "small_gen_eig: n out of range");
-    hipLaunchKernelGGL(k_gen_eig, dim3(1), dim3(kThreads), 0, s, n, a, lda, b, ldb, 1, w, c, status,
+    hipLaunchKernelGGL(k_gen_eig, dim3(1), dim3(small_threads(n)), 0, s, n, a, lda, b, ldb, 1, w, c, status,
                        (int64_t)0, (int64_t)0);
     SI_LAUNCH_CHECK();
 }
@@ -431,7 +437,7 @@ void small_gen_eig_scaled_batched(int count, int n, const double* a, int lda, in
                                   int* status, hipStream_t s) {
     SI_CHECK(n >= 1 && n <= kSmallMax && count >= 1, kErrInvalidArgument,
              "small_gen_eig_batched: size out of range");
-    hipLaunchKernelGGL(k_gen_eig, dim3(count), dim3(kThreads), 0, s, n, a, lda, b, ldb, 1, w, c,
+    hipLaunchKernelGGL(k_gen_eig, dim3(count), dim3(small_threads(n)), 0, s, n, a, lda, b, ldb, 1, w, c,
                        status, sa, sb);
     SI_LAUNCH_CHECK();
 }
@@ -445,7 +451,7 @@ void dense_seg_eig(const DevLaplacian& L, int count, const int64_t* seg_row0, co
 
 void small_chol_inv(int n, const double* g, double* t, int* status, hipStream_t s) {
     SI_CHECK(n >= 1 && n <= kSmallMax, kErrInvalidArgument, "small_chol_inv: n out of range");
-    hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(kThreads), 0, s, n, g, t, status);
+    hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(small_threads(n)), 0, s, n, g, t, status);
     SI_LAUNCH_CHECK();
 }
 

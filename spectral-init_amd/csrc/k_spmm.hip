@@ -12,7 +12,8 @@
 //                     remaining lanes split the row's nonzeros (slots); rows
 //                     per wavefront = 64 / (C * slots); slot reduction by DPP.
 //   long rows         rows with > kLongRow entries (degree hubs) get a 256-thread
-//                     workgroup each, reduced through LDS in a fixed order.
+//                     workgroup each, appended to the same launch, reduced by
+//                     wave butterflies and a fixed-order sum of the 4 waves.
 // Every reduction has a fixed order, so results are deterministic.
 // Blocks are remapped so that each of the 8 XCDs streams a contiguous range
 // of rows (its L2 then holds the x rows that neighbouring rows gather).
@@ -58,13 +59,66 @@ __device__ inline void seg_scalars(const int32_t* __restrict__ rseg, const T* __
     }
 }
 
+// -------------------------------------------------------------- long rows
+// A row with more than kLongRow entries (a degree hub) gets a whole 256-thread
+// workgroup, appended after the row blocks of the same launch: threads stride
+// the row's entries, each wave reduces by butterfly, the 4 wave sums are added
+// in order (deterministic).  16 columns per pass.
+template <typename T>
+__device__ void long_row(int64_t i, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                         const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ out,
+                         const T* __restrict__ z, int ld, int k, T s_l, T s_x, T s_z,
+                         const int32_t* __restrict__ rseg, const T* __restrict__ coef) {
+    constexpr int CH = 16;
+    __shared__ T red[4][CH];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    int32_t b = rp[i], e = rp[i + 1];
+    seg_scalars<T>(rseg, coef, i, s_l, s_x, s_z);
+    if (s_l == T(0)) e = b;
+    for (int c0 = 0; c0 < k; c0 += CH) {
+        const int cw = (k - c0) < CH ? (k - c0) : CH;
+        T acc[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) acc[c] = T(0);
+        for (int32_t p = b + t; p < e; p += 256) {
+            const T v = val[p];
+            const T* xr = x + (int64_t)col[p] * ld + c0;
+#pragma unroll
+            for (int c = 0; c < CH; ++c)
+                if (c < cw) acc[c] += v * xr[c];
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int c = 0; c < CH; ++c) acc[c] += __shfl_xor(acc[c], off, 64);
+        if (lane == 0)
+#pragma unroll
+            for (int c = 0; c < CH; ++c) red[wave][c] = acc[c];
+        __syncthreads();
+        if (t < cw) {
+            const T sum = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
+            const int64_t o = i * ld + c0 + t;
+            const T xi = x[o];
+            T r = s_l * apply_diag<T>(xi, sum) + s_x * xi;
+            if (z) r += s_z * z[o];
+            out[o] = r;
+        }
+        __syncthreads();
+    }
+}
+
 template <typename T, int K, int G>
 __global__ __launch_bounds__(256) void k_spmm_narrow(
     const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const T* __restrict__ val,
     const T* __restrict__ x, T* __restrict__ out, const T* __restrict__ z, int ld, int64_t n,
-    T s_l, T s_x, T s_z, const int32_t* __restrict__ rseg, const T* __restrict__ coef) {
-    const int64_t nb = gridDim.x;
-    const int64_t blk = xcd_block(blockIdx.x, nb);
+    T s_l, T s_x, T s_z, const int32_t* __restrict__ rseg, const T* __restrict__ coef,
+    const int32_t* __restrict__ long_rows, int64_t nb_main) {
+    if ((int64_t)blockIdx.x >= nb_main) {  // appended hub-row workgroups
+        long_row<T>(long_rows[blockIdx.x - nb_main], rp, col, val, x, out, z, ld, K, s_l, s_x,
+                    s_z, rseg, coef);
+        return;
+    }
+    const int64_t blk = xcd_block(blockIdx.x, nb_main);
     const int lane = threadIdx.x % G;
     const int64_t i = (blk * blockDim.x + threadIdx.x) / G;
     if (i >= n) return;
@@ -125,14 +179,19 @@ template <typename T, int K, int SLOTS, int U>
 __global__ __launch_bounds__(256) void k_spmm_wide(
     const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const T* __restrict__ val,
     const T* __restrict__ x, T* __restrict__ out, const T* __restrict__ z, int ld, int64_t n,
-    T s_l, T s_x, T s_z, const int32_t* __restrict__ rseg, const T* __restrict__ coef) {
+    T s_l, T s_x, T s_z, const int32_t* __restrict__ rseg, const T* __restrict__ coef,
+    const int32_t* __restrict__ long_rows, int64_t nb_main) {
     using V = typename Vec4<T>::type;
     constexpr int VEC = Vec4<T>::n;
     constexpr int C = K / VEC;                 // lanes across one x row
     constexpr int RPW = 64 / (C * SLOTS);      // rows per wavefront
     static_assert(C * SLOTS * RPW == 64, "bad wide SpMM shape");
-    const int64_t nb = gridDim.x;
-    const int64_t blk = xcd_block(blockIdx.x, nb);
+    if ((int64_t)blockIdx.x >= nb_main) {  // appended hub-row workgroups
+        long_row<T>(long_rows[blockIdx.x - nb_main], rp, col, val, x, out, z, ld, K, s_l, s_x,
+                    s_z, rseg, coef);
+        return;
+    }
+    const int64_t blk = xcd_block(blockIdx.x, nb_main);
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int chunk = lane % C;
@@ -204,59 +263,6 @@ __global__ __launch_bounds__(256) void k_spmm_wide(
     }
 }
 
-// -------------------------------------------------------------- long rows
-template <typename T>
-__global__ __launch_bounds__(256) void k_spmm_long(
-    const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const T* __restrict__ val,
-    const T* __restrict__ x, T* __restrict__ out, const T* __restrict__ z, int ld, int k,
-    const int32_t* __restrict__ rows, T s_l, T s_x, T s_z, const int32_t* __restrict__ rseg,
-    const T* __restrict__ coef) {
-    constexpr int CH = 16;                 // columns per pass
-    static_assert(CH * 16 == 256, "long-row reduction assumes 16 columns per pass");
-    __shared__ T part[256][CH + 1];
-    __shared__ T segs[16][CH + 1];
-    const int64_t i = rows[blockIdx.x];
-    int32_t b = rp[i], e = rp[i + 1];
-    seg_scalars<T>(rseg, coef, i, s_l, s_x, s_z);
-    if (s_l == T(0)) e = b;
-    const int t = threadIdx.x;
-    for (int c0 = 0; c0 < k; c0 += CH) {
-        const int cw = (k - c0) < CH ? (k - c0) : CH;
-        T acc[CH];
-#pragma unroll
-        for (int c = 0; c < CH; ++c) acc[c] = T(0);
-        for (int32_t p = b + t; p < e; p += 256) {
-            const T v = val[p];
-            const T* xr = x + (int64_t)col[p] * ld + c0;
-#pragma unroll
-            for (int c = 0; c < CH; ++c)
-                if (c < cw) acc[c] += v * xr[c];
-        }
-#pragma unroll
-        for (int c = 0; c < CH; ++c) part[t][c] = acc[c];
-        __syncthreads();
-        // fixed-order two-level sum: 16 segments of 16 threads per column
-        {
-            const int c = t % CH, q = t / CH;
-            T seg = T(0);
-            for (int r = 0; r < 16; ++r) seg += part[q * 16 + r][c];
-            segs[q][c] = seg;
-        }
-        __syncthreads();
-        if (t < cw) {
-            T sum = T(0);
-            for (int q = 0; q < 16; ++q) sum += segs[q][t];
-            const int64_t o = i * ld + c0 + t;
-            const T xi = x[o];
-            T r = s_l * apply_diag<T>(xi, sum) + s_x * xi;
-            if (z) r += s_z * z[o];
-            out[o] = r;
-        }
-        __syncthreads();
-    }
-}
-
-// Epilogue scalars of one launch: uniform, or per row segment (rseg, coef).
 template <typename T>
 struct Epi {
     T s_l, s_x, s_z;
@@ -270,8 +276,9 @@ void launch_narrow(const DevLaplacian& L, const T* val, const T* x, T* out, int 
                    const Epi<T>& ep, hipStream_t s) {
     const int64_t threads = L.n * G;
     const int64_t blocks = ceil_div(threads, 256);
-    hipLaunchKernelGGL((k_spmm_narrow<T, K, G>), dim3(blocks), dim3(256), 0, s, L.rowptr, L.col,
-                       val, x, out, ep.z, ld, L.n, ep.s_l, ep.s_x, ep.s_z, ep.rseg, ep.coef);
+    hipLaunchKernelGGL((k_spmm_narrow<T, K, G>), dim3(blocks + L.n_long), dim3(256), 0, s, L.rowptr,
+                       L.col, val, x, out, ep.z, ld, L.n, ep.s_l, ep.s_x, ep.s_z, ep.rseg, ep.coef,
+                       L.long_rows, blocks);
 }
 
 template <typename T, int K, int SLOTS, int U>
@@ -282,8 +289,9 @@ void launch_wide(const DevLaplacian& L, const T* val, const T* x, T* out, int ld
     constexpr int RPW = 64 / (C * SLOTS);
     const int64_t waves = ceil_div(L.n, RPW);
     const int64_t blocks = ceil_div(waves, 4);
-    hipLaunchKernelGGL((k_spmm_wide<T, K, SLOTS, U>), dim3(blocks), dim3(256), 0, s, L.rowptr, L.col,
-                       val, x, out, ep.z, ld, L.n, ep.s_l, ep.s_x, ep.s_z, ep.rseg, ep.coef);
+    hipLaunchKernelGGL((k_spmm_wide<T, K, SLOTS, U>), dim3(blocks + L.n_long), dim3(256), 0, s,
+                       L.rowptr, L.col, val, x, out, ep.z, ld, L.n, ep.s_l, ep.s_x, ep.s_z, ep.rseg,
+                       ep.coef, L.long_rows, blocks);
 }
 
 // One launch of a directly supported width (1, 2, 3, 4, 8, 16, 32, 64).
@@ -336,8 +344,8 @@ void spmm_chunk(const DevLaplacian& L, const T* val, const T* x, T* out, int k, 
 
 // Any width: split the columns into directly supported chunks, widest first
 // (the 16-byte vector paths need 16-byte aligned chunk offsets and leading
-// dimension; otherwise chunks of at most 8 columns are used).  Hub rows are
-// handled once for all columns.
+// dimension; otherwise chunks of at most 8 columns are used).  Hub rows ride
+// along as extra workgroups of every chunk's launch.
 template <typename T>
 void spmm_impl(const DevLaplacian& L, const T* val, const T* x, T* out, int k, int ld, T s_l,
                T s_x, T s_z, const T* z, hipStream_t s, const int32_t* rseg = nullptr,
@@ -371,11 +379,6 @@ void spmm_impl(const DevLaplacian& L, const T* val, const T* x, T* out, int k, i
         const Epi<T> ep{s_l, s_x, s_z, z ? z + off : nullptr, rseg, coef};
         spmm_chunk<T>(L, val, x + off, out + off, w, ld, ep, s);
         off += w;
-    }
-    if (L.n_long > 0) {
-        hipLaunchKernelGGL((k_spmm_long<T>), dim3(L.n_long), dim3(256), 0, s, L.rowptr, L.col,
-                           val, x, out, z, ld, k, L.long_rows, s_l, s_x, s_z, rseg, coef);
-        SI_LAUNCH_CHECK();
     }
     if (ev0) {
         SI_HIP(hipEventRecord(ev1, s));
@@ -447,6 +450,9 @@ void spmm_autotune(DevLaplacian& L, hipStream_t s) {
                 if (c == 0 || ms < best) { best = ms; pick = c; }
             }
             (width == 16 ? L.shape16 : L.shape32) = pick;
+            if (trace_on())
+                std::fprintf(stderr, "[autotune] FP32 width %d: shape %d (%.1f us/launch)\n", width,
+                             pick, 1e3 * best / reps);
         }
         L.tuned = 1;
     } catch (...) {
