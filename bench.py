@@ -49,12 +49,12 @@ def largest_component(indptr, indices, data):
             sub.data.astype(np.float32), n, nc)
 
 
-def cpu_baseline(lap, si, n_components, plain_iters, budget_s):
-    """Time the CPU oracle's LOBPCG iteration (k = n_components+1, FP64, 1 thread)
-    on the same Laplacian for a bounded number of iterations."""
+def cpu_baseline(lap, si, n_components, plain_iters, budget_s, threads=1):
+    """Time the CPU oracle's LOBPCG iteration (k = n_components+1, FP64) on the
+    same Laplacian for a bounded number of iterations, on `threads` host cores."""
     import oracle as O
     O.build()
-    O.set_threads(1)
+    O.set_threads(threads)
     _, sq, L = lap.export()
     k = n_components + 1
     n = sq.size
@@ -68,7 +68,7 @@ def cpu_baseline(lap, si, n_components, plain_iters, budget_s):
     r = O.lobpcg_core(L, x0, tol=0.0, maxiter=iters)
     dt = time.perf_counter() - t
     ms_it = 1e3 * dt / max(r["iterations"], 1)
-    return dict(value=round(ms_it, 3), unit="ms per LOBPCG iteration (FP64, k=%d)" % k, cores=1,
+    return dict(value=round(ms_it, 3), unit="ms per LOBPCG iteration (FP64, k=%d)" % k, cores=threads,
                 kind="port",
                 sample=f"{r['iterations']} LOBPCG iterations of the oracle on the full graph "
                        f"({dt:.1f} s); projected time to eigvecs at the plain GPU solver's "
@@ -352,6 +352,15 @@ def main():
         log("CPU oracle sample ...")
         cpu = cpu_baseline(lap, si, args.n_components, plain_iters, args.cpu_seconds)
         log(f"CPU oracle: {cpu['value']} {cpu['unit']}")
+        # the same iteration on the host cores this process may use (OpenMP SpMM)
+        try:
+            ncores = len(os.sched_getaffinity(0))
+        except AttributeError:
+            ncores = os.cpu_count() or 1
+        ncores = max(1, min(ncores, 16))
+        cpu_all = cpu_baseline(lap, si, args.n_components, plain_iters, args.cpu_seconds / 3,
+                               threads=ncores)
+        log(f"CPU oracle on {ncores} cores: {cpu_all['value']} {cpu_all['unit']}")
 
     traffic = read_traffic(args.traffic, args.workload, last.block_size)
 
@@ -387,6 +396,7 @@ def main():
                      "avg_launch_us": round(spmm_avg_us, 2),
                      "launches": sp["launches_f32"]},
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all if not args.no_cpu else None,
     }
     print(json.dumps(out), flush=True)
 

@@ -159,8 +159,10 @@ si_status si_rsvd_solve(si_context* ctx, const si_laplacian* lap, int32_t n_comp
                         uint64_t seed, int32_t oversampling, int32_t nbiter, double* evals,
                         double* evecs, si_report* rep);
 
-/* dense_evd (src/solvers/dense.rs:9-46): the k smallest eigenpairs of L for
- * n <= 64 (one workgroup, FP64 Jacobi); evals host, evecs device. */
+/* dense_evd (src/solvers/dense.rs:9-46): the k smallest eigenpairs of L, FP64,
+ * for n <= 4096 and k <= 64: one-workgroup Jacobi up to n = 64, Householder
+ * tridiagonalisation + Sturm multisection + inverse iteration above.  evals
+ * host (ascending), evecs device (n x k).  SI_ERR_UNSUPPORTED above n = 4096. */
 si_status si_dense_evd(si_context* ctx, const si_laplacian* lap, int32_t k, double* evals,
                        double* evecs);
 

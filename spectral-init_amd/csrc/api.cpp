@@ -645,7 +645,7 @@ si_status si_dense_evd(si_context* ctx, const si_laplacian* lap, int32_t k, doub
     if (!ctx || !lap || !evals || !evecs) return SI_ERR_INVALID_ARGUMENT;
     return guard(ctx, [&] {
         si::DevBuf<double> vi((size_t)lap->L.n * (k > 0 ? k : 1));
-        si::dense_evd_small(lap->L, k, evals, vi, ctx->c.stream);
+        si::dense_evd_device(lap->L, k, evals, vi, ctx->c.stream);
         si::rows_to_external(lap->L, vi.get(), k, evecs, k, k, ctx->c.stream);
         SI_HIP(hipStreamSynchronize(ctx->c.stream));
     });

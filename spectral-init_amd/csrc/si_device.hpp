@@ -224,6 +224,8 @@ void mpnn_adam(int64_t count, float* p, const double* g, float* m, float* v, flo
 // ----------------------------------------------------------------- small dense (one workgroup)
 // All small matrices are FP64 row-major, n <= kSmallMax, resident on the device.
 constexpr int kSmallMax = 64;
+// Largest n of the device dense eigensolver (an n x n FP64 matrix, 128 MiB).
+constexpr int kDenseMax = 4096;
 // Symmetric eigen-decomposition of a (lower triangle read): w ascending, v columns.
 void small_sym_eig(int n, const double* a, int lda, double* w, double* v, hipStream_t s);
 // Generalised problem A c = theta B c.  status[0] = 0 ok, 1 if B not SPD.

@@ -49,7 +49,7 @@ enum WsSlot : int {
     kWsSmall0, kWsSmall1, kWsSmall2, kWsSmall3, kWsSmall4, kWsSmall5, kWsSmall6, kWsSmall7,
     kWsPlainS, kWsPlainAS, kWsPlainR, kWsPlainP, kWsPlainAP, kWsPlainPn, kWsPlainAPn,
     kWsPlainX, kWsPlainT, kWsPlainXr, kWsPlainBest, kWsStatus,
-    kWsBatchSmall, kWsBatchCoef, kWsBatchRr,
+    kWsBatchSmall, kWsBatchCoef, kWsBatchRr, kWsDenseA, kWsDenseAux, kWsDensePiv,
     kWsNumSlots
 };
 void* workspace_get(int slot, size_t bytes);
@@ -167,9 +167,17 @@ void rsvd_solve_device(const DevLaplacian& L, int n_components, uint64_t seed, i
                        int nbiter, double* evals_host, double* evecs_dev, SolveStats& st,
                        hipStream_t s);
 
-// dense_evd (dense.rs:9-46) for n <= kSmallMax.
+// dense_evd (dense.rs:9-46) for n <= kSmallMax (one-workgroup Jacobi).
 void dense_evd_small(const DevLaplacian& L, int k, double* evals_host, double* evecs_dev,
                      hipStream_t s);
+// dense_evd for n <= kDenseMax (Householder tridiagonalisation, multisection,
+// inverse iteration, back-transformation).
+void dense_evd_large(const DevLaplacian& L, int k, double* evals_host, double* evecs_dev,
+                     hipStream_t s);
+// dense_evd for any n <= kDenseMax: the Jacobi path up to kSmallMax nodes,
+// the tridiagonal path above.
+void dense_evd_device(const DevLaplacian& L, int k, double* evals_host, double* evecs_dev,
+                      hipStream_t s);
 
 // solve_eigenproblem (solvers/mod.rs:70-224).
 struct ChainOptions {

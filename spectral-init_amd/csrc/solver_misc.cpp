@@ -108,9 +108,16 @@ void dense_evd_small(const DevLaplacian& L, int k, double* evals_host, double* e
     for (int c = 0; c < k; ++c) evals_host[c] = hw[c];
 }
 
+void dense_evd_device(const DevLaplacian& L, int k, double* evals_host, double* evecs_dev,
+                      hipStream_t s) {
+    if (L.n <= kSmallMax) dense_evd_small(L, k, evals_host, evecs_dev, s);
+    else dense_evd_large(L, k, evals_host, evecs_dev, s);
+}
+
 // solve_eigenproblem (src/solvers/mod.rs:70-224) with thresholds from
 // src/metrics.rs:17-46.  Level 2 (sparse-Cholesky shift-invert) has no device
-// counterpart here; a dense level-0/5 solve exists on the device for n <= 64.
+// counterpart here; the dense levels 0 and 5 run on the device for
+// n <= kDenseMax.
 int solve_eigenproblem_device(const DevLaplacian& L, int n_components, uint64_t seed,
                               const ChainOptions& opt, double* evals_host, double* evecs_dev,
                               SolveStats& st, hipStream_t s) {
@@ -127,8 +134,8 @@ int solve_eigenproblem_device(const DevLaplacian& L, int n_components, uint64_t 
         return q;
     };
     const double gate_l1 = opt.tol;  // LOBPCG_QUALITY_THRESHOLD tracks the solver tolerance
-    if (n < opt.dense_threshold && n <= kSmallMax && k <= n) {
-        dense_evd_small(L, k, evals_host, evecs_dev, s);
+    if (n < opt.dense_threshold && n <= kDenseMax && k <= n) {
+        dense_evd_device(L, k, evals_host, evecs_dev, s);
         const double q = quality(evals_host);
         if (q < 1e-6) { st.level = 0; st.max_residual = q; return 0; }
     }
@@ -160,8 +167,8 @@ int solve_eigenproblem_device(const DevLaplacian& L, int n_components, uint64_t 
         const double q = quality(evals_host);
         if (q < 1e-2) { st.level = 4; st.max_residual = q; return 4; }
     }
-    if (n <= kSmallMax) {
-        dense_evd_small(L, k, evals_host, evecs_dev, s);
+    if (n <= kDenseMax) {
+        dense_evd_device(L, k, evals_host, evecs_dev, s);
         st.level = 5;
         st.max_residual = quality(evals_host);
         return 5;

@@ -399,7 +399,7 @@ def rsvd_solve(lap: Laplacian, n_components: int, seed: int, oversampling=-1, nb
 
 
 def dense_evd(lap: Laplacian, k: int):
-    """dense_evd (src/solvers/dense.rs:9-46), n <= 64 on the device."""
+    """dense_evd (src/solvers/dense.rs:9-46) on the device, n <= 4096, k <= 64."""
     ev = np.empty(k)
     vec = _evecs(lap, k)
     lap.ctx._check(lib().si_dense_evd(lap.ctx.handle, lap.handle, k, _ptr(ev), _ptr(vec)))
