@@ -145,9 +145,20 @@ si_status si_lobpcg_solve(si_context* ctx, const si_laplacian* lap, int32_t n_co
                           uint64_t seed, int32_t regularize, double tol, double* evals,
                           double* evecs, si_report* rep);
 
+/* lobpcg_sinv_solve (src/solvers/sinv.rs:106-189): LOBPCG on (L + 1e-4 I)^-1
+ * with the reference's start block, tolerance (1e-8) and iteration cap
+ * (min(5n, 300)); eigenvalues lambda = 1/mu - 1e-4, ascending.  L + 1e-4 I is
+ * factorised by a banded Cholesky in the stored node order (reverse
+ * Cuthill-McKee for n >= 4096); SI_ERR_CONVERGENCE for the reference's None,
+ * which includes a band above 2^27 stored entries or 4e12 factorisation
+ * flops.  evals host, evecs device (n x (n_components+1)). */
+si_status si_lobpcg_sinv_solve(si_context* ctx, const si_laplacian* lap, int32_t n_components,
+                               uint64_t seed, double* evals, double* evecs, si_report* rep);
+
 /* solve_eigenproblem (src/solvers/mod.rs:70-224): escalation L0 dense EVD
- * (n < dense_threshold), L1 (cfg->solver), L3 regularised LOBPCG, L4 rSVD,
- * L5 dense EVD, each gated on the FP64 residual.  evals host, evecs device. */
+ * (n < dense_threshold), L1 (cfg->solver), L2 shift-and-invert LOBPCG, L3
+ * regularised LOBPCG, L4 rSVD, L5 dense EVD (n <= 4096), each gated on the
+ * FP64 residual.  evals host, evecs device. */
 si_status si_solve_eigenproblem(si_context* ctx, const si_laplacian* lap, int32_t n_components,
                                 uint64_t seed, const si_config* cfg, double* evals,
                                 double* evecs, si_report* rep);

@@ -37,6 +37,12 @@ si_status si_test_fast_solve(si_context* ctx, const si_laplacian* lap, int32_t n
                              uint64_t seed, const si_config* cfg, double* evals, double* evecs,
                              si_report* rep);
 
+/* y = (L + shift I)^-1 x for k device columns (caller's node order) through
+ * the banded Cholesky of the shift-and-invert level (src/solvers/sinv.rs:125,
+ * 139-153); *bandwidth receives the band used (-1 when unavailable). */
+si_status si_test_band_solve(si_context* ctx, const si_laplacian* lap, double shift,
+                             const double* x, double* y, int32_t k, int32_t* bandwidth);
+
 /* Batched component solves (si_spectral_init with batch_components): enable
  * >= 0 switches hipGraph capture of the FP32 rounds on (1) or off (0), -1
  * leaves it; captured / launches (may be NULL) receive the graph counts of

@@ -588,6 +588,31 @@ si_status si_lobpcg_solve(si_context* ctx, const si_laplacian* lap, int32_t n_co
     });
 }
 
+si_status si_lobpcg_sinv_solve(si_context* ctx, const si_laplacian* lap, int32_t n_components,
+                               uint64_t seed, double* evals, double* evecs, si_report* rep) {
+    if (!ctx || !lap || !evals || !evecs) return SI_ERR_INVALID_ARGUMENT;
+    return guard(ctx, [&] {
+        SI_CHECK(n_components >= 0 && n_components + 1 <= 64, si::kErrUnsupported,
+                 "n_components above 63 is not supported");
+        si::SolveStats st;
+        const int k = n_components + 1;
+        double* vi = si::ws_get<double>(si::kWsChainVec, (size_t)lap->L.n * k);
+        si::EventTimer t(ctx->c.stream);
+        const bool ok = si::lobpcg_sinv_solve_device(lap->L, n_components, seed, evals, vi, st,
+                                                     ctx->c.stream);
+        if (ok) {
+            std::vector<double> res(k);
+            si::fp64_residuals(lap->L, vi, k, evals, res.data(), ctx->c.stream);
+            st.max_residual = *std::max_element(res.begin(), res.end());
+            si::rows_to_external(lap->L, vi, k, evecs, k, k, ctx->c.stream);
+        }
+        const double ms = t.stop_ms();
+        st.level = 2;
+        fill_report(rep, st, evals, n_components + 1, ms);
+        SI_CHECK(ok, si::kErrConvergence, "lobpcg_sinv_solve produced no result");
+    });
+}
+
 si_status si_solve_eigenproblem(si_context* ctx, const si_laplacian* lap, int32_t n_components,
                                 uint64_t seed, const si_config* cfg, double* evals,
                                 double* evecs, si_report* rep) {

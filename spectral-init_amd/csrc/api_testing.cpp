@@ -145,6 +145,24 @@ si_status si_test_fast_solve(si_context* ctx, const si_laplacian* lap, int32_t n
     });
 }
 
+si_status si_test_band_solve(si_context* ctx, const si_laplacian* lap, double shift,
+                             const double* x, double* y, int32_t k, int32_t* bandwidth) {
+    if (!ctx || !lap || !x || !y || k < 1) return SI_ERR_INVALID_ARGUMENT;
+    return tguard(ctx, [&] {
+        hipStream_t s = ctx->c.stream;
+        const int64_t n = lap->L.n;
+        si::BandFactor F;
+        const bool ok = si::band_factor(lap->L, shift, F, s);
+        if (bandwidth) *bandwidth = ok ? F.b : -1;
+        SI_CHECK(ok, si::kErrUnsupported, "band_solve: band too large or not positive definite");
+        si::DevBuf<double> xi((size_t)n * k), yi((size_t)n * k);
+        si::rows_to_internal(lap->L, x, k, xi.get(), k, k, s);
+        si::band_solve(F, xi.get(), k, yi.get(), k, k, 1.0, s);
+        si::rows_to_external(lap->L, yi.get(), k, y, k, k, s);
+        SI_HIP(hipStreamSynchronize(s));
+    });
+}
+
 si_status si_profile_spmm_begin(si_context* ctx) {
     if (!ctx) return SI_ERR_INVALID_ARGUMENT;
     return tguard(ctx, [&] {

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -178,6 +179,43 @@ void dense_evd_large(const DevLaplacian& L, int k, double* evals_host, double* e
 // the tridiagonal path above.
 void dense_evd_device(const DevLaplacian& L, int k, double* evals_host, double* evecs_dev,
                       hipStream_t s);
+
+// A symmetric block operator for lobpcg_core: out = A in, k columns, leading
+// dimension ld for both blocks, enqueued on the solver's stream.
+using BlockOperator = std::function<void(const double* in, double* out, int k, int ld)>;
+// Knyazev's block LOBPCG (the algorithm of linfa_linalg::lobpcg) for the m
+// smallest eigenpairs of op; see solver_plain.cpp.
+// robust: normalise the active residual columns before their Cholesky-QR and,
+// if it still fails, lock the most converged active column and retry, where
+// the plain method stops.
+bool lobpcg_core(int64_t n, const BlockOperator& op, double* x, int m, double tol, int maxiter,
+                 double* lam_h, double* rnorm_h, int64_t& iters, int64_t& op_calls,
+                 hipStream_t s, bool robust = false);
+
+// Banded Cholesky factor of M = L + shift I in the device's node order
+// (k_band.hip).  The band is bounded by kBandMaxEntries stored doubles and
+// kBandMaxFlops for the factorisation; larger bands are not factorised.
+constexpr double kBandMaxEntries = 134217728.0;  // 1 GiB of FP64
+constexpr double kBandMaxFlops = 4e12;
+struct BandFactor {
+    int n = 0;
+    int b = 0;  // bandwidth: M[i][j] == 0 for i - j > b
+    DevBuf<double> B;
+};
+int band_width(const DevLaplacian& L, hipStream_t s);
+// false when the band is too large or M is not positive definite
+bool band_factor(const DevLaplacian& L, double shift, BandFactor& F, hipStream_t s);
+// y = scale * M^-1 x (scale +1 or -1) for k columns
+void band_solve(const BandFactor& F, const double* x, int ldx, double* y, int ldy, int k,
+                double scale, hipStream_t s);
+
+// lobpcg_sinv_solve (sinv.rs:106-189): LOBPCG on (L + 1e-4 I)^-1; false for
+// the reference's None (band too large, factorisation failed, no convergence).
+constexpr double kSinvShift = 1e-4;             // SINV_SHIFT (sinv.rs:18)
+constexpr double kSinvQualityThreshold = 1e-6;  // SINV_LOBPCG_QUALITY_THRESHOLD (metrics.rs:35)
+bool lobpcg_sinv_solve_device(const DevLaplacian& L, int n_components, uint64_t seed,
+                              double* evals_host, double* evecs_dev, SolveStats& st,
+                              hipStream_t s);
 
 // solve_eigenproblem (solvers/mod.rs:70-224).
 struct ChainOptions {

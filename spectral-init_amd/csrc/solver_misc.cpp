@@ -115,9 +115,9 @@ void dense_evd_device(const DevLaplacian& L, int k, double* evals_host, double* 
 }
 
 // solve_eigenproblem (src/solvers/mod.rs:70-224) with thresholds from
-// src/metrics.rs:17-46.  Level 2 (sparse-Cholesky shift-invert) has no device
-// counterpart here; the dense levels 0 and 5 run on the device for
-// n <= kDenseMax.
+// src/metrics.rs:17-46.  Level 2 factorises L + 1e-4 I as a band (it is skipped,
+// as the reference skips a failed Cholesky, when the band is too large); the
+// dense levels 0 and 5 run on the device for n <= kDenseMax.
 int solve_eigenproblem_device(const DevLaplacian& L, int n_components, uint64_t seed,
                               const ChainOptions& opt, double* evals_host, double* evecs_dev,
                               SolveStats& st, hipStream_t s) {
@@ -153,6 +153,12 @@ int solve_eigenproblem_device(const DevLaplacian& L, int n_components, uint64_t 
         // these vectors against the exact Laplacian (metrics.rs:57-92 semantics)
         const double q = st.residual_is_fp64 ? st.max_residual : quality(evals_host);
         if (q < gate_l1) { st.level = 1; st.max_residual = q; return 1; }
+    }
+    SolveStats st2;
+    if (lobpcg_sinv_solve_device(L, n_components, seed, evals_host, evecs_dev, st2, s)) {
+        const double q = quality(evals_host);
+        st.iterations += st2.iterations;
+        if (q < kSinvQualityThreshold) { st.level = 2; st.max_residual = q; return 2; }
     }
     SolveStats st3;
     if (lobpcg_solve_plain(L, n_components, seed, true, opt.tol, evals_host, evecs_dev, st3, s)) {

@@ -11,6 +11,8 @@
 // launch per operation and a host-side convergence check every iteration --
 // this is the baseline the FP32-bulk solver is measured against.
 #include <cmath>
+#include <cstdio>
+#include <functional>
 
 #include "si_host.hpp"
 
@@ -23,7 +25,7 @@ struct PlainWork {
     int m = 0;
     // n-sized blocks come from the thread's workspace pool
     double *S, *AS, *R, *P, *AP, *Pn, *APn, *ws;
-    DevBuf<double> g1, g2, t1, w, c, csub, sel, eye, lam;
+    DevBuf<double> g1, g2, t1, w, c, csub, sel, eye, lam, colsc;
     DevBuf<int> status;
     PlainWork(int64_t n_, int m_) : n(n_), m(m_) {
         const int s3 = 3 * m;
@@ -44,6 +46,7 @@ struct PlainWork {
         sel.alloc((size_t)m * m);
         eye.alloc((size_t)m * m);
         lam.alloc(m);
+        colsc.alloc(m);
         status.alloc(1);
     }
 };
@@ -60,20 +63,23 @@ void upload_eye(DevBuf<double>& eye, int m, hipStream_t s) {
     to_device(eye.get(), h.data(), h.size(), s);
 }
 
-// One LOBPCG pass.  x: device [n x m] (ld m), replaced by the final Ritz
-// vectors; lam_h/rnorm_h receive Ritz values and residual norms.
-bool lobpcg_core_plain(const DevLaplacian& L, double shift, double* x, int m, double tol,
-                       int maxiter, double* lam_h, double* rnorm_h, int64_t& iters,
-                       int64_t& spmm_calls, hipStream_t s) {
-    const int64_t n = L.n;
+}  // namespace
+
+// One LOBPCG pass for the m smallest eigenpairs of the symmetric operator op
+// (op(in, out, k, ld): out = A in for k columns, both with leading dimension
+// ld).  x: device [n x m] (ld m), replaced by the final Ritz vectors;
+// lam_h/rnorm_h receive the Ritz values (ascending) and residual norms.
+bool lobpcg_core(int64_t n, const BlockOperator& op, double* x, int m, double tol, int maxiter,
+                 double* lam_h, double* rnorm_h, int64_t& iters, int64_t& op_calls,
+                 hipStream_t s, bool robust) {
     const int s3 = 3 * m;
     PlainWork W(n, m);
     upload_eye(W.eye, m, s);
     double* X = W.S;
     double* AX = W.AS;
     auto apply = [&](const double* in, double* out, int k) {
-        spmm_f64(L, in, out, k, s3, 1.0, shift, 0.0, nullptr, s);
-        ++spmm_calls;
+        op(in, out, k, s3);
+        ++op_calls;
     };
     copy_cols_f64(x, m, X, s3, n, m, s);
     // orthonormalise X (Cholesky-QR)
@@ -103,27 +109,61 @@ bool lobpcg_core_plain(const DevLaplacian& L, double shift, double* x, int m, do
             act[c] = rnorm_h[c] > tol;
             na += act[c];
         }
+        if (trace_on() && (it % 25 == 0 || it >= maxiter)) {
+            std::fprintf(stderr, "[si] lobpcg it %lld active %d rnorm", (long long)it, na);
+            for (int c = 0; c < m; ++c) std::fprintf(stderr, " %.3e", rnorm_h[c]);
+            std::fprintf(stderr, "\n");
+        }
         if (na == 0) { converged = true; break; }
         if (it >= maxiter) break;
-        // selection matrix of the active columns
-        std::vector<double> hs((size_t)m * na, 0.0);
-        for (int c = 0, q = 0; c < m; ++c)
-            if (act[c]) hs[c * na + q++] = 1.0;
-        to_device(W.sel.get(), hs.data(), hs.size(), s);
         double* Ra = X + m;
         double* ARa = AX + m;
-        block_mul_f64(n, W.R, m, m, W.sel, na, nullptr, 0, 0, nullptr, Ra, s3, s);
-        // R_a <- R_a - X (X^T R_a)
-        gram_f64(n, X, s3, m, Ra, s3, na, W.g1, W.ws, s);
-        small_copy_sub(m, na, W.g1, na, W.g2, na, -1.0, s);
-        std::vector<double> he((size_t)na * na, 0.0);
-        for (int i = 0; i < na; ++i) he[i * na + i] = 1.0;
-        to_device(W.t1.get(), he.data(), he.size(), s);
-        block_mul_f64(n, Ra, s3, na, W.t1, na, X, s3, m, W.g2, Ra, s3, s);
-        // Cholesky-QR of the active residuals
-        gram_f64(n, Ra, s3, na, Ra, s3, na, W.g1, W.ws, s);
-        small_chol_inv(na, W.g1, W.t1, W.status, s);
-        if (read_status(W.status, s) != 0) break;
+        bool r_ok = false;
+        for (;;) {
+            // selection matrix of the active columns
+            std::vector<double> hs((size_t)m * na, 0.0);
+            for (int c = 0, q = 0; c < m; ++c)
+                if (act[c]) hs[c * na + q++] = 1.0;
+            to_device(W.sel.get(), hs.data(), hs.size(), s);
+            block_mul_f64(n, W.R, m, m, W.sel, na, nullptr, 0, 0, nullptr, Ra, s3, s);
+            // R_a <- R_a - X (X^T R_a)
+            gram_f64(n, X, s3, m, Ra, s3, na, W.g1, W.ws, s);
+            small_copy_sub(m, na, W.g1, na, W.g2, na, -1.0, s);
+            std::vector<double> he((size_t)na * na, 0.0);
+            for (int i = 0; i < na; ++i) he[i * na + i] = 1.0;
+            to_device(W.t1.get(), he.data(), he.size(), s);
+            block_mul_f64(n, Ra, s3, na, W.t1, na, X, s3, m, W.g2, Ra, s3, s);
+            // Cholesky-QR of the active residuals.  In the robust mode their
+            // columns are first scaled to unit norm (the same span and Q in
+            // exact arithmetic, a unit-diagonal Gram matrix), and when the
+            // residuals are numerically dependent -- as for the two members of
+            // an exactly degenerate pair -- the most converged active column is
+            // locked and the factorisation retried.
+            gram_f64(n, Ra, s3, na, Ra, s3, na, W.g1, W.ws, s);
+            if (robust) {
+                std::vector<double> hd((size_t)na * na), hsc(na);
+                to_host(hd.data(), W.g1.get(), hd.size(), s);
+                for (int c = 0; c < na; ++c) {
+                    const double d = hd[(size_t)c * na + c];
+                    hsc[c] = d > 1e-300 ? 1.0 / std::sqrt(d) : 1.0;
+                }
+                to_device(W.colsc.get(), hsc.data(), na, s);
+                scale_cols_f64(Ra, s3, n, na, W.colsc, s);
+                gram_f64(n, Ra, s3, na, Ra, s3, na, W.g1, W.ws, s);
+            }
+            small_chol_inv(na, W.g1, W.t1, W.status, s);
+            if (read_status(W.status, s) == 0) { r_ok = true; break; }
+            if (trace_on())
+                std::fprintf(stderr, "[si] lobpcg it %lld: residual Cholesky failed (%d active)\n",
+                             (long long)it, na);
+            if (!robust || na <= 1) break;
+            int drop = -1;
+            for (int c = 0; c < m; ++c)
+                if (act[c] && (drop < 0 || rnorm_h[c] < rnorm_h[drop])) drop = c;
+            act[drop] = 0;
+            --na;
+        }
+        if (!r_ok) break;
         block_mul_f64(n, Ra, s3, na, W.t1, na, nullptr, 0, 0, nullptr, Ra, s3, s);
         apply(Ra, ARa, na);
         bool use_p = false;
@@ -153,7 +193,10 @@ bool lobpcg_core_plain(const DevLaplacian& L, double shift, double* x, int m, do
             small_gen_eig(sdim, W.g1, W.g2, W.w, W.c, W.status, s);
             ok = read_status(W.status, s) == 0;
         }
-        if (!ok) break;
+        if (!ok) {
+            if (trace_on()) std::fprintf(stderr, "[si] lobpcg it %lld: Rayleigh-Ritz failed\n", (long long)it);
+            break;
+        }
         small_copy_sub(sdim, m, W.c, sdim, W.csub, m, 1.0, s);
         const double* csub_rp = W.csub.get() + (size_t)m * m;
         block_mul_f64(n, X + m, s3, sdim - m, csub_rp, m, nullptr, 0, 0, nullptr, W.Pn, m, s);
@@ -171,6 +214,17 @@ bool lobpcg_core_plain(const DevLaplacian& L, double shift, double* x, int m, do
     SI_HIP(hipStreamSynchronize(s));
     iters += it;
     return converged;
+}
+
+namespace {
+
+bool lobpcg_core_plain(const DevLaplacian& L, double shift, double* x, int m, double tol,
+                       int maxiter, double* lam_h, double* rnorm_h, int64_t& iters,
+                       int64_t& spmm_calls, hipStream_t s) {
+    const BlockOperator op = [&](const double* in, double* out, int k, int ld) {
+        spmm_f64(L, in, out, k, ld, 1.0, shift, 0.0, nullptr, s);
+    };
+    return lobpcg_core(L.n, op, x, m, tol, maxiter, lam_h, rnorm_h, iters, spmm_calls, s);
 }
 
 }  // namespace

@@ -36,10 +36,12 @@ EXPORTED = [
     "si_last_error", "si_laplacian_build", "si_laplacian_build_ex", "si_laplacian_destroy",
     "si_laplacian_info",
     "si_laplacian_export", "si_spmm", "si_chebyshev_filter", "si_max_eigenpair_residual",
-    "si_lobpcg_solve", "si_solve_eigenproblem", "si_rsvd_solve", "si_dense_evd",
+    "si_lobpcg_solve", "si_lobpcg_sinv_solve", "si_solve_eigenproblem", "si_rsvd_solve",
+    "si_dense_evd",
     "si_spectral_init",
     "si_test_gram", "si_test_block_mul", "si_test_small_sym_eig", "si_test_small_gen_eig",
     "si_test_fill_normal", "si_test_fast_solve", "si_test_batch_graphs", "si_test_spmm_shapes",
+    "si_test_band_solve",
     "si_profile_spmm_begin", "si_profile_spmm_end",
     "si_mpnn_param_count", "si_mpnn_create", "si_mpnn_destroy", "si_mpnn_get_params",
     "si_mpnn_set_params", "si_mpnn_features", "si_mpnn_forward", "si_mpnn_gradient",
@@ -184,6 +186,7 @@ def lib():
         L.si_chebyshev_filter.argtypes = [vp, vp, vp, i32, dbl, dbl, i32, vp]
         L.si_max_eigenpair_residual.argtypes = [vp, vp, vp, vp, i32, C.POINTER(dbl)]
         L.si_lobpcg_solve.argtypes = [vp, vp, i32, u64, i32, dbl, vp, vp, C.POINTER(_Report)]
+        L.si_lobpcg_sinv_solve.argtypes = [vp, vp, i32, u64, vp, vp, C.POINTER(_Report)]
         L.si_solve_eigenproblem.argtypes = [vp, vp, i32, u64, C.POINTER(_Config), vp, vp,
                                             C.POINTER(_Report)]
         L.si_rsvd_solve.argtypes = [vp, vp, i32, u64, i32, i32, vp, vp, C.POINTER(_Report)]
@@ -197,6 +200,7 @@ def lib():
         L.si_test_fill_normal.argtypes = [vp, vp, i64, i32, u64, C.c_uint32]
         L.si_test_fast_solve.argtypes = [vp, vp, i32, u64, C.POINTER(_Config), vp, vp,
                                          C.POINTER(_Report)]
+        L.si_test_band_solve.argtypes = [vp, vp, dbl, vp, vp, i32, C.POINTER(i32)]
         L.si_profile_spmm_begin.argtypes = [vp]
         L.si_profile_spmm_end.argtypes = [vp, C.POINTER(_SpmmProfile)]
         for name in EXPORTED:
@@ -355,6 +359,20 @@ def lobpcg_solve(lap: Laplacian, n_components: int, seed: int, regularize=False,
     rep.struct_size = C.sizeof(_Report)
     lap.ctx._check(lib().si_lobpcg_solve(lap.ctx.handle, lap.handle, n_components, seed,
                                          int(regularize), tol, _ptr(ev), _ptr(vec), C.byref(rep)))
+    return ev, vec, Report._from(rep, k)
+
+
+def lobpcg_sinv_solve(lap: Laplacian, n_components: int, seed: int):
+    """lobpcg_sinv_solve (src/solvers/sinv.rs:106-189): shift-and-invert LOBPCG
+    on (L + 1e-4 I)^-1, device FP64 (banded Cholesky).  Raises
+    ConvergenceFailure for the reference's None."""
+    k = n_components + 1
+    ev = np.empty(k)
+    vec = _evecs(lap, k)
+    rep = _Report()
+    rep.struct_size = C.sizeof(_Report)
+    lap.ctx._check(lib().si_lobpcg_sinv_solve(lap.ctx.handle, lap.handle, n_components, seed,
+                                              _ptr(ev), _ptr(vec), C.byref(rep)))
     return ev, vec, Report._from(rep, k)
 
 
@@ -519,6 +537,19 @@ def spmm_shapes(lap: "Laplacian", shape16: int = 0, shape32: int = 0, autotune: 
                                              int(shape32), int(bool(autotune)), C.byref(a),
                                              C.byref(b)))
     return a.value, b.value
+
+
+def band_solve(lap: "Laplacian", x, shift: float = 1e-4):
+    """Testing seam: (L + shift I)^-1 x for a device [n x k] f64 block through the
+    banded Cholesky of the shift-and-invert level; returns (y, bandwidth)."""
+    import torch
+    x = x.contiguous()
+    k = x.shape[1] if x.dim() == 2 else 1
+    y = torch.empty_like(x)
+    bw = C.c_int32(0)
+    lap.ctx._check(lib().si_test_band_solve(lap.ctx.handle, lap.handle, float(shift), _ptr(x),
+                                            _ptr(y), k, C.byref(bw)))
+    return y, bw.value
 
 
 # ---------------------------------------------------------------- learned warm start

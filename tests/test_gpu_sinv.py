@@ -1,0 +1,111 @@
+"""GPU tests of the shift-and-invert LOBPCG (escalation level 2,
+src/solvers/sinv.rs:106-189) through si_lobpcg_sinv_solve.
+
+The reference's own bar (tests/integration/test_comp_g_sinv.rs:36-93): on every
+connected fixture, eigenvalues within 1e-8 of the fixture eigenvalues and
+max residual ||Lv - lambda v|| below 1e-8; disconnected fixtures are skipped
+there and here.  Ring C_2500 must give non-zero eigenvectors
+(test_solver_invariants.rs:255-270); rings and paths are also checked against
+their closed-form spectra, including one stored in reverse Cuthill-McKee order
+(n >= 4096, bandwidth 2).  The chain test forces level 1 to fail (one filter
+round) and checks that level 2 produces the result.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import load_fixture
+from helpers import path_graph, random_symmetric_graph, ring_eigenvalues, ring_graph
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+si = pytest.importorskip("spectral_mi355x")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    c = si.Context(0)
+    yield c
+    c.close()
+
+
+def _residual(ip, ix, w, ev, vec):
+    L, _, _ = O.laplacian_from_graph(ip, ix, w)
+    return O.max_eigenpair_residual(L, np.asarray(ev), vec.cpu().numpy())
+
+
+@pytest.mark.parametrize("name", ["near_dupes_100", "blobs_connected_200", "circles_300",
+                                  "moons_200", "blobs_connected_2000"])
+def test_sinv_matches_fixture_eigenvalues(ctx, name):
+    z = load_fixture(name)
+    assert int(z["comp_n"]) == 1
+    ip, ix, w = z["w_indptr"], z["w_indices"], z["w_data"]
+    lap = si.Laplacian(ctx, ip, ix, w)
+    print(f"sinv fixture {name} n={lap.n}", flush=True)
+    ev, vec, rep = si.lobpcg_sinv_solve(lap, 2, 42)
+    ref = np.asarray(z["eig_values"])
+    k = min(len(ev), len(ref))
+    np.testing.assert_allclose(ev[:k], ref[:k], rtol=0, atol=1e-8)
+    assert np.all(np.diff(ev) >= -1e-12)
+    res = _residual(ip, ix, w, ev, vec)
+    assert res < 1e-8, res
+    assert rep.level == 2 and rep.max_residual < 1e-8
+
+
+def test_sinv_ring_2500_non_zero_and_closed_form(ctx):
+    n = 2500
+    ip, ix, w = ring_graph(n)
+    lap = si.Laplacian(ctx, ip, ix, w)
+    ev, vec, rep = si.lobpcg_sinv_solve(lap, 2, 42)
+    v = vec.cpu().numpy()
+    assert np.all(np.linalg.norm(v, axis=0) > 1e-6)
+    np.testing.assert_allclose(ev, ring_eigenvalues(n, 3), rtol=0, atol=1e-8)
+    assert _residual(ip, ix, w, ev, vec) < 1e-8
+
+
+@pytest.mark.parametrize("kind,n", [("ring", 6000), ("path", 5000)])
+def test_sinv_locality_ordered_band(ctx, kind, n):
+    # n >= 4096: stored in reverse Cuthill-McKee order, bandwidth 1 (path) or 2 (ring)
+    ip, ix, w = ring_graph(n) if kind == "ring" else path_graph(n)
+    lap = si.Laplacian(ctx, ip, ix, w)
+    print(f"sinv {kind} n={n}", flush=True)
+    ev, vec, rep = si.lobpcg_sinv_solve(lap, 2, 42)
+    if kind == "ring":
+        ref = ring_eigenvalues(n, 3)
+    else:
+        ref = 1.0 - np.cos(np.pi * np.arange(3) / (n - 1))
+    np.testing.assert_allclose(ev, ref, rtol=0, atol=1e-9)
+    assert _residual(ip, ix, w, ev, vec) < 1e-8
+
+
+def test_sinv_returns_none_when_band_too_large(ctx):
+    # a random graph keeps a wide band under any ordering: sinv.rs returns None
+    # when its factorisation is unavailable, here when the band exceeds the cap
+    ip, ix, w = random_symmetric_graph(60000, 8, seed=11)
+    lap = si.Laplacian(ctx, ip, ix, w)
+    with pytest.raises(si.ConvergenceFailure):
+        si.lobpcg_sinv_solve(lap, 2, 42)
+
+
+def test_sinv_deterministic(ctx):
+    z = load_fixture("blobs_connected_200")
+    lap = si.Laplacian(ctx, z["w_indptr"], z["w_indices"], z["w_data"])
+    ev1, v1, _ = si.lobpcg_sinv_solve(lap, 2, 42)
+    ev2, v2, _ = si.lobpcg_sinv_solve(lap, 2, 42)
+    assert np.array_equal(ev1, ev2)
+    assert torch.equal(v1, v2)
+
+
+def test_chain_escalates_to_level_2(ctx):
+    # level 0 off (dense threshold 0) and level 1 capped at one filter round, so
+    # the chain reaches the shift-and-invert solve (solvers/mod.rs:131-153)
+    n = 5000
+    ip, ix, w = ring_graph(n)
+    lap = si.Laplacian(ctx, ip, ix, w)
+    cfg = si.SpectralInitConfig(tol=1e-6, dense_threshold=0, max_outer=1)
+    ev, vec, rep = si.solve_eigenproblem(lap, 2, 42, cfg)
+    assert rep.level == 2, rep.level
+    np.testing.assert_allclose(ev, ring_eigenvalues(n, 3), rtol=0, atol=1e-9)
+    assert rep.max_residual < 1e-6
