@@ -109,3 +109,30 @@ def test_chain_escalates_to_level_2(ctx):
     assert rep.level == 2, rep.level
     np.testing.assert_allclose(ev, ring_eigenvalues(n, 3), rtol=0, atol=1e-9)
     assert rep.max_residual < 1e-6
+
+
+@pytest.mark.parametrize("kind,n", [("ring", 2500), ("ring", 6000), ("path", 5000), ("fixture", 0)])
+def test_band_solve_matches_scipy(ctx, kind, n):
+    # the operator of sinv.rs:139-153: (L + 1e-4 I)^-1 applied to a block,
+    # against scipy's sparse direct solve on the oracle's FP64 Laplacian
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    if kind == "fixture":
+        z = load_fixture("blobs_connected_2000")
+        ip, ix, w = z["w_indptr"], z["w_indices"], z["w_data"]
+    else:
+        ip, ix, w = ring_graph(n) if kind == "ring" else path_graph(n)
+    lap = si.Laplacian(ctx, ip, ix, w)
+    L, _, _ = O.laplacian_from_graph(ip, ix, w)
+    nn = lap.n
+    m = sp.csr_matrix((L[2], L[1], L[0]), shape=(nn, nn)) + 1e-4 * sp.eye(nn)
+    x = torch.from_numpy(np.random.default_rng(nn).standard_normal((nn, 3))).cuda()
+    y, bw = si.band_solve(lap, x, 1e-4)
+    ref = sla.spsolve(m.tocsc(), x.cpu().numpy())
+    # cond(M) <= 2e4: a backward-stable solve is accurate to ~cond * eps
+    err = np.abs(y.cpu().numpy() - ref).max() / np.abs(ref).max()
+    assert err < 1e-10, (bw, err)
+    if kind == "path":
+        assert bw == 1
+    if kind == "ring" and n >= 4096:
+        assert bw == 2
