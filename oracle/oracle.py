@@ -338,7 +338,7 @@ def spectral_init(indptr, indices, data, n_components, seed, raw_data=None, mode
 
 
 # ---------------------------------------------------------------- warm-start network
-MPNN_HID, MPNN_NF, MPNN_NOUT = 64, 16, 2
+MPNN_HID, MPNN_NF, MPNN_NOUT = 64, 16, 8
 
 
 def adjacency_from_graph(indptr, indices, data, mode: int = 0):

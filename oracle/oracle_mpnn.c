@@ -22,13 +22,18 @@
  *
  * Network (hidden 64, tanh), parameters packed in this order:
  *   layer l = 1..3: Ws_l [d_in x 64], Wn_l [d_in x 64], b_l [64]  (d_in = 16, 64, 64)
- *   output: Wo [64 x 2]
+ *   output: Wo [64 x 8]
  *   H_0 = F;  A_l = A_hat H_{l-1};  H_l = tanh(H_{l-1} Ws_l + A_l Wn_l + b_l);  X = H_3 Wo
  *
- * Loss against an orthonormal target basis V [n x 2] (the two non-trivial
- * eigenvectors the solver returns): for each column x_j,
- *   f_j = ||x_j - V V^T x_j||^2 / ||x_j||^2,   dec = (x_1.x_2)^2 / (||x_1||^2 ||x_2||^2)
- *   loss = f_1 + f_2 + dec   (invariant to the scale, sign and rotation of V)
+ * The 8 output columns are a subspace, not the eigenvectors themselves: the
+ * warm start takes the Rayleigh-Ritz vectors of [D^{1/2} 1 | X].  The loss is
+ * how much of the target V [n x 2] (the two non-trivial eigenvectors the
+ * solver returns) lies outside span(X), with a ridge that keeps the Gram
+ * matrix invertible:
+ *   G = X^T X,  B = X^T V,  delta = RIDGE tr(G) / 8,  C = (G + delta I)^-1 B
+ *   loss = tr(V^T V) - tr(B^T C)     (sum_j ||v_j - P_X v_j||^2 as RIDGE -> 0)
+ *   dloss/dX = -2 V C^T + 2 X (C C^T + (RIDGE / 4) tr(C^T C) I)
+ * invariant to the scale of X and to the sign and rotation of V.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -38,7 +43,8 @@
 
 #define HID 64
 #define NF 16
-#define NOUT 2
+#define NOUT 8
+#define RIDGE 1e-3
 
 int64_t or_mpnn_param_count(void) {
     return (int64_t)(NF * HID * 2 + HID) + 2 * (int64_t)(HID * HID * 2 + HID) + HID * NOUT;
@@ -159,37 +165,76 @@ void or_mpnn_forward(int64_t n, const int64_t* ip, const int32_t* ix, const doub
     if (h) memcpy(h, f, sizeof(double) * (size_t)n * NF);
 }
 
-double or_mpnn_loss(int64_t n, const double* x, const double* v, double* dx) {
-    double s[2] = {0, 0}, g[2][2] = {{0, 0}, {0, 0}}, c = 0.0;
-    for (int64_t i = 0; i < n; ++i) {
-        for (int j = 0; j < 2; ++j) {
-            s[j] += x[i * 2 + j] * x[i * 2 + j];
-            for (int t = 0; t < 2; ++t) g[j][t] += v[i * 2 + t] * x[i * 2 + j];
-        }
-        c += x[i * 2] * x[i * 2 + 1];
-    }
-    double f[2];
-    for (int j = 0; j < 2; ++j) {
-        double r2 = s[j] - g[j][0] * g[j][0] - g[j][1] * g[j][1];
-        if (r2 < 0.0) r2 = 0.0;
-        f[j] = r2 / s[j];
-    }
-    const double dec = c * c / (s[0] * s[1]);
-    if (dx) {
-        for (int64_t i = 0; i < n; ++i) {
-            for (int j = 0; j < 2; ++j) {
-                const double xj = x[i * 2 + j], xo = x[i * 2 + (1 - j)];
-                const double proj = v[i * 2] * g[j][0] + v[i * 2 + 1] * g[j][1];
-                double d = 2.0 * (xj - proj) / s[j] - 2.0 * f[j] * xj / s[j];
-                d += 2.0 * c * xo / (s[0] * s[1]) - 2.0 * dec * xj / s[j];
-                dx[i * 2 + j] = d;
-            }
+/* Cholesky solve of the SPD system a [m x m] c = b [m x r] in place of b */
+static int chol_solve(int m, double* a, double* b, int r) {
+    for (int j = 0; j < m; ++j) {
+        double d = a[j * m + j];
+        for (int k = 0; k < j; ++k) d -= a[j * m + k] * a[j * m + k];
+        if (!(d > 0.0)) return -1;
+        d = sqrt(d);
+        a[j * m + j] = d;
+        for (int i = j + 1; i < m; ++i) {
+            double e = a[i * m + j];
+            for (int k = 0; k < j; ++k) e -= a[i * m + k] * a[j * m + k];
+            a[i * m + j] = e / d;
         }
     }
-    return f[0] + f[1] + dec;
+    for (int c = 0; c < r; ++c) {
+        for (int i = 0; i < m; ++i) {
+            double e = b[i * r + c];
+            for (int k = 0; k < i; ++k) e -= a[i * m + k] * b[k * r + c];
+            b[i * r + c] = e / a[i * m + i];
+        }
+        for (int i = m - 1; i >= 0; --i) {
+            double e = b[i * r + c];
+            for (int k = i + 1; k < m; ++k) e -= a[k * m + i] * b[k * r + c];
+            b[i * r + c] = e / a[i * m + i];
+        }
+    }
+    return 0;
 }
 
-/* Backward pass for a loss gradient dx [n x 2]; grad (packed like params). */
+double or_mpnn_loss(int64_t n, const double* x, const double* v, double* dx) {
+    double g[NOUT * NOUT] = {0}, b[NOUT * 2] = {0}, vv = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double* xi = x + i * NOUT;
+        const double* vi = v + i * 2;
+        for (int p = 0; p < NOUT; ++p) {
+            for (int q = 0; q < NOUT; ++q) g[p * NOUT + q] += xi[p] * xi[q];
+            b[p * 2] += xi[p] * vi[0];
+            b[p * 2 + 1] += xi[p] * vi[1];
+        }
+        vv += vi[0] * vi[0] + vi[1] * vi[1];
+    }
+    double tr = 0.0;
+    for (int p = 0; p < NOUT; ++p) tr += g[p * NOUT + p];
+    const double delta = RIDGE * tr / NOUT;
+    double gd[NOUT * NOUT], c[NOUT * 2];
+    memcpy(gd, g, sizeof gd);
+    for (int p = 0; p < NOUT; ++p) gd[p * NOUT + p] += delta;
+    memcpy(c, b, sizeof c);
+    if (chol_solve(NOUT, gd, c, 2) != 0) return NAN;
+    double btc = 0.0, ctc = 0.0;
+    for (int e = 0; e < NOUT * 2; ++e) { btc += b[e] * c[e]; ctc += c[e] * c[e]; }
+    if (dx) {
+        double pm[NOUT * NOUT], qm[2 * NOUT];
+        for (int p = 0; p < NOUT; ++p)
+            for (int q = 0; q < NOUT; ++q)
+                pm[p * NOUT + q] = 2.0 * (c[p * 2] * c[q * 2] + c[p * 2 + 1] * c[q * 2 + 1]) +
+                                   (p == q ? 2.0 * RIDGE / NOUT * ctc : 0.0);
+        for (int t = 0; t < 2; ++t)
+            for (int q = 0; q < NOUT; ++q) qm[t * NOUT + q] = -2.0 * c[q * 2 + t];
+        for (int64_t i = 0; i < n; ++i)
+            for (int q = 0; q < NOUT; ++q) {
+                double d = v[i * 2] * qm[q] + v[i * 2 + 1] * qm[NOUT + q];
+                for (int p = 0; p < NOUT; ++p) d += x[i * NOUT + p] * pm[p * NOUT + q];
+                dx[i * NOUT + q] = d;
+            }
+    }
+    return vv - btc;
+}
+
+/* Backward pass for a loss gradient dx [n x 8]; grad (packed like params). */
 void or_mpnn_backward(int64_t n, const int64_t* ip, const int32_t* ix, const double* a,
                       const double* f, const double* params, const double* h,
                       const double* av, const double* dx, double* grad) {

@@ -152,14 +152,22 @@ __global__ void k_mpnn_deflate(int64_t n, double* __restrict__ p, const double* 
 }
 
 // dX = [X | V] coef:  dx[i][j] = sum_t X[i][t] cx[t][j] + sum_t V[i][t] cv[t][j]
+// dX = X P + V Q row by row: P [8 x 8], Q [2 x 8] (the coverage-loss gradient)
 __global__ void k_mpnn_dx(int64_t n, const float* __restrict__ x, const double* __restrict__ v,
-                          double c00, double c01, double c10, double c11, double v00, double v01,
-                          double v10, double v11, float* __restrict__ dx) {
+                          MpnnDxCoef c, float* __restrict__ dx) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double x0 = x[2 * i], x1 = x[2 * i + 1], w0 = v[2 * i], w1 = v[2 * i + 1];
-    dx[2 * i] = (float)(x0 * c00 + x1 * c10 + w0 * v00 + w1 * v10);
-    dx[2 * i + 1] = (float)(x0 * c01 + x1 * c11 + w0 * v01 + w1 * v11);
+    double xi[kMpnnOut];
+#pragma unroll
+    for (int p = 0; p < kMpnnOut; ++p) xi[p] = x[kMpnnOut * i + p];
+    const double w0 = v[2 * i], w1 = v[2 * i + 1];
+#pragma unroll
+    for (int q = 0; q < kMpnnOut; ++q) {
+        double d = w0 * c.q[q] + w1 * c.q[kMpnnOut + q];
+#pragma unroll
+        for (int p = 0; p < kMpnnOut; ++p) d += xi[p] * c.p[p * kMpnnOut + q];
+        dx[kMpnnOut * i + q] = (float)d;
+    }
 }
 
 // FP32 master weights -> bf16 [K x Q] (wt = 0) or its transpose [Q x K] (wt = 1)
@@ -227,10 +235,9 @@ void mpnn_deflate(int64_t n, double* p, const double* v, double alpha, float* pf
     SI_LAUNCH_CHECK();
 }
 
-void mpnn_dx(int64_t n, const float* x, const double* v, const double cx[4], const double cv[4],
-             float* dx, hipStream_t s) {
-    hipLaunchKernelGGL(k_mpnn_dx, g256(n), dim3(256), 0, s, n, x, v, cx[0], cx[1], cx[2], cx[3],
-                       cv[0], cv[1], cv[2], cv[3], dx);
+void mpnn_dx(int64_t n, const float* x, const double* v, const MpnnDxCoef& c, float* dx,
+             hipStream_t s) {
+    hipLaunchKernelGGL(k_mpnn_dx, g256(n), dim3(256), 0, s, n, x, v, c, dx);
     SI_LAUNCH_CHECK();
 }
 

@@ -5,11 +5,14 @@
 // The reference starts LOBPCG from N(0,1) plus the sqrt-degree hint
 // (src/solvers/lobpcg.rs:243-252); here a 3-layer message-passing network
 // (hidden 64, tanh; node features: degree, local normalised-weight statistics
-// and 8 random-walk probes) predicts the two non-trivial eigenvector
-// directions.  Dense layers run on the bf16 matrix cores (k_mpnn.hip), the
-// neighbour aggregation A_hat H = (I - L) H on the FP32 SpMM, the weight
-// gradients on the FP64 Gram kernel.  The block it proposes is used only if
-// its Rayleigh-Ritz residual beats a random start's.
+// and 8 random-walk probes) maps each node to 8 outputs whose span should
+// contain the two non-trivial eigenvectors (a subspace-coverage loss).  The
+// warm start evaluates it on three independent probe sets and takes the
+// Rayleigh-Ritz vectors of [D^{1/2} 1 | the 24 outputs].  Dense layers run on
+// the bf16 matrix cores (k_mpnn.hip), the neighbour aggregation
+// A_hat H = (I - L) H on the FP32 SpMM, the weight gradients on the FP64 Gram
+// kernel.  The block it proposes is used only if its Rayleigh-Ritz residual
+// beats a random start's.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -21,7 +24,11 @@ namespace si {
 
 namespace {
 
-constexpr int kHid = 64, kNf = 16, kNout = 2;
+constexpr int kHid = 64, kNf = 16, kNout = kMpnnOut;
+// independent probe sets the warm start evaluates the network on
+constexpr int kProbeSets = 3;
+// ridge of the coverage loss: G + kRidge tr(G) / kNout I
+constexpr double kRidge = 1e-3;
 // walk lengths at which the 8 random-walk probes are recorded (multi-scale)
 constexpr int kProbeSteps[8] = {1, 2, 3, 4, 6, 8, 12, 16};
 
@@ -53,7 +60,7 @@ struct Mpnn {
     DevBuf<float> p, m, v;   // master parameters, Adam moments (FP32)
     DevBuf<double> grad;     // gradient (FP64)
     DevBuf<short> w[3][4];   // layer l: Ws, Wn (bf16 [din x 64]), WsT, WnT (bf16 [64 x din])
-    DevBuf<double> wo, wot;  // output weights [64 x 2] and [2 x 64] for the block products
+    DevBuf<double> wo, wot;  // output weights [64 x 8] and [8 x 64] for the block products
     int step = 0;
     bool packed = false;
     int64_t cap = 0;
@@ -175,7 +182,7 @@ void mpnn_features(const DevLaplacian& L, uint64_t seed, float* f, hipStream_t s
     SI_HIP(hipStreamSynchronize(s));
 }
 
-// X = net(F) [n x 2], activations kept for the backward pass
+// X = net(F) [n x 8], activations kept for the backward pass
 void mpnn_forward(Mpnn& M, const DevLaplacian& L, const float* f, hipStream_t s) {
     const int64_t n = L.n;
     ensure(M, n, s);
@@ -192,33 +199,69 @@ void mpnn_forward(Mpnn& M, const DevLaplacian& L, const float* f, hipStream_t s)
     block_mul_f32(n, M.h[2], kHid, kHid, M.wo, kNout, nullptr, 0, 0, nullptr, M.x, kNout, s);
 }
 
-// Subspace loss against the orthonormal target V [n x 2] (FP64, Laplacian row
-// order) and its gradient in M.dx; returns the loss.
+// Cholesky solve of the SPD system a [m x m] c = b [m x r], b overwritten by c
+bool chol_solve_small(int m, double* a, double* b, int r) {
+    for (int j = 0; j < m; ++j) {
+        double d = a[j * m + j];
+        for (int k = 0; k < j; ++k) d -= a[j * m + k] * a[j * m + k];
+        if (!(d > 0.0)) return false;
+        d = std::sqrt(d);
+        a[j * m + j] = d;
+        for (int i = j + 1; i < m; ++i) {
+            double e = a[i * m + j];
+            for (int k = 0; k < j; ++k) e -= a[i * m + k] * a[j * m + k];
+            a[i * m + j] = e / d;
+        }
+    }
+    for (int c = 0; c < r; ++c) {
+        for (int i = 0; i < m; ++i) {
+            double e = b[i * r + c];
+            for (int k = 0; k < i; ++k) e -= a[i * m + k] * b[k * r + c];
+            b[i * r + c] = e / a[i * m + i];
+        }
+        for (int i = m - 1; i >= 0; --i) {
+            double e = b[i * r + c];
+            for (int k = i + 1; k < m; ++k) e -= a[k * m + i] * b[k * r + c];
+            b[i * r + c] = e / a[i * m + i];
+        }
+    }
+    return true;
+}
+
+// Coverage loss of the target V [n x 2] (FP64, Laplacian row order) by
+// span(X), X = the network's 8 outputs, and its gradient in M.dx:
+//   G = X^T X, B = X^T V, delta = kRidge tr(G) / 8, C = (G + delta I)^-1 B,
+//   loss = tr(V^T V) - tr(B^T C),
+//   dX = X (2 C C^T + (2 kRidge / 8) tr(C^T C) I) - 2 V C^T.
 double mpnn_loss(Mpnn& M, int64_t n, const double* vt, hipStream_t s) {
-    convert_f32_to_f64(M.x, M.xd, n * kNout, s);
-    double* g = ws_get<double>(kWsSmall6, 8);
-    double* ws = ws_get<double>(kWsGram, gram_workspace_doubles(n, 2, 2));
-    gram_f64(n, M.xd, 2, 2, M.xd, 2, 2, g, ws, s);
-    gram_f64(n, vt, 2, 2, M.xd, 2, 2, g + 4, ws, s);
-    double h[8];
-    to_host(h, g, 8, s);
-    const double sj[2] = {h[0], h[3]}, c = h[1];
-    // gv[t][j] = v_t . x_j
-    const double gv[2][2] = {{h[4], h[5]}, {h[6], h[7]}};
-    double f[2];
-    for (int j = 0; j < 2; ++j) {
-        double r2 = sj[j] - gv[0][j] * gv[0][j] - gv[1][j] * gv[1][j];
-        f[j] = std::max(r2, 0.0) / sj[j];
-    }
-    const double dec = c * c / (sj[0] * sj[1]);
-    double cx[4], cv[4];  // [t][j] row-major
-    for (int j = 0; j < 2; ++j) {
-        cx[j * 2 + j] = 2.0 / sj[j] - 2.0 * f[j] / sj[j] - 2.0 * dec / sj[j];
-        cx[(1 - j) * 2 + j] = 2.0 * c / (sj[0] * sj[1]);
-        for (int t = 0; t < 2; ++t) cv[t * 2 + j] = -2.0 * gv[t][j] / sj[j];
-    }
-    mpnn_dx(n, M.x, vt, cx, cv, M.dx, s);
-    return f[0] + f[1] + dec;
+    constexpr int P = kNout;
+    convert_f32_to_f64(M.x, M.xd, n * P, s);
+    double* g = ws_get<double>(kWsSmall6, P * P + 2 * P + 4);
+    double* ws = ws_get<double>(kWsGram, gram_workspace_doubles(n, P, P));
+    gram_f64(n, M.xd, P, P, M.xd, P, P, g, ws, s);
+    gram_f64(n, M.xd, P, P, vt, 2, 2, g + P * P, ws, s);
+    gram_f64(n, vt, 2, 2, vt, 2, 2, g + P * P + 2 * P, ws, s);
+    std::vector<double> h(P * P + 2 * P + 4);
+    to_host(h.data(), g, h.size(), s);
+    const double* G = h.data();
+    const double* B = h.data() + P * P;
+    const double vv = h[P * P + 2 * P] + h[P * P + 2 * P + 3];
+    double tr = 0.0;
+    for (int p = 0; p < P; ++p) tr += G[p * P + p];
+    std::vector<double> gd(G, G + P * P), c(B, B + 2 * P);
+    for (int p = 0; p < P; ++p) gd[p * P + p] += kRidge * tr / P;
+    if (!chol_solve_small(P, gd.data(), c.data(), 2)) return NAN;
+    double btc = 0.0, ctc = 0.0;
+    for (int e = 0; e < 2 * P; ++e) { btc += B[e] * c[e]; ctc += c[e] * c[e]; }
+    MpnnDxCoef k{};
+    for (int p = 0; p < P; ++p)
+        for (int q = 0; q < P; ++q)
+            k.p[p * P + q] = 2.0 * (c[p * 2] * c[q * 2] + c[p * 2 + 1] * c[q * 2 + 1]) +
+                             (p == q ? 2.0 * kRidge / P * ctc : 0.0);
+    for (int t = 0; t < 2; ++t)
+        for (int q = 0; q < P; ++q) k.q[t * P + q] = -2.0 * c[q * 2 + t];
+    mpnn_dx(n, M.x, vt, k, M.dx, s);
+    return vv - btc;
 }
 
 // Backward pass from M.dx; gradient (FP64, packed like the parameters) in M.grad.
@@ -345,7 +388,7 @@ si_status si_mpnn_forward(si_context* ctx, si_mpnn* m, const si_laplacian* lap, 
         si::DevBuf<float> fi((size_t)L.n * 16);
         si::rows_to_internal(L, f, 16, fi.get(), 16, 16, s);
         si::mpnn_forward(m->m, L, fi, s);
-        si::rows_to_external(L, m->m.x.get(), 2, x, 2, 2, s);
+        si::rows_to_external(L, m->m.x.get(), si::kMpnnOut, x, si::kMpnnOut, si::kMpnnOut, s);
         SI_HIP(hipStreamSynchronize(s));
     });
 }
@@ -393,40 +436,48 @@ si_status si_mpnn_warm_start(si_context* ctx, si_mpnn* m, const si_laplacian* la
         const si::DevLaplacian& L = lap->L;
         const int64_t n = L.n;
         const int k = 3;
+        const int W = 1 + si::kMpnnOut * si::kProbeSets;
         hipStream_t s = ctx->c.stream;
+        // candidate subspaces: [sqrt_deg | X(probe set 0) | X(set 1) | X(set 2)] and
+        // the solver's own random start [sqrt_deg | N(0,1)]
         si::DevBuf<float> fi((size_t)n * 16);
-        si::mpnn_features(L, seed, fi, s);
-        si::mpnn_forward(m->m, L, fi, s);
-        // candidate blocks [sqrt_deg | X] and [sqrt_deg | N(0,1)], Rayleigh-Ritz, FP64 residual
-        si::DevBuf<double> bn((size_t)n * k), br((size_t)n * k);
+        si::DevBuf<double> bn((size_t)n * W), br((size_t)n * k);
+        si::set_column_f64(bn, W, n, 0, L.sqrt_deg, 1.0, s);
+        for (int set = 0; set < si::kProbeSets; ++set) {
+            si::mpnn_features(L, seed + (uint64_t)set, fi, s);
+            si::mpnn_forward(m->m, L, fi, s);
+            si::convert_f32_to_f64(m->m.x, m->m.xd, n * si::kMpnnOut, s);
+            si::copy_cols_f64(m->m.xd, si::kMpnnOut, bn.get() + 1 + set * si::kMpnnOut, W, n,
+                              si::kMpnnOut, s);
+        }
         si::fill_normal_f64(br, n, k, k, seed, si::kStreamLobpcgInit, s, L.perm);
         si::set_column_f64(br, k, n, 0, L.sqrt_deg, 1.0, s);
-        si::convert_f32_to_f64(m->m.x, m->m.xd, n * 2, s);
-        si::set_column_f64(bn, k, n, 0, L.sqrt_deg, 1.0, s);
-        si::copy_cols_f64(m->m.xd, 2, bn.get() + 1, k, n, 2, s);
         double ev[3], rs[3];
-        // Rayleigh-Ritz of a block that need not be orthonormal:
-        // (B^T L B) c = theta (B^T B) c, B <- B C, then the FP64 residuals
-        si::DevBuf<double> lb((size_t)n * k), gg((size_t)k * 2 * k), th(k), cm((size_t)k * k),
-            gws(si::gram_workspace_doubles(n, k, 2 * k));
+        // Rayleigh-Ritz of a subspace basis that need not be orthonormal:
+        // (B^T L B) c = theta (B^T B) c; the k lowest Ritz vectors go to out,
+        // judged by their FP64 residuals
+        si::DevBuf<double> lb((size_t)n * W), gg((size_t)W * 2 * W), th(W), cm((size_t)W * W),
+            csub((size_t)W * k), gws(si::gram_workspace_doubles(n, W, 2 * W)), on((size_t)n * k),
+            orr((size_t)n * k);
         si::DevBuf<int> stt(1);
-        auto judge = [&](double* b) -> double {
-            si::spmm_f64(L, b, lb, k, k, 1.0, 0.0, 0.0, nullptr, s);
-            si::gram2_f64(n, b, k, k, b, k, k, lb, k, k, gg, gws, s);
-            si::small_gen_eig_scaled(k, gg.get() + k, 2 * k, gg, 2 * k, th, cm, stt, s);
+        auto judge = [&](const double* b, int w, double* out) -> double {
+            si::spmm_f64(L, b, lb, w, w, 1.0, 0.0, 0.0, nullptr, s);
+            si::gram2_f64(n, b, w, w, b, w, w, lb, w, w, gg, gws, s);
+            si::small_gen_eig_scaled(w, gg.get() + w, 2 * w, gg, 2 * w, th, cm, stt, s);
             int st = 0;
             si::to_host(&st, stt.get(), 1, s);
             if (st != 0) return INFINITY;
-            si::block_mul_f64(n, b, k, k, cm, k, nullptr, 0, 0, nullptr, b, k, s);
+            si::small_copy_sub(w, k, cm, w, csub, k, 1.0, s);
+            si::block_mul_f64(n, b, w, w, csub, k, nullptr, 0, 0, nullptr, out, k, s);
             si::to_host(ev, th.get(), k, s);
-            si::fp64_residuals(L, b, k, ev, rs, s);
-            double w = 0.0;
-            for (double r : rs) w = std::isnan(r) ? INFINITY : std::max(w, r);
-            return w;
+            si::fp64_residuals(L, out, k, ev, rs, s);
+            double r = 0.0;
+            for (double x : rs) r = std::isnan(x) ? INFINITY : std::max(r, x);
+            return r;
         };
-        const double rn = judge(bn), rr = judge(br);
+        const double rn = judge(bn, W, on), rr = judge(br, k, orr);
         const bool net = rn < rr;
-        si::rows_to_external(L, net ? bn.get() : br.get(), k, block, k, k, s);
+        si::rows_to_external(L, net ? on.get() : orr.get(), k, block, k, k, s);
         SI_HIP(hipStreamSynchronize(s));
         if (used_net) *used_net = net ? 1 : 0;
         if (res_net) *res_net = rn;

@@ -215,8 +215,15 @@ void mpnn_dtanh(int64_t count, const float* x, const float* h, float* y, hipStre
 void mpnn_rowfeat(const DevLaplacian& L, double inv_max_sq, float* f, hipStream_t s);
 void mpnn_setcol(int64_t n, int j, const float* p, float scale, float* f, hipStream_t s);
 void mpnn_deflate(int64_t n, double* p, const double* v, double alpha, float* pf, hipStream_t s);
-void mpnn_dx(int64_t n, const float* x, const double* v, const double cx[4], const double cv[4],
-             float* dx, hipStream_t s);
+// output columns of the warm-start network and the coefficients of its loss
+// gradient dX = X P + V Q
+constexpr int kMpnnOut = 8;
+struct MpnnDxCoef {
+    double p[kMpnnOut * kMpnnOut];
+    double q[2 * kMpnnOut];
+};
+void mpnn_dx(int64_t n, const float* x, const double* v, const MpnnDxCoef& c, float* dx,
+             hipStream_t s);
 void mpnn_pack(const float* w, int k, int q, int transpose, short* out, hipStream_t s);
 void mpnn_adam(int64_t count, float* p, const double* g, float* m, float* v, float lr, float b1,
                float b2, float eps, int step, hipStream_t s);

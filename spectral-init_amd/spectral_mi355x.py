@@ -597,7 +597,7 @@ class WarmStartNet:
 
     def forward(self, lap: "Laplacian", f):
         import torch
-        x = torch.empty((lap.n, 2), dtype=torch.float32, device="cuda")
+        x = torch.empty((lap.n, 8), dtype=torch.float32, device="cuda")
         self.ctx._check(lib().si_mpnn_forward(self.ctx.handle, self._h, lap.handle, f.data_ptr(),
                                               x.data_ptr()))
         return x

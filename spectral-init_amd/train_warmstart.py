@@ -4,15 +4,18 @@
 Training data are the solver's own converged eigenvectors: seeded Gaussian-
 mixture fuzzy 15-NN graphs (20 clusters in 50-d, Dirichlet(2) sizes; the
 construction of graphs.workload("gmm100k")) with node counts drawn
-log-uniformly from [--n-lo, --n-hi].  Each graph contributes its node features
-and the two non-trivial eigenvectors of its Laplacian (FP32-bulk solver,
-residual <= 1e-6).  Training runs the network's own forward / backward
-kernels and the device Adam step (spectral_mi355x.WarmStartNet.train_step).
+log-uniformly from [--n-lo, --n-hi].  Each graph contributes node features for
+three probe seeds and the two non-trivial eigenvectors of its Laplacian
+(FP32-bulk solver, residual <= 1e-6).  Training minimises the subspace-coverage
+loss with the network's own forward / backward kernels and the device Adam
+step (spectral_mi355x.WarmStartNet.train_step), cycling through the probe
+seeds.
 
 Evaluation on held-out seeds compares, per graph, the plain FP64 LOBPCG and
 the FP32-bulk solver started from N(0,1) draws with the same solvers started
-from the warm-start block (the network's block when its Rayleigh-Ritz residual
-beats the random start's, else the random start).
+from the warm-start block (the Rayleigh-Ritz block of the network's outputs
+when its residual beats the random start's, else the random start).  With
+--train 0 the script only evaluates the parameters given by --init.
 
     python spectral-init_amd/train_warmstart.py --out spectral-init_amd/warmstart_mpnn.npy
 """
@@ -57,7 +60,7 @@ def evaluate(si, graphs, net, seeds, n, tol):
     for s in seeds:
         ip, ix, wt = make_graph(graphs, s, n)
         lap = si.Laplacian(net.ctx, ip, ix, wt)
-        row = dict(seed=s, n=int(lap.n), nnz=int(lap.nnz))
+        row = dict(seed=s, size=int(n), n=int(lap.n), nnz=int(lap.nnz))
         for name, solver in (("plain", si.Solver.PlainLobpcg), ("fast", si.Solver.Fast)):
             cfg = si.SpectralInitConfig(tol=tol, solver=solver)
             solve(si, lap, cfg)  # warm the solver's buffers
@@ -92,10 +95,11 @@ def evaluate(si, graphs, net, seeds, n, tol):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--train", type=int, default=48, help="training graphs")
-    ap.add_argument("--n-lo", type=int, default=20000)
-    ap.add_argument("--n-hi", type=int, default=100000)
-    ap.add_argument("--epochs", type=int, default=150)
+    ap.add_argument("--train", type=int, default=400, help="training graphs (0: evaluate --init)")
+    ap.add_argument("--n-lo", type=int, default=50000)
+    ap.add_argument("--n-hi", type=int, default=200000)
+    ap.add_argument("--epochs", type=int, default=100)
+    ap.add_argument("--probe-seeds", type=int, default=3, help="feature probe seeds per graph")
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--seed0", type=int, default=1000, help="first training graph seed")
     ap.add_argument("--eval", default="50000:4,200000:4",
@@ -117,16 +121,16 @@ def main():
     sizes = np.exp(rng.uniform(math.log(args.n_lo), math.log(args.n_hi), args.train)).astype(int)
     data = []
     t0 = time.perf_counter()
-    for i, n in enumerate(sizes):
+    for i, n in enumerate(sizes if args.train > 0 else []):
         ip, ix, wt = make_graph(graphs, args.seed0 + i, int(n))
         lap = si.Laplacian(ctx, ip, ix, wt)
-        f = si.WarmStartNet.features(lap, seed=0)
+        fs = [si.WarmStartNet.features(lap, seed=p) for p in range(args.probe_seeds)]
         vec, r = solve(si, lap, si.SpectralInitConfig(tol=args.tol))
         if not r["residual"] <= args.tol:
             log(f"graph {i}: solver residual {r['residual']:.2e}, skipped")
             continue
-        data.append((lap, f, vec[:, 1:3].contiguous()))
-        if i % 8 == 0:
+        data.append((lap, fs, vec[:, 1:3].contiguous()))
+        if i % 16 == 0:
             log(f"graph {i}: n={lap.n} nnz={lap.nnz} ({time.perf_counter() - t0:.0f} s)")
     log(f"{len(data)} training graphs ready in {time.perf_counter() - t0:.0f} s")
 
@@ -134,20 +138,21 @@ def main():
     hist = []
     step = 0
     t0 = time.perf_counter()
-    for ep in range(args.epochs):
+    for ep in range(args.epochs if data else 0):
         order = rng.permutation(len(data))
         losses = []
         for j in order:
-            lap, f, v = data[j]
+            lap, fs, v = data[j]
             lr = args.lr * 0.5 * (1.0 + math.cos(math.pi * step / steps))
-            losses.append(net.train_step(lap, f, v, lr))
+            losses.append(net.train_step(lap, fs[(step + j) % len(fs)], v, lr))
             step += 1
         hist.append(float(np.mean(losses)))
         log(f"epoch {ep}: mean loss {hist[-1]:.4f}")
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
-    np.save(args.out, net.params())
-    log(f"trained {steps} steps in {train_s:.1f} s -> {args.out}")
+    if data:
+        np.save(args.out, net.params())
+        log(f"trained {steps} steps in {train_s:.1f} s -> {args.out}")
     del data
 
     rows = []
@@ -158,7 +163,7 @@ def main():
         seed += count
     summary = {}
     for r in rows:
-        d = summary.setdefault(r["n"] // 1000 * 1000, dict(graphs=0, plain_it_random=0,
+        d = summary.setdefault(r["size"], dict(graphs=0, plain_it_random=0,
                                                           plain_it_warm=0, plain_ms_random=0.0,
                                                           plain_ms_warm=0.0, net_used=0))
         d["graphs"] += 1
@@ -173,12 +178,13 @@ def main():
             f"{d['plain_it_warm'] / g:.1f}, mean ms {d['plain_ms_random'] / g:.1f} -> "
             f"{d['plain_ms_warm'] / g:.1f} (incl. warm start), network block used {d['net_used']}/{g}")
     rep = dict(train_graphs=int(args.train), n_range=[args.n_lo, args.n_hi], epochs=args.epochs,
-               lr=args.lr, steps=steps, train_seconds=train_s, loss_history=hist,
-               eval=rows, summary={str(k): v for k, v in summary.items()})
+               lr=args.lr, probe_seeds=args.probe_seeds, steps=steps, train_seconds=train_s,
+               loss_history=hist, eval=rows, summary={str(k): v for k, v in summary.items()},
+               command="python -u spectral-init_amd/train_warmstart.py " + " ".join(sys.argv[1:]))
     if args.report:
         with open(args.report, "w") as fh:
             json.dump(rep, fh, indent=1)
-    print(json.dumps({"loss_first": hist[0], "loss_last": hist[-1],
+    print(json.dumps({"loss_first": hist[0] if hist else None, "loss_last": hist[-1] if hist else None,
                       "eval": [{k: (v if not isinstance(v, dict) else
                                     {kk: v[kk] for kk in ("iterations", "ms", "residual")})
                                 for k, v in r.items()} for r in rows]}), flush=True)

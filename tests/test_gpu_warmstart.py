@@ -1,7 +1,8 @@
 """The learned warm start on the GPU against the FP64 oracle
-(oracle/oracle_mpnn.c): features, the bf16 forward pass, the gradients of the
-hand-written backward pass, Adam training and the residual-judged choice
-between the network's block and a random start."""
+(oracle/oracle_mpnn.c): features, the bf16 forward pass (8 outputs), the
+coverage loss and the gradients of the hand-written backward pass, Adam
+training and the residual-judged choice between the network's Rayleigh-Ritz
+block and a random start."""
 import numpy as np
 import pytest
 
@@ -13,7 +14,7 @@ pytestmark = pytest.mark.gpu
 si = pytest.importorskip("spectral_mi355x")
 
 BLOCKS = [("Ws1", 16 * 64), ("Wn1", 16 * 64), ("b1", 64), ("Ws2", 64 * 64), ("Wn2", 64 * 64),
-          ("b2", 64), ("Ws3", 64 * 64), ("Wn3", 64 * 64), ("b3", 64), ("Wo", 128)]
+          ("b2", 64), ("Ws3", 64 * 64), ("Wn3", 64 * 64), ("b3", 64), ("Wo", 64 * 8)]
 
 
 @pytest.fixture(scope="module")
@@ -114,10 +115,11 @@ def test_warm_start_is_judged_by_residual(ctx, gmm20k):
 
 def test_trained_network_gains_on_held_out_graphs(ctx):
     """The committed network (spectral-init_amd/warmstart_mpnn.npy, trained on
-    50K-200K-node graphs of seeds 1000-1119) against N(0,1) starts on held-out
-    seeds 5004-5007 at n = 200 000: plain LOBPCG iterations fall on average, the
-    network's block is taken only when its Rayleigh-Ritz residual is lower, and
-    every warm-started solve reaches the same residual gate."""
+    400 graphs of 50K-200K nodes, seeds 1000-1399) against N(0,1) starts on
+    held-out seeds 5004-5007 at n = 200 000: plain LOBPCG needs at most 3/4 of
+    the iterations on average (121 -> 52 when trained), the network's block is
+    taken only when its Rayleigh-Ritz residual is lower, and every warm-started
+    solve reaches the same residual gate."""
     import os
     import graphs
     path = os.path.join(os.path.dirname(si.__file__), "warmstart_mpnn.npy")
@@ -140,4 +142,4 @@ def test_trained_network_gains_on_held_out_graphs(ctx):
         it_warm.append(rw.iterations)
         print(f"  iterations random {rr.iterations} warm {rw.iterations} (net used {used})", flush=True)
         del blk, lap
-    assert np.mean(it_warm) < 0.9 * np.mean(it_rand), (it_rand, it_warm)
+    assert np.mean(it_warm) < 0.75 * np.mean(it_rand), (it_rand, it_warm)

@@ -31,7 +31,7 @@ def test_param_gradient_matches_central_differences(small):
     g = O.mpnn_backward(adj, f, p, h, av, dx)
     rng = np.random.default_rng(1)
     # every parameter block: Ws1, Wn1, b1, Ws2, Wn2, b2, Ws3, Wn3, b3, Wo
-    sizes = [16 * 64, 16 * 64, 64, 64 * 64, 64 * 64, 64, 64 * 64, 64 * 64, 64, 128]
+    sizes = [16 * 64, 16 * 64, 64, 64 * 64, 64 * 64, 64, 64 * 64, 64 * 64, 64, 64 * 8]
     offs = np.concatenate([[0], np.cumsum(sizes)])
     idx = np.concatenate([rng.integers(offs[b], offs[b + 1], 4) for b in range(len(sizes))])
     eps = 1e-6
@@ -67,7 +67,7 @@ def test_loss_gradient_matches_central_differences(small):
     x, _, _ = O.mpnn_forward(adj, f, p)
     loss, dx = O.mpnn_loss(x, v)
     eps = 1e-7
-    for i, j in [(0, 0), (7, 1), (33, 0), (59, 1)]:
+    for i, j in [(0, 0), (7, 1), (33, 5), (59, 7), (12, 3)]:
         xp, xm = x.copy(), x.copy()
         xp[i, j] += eps
         xm[i, j] -= eps
@@ -78,14 +78,35 @@ def test_loss_gradient_matches_central_differences(small):
 def test_loss_invariances(small):
     _, _, _, _, v = small
     rng = np.random.default_rng(2)
-    x = rng.standard_normal(v.shape)
+    x = rng.standard_normal((v.shape[0], 8))
     base = O.mpnn_loss(x, v, grad=False)
     c, s = np.cos(0.7), np.sin(0.7)
     assert np.isclose(O.mpnn_loss(x, v @ np.array([[c, -s], [s, c]]), grad=False), base)
     assert np.isclose(O.mpnn_loss(x, -v, grad=False), base)
     assert np.isclose(O.mpnn_loss(3.5 * x, v, grad=False), base)
-    assert O.mpnn_loss(v, v, grad=False) < 1e-12           # in the subspace, decorrelated
-    assert np.isclose(O.mpnn_loss(np.column_stack([v[:, 0], v[:, 0]]), v, grad=False), 1.0)
+    q, _ = np.linalg.qr(rng.standard_normal((8, 8)))
+    assert np.isclose(O.mpnn_loss(x @ q, v, grad=False), base)   # any orthonormal mix of X
+
+
+def test_loss_closed_forms(small):
+    """tr(V^T V) - tr(B^T (G + delta I)^-1 B) on bases where it is known."""
+    _, _, _, _, v = small
+    n = v.shape[0]
+    rng = np.random.default_rng(3)
+    w = rng.standard_normal((n, 7))
+    w -= v @ (v.T @ w)
+    w, _ = np.linalg.qr(w)                     # orthonormal, orthogonal to V
+    # X = [V | W_6]: G = I, delta = 1e-3, loss = 2 - 2 / (1 + 1e-3)
+    x = np.column_stack([v, w[:, :6]])
+    assert np.isclose(O.mpnn_loss(x, v, grad=False), 2.0 - 2.0 / 1.001, rtol=1e-12)
+    # X orthogonal to V: nothing covered
+    x = np.column_stack([w[:, :6], w[:, :2] * 0.5])
+    assert np.isclose(O.mpnn_loss(x, v, grad=False), 2.0, rtol=1e-12)
+    # one eigenvector covered, X = [v_1 | W_7] orthonormal: loss = 2 - 1 / (1 + 1e-3)
+    x = np.column_stack([v[:, 0], w])
+    assert np.isclose(O.mpnn_loss(x, v, grad=False), 2.0 - 1.0 / 1.001, rtol=1e-12)
+    # scaled to 3x: unchanged (delta scales with tr(G))
+    assert np.isclose(O.mpnn_loss(3.0 * x, v, grad=False), 2.0 - 1.0 / 1.001, rtol=1e-12)
 
 
 def test_features_definition(small):
