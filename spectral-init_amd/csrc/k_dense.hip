@@ -4,10 +4,11 @@
 // (src/solvers/mod.rs:83-109, 211-223) and by si_dense_evd.
 //
 //   1. densify L (FP64, row-major n x n, unit diagonal);
-//   2. Householder tridiagonalisation in n - 2 steps.  Step j: k_house forms
-//      the reflector of row j's tail (stored over that tail), k_symv computes
-//      p = 2 B v for the trailing block B, k_rank2 applies
-//      B -= v w^T + w v^T with w = p - (v.p) v;
+//   2. Householder tridiagonalisation, two launches per step: the next
+//      reflector (k_tri_reflect), then the rank-2 update B -= v w^T + w v^T
+//      (w = p - (v.p) v) of the trailing block fused with the next p = 2 B v
+//      (k_tri_update), so the trailing block is read and written once per
+//      step; reflectors are kept in R;
 //   3. multisection on Sturm counts for the k smallest eigenvalues of the
 //      tridiagonal matrix (one workgroup per eigenvalue, 256 points a round);
 //   4. inverse iteration for their tridiagonal eigenvectors (LU of T - lambda I
@@ -23,7 +24,6 @@ namespace {
 
 constexpr int kT = 256;
 constexpr int kWaves = kT / 64;
-constexpr int kRank2Rows = 8;
 constexpr double kEps = 2.220446049250313e-16;
 
 // Block-wide sum: wave butterfly, then the kWaves wave totals in order.  Every
@@ -38,6 +38,11 @@ __device__ double block_sum(double v, double* red) {
     return r;
 }
 
+__device__ inline double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
 // one thread per row: A[i][:] = I[i][:] + off-diagonal entries of L
 __global__ void k_densify(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                           const double* __restrict__ l64, int n, double* __restrict__ a) {
@@ -49,86 +54,205 @@ __global__ void k_densify(const int32_t* __restrict__ rp, const int32_t* __restr
         if (col[p] != i) ai[col[p]] += l64[p];
 }
 
-// Reflector of step j (one workgroup).  x = A[j][j+1..n-1], alpha = -sign(x0)|x|,
-// v = (x - alpha e1) / |x - alpha e1| written over x; d[j] = A[j][j],
-// e[j] = alpha.  vn[j] = 1 when a reflection is applied, 0 when x is already
-// a multiple of e1 (then e[j] = x0).
-__global__ __launch_bounds__(kT) void k_house(double* __restrict__ a, int n, int j,
-                                              double* __restrict__ d, double* __restrict__ e,
-                                              double* __restrict__ vn) {
+// One step j of the tridiagonalisation in two launches.  On entry the rows
+// j+1.. of A hold B_j, the trailing block of step j (full symmetric storage);
+// R row j holds the reflector v_j (vn[j] = 1) and p_in = 2 B_j v_j.
+// k_tri_reflect (one workgroup): kk = v_j . p_in, the updated first row of B_j
+// (matrix row j+1) and from it d[j+1] and the next reflector v_{j+1} into R
+// row j+1 (vn[j+1], e[j+1]).  k_tri_update: every other row of B_j becomes a
+// row of B_{j+1} = B_j - v w^T - w v^T (w = p_in - kk v_j), and its entry of
+// p_out = 2 B_{j+1} v_{j+1} is formed while the row streams through, so the
+// trailing block is read and written once per step.  j = -1 starts from the
+// densified matrix (no update).
+__global__ __launch_bounds__(kT) void k_tri_reflect(const double* __restrict__ A,
+                                                    double* __restrict__ R, int n, int j,
+                                                    double* __restrict__ d,
+                                                    double* __restrict__ e,
+                                                    double* __restrict__ vn,
+                                                    const double* __restrict__ p_in,
+                                                    double* __restrict__ scal) {
     __shared__ double red[kWaves];
     const int t = threadIdx.x;
-    double* x = a + (int64_t)j * n + (j + 1);
-    const int m = n - j - 1;
-    const double x0 = x[0];  // every thread reads x0 before block_sum's barriers
-    double s = 0.0;
-    for (int q = 1 + t; q < m; q += kT) s += x[q] * x[q];
-    const double tail2 = block_sum(s, red);
-    if (t == 0) d[j] = a[(int64_t)j * n + j];
-    if (tail2 == 0.0) {
-        if (t == 0) { e[j] = x0; vn[j] = 0.0; }
-        return;
+    const int m = n - j - 1;  // order of B_j (>= 2)
+    const bool refl = j >= 0 && vn[j] != 0.0;
+    const double* vj = refl ? R + (int64_t)j * n + (j + 1) : nullptr;
+    double kk = 0.0;
+    if (refl) {
+        double s = 0.0;
+#pragma unroll 4
+        for (int q = t; q < m; q += kT) s += vj[q] * p_in[q];
+        kk = block_sum(s, red);
     }
-    const double nx = sqrt(x0 * x0 + tail2);
-    const double alpha = x0 >= 0.0 ? -nx : nx;
-    const double v0 = x0 - alpha;
-    const double inv = 1.0 / sqrt(v0 * v0 + tail2);
-    for (int q = t; q < m; q += kT) x[q] = (q == 0 ? v0 : x[q]) * inv;
-    if (t == 0) { e[j] = alpha; vn[j] = 1.0; }
+    const double* b0 = A + (int64_t)(j + 1) * n + (j + 1);  // B_j row 0
+    const double v0 = refl ? vj[0] : 0.0;
+    const double w0 = refl ? p_in[0] - kk * v0 : 0.0;
+    auto nr = [&](int q) -> double {  // updated B_j row 0, entry q
+        double val = b0[q];
+        if (refl) val -= v0 * (p_in[q] - kk * vj[q]) + w0 * vj[q];
+        return val;
+    };
+    const double x0 = nr(1);
+    double s2 = 0.0;
+#pragma unroll 4
+    for (int q = 2 + t; q < m; q += kT) {
+        const double xq = nr(q);
+        s2 += xq * xq;
+    }
+    const double tail2 = block_sum(s2, red);
+    const bool refl1 = tail2 != 0.0;
+    double alpha = x0;
+    if (refl1) {
+        const double nx = sqrt(x0 * x0 + tail2);
+        alpha = x0 >= 0.0 ? -nx : nx;
+        const double vv0 = x0 - alpha;
+        const double inv = 1.0 / sqrt(vv0 * vv0 + tail2);
+        double* r1 = R + (int64_t)(j + 1) * n + (j + 2);
+        for (int q = t; q < m - 1; q += kT) r1[q] = (q == 0 ? vv0 : nr(q + 1)) * inv;
+    }
+    if (t == 0) {
+        d[j + 1] = nr(0);
+        e[j + 1] = alpha;
+        vn[j + 1] = refl1 ? 1.0 : 0.0;
+        scal[0] = kk;
+    }
 }
 
-// p = 2 B v over the trailing block B = A[j+1.., j+1..]; one wavefront per row
-__global__ __launch_bounds__(kT) void k_symv(const double* __restrict__ a, int n, int j,
-                                             const double* __restrict__ vn,
-                                             double* __restrict__ p) {
-    if (vn[j] == 0.0) return;
+constexpr int kRowsPerWave = 1;
+constexpr int kStepRows = kRowsPerWave * kWaves;  // rows of B_j per workgroup
+__global__ __launch_bounds__(kT) void k_tri_update(double* __restrict__ A,
+                                                   const double* __restrict__ R, int n, int j,
+                                                   const double* __restrict__ vn,
+                                                   const double* __restrict__ p_in,
+                                                   const double* __restrict__ scal,
+                                                   double* __restrict__ p_out) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int m = n - j - 1;
-    const int lane = threadIdx.x & 63;
-    const int r = blockIdx.x * kWaves + (threadIdx.x >> 6);
-    if (r >= m) return;
-    const double* brow = a + (int64_t)(j + 1 + r) * n + (j + 1);
-    const double* v = a + (int64_t)j * n + (j + 1);
-    double s = 0.0;
-    for (int q = lane; q < m; q += 64) s += brow[q] * v[q];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) p[r] = 2.0 * s;
-}
-
-// B -= v w^T + w v^T, w = p - (v.p) v.  Every workgroup computes v.p in the
-// same order, then updates its kRank2Rows rows of B.
-__global__ __launch_bounds__(kT) void k_rank2(double* __restrict__ a, int n, int j,
-                                              const double* __restrict__ vn,
-                                              const double* __restrict__ p) {
-    __shared__ double red[kWaves];
-    if (vn[j] == 0.0) return;
-    const int m = n - j - 1;
-    const double* v = a + (int64_t)j * n + (j + 1);
-    double s = 0.0;
-    for (int q = threadIdx.x; q < m; q += kT) s += v[q] * p[q];
-    const double kk = block_sum(s, red);
-    for (int rr = 0; rr < kRank2Rows; ++rr) {
-        const int r = blockIdx.x * kRank2Rows + rr;
-        if (r >= m) break;
-        const double vr = v[r];
-        const double wr = p[r] - kk * vr;
-        double* brow = a + (int64_t)(j + 1 + r) * n + (j + 1);
-        for (int q = threadIdx.x; q < m; q += kT) {
-            const double vq = v[q];
-            const double wq = p[q] - kk * vq;
-            brow[q] -= vr * wq + wr * vq;
+    const bool refl = j >= 0 && vn[j] != 0.0;
+    const bool refl1 = vn[j + 1] != 0.0;
+    const double kk = scal[0];
+    const double* vj = R + (int64_t)(j >= 0 ? j : 0) * n + (j + 1);
+    const double* vnext = R + (int64_t)(j + 1) * n + (j + 2);
+    // rows r >= 1 of B_j become rows r - 1 of B_{j+1}; a wavefront streams its
+    // rows together (independent loads in flight, v_j and p_in loaded once)
+    const int rbase = blockIdx.x * kStepRows + wv * kRowsPerWave;
+    double vr[kRowsPerWave], wr[kRowsPerWave], acc[kRowsPerWave];
+    double* rows[kRowsPerWave];
+    bool act[kRowsPerWave];
+#pragma unroll
+    for (int u = 0; u < kRowsPerWave; ++u) {
+        const int r = rbase + u;
+        act[u] = r >= 1 && r < m;
+        rows[u] = A + (int64_t)(j + 1 + (act[u] ? r : 1)) * n + (j + 1);
+        vr[u] = (refl && act[u]) ? vj[r] : 0.0;
+        wr[u] = (refl && act[u]) ? p_in[r] - kk * vr[u] : 0.0;
+        acc[u] = 0.0;
+    }
+#pragma unroll 4
+    for (int q = 1 + lane; q < m; q += 64) {
+        const double vq = refl ? vj[q] : 0.0;
+        const double wq = refl ? p_in[q] - kk * vq : 0.0;
+        const double vnq = refl1 ? vnext[q - 1] : 0.0;
+#pragma unroll
+        for (int u = 0; u < kRowsPerWave; ++u) {
+            if (!act[u]) continue;
+            double val = rows[u][q];
+            if (refl) {
+                val -= vr[u] * wq + wr[u] * vq;
+                rows[u][q] = val;
+            }
+            acc[u] += val * vnq;
         }
     }
+#pragma unroll
+    for (int u = 0; u < kRowsPerWave; ++u) {
+        const double a = wave_sum(acc[u]);
+        if (lane == 0 && act[u]) p_out[rbase + u - 1] = refl1 ? 2.0 * a : 0.0;
+    }
 }
 
-// the last 2 x 2 block of the tridiagonal matrix
-__global__ void k_tail(const double* __restrict__ a, int n, double* __restrict__ d,
-                       double* __restrict__ e) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (n >= 2) {
-        d[n - 2] = a[(int64_t)(n - 2) * n + (n - 2)];
-        e[n - 2] = a[(int64_t)(n - 2) * n + (n - 1)];
+// The remaining steps j0 .. n-3 once the trailing block has at most kFinish
+// rows, in one workgroup with the block in LDS: the same arithmetic as
+// k_tri_reflect + k_tri_update (rank-2 update, next reflector, next p) with
+// barriers instead of launches.  Entry state as for step j0 (j0 = -1: the whole
+// matrix, no pending reflector); also writes d[n-1].
+constexpr int kFinish = 128;
+__global__ __launch_bounds__(kT) void k_tri_finish(const double* __restrict__ A,
+                                                   double* __restrict__ R, int n, int j0,
+                                                   double* __restrict__ d, double* __restrict__ e,
+                                                   double* __restrict__ vn,
+                                                   const double* __restrict__ p_in) {
+    __shared__ double Bs[kFinish][kFinish + 1];
+    __shared__ double vs[kFinish], ps[kFinish], ws[kFinish];
+    __shared__ double red[kWaves];
+    const int t = threadIdx.x;
+    const int m0 = n - j0 - 1;
+    for (int x = t; x < m0 * m0; x += kT) {
+        const int r = x / m0, q = x % m0;
+        Bs[r][q] = A[(int64_t)(j0 + 1 + r) * n + (j0 + 1 + q)];
     }
-    d[n - 1] = a[(int64_t)(n - 1) * n + (n - 1)];
+    bool refl = j0 >= 0 && vn[j0] != 0.0;
+    if (refl)
+        for (int q = t; q < m0; q += kT) {
+            vs[q] = R[(int64_t)j0 * n + (j0 + 1 + q)];
+            ps[q] = p_in[q];
+        }
+    __syncthreads();
+    for (int j = j0; j + 2 < n; ++j) {
+        const int o = j - j0;  // B_j = Bs[o.., o..]
+        const int m = n - j - 1;
+        if (refl) {
+            double s = 0.0;
+            for (int q = t; q < m; q += kT) s += vs[q] * ps[q];
+            const double kk = block_sum(s, red);
+            for (int q = t; q < m; q += kT) ws[q] = ps[q] - kk * vs[q];
+            __syncthreads();
+            for (int x = t; x < m * m; x += kT) {
+                const int r = x / m, q = x % m;
+                Bs[o + r][o + q] -= vs[r] * ws[q] + ws[r] * vs[q];
+            }
+            __syncthreads();
+        }
+        const double x0 = Bs[o][o + 1];
+        double s2 = 0.0;
+        for (int q = 2 + t; q < m; q += kT) s2 += Bs[o][o + q] * Bs[o][o + q];
+        const double tail2 = block_sum(s2, red);
+        const bool refl1 = tail2 != 0.0;
+        double alpha = x0;
+        if (refl1) {
+            const double nx = sqrt(x0 * x0 + tail2);
+            alpha = x0 >= 0.0 ? -nx : nx;
+            const double vv0 = x0 - alpha;
+            const double inv = 1.0 / sqrt(vv0 * vv0 + tail2);
+            double* r1 = R + (int64_t)(j + 1) * n + (j + 2);
+            for (int q = t; q < m - 1; q += kT) {
+                const double vq = (q == 0 ? vv0 : Bs[o][o + 1 + q]) * inv;
+                ws[q] = vq;
+                r1[q] = vq;
+            }
+        }
+        if (t == 0) {
+            d[j + 1] = Bs[o][o];
+            e[j + 1] = alpha;
+            vn[j + 1] = refl1 ? 1.0 : 0.0;
+        }
+        __syncthreads();
+        if (refl1) {
+            for (int r = t; r < m - 1; r += kT) {
+                double acc = 0.0;
+                for (int q = 0; q < m - 1; ++q) acc += Bs[o + 1 + r][o + 1 + q] * ws[q];
+                ps[r] = 2.0 * acc;
+            }
+            for (int q = t; q < m - 1; q += kT) vs[q] = ws[q];
+        }
+        __syncthreads();
+        refl = refl1;
+    }
+    if (t == 0) d[n - 1] = Bs[n - 2 - j0][n - 2 - j0];
+}
+
+// the last diagonal entry once every step has run
+__global__ void k_tri_last(const double* __restrict__ A, int n, double* __restrict__ d) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) d[n - 1] = A[(int64_t)(n - 1) * n + (n - 1)];
 }
 
 // e2[i] = e[i]^2 and the scale of T (max Gershgorin radius bound), one workgroup
@@ -240,7 +364,6 @@ __global__ __launch_bounds__(kT) void k_invit(const double* __restrict__ d,
     double* dl = lu;          // multipliers
     double* u0 = lu + n;      // U diagonal
     double* u1 = lu + 2 * n;  // U first super-diagonal
-    double* u2 = lu + 3 * n;  // U second super-diagonal (non-zero after a swap)
     for (int j = 0; j < k; ++j) {
         double* yj = y + (int64_t)j * n;
         for (int i = t; i < n; i += kT) yj[i] = normal_at(seed, kStreamDenseInvit, (uint64_t)j * n + i);
@@ -255,12 +378,12 @@ __global__ __launch_bounds__(kT) void k_invit(const double* __restrict__ d,
                 if (fabs(cp) >= fabs(sub)) {
                     const double pv = fabs(cp) < pivmin ? (cp < 0.0 ? -pivmin : pivmin) : cp;
                     const double l = sub / pv;
-                    u0[i] = pv; u1[i] = cq; u2[i] = 0.0; dl[i] = l; piv[i] = 0;
+                    u0[i] = pv; u1[i] = cq; dl[i] = l; piv[i] = 0;
                     cp = dn - l * cq;
                     cq = en;
                 } else {
                     const double l = cp / sub;
-                    u0[i] = sub; u1[i] = dn; u2[i] = en; dl[i] = l; piv[i] = 1;
+                    u0[i] = sub; u1[i] = dn; dl[i] = l; piv[i] = 1;
                     cp = cq - l * dn;
                     cq = -l * en;
                 }
@@ -270,19 +393,30 @@ __global__ __launch_bounds__(kT) void k_invit(const double* __restrict__ d,
         __syncthreads();
         for (int sweep = 0; sweep < 3; ++sweep) {
             if (t == 0) {
+                // forward: row swaps and multipliers; the running row value stays
+                // in a register, so no load waits on the previous store
+                double c = yj[0];
                 for (int i = 0; i + 1 < n; ++i) {
-                    if (piv[i]) { const double tmp = yj[i]; yj[i] = yj[i + 1]; yj[i + 1] = tmp; }
-                    yj[i + 1] -= dl[i] * yj[i];
+                    const double o = yj[i + 1];
+                    const double l = dl[i];
+                    double ri, ri1;
+                    if (piv[i]) { ri = o; ri1 = c - l * o; }
+                    else { ri = c; ri1 = o - l * c; }
+                    yj[i] = ri;
+                    c = ri1;
                 }
+                yj[n - 1] = c;
+                // back substitution with the upper band (u0, u1, u2 = e[i+1] after a swap)
+                double z1 = 0.0, z2 = 0.0, mx = 0.0;
                 for (int i = n - 1; i >= 0; --i) {
-                    double s = yj[i];
-                    if (i + 1 < n) s -= u1[i] * yj[i + 1];
-                    if (i + 2 < n) s -= u2[i] * yj[i + 2];
-                    yj[i] = s / u0[i];
+                    const double u2 = (piv[i] && i + 2 < n) ? e[i + 1] : 0.0;
+                    const double zi = (yj[i] - u1[i] * z1 - u2 * z2) / u0[i];
+                    yj[i] = zi;
+                    mx = fmax(mx, fabs(zi));
+                    z2 = z1;
+                    z1 = zi;
                 }
                 // scale down to keep later solves finite
-                double mx = 0.0;
-                for (int i = 0; i < n; ++i) mx = fmax(mx, fabs(yj[i]));
                 const double sc = mx > 0.0 ? 1.0 / mx : 1.0;
                 for (int i = 0; i < n; ++i) yj[i] *= sc;
             }
@@ -340,37 +474,49 @@ void dense_evd_large(const DevLaplacian& L, int k, double* evals_host, double* e
              "dense_evd: the device dense solver supports n <= 4096");
     SI_CHECK(k >= 1 && k <= n, kErrTooFewNodes, "dense_evd: k > n");
     SI_CHECK(k <= kSmallMax, kErrUnsupported, "dense_evd: k above 64 is not supported");
-    double* a = ws_get<double>(kWsDenseA, (size_t)n * n);
-    // d, e, e2, vn, p [5n] | lu [4n] | y [n k] | w [k] | bounds [4]
-    double* aux = ws_get<double>(kWsDenseAux, (size_t)n * (9 + k) + k + 4);
+    // working matrix A and the reflectors R (row j = v_j on columns j+1..)
+    double* a = ws_get<double>(kWsDenseA, 2 * (size_t)n * n);
+    double* rf = a + (size_t)n * n;
+    // d, e, e2, vn, p (2 n, double-buffered) [6n] | lu [4n] | y [n k] | w [k] | bounds [4]
+    double* aux = ws_get<double>(kWsDenseAux, (size_t)n * (10 + k) + k + 4);
     int* piv = ws_get<int>(kWsDensePiv, (size_t)n);
     double* d = aux;
     double* e = d + n;
     double* e2 = e + n;
     double* vn = e2 + n;
-    double* p = vn + n;
-    double* lu = p + n;
+    double* p0 = vn + n;
+    double* p1 = p0 + n;
+    double* lu = p1 + n;
     double* y = lu + 4 * (size_t)n;
     double* w = y + (size_t)n * k;
     double* bounds = w + k;
     hipLaunchKernelGGL(k_densify, dim3(ceil_div(n, 256)), dim3(256), 0, s, L.rowptr, L.col, L.l64,
                        n, a);
     SI_LAUNCH_CHECK();
-    for (int j = 0; j + 2 < n; ++j) {
+    double* pin = p1;
+    double* pout = p0;
+    int j = -1;
+    for (; n - j - 1 > kFinish; ++j) {
         const int m = n - j - 1;
-        hipLaunchKernelGGL(k_house, dim3(1), dim3(kT), 0, s, a, n, j, d, e, vn);
-        hipLaunchKernelGGL(k_symv, dim3(ceil_div(m, kWaves)), dim3(kT), 0, s, a, n, j,
-                           (const double*)vn, p);
-        hipLaunchKernelGGL(k_rank2, dim3(ceil_div(m, kRank2Rows)), dim3(kT), 0, s, a, n, j,
-                           (const double*)vn, (const double*)p);
+        hipLaunchKernelGGL(k_tri_reflect, dim3(1), dim3(kT), 0, s, (const double*)a, rf, n, j, d,
+                           e, vn, (const double*)pin, bounds + 3);
+        hipLaunchKernelGGL(k_tri_update, dim3(ceil_div(m, kStepRows)), dim3(kT), 0, s, a,
+                           (const double*)rf, n, j, (const double*)vn, (const double*)pin,
+                           (const double*)(bounds + 3), pout);
+        std::swap(pin, pout);
     }
     SI_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_tail, dim3(1), dim3(64), 0, s, a, n, d, e);
+    if (j + 2 < n) {
+        hipLaunchKernelGGL(k_tri_finish, dim3(1), dim3(kT), 0, s, (const double*)a, rf, n, j, d, e,
+                           vn, (const double*)pin);
+    } else {
+        hipLaunchKernelGGL(k_tri_last, dim3(1), dim3(64), 0, s, (const double*)a, n, d);
+    }
     hipLaunchKernelGGL(k_tri_prep, dim3(1), dim3(kT), 0, s, d, e, n, e2, bounds);
     hipLaunchKernelGGL(k_bisect, dim3(k), dim3(kT), 0, s, d, e2, n, (const double*)bounds, w);
     hipLaunchKernelGGL(k_invit, dim3(1), dim3(kT), 0, s, d, e, n, k, (const double*)w,
                        (const double*)bounds, lu, piv, y, (uint64_t)0x5eedULL);
-    hipLaunchKernelGGL(k_backtransform, dim3(k), dim3(kT), 0, s, (const double*)a, n,
+    hipLaunchKernelGGL(k_backtransform, dim3(k), dim3(kT), 0, s, (const double*)rf, n,
                        (const double*)vn, y, evecs_dev, k);
     SI_LAUNCH_CHECK();
     to_host(evals_host, w, k, s);

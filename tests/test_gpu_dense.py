@@ -118,7 +118,7 @@ def test_dense_evd_path_graph(ctx):
     _check(ev, vec, a, 6)
 
 
-@pytest.mark.parametrize("n,k", [(65, 3), (1024, 16), (4096, 8)])
+@pytest.mark.parametrize("n,k", [(65, 3), (128, 3), (129, 5), (1024, 16), (4096, 8)])
 def test_dense_evd_random_graphs(ctx, n, k):
     ip, ix, w = random_symmetric_graph(n, 8, seed=n)
     lap = si.Laplacian(ctx, ip, ix, w)
