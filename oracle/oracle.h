@@ -22,6 +22,8 @@
  *   or_rsvd_solve                            -> ring-6 closed form, numpy eigh
  *   or_find_components / selection / signs  -> fixtures comp_c / comp_e / comp_f
  *   or_mpnn_*                                -> central finite differences
+ *   or_knn_exact .. or_fuzzy_union           -> fixtures step1_knn_exact ..
+ *                                               step5a_pruned_exact
  */
 #ifndef SI_ORACLE_H
 #define SI_ORACLE_H
@@ -138,6 +140,15 @@ int or_spectral_init(int64_t n, const int64_t* indptr, const int32_t* indices,
                      const float* raw_data, int64_t n_features, int mode,
                      int64_t dense_threshold, double lobpcg_tol, int add_noise,
                      float* out);
+
+/* ---- fuzzy k-NN graph (the input W; oracle_graph.c) ----
+ * Fixture pipeline tests/generate_fixtures.py:70-182 (umap-learn 0.5.11). */
+void or_knn_exact(int64_t n, int d, const double* x, int k, int32_t* idx, float* dist);
+void or_smooth_knn_dist(int64_t n, int k, const float* dist, float* sigmas, float* rhos);
+void or_membership(int64_t n, int k, const int32_t* idx, const float* dist, const float* sigmas,
+                   const float* rhos, float* vals);
+int64_t or_fuzzy_union(int64_t n, int k, const int32_t* idx, const float* vals, int n_epochs,
+                       int64_t* indptr, int32_t* indices, float* data);
 
 /* ---- learned warm start (no reference counterpart; oracle_mpnn.c) ---- */
 int64_t or_mpnn_param_count(void);

@@ -92,6 +92,12 @@ def lib():
                                        C.c_int, _f32p]
         if hasattr(L, "or_mpnn_forward"):
             _bind_mpnn(L)
+        L.or_knn_exact.argtypes = [C.c_int64, C.c_int, _f64p, C.c_int, _i32p, _f32p]
+        L.or_smooth_knn_dist.argtypes = [C.c_int64, C.c_int, _f32p, _f32p, _f32p]
+        L.or_membership.argtypes = [C.c_int64, C.c_int, _i32p, _f32p, _f32p, _f32p, _f32p]
+        L.or_fuzzy_union.restype = C.c_int64
+        L.or_fuzzy_union.argtypes = [C.c_int64, C.c_int, _i32p, _f32p, C.c_int, C.c_void_p,
+                                     C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -392,3 +398,61 @@ def mpnn_backward(adj, f, params, h, av, dx):
                            np.ascontiguousarray(params, np.float64), h, av,
                            np.ascontiguousarray(dx, np.float64), g)
     return g
+
+
+# ------------------------------------------------------------------ fuzzy k-NN graph
+def knn_exact(x, k: int):
+    """Exact k-NN, self excluded: (indices int32 [n, k], distances float32 [n, k])."""
+    x = np.ascontiguousarray(x, np.float64)
+    n, d = x.shape
+    idx = np.empty((n, k), np.int32)
+    dist = np.empty((n, k), np.float32)
+    lib().or_knn_exact(n, d, x, k, idx, dist)
+    return idx, dist
+
+
+def smooth_knn_dist(dist):
+    """umap smooth_knn_dist: (sigmas, rhos), float32 [n]."""
+    dist = np.ascontiguousarray(dist, np.float32)
+    n, k = dist.shape
+    sig = np.empty(n, np.float32)
+    rho = np.empty(n, np.float32)
+    lib().or_smooth_knn_dist(n, k, dist, sig, rho)
+    return sig, rho
+
+
+def membership(idx, dist, sigmas, rhos):
+    """umap compute_membership_strengths: float32 [n, k] values of A."""
+    idx = np.ascontiguousarray(idx, np.int32)
+    dist = np.ascontiguousarray(dist, np.float32)
+    n, k = idx.shape
+    vals = np.empty((n, k), np.float32)
+    lib().or_membership(n, k, idx, dist, np.ascontiguousarray(sigmas, np.float32),
+                        np.ascontiguousarray(rhos, np.float32), vals)
+    return vals
+
+
+def fuzzy_union(idx, vals, n_epochs: int = 0):
+    """W = A + A^T - A o A^T (zeros dropped), pruned below max/n_epochs when
+    n_epochs > 0: CSR (int64 indptr, int32 indices, float32 data)."""
+    idx = np.ascontiguousarray(idx, np.int32)
+    vals = np.ascontiguousarray(vals, np.float32)
+    n, k = idx.shape
+    nnz = lib().or_fuzzy_union(n, k, idx, vals, int(n_epochs), None, None, None)
+    ip = np.empty(n + 1, np.int64)
+    ix = np.empty(nnz, np.int32)
+    w = np.empty(nnz, np.float32)
+    lib().or_fuzzy_union(n, k, idx, vals, int(n_epochs), ip.ctypes.data, ix.ctypes.data,
+                         w.ctypes.data)
+    return ip, ix, w
+
+
+def fuzzy_graph(x, k: int = 15, n_epochs: int = -1):
+    """Steps 1-5a from raw points; n_epochs -1 = the fixtures' rule (500 for
+    n <= 10000, else 200), 0 = no pruning."""
+    n = np.asarray(x).shape[0]
+    if n_epochs < 0:
+        n_epochs = 500 if n <= 10000 else 200
+    idx, dist = knn_exact(x, k)
+    sig, rho = smooth_knn_dist(dist)
+    return fuzzy_union(idx, membership(idx, dist, sig, rho), n_epochs)

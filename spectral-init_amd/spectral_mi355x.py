@@ -46,6 +46,8 @@ EXPORTED = [
     "si_mpnn_param_count", "si_mpnn_create", "si_mpnn_destroy", "si_mpnn_get_params",
     "si_mpnn_set_params", "si_mpnn_features", "si_mpnn_forward", "si_mpnn_gradient",
     "si_mpnn_train_step", "si_mpnn_warm_start",
+    "si_knn_exact", "si_smooth_knn_dist", "si_membership_strengths", "si_fuzzy_union",
+    "si_fuzzy_graph_build", "si_graph_info", "si_graph_copy", "si_graph_destroy",
 ]
 
 
@@ -206,7 +208,7 @@ def lib():
         for name in EXPORTED:
             if name not in ("si_version", "si_last_error", "si_config_default",
                             "si_context_destroy", "si_laplacian_destroy", "si_mpnn_destroy",
-                            "si_mpnn_param_count"):
+                            "si_mpnn_param_count", "si_graph_destroy"):
                 getattr(L, name).restype = C.c_int
         L.si_mpnn_param_count.restype = C.c_int64
         L.si_mpnn_destroy.restype = None
@@ -220,6 +222,15 @@ def lib():
         L.si_mpnn_train_step.argtypes = [C.c_void_p] * 5 + [C.c_float, C.c_void_p]
         L.si_mpnn_warm_start.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int32,
                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.si_knn_exact.argtypes = [vp, vp, i32, i64, i64, i64, vp, vp]
+        L.si_smooth_knn_dist.argtypes = [vp, vp, i64, i64, vp, vp]
+        L.si_membership_strengths.argtypes = [vp, vp, vp, vp, vp, i64, i64, vp]
+        L.si_fuzzy_union.argtypes = [vp, vp, vp, i64, i64, i32, C.POINTER(vp)]
+        L.si_fuzzy_graph_build.argtypes = [vp, vp, i32, i64, i64, i64, i32, C.POINTER(vp)]
+        L.si_graph_info.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
+        L.si_graph_copy.argtypes = [vp, vp, vp, vp, vp]
+        L.si_graph_destroy.restype = None
+        L.si_graph_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -624,3 +635,100 @@ class WarmStartNet:
                                                  n_components, blk.data_ptr(), C.byref(used),
                                                  C.byref(rn), C.byref(rr)))
         return blk, bool(used.value), rn.value, rr.value
+
+
+# ------------------------------------------------------------------ fuzzy k-NN graph
+def _points(x):
+    x = np.ascontiguousarray(x)
+    if x.dtype == np.float64:
+        return x, SI_F64
+    return np.ascontiguousarray(x, np.float32), SI_F32
+
+
+def knn_exact(ctx: Context, x, k: int):
+    """Exact k-NN on the device, self excluded (tests/generate_fixtures.py:70-93):
+    (indices int32 [n, k], distances float32 [n, k])."""
+    x, prec = _points(x)
+    n, d = x.shape
+    idx = np.empty((n, k), np.int32)
+    dist = np.empty((n, k), np.float32)
+    ctx._check(lib().si_knn_exact(ctx.handle, x.ctypes.data, prec, n, d, k, idx.ctypes.data,
+                                  dist.ctypes.data))
+    return idx, dist
+
+
+def smooth_knn_dist(ctx: Context, dist):
+    """umap smooth_knn_dist on the device: (sigmas, rhos) float32 [n]."""
+    dist = np.ascontiguousarray(dist, np.float32)
+    n, k = dist.shape
+    sig = np.empty(n, np.float32)
+    rho = np.empty(n, np.float32)
+    ctx._check(lib().si_smooth_knn_dist(ctx.handle, dist.ctypes.data, n, k, sig.ctypes.data,
+                                        rho.ctypes.data))
+    return sig, rho
+
+
+def membership_strengths(ctx: Context, idx, dist, sigmas, rhos):
+    """umap compute_membership_strengths on the device: float32 [n, k]."""
+    idx = np.ascontiguousarray(idx, np.int32)
+    dist = np.ascontiguousarray(dist, np.float32)
+    sigmas = np.ascontiguousarray(sigmas, np.float32)
+    rhos = np.ascontiguousarray(rhos, np.float32)
+    n, k = idx.shape
+    vals = np.empty((n, k), np.float32)
+    ctx._check(lib().si_membership_strengths(ctx.handle, idx.ctypes.data, dist.ctypes.data,
+                                             sigmas.ctypes.data, rhos.ctypes.data, n, k,
+                                             vals.ctypes.data))
+    return vals
+
+
+class Graph:
+    """si_graph: a CSR graph on the device (int64 indptr, int32 indices, float32)."""
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx = ctx
+        self._h = handle
+        n, nnz = C.c_int64(0), C.c_int64(0)
+        ctx._check(lib().si_graph_info(self._h, C.byref(n), C.byref(nnz)))
+        self.n, self.nnz = n.value, nnz.value
+
+    def to_host(self):
+        ip = np.empty(self.n + 1, np.int64)
+        ix = np.empty(max(self.nnz, 1), np.int32)
+        w = np.empty(max(self.nnz, 1), np.float32)
+        self.ctx._check(lib().si_graph_copy(self.ctx.handle, self._h, ip.ctypes.data,
+                                            ix.ctypes.data, w.ctypes.data))
+        return ip, ix[:self.nnz], w[:self.nnz]
+
+    def close(self):
+        if self._h:
+            lib().si_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def fuzzy_union(ctx: Context, idx, vals, n_epochs: int = -1) -> Graph:
+    """W = A + A^T - A o A^T on the device, pruned below max/n_epochs
+    (n_epochs -1: 500 for n <= 10000 else 200; 0: no pruning)."""
+    idx = np.ascontiguousarray(idx, np.int32)
+    vals = np.ascontiguousarray(vals, np.float32)
+    n, k = idx.shape
+    h = C.c_void_p()
+    ctx._check(lib().si_fuzzy_union(ctx.handle, idx.ctypes.data, vals.ctypes.data, n, k,
+                                    int(n_epochs), C.byref(h)))
+    return Graph(ctx, h)
+
+
+def fuzzy_graph(ctx: Context, x, k: int = 15, n_epochs: int = -1) -> Graph:
+    """Steps 1-5a of the fuzzy k-NN graph on the device, intermediates resident."""
+    x, prec = _points(x)
+    n, d = x.shape
+    h = C.c_void_p()
+    ctx._check(lib().si_fuzzy_graph_build(ctx.handle, x.ctypes.data, prec, n, d, k,
+                                          int(n_epochs), C.byref(h)))
+    return Graph(ctx, h)

@@ -54,7 +54,7 @@ def solve(si, lap, cfg):
                      spmm_calls=rep.spmm_calls, residual=rep.max_residual, level=rep.level)
 
 
-def evaluate(si, graphs, net, seeds, n, tol):
+def evaluate(si, graphs, net, seeds, n, tol, on_row=None):
     import torch
     rows = []
     for s in seeds:
@@ -90,7 +90,33 @@ def evaluate(si, graphs, net, seeds, n, tol):
             f"(block residual {row['plain_warm']['res_net_block']:.3e} vs "
             f"{row['plain_warm']['res_random_block']:.3e})")
         del lap
+        if on_row:
+            on_row(row)
     return rows
+
+
+def summarise(rows):
+    """Per evaluation size: sums over graphs (divide by "graphs" for means)."""
+    summary = {}
+    for r in rows:
+        d = summary.setdefault(r["size"], dict(graphs=0, plain_it_random=0, plain_it_warm=0,
+                                               plain_ms_random=0.0, plain_ms_warm=0.0,
+                                               net_used=0, plain_fewer_iterations=0,
+                                               plain_max_residual_warm=0.0,
+                                               fast_ms_random=0.0, fast_ms_warm=0.0))
+        d["graphs"] += 1
+        d["plain_it_random"] += r["plain_random"]["iterations"]
+        d["plain_it_warm"] += r["plain_warm"]["iterations"]
+        d["plain_ms_random"] += r["plain_random"]["ms"]
+        d["plain_ms_warm"] += r["plain_warm"]["total_ms"]
+        d["net_used"] += int(r["plain_warm"]["used_net"])
+        d["plain_fewer_iterations"] += int(r["plain_warm"]["iterations"] <
+                                           r["plain_random"]["iterations"])
+        d["plain_max_residual_warm"] = max(d["plain_max_residual_warm"],
+                                           r["plain_warm"]["residual"])
+        d["fast_ms_random"] += r["fast_random"]["ms"]
+        d["fast_ms_warm"] += r["fast_warm"]["total_ms"]
+    return summary
 
 
 def main():
@@ -156,34 +182,33 @@ def main():
     del data
 
     rows = []
+
+    def write_report():
+        summary = summarise(rows)
+        rep = dict(train_graphs=int(args.train), n_range=[args.n_lo, args.n_hi],
+                   epochs=args.epochs, lr=args.lr, probe_seeds=args.probe_seeds, steps=steps,
+                   train_seconds=train_s, loss_history=hist, eval=rows,
+                   summary={str(k): v for k, v in summary.items()},
+                   command="python -u spectral-init_amd/train_warmstart.py " + " ".join(sys.argv[1:]))
+        if args.report:  # rewritten after every graph, so a cut-off run keeps its rows
+            with open(args.report, "w") as fh:
+                json.dump(rep, fh, indent=1)
+        return summary
+
     seed = args.eval_seed0
     for spec in args.eval.split(","):
         size, count = (int(v) for v in spec.split(":"))
-        rows += evaluate(si, graphs, net, list(range(seed, seed + count)), size, args.tol)
+        evaluate(si, graphs, net, list(range(seed, seed + count)), size, args.tol,
+                 on_row=lambda r: (rows.append(r), write_report()))
         seed += count
-    summary = {}
-    for r in rows:
-        d = summary.setdefault(r["size"], dict(graphs=0, plain_it_random=0,
-                                                          plain_it_warm=0, plain_ms_random=0.0,
-                                                          plain_ms_warm=0.0, net_used=0))
-        d["graphs"] += 1
-        d["plain_it_random"] += r["plain_random"]["iterations"]
-        d["plain_it_warm"] += r["plain_warm"]["iterations"]
-        d["plain_ms_random"] += r["plain_random"]["ms"]
-        d["plain_ms_warm"] += r["plain_warm"]["total_ms"]
-        d["net_used"] += int(r["plain_warm"]["used_net"])
+    summary = write_report()
     for size, d in sorted(summary.items()):
         g = d["graphs"]
         log(f"n~{size}: {g} graphs, plain LOBPCG mean iterations {d['plain_it_random'] / g:.1f} -> "
             f"{d['plain_it_warm'] / g:.1f}, mean ms {d['plain_ms_random'] / g:.1f} -> "
-            f"{d['plain_ms_warm'] / g:.1f} (incl. warm start), network block used {d['net_used']}/{g}")
-    rep = dict(train_graphs=int(args.train), n_range=[args.n_lo, args.n_hi], epochs=args.epochs,
-               lr=args.lr, probe_seeds=args.probe_seeds, steps=steps, train_seconds=train_s,
-               loss_history=hist, eval=rows, summary={str(k): v for k, v in summary.items()},
-               command="python -u spectral-init_amd/train_warmstart.py " + " ".join(sys.argv[1:]))
-    if args.report:
-        with open(args.report, "w") as fh:
-            json.dump(rep, fh, indent=1)
+            f"{d['plain_ms_warm'] / g:.1f} (incl. warm start), network block used {d['net_used']}/{g}; "
+            f"fewer iterations on {d['plain_fewer_iterations']}/{g}; FP32-bulk solver mean ms "
+            f"{d['fast_ms_random'] / g:.2f} -> {d['fast_ms_warm'] / g:.2f} (incl. warm start)")
     print(json.dumps({"loss_first": hist[0] if hist else None, "loss_last": hist[-1] if hist else None,
                       "eval": [{k: (v if not isinstance(v, dict) else
                                     {kk: v[kk] for kk in ("iterations", "ms", "residual")})

@@ -15,6 +15,9 @@ Sources (relative to the reference checkout):
   tests/fixtures/<dataset>/comp_d_eigensolver.npz scipy eigsh eigenpairs (k=3)
   tests/fixtures/<dataset>/comp_e_selection.npz  selected columns
   tests/fixtures/<dataset>/comp_f_scaling.npz    pre_noise scaled coordinates
+  tests/fixtures/<dataset>/step0_raw_data.npz .. step5a_pruned_exact.npz
+                                                 the fuzzy k-NN graph steps (exact k-NN path)
+                                                 -> tests/golden/graph/<dataset>.npz
   research/2026-03-27-computemode-simd-spmv-parity/results/eigenvectors/*.npy
   research/2026-03-25-ci-test-optimization/results/rq3_threshold/*_n200_eigvecs.npy
 
@@ -77,7 +80,44 @@ def extract_dataset(name: str) -> dict:
     return out
 
 
+def _csr(z):
+    """CSR arrays of a scipy save_npz file (format 'csr'), read as plain data."""
+    assert z["format"].item() in (b"csr", "csr")
+    return (z["indptr"].astype(np.int64), z["indices"].astype(np.int32),
+            z["data"].astype(np.float32))
+
+
+def extract_graph_steps(name: str) -> dict:
+    """Steps 0-5a of the exact k-NN path (generate_fixtures.py:451-471)."""
+    d = os.path.join(REF, "tests", "fixtures", name)
+    out = {}
+    x = _load(os.path.join(d, "step0_raw_data.npz"))
+    out["x"] = x["X"].astype(np.float64)
+    s1 = _load(os.path.join(d, "step1_knn_exact.npz"))
+    out["knn_indices"] = s1["knn_indices"].astype(np.int32)
+    out["knn_dists"] = s1["knn_dists"].astype(np.float32)
+    out["n_neighbors"] = np.asarray(s1["n_neighbors"], np.int32)
+    s2 = _load(os.path.join(d, "step2_smooth_knn_exact.npz"))
+    out["sigmas"] = s2["sigmas"].astype(np.float32)
+    out["rhos"] = s2["rhos"].astype(np.float32)
+    s3 = _load(os.path.join(d, "step3_membership_exact.npz"))
+    assert s3["format"].item() in (b"coo", "coo")
+    out["s3_row"] = s3["row"].astype(np.int32)
+    out["s3_col"] = s3["col"].astype(np.int32)
+    out["s3_data"] = s3["data"].astype(np.float32)
+    for tag, fname in (("s4", "step4_symmetrized_exact.npz"), ("s5", "step5a_pruned_exact.npz")):
+        ip, ix, dv = _csr(_load(os.path.join(d, fname)))
+        out[f"{tag}_indptr"], out[f"{tag}_indices"], out[f"{tag}_data"] = ip, ix, dv
+    return out
+
+
 def main() -> int:
+    os.makedirs(os.path.join(HERE, "graph"), exist_ok=True)
+    for name in DATASETS:
+        arrays = extract_graph_steps(name)
+        np.savez_compressed(os.path.join(HERE, "graph", f"{name}.npz"), **arrays)
+        print(f"{name}: graph steps, n={arrays['x'].shape[0]} d={arrays['x'].shape[1]} "
+              f"nnz(step4)={arrays['s4_data'].size} nnz(step5a)={arrays['s5_data'].size}")
     os.makedirs(OUT_FIX, exist_ok=True)
     os.makedirs(OUT_RUST, exist_ok=True)
     for name in DATASETS:

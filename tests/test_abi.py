@@ -11,7 +11,8 @@ import spectral_mi355x as si
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADERS = [os.path.join(ROOT, "include", "spectral_mi355x.h"),
            os.path.join(ROOT, "include", "spectral_mi355x_testing.h"),
-           os.path.join(ROOT, "include", "spectral_mi355x_warmstart.h")]
+           os.path.join(ROOT, "include", "spectral_mi355x_warmstart.h"),
+           os.path.join(ROOT, "include", "spectral_mi355x_graph.h")]
 
 
 def _declared():
