@@ -76,6 +76,52 @@ def cpu_baseline(lap, si, n_components, plain_iters, budget_s, threads=1):
                 projected_ms_to_eigvecs=round(ms_it * plain_iters, 1) if plain_iters else None)
 
 
+def bench_graph_builder(args, torch, graphs, si, ctx):
+    """The fuzzy k-NN graph W of the workload's own points built on the device
+    (si_fuzzy_graph_build: exact FP64 k-NN, smooth_knn_dist, membership, fuzzy
+    union, prune), its k-NN kernel's FP64 rate, and the CPU oracle's k-NN on a
+    bounded sample of the same points."""
+    import oracle as O
+    x, _ = graphs.workload_points(args.workload, args.seed, args.n)
+    n, d = x.shape
+    k = 15
+    si.fuzzy_graph(ctx, x[:4096], k).close()  # warm the kernels
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    si.knn_exact(ctx, x, k)
+    knn_ms = 1e3 * (time.perf_counter() - t)
+    t = time.perf_counter()
+    g = si.fuzzy_graph(ctx, x, k)
+    graph_ms = 1e3 * (time.perf_counter() - t)
+    nnz = g.nnz
+    g.close()
+    # algorithmic work of the exact k-NN: n (n - 1) pairs, 3 d FP64 flops each
+    # (difference, square, add); points and lists in and out of HBM are < 0.1 %
+    flops = 3.0 * d * n * (n - 1)
+    tflops = flops / (knn_ms * 1e-3) / 1e12
+    log(f"graph builder: k-NN {knn_ms:.1f} ms ({tflops:.1f} TFLOP/s FP64), whole graph "
+        f"{graph_ms:.1f} ms, nnz {nnz}")
+    cpu = None
+    if not args.no_cpu:
+        O.build()
+        O.set_threads(1)
+        m = min(n, 4000)
+        t = time.perf_counter()
+        O.knn_exact(x[:m].astype(np.float64), k)
+        dt = time.perf_counter() - t
+        per_pair_ns = 1e9 * dt / (m * (m - 1))
+        cpu = dict(unit="ns per k-NN pair (FP64, 1 thread)", value=round(per_pair_ns, 3), cores=1,
+                   kind="port",
+                   sample=f"oracle exact k-NN of the first {m} points ({dt:.2f} s); projected "
+                          f"for all {n} points: {per_pair_ns * n * (n - 1) * 1e-9:.0f} s")
+        log(f"graph builder CPU oracle: {cpu['value']} ns/pair -> {cpu['sample']}")
+    return dict(points=n, dim=d, k=k, nnz=nnz, ms=round(graph_ms, 1), knn_ms=round(knn_ms, 1),
+                knn_roofline=dict(bound="fp64 valu", achieved=round(tflops, 2), peak=78.6,
+                                  unit="TFLOP/s", frac=round(tflops / 78.6, 4),
+                                  flops_per_pair=3 * d),
+                cpu_baseline=cpu)
+
+
 def bench_components(args, torch, graphs, si):
     """64 disconnected components (sizes log-uniform in [50, 200 000], each an
     independent 30-d GMM 15-NN graph): spectral_init with all components in one
@@ -190,6 +236,8 @@ def main():
                     help="load the workload graph from this .npz (written on first use)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one solve plus a 1 GiB device copy (scale reference for HBM counters), then exit")
+    ap.add_argument("--no-graph-builder", action="store_true",
+                    help="skip timing the device fuzzy k-NN graph builder on the workload's points")
     ap.add_argument("--spmm-shape16", type=int, default=-1,
                     help="pin the FP32 16-column SpMM shape (0-3) instead of autotuning; "
                          "counter runs pin the shape the autotuner picks unprofiled")
@@ -364,6 +412,10 @@ def main():
 
     traffic = read_traffic(args.traffic, args.workload, last.block_size)
 
+    gbuild = None
+    if not args.no_graph_builder and args.workload in ("gmm1m", "gmm100k", "blobs2k"):
+        gbuild = bench_graph_builder(args, torch, graphs, si, ctx)
+
     out = {
         "metric": "ms to eigvecs @ residual<=%g (FP32-bulk Chebyshev block solver)" % args.tol,
         "value": round(fast_ms, 3),
@@ -397,6 +449,7 @@ def main():
                      "launches": sp["launches_f32"]},
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all if not args.no_cpu else None,
+        "graph_builder": gbuild,
     }
     print(json.dumps(out), flush=True)
 

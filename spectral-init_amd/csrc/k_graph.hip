@@ -4,10 +4,11 @@
 //
 //   k_knn            exact k-NN, self excluded (generate_fixtures.py:70-93).
 //                    One query per lane, Q queries per thread; candidate points
-//                    staged through LDS in tiles of 64; FP64 sums of squared
-//                    differences in dimension order (fused), top-k kept sorted
-//                    in registers, ties to the smaller index.  VALU-bound:
-//                    2 FP64 ops per dimension and pair.
+//                    staged through LDS in tiles of 64 (in the input precision);
+//                    FP64 sums of squared differences in dimension order
+//                    (fused), top-k kept sorted in registers, ties to the
+//                    smaller index.  Bound by the LDS broadcast reads and the
+//                    FP64 VALU work (2-3 instructions per dimension and pair).
 //   k_smooth_knn     umap smooth_knn_dist, one row per thread (float32 bisection).
 //   k_membership     umap compute_membership_strengths, one entry per thread.
 //   fuzzy union      W = A + A^T - A o A^T: every directed entry is written
@@ -61,12 +62,12 @@ __device__ inline float expf_tab(float x) {
 
 template <typename T>
 __global__ void k_pad_points(const T* __restrict__ src, int64_t n, int d, int dp,
-                             double* __restrict__ dst) {
+                             T* __restrict__ dst) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n * dp) return;
     const int64_t i = e / dp;
     const int t = (int)(e % dp);
-    dst[e] = t < d ? (double)src[i * d + t] : 0.0;
+    dst[e] = t < d ? src[i * d + t] : T(0);
 }
 
 // Top-KP insertion of (s, j) into the ascending list (bd, bi); strict
@@ -86,12 +87,21 @@ __device__ inline void topk_insert(double (&bd)[KP], int32_t (&bi)[KP], double s
 constexpr int kKnnTile = 64;   // candidates per LDS tile
 constexpr int kKnnThreads = 256;
 
-template <int DP, int KP, int Q>
-__global__ __launch_bounds__(kKnnThreads) void k_knn(const double* __restrict__ x, int64_t n,
+template <typename CT> struct Lds16;
+template <> struct Lds16<float> { using type = float4; };
+template <> struct Lds16<double> { using type = double2; };
+
+// CT is the storage type of the points (float32 inputs stay float32 in HBM
+// and LDS, which halves the LDS bytes of the broadcast reads that bound this
+// kernel; every coordinate is widened to FP64 exactly before it is used).
+template <int DP, int KP, int Q, typename CT>
+__global__ __launch_bounds__(kKnnThreads) void k_knn(const CT* __restrict__ x, int64_t n,
                                                      int k, int32_t* __restrict__ oi,
                                                      float* __restrict__ od) {
-    static_assert(DP % 2 == 0, "padded dimension must be even");
-    __shared__ double tile[kKnnTile * DP];
+    using V = typename Lds16<CT>::type;
+    constexpr int VN = 16 / (int)sizeof(CT);  // coordinates per 16-byte LDS read
+    static_assert(DP % VN == 0, "padded dimension must fill 16-byte reads");
+    __shared__ CT tile[kKnnTile * DP];
     int64_t qi[Q];
     double q[Q][DP];
     double bd[Q][KP];
@@ -101,7 +111,7 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn(const double* __restrict__ 
         qi[u] = ((int64_t)blockIdx.x * Q + u) * kKnnThreads + threadIdx.x;
         const bool ok = qi[u] < n;
 #pragma unroll
-        for (int t = 0; t < DP; ++t) q[u][t] = ok ? x[qi[u] * DP + t] : 0.0;
+        for (int t = 0; t < DP; ++t) q[u][t] = ok ? (double)x[qi[u] * DP + t] : 0.0;
 #pragma unroll
         for (int p = 0; p < KP; ++p) { bd[u][p] = INFINITY; bi[u][p] = -1; }
     }
@@ -109,21 +119,24 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn(const double* __restrict__ 
         const int cnt = (int)((n - c0) < kKnnTile ? (n - c0) : kKnnTile);
         __syncthreads();
         for (int e = threadIdx.x; e < kKnnTile * DP; e += kKnnThreads)
-            tile[e] = e < cnt * DP ? x[c0 * DP + e] : 0.0;
+            tile[e] = e < cnt * DP ? x[c0 * DP + e] : CT(0);
         __syncthreads();
         for (int c = 0; c < cnt; ++c) {
             double s[Q];
 #pragma unroll
             for (int u = 0; u < Q; ++u) s[u] = 0.0;
 #pragma unroll
-            for (int t = 0; t < DP; t += 2) {
-                const double2 v = *reinterpret_cast<const double2*>(&tile[c * DP + t]);
+            for (int t = 0; t < DP; t += VN) {
+                const V v = *reinterpret_cast<const V*>(&tile[c * DP + t]);
+                const CT* vc = reinterpret_cast<const CT*>(&v);
 #pragma unroll
-                for (int u = 0; u < Q; ++u) {
-                    const double d0 = q[u][t] - v.x;
-                    s[u] = fma(d0, d0, s[u]);
-                    const double d1 = q[u][t + 1] - v.y;
-                    s[u] = fma(d1, d1, s[u]);
+                for (int e = 0; e < VN; ++e) {
+                    const double cd = (double)vc[e];
+#pragma unroll
+                    for (int u = 0; u < Q; ++u) {
+                        const double df = q[u][t + e] - cd;
+                        s[u] = fma(df, df, s[u]);
+                    }
                 }
             }
             const int64_t j = c0 + c;
@@ -281,18 +294,23 @@ __global__ void k_indptr(const int32_t* __restrict__ start, int64_t n, int64_t n
     if (i == n) ip[n] = nnz;
 }
 
-__global__ void k_sum_f32(const float* __restrict__ v, int64_t m, double* __restrict__ out) {
-    // one workgroup, fixed order: per-thread strided sums, then a tree
+// Sum of m floats in double, fixed order: block b sums the strided elements
+// b*256 + t + j*gridDim*256 per thread, then a tree; a second launch with one
+// block sums the block partials the same way.
+template <typename T>
+__global__ void k_sum_to_f64(const T* __restrict__ v, int64_t m, double* __restrict__ out) {
     __shared__ double red[256];
     double s = 0.0;
-    for (int64_t t = threadIdx.x; t < m; t += 256) s += (double)v[t];
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < m; t += stride)
+        s += (double)v[t];
     red[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = red[0];
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
 void upload_exp_table() {
@@ -311,29 +329,55 @@ void upload_exp_table() {
     if (dev < 64) done[dev] = true;
 }
 
-template <int DP, int KP>
-void launch_knn_q(const double* x, int64_t n, int k, int32_t* oi, float* od, hipStream_t s) {
+template <int DP, int KP, typename CT>
+void launch_knn_q(const CT* x, int64_t n, int k, int32_t* oi, float* od, hipStream_t s) {
     // two queries per thread halve the LDS reads per distance where the
     // registers allow it (DP <= 16, k <= 16; above, occupancy drops to one wave)
     constexpr int Q = (DP <= 16 && KP <= 16) ? 2 : 1;
     const int64_t blocks = ceil_div(n, (int64_t)kKnnThreads * Q);
-    hipLaunchKernelGGL((k_knn<DP, KP, Q>), dim3(blocks), dim3(kKnnThreads), 0, s, x, n, k, oi, od);
+    hipLaunchKernelGGL((k_knn<DP, KP, Q, CT>), dim3(blocks), dim3(kKnnThreads), 0, s, x, n, k,
+                       oi, od);
     SI_LAUNCH_CHECK();
 }
 
-template <int DP>
-void launch_knn_k(const double* x, int64_t n, int k, int32_t* oi, float* od, hipStream_t s) {
+template <int DP, typename CT>
+void launch_knn_k(const CT* x, int64_t n, int k, int32_t* oi, float* od, hipStream_t s) {
     if (k <= 8) launch_knn_q<DP, 8>(x, n, k, oi, od, s);
     else if (k <= 16) launch_knn_q<DP, 16>(x, n, k, oi, od, s);
     else if (k <= 32) launch_knn_q<DP, 32>(x, n, k, oi, od, s);
     else launch_knn_q<DP, 64>(x, n, k, oi, od, s);
 }
 
-int padded_dim(int64_t d) {
+int padded_dim(int64_t d, int elem_bytes) {
     const int dps[] = {2, 4, 8, 16, 32, 48, 64};
+    const int vn = 16 / elem_bytes;
     for (int p : dps)
-        if (d <= p) return p;
+        if (d <= p && p % vn == 0) return p;
     return -1;
+}
+
+// Pads the points to dp coordinates with zeros (which leave every distance
+// unchanged) and runs the k-NN kernel of that width.
+template <typename CT>
+void knn_typed(const CT* pts, int64_t n, int64_t d, int64_t k, int32_t* idx, float* dist,
+               hipStream_t s) {
+    const int dp = padded_dim(d, (int)sizeof(CT));
+    SI_CHECK(dp > 0, kErrUnsupported, "knn: dimension > 64 is not supported");
+    DevBuf<CT> xp((size_t)n * dp);
+    const int64_t tot = n * dp;
+    hipLaunchKernelGGL((k_pad_points<CT>), dim3(ceil_div(tot, 256)), dim3(256), 0, s, pts, n,
+                       (int)d, dp, xp.get());
+    SI_LAUNCH_CHECK();
+    switch (dp) {
+        case 2: if constexpr (sizeof(CT) == 8) launch_knn_k<2>(xp.get(), n, (int)k, idx, dist, s); break;
+        case 4: launch_knn_k<4>(xp.get(), n, (int)k, idx, dist, s); break;
+        case 8: launch_knn_k<8>(xp.get(), n, (int)k, idx, dist, s); break;
+        case 16: launch_knn_k<16>(xp.get(), n, (int)k, idx, dist, s); break;
+        case 32: launch_knn_k<32>(xp.get(), n, (int)k, idx, dist, s); break;
+        case 48: launch_knn_k<48>(xp.get(), n, (int)k, idx, dist, s); break;
+        default: launch_knn_k<64>(xp.get(), n, (int)k, idx, dist, s); break;
+    }
+    SI_HIP(hipStreamSynchronize(s));  // xp is freed on return
 }
 
 }  // namespace
@@ -344,27 +388,11 @@ void knn_exact_device(const void* points_dev, int precision, int64_t n, int64_t 
     SI_CHECK(k < n, kErrInvalidArgument, "knn: need k < n (self is excluded)");
     SI_CHECK(k <= 64, kErrUnsupported, "knn: k > 64 is not supported");
     SI_CHECK(n < ((int64_t)1 << 31), kErrUnsupported, "knn: n >= 2^31 is not supported");
-    const int dp = padded_dim(d);
-    SI_CHECK(dp > 0, kErrUnsupported, "knn: dimension > 64 is not supported");
-    DevBuf<double> xp((size_t)n * dp);
-    const int64_t tot = n * dp;
+    SI_CHECK(d <= 64, kErrUnsupported, "knn: dimension > 64 is not supported");
     if (precision == 0)
-        hipLaunchKernelGGL((k_pad_points<float>), dim3(ceil_div(tot, 256)), dim3(256), 0, s,
-                           static_cast<const float*>(points_dev), n, (int)d, dp, xp.get());
+        knn_typed<float>(static_cast<const float*>(points_dev), n, d, k, idx, dist, s);
     else
-        hipLaunchKernelGGL((k_pad_points<double>), dim3(ceil_div(tot, 256)), dim3(256), 0, s,
-                           static_cast<const double*>(points_dev), n, (int)d, dp, xp.get());
-    SI_LAUNCH_CHECK();
-    switch (dp) {
-        case 2: launch_knn_k<2>(xp, n, (int)k, idx, dist, s); break;
-        case 4: launch_knn_k<4>(xp, n, (int)k, idx, dist, s); break;
-        case 8: launch_knn_k<8>(xp, n, (int)k, idx, dist, s); break;
-        case 16: launch_knn_k<16>(xp, n, (int)k, idx, dist, s); break;
-        case 32: launch_knn_k<32>(xp, n, (int)k, idx, dist, s); break;
-        case 48: launch_knn_k<48>(xp, n, (int)k, idx, dist, s); break;
-        default: launch_knn_k<64>(xp, n, (int)k, idx, dist, s); break;
-    }
-    SI_HIP(hipStreamSynchronize(s));  // xp is freed on return
+        knn_typed<double>(static_cast<const double*>(points_dev), n, d, k, idx, dist, s);
 }
 
 void smooth_knn_device(const float* dist, int64_t n, int64_t k, float* sig, float* rho,
@@ -372,8 +400,12 @@ void smooth_knn_device(const float* dist, int64_t n, int64_t k, float* sig, floa
     SI_CHECK(n >= 1 && k >= 2, kErrInvalidArgument, "smooth_knn_dist: need n >= 1, k >= 2");
     upload_exp_table();
     // mean of all distances, for rows without a non-zero distance
-    DevBuf<double> sum(1);
-    hipLaunchKernelGGL(k_sum_f32, dim3(1), dim3(256), 0, s, dist, n * k, sum.get());
+    constexpr int kParts = 512;
+    DevBuf<double> part(kParts), sum(1);
+    hipLaunchKernelGGL((k_sum_to_f64<float>), dim3(kParts), dim3(256), 0, s, dist, n * k,
+                       part.get());
+    hipLaunchKernelGGL((k_sum_to_f64<double>), dim3(1), dim3(256), 0, s, part.get(),
+                       (int64_t)kParts, sum.get());
     SI_LAUNCH_CHECK();
     double h = 0.0;
     SI_HIP(hipMemcpyAsync(&h, sum.get(), sizeof(double), hipMemcpyDeviceToHost, s));

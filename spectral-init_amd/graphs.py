@@ -228,6 +228,22 @@ def components_graph(n_comp, lo, hi, dim=30, seed=0, device=None, clusters=3,
 
 
 # ----------------------------------------------------------------- workloads
+def workload_points(name, seed=0, n=None):
+    """The points (float32 [n, d]) and cluster labels of the point-cloud
+    workloads blobs2k, gmm100k and gmm1m, as workload() draws them."""
+    rng = np.random.default_rng(10_000 + seed)
+    if name == "blobs2k":
+        return gmm_points(n or 2000, 50, np.ones(10), seed, center_scale=CENTER_SCALE["blobs2k"])
+    if name == "gmm100k":
+        sizes = rng.dirichlet(np.full(20, 2.0))
+        return gmm_points(n or 100_000, 50, sizes, seed, center_scale=CENTER_SCALE["gmm100k"])
+    if name == "gmm1m":
+        sizes = 1.0 / np.arange(1, 41) ** 1.2
+        return gmm_points(n or 1_000_000, 30, sizes, seed, center_scale=CENTER_SCALE["gmm1m"],
+                          background_frac=0.05)
+    raise ValueError(name)
+
+
 def workload(name, seed=0, device=None, n=None):
     """Seeded inputs of the named workloads.
 
@@ -237,20 +253,9 @@ def workload(name, seed=0, device=None, n=None):
     comp64    : 64 independent 30-d GMM 15-NN components, sizes log-uniform in [50, 200 000]
     n overrides the node count (same construction, for reduced-size runs).
     """
-    rng = np.random.default_rng(10_000 + seed)
-    if name == "blobs2k":
-        n = n or 2000
-        sizes = np.ones(10)
-        x, lab = gmm_points(n, 50, sizes, seed, center_scale=CENTER_SCALE["blobs2k"])
-    elif name == "gmm100k":
-        n = n or 100_000
-        sizes = rng.dirichlet(np.full(20, 2.0))
-        x, lab = gmm_points(n, 50, sizes, seed, center_scale=CENTER_SCALE["gmm100k"])
-    elif name == "gmm1m":
-        n = n or 1_000_000
-        sizes = 1.0 / np.arange(1, 41) ** 1.2
-        x, lab = gmm_points(n, 30, sizes, seed, center_scale=CENTER_SCALE["gmm1m"],
-                            background_frac=0.05)
+    if name in ("blobs2k", "gmm100k", "gmm1m"):
+        x, lab = workload_points(name, seed, n)
+        n = x.shape[0]
     elif name == "comp64":
         # 64 disconnected components, sizes log-uniform in [50, 200 000] (~2M nodes)
         # component offsets ~N(0, 0.5^2 I): centroid affinities exp(-d^2) stay well
