@@ -395,6 +395,21 @@ def main():
         log(f"rSVD run {i}: {dt:.1f} ms residual={rres:.3e}")
         del rvec
 
+    dense = None
+    if n <= 4096:
+        # level 0/5 of the chain: the k smallest pairs of the dense Laplacian
+        times_d = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            dev, dvec = si.dense_evd(lap, k)
+            torch.cuda.synchronize()
+            times_d.append(1e3 * (time.perf_counter() - t))
+        dres = si.max_eigenpair_residual(lap, dev, dvec)
+        dense = dict(ms=round(float(np.median(times_d)), 2), max_residual=dres,
+                     eigenvalues=[float(v) for v in dev])
+        log(f"dense EVD: {dense['ms']} ms residual={dres:.2e}")
+
     cpu = None
     if not args.no_cpu:
         log("CPU oracle sample ...")
@@ -434,6 +449,7 @@ def main():
                    "components_generated": ncomp},
         "plain_gpu_lobpcg": {"ms": plain_ms, "runs": plain},
         "rsvd_oversampling10_4iter": rsvd or None,
+        "dense_evd": dense,
         "speedup_vs_plain_gpu_lobpcg": round(plain_ms / fast_ms, 2) if plain_ms else None,
         "fast_solver": {"rounds": last.iterations, "spmm_calls": last.spmm_calls,
                         "max_residual": last.max_residual, "eigenvalues": last.eigenvalues[:k],

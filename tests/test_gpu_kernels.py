@@ -177,6 +177,12 @@ def test_spmm_respects_leading_dimension(ctx):
     y = yd.cpu().numpy()
     np.testing.assert_allclose(y[:, :3], O.spmm(L, xfull[:, :3]), rtol=1e-12, atol=1e-13)
     assert np.all(y[:, 3:] == 0)
+    # 8 columns with an odd leading dimension: no 16-byte vector schedule
+    yd.zero_()
+    lap.spmm(xd[:, :8], yd[:, :8], si.SI_F64)
+    y = yd.cpu().numpy()
+    np.testing.assert_allclose(y[:, :8], O.spmm(L, xfull[:, :8]), rtol=1e-12, atol=1e-13)
+    assert np.all(y[:, 8:] == 0)
 
 
 def test_spmm_is_deterministic(ctx):
