@@ -430,6 +430,8 @@ void fuzzy_union_device(const int32_t* idx, const float* vals, int64_t n, int64_
     SI_CHECK(n >= 1 && k >= 1, kErrInvalidArgument, "fuzzy_union: need n >= 1, k >= 1");
     SI_CHECK(n < ((int64_t)1 << 31), kErrUnsupported, "fuzzy_union: n >= 2^31 is not supported");
     const int64_t m = 2 * n * k;
+    SI_CHECK(m < ((int64_t)1 << 31), kErrUnsupported,
+             "fuzzy_union: 2 * n * k >= 2^31 entries is not supported");
     const uint64_t sentinel = (uint64_t)n * (uint64_t)n;
     int end_bit = 1;
     while (end_bit < 64 && (sentinel >> end_bit) != 0) ++end_bit;
@@ -457,6 +459,16 @@ void fuzzy_union_device(const int32_t* idx, const float* vals, int64_t n, int64_
     SI_HIP(hipMemcpyAsync(&u32, pos.get() + m, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     SI_HIP(hipStreamSynchronize(s));
     const int64_t u = u32;
+    if (u == 0) {  // no non-zero membership: a graph without edges
+        g.n = n;
+        g.nnz = 0;
+        g.ip.alloc(n + 1);
+        g.ix.alloc(0);
+        g.w.alloc(0);
+        SI_HIP(hipMemsetAsync(g.ip.get(), 0, sizeof(int64_t) * (n + 1), s));
+        SI_HIP(hipStreamSynchronize(s));
+        return;
+    }
     DevBuf<uint64_t> ukey(u);
     DevBuf<float> uw(u);
     hipLaunchKernelGGL(k_union_combine, dim3(ceil_div(m, 256)), dim3(256), 0, s, key2.get(),

@@ -196,6 +196,50 @@ def test_fuzzy_graph_100k_properties_and_sampled_rows(ctx):
         np.testing.assert_array_equal(a, b)
 
 
+def test_fuzzy_graph_1m_points_properties(ctx):
+    """The 1M-point, 30-d workload at full size: sampled neighbour lists equal a
+    float64 brute force, sigma/rho equal the oracle's on the device's lists, W
+    is bitwise symmetric with weights in (0, 1] and every row non-empty."""
+    import graphs
+    print("case fuzzy_graph 1m", flush=True)
+    x, _ = graphs.workload_points("gmm1m", 0)
+    n, k = x.shape[0], 15
+    idx, dist = si.knn_exact(ctx, x, k)
+    x64 = x.astype(np.float64)
+    for i in np.random.default_rng(2).choice(n, 16, replace=False):
+        d2 = ((x64 - x64[i]) ** 2).sum(1)
+        d2[i] = np.inf
+        np.testing.assert_array_equal(idx[i], np.argsort(d2, kind="stable")[:k])
+    sig, rho = si.smooth_knn_dist(ctx, dist)
+    osig, orho = O.smooth_knn_dist(dist)
+    np.testing.assert_array_equal(sig, osig)
+    np.testing.assert_array_equal(rho, orho)
+    ip, ix, w = si.fuzzy_graph(ctx, x, k).to_host()
+    W = sp.csr_matrix((w, ix, ip), shape=(n, n))
+    assert (W != W.T).nnz == 0
+    assert w.min() > 0 and w.max() <= 1.0
+    assert np.all(np.diff(ip) >= 1)
+    # columns strictly ascending inside every row
+    within = np.ones(ix.size - 1, bool)
+    within[ip[1:-1] - 1] = False
+    assert np.all(np.diff(ix.astype(np.int64))[within] > 0)
+
+
+def test_fuzzy_union_edge_cases(ctx):
+    """All memberships zero: a graph without edges; k = 1; missing (-1) entries."""
+    n, k = 50, 4
+    idx = np.stack([(np.arange(k) + i + 1) % n for i in range(n)]).astype(np.int32)
+    ip, ix, w = si.fuzzy_union(ctx, idx, np.zeros((n, k), np.float32), n_epochs=0).to_host()
+    assert np.all(ip == 0) and ix.size == 0 and w.size == 0
+    v = np.random.default_rng(1).uniform(0.1, 1.0, (n, k)).astype(np.float32)
+    idx[::5, 2] = -1
+    for kk in (1, k):
+        g = si.fuzzy_union(ctx, idx[:, :kk], v[:, :kk], n_epochs=0).to_host()
+        og = O.fuzzy_union(idx[:, :kk], v[:, :kk], 0)
+        for a, b in zip(g, og):
+            np.testing.assert_array_equal(a, b)
+
+
 def test_graph_builder_errors(ctx):
     x = np.zeros((10, 65), np.float32)
     with pytest.raises(si.Unsupported):
