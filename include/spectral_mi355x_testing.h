@@ -69,6 +69,12 @@ typedef struct {
 si_status si_profile_spmm_begin(si_context* ctx);
 si_status si_profile_spmm_end(si_context* ctx, si_spmm_profile* out);
 
+/* The locality order of si_laplacian_build (reverse Cuthill-McKee; perm[new] =
+ * old) of a host CSR pattern, on `threads` host threads (0: as many as the
+ * hardware allows, at most 16).  Host-only: needs no device. */
+si_status si_test_rcm_order(int64_t n, const int64_t* indptr, const int32_t* indices,
+                            int32_t threads, int32_t* perm);
+
 #ifdef __cplusplus
 }
 #endif

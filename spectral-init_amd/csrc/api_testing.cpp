@@ -214,4 +214,16 @@ si_status si_test_spmm_shapes(si_context* ctx, si_laplacian* lap, int32_t shape1
     });
 }
 
+si_status si_test_rcm_order(int64_t n, const int64_t* indptr, const int32_t* indices,
+                            int32_t threads, int32_t* perm) {
+    if (n < 1 || !indptr || !perm || (indptr[n] > 0 && !indices)) return SI_ERR_INVALID_ARGUMENT;
+    try {
+        const std::vector<int32_t> p = si::rcm_order(n, indptr, indices, threads);
+        std::copy(p.begin(), p.end(), perm);
+        return SI_OK;
+    } catch (...) {
+        return SI_ERR_INTERNAL;
+    }
+}
+
 }  // extern "C"

@@ -25,6 +25,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "si_graph.hpp"
@@ -314,9 +315,11 @@ __global__ void k_sum_to_f64(const T* __restrict__ v, int64_t m, double* __restr
 }
 
 void upload_exp_table() {
+    static std::mutex mu;
     static bool done[64] = {false};
     int dev = 0;
     SI_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
     if (dev < 64 && done[dev]) return;
     uint64_t tab[32];
     for (int i = 0; i < 32; ++i) {

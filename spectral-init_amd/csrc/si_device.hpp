@@ -64,8 +64,10 @@ void laplacian_from_values_device(DevLaplacian& L, int64_t n, const int64_t* h_i
                                   const int32_t* h_ix, const double* h_l, const double* h_deg,
                                   hipStream_t s);
 void laplacian_free(DevLaplacian& L);
-// perm[new] = old of a symmetric CSR pattern (reverse Cuthill-McKee).
-std::vector<int32_t> rcm_order(int64_t n, const int64_t* ip, const int32_t* ix);
+// perm[new] = old of a symmetric CSR pattern (reverse Cuthill-McKee) on up to
+// `threads` host threads (0: as the hardware allows, at most 16); the order
+// does not depend on the thread count.
+std::vector<int32_t> rcm_order(int64_t n, const int64_t* ip, const int32_t* ix, int threads = 0);
 // dst[i, :k] = src[idx[i], :k] (row gather); idx == nullptr copies.
 void gather_rows_f64(const double* src, int lds, double* dst, int ldd, int64_t n, int k,
                      const int32_t* idx, hipStream_t s);

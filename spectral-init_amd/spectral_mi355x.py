@@ -41,7 +41,7 @@ EXPORTED = [
     "si_spectral_init",
     "si_test_gram", "si_test_block_mul", "si_test_small_sym_eig", "si_test_small_gen_eig",
     "si_test_fill_normal", "si_test_fast_solve", "si_test_batch_graphs", "si_test_spmm_shapes",
-    "si_test_band_solve",
+    "si_test_band_solve", "si_test_rcm_order",
     "si_profile_spmm_begin", "si_profile_spmm_end",
     "si_mpnn_param_count", "si_mpnn_create", "si_mpnn_destroy", "si_mpnn_get_params",
     "si_mpnn_set_params", "si_mpnn_features", "si_mpnn_forward", "si_mpnn_gradient",
@@ -203,6 +203,7 @@ def lib():
         L.si_test_fast_solve.argtypes = [vp, vp, i32, u64, C.POINTER(_Config), vp, vp,
                                          C.POINTER(_Report)]
         L.si_test_band_solve.argtypes = [vp, vp, dbl, vp, vp, i32, C.POINTER(i32)]
+        L.si_test_rcm_order.argtypes = [i64, vp, vp, i32, vp]
         L.si_profile_spmm_begin.argtypes = [vp]
         L.si_profile_spmm_end.argtypes = [vp, C.POINTER(_SpmmProfile)]
         for name in EXPORTED:
@@ -732,3 +733,17 @@ def fuzzy_graph(ctx: Context, x, k: int = 15, n_epochs: int = -1) -> Graph:
     ctx._check(lib().si_fuzzy_graph_build(ctx.handle, x.ctypes.data, prec, n, d, k,
                                           int(n_epochs), C.byref(h)))
     return Graph(ctx, h)
+
+
+def rcm_order(indptr, indices, threads: int = 0) -> np.ndarray:
+    """Testing seam (host only): the locality order of si_laplacian_build,
+    perm[new] = old."""
+    ip = np.ascontiguousarray(indptr, np.int64)
+    ix = np.ascontiguousarray(indices, np.int32)
+    n = ip.size - 1
+    perm = np.empty(n, np.int32)
+    st = lib().si_test_rcm_order(n, ip.ctypes.data, ix.ctypes.data if ix.size else None,
+                                 int(threads), perm.ctypes.data)
+    if st != SI_OK:
+        raise _ERRORS.get(st, SpectralError)(f"status {st}")
+    return perm
