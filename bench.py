@@ -244,8 +244,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(see DESIGN.md for how the solve would shard); run with --gpus 1",
+        print("bench.py measures one GPU: the library has no multi-GPU path "
+              "(the design notes describe how the solve would shard); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
 
