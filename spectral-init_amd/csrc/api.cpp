@@ -368,6 +368,7 @@ void si_context_destroy(si_context* ctx) {
         hipStreamSynchronize(ctx->c.stream);
         hipStreamDestroy(ctx->c.stream);
     }
+    si::spmm_profile_release(&ctx->c.profile);
     delete ctx;
 }
 

@@ -166,7 +166,15 @@ si_status si_test_band_solve(si_context* ctx, const si_laplacian* lap, double sh
 si_status si_profile_spmm_begin(si_context* ctx) {
     if (!ctx) return SI_ERR_INVALID_ARGUMENT;
     return tguard(ctx, [&] {
+        std::vector<hipEvent_t> pool = std::move(ctx->c.profile.pool);  // kept across regions
         ctx->c.profile = si::SpmmProfile{};
+        ctx->c.profile.pool = std::move(pool);
+        // create the events of a long region up front, outside the timed work
+        while (ctx->c.profile.pool.size() < 1024) {
+            hipEvent_t e = nullptr;
+            SI_HIP(hipEventCreate(&e));
+            ctx->c.profile.pool.push_back(e);
+        }
         ctx->c.profile.enabled = true;
         si::spmm_profile_attach(&ctx->c.profile);
     });

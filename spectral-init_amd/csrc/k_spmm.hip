@@ -17,6 +17,8 @@
 // Every reduction has a fixed order, so results are deterministic.
 // Blocks are remapped so that each of the 8 XCDs streams a contiguous range
 // of rows (its L2 then holds the x rows that neighbouring rows gather).
+#include <hip/hip_ext.h>
+
 #include "si_device.hpp"
 
 namespace si {
@@ -24,6 +26,15 @@ namespace si {
 namespace {
 
 SpmmProfile* g_profile = nullptr;  // set by the benchmark seam; host-side only
+
+hipEvent_t pooled_event(SpmmProfile* p) {
+    if (p->pool_next == p->pool.size()) {
+        hipEvent_t e = nullptr;
+        SI_HIP(hipEventCreate(&e));
+        p->pool.push_back(e);
+    }
+    return p->pool[p->pool_next++];
+}
 
 template <typename T> struct Vec4;
 template <> struct Vec4<float> { using type = float4; static constexpr int n = 4; };
@@ -269,16 +280,29 @@ struct Epi {
     const T* z;
     const int32_t* rseg;
     const T* coef;
+    // profiled launches: events recorded by the kernel's own dispatch
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
+
+// Launches `kernel`; with profiling events set, through hipExtLaunchKernelGGL,
+// whose dispatch packet carries the start and stop timestamps (no separate
+// marker packets between the solver's launches).
+template <typename F, typename... Args>
+void launch(F kernel, dim3 grid, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, Args... args) {
+    if (ev0)
+        hipExtLaunchKernelGGL(kernel, grid, dim3(256), 0, s, ev0, ev1, 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, dim3(256), 0, s, args...);
+}
 
 template <typename T, int K, int G>
 void launch_narrow(const DevLaplacian& L, const T* val, const T* x, T* out, int ld,
                    const Epi<T>& ep, hipStream_t s) {
     const int64_t threads = L.n * G;
     const int64_t blocks = ceil_div(threads, 256);
-    hipLaunchKernelGGL((k_spmm_narrow<T, K, G>), dim3(blocks + L.n_long), dim3(256), 0, s, L.rowptr,
-                       L.col, val, x, out, ep.z, ld, L.n, ep.s_l, ep.s_x, ep.s_z, ep.rseg, ep.coef,
-                       L.long_rows, blocks);
+    launch(k_spmm_narrow<T, K, G>, dim3(blocks + L.n_long), s, ep.ev0, ep.ev1,
+           (const int32_t*)L.rowptr, (const int32_t*)L.col, val, x, out, ep.z, ld, L.n, ep.s_l,
+           ep.s_x, ep.s_z, ep.rseg, ep.coef, (const int32_t*)L.long_rows, blocks);
 }
 
 template <typename T, int K, int SLOTS, int U>
@@ -289,9 +313,9 @@ void launch_wide(const DevLaplacian& L, const T* val, const T* x, T* out, int ld
     constexpr int RPW = 64 / (C * SLOTS);
     const int64_t waves = ceil_div(L.n, RPW);
     const int64_t blocks = ceil_div(waves, 4);
-    hipLaunchKernelGGL((k_spmm_wide<T, K, SLOTS, U>), dim3(blocks + L.n_long), dim3(256), 0, s,
-                       L.rowptr, L.col, val, x, out, ep.z, ld, L.n, ep.s_l, ep.s_x, ep.s_z, ep.rseg,
-                       ep.coef, L.long_rows, blocks);
+    launch(k_spmm_wide<T, K, SLOTS, U>, dim3(blocks + L.n_long), s, ep.ev0, ep.ev1,
+           (const int32_t*)L.rowptr, (const int32_t*)L.col, val, x, out, ep.z, ld, L.n, ep.s_l,
+           ep.s_x, ep.s_z, ep.rseg, ep.coef, (const int32_t*)L.long_rows, blocks);
 }
 
 // One launch of a directly supported width (1, 2, 3, 4, 8, 16, 32, 64).
@@ -355,12 +379,7 @@ void spmm_impl(const DevLaplacian& L, const T* val, const T* x, T* out, int k, i
     SI_CHECK(k >= 1 && ld >= k, kErrInvalidArgument, "spmm: need 1 <= k <= ld");
     SI_CHECK(x != out && (z == nullptr || z != out), kErrInvalidArgument,
              "spmm: output must not alias an input");
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    if (g_profile && g_profile->enabled) {
-        SI_HIP(hipEventCreate(&ev0));
-        SI_HIP(hipEventCreate(&ev1));
-        SI_HIP(hipEventRecord(ev0, s));
-    }
+    const bool prof = g_profile && g_profile->enabled;
     const bool vec_ok = (ld % Vec4<T>::n) == 0 &&
                         (reinterpret_cast<uintptr_t>(x) % 16) == 0 &&
                         (reinterpret_cast<uintptr_t>(out) % 16) == 0 &&
@@ -378,16 +397,18 @@ void spmm_impl(const DevLaplacian& L, const T* val, const T* x, T* out, int k, i
             for (int c : widths_narrow)
                 if (c <= rem) { w = c; break; }
         }
-        const Epi<T> ep{s_l, s_x, s_z, z ? z + off : nullptr, rseg, coef};
+        Epi<T> ep{s_l, s_x, s_z, z ? z + off : nullptr, rseg, coef};
+        if (prof) {
+            ep.ev0 = pooled_event(g_profile);
+            ep.ev1 = pooled_event(g_profile);
+            // algorithmic bytes of this launch
+            const double bytes = (double)L.nnz * (sizeof(T) + 4) + 4.0 * (L.n + 1) +
+                                 (double)sizeof(T) * L.n * w * (z ? 3.0 : 2.0);
+            g_profile->pending.emplace_back(ep.ev0, ep.ev1);
+            g_profile->pending_bytes.emplace_back(bytes, sizeof(T) == 4 ? 1 : 0);
+        }
         spmm_chunk<T>(L, val, x + off, out + off, w, ld, ep, s);
         off += w;
-    }
-    if (ev0) {
-        SI_HIP(hipEventRecord(ev1, s));
-        const double bytes = (double)L.nnz * (sizeof(T) + 4) + 4.0 * (L.n + 1) +
-                             (double)sizeof(T) * L.n * k * (z ? 3.0 : 2.0);
-        g_profile->pending.emplace_back(ev0, ev1);
-        g_profile->pending_bytes.emplace_back(bytes, sizeof(T) == 4 ? 1 : 0);
     }
 }
 
@@ -411,11 +432,17 @@ void spmm_profile_collect(SpmmProfile* p) {
             p->bytes_f32 += p->pending_bytes[i].first;
             p->launches_f32 += 1;
         }
-        hipEventDestroy(p->pending[i].first);
-        hipEventDestroy(p->pending[i].second);
     }
     p->pending.clear();
     p->pending_bytes.clear();
+    p->pool_next = 0;  // every pooled event has completed: reuse them
+}
+
+void spmm_profile_release(SpmmProfile* p) {
+    if (!p) return;
+    for (hipEvent_t e : p->pool) (void)hipEventDestroy(e);
+    p->pool.clear();
+    p->pool_next = 0;
 }
 
 bool spmm_width_supported(int k) { return k >= 1 && k <= 4096; }

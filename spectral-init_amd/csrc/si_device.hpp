@@ -109,7 +109,12 @@ struct SpmmProfile {
     int64_t launches_f32 = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
     std::vector<std::pair<double, int>> pending_bytes;  // (bytes, is_f32)
+    // events are created once and reused by every later profiled launch, so a
+    // profiled region pays two hipEventRecord calls per SpMM and no creation
+    std::vector<hipEvent_t> pool;
+    size_t pool_next = 0;
 };
+void spmm_profile_release(SpmmProfile* p);    // destroys the pooled events
 void spmm_profile_attach(SpmmProfile* p);     // nullptr detaches
 void spmm_profile_collect(SpmmProfile* p);    // sync + fold pending events into the sums
 
