@@ -55,6 +55,9 @@ enum WsSlot : int {
 };
 void* workspace_get(int slot, size_t bytes);
 void workspace_release();
+// Per-thread page-locked host buffer of at least `bytes` (grow-only; released
+// with the workspace).  Its contents are valid until the next call.
+void* pinned_stage(size_t bytes);
 template <typename T>
 inline T* ws_get(int slot, size_t count) {
     return static_cast<T*>(workspace_get(slot, sizeof(T) * (count ? count : 1)));

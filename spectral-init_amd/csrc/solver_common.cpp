@@ -12,7 +12,26 @@ struct Workspace {
     Workspace() : slots(kWsNumSlots) {}
 };
 thread_local Workspace* g_ws = nullptr;
+// page-locked host staging for the solvers' small per-round downloads (a copy
+// into pageable memory goes through the runtime's own staging blits)
+struct PinnedStage {
+    void* p = nullptr;
+    size_t n = 0;
+};
+thread_local PinnedStage* g_pin = nullptr;
 }  // namespace
+
+void* pinned_stage(size_t bytes) {
+    if (!g_pin) g_pin = new PinnedStage();
+    if (g_pin->n < bytes) {
+        if (g_pin->p) SI_HIP(hipHostFree(g_pin->p));
+        g_pin->p = nullptr;
+        g_pin->n = 0;
+        SI_HIP(hipHostMalloc(&g_pin->p, bytes, hipHostMallocDefault));
+        g_pin->n = bytes;
+    }
+    return g_pin->p;
+}
 
 void* workspace_get(int slot, size_t bytes) {
     if (!g_ws) g_ws = new Workspace();
@@ -27,6 +46,11 @@ void* workspace_get(int slot, size_t bytes) {
 void workspace_release() {
     delete g_ws;
     g_ws = nullptr;
+    if (g_pin) {
+        if (g_pin->p) (void)hipHostFree(g_pin->p);
+        delete g_pin;
+        g_pin = nullptr;
+    }
 }
 
 // ||L v_j - lambda_j v_j|| / max(||v_j||, eps): one block SpMM, one residual

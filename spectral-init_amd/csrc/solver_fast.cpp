@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 
 #include "si_host.hpp"
 
@@ -137,15 +138,20 @@ inline void residual_t(int64_t n, const double* ax, const double* x, int ld, int
 }
 
 // Small FP64 matrices of one solve, carved from one workspace slot.
+// Layout: g, g2, t, v (m x m each), then w | sc | res | status contiguous, so
+// one download returns the Ritz values, the residual norms and the status.
 struct Small {
     double *g, *g2, *t, *w, *v, *sc, *res;
     int* status;
-    explicit Small(int m) {
+    int m;
+    explicit Small(int m_) : m(m_) {
         const size_t mm = (size_t)m * m;
         double* base = ws_get<double>(kWsSmall6, 4 * mm + 4 * (size_t)m + 32);
         g = base; g2 = g + mm; t = g2 + mm; v = t + mm; w = v + mm; sc = w + m; res = sc + m;
-        status = ws_get<int>(kWsStatus, 1);
+        status = reinterpret_cast<int*>(res + 2 * m + 16);  // res uses at most 2m + 16
     }
+    // bytes from w to the end of status
+    size_t tail_bytes() const { return sizeof(double) * (4 * (size_t)m + 16) + sizeof(int); }
 };
 
 // Normalise columns, then Cholesky-QR twice (FP64 Gram).  Returns false if a
@@ -219,13 +225,15 @@ bool fused_rr(const DevLaplacian& L, int64_t n, T* y, T* ly, int w, int ld, int 
     gram2_t(n, y, ld, w, ly, g, gws, s);
     small_gen_eig_scaled(w, g + w, 2 * w, g, 2 * w, sm.w, sm.v, sm.status, s);
     rr_update_t(n, y, ly, ld, w, sm.v, sm.w, k, x, ax, sm.res, rr_ws, s);
-    int st = 0;
-    std::vector<double> nrm(2 * k);
-    SI_HIP(hipMemcpyAsync(&st, sm.status, sizeof(int), hipMemcpyDeviceToHost, s));
-    SI_HIP(hipMemcpyAsync(theta_host, sm.w, sizeof(double) * w, hipMemcpyDeviceToHost, s));
-    SI_HIP(hipMemcpyAsync(nrm.data(), sm.res, sizeof(double) * 2 * k, hipMemcpyDeviceToHost, s));
+    // one download into page-locked memory: Ritz values, norms and status
+    double* h = static_cast<double*>(pinned_stage(sm.tail_bytes()));
+    SI_HIP(hipMemcpyAsync(h, sm.w, sm.tail_bytes(), hipMemcpyDeviceToHost, s));
     SI_HIP(hipStreamSynchronize(s));
+    int st = 0;
+    std::memcpy(&st, h + 4 * sm.m + 16, sizeof(int));
     if (st != 0) return false;
+    for (int j = 0; j < w; ++j) theta_host[j] = h[j];
+    const double* nrm = h + 2 * sm.m;  // sm.res
     for (int j = 0; j < k; ++j) {
         double xn = std::sqrt(std::fmax(nrm[k + j], 0.0));
         if (xn < 2.220446049250313e-16) xn = 2.220446049250313e-16;
