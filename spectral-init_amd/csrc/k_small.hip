@@ -47,7 +47,8 @@ __device__ void jacobi_lds(double (*a)[LDA], double (*v)[LDA], int n, double* cs
     for (int e = t; e < n * n; e += (int)blockDim.x) v[e / n][e % n] = (e / n == e % n) ? 1.0 : 0.0;
     __syncthreads();
     for (int sweep = 0; sweep < 60; ++sweep) {
-        // convergence test: off-diagonal mass relative to total
+        // convergence test: off-diagonal mass relative to total (wavefront
+        // butterflies, then the per-wave sums in a fixed order)
         double off = 0.0, tot = 0.0;
         for (int e = t; e < n * n; e += (int)blockDim.x) {
             const int i = e / n, j = e % n;
@@ -55,14 +56,15 @@ __device__ void jacobi_lds(double (*a)[LDA], double (*v)[LDA], int n, double* cs
             tot += q2;
             if (i != j) off += q2;
         }
-        red[t] = off;
-        red[blockDim.x + t] = tot;
-        __syncthreads();
-        for (int s = (int)blockDim.x / 2; s > 0; s >>= 1) {
-            if (t < s) { red[t] += red[t + s]; red[blockDim.x + t] += red[blockDim.x + t + s]; }
-            __syncthreads();
+        for (int o = 32; o > 0; o >>= 1) {
+            off += __shfl_xor(off, o, 64);
+            tot += __shfl_xor(tot, o, 64);
         }
-        const double offs = red[0], tots = red[blockDim.x];
+        const int nw = ((int)blockDim.x + 63) / 64;
+        if ((t & 63) == 0) { red[t >> 6] = off; red[nw + (t >> 6)] = tot; }
+        __syncthreads();
+        double offs = 0.0, tots = 0.0;
+        for (int w = 0; w < nw; ++w) { offs += red[w]; tots += red[nw + w]; }
         __syncthreads();
         if (offs <= 1e-30 * tots || offs == 0.0 || n < 2) break;
         for (int r = 0; r < m2 - 1; ++r) {
@@ -97,23 +99,19 @@ __device__ void jacobi_lds(double (*a)[LDA], double (*v)[LDA], int n, double* cs
                 a[q][j] = s * apj + c * aqj;
             }
             __syncthreads();
-            // columns: A <- A J, V <- V J
+            // columns: A <- A J, V <- V J; the annihilated pair (p, q) is set
+            // to exactly zero by the lanes of rows p and q
             for (int e = t; e < half * n; e += (int)blockDim.x) {
                 const int pi = e / n, j = e % n;
                 const int p = pp[pi], q = qq[pi];
                 if (p == q) continue;
                 const double c = cs[pi], s = sn[pi];
                 const double ajp = a[j][p], ajq = a[j][q];
-                a[j][p] = c * ajp - s * ajq;
-                a[j][q] = s * ajp + c * ajq;
+                a[j][p] = (j == q) ? 0.0 : c * ajp - s * ajq;
+                a[j][q] = (j == p) ? 0.0 : s * ajp + c * ajq;
                 const double vjp = v[j][p], vjq = v[j][q];
                 v[j][p] = c * vjp - s * vjq;
                 v[j][q] = s * vjp + c * vjq;
-            }
-            __syncthreads();
-            if (t < half) {
-                const int p = pp[t], q = qq[t];
-                if (p != q) { a[p][q] = 0.0; a[q][p] = 0.0; }
             }
             __syncthreads();
         }
@@ -125,14 +123,16 @@ __device__ void jacobi_lds(double (*a)[LDA], double (*v)[LDA], int n, double* cs
 __device__ void sort_and_store(double (*a)[LDA], double (*v)[LDA], int n, int* order, double* w,
                                double* out) {
     const int t = threadIdx.x;
-    if (t == 0) {
-        for (int i = 0; i < n; ++i) order[i] = i;
-        for (int i = 1; i < n; ++i) {
-            const int oi = order[i];
-            int j = i - 1;
-            while (j >= 0 && a[order[j]][order[j]] > a[oi][oi]) { order[j + 1] = order[j]; --j; }
-            order[j + 1] = oi;
+    // stable ascending order by rank: element i goes to the number of elements
+    // below it plus the equal ones before it (NaN ranks as +inf)
+    for (int i = t; i < n; i += (int)blockDim.x) {
+        const double di = isnan(a[i][i]) ? INFINITY : a[i][i];
+        int rank = 0;
+        for (int j = 0; j < n; ++j) {
+            const double dj = isnan(a[j][j]) ? INFINITY : a[j][j];
+            rank += (dj < di || (dj == di && j < i)) ? 1 : 0;
         }
+        order[rank] = i;
     }
     __syncthreads();
     for (int i = t; i < n; i += (int)blockDim.x) w[i] = a[order[i]][order[i]];
