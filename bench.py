@@ -140,8 +140,11 @@ def bench_components(args, torch, graphs, si):
         f"in {time.perf_counter() - t0:.1f}s")
     ctx = si.Context(0)
     res = {}
-    for batch in (True, False):
-        cfg = si.SpectralInitConfig(tol=args.tol, batch_components=batch)
+    # legs: the batched solve; the reference's loop (level 0 = dense EVD below
+    # 2000 nodes); the loop with the batched path's levels (dense below 65 nodes)
+    for batch, dense_thr in ((True, 2000), (False, 2000), ("loop65", 65)):
+        cfg = si.SpectralInitConfig(tol=args.tol, batch_components=batch is True,
+                                    dense_threshold=dense_thr)
         runs = []
         for i in range(args.warmup + args.steps):
             torch.cuda.synchronize()
@@ -179,8 +182,15 @@ def bench_components(args, torch, graphs, si):
                     "graphs_captured": cap, "graph_launches": launches, "runs": res[True][0]},
         "per_component_loop": {"solve_ms": ls, "end_to_end_ms": med(res[False][0], "end_to_end_ms"),
                                "laplacian_ms": med(res[False][0], "laplacian_ms"),
+                               "levels": "reference: dense EVD below 2000 nodes",
                                "runs": res[False][0]},
+        "per_component_loop_same_levels": {
+            "solve_ms": med(res["loop65"][0], "solve_ms"),
+            "end_to_end_ms": med(res["loop65"][0], "end_to_end_ms"),
+            "levels": "dense EVD below 65 nodes, as the batched path",
+            "runs": res["loop65"][0]},
         "speedup_solve_vs_loop": round(ls / bs, 2),
+        "speedup_solve_vs_loop_same_levels": round(med(res["loop65"][0], "solve_ms") / bs, 2),
         "max_abs_coord_diff_vs_loop": float(np.abs(res[True][1] - res[False][1]).max()),
     }
     print(json.dumps(out), flush=True)
