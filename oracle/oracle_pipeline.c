@@ -162,6 +162,9 @@ static int embed_disconnected(int64_t n, const int64_t* ip, const int32_t* ix, c
         for (int64_t i = 0; i < nc * dims; ++i) if (fabs(meta[i]) > mx) mx = fabs(meta[i]);
         if (mx == 0.0) status = 2;
         else for (int64_t i = 0; i < nc * dims; ++i) meta[i] /= mx;
+        /* the eigenvector signs are the eigensolver's choice (faer's in the
+         * reference, unpinned): fixed here by the normalize_signs rule */
+        or_normalize_signs(meta, nc, dims);
         free(cen); free(aff); free(deg); free(isd); free(sq); free(lip); free(lix); free(ld);
         free(ev); free(evec);
         if (status) goto cleanup;

@@ -784,6 +784,16 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
                 if (mx == 0.0)
                     throw si::Error(si::kErrInvalidGraph, "invalid graph: spectral meta-embedding produced all-zero result");
                 for (double& v : meta) v /= mx;
+                // eigenvector signs are the eigensolver's choice (faer's in the
+                // reference): fix them as normalize_signs does, so that the
+                // placement does not depend on the solver's internals
+                for (int c = 0; c < d; ++c) {
+                    double best = meta[c];
+                    for (int64_t r = 1; r < nc; ++r)
+                        if (std::fabs(meta[r * d + c]) > std::fabs(best)) best = meta[r * d + c];
+                    if (std::signbit(best))
+                        for (int64_t r = 0; r < nc; ++r) meta[r * d + c] = -meta[r * d + c];
+                }
             } else {
                 // orthogonal_meta_embedding (multi_component.rs:127-139)
                 for (int64_t r = 0; r < nc; ++r) meta[r * d + r % d] = r < d ? 1.0 : -1.0;

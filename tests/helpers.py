@@ -67,11 +67,23 @@ def orthonormal(x):
 
 
 def principal_angles(a, b):
-    """Principal angles (radians, descending) between span(a) and span(b)."""
+    """Principal angles (radians, descending) between span(a) and span(b).
+
+    Angles below pi/4 come from the sines (singular values of the part of
+    span(b) outside span(a)), the rest from the cosines: arccos of a cosine
+    one ulp below 1 is already 1.49e-8, so cosines alone cannot resolve small
+    angles (Knyazev & Argentati, SIAM J. Sci. Comput. 23(6), 2002)."""
     qa, qb = orthonormal(np.asarray(a, float)), orthonormal(np.asarray(b, float))
-    s = np.linalg.svd(qa.T @ qb, compute_uv=False)
-    s = np.clip(s, -1.0, 1.0)
-    return np.sort(np.arccos(s))[::-1]
+    c = qa.T @ qb
+    cos = np.clip(np.linalg.svd(c, compute_uv=False), -1.0, 1.0)
+    sin = np.clip(np.linalg.svd(qb - qa @ c, compute_uv=False), 0.0, 1.0)
+    big = np.arccos(cos)
+    big = big[big >= np.pi / 4]
+    small = np.arcsin(sin)
+    small = small[small < np.pi / 4]
+    k = min(qa.shape[1], qb.shape[1])
+    ang = np.sort(np.concatenate([big, small]))[::-1]
+    return ang[:k]
 
 
 def max_subspace_angle(a, b):
