@@ -23,6 +23,8 @@
 // sigma bisections round as the reference fixtures (generated on glibc) do.
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -73,8 +75,8 @@ __global__ void k_pad_points(const T* __restrict__ src, int64_t n, int d, int dp
 
 // Top-KP insertion of (s, j) into the ascending list (bd, bi); strict
 // comparisons keep the earlier index first on equal distances.
-template <int KP>
-__device__ inline void topk_insert(double (&bd)[KP], int32_t (&bi)[KP], double s, int32_t j) {
+template <int KP, typename D>
+__device__ inline void topk_insert(D (&bd)[KP], int32_t (&bi)[KP], D s, int32_t j) {
 #pragma unroll
     for (int p = KP - 1; p > 0; --p) {
         const bool mv = s < bd[p - 1];
@@ -95,10 +97,13 @@ template <> struct Lds16<double> { using type = double2; };
 // CT is the storage type of the points (float32 inputs stay float32 in HBM
 // and LDS, which halves the LDS bytes of the broadcast reads that bound this
 // kernel; every coordinate is widened to FP64 exactly before it is used).
+// With qlist set, the queries are qlist[0, nq) instead of 0 .. n-1.
 template <int DP, int KP, int Q, typename CT>
 __global__ __launch_bounds__(kKnnThreads) void k_knn(const CT* __restrict__ x, int64_t n,
                                                      int k, int32_t* __restrict__ oi,
-                                                     float* __restrict__ od) {
+                                                     float* __restrict__ od,
+                                                     const int32_t* __restrict__ qlist,
+                                                     int64_t nq) {
     using V = typename Lds16<CT>::type;
     constexpr int VN = 16 / (int)sizeof(CT);  // coordinates per 16-byte LDS read
     static_assert(DP % VN == 0, "padded dimension must fill 16-byte reads");
@@ -109,7 +114,8 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn(const CT* __restrict__ x, i
     int32_t bi[Q][KP];
 #pragma unroll
     for (int u = 0; u < Q; ++u) {
-        qi[u] = ((int64_t)blockIdx.x * Q + u) * kKnnThreads + threadIdx.x;
+        const int64_t slot = ((int64_t)blockIdx.x * Q + u) * kKnnThreads + threadIdx.x;
+        qi[u] = slot < nq ? (qlist ? (int64_t)qlist[slot] : slot) : n;
         const bool ok = qi[u] < n;
 #pragma unroll
         for (int t = 0; t < DP; ++t) q[u][t] = ok ? (double)x[qi[u] * DP + t] : 0.0;
@@ -155,6 +161,317 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn(const CT* __restrict__ x, i
                 oi[qi[u] * k + p] = bi[u][p];
                 od[qi[u] * k + p] = bi[u][p] >= 0 ? (float)sqrt(bd[u][p]) : INFINITY;
             }
+        }
+    }
+}
+
+// ------------------------------------------------------------ MFMA filter
+// Float32 points with k <= kFiltL take two stages that give exactly k_knn's
+// result:
+//   1. k_knn_filter: a = |q'|^2 + |c'|^2 - 2 q'.c' on the FP32 matrix cores
+//      (v_mfma_f32_16x16x4_f32) for the centred, float-rounded points q', c',
+//      and the lower bound l = a - E of the exact squared distance d^2, where
+//      E = 2 alpha (|q'|^2 + |c'|^2) + eabs bounds |a - d^2| (rounding of the
+//      centred coordinates, the norms, the MFMA sums and the epilogue, with
+//      a factor 2 to spare; eabs covers flushed subnormal products).  The
+//      factors fold into per-point constants, so l = fma(-2, q'.c', nq + nc)
+//      costs what a does.  Every lane keeps, for each of its queries, the
+//      kFiltL smallest l of the quarter of the candidates that its MFMA
+//      output rows cover (the query itself included; stage 2 skips it).
+//   2. k_knn_rerank: u = l + 2E >= d^2 for every kept candidate, and U_k, the
+//      k-th smallest u of the query, bounds its k-th distance d_k^2.  A true
+//      neighbour has l <= d^2 <= U_k, and a list drops a candidate only with
+//      l >= its final largest entry; so when every full list ends above U_k
+//      all the neighbours are kept, and their exact FP64 distances (k_knn's
+//      arithmetic) over the candidates with l <= U_k are ranked by
+//      (distance, index).  Every other query goes to k_knn through a list.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// 16-query tiles per wavefront: 4, or 2 above 48 coordinates (registers)
+template <int KS> constexpr int filt_tiles() { return KS > 12 ? 2 : 4; }
+constexpr int kFiltL = 16;                     // kept candidates per lane and query
+constexpr int kFiltTile = 256;                 // candidates per LDS tile
+constexpr int kFiltGroups = 4;                 // lanes sharing one query (one per quarter)
+template <int KS> constexpr int filt_queries() { return 4 * 16 * filt_tiles<KS>(); }  // per workgroup
+constexpr int kColParts = 2048;
+
+// Column sums of the padded points in FP64, part b over rows
+// [b n / P, (b + 1) n / P); one thread per column (dp <= 64 = blockDim).
+__global__ void k_col_partials(const float* __restrict__ x, int64_t n, int dp,
+                               double* __restrict__ part) {
+    const int64_t P = gridDim.x;
+    const int64_t r0 = n * blockIdx.x / P, r1 = n * (blockIdx.x + 1) / P;
+    const int t = threadIdx.x;
+    if (t >= dp) return;
+    double s = 0.0;
+    for (int64_t r = r0; r < r1; ++r) s += (double)x[r * dp + t];
+    part[(int64_t)blockIdx.x * dp + t] = s;
+}
+
+// 16 threads per column (1024 threads, dp <= 64) sum strided parts, then
+// lane 0 of each column adds the 16 sums in order (a fixed order).
+__global__ __launch_bounds__(1024) void k_col_mean(const double* __restrict__ part, int parts,
+                                                   int64_t n, int dp,
+                                                   double* __restrict__ mean) {
+    __shared__ double red[16][64];
+    const int t = threadIdx.x & 63, g = threadIdx.x >> 6;
+    double s = 0.0;
+    if (t < dp)
+        for (int b = g; b < parts; b += 16) s += part[(int64_t)b * dp + t];
+    red[g][t] = s;
+    __syncthreads();
+    if (g == 0 && t < dp) {
+        double m = 0.0;
+        for (int i = 0; i < 16; ++i) m += red[i][t];
+        mean[t] = m / (double)n;
+    }
+}
+
+// xc = float(x - mean), nrm = float(|xc|^2) (FP64 sum); the largest nrm's bit
+// pattern is max'ed into *nmax (non-negative floats order as their unsigned
+// bit patterns, and a NaN orders above +inf, so a non-finite input shows).
+template <int DP>
+__global__ void k_center_norms(const float* __restrict__ x, int64_t n,
+                               const double* __restrict__ mean, float* __restrict__ xc,
+                               float* __restrict__ nrm, unsigned* __restrict__ nmax) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float nv = 0.0f;
+    if (i < n) {
+        double s = 0.0;
+#pragma unroll
+        for (int t = 0; t < DP; ++t) {
+            const float c = (float)((double)x[i * DP + t] - mean[t]);
+            xc[i * DP + t] = c;
+            s += (double)c * (double)c;
+        }
+        nv = (float)s;
+        nrm[i] = nv;
+    }
+    unsigned b = __float_as_uint(nv);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned o = __shfl_xor(b, off);
+        b = o > b ? o : b;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(nmax, b);
+}
+
+// Stage 1.  A workgroup owns filt_queries() queries of [q0, q1) (16 per
+// MFMA tile, filt_tiles() tiles per wavefront) and streams the candidates of its split
+// (blockIdx.y of gridDim.y equal ranges of 0 .. n-1) through LDS; the split
+// keeps small inputs on enough workgroups to fill the chip.
+// Candidates are the A operand (rows), queries the B operand (columns):
+// lane l supplies row / column l % 16 and coordinates (l / 16) KS + s of
+// k-step s (A and B share that split, so the dot products are whole).  The
+// output row each element holds is read off an MFMA of index-valued operands
+// at the start; every element's column must be l % 16 (checked).
+template <int KS>
+__global__ __launch_bounds__(256) void k_knn_filter(const float* __restrict__ xc,
+                                                    const float* __restrict__ nrm, int64_t n,
+                                                    int64_t q0, int64_t q1,
+                                                    float shrink, float eabs,
+                                                    int32_t* __restrict__ kidx,
+                                                    float* __restrict__ kd2,
+                                                    int32_t* __restrict__ layout_err) {
+    constexpr int DP = 4 * KS;
+    constexpr int kFiltR = filt_tiles<KS>();
+    constexpr int DS = DP + 4;  // LDS row stride: a 16-byte shift per row spreads the banks
+    __shared__ __align__(16) float tile[kFiltTile * DS];
+    __shared__ float tn[kFiltTile];  // (1 - 2 alpha) |c'|^2, +inf past the last candidate
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lq = lane & 15, lg = lane >> 4;
+
+    const v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    const v4f pr = __builtin_amdgcn_mfma_f32_16x16x4f32(lg == 0 ? (float)lq : 0.0f,
+                                                        lg == 0 ? 1.0f : 0.0f, zero, 0, 0, 0);
+    const v4f pc = __builtin_amdgcn_mfma_f32_16x16x4f32(lg == 0 ? 1.0f : 0.0f,
+                                                        lg == 0 ? (float)lq : 0.0f, zero, 0, 0, 0);
+    int rowof[4];
+    bool bad = false;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        rowof[v] = (int)pr[v] & 15;
+        bad = bad || (int)pc[v] != lq || (float)rowof[v] != pr[v];
+    }
+    if (bad) *layout_err = 1;
+
+    const int64_t qb = q0 + ((int64_t)blockIdx.x * 4 + wave) * (16 * kFiltR);
+    const int64_t cb = n * blockIdx.y / gridDim.y, ce = n * (blockIdx.y + 1) / gridDim.y;
+    float bq[kFiltR][KS], nq[kFiltR];
+    int32_t qi[kFiltR];
+    float bd[kFiltR][kFiltL];
+    int32_t bi[kFiltR][kFiltL];
+#pragma unroll
+    for (int r = 0; r < kFiltR; ++r) {
+        const int64_t q = qb + 16 * r + lq;
+        const bool ok = q < q1;
+        qi[r] = ok ? (int32_t)q : -1;
+#pragma unroll
+        for (int s = 0; s < KS; ++s) bq[r][s] = ok ? xc[q * DP + lg * KS + s] : 0.0f;
+        nq[r] = ok ? nrm[q] * shrink - eabs : 0.0f;
+#pragma unroll
+        for (int p = 0; p < kFiltL; ++p) { bd[r][p] = INFINITY; bi[r][p] = -1; }
+    }
+
+    // sub-tile m: dot products of the 16 candidates with the 4 query tiles
+    auto dots = [&](int m, v4f (&acc)[kFiltR]) {
+        float a[KS];
+        const float* src = &tile[(16 * m + lq) * DS + lg * KS];
+        if constexpr (KS % 4 == 0) {
+#pragma unroll
+            for (int s = 0; s < KS; s += 4) {
+                const float4 w = *reinterpret_cast<const float4*>(src + s);
+                a[s] = w.x; a[s + 1] = w.y; a[s + 2] = w.z; a[s + 3] = w.w;
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < KS; ++s) a[s] = src[s];
+        }
+#pragma unroll
+        for (int r = 0; r < kFiltR; ++r) acc[r] = zero;
+#pragma unroll
+        for (int s = 0; s < KS; ++s)
+#pragma unroll
+            for (int r = 0; r < kFiltR; ++r)
+                acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], bq[r][s], acc[r], 0, 0, 0);
+    };
+    // bounds l of sub-tile m into the kept lists
+    auto keep = [&](int m, int64_t c0, const v4f (&acc)[kFiltR]) {
+        float cn[4];
+        int32_t cj[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = 16 * m + rowof[v];
+            cn[v] = tn[row];
+            cj[v] = (int32_t)(c0 + row);  // rows past the end have l = +inf
+        }
+#pragma unroll
+        for (int r = 0; r < kFiltR; ++r)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const float lo = fmaf(-2.0f, acc[r][v], nq[r] + cn[v]);
+                if (lo < bd[r][kFiltL - 1]) topk_insert<kFiltL>(bd[r], bi[r], lo, cj[v]);
+            }
+    };
+    for (int64_t c0 = cb; c0 < ce; c0 += kFiltTile) {
+        const int cnt = (int)((ce - c0) < kFiltTile ? (ce - c0) : kFiltTile);
+        __syncthreads();
+        for (int e = threadIdx.x; e < kFiltTile * KS; e += 256) {
+            const int row = e / KS, c4 = e - row * KS;
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (row < cnt) v = *reinterpret_cast<const float4*>(xc + (c0 + row) * DP + 4 * c4);
+            *reinterpret_cast<float4*>(&tile[row * DS + 4 * c4]) = v;
+        }
+        for (int e = threadIdx.x; e < kFiltTile; e += 256)
+            tn[e] = e < cnt ? nrm[c0 + e] * shrink : INFINITY;
+        __syncthreads();
+        // software pipeline: sub-tile m + 1's MFMAs are issued before sub-tile
+        // m's list updates, which then run while the matrix cores work
+        const int subs = (cnt + 15) / 16;
+        v4f acc[kFiltR];
+        dots(0, acc);
+        for (int m = 0; m + 1 < subs; ++m) {
+            v4f nxt[kFiltR];
+            dots(m + 1, nxt);
+            keep(m, c0, acc);
+#pragma unroll
+            for (int r = 0; r < kFiltR; ++r) acc[r] = nxt[r];
+        }
+        keep(subs - 1, c0, acc);
+    }
+#pragma unroll
+    for (int r = 0; r < kFiltR; ++r) {
+        if (qi[r] < 0) continue;
+        const int64_t o =
+            (((int64_t)(qi[r] - q0) * gridDim.y + blockIdx.y) * kFiltGroups + lg) * kFiltL;
+#pragma unroll
+        for (int p = 0; p < kFiltL; ++p) {
+            kidx[o + p] = bi[r][p];
+            kd2[o + p] = bd[r][p];
+        }
+    }
+}
+
+// (s, j) before (bs, bj): smaller distance, then smaller index.
+__device__ inline bool rank_before(double s, int32_t j, double bs, int32_t bj) {
+    return s < bs || (s == bs && j < bj);
+}
+
+// Stage 2, one query per thread (see the comment above filt_tiles).  A query that
+// fails the certificate, or every query with force_direct, is appended to
+// qlist for k_knn.
+template <int DP>
+__global__ __launch_bounds__(256) void k_knn_rerank(const float* __restrict__ xp, const float* __restrict__ nrm,
+                             int64_t q0, int64_t q1, int k, int lists, double alpha,
+                             double eabs, const int32_t* __restrict__ kidx,
+                             const float* __restrict__ kd2,
+                             int force_direct, int32_t* __restrict__ oi,
+                             float* __restrict__ od, int32_t* __restrict__ qlist,
+                             int32_t* __restrict__ qcount) {
+    const int64_t q = q0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= q1) return;
+    const int slots = lists * kFiltL;
+    const int64_t base = (q - q0) * slots;
+    const double nq = (double)nrm[q];
+    // U_k: the k-th smallest upper bound u = l + 2E over the kept candidates
+    double ub[kFiltL];
+#pragma unroll
+    for (int p = 0; p < kFiltL; ++p) ub[p] = INFINITY;
+    for (int e = 0; e < slots; ++e) {
+        const int32_t j = kidx[base + e];
+        if (j < 0 || j == q) continue;
+        const double u = (double)kd2[base + e] + 2.0 * (2.0 * alpha * (nq + (double)nrm[j]) + eabs);
+        if (!(u < ub[kFiltL - 1])) continue;
+#pragma unroll
+        for (int p = kFiltL - 1; p > 0; --p) {
+            const bool mv = u < ub[p - 1];
+            const bool here = !mv && u < ub[p];
+            ub[p] = mv ? ub[p - 1] : (here ? u : ub[p]);
+        }
+        if (u < ub[0]) ub[0] = u;
+    }
+    double uk = INFINITY;
+#pragma unroll
+    for (int p = 0; p < kFiltL; ++p)
+        if (p == k - 1) uk = ub[p];
+    bool direct = force_direct != 0;
+    for (int g = 0; g < lists; ++g) {
+        const int64_t e = base + g * kFiltL + kFiltL - 1;
+        if (kidx[e] >= 0 && !((double)kd2[e] > uk)) direct = true;
+    }
+    if (direct) {
+        qlist[atomicAdd(qcount, 1)] = (int32_t)q;
+        return;
+    }
+    const float* qx = xp + q * DP;
+    double bs[kFiltL];
+    int32_t bj[kFiltL];
+#pragma unroll
+    for (int p = 0; p < kFiltL; ++p) { bs[p] = INFINITY; bj[p] = -1; }
+    for (int e = 0; e < slots; ++e) {
+        const int32_t j = kidx[base + e];
+        if (j < 0 || j == q || (double)kd2[base + e] > uk) continue;
+        double s = 0.0;
+#pragma unroll
+        for (int t = 0; t < DP; ++t) {
+            const double df = (double)qx[t] - (double)xp[(int64_t)j * DP + t];
+            s = fma(df, df, s);
+        }
+        if (!rank_before(s, j, bs[kFiltL - 1], bj[kFiltL - 1])) continue;
+#pragma unroll
+        for (int p = kFiltL - 1; p > 0; --p) {
+            const bool mv = rank_before(s, j, bs[p - 1], bj[p - 1]);
+            const bool here = !mv && rank_before(s, j, bs[p], bj[p]);
+            bs[p] = mv ? bs[p - 1] : (here ? s : bs[p]);
+            bj[p] = mv ? bj[p - 1] : (here ? j : bj[p]);
+        }
+        if (rank_before(s, j, bs[0], bj[0])) { bs[0] = s; bj[0] = j; }
+    }
+#pragma unroll
+    for (int p = 0; p < kFiltL; ++p) {
+        if (p < k) {
+            oi[q * k + p] = bj[p];
+            od[q * k + p] = bj[p] >= 0 ? (float)sqrt(bs[p]) : INFINITY;
         }
     }
 }
@@ -332,23 +649,104 @@ void upload_exp_table() {
     if (dev < 64) done[dev] = true;
 }
 
+// k-NN path of later calls (testing seam): 0 automatic, 1 the direct FP64
+// kernel only, 2 the filter with every query sent to the direct kernel.
+std::atomic<int> g_knn_path{0};
+std::atomic<int64_t> g_knn_direct{0};  // queries of the last filtered call re-run directly
+
+// nq queries: qlist[0, nq), or 0 .. nq-1 without a list
 template <int DP, int KP, typename CT>
-void launch_knn_q(const CT* x, int64_t n, int k, int32_t* oi, float* od, hipStream_t s) {
+void launch_knn_q(const CT* x, int64_t n, int k, int32_t* oi, float* od, hipStream_t s,
+                  const int32_t* qlist, int64_t nq) {
     // two queries per thread halve the LDS reads per distance where the
     // registers allow it (DP <= 16, k <= 16; above, occupancy drops to one wave)
     constexpr int Q = (DP <= 16 && KP <= 16) ? 2 : 1;
-    const int64_t blocks = ceil_div(n, (int64_t)kKnnThreads * Q);
+    const int64_t blocks = ceil_div(nq, (int64_t)kKnnThreads * Q);
     hipLaunchKernelGGL((k_knn<DP, KP, Q, CT>), dim3(blocks), dim3(kKnnThreads), 0, s, x, n, k,
-                       oi, od);
+                       oi, od, qlist, nq);
     SI_LAUNCH_CHECK();
 }
 
 template <int DP, typename CT>
-void launch_knn_k(const CT* x, int64_t n, int k, int32_t* oi, float* od, hipStream_t s) {
-    if (k <= 8) launch_knn_q<DP, 8>(x, n, k, oi, od, s);
-    else if (k <= 16) launch_knn_q<DP, 16>(x, n, k, oi, od, s);
-    else if (k <= 32) launch_knn_q<DP, 32>(x, n, k, oi, od, s);
-    else launch_knn_q<DP, 64>(x, n, k, oi, od, s);
+void launch_knn_k(const CT* x, int64_t n, int k, int32_t* oi, float* od, hipStream_t s,
+                  const int32_t* qlist, int64_t nq) {
+    if (k <= 8) launch_knn_q<DP, 8>(x, n, k, oi, od, s, qlist, nq);
+    else if (k <= 16) launch_knn_q<DP, 16>(x, n, k, oi, od, s, qlist, nq);
+    else if (k <= 32) launch_knn_q<DP, 32>(x, n, k, oi, od, s, qlist, nq);
+    else launch_knn_q<DP, 64>(x, n, k, oi, od, s, qlist, nq);
+}
+
+// The two-stage path for float32 points padded to DP = 4 KS coordinates
+// (k <= kFiltL).  Returns false, having written nothing, when a squared norm
+// of the centred points is not finite or so large that the FP32 sums could
+// overflow; the caller then runs the direct kernel.
+template <int KS>
+bool knn_filtered(const float* xp, int64_t n, int k, int32_t* oi, float* od, hipStream_t s) {
+    constexpr int DP = 4 * KS;
+    DevBuf<double> part((size_t)kColParts * DP), mean(DP);
+    hipLaunchKernelGGL(k_col_partials, dim3(kColParts), dim3(64), 0, s, xp, n, DP, part.get());
+    hipLaunchKernelGGL(k_col_mean, dim3(1), dim3(1024), 0, s, part.get(), kColParts, n, DP,
+                       mean.get());
+    SI_LAUNCH_CHECK();
+    DevBuf<float> xc((size_t)n * DP), nrm(n);
+    DevBuf<int32_t> ctl(4);  // norm max bits, layout error, direct-query count
+    SI_HIP(hipMemsetAsync(ctl.get(), 0, 4 * sizeof(int32_t), s));
+    hipLaunchKernelGGL((k_center_norms<DP>), dim3(ceil_div(n, 256)), dim3(256), 0, s, xp, n,
+                       mean.get(), xc.get(), nrm.get(), reinterpret_cast<unsigned*>(ctl.get()));
+    SI_LAUNCH_CHECK();
+    uint32_t mbits = 0;
+    SI_HIP(hipMemcpyAsync(&mbits, ctl.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    SI_HIP(hipStreamSynchronize(s));
+    float m2 = 0.0f;
+    std::memcpy(&m2, &mbits, sizeof(float));
+    if (!(m2 <= 1e36f)) return false;  // also false for NaN
+    // |a - d^2| <= (DP + 12) u (|q'|^2 + |c'|^2) + eabs from the FP32 rounding
+    // of the centred coordinates, the norms, the MFMA sums and the epilogue
+    // (with (x + y)^2 <= 2 (x^2 + y^2)); 2 alpha = 2 (DP + 16) u leaves room
+    // for the rounding of l itself.  eabs covers flushed subnormal products.
+    const double u = std::ldexp(1.0, -24);
+    const double alpha = (double)(DP + 16) * u;
+    const double eabs = (double)DP * 1e-30;
+    // candidate splits: about 2048 workgroups, at most 16 splits
+    const int64_t qblocks = ceil_div(n, (int64_t)filt_queries<KS>());
+    const int splits = (int)std::max<int64_t>(1, std::min<int64_t>(16, ceil_div((int64_t)2048, qblocks)));
+    const int lists = kFiltGroups * splits;
+    // query chunks keep the kept lists within 4 GiB
+    const int64_t chunk = std::min<int64_t>(n, ((int64_t)1 << 29) / (lists * kFiltL));
+    DevBuf<int32_t> kidx((size_t)chunk * lists * kFiltL), qlist(n);
+    DevBuf<float> kd2((size_t)chunk * lists * kFiltL);
+    const int force = g_knn_path.load() == 2 ? 1 : 0;
+    for (int64_t q0 = 0; q0 < n; q0 += chunk) {
+        const int64_t q1 = std::min(n, q0 + chunk);
+        hipLaunchKernelGGL((k_knn_filter<KS>),
+                           dim3(ceil_div(q1 - q0, (int64_t)filt_queries<KS>()), splits), dim3(256), 0, s, xc.get(), nrm.get(), n, q0, q1,
+                           (float)(1.0 - 2.0 * alpha), (float)eabs, kidx.get(), kd2.get(),
+                           ctl.get() + 1);
+        hipLaunchKernelGGL((k_knn_rerank<DP>), dim3(ceil_div(q1 - q0, 256)), dim3(256), 0, s, xp,
+                           nrm.get(), q0, q1, k, lists, alpha, eabs, kidx.get(), kd2.get(), force,
+                           oi, od, qlist.get(), ctl.get() + 2);
+        SI_LAUNCH_CHECK();
+    }
+    int32_t h[2] = {0, 0};
+    SI_HIP(hipMemcpyAsync(h, ctl.get() + 1, sizeof(h), hipMemcpyDeviceToHost, s));
+    SI_HIP(hipStreamSynchronize(s));
+    SI_CHECK(h[0] == 0, kErrInternal, "knn: unexpected MFMA output layout");
+    g_knn_direct.store(h[1]);
+    if (h[1] > 0) launch_knn_k<DP, float>(xp, n, k, oi, od, s, qlist.get(), h[1]);
+    SI_HIP(hipStreamSynchronize(s));  // the buffers are freed on return
+    return true;
+}
+
+bool knn_filtered_dp(int dp, const float* xp, int64_t n, int k, int32_t* oi, float* od,
+                     hipStream_t s) {
+    switch (dp) {
+        case 4: return knn_filtered<1>(xp, n, k, oi, od, s);
+        case 8: return knn_filtered<2>(xp, n, k, oi, od, s);
+        case 16: return knn_filtered<4>(xp, n, k, oi, od, s);
+        case 32: return knn_filtered<8>(xp, n, k, oi, od, s);
+        case 48: return knn_filtered<12>(xp, n, k, oi, od, s);
+        default: return knn_filtered<16>(xp, n, k, oi, od, s);
+    }
 }
 
 int padded_dim(int64_t d, int elem_bytes) {
@@ -371,19 +769,28 @@ void knn_typed(const CT* pts, int64_t n, int64_t d, int64_t k, int32_t* idx, flo
     hipLaunchKernelGGL((k_pad_points<CT>), dim3(ceil_div(tot, 256)), dim3(256), 0, s, pts, n,
                        (int)d, dp, xp.get());
     SI_LAUNCH_CHECK();
+    if constexpr (sizeof(CT) == 4) {
+        if (g_knn_path.load() != 1 && k <= kFiltL &&
+            knn_filtered_dp(dp, xp.get(), n, (int)k, idx, dist, s))
+            return;
+    }
+    const CT* x = xp.get();
     switch (dp) {
-        case 2: if constexpr (sizeof(CT) == 8) launch_knn_k<2>(xp.get(), n, (int)k, idx, dist, s); break;
-        case 4: launch_knn_k<4>(xp.get(), n, (int)k, idx, dist, s); break;
-        case 8: launch_knn_k<8>(xp.get(), n, (int)k, idx, dist, s); break;
-        case 16: launch_knn_k<16>(xp.get(), n, (int)k, idx, dist, s); break;
-        case 32: launch_knn_k<32>(xp.get(), n, (int)k, idx, dist, s); break;
-        case 48: launch_knn_k<48>(xp.get(), n, (int)k, idx, dist, s); break;
-        default: launch_knn_k<64>(xp.get(), n, (int)k, idx, dist, s); break;
+        case 2: if constexpr (sizeof(CT) == 8) launch_knn_k<2>(x, n, (int)k, idx, dist, s, nullptr, n); break;
+        case 4: launch_knn_k<4>(x, n, (int)k, idx, dist, s, nullptr, n); break;
+        case 8: launch_knn_k<8>(x, n, (int)k, idx, dist, s, nullptr, n); break;
+        case 16: launch_knn_k<16>(x, n, (int)k, idx, dist, s, nullptr, n); break;
+        case 32: launch_knn_k<32>(x, n, (int)k, idx, dist, s, nullptr, n); break;
+        case 48: launch_knn_k<48>(x, n, (int)k, idx, dist, s, nullptr, n); break;
+        default: launch_knn_k<64>(x, n, (int)k, idx, dist, s, nullptr, n); break;
     }
     SI_HIP(hipStreamSynchronize(s));  // xp is freed on return
 }
 
 }  // namespace
+
+void knn_set_path(int mode) { g_knn_path.store(mode); }
+int64_t knn_direct_count() { return g_knn_direct.load(); }
 
 void knn_exact_device(const void* points_dev, int precision, int64_t n, int64_t d, int64_t k,
                       int32_t* idx, float* dist, hipStream_t s) {

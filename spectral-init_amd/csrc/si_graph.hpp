@@ -21,6 +21,12 @@ struct DevGraph {
 // device array), itself excluded; idx/dist [n x k] on the device.
 void knn_exact_device(const void* points_dev, int precision, int64_t n, int64_t d, int64_t k,
                       int32_t* idx, float* dist, hipStream_t s);
+// Testing seam of step 1: 0 automatic (float32 points with k <= 16 take the
+// MFMA filter and the certified FP64 re-rank), 1 the direct FP64 kernel only,
+// 2 the filter with every query re-run by the direct kernel; and the number
+// of queries the last filtered call re-ran directly.
+void knn_set_path(int mode);
+int64_t knn_direct_count();
 // Step 2: sigma, rho [n] from the distances [n x k] (device arrays).
 void smooth_knn_device(const float* dist, int64_t n, int64_t k, float* sig, float* rho,
                        hipStream_t s);

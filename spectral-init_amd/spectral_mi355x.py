@@ -41,7 +41,7 @@ EXPORTED = [
     "si_spectral_init",
     "si_test_gram", "si_test_block_mul", "si_test_small_sym_eig", "si_test_small_gen_eig",
     "si_test_fill_normal", "si_test_fast_solve", "si_test_batch_graphs", "si_test_spmm_shapes",
-    "si_test_band_solve", "si_test_rcm_order",
+    "si_test_band_solve", "si_test_rcm_order", "si_test_knn_path",
     "si_profile_spmm_begin", "si_profile_spmm_end",
     "si_mpnn_param_count", "si_mpnn_create", "si_mpnn_destroy", "si_mpnn_get_params",
     "si_mpnn_set_params", "si_mpnn_features", "si_mpnn_forward", "si_mpnn_gradient",
@@ -204,6 +204,7 @@ def lib():
                                          C.POINTER(_Report)]
         L.si_test_band_solve.argtypes = [vp, vp, dbl, vp, vp, i32, C.POINTER(i32)]
         L.si_test_rcm_order.argtypes = [i64, vp, vp, i32, vp]
+        L.si_test_knn_path.argtypes = [i32, C.POINTER(i64)]
         L.si_profile_spmm_begin.argtypes = [vp]
         L.si_profile_spmm_end.argtypes = [vp, C.POINTER(_SpmmProfile)]
         for name in EXPORTED:
@@ -747,3 +748,15 @@ def rcm_order(indptr, indices, threads: int = 0) -> np.ndarray:
     if st != SI_OK:
         raise _ERRORS.get(st, SpectralError)(f"status {st}")
     return perm
+
+
+def knn_path(mode: int = -1) -> int:
+    """Testing seam: select the exact k-NN arithmetic of later calls (0
+    automatic, 1 the direct FP64 kernel only, 2 the FP32 MFMA filter with every
+    query re-run directly; -1 keeps the mode) and return the number of queries
+    the last filtered call re-ran with the direct kernel."""
+    out = C.c_int64(0)
+    st = lib().si_test_knn_path(int(mode), C.byref(out))
+    if st != SI_OK:
+        raise _ERRORS.get(st, SpectralError)(f"status {st}")
+    return int(out.value)

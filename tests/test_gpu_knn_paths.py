@@ -1,0 +1,116 @@
+"""GPU checks of the two exact k-NN paths of si_knn_exact for float32 points
+(spectral-init_amd/csrc/k_graph.hip): the FP32 MFMA filter with the certified
+FP64 re-rank, and the direct FP64 kernel.  Every path must return the direct
+kernel's result bit for bit; the direct kernel itself is pinned against the
+oracle and the reference fixtures (test_gpu_graph.py), and the cases below that
+fit the oracle are checked against it too.
+
+The certificate sends a query to the direct kernel when one of its kept
+candidate lists might have dropped a true neighbour: heavy exact ties,
+coordinates whose squares underflow in float32, and coordinates whose squares
+overflow it (those skip the filter altogether) are the inputs that exercise
+it."""
+import numpy as np
+import pytest
+
+import oracle as O
+import spectral_mi355x as si
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("needs a HIP device")
+    c = si.Context(0)
+    yield c
+    si.knn_path(0)
+    c.close()
+
+
+def run(ctx, x, k, mode):
+    si.knn_path(mode)
+    try:
+        idx, dist = si.knn_exact(ctx, x, k)
+        return idx, dist, si.knn_path(-1)
+    finally:
+        si.knn_path(0)
+
+
+def gmm(n, d, clusters, seed, scale=3.0):
+    rng = np.random.default_rng(seed)
+    centres = rng.standard_normal((clusters, d)) * scale
+    lab = rng.integers(0, clusters, n)
+    return (centres[lab] + rng.standard_normal((n, d))).astype(np.float32)
+
+
+@pytest.mark.parametrize("d", [30, 50])
+def test_filter_equals_direct_kernel_100k(ctx, d):
+    print(f"case knn paths 100k d={d}", flush=True)
+    x = gmm(100_000, d, 20, d)
+    x[17] = x[99_000]          # an exact duplicate pair
+    x[500:520] = x[500]        # twenty copies: ties broken by index
+    i1, d1, _ = run(ctx, x, 15, 1)
+    i0, d0, direct = run(ctx, x, 15, 0)
+    np.testing.assert_array_equal(i0, i1)
+    np.testing.assert_array_equal(d0.view(np.uint32), d1.view(np.uint32))
+    # the certificate holds for nearly every query of well-spread points
+    assert direct < 0.01 * x.shape[0], direct
+    assert i0[17, 0] == 99_000 and i0[99_000, 0] == 17
+    np.testing.assert_array_equal(i0[500, :15], np.arange(501, 516))
+
+
+def test_forced_direct_requeries_every_query(ctx):
+    print("case knn paths forced direct", flush=True)
+    x = gmm(20_000, 17, 8, 5)
+    i1, d1, _ = run(ctx, x, 10, 1)
+    i2, d2, direct = run(ctx, x, 10, 2)
+    assert direct == x.shape[0]
+    np.testing.assert_array_equal(i2, i1)
+    np.testing.assert_array_equal(d2.view(np.uint32), d1.view(np.uint32))
+
+
+@pytest.mark.parametrize("k", [1, 8, 16, 17])
+def test_k_boundary_matches_oracle(ctx, k):
+    print(f"case knn paths k={k}", flush=True)
+    x = gmm(3000, 13, 6, 100 + k)
+    idx, dist, _ = run(ctx, x, k, 0)
+    oi, od = O.knn_exact(x.astype(np.float64), k)
+    np.testing.assert_array_equal(idx, oi)
+    np.testing.assert_allclose(dist, od, rtol=1.2e-7, atol=0)
+
+
+def test_heavy_ties_fall_back_and_match_oracle(ctx):
+    print("case knn paths heavy ties", flush=True)
+    # 100 consecutive copies of each of 40 points: 99 candidates at distance 0,
+    # more than one kept list of 16 can hold, in every query's own range
+    base = gmm(40, 7, 3, 9)
+    x = np.repeat(base, 100, axis=0)
+    idx, dist, direct = run(ctx, x, 15, 0)
+    oi, od = O.knn_exact(x.astype(np.float64), 15)
+    np.testing.assert_array_equal(idx, oi)
+    np.testing.assert_allclose(dist, od, rtol=1.2e-7, atol=0)
+    assert direct > 0
+
+
+@pytest.mark.parametrize("scale", [1e-25, 1e20])
+def test_extreme_scales_match_oracle(ctx, scale):
+    print(f"case knn paths scale={scale}", flush=True)
+    x = (gmm(2000, 9, 4, 3).astype(np.float64) * scale).astype(np.float32)
+    idx, dist, _ = run(ctx, x, 15, 0)
+    oi, od = O.knn_exact(x.astype(np.float64), 15)
+    np.testing.assert_array_equal(idx, oi)
+    np.testing.assert_allclose(dist, od, rtol=1.2e-7, atol=0)
+
+
+def test_outlier_points_match_oracle(ctx):
+    """A few points far from the rest widen every query's error bound."""
+    print("case knn paths outliers", flush=True)
+    x = gmm(4000, 30, 10, 21)
+    x[:5] *= 1e4
+    idx, dist, _ = run(ctx, x, 15, 0)
+    oi, od = O.knn_exact(x.astype(np.float64), 15)
+    np.testing.assert_array_equal(idx, oi)
+    np.testing.assert_allclose(dist, od, rtol=1.2e-7, atol=0)
