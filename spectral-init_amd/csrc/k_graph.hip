@@ -9,6 +9,13 @@
 //                    (fused), top-k kept sorted in registers, ties to the
 //                    smaller index.  Bound by the LDS broadcast reads and the
 //                    FP64 VALU work (2-3 instructions per dimension and pair).
+//                    Float64 points, k > 16, and the queries the filter below
+//                    cannot certify take this kernel.
+//   k_knn_filter,    the same k-NN for float32 points with k <= 16: FP32
+//   k_knn_rerank     matrix-core distances with rigorous error bounds keep a
+//                    short candidate list per query, and FP64 distances of
+//                    the certified candidates (k_knn's arithmetic) rank them;
+//                    the result is k_knn's, bit for bit.
 //   k_smooth_knn     umap smooth_knn_dist, one row per thread (float32 bisection).
 //   k_membership     umap compute_membership_strengths, one entry per thread.
 //   fuzzy union      W = A + A^T - A o A^T: every directed entry is written
