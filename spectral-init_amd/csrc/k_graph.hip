@@ -104,24 +104,30 @@ template <> struct Lds16<double> { using type = double2; };
 // CT is the storage type of the points (float32 inputs stay float32 in HBM
 // and LDS, which halves the LDS bytes of the broadcast reads that bound this
 // kernel; every coordinate is widened to FP64 exactly before it is used).
-// With qlist set, the queries are qlist[0, nq) instead of 0 .. n-1.
+// With qlist set, the queries are qlist[0, nq) instead of 0 .. n-1.  With
+// gridDim.y = S > 1, workgroup (x, y) scans the candidates [y n / S, (y + 1) n
+// / S) and writes its sorted top KP (squared distance, index) to
+// pd/pi[(slot * S + y) * KP ...]; k_knn_merge ranks the S lists.
 template <int DP, int KP, int Q, typename CT>
 __global__ __launch_bounds__(kKnnThreads) void k_knn(const CT* __restrict__ x, int64_t n,
                                                      int k, int32_t* __restrict__ oi,
                                                      float* __restrict__ od,
                                                      const int32_t* __restrict__ qlist,
-                                                     int64_t nq) {
+                                                     int64_t nq, double* __restrict__ pd,
+                                                     int32_t* __restrict__ pi) {
     using V = typename Lds16<CT>::type;
     constexpr int VN = 16 / (int)sizeof(CT);  // coordinates per 16-byte LDS read
     static_assert(DP % VN == 0, "padded dimension must fill 16-byte reads");
     __shared__ CT tile[kKnnTile * DP];
-    int64_t qi[Q];
+    int64_t qi[Q], sl[Q];
     double q[Q][DP];
     double bd[Q][KP];
     int32_t bi[Q][KP];
+    const int64_t cb = n * blockIdx.y / gridDim.y, ce = n * (blockIdx.y + 1) / gridDim.y;
 #pragma unroll
     for (int u = 0; u < Q; ++u) {
         const int64_t slot = ((int64_t)blockIdx.x * Q + u) * kKnnThreads + threadIdx.x;
+        sl[u] = slot;
         qi[u] = slot < nq ? (qlist ? (int64_t)qlist[slot] : slot) : n;
         const bool ok = qi[u] < n;
 #pragma unroll
@@ -129,8 +135,8 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn(const CT* __restrict__ x, i
 #pragma unroll
         for (int p = 0; p < KP; ++p) { bd[u][p] = INFINITY; bi[u][p] = -1; }
     }
-    for (int64_t c0 = 0; c0 < n; c0 += kKnnTile) {
-        const int cnt = (int)((n - c0) < kKnnTile ? (n - c0) : kKnnTile);
+    for (int64_t c0 = cb; c0 < ce; c0 += kKnnTile) {
+        const int cnt = (int)((ce - c0) < kKnnTile ? (ce - c0) : kKnnTile);
         __syncthreads();
         for (int e = threadIdx.x; e < kKnnTile * DP; e += kKnnThreads)
             tile[e] = e < cnt * DP ? x[c0 * DP + e] : CT(0);
@@ -162,6 +168,15 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn(const CT* __restrict__ x, i
 #pragma unroll
     for (int u = 0; u < Q; ++u) {
         if (qi[u] >= n) continue;
+        if (gridDim.y > 1) {
+            const int64_t o = (sl[u] * gridDim.y + blockIdx.y) * KP;
+#pragma unroll
+            for (int p = 0; p < KP; ++p) {
+                pd[o + p] = bd[u][p];
+                pi[o + p] = bi[u][p];
+            }
+            continue;
+        }
 #pragma unroll
         for (int p = 0; p < KP; ++p) {
             if (p < k) {
@@ -172,8 +187,39 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn(const CT* __restrict__ x, i
     }
 }
 
+// First k of the S ascending (squared distance, index) lists of each query
+// slot, ordered by distance, then index: the single-pass result, whose strict
+// insertions in index order give that same order.
+__global__ void k_knn_merge(const double* __restrict__ pd, const int32_t* __restrict__ pi,
+                            int splits, int kp, int64_t n, int k,
+                            const int32_t* __restrict__ qlist, int64_t nq,
+                            int32_t* __restrict__ oi, float* __restrict__ od) {
+    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= nq) return;
+    const int64_t q = qlist ? (int64_t)qlist[slot] : slot;
+    const int64_t base = slot * splits * kp;
+    int h[16];
+    for (int g = 0; g < splits; ++g) h[g] = 0;
+    for (int p = 0; p < k; ++p) {
+        int gb = -1;
+        double sb = INFINITY;
+        int32_t jb = -1;
+        for (int g = 0; g < splits; ++g) {
+            if (h[g] >= kp) continue;
+            const int64_t e = base + (int64_t)g * kp + h[g];
+            const int32_t j = pi[e];
+            if (j < 0) continue;
+            const double sv = pd[e];
+            if (gb < 0 || sv < sb || (sv == sb && j < jb)) { gb = g; sb = sv; jb = j; }
+        }
+        if (gb >= 0) ++h[gb];
+        oi[q * k + p] = jb;
+        od[q * k + p] = jb >= 0 ? (float)sqrt(sb) : INFINITY;
+    }
+}
+
 // ------------------------------------------------------------ MFMA filter
-// Float32 points with k <= kFiltL take two stages that give exactly k_knn's
+// Float32 points with k <= kFiltK take two stages that give exactly k_knn's
 // result:
 //   1. k_knn_filter: a = |q'|^2 + |c'|^2 - 2 q'.c' on the FP32 matrix cores
 //      (v_mfma_f32_16x16x4_f32) for the centred, float-rounded points q', c',
@@ -196,6 +242,7 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 // 16-query tiles per wavefront: 4, or 2 above 48 coordinates (registers)
 template <int KS> constexpr int filt_tiles() { return KS > 12 ? 2 : 4; }
+constexpr int kFiltK = 16;                     // largest k of the filtered path
 constexpr int kFiltL = 16;                     // kept candidates per lane and query
 constexpr int kFiltTile = 256;                 // candidates per LDS tile
 constexpr int kFiltGroups = 4;                 // lanes sharing one query (one per quarter)
@@ -421,16 +468,16 @@ __global__ __launch_bounds__(256) void k_knn_rerank(const float* __restrict__ xp
     const int64_t base = (q - q0) * slots;
     const double nq = (double)nrm[q];
     // U_k: the k-th smallest upper bound u = l + 2E over the kept candidates
-    double ub[kFiltL];
+    double ub[kFiltK];
 #pragma unroll
-    for (int p = 0; p < kFiltL; ++p) ub[p] = INFINITY;
+    for (int p = 0; p < kFiltK; ++p) ub[p] = INFINITY;
     for (int e = 0; e < slots; ++e) {
         const int32_t j = kidx[base + e];
         if (j < 0 || j == q) continue;
         const double u = (double)kd2[base + e] + 2.0 * (2.0 * alpha * (nq + (double)nrm[j]) + eabs);
-        if (!(u < ub[kFiltL - 1])) continue;
+        if (!(u < ub[kFiltK - 1])) continue;
 #pragma unroll
-        for (int p = kFiltL - 1; p > 0; --p) {
+        for (int p = kFiltK - 1; p > 0; --p) {
             const bool mv = u < ub[p - 1];
             const bool here = !mv && u < ub[p];
             ub[p] = mv ? ub[p - 1] : (here ? u : ub[p]);
@@ -439,7 +486,7 @@ __global__ __launch_bounds__(256) void k_knn_rerank(const float* __restrict__ xp
     }
     double uk = INFINITY;
 #pragma unroll
-    for (int p = 0; p < kFiltL; ++p)
+    for (int p = 0; p < kFiltK; ++p)
         if (p == k - 1) uk = ub[p];
     bool direct = force_direct != 0;
     for (int g = 0; g < lists; ++g) {
@@ -451,10 +498,10 @@ __global__ __launch_bounds__(256) void k_knn_rerank(const float* __restrict__ xp
         return;
     }
     const float* qx = xp + q * DP;
-    double bs[kFiltL];
-    int32_t bj[kFiltL];
+    double bs[kFiltK];
+    int32_t bj[kFiltK];
 #pragma unroll
-    for (int p = 0; p < kFiltL; ++p) { bs[p] = INFINITY; bj[p] = -1; }
+    for (int p = 0; p < kFiltK; ++p) { bs[p] = INFINITY; bj[p] = -1; }
     for (int e = 0; e < slots; ++e) {
         const int32_t j = kidx[base + e];
         if (j < 0 || j == q || (double)kd2[base + e] > uk) continue;
@@ -464,9 +511,9 @@ __global__ __launch_bounds__(256) void k_knn_rerank(const float* __restrict__ xp
             const double df = (double)qx[t] - (double)xp[(int64_t)j * DP + t];
             s = fma(df, df, s);
         }
-        if (!rank_before(s, j, bs[kFiltL - 1], bj[kFiltL - 1])) continue;
+        if (!rank_before(s, j, bs[kFiltK - 1], bj[kFiltK - 1])) continue;
 #pragma unroll
-        for (int p = kFiltL - 1; p > 0; --p) {
+        for (int p = kFiltK - 1; p > 0; --p) {
             const bool mv = rank_before(s, j, bs[p - 1], bj[p - 1]);
             const bool here = !mv && rank_before(s, j, bs[p], bj[p]);
             bs[p] = mv ? bs[p - 1] : (here ? s : bs[p]);
@@ -475,7 +522,7 @@ __global__ __launch_bounds__(256) void k_knn_rerank(const float* __restrict__ xp
         if (rank_before(s, j, bs[0], bj[0])) { bs[0] = s; bj[0] = j; }
     }
 #pragma unroll
-    for (int p = 0; p < kFiltL; ++p) {
+    for (int p = 0; p < kFiltK; ++p) {
         if (p < k) {
             oi[q * k + p] = bj[p];
             od[q * k + p] = bj[p] >= 0 ? (float)sqrt(bs[p]) : INFINITY;
@@ -669,9 +716,25 @@ void launch_knn_q(const CT* x, int64_t n, int k, int32_t* oi, float* od, hipStre
     // registers allow it (DP <= 16, k <= 16; above, occupancy drops to one wave)
     constexpr int Q = (DP <= 16 && KP <= 16) ? 2 : 1;
     const int64_t blocks = ceil_div(nq, (int64_t)kKnnThreads * Q);
-    hipLaunchKernelGGL((k_knn<DP, KP, Q, CT>), dim3(blocks), dim3(kKnnThreads), 0, s, x, n, k,
-                       oi, od, qlist, nq);
+    // few queries (a small input, or the queries a filter did not certify):
+    // split the candidates so that about 2048 workgroups share the scan
+    const int splits =
+        (int)std::max<int64_t>(1, std::min<int64_t>(16, ceil_div((int64_t)2048, blocks)));
+    if (splits == 1) {
+        hipLaunchKernelGGL((k_knn<DP, KP, Q, CT>), dim3(blocks), dim3(kKnnThreads), 0, s, x, n,
+                           k, oi, od, qlist, nq, nullptr, nullptr);
+        SI_LAUNCH_CHECK();
+        return;
+    }
+    const int64_t slots = blocks * kKnnThreads * Q;
+    DevBuf<double> pd((size_t)slots * splits * KP);
+    DevBuf<int32_t> pi((size_t)slots * splits * KP);
+    hipLaunchKernelGGL((k_knn<DP, KP, Q, CT>), dim3(blocks, splits), dim3(kKnnThreads), 0, s, x,
+                       n, k, oi, od, qlist, nq, pd.get(), pi.get());
+    hipLaunchKernelGGL(k_knn_merge, dim3(ceil_div(nq, (int64_t)256)), dim3(256), 0, s, pd.get(),
+                       pi.get(), splits, KP, n, k, qlist, nq, oi, od);
     SI_LAUNCH_CHECK();
+    SI_HIP(hipStreamSynchronize(s));  // pd, pi are freed on return
 }
 
 template <int DP, typename CT>
@@ -684,7 +747,7 @@ void launch_knn_k(const CT* x, int64_t n, int k, int32_t* oi, float* od, hipStre
 }
 
 // The two-stage path for float32 points padded to DP = 4 KS coordinates
-// (k <= kFiltL).  Returns false, having written nothing, when a squared norm
+// (k <= kFiltK).  Returns false, having written nothing, when a squared norm
 // of the centred points is not finite or so large that the FP32 sums could
 // overflow; the caller then runs the direct kernel.
 template <int KS>
@@ -777,7 +840,7 @@ void knn_typed(const CT* pts, int64_t n, int64_t d, int64_t k, int32_t* idx, flo
                        (int)d, dp, xp.get());
     SI_LAUNCH_CHECK();
     if constexpr (sizeof(CT) == 4) {
-        if (g_knn_path.load() != 1 && k <= kFiltL &&
+        if (g_knn_path.load() != 1 && k <= kFiltK &&
             knn_filtered_dp(dp, xp.get(), n, (int)k, idx, dist, s))
             return;
     }
