@@ -236,7 +236,7 @@ si_status si_test_rcm_order(int64_t n, const int64_t* indptr, const int32_t* ind
 }
 
 si_status si_test_knn_path(int32_t mode, int64_t* last_direct) {
-    if (mode < -1 || mode > 2) return SI_ERR_INVALID_ARGUMENT;
+    if (mode < -1 || mode > 3) return SI_ERR_INVALID_ARGUMENT;
     if (mode >= 0) si::knn_set_path(mode);
     if (last_direct) *last_direct = si::knn_direct_count();
     return SI_OK;

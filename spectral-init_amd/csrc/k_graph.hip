@@ -446,6 +446,193 @@ __global__ __launch_bounds__(256) void k_knn_filter(const float* __restrict__ xc
     }
 }
 
+// ------------------------------------------------------------ bf16x3 filter
+// The same stage 1 on the bf16 matrix cores (v_mfma_f32_16x16x32_bf16): every
+// FP32 value v is split into bf16 parts h = bf16(v), l = bf16(v - h), and a
+// product x y is taken as hx hy + hx ly + lx hy (each part product exact in
+// the FP32 accumulator; the dropped terms are below 3.02 2^-16 |x y|).  The
+// squared norms ride in two spare k-slots: candidate rows are
+// [c', nc, 1, 0 ...] and query rows [-2 q', 1, nq, 0 ...] with nc, nq the
+// shrunk norms, so the MFMA output is the lower bound l itself and the
+// epilogue is one compare per value.  Rows are DB = 32 KB wide (d + 2 <= DB).
+
+// FP32 -> bf16 bits, round to nearest even (finite inputs)
+__device__ inline uint16_t bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ inline void split_bf16(float v, uint16_t& h, uint16_t& l) {
+    h = bf16_rne(v);
+    const float hv = __uint_as_float((uint32_t)h << 16);
+    l = bf16_rne(v - hv);  // v - hv is exact in FP32
+}
+
+// Candidate rows (ah, al) and query rows (bh, bl) [n x DB] of the centred
+// points xc [n x dp] and their norms: nc = shrink |c'|^2, nq = shrink |q'|^2 - eabs.
+__global__ void k_split_rows(const float* __restrict__ xc, const float* __restrict__ nrm,
+                             int64_t n, int d, int dp, int db, float shrink, float eabs,
+                             uint16_t* __restrict__ ah, uint16_t* __restrict__ al,
+                             uint16_t* __restrict__ bh, uint16_t* __restrict__ bl) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * db) return;
+    const int64_t i = e / db;
+    const int t = (int)(e - i * db);
+    float a = 0.0f, b = 0.0f;
+    if (t < d) {
+        a = xc[i * dp + t];
+        b = -2.0f * a;
+    } else if (t == d) {
+        a = nrm[i] * shrink;
+        b = 1.0f;
+    } else if (t == d + 1) {
+        a = 1.0f;
+        b = nrm[i] * shrink - eabs;
+    }
+    split_bf16(a, ah[e], al[e]);
+    split_bf16(b, bh[e], bl[e]);
+}
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef uint16_t v8u16 __attribute__((ext_vector_type(8)));
+
+template <int KB>
+__global__ __launch_bounds__(256) void k_knn_filter3(
+    const uint16_t* __restrict__ ah, const uint16_t* __restrict__ al,
+    const uint16_t* __restrict__ bh, const uint16_t* __restrict__ bl, int64_t n, int d,
+    int64_t q0, int64_t q1, int32_t* __restrict__ kidx, float* __restrict__ kd2,
+    int32_t* __restrict__ layout_err) {
+    constexpr int DB = 32 * KB;
+    constexpr int DS = DB + 8;       // LDS row stride (bf16): spreads the 16-byte reads over the banks
+    constexpr int R = KB == 1 ? 4 : 2;  // 16-query tiles per wavefront (registers)
+    constexpr float kEmpty = 1e30f;  // list sentinel, above every real bound
+    __shared__ __align__(16) uint16_t th[kFiltTile * DS], tl[kFiltTile * DS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lq = lane & 15, lg = lane >> 4;
+
+    // output layout probe, as in k_knn_filter, with this instruction
+    const v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    v8bf pa, pb, pc1, pc2;
+    for (int e = 0; e < 8; ++e) {
+        const bool first = lg == 0 && e == 0;
+        pa[e] = (__bf16)(first ? (float)lq : 0.0f);
+        pb[e] = (__bf16)(first ? 1.0f : 0.0f);
+        pc1[e] = (__bf16)(first ? 1.0f : 0.0f);
+        pc2[e] = (__bf16)(first ? (float)lq : 0.0f);
+    }
+    const v4f pr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(pa, pb, zero, 0, 0, 0);
+    const v4f pcol = __builtin_amdgcn_mfma_f32_16x16x32_bf16(pc1, pc2, zero, 0, 0, 0);
+    int rowof[4];
+    bool bad = false;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        rowof[v] = (int)pr[v] & 15;
+        bad = bad || (int)pcol[v] != lq || (float)rowof[v] != pr[v];
+    }
+    if (bad) *layout_err = 1;
+
+    const int64_t qb = q0 + ((int64_t)blockIdx.x * 4 + wave) * (16 * R);
+    const int64_t cb = n * blockIdx.y / gridDim.y, ce = n * (blockIdx.y + 1) / gridDim.y;
+    v8bf qh[R][KB], ql[R][KB];
+    int32_t qi[R];
+    float bd[R][kFiltL];
+    int32_t bi[R][kFiltL];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int64_t q = qb + 16 * r + lq;
+        const bool ok = q < q1;
+        qi[r] = ok ? (int32_t)q : -1;
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) {
+            v8u16 h = {0, 0, 0, 0, 0, 0, 0, 0}, l = h;
+            if (ok) {
+                h = *reinterpret_cast<const v8u16*>(bh + q * DB + 32 * kb + 8 * lg);
+                l = *reinterpret_cast<const v8u16*>(bl + q * DB + 32 * kb + 8 * lg);
+            }
+            qh[r][kb] = __builtin_bit_cast(v8bf, h);
+            ql[r][kb] = __builtin_bit_cast(v8bf, l);
+        }
+#pragma unroll
+        for (int p = 0; p < kFiltL; ++p) { bd[r][p] = kEmpty; bi[r][p] = -1; }
+    }
+    // a row past the end: 2e30 in the norm slot (bf16 parts of 2e30), so its
+    // bound exceeds the list sentinel and it is never kept
+    const float pad = 2e30f;
+    const uint16_t pad_h = bf16_rne(pad);
+    const uint16_t pad_l = bf16_rne(pad - __uint_as_float((uint32_t)pad_h << 16));
+
+    auto dots = [&](int m, v4f (&acc)[R]) {
+        v8bf a_h[KB], a_l[KB];
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) {
+            const int o = (16 * m + lq) * DS + 32 * kb + 8 * lg;
+            a_h[kb] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v8u16*>(&th[o]));
+            a_l[kb] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v8u16*>(&tl[o]));
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = zero;
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_l[kb], qh[r][kb], acc[r], 0, 0, 0);
+                acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h[kb], ql[r][kb], acc[r], 0, 0, 0);
+                acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h[kb], qh[r][kb], acc[r], 0, 0, 0);
+            }
+    };
+    auto keep = [&](int m, int64_t c0, const v4f (&acc)[R]) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const float lo = acc[r][v];
+                if (lo < bd[r][kFiltL - 1])
+                    topk_insert<kFiltL>(bd[r], bi[r], lo, (int32_t)(c0 + 16 * m + rowof[v]));
+            }
+    };
+    for (int64_t c0 = cb; c0 < ce; c0 += kFiltTile) {
+        const int cnt = (int)((ce - c0) < kFiltTile ? (ce - c0) : kFiltTile);
+        __syncthreads();
+        for (int e = threadIdx.x; e < kFiltTile * (DB / 8); e += 256) {
+            const int row = e / (DB / 8), c8 = e - row * (DB / 8);
+            v8u16 h = {0, 0, 0, 0, 0, 0, 0, 0}, l = h;
+            if (row < cnt) {
+                h = *reinterpret_cast<const v8u16*>(ah + (c0 + row) * DB + 8 * c8);
+                l = *reinterpret_cast<const v8u16*>(al + (c0 + row) * DB + 8 * c8);
+            } else if (c8 == d / 8) {
+                h[d % 8] = pad_h;
+                l[d % 8] = pad_l;
+            }
+            *reinterpret_cast<v8u16*>(&th[row * DS + 8 * c8]) = h;
+            *reinterpret_cast<v8u16*>(&tl[row * DS + 8 * c8]) = l;
+        }
+        __syncthreads();
+        const int subs = (cnt + 15) / 16;
+        v4f acc[R];
+        dots(0, acc);
+        for (int m = 0; m + 1 < subs; ++m) {
+            v4f nxt[R];
+            dots(m + 1, nxt);
+            keep(m, c0, acc);
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] = nxt[r];
+        }
+        keep(subs - 1, c0, acc);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (qi[r] < 0) continue;
+        const int64_t o =
+            (((int64_t)(qi[r] - q0) * gridDim.y + blockIdx.y) * kFiltGroups + lg) * kFiltL;
+#pragma unroll
+        for (int p = 0; p < kFiltL; ++p) {
+            kidx[o + p] = bi[r][p];
+            kd2[o + p] = bi[r][p] >= 0 ? bd[r][p] : INFINITY;
+        }
+    }
+}
+
 // (s, j) before (bs, bj): smaller distance, then smaller index.
 __device__ inline bool rank_before(double s, int32_t j, double bs, int32_t bj) {
     return s < bs || (s == bs && j < bj);
@@ -704,7 +891,8 @@ void upload_exp_table() {
 }
 
 // k-NN path of later calls (testing seam): 0 automatic, 1 the direct FP64
-// kernel only, 2 the filter with every query sent to the direct kernel.
+// kernel only, 2 the filter with every query sent to the direct kernel, 3 the
+// FP32 filter instead of the bf16x3 one.
 std::atomic<int> g_knn_path{0};
 std::atomic<int64_t> g_knn_direct{0};  // queries of the last filtered call re-run directly
 
@@ -746,12 +934,16 @@ void launch_knn_k(const CT* x, int64_t n, int k, int32_t* oi, float* od, hipStre
     else launch_knn_q<DP, 64>(x, n, k, oi, od, s, qlist, nq);
 }
 
-// The two-stage path for float32 points padded to DP = 4 KS coordinates
-// (k <= kFiltK).  Returns false, having written nothing, when a squared norm
-// of the centred points is not finite or so large that the FP32 sums could
-// overflow; the caller then runs the direct kernel.
+// The two-stage path for float32 points of dimension d padded to DP = 4 KS
+// coordinates (k <= kFiltK).  Stage 1 runs on the bf16 matrix cores (three
+// bf16 products per FP32 product) when the squared norms of the centred
+// points are at most 1e20 and d + 2 <= 96; if that leaves more than 1 % of the
+// queries uncertified, or otherwise, the FP32 matrix cores run it.  Returns
+// false, having written nothing, when a squared norm is not finite or above
+// 1e36; the caller then runs the direct kernel.
 template <int KS>
-bool knn_filtered(const float* xp, int64_t n, int k, int32_t* oi, float* od, hipStream_t s) {
+bool knn_filtered(const float* xp, int64_t n, int d, int k, int32_t* oi, float* od,
+                  hipStream_t s) {
     constexpr int DP = 4 * KS;
     DevBuf<double> part((size_t)kColParts * DP), mean(DP);
     hipLaunchKernelGGL(k_col_partials, dim3(kColParts), dim3(64), 0, s, xp, n, DP, part.get());
@@ -770,52 +962,100 @@ bool knn_filtered(const float* xp, int64_t n, int k, int32_t* oi, float* od, hip
     float m2 = 0.0f;
     std::memcpy(&m2, &mbits, sizeof(float));
     if (!(m2 <= 1e36f)) return false;  // also false for NaN
-    // |a - d^2| <= (DP + 12) u (|q'|^2 + |c'|^2) + eabs from the FP32 rounding
-    // of the centred coordinates, the norms, the MFMA sums and the epilogue
-    // (with (x + y)^2 <= 2 (x^2 + y^2)); 2 alpha = 2 (DP + 16) u leaves room
-    // for the rounding of l itself.  eabs covers flushed subnormal products.
+    const int mode = g_knn_path.load();
+    const int kb = (d + 2 + 31) / 32;
+    const bool bf16x3 = mode != 3 && m2 <= 1e20f && kb <= 3;
     const double u = std::ldexp(1.0, -24);
-    const double alpha = (double)(DP + 16) * u;
-    const double eabs = (double)DP * 1e-30;
+    // FP32 stage 1: |a - d^2| <= (DP + 12) u (|q'|^2 + |c'|^2) + eabs from the
+    // FP32 rounding of the centred coordinates, the norms, the MFMA sums and
+    // the epilogue (with (x + y)^2 <= 2 (x^2 + y^2)); 2 alpha = 2 (DP + 16) u
+    // leaves room for the rounding of l itself.  eabs covers flushed subnormal
+    // products.
+    const double alpha32 = (double)(DP + 16) * u;
+    // bf16x3 stage 1: the dropped part products (3.02 2^-16 |x y| per
+    // coordinate, 2^-16 of each norm slot), the FP32 sums of 3 DB terms and
+    // the rounding of the centred coordinates and norms give
+    // (4.02 2^-16 + (6 DB + 10) 2^-24) (|q'|^2 + |c'|^2); 2 alpha doubles it.
+    const int db = 32 * kb;
+    const double alpha3 = 4.02 * std::ldexp(1.0, -16) + (6.0 * db + 10.0) * u;
+    const double eabs = 96.0 * 1e-30;
+    const int qpb = bf16x3 ? 4 * 16 * (kb == 1 ? 4 : 2) : filt_queries<KS>();
     // candidate splits: about 2048 workgroups, at most 16 splits
-    const int64_t qblocks = ceil_div(n, (int64_t)filt_queries<KS>());
-    const int splits = (int)std::max<int64_t>(1, std::min<int64_t>(16, ceil_div((int64_t)2048, qblocks)));
+    const int64_t qblocks = ceil_div(n, (int64_t)qpb);
+    const int splits =
+        (int)std::max<int64_t>(1, std::min<int64_t>(16, ceil_div((int64_t)2048, qblocks)));
     const int lists = kFiltGroups * splits;
     // query chunks keep the kept lists within 4 GiB
     const int64_t chunk = std::min<int64_t>(n, ((int64_t)1 << 29) / (lists * kFiltL));
     DevBuf<int32_t> kidx((size_t)chunk * lists * kFiltL), qlist(n);
     DevBuf<float> kd2((size_t)chunk * lists * kFiltL);
-    const int force = g_knn_path.load() == 2 ? 1 : 0;
-    for (int64_t q0 = 0; q0 < n; q0 += chunk) {
-        const int64_t q1 = std::min(n, q0 + chunk);
-        hipLaunchKernelGGL((k_knn_filter<KS>),
-                           dim3(ceil_div(q1 - q0, (int64_t)filt_queries<KS>()), splits), dim3(256), 0, s, xc.get(), nrm.get(), n, q0, q1,
-                           (float)(1.0 - 2.0 * alpha), (float)eabs, kidx.get(), kd2.get(),
-                           ctl.get() + 1);
-        hipLaunchKernelGGL((k_knn_rerank<DP>), dim3(ceil_div(q1 - q0, 256)), dim3(256), 0, s, xp,
-                           nrm.get(), q0, q1, k, lists, alpha, eabs, kidx.get(), kd2.get(), force,
-                           oi, od, qlist.get(), ctl.get() + 2);
+    DevBuf<uint16_t> ah, al, bh, bl;
+    if (bf16x3) {
+        const size_t rows = (size_t)n * db;
+        ah.alloc(rows); al.alloc(rows); bh.alloc(rows); bl.alloc(rows);
+        hipLaunchKernelGGL(k_split_rows, dim3(ceil_div((int64_t)rows, (int64_t)256)), dim3(256), 0,
+                           s, xc.get(), nrm.get(), n, d, DP, db, (float)(1.0 - 2.0 * alpha3),
+                           (float)eabs, ah.get(), al.get(), bh.get(), bl.get());
         SI_LAUNCH_CHECK();
     }
-    int32_t h[2] = {0, 0};
-    SI_HIP(hipMemcpyAsync(h, ctl.get() + 1, sizeof(h), hipMemcpyDeviceToHost, s));
-    SI_HIP(hipStreamSynchronize(s));
-    SI_CHECK(h[0] == 0, kErrInternal, "knn: unexpected MFMA output layout");
-    g_knn_direct.store(h[1]);
-    if (h[1] > 0) launch_knn_k<DP, float>(xp, n, k, oi, od, s, qlist.get(), h[1]);
+    const int force = mode == 2 ? 1 : 0;
+    auto pass = [&](bool three) -> int32_t {
+        SI_HIP(hipMemsetAsync(ctl.get() + 2, 0, sizeof(int32_t), s));
+        const double alpha = three ? alpha3 : alpha32;
+        for (int64_t q0 = 0; q0 < n; q0 += chunk) {
+            const int64_t q1 = std::min(n, q0 + chunk);
+            const dim3 grid((unsigned)ceil_div(q1 - q0, (int64_t)qpb), (unsigned)splits);
+            if (three) {
+                switch (kb) {
+                    case 1:
+                        hipLaunchKernelGGL((k_knn_filter3<1>), grid, dim3(256), 0, s, ah.get(),
+                                           al.get(), bh.get(), bl.get(), n, d, q0, q1, kidx.get(),
+                                           kd2.get(), ctl.get() + 1);
+                        break;
+                    case 2:
+                        hipLaunchKernelGGL((k_knn_filter3<2>), grid, dim3(256), 0, s, ah.get(),
+                                           al.get(), bh.get(), bl.get(), n, d, q0, q1, kidx.get(),
+                                           kd2.get(), ctl.get() + 1);
+                        break;
+                    default:
+                        hipLaunchKernelGGL((k_knn_filter3<3>), grid, dim3(256), 0, s, ah.get(),
+                                           al.get(), bh.get(), bl.get(), n, d, q0, q1, kidx.get(),
+                                           kd2.get(), ctl.get() + 1);
+                        break;
+                }
+            } else {
+                hipLaunchKernelGGL((k_knn_filter<KS>), grid, dim3(256), 0, s, xc.get(), nrm.get(),
+                                   n, q0, q1, (float)(1.0 - 2.0 * alpha), (float)eabs,
+                                   kidx.get(), kd2.get(), ctl.get() + 1);
+            }
+            hipLaunchKernelGGL((k_knn_rerank<DP>), dim3(ceil_div(q1 - q0, 256)), dim3(256), 0, s,
+                               xp, nrm.get(), q0, q1, k, lists, alpha, eabs, kidx.get(),
+                               kd2.get(), force, oi, od, qlist.get(), ctl.get() + 2);
+            SI_LAUNCH_CHECK();
+        }
+        int32_t h[2] = {0, 0};
+        SI_HIP(hipMemcpyAsync(h, ctl.get() + 1, sizeof(h), hipMemcpyDeviceToHost, s));
+        SI_HIP(hipStreamSynchronize(s));
+        SI_CHECK(h[0] == 0, kErrInternal, "knn: unexpected MFMA output layout");
+        return h[1];
+    };
+    int32_t direct = pass(bf16x3);
+    if (bf16x3 && !force && direct > n / 100) direct = pass(false);
+    g_knn_direct.store(direct);
+    if (direct > 0) launch_knn_k<DP, float>(xp, n, k, oi, od, s, qlist.get(), direct);
     SI_HIP(hipStreamSynchronize(s));  // the buffers are freed on return
     return true;
 }
 
-bool knn_filtered_dp(int dp, const float* xp, int64_t n, int k, int32_t* oi, float* od,
+bool knn_filtered_dp(int dp, const float* xp, int64_t n, int d, int k, int32_t* oi, float* od,
                      hipStream_t s) {
     switch (dp) {
-        case 4: return knn_filtered<1>(xp, n, k, oi, od, s);
-        case 8: return knn_filtered<2>(xp, n, k, oi, od, s);
-        case 16: return knn_filtered<4>(xp, n, k, oi, od, s);
-        case 32: return knn_filtered<8>(xp, n, k, oi, od, s);
-        case 48: return knn_filtered<12>(xp, n, k, oi, od, s);
-        default: return knn_filtered<16>(xp, n, k, oi, od, s);
+        case 4: return knn_filtered<1>(xp, n, d, k, oi, od, s);
+        case 8: return knn_filtered<2>(xp, n, d, k, oi, od, s);
+        case 16: return knn_filtered<4>(xp, n, d, k, oi, od, s);
+        case 32: return knn_filtered<8>(xp, n, d, k, oi, od, s);
+        case 48: return knn_filtered<12>(xp, n, d, k, oi, od, s);
+        default: return knn_filtered<16>(xp, n, d, k, oi, od, s);
     }
 }
 
@@ -841,7 +1081,7 @@ void knn_typed(const CT* pts, int64_t n, int64_t d, int64_t k, int32_t* idx, flo
     SI_LAUNCH_CHECK();
     if constexpr (sizeof(CT) == 4) {
         if (g_knn_path.load() != 1 && k <= kFiltK &&
-            knn_filtered_dp(dp, xp.get(), n, (int)k, idx, dist, s))
+            knn_filtered_dp(dp, xp.get(), n, (int)d, (int)k, idx, dist, s))
             return;
     }
     const CT* x = xp.get();

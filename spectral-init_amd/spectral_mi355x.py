@@ -752,9 +752,10 @@ def rcm_order(indptr, indices, threads: int = 0) -> np.ndarray:
 
 def knn_path(mode: int = -1) -> int:
     """Testing seam: select the exact k-NN arithmetic of later calls (0
-    automatic, 1 the direct FP64 kernel only, 2 the FP32 MFMA filter with every
-    query re-run directly; -1 keeps the mode) and return the number of queries
-    the last filtered call re-ran with the direct kernel."""
+    automatic, 1 the direct FP64 kernel only, 2 the matrix-core filter with
+    every query re-run directly, 3 the filter on the FP32 matrix cores only; -1
+    keeps the mode) and return the number of queries the last filtered call
+    re-ran with the direct kernel."""
     out = C.c_int64(0)
     st = lib().si_test_knn_path(int(mode), C.byref(out))
     if st != SI_OK:

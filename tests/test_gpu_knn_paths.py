@@ -5,7 +5,10 @@ kernel's result bit for bit; the direct kernel itself is pinned against the
 oracle and the reference fixtures (test_gpu_graph.py), and the cases below that
 fit the oracle are checked against it too.
 
-The certificate sends a query to the direct kernel when one of its kept
+Stage 1 runs on the bf16 matrix cores (three bf16 products per FP32 product)
+and falls back to the FP32 matrix cores when that leaves more than 1 % of the
+queries uncertified (or the norms are too large for it); mode 3 forces the FP32
+stage.  The certificate sends a query to the direct kernel when one of its kept
 candidate lists might have dropped a true neighbour: heavy exact ties,
 coordinates whose squares underflow in float32, and coordinates whose squares
 overflow it (those skip the filter altogether) are the inputs that exercise
@@ -58,6 +61,10 @@ def test_filter_equals_direct_kernel_100k(ctx, d):
     np.testing.assert_array_equal(d0.view(np.uint32), d1.view(np.uint32))
     # the certificate holds for nearly every query of well-spread points
     assert direct < 0.01 * x.shape[0], direct
+    i3, d3, direct3 = run(ctx, x, 15, 3)     # the FP32 stage 1
+    np.testing.assert_array_equal(i3, i1)
+    np.testing.assert_array_equal(d3.view(np.uint32), d1.view(np.uint32))
+    assert direct3 < 0.01 * x.shape[0], direct3
     assert i0[17, 0] == 99_000 and i0[99_000, 0] == 17
     np.testing.assert_array_equal(i0[500, :15], np.arange(501, 516))
 
@@ -114,3 +121,20 @@ def test_outlier_points_match_oracle(ctx):
     oi, od = O.knn_exact(x.astype(np.float64), 15)
     np.testing.assert_array_equal(idx, oi)
     np.testing.assert_allclose(dist, od, rtol=1.2e-7, atol=0)
+
+
+def test_far_clusters_take_the_fp32_stage_and_match_oracle(ctx):
+    """Two tight clusters 2000 units apart: the bf16 parts cannot resolve the
+    neighbours against |q'|^2 ~ 1e6, so the FP32 stage (and the direct kernel
+    for what it leaves) must produce the result."""
+    print("case knn paths far clusters", flush=True)
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((3000, 8))
+    x[:1500, 0] += 1000.0
+    x[1500:, 0] -= 1000.0
+    x = x.astype(np.float32)
+    oi, od = O.knn_exact(x.astype(np.float64), 15)
+    for mode in (0, 3):
+        idx, dist, _ = run(ctx, x, 15, mode)
+        np.testing.assert_array_equal(idx, oi)
+        np.testing.assert_allclose(dist, od, rtol=1.2e-7, atol=0)
