@@ -399,13 +399,22 @@ __global__ __launch_bounds__(256) void k_knn_filter(const float* __restrict__ xc
             cn[v] = tn[row];
             cj[v] = (int32_t)(c0 + row);  // rows past the end have l = +inf
         }
+        float lo[kFiltR][4];
+        bool any = false;
 #pragma unroll
         for (int r = 0; r < kFiltR; ++r)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const float lo = fmaf(-2.0f, acc[r][v], nq[r] + cn[v]);
-                if (lo < bd[r][kFiltL - 1]) topk_insert<kFiltL>(bd[r], bi[r], lo, cj[v]);
+                lo[r][v] = fmaf(-2.0f, acc[r][v], nq[r] + cn[v]);
+                any = any || lo[r][v] < bd[r][kFiltL - 1];
             }
+        if (!any) return;  // one branch for the common case
+#pragma unroll
+        for (int r = 0; r < kFiltR; ++r)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                if (lo[r][v] < bd[r][kFiltL - 1])
+                    topk_insert<kFiltL>(bd[r], bi[r], lo[r][v], cj[v]);
     };
     for (int64_t c0 = cb; c0 < ce; c0 += kFiltTile) {
         const int cnt = (int)((ce - c0) < kFiltTile ? (ce - c0) : kFiltTile);
@@ -581,7 +590,15 @@ __global__ __launch_bounds__(256) void k_knn_filter3(
                 acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h[kb], qh[r][kb], acc[r], 0, 0, 0);
             }
     };
+    // one test of all 16 values first: late in the scan almost no lane
+    // inserts, and the per-value branches are then skipped with one
     auto keep = [&](int m, int64_t c0, const v4f (&acc)[R]) {
+        bool any = false;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) any = any || acc[r][v] < bd[r][kFiltL - 1];
+        if (!any) return;
 #pragma unroll
         for (int r = 0; r < R; ++r)
 #pragma unroll
