@@ -27,7 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.join(ROOT, "spectral-init_amd"), os.path.join(ROOT, "oracle")]
 
 HBM_PEAK_GBS = 8000.0
-FP32_MFMA_PEAK = 157.3  # TFLOP/s, dense FP32 on the MI355X matrix cores
+BF16_MFMA_PEAK = 2500.0  # TFLOP/s, dense bf16 on the MI355X matrix cores
 
 
 def log(msg):
@@ -98,16 +98,22 @@ def bench_graph_builder(args, torch, graphs, si, ctx):
     nnz = g.nnz
     g.close()
     # algorithmic work of the exact k-NN for float32 points: the dot product of
-    # each of the n (n - 1) pairs (2 d flops) on the FP32 matrix cores; the
-    # per-point norms, the FP64 re-rank of ~k candidates per query and the
-    # point and list transfers are < 1 % of it.  The time is the whole
-    # si_knn_exact call (upload, filter, re-rank, download).
+    # each of the n (n - 1) pairs, 2 d flops; the per-point norms, the FP64
+    # re-rank of ~k candidates per query and the point and list transfers are
+    # < 1 % of it.  The time is the whole si_knn_exact call (upload, filter,
+    # re-rank, download).  The filter takes each FP32 product as three bf16
+    # part products over DB = 32 ceil((d + 2) / 32) k-slots (coordinates and
+    # two norm slots), so the bf16 matrix cores execute 6 DB flops per pair and
+    # their dense peak caps the algorithmic rate at peak * 2 d / (6 DB).
     flops = 2.0 * d * n * (n - 1)
     tflops = flops / (knn_ms * 1e-3) / 1e12
+    db = 32 * ((d + 2 + 31) // 32)
+    executed = 6.0 * db * n * (n - 1) / (knn_ms * 1e-3) / 1e12
+    ceiling = BF16_MFMA_PEAK * 2.0 * d / (6.0 * db)
     direct = si.knn_path(-1)
-    log(f"graph builder: k-NN {knn_ms:.1f} ms ({tflops:.1f} TFLOP/s on the FP32 matrix cores, "
-        f"{direct} queries re-run by the direct FP64 kernel), whole graph {graph_ms:.1f} ms, "
-        f"nnz {nnz}")
+    log(f"graph builder: k-NN {knn_ms:.1f} ms ({tflops:.1f} TFLOP/s algorithmic, "
+        f"{executed:.0f} TFLOP/s executed on the bf16 matrix cores, {direct} queries re-run "
+        f"by the direct FP64 kernel), whole graph {graph_ms:.1f} ms, nnz {nnz}")
     cpu = None
     if not args.no_cpu:
         O.build()
@@ -124,9 +130,11 @@ def bench_graph_builder(args, torch, graphs, si, ctx):
         log(f"graph builder CPU oracle: {cpu['value']} ns/pair -> {cpu['sample']}")
     return dict(points=n, dim=d, k=k, nnz=nnz, ms=round(graph_ms, 1), knn_ms=round(knn_ms, 1),
                 knn_direct_queries=direct,
-                knn_roofline=dict(bound="mfma", achieved=round(tflops, 2), peak=FP32_MFMA_PEAK,
-                                  unit="TFLOP/s", frac=round(tflops / FP32_MFMA_PEAK, 4),
-                                  flops_per_pair=2 * d),
+                knn_roofline=dict(bound="mfma", achieved=round(tflops, 2),
+                                  peak=round(ceiling, 1), unit="TFLOP/s",
+                                  frac=round(tflops / ceiling, 4), flops_per_pair=2 * d,
+                                  executed_bf16_tflops=round(executed, 1),
+                                  executed_frac_of_bf16_peak=round(executed / BF16_MFMA_PEAK, 4)),
                 cpu_baseline=cpu)
 
 
