@@ -138,3 +138,16 @@ def test_far_clusters_take_the_fp32_stage_and_match_oracle(ctx):
         idx, dist, _ = run(ctx, x, 15, mode)
         np.testing.assert_array_equal(idx, oi)
         np.testing.assert_allclose(dist, od, rtol=1.2e-7, atol=0)
+
+
+@pytest.mark.parametrize("d", [1, 2, 30, 31, 62, 63, 64])
+def test_widths_across_k_blocks_match_oracle(ctx, d):
+    """One, two and three 32-slot k-blocks of the bf16 stage (d + 2 norm slots),
+    at and around each block boundary."""
+    print(f"case knn paths width d={d}", flush=True)
+    x = gmm(2500, d, 5, 300 + d)
+    x[11] = x[2222]
+    idx, dist, _ = run(ctx, x, 15, 0)
+    oi, od = O.knn_exact(x.astype(np.float64), 15)
+    np.testing.assert_array_equal(idx, oi)
+    np.testing.assert_allclose(dist, od, rtol=1.2e-7, atol=0)
