@@ -94,6 +94,22 @@ __device__ inline void topk_insert(D (&bd)[KP], int32_t (&bi)[KP], D s, int32_t 
     if (s < bd[0]) { bd[0] = s; bi[0] = j; }
 }
 
+// The same insertion for float lists (callers guarantee s < bd[KP-1]): slot
+// p becomes median(bd[p-1], bd[p], s) = max(bd[p-1], min(bd[p], s)), one
+// v_med3_f32, and each compare s < bd[p] serves slots p and p + 1.
+template <int KP>
+__device__ inline void topk_insert_f(float (&bd)[KP], int32_t (&bi)[KP], float s, int32_t j) {
+    bool lt[KP];
+#pragma unroll
+    for (int p = 0; p < KP; ++p) lt[p] = s < bd[p];
+#pragma unroll
+    for (int p = KP - 1; p > 0; --p) {
+        bd[p] = __builtin_amdgcn_fmed3f(bd[p - 1], bd[p], s);
+        bi[p] = lt[p - 1] ? bi[p - 1] : (lt[p] ? j : bi[p]);
+    }
+    if (lt[0]) { bd[0] = s; bi[0] = j; }
+}
+
 constexpr int kKnnTile = 64;   // candidates per LDS tile
 constexpr int kKnnThreads = 256;
 
@@ -414,7 +430,7 @@ __global__ __launch_bounds__(256) void k_knn_filter(const float* __restrict__ xc
 #pragma unroll
             for (int v = 0; v < 4; ++v)
                 if (lo[r][v] < bd[r][kFiltL - 1])
-                    topk_insert<kFiltL>(bd[r], bi[r], lo[r][v], cj[v]);
+                    topk_insert_f<kFiltL>(bd[r], bi[r], lo[r][v], cj[v]);
     };
     for (int64_t c0 = cb; c0 < ce; c0 += kFiltTile) {
         const int cnt = (int)((ce - c0) < kFiltTile ? (ce - c0) : kFiltTile);
@@ -514,9 +530,13 @@ __global__ __launch_bounds__(256) void k_knn_filter3(
     int32_t* __restrict__ layout_err) {
     constexpr int DB = 32 * KB;
     constexpr int DS = DB + 8;       // LDS row stride (bf16): spreads the 16-byte reads over the banks
+    // candidates per LDS tile, a multiple of the 16-row sub-tile (every
+    // sub-tile read stays inside the tile): 512, 256, 160 rows, <= 80 KB of LDS
+    constexpr int kT = (512 / KB) / 16 * 16;
+    static_assert(kT % 16 == 0, "tile rows must be whole sub-tiles");
     constexpr int R = KB == 1 ? 4 : 2;  // 16-query tiles per wavefront (registers)
     constexpr float kEmpty = 1e30f;  // list sentinel, above every real bound
-    __shared__ __align__(16) uint16_t th[kFiltTile * DS], tl[kFiltTile * DS];
+    __shared__ __align__(16) uint16_t th[kT * DS], tl[kT * DS];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lq = lane & 15, lg = lane >> 4;
 
@@ -581,14 +601,20 @@ __global__ __launch_bounds__(256) void k_knn_filter3(
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) acc[r] = zero;
+        // term-major order: consecutive MFMAs go to different query tiles, so
+        // none waits for the one before it
 #pragma unroll
-        for (int kb = 0; kb < KB; ++kb)
+        for (int kb = 0; kb < KB; ++kb) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
+            for (int r = 0; r < R; ++r)
                 acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_l[kb], qh[r][kb], acc[r], 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < R; ++r)
                 acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h[kb], ql[r][kb], acc[r], 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < R; ++r)
                 acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h[kb], qh[r][kb], acc[r], 0, 0, 0);
-            }
+        }
     };
     // one test of all 16 values first: late in the scan almost no lane
     // inserts, and the per-value branches are then skipped with one
@@ -605,13 +631,13 @@ __global__ __launch_bounds__(256) void k_knn_filter3(
             for (int v = 0; v < 4; ++v) {
                 const float lo = acc[r][v];
                 if (lo < bd[r][kFiltL - 1])
-                    topk_insert<kFiltL>(bd[r], bi[r], lo, (int32_t)(c0 + 16 * m + rowof[v]));
+                    topk_insert_f<kFiltL>(bd[r], bi[r], lo, (int32_t)(c0 + 16 * m + rowof[v]));
             }
     };
-    for (int64_t c0 = cb; c0 < ce; c0 += kFiltTile) {
-        const int cnt = (int)((ce - c0) < kFiltTile ? (ce - c0) : kFiltTile);
+    for (int64_t c0 = cb; c0 < ce; c0 += kT) {
+        const int cnt = (int)((ce - c0) < kT ? (ce - c0) : kT);
         __syncthreads();
-        for (int e = threadIdx.x; e < kFiltTile * (DB / 8); e += 256) {
+        for (int e = threadIdx.x; e < kT * (DB / 8); e += 256) {
             const int row = e / (DB / 8), c8 = e - row * (DB / 8);
             v8u16 h = {0, 0, 0, 0, 0, 0, 0, 0}, l = h;
             if (row < cnt) {
