@@ -525,9 +525,9 @@ typedef uint16_t v8u16 __attribute__((ext_vector_type(8)));
 template <int KB>
 __global__ __launch_bounds__(256) void k_knn_filter3(
     const uint16_t* __restrict__ ah, const uint16_t* __restrict__ al,
-    const uint16_t* __restrict__ bh, const uint16_t* __restrict__ bl, int64_t n, int d,
-    int64_t q0, int64_t q1, int32_t* __restrict__ kidx, float* __restrict__ kd2,
-    int32_t* __restrict__ layout_err) {
+    const uint16_t* __restrict__ bh, const uint16_t* __restrict__ bl, int64_t n, int d, int k,
+    const float* __restrict__ nrm, float e2a, float e2b, int64_t q0, int64_t q1,
+    int32_t* __restrict__ kidx, float* __restrict__ kd2, int32_t* __restrict__ layout_err) {
     constexpr int DB = 32 * KB;
     constexpr int DS = DB + 8;       // LDS row stride (bf16): spreads the 16-byte reads over the banks
     // candidates per LDS tile, a multiple of the 16-row sub-tile (every
@@ -618,21 +618,49 @@ __global__ __launch_bounds__(256) void k_knn_filter3(
     };
     // one test of all 16 values first: late in the scan almost no lane
     // inserts, and the per-value branches are then skipped with one
+    // thr[r]: a candidate at or above it is no true neighbour (see below),
+    // and none above the list's last entry can enter the list
+    float thr[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) thr[r] = kEmpty;
+    const int pk = (k + 1 + 3) / 4 - 1;  // 4 (pk + 1) >= k + 1 entries
     auto keep = [&](int m, int64_t c0, const v4f (&acc)[R]) {
         bool any = false;
 #pragma unroll
         for (int r = 0; r < R; ++r)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) any = any || acc[r][v] < bd[r][kFiltL - 1];
+            for (int v = 0; v < 4; ++v) any = any || acc[r][v] < thr[r];
         if (!any) return;
 #pragma unroll
         for (int r = 0; r < R; ++r)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const float lo = acc[r][v];
-                if (lo < bd[r][kFiltL - 1])
+                if (lo < thr[r]) {  // thr[r] <= bd[r][kFiltL - 1] always
                     topk_insert_f<kFiltL>(bd[r], bi[r], lo, (int32_t)(c0 + 16 * m + rowof[v]));
+                    thr[r] = fminf(thr[r], bd[r][kFiltL - 1]);
+                }
             }
+    };
+    // The shared threshold, once per tile.  The four lanes of a query (l % 16
+    // equal) hold lists of disjoint candidates; theta = the largest of their
+    // entries pk bounds 4 (pk + 1) >= k + 1 kept bounds l, so at least k
+    // candidates other than the query itself have l <= theta, hence
+    // d^2 <= l + 2E <= theta + Delta (Delta = 2 (2 alpha (|q'|^2 + max |c'|^2)
+    // + eabs) >= 2E) and d_k^2 <= theta + Delta.  A true neighbour has
+    // l <= d^2 <= d_k^2, so one with l >= thr (theta + Delta rounded up) is not.
+    auto refresh = [&]() {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            float t = kEmpty;
+#pragma unroll
+            for (int p = 0; p < kFiltL; ++p) t = p == pk ? bd[r][p] : t;
+            t = fmaxf(t, __shfl_xor(t, 16));
+            t = fmaxf(t, __shfl_xor(t, 32));
+            const float b = t + (qi[r] >= 0 ? fmaf(e2a, nrm[qi[r]], e2b) : 0.0f);
+            const float up = b + fabsf(b) * 0x1p-20f + 1e-30f;
+            thr[r] = t < kEmpty ? fminf(up, bd[r][kFiltL - 1]) : bd[r][kFiltL - 1];
+        }
     };
     for (int64_t c0 = cb; c0 < ce; c0 += kT) {
         const int cnt = (int)((ce - c0) < kT ? (ce - c0) : kT);
@@ -651,6 +679,7 @@ __global__ __launch_bounds__(256) void k_knn_filter3(
             *reinterpret_cast<v8u16*>(&tl[row * DS + 8 * c8]) = l;
         }
         __syncthreads();
+        refresh();
         const int subs = (cnt + 15) / 16;
         v4f acc[R];
         dots(0, acc);
@@ -1041,6 +1070,10 @@ bool knn_filtered(const float* xp, int64_t n, int d, int k, int32_t* oi, float* 
                            (float)eabs, ah.get(), al.get(), bh.get(), bl.get());
         SI_LAUNCH_CHECK();
     }
+    // Delta of the bf16 filter's shared threshold: 2 (2 alpha (nq + max nc) +
+    // eabs) = e2a nq + e2b, rounded up
+    const float e2a = (float)(4.0 * alpha3 * (1.0 + 1e-6));
+    const float e2b = (float)((4.0 * alpha3 * (double)m2 + 2.0 * eabs) * (1.0 + 1e-6));
     const int force = mode == 2 ? 1 : 0;
     auto pass = [&](bool three) -> int32_t {
         SI_HIP(hipMemsetAsync(ctl.get() + 2, 0, sizeof(int32_t), s));
@@ -1052,18 +1085,18 @@ bool knn_filtered(const float* xp, int64_t n, int d, int k, int32_t* oi, float* 
                 switch (kb) {
                     case 1:
                         hipLaunchKernelGGL((k_knn_filter3<1>), grid, dim3(256), 0, s, ah.get(),
-                                           al.get(), bh.get(), bl.get(), n, d, q0, q1, kidx.get(),
-                                           kd2.get(), ctl.get() + 1);
+                                           al.get(), bh.get(), bl.get(), n, d, k, nrm.get(), e2a,
+                                           e2b, q0, q1, kidx.get(), kd2.get(), ctl.get() + 1);
                         break;
                     case 2:
                         hipLaunchKernelGGL((k_knn_filter3<2>), grid, dim3(256), 0, s, ah.get(),
-                                           al.get(), bh.get(), bl.get(), n, d, q0, q1, kidx.get(),
-                                           kd2.get(), ctl.get() + 1);
+                                           al.get(), bh.get(), bl.get(), n, d, k, nrm.get(), e2a,
+                                           e2b, q0, q1, kidx.get(), kd2.get(), ctl.get() + 1);
                         break;
                     default:
                         hipLaunchKernelGGL((k_knn_filter3<3>), grid, dim3(256), 0, s, ah.get(),
-                                           al.get(), bh.get(), bl.get(), n, d, q0, q1, kidx.get(),
-                                           kd2.get(), ctl.get() + 1);
+                                           al.get(), bh.get(), bl.get(), n, d, k, nrm.get(), e2a,
+                                           e2b, q0, q1, kidx.get(), kd2.get(), ctl.get() + 1);
                         break;
                 }
             } else {
