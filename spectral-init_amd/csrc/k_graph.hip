@@ -256,8 +256,8 @@ __global__ void k_knn_merge(const double* __restrict__ pd, const int32_t* __rest
 //      (distance, index).  Every other query goes to k_knn through a list.
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-// 16-query tiles per wavefront: 4, or 2 above 48 coordinates (registers)
-template <int KS> constexpr int filt_tiles() { return KS > 12 ? 2 : 4; }
+// 16-query tiles per wavefront: 4, or 2 above 32 coordinates (registers)
+template <int KS> constexpr int filt_tiles() { return KS > 8 ? 2 : 4; }
 constexpr int kFiltK = 16;                     // largest k of the filtered path
 constexpr int kFiltL = 16;                     // kept candidates per lane and query
 constexpr int kFiltTile = 256;                 // candidates per LDS tile
@@ -339,7 +339,8 @@ template <int KS>
 __global__ __launch_bounds__(256) void k_knn_filter(const float* __restrict__ xc,
                                                     const float* __restrict__ nrm, int64_t n,
                                                     int64_t q0, int64_t q1,
-                                                    float shrink, float eabs,
+                                                    float shrink, float eabs, int k,
+                                                    float e2a, float e2b,
                                                     int32_t* __restrict__ kidx,
                                                     float* __restrict__ kd2,
                                                     int32_t* __restrict__ layout_err) {
@@ -383,6 +384,10 @@ __global__ __launch_bounds__(256) void k_knn_filter(const float* __restrict__ xc
         for (int p = 0; p < kFiltL; ++p) { bd[r][p] = INFINITY; bi[r][p] = -1; }
     }
 
+    float thr[kFiltR];
+#pragma unroll
+    for (int r = 0; r < kFiltR; ++r) thr[r] = INFINITY;
+    const int pk = (k + 1 + 3) / 4 - 1;  // 4 (pk + 1) >= k + 1 entries
     // sub-tile m: dot products of the 16 candidates with the 4 query tiles
     auto dots = [&](int m, v4f (&acc)[kFiltR]) {
         float a[KS];
@@ -422,15 +427,32 @@ __global__ __launch_bounds__(256) void k_knn_filter(const float* __restrict__ xc
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 lo[r][v] = fmaf(-2.0f, acc[r][v], nq[r] + cn[v]);
-                any = any || lo[r][v] < bd[r][kFiltL - 1];
+                any = any || lo[r][v] < thr[r];
             }
         if (!any) return;  // one branch for the common case
 #pragma unroll
         for (int r = 0; r < kFiltR; ++r)
 #pragma unroll
             for (int v = 0; v < 4; ++v)
-                if (lo[r][v] < bd[r][kFiltL - 1])
+                if (lo[r][v] < thr[r]) {  // thr[r] <= bd[r][kFiltL - 1] always
                     topk_insert_f<kFiltL>(bd[r], bi[r], lo[r][v], cj[v]);
+                    thr[r] = fminf(thr[r], bd[r][kFiltL - 1]);
+                }
+    };
+    // the threshold shared by the four lanes of a query, as in k_knn_filter3
+    // (same argument with this stage's alpha), once per tile
+    auto refresh = [&]() {
+#pragma unroll
+        for (int r = 0; r < kFiltR; ++r) {
+            float t = INFINITY;
+#pragma unroll
+            for (int p = 0; p < kFiltL; ++p) t = p == pk ? bd[r][p] : t;
+            t = fmaxf(t, __shfl_xor(t, 16));
+            t = fmaxf(t, __shfl_xor(t, 32));
+            const float b = t + (qi[r] >= 0 ? fmaf(e2a, nrm[qi[r]], e2b) : 0.0f);
+            const float up = b + fabsf(b) * 0x1p-20f + 1e-30f;
+            thr[r] = t < INFINITY ? fminf(up, bd[r][kFiltL - 1]) : bd[r][kFiltL - 1];
+        }
     };
     for (int64_t c0 = cb; c0 < ce; c0 += kFiltTile) {
         const int cnt = (int)((ce - c0) < kFiltTile ? (ce - c0) : kFiltTile);
@@ -444,6 +466,7 @@ __global__ __launch_bounds__(256) void k_knn_filter(const float* __restrict__ xc
         for (int e = threadIdx.x; e < kFiltTile; e += 256)
             tn[e] = e < cnt ? nrm[c0 + e] * shrink : INFINITY;
         __syncthreads();
+        refresh();
         // software pipeline: sub-tile m + 1's MFMAs are issued before sub-tile
         // m's list updates, which then run while the matrix cores work
         const int subs = (cnt + 15) / 16;
@@ -1074,6 +1097,8 @@ bool knn_filtered(const float* xp, int64_t n, int d, int k, int32_t* oi, float* 
     // eabs) = e2a nq + e2b, rounded up
     const float e2a = (float)(4.0 * alpha3 * (1.0 + 1e-6));
     const float e2b = (float)((4.0 * alpha3 * (double)m2 + 2.0 * eabs) * (1.0 + 1e-6));
+    const float e2a32 = (float)(4.0 * alpha32 * (1.0 + 1e-6));
+    const float e2b32 = (float)((4.0 * alpha32 * (double)m2 + 2.0 * eabs) * (1.0 + 1e-6));
     const int force = mode == 2 ? 1 : 0;
     auto pass = [&](bool three) -> int32_t {
         SI_HIP(hipMemsetAsync(ctl.get() + 2, 0, sizeof(int32_t), s));
@@ -1101,8 +1126,8 @@ bool knn_filtered(const float* xp, int64_t n, int d, int k, int32_t* oi, float* 
                 }
             } else {
                 hipLaunchKernelGGL((k_knn_filter<KS>), grid, dim3(256), 0, s, xc.get(), nrm.get(),
-                                   n, q0, q1, (float)(1.0 - 2.0 * alpha), (float)eabs,
-                                   kidx.get(), kd2.get(), ctl.get() + 1);
+                                   n, q0, q1, (float)(1.0 - 2.0 * alpha), (float)eabs, k,
+                                   e2a32, e2b32, kidx.get(), kd2.get(), ctl.get() + 1);
             }
             hipLaunchKernelGGL((k_knn_rerank<DP>), dim3(ceil_div(q1 - q0, 256)), dim3(256), 0, s,
                                xp, nrm.get(), q0, q1, k, lists, alpha, eabs, kidx.get(),
