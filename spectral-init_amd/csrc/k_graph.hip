@@ -1105,7 +1105,9 @@ bool knn_filtered(const float* xp, int64_t n, int d, int k, int32_t* oi, float* 
         const double alpha = three ? alpha3 : alpha32;
         for (int64_t q0 = 0; q0 < n; q0 += chunk) {
             const int64_t q1 = std::min(n, q0 + chunk);
-            const dim3 grid((unsigned)ceil_div(q1 - q0, (int64_t)qpb), (unsigned)splits);
+            // queries per workgroup of the kernel this pass runs
+            const int64_t qw = three ? qpb : filt_queries<KS>();
+            const dim3 grid((unsigned)ceil_div(q1 - q0, qw), (unsigned)splits);
             if (three) {
                 switch (kb) {
                     case 1:

@@ -123,13 +123,15 @@ def test_outlier_points_match_oracle(ctx):
     np.testing.assert_allclose(dist, od, rtol=1.2e-7, atol=0)
 
 
-def test_far_clusters_take_the_fp32_stage_and_match_oracle(ctx):
+@pytest.mark.parametrize("d", [8, 32])
+def test_far_clusters_take_the_fp32_stage_and_match_oracle(ctx, d):
     """Two tight clusters 2000 units apart: the bf16 parts cannot resolve the
     neighbours against |q'|^2 ~ 1e6, so the FP32 stage (and the direct kernel
-    for what it leaves) must produce the result."""
-    print("case knn paths far clusters", flush=True)
+    for what it leaves) must produce the result.  At d = 32 the two stages use
+    different queries per workgroup."""
+    print(f"case knn paths far clusters d={d}", flush=True)
     rng = np.random.default_rng(4)
-    x = rng.standard_normal((3000, 8))
+    x = rng.standard_normal((3000, d))
     x[:1500, 0] += 1000.0
     x[1500:, 0] -= 1000.0
     x = x.astype(np.float32)
