@@ -9,9 +9,9 @@
 //                    (fused), top-k kept sorted in registers, ties to the
 //                    smaller index.  Bound by the LDS broadcast reads and the
 //                    FP64 VALU work (2-3 instructions per dimension and pair).
-//                    Float64 points, k > 16, and the queries the filter below
-//                    cannot certify take this kernel.
-//   k_knn_filter,    the same k-NN for float32 points with k <= 16: FP32
+//                    k > 16 and the queries the filter below cannot certify
+//                    take this kernel.
+//   k_knn_filter(3), the same k-NN for k <= 16 (float32 or float64 points):
 //   k_knn_rerank     matrix-core distances with rigorous error bounds keep a
 //                    short candidate list per query, and FP64 distances of
 //                    the certified candidates (k_knn's arithmetic) rank them;
@@ -235,8 +235,8 @@ __global__ void k_knn_merge(const double* __restrict__ pd, const int32_t* __rest
 }
 
 // ------------------------------------------------------------ MFMA filter
-// Float32 points with k <= kFiltK take two stages that give exactly k_knn's
-// result:
+// Points (float32 or float64) with k <= kFiltK take two stages that give
+// exactly k_knn's result:
 //   1. k_knn_filter: a = |q'|^2 + |c'|^2 - 2 q'.c' on the FP32 matrix cores
 //      (v_mfma_f32_16x16x4_f32) for the centred, float-rounded points q', c',
 //      and the lower bound l = a - E of the exact squared distance d^2, where
@@ -265,17 +265,19 @@ constexpr int kFiltGroups = 4;                 // lanes sharing one query (one p
 template <int KS> constexpr int filt_queries() { return 4 * 16 * filt_tiles<KS>(); }  // per workgroup
 constexpr int kColParts = 2048;
 
-// Column sums of the padded points in FP64, part b over rows
-// [b n / P, (b + 1) n / P); one thread per column (dp <= 64 = blockDim).
-__global__ void k_col_partials(const float* __restrict__ x, int64_t n, int dp,
+// Column sums of the points (rows of dx stored coordinates, the first d
+// used) in FP64, part b over rows [b n / P, (b + 1) n / P); one thread per
+// column (d <= 64 = blockDim).
+template <typename CT>
+__global__ void k_col_partials(const CT* __restrict__ x, int64_t n, int dx, int d,
                                double* __restrict__ part) {
     const int64_t P = gridDim.x;
     const int64_t r0 = n * blockIdx.x / P, r1 = n * (blockIdx.x + 1) / P;
     const int t = threadIdx.x;
-    if (t >= dp) return;
+    if (t >= d) return;
     double s = 0.0;
-    for (int64_t r = r0; r < r1; ++r) s += (double)x[r * dp + t];
-    part[(int64_t)blockIdx.x * dp + t] = s;
+    for (int64_t r = r0; r < r1; ++r) s += (double)x[r * dx + t];
+    part[(int64_t)blockIdx.x * d + t] = s;
 }
 
 // 16 threads per column (1024 threads, dp <= 64) sum strided parts, then
@@ -297,11 +299,12 @@ __global__ __launch_bounds__(1024) void k_col_mean(const double* __restrict__ pa
     }
 }
 
-// xc = float(x - mean), nrm = float(|xc|^2) (FP64 sum); the largest nrm's bit
-// pattern is max'ed into *nmax (non-negative floats order as their unsigned
-// bit patterns, and a NaN orders above +inf, so a non-finite input shows).
-template <int DP>
-__global__ void k_center_norms(const float* __restrict__ x, int64_t n,
+// xc [n x DP] = float(x - mean) on the first d coordinates (zeros after),
+// nrm = float(|xc|^2) (FP64 sum); the largest nrm's bit pattern is max'ed into
+// *nmax (non-negative floats order as their unsigned bit patterns, and a NaN
+// orders above +inf, so a non-finite input shows).  x has rows of dx.
+template <int DP, typename CT>
+__global__ void k_center_norms(const CT* __restrict__ x, int64_t n, int dx, int d,
                                const double* __restrict__ mean, float* __restrict__ xc,
                                float* __restrict__ nrm, unsigned* __restrict__ nmax) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -310,7 +313,7 @@ __global__ void k_center_norms(const float* __restrict__ x, int64_t n,
         double s = 0.0;
 #pragma unroll
         for (int t = 0; t < DP; ++t) {
-            const float c = (float)((double)x[i * DP + t] - mean[t]);
+            const float c = t < d ? (float)((double)x[i * dx + t] - mean[t]) : 0.0f;
             xc[i * DP + t] = c;
             s += (double)c * (double)c;
         }
@@ -736,8 +739,11 @@ __device__ inline bool rank_before(double s, int32_t j, double bs, int32_t bj) {
 // Stage 2, one query per thread (see the comment above filt_tiles).  A query that
 // fails the certificate, or every query with force_direct, is appended to
 // qlist for k_knn.
-template <int DP>
-__global__ __launch_bounds__(256) void k_knn_rerank(const float* __restrict__ xp, const float* __restrict__ nrm,
+// xp: the points as stored for k_knn (rows of dx coordinates, zero padded);
+// the exact distances run over those dx coordinates in k_knn's order.
+template <int DP, typename CT>
+__global__ __launch_bounds__(256) void k_knn_rerank(const CT* __restrict__ xp, int dx,
+                             const float* __restrict__ nrm,
                              int64_t q0, int64_t q1, int k, int lists, double alpha,
                              double eabs, const int32_t* __restrict__ kidx,
                              const float* __restrict__ kd2,
@@ -779,7 +785,7 @@ __global__ __launch_bounds__(256) void k_knn_rerank(const float* __restrict__ xp
         qlist[atomicAdd(qcount, 1)] = (int32_t)q;
         return;
     }
-    const float* qx = xp + q * DP;
+    const CT* qx = xp + q * dx;
     double bs[kFiltK];
     int32_t bj[kFiltK];
 #pragma unroll
@@ -790,8 +796,10 @@ __global__ __launch_bounds__(256) void k_knn_rerank(const float* __restrict__ xp
         double s = 0.0;
 #pragma unroll
         for (int t = 0; t < DP; ++t) {
-            const double df = (double)qx[t] - (double)xp[(int64_t)j * DP + t];
-            s = fma(df, df, s);
+            if (t < dx) {
+                const double df = (double)qx[t] - (double)xp[(int64_t)j * dx + t];
+                s = fma(df, df, s);
+            }
         }
         if (!rank_before(s, j, bs[kFiltK - 1], bj[kFiltK - 1])) continue;
 #pragma unroll
@@ -1029,27 +1037,45 @@ void launch_knn_k(const CT* x, int64_t n, int k, int32_t* oi, float* od, hipStre
     else launch_knn_q<DP, 64>(x, n, k, oi, od, s, qlist, nq);
 }
 
-// The two-stage path for float32 points of dimension d padded to DP = 4 KS
+// The direct kernel at the stored width dp (runtime).
+template <typename CT>
+void launch_knn_dp(int dp, const CT* x, int64_t n, int k, int32_t* oi, float* od, hipStream_t s,
+                   const int32_t* qlist, int64_t nq) {
+    switch (dp) {
+        case 2: if constexpr (sizeof(CT) == 8) launch_knn_k<2>(x, n, k, oi, od, s, qlist, nq); break;
+        case 4: launch_knn_k<4>(x, n, k, oi, od, s, qlist, nq); break;
+        case 8: launch_knn_k<8>(x, n, k, oi, od, s, qlist, nq); break;
+        case 16: launch_knn_k<16>(x, n, k, oi, od, s, qlist, nq); break;
+        case 32: launch_knn_k<32>(x, n, k, oi, od, s, qlist, nq); break;
+        case 48: launch_knn_k<48>(x, n, k, oi, od, s, qlist, nq); break;
+        default: launch_knn_k<64>(x, n, k, oi, od, s, qlist, nq); break;
+    }
+}
+
+// The two-stage path for points of dimension d (float32 or float64, stored in
+// rows of dx as k_knn reads them), filtered at DP = 4 KS coordinates
 // coordinates (k <= kFiltK).  Stage 1 runs on the bf16 matrix cores (three
 // bf16 products per FP32 product) when the squared norms of the centred
 // points are at most 1e20 and d + 2 <= 96; if that leaves more than 1 % of the
 // queries uncertified, or otherwise, the FP32 matrix cores run it.  Returns
 // false, having written nothing, when a squared norm is not finite or above
 // 1e36; the caller then runs the direct kernel.
-template <int KS>
-bool knn_filtered(const float* xp, int64_t n, int d, int k, int32_t* oi, float* od,
+template <int KS, typename CT>
+bool knn_filtered(const CT* xp, int dx, int64_t n, int d, int k, int32_t* oi, float* od,
                   hipStream_t s) {
     constexpr int DP = 4 * KS;
     DevBuf<double> part((size_t)kColParts * DP), mean(DP);
-    hipLaunchKernelGGL(k_col_partials, dim3(kColParts), dim3(64), 0, s, xp, n, DP, part.get());
-    hipLaunchKernelGGL(k_col_mean, dim3(1), dim3(1024), 0, s, part.get(), kColParts, n, DP,
+    hipLaunchKernelGGL((k_col_partials<CT>), dim3(kColParts), dim3(64), 0, s, xp, n, dx, d,
+                       part.get());
+    hipLaunchKernelGGL(k_col_mean, dim3(1), dim3(1024), 0, s, part.get(), kColParts, n, d,
                        mean.get());
     SI_LAUNCH_CHECK();
     DevBuf<float> xc((size_t)n * DP), nrm(n);
     DevBuf<int32_t> ctl(4);  // norm max bits, layout error, direct-query count
     SI_HIP(hipMemsetAsync(ctl.get(), 0, 4 * sizeof(int32_t), s));
-    hipLaunchKernelGGL((k_center_norms<DP>), dim3(ceil_div(n, 256)), dim3(256), 0, s, xp, n,
-                       mean.get(), xc.get(), nrm.get(), reinterpret_cast<unsigned*>(ctl.get()));
+    hipLaunchKernelGGL((k_center_norms<DP, CT>), dim3(ceil_div(n, 256)), dim3(256), 0, s, xp, n,
+                       dx, d, mean.get(), xc.get(), nrm.get(),
+                       reinterpret_cast<unsigned*>(ctl.get()));
     SI_LAUNCH_CHECK();
     uint32_t mbits = 0;
     SI_HIP(hipMemcpyAsync(&mbits, ctl.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -1131,8 +1157,8 @@ bool knn_filtered(const float* xp, int64_t n, int d, int k, int32_t* oi, float* 
                                    n, q0, q1, (float)(1.0 - 2.0 * alpha), (float)eabs, k,
                                    e2a32, e2b32, kidx.get(), kd2.get(), ctl.get() + 1);
             }
-            hipLaunchKernelGGL((k_knn_rerank<DP>), dim3(ceil_div(q1 - q0, 256)), dim3(256), 0, s,
-                               xp, nrm.get(), q0, q1, k, lists, alpha, eabs, kidx.get(),
+            hipLaunchKernelGGL((k_knn_rerank<DP, CT>), dim3(ceil_div(q1 - q0, 256)), dim3(256), 0,
+                               s, xp, dx, nrm.get(), q0, q1, k, lists, alpha, eabs, kidx.get(),
                                kd2.get(), force, oi, od, qlist.get(), ctl.get() + 2);
             SI_LAUNCH_CHECK();
         }
@@ -1145,20 +1171,22 @@ bool knn_filtered(const float* xp, int64_t n, int d, int k, int32_t* oi, float* 
     int32_t direct = pass(bf16x3);
     if (bf16x3 && !force && direct > n / 100) direct = pass(false);
     g_knn_direct.store(direct);
-    if (direct > 0) launch_knn_k<DP, float>(xp, n, k, oi, od, s, qlist.get(), direct);
+    if (direct > 0) launch_knn_dp<CT>(dx, xp, n, k, oi, od, s, qlist.get(), direct);
     SI_HIP(hipStreamSynchronize(s));  // the buffers are freed on return
     return true;
 }
 
-bool knn_filtered_dp(int dp, const float* xp, int64_t n, int d, int k, int32_t* oi, float* od,
-                     hipStream_t s) {
+// dp: the filter's width (a multiple of 4 >= d); dx: the stored width of xp
+template <typename CT>
+bool knn_filtered_dp(int dp, const CT* xp, int dx, int64_t n, int d, int k, int32_t* oi,
+                     float* od, hipStream_t s) {
     switch (dp) {
-        case 4: return knn_filtered<1>(xp, n, d, k, oi, od, s);
-        case 8: return knn_filtered<2>(xp, n, d, k, oi, od, s);
-        case 16: return knn_filtered<4>(xp, n, d, k, oi, od, s);
-        case 32: return knn_filtered<8>(xp, n, d, k, oi, od, s);
-        case 48: return knn_filtered<12>(xp, n, d, k, oi, od, s);
-        default: return knn_filtered<16>(xp, n, d, k, oi, od, s);
+        case 4: return knn_filtered<1>(xp, dx, n, d, k, oi, od, s);
+        case 8: return knn_filtered<2>(xp, dx, n, d, k, oi, od, s);
+        case 16: return knn_filtered<4>(xp, dx, n, d, k, oi, od, s);
+        case 32: return knn_filtered<8>(xp, dx, n, d, k, oi, od, s);
+        case 48: return knn_filtered<12>(xp, dx, n, d, k, oi, od, s);
+        default: return knn_filtered<16>(xp, dx, n, d, k, oi, od, s);
     }
 }
 
@@ -1182,21 +1210,10 @@ void knn_typed(const CT* pts, int64_t n, int64_t d, int64_t k, int32_t* idx, flo
     hipLaunchKernelGGL((k_pad_points<CT>), dim3(ceil_div(tot, 256)), dim3(256), 0, s, pts, n,
                        (int)d, dp, xp.get());
     SI_LAUNCH_CHECK();
-    if constexpr (sizeof(CT) == 4) {
-        if (g_knn_path.load() != 1 && k <= kFiltK &&
-            knn_filtered_dp(dp, xp.get(), n, (int)d, (int)k, idx, dist, s))
-            return;
-    }
-    const CT* x = xp.get();
-    switch (dp) {
-        case 2: if constexpr (sizeof(CT) == 8) launch_knn_k<2>(x, n, (int)k, idx, dist, s, nullptr, n); break;
-        case 4: launch_knn_k<4>(x, n, (int)k, idx, dist, s, nullptr, n); break;
-        case 8: launch_knn_k<8>(x, n, (int)k, idx, dist, s, nullptr, n); break;
-        case 16: launch_knn_k<16>(x, n, (int)k, idx, dist, s, nullptr, n); break;
-        case 32: launch_knn_k<32>(x, n, (int)k, idx, dist, s, nullptr, n); break;
-        case 48: launch_knn_k<48>(x, n, (int)k, idx, dist, s, nullptr, n); break;
-        default: launch_knn_k<64>(x, n, (int)k, idx, dist, s, nullptr, n); break;
-    }
+    if (g_knn_path.load() != 1 && k <= kFiltK &&
+        knn_filtered_dp<CT>(padded_dim(d, 4), xp.get(), dp, n, (int)d, (int)k, idx, dist, s))
+        return;
+    launch_knn_dp<CT>(dp, xp.get(), n, (int)k, idx, dist, s, nullptr, n);
     SI_HIP(hipStreamSynchronize(s));  // xp is freed on return
 }
 

@@ -1,4 +1,5 @@
-"""GPU checks of the two exact k-NN paths of si_knn_exact for float32 points
+"""GPU checks of the two exact k-NN paths of si_knn_exact (float32 and float64
+points)
 (spectral-init_amd/csrc/k_graph.hip): the FP32 MFMA filter with the certified
 FP64 re-rank, and the direct FP64 kernel.  Every path must return the direct
 kernel's result bit for bit; the direct kernel itself is pinned against the
@@ -151,5 +152,31 @@ def test_widths_across_k_blocks_match_oracle(ctx, d):
     x[11] = x[2222]
     idx, dist, _ = run(ctx, x, 15, 0)
     oi, od = O.knn_exact(x.astype(np.float64), 15)
+    np.testing.assert_array_equal(idx, oi)
+    np.testing.assert_allclose(dist, od, rtol=1.2e-7, atol=0)
+
+
+def test_float64_points_filter_equals_direct_kernel(ctx):
+    """float64 points (the reference fixtures' type) take the same filter: the
+    centred coordinates are rounded to float32 for stage 1, and the re-rank uses
+    the float64 coordinates, exactly as the direct kernel does."""
+    print("case knn paths float64 100k", flush=True)
+    x = gmm(100_000, 30, 20, 77).astype(np.float64)
+    x += np.random.default_rng(5).standard_normal(x.shape) * 1e-9  # not float32 values
+    x[3] = x[77_777]
+    i1, d1, _ = run(ctx, x, 15, 1)
+    i0, d0, direct = run(ctx, x, 15, 0)
+    np.testing.assert_array_equal(i0, i1)
+    np.testing.assert_array_equal(d0.view(np.uint32), d1.view(np.uint32))
+    assert direct < 0.01 * x.shape[0], direct
+    assert i0[3, 0] == 77_777
+
+
+@pytest.mark.parametrize("d", [1, 2, 30])
+def test_float64_widths_match_oracle(ctx, d):
+    print(f"case knn paths float64 d={d}", flush=True)
+    x = gmm(2500, d, 5, 900 + d).astype(np.float64) * 1.0000001
+    idx, dist, _ = run(ctx, x, 15, 0)
+    oi, od = O.knn_exact(x, 15)
     np.testing.assert_array_equal(idx, oi)
     np.testing.assert_allclose(dist, od, rtol=1.2e-7, atol=0)
