@@ -76,7 +76,7 @@ si_status si_test_rcm_order(int64_t n, const int64_t* indptr, const int32_t* ind
                             int32_t threads, int32_t* perm);
 
 /* The arithmetic of the exact k-NN step (si_knn_exact, si_fuzzy_graph_build)
- * in later calls of this process: mode 0 = automatic (points with k <= 16
+ * in later calls of this process: mode 0 = automatic (points with k <= 32
  * take the matrix-core filter, bf16x3 or FP32, and the certified FP64
  * re-rank), 1 = the direct FP64 kernel only, 2 = the filter with every query
  * re-run by the direct kernel, 3 = the filter on the FP32 matrix cores only;

@@ -9,9 +9,9 @@
 //                    (fused), top-k kept sorted in registers, ties to the
 //                    smaller index.  Bound by the LDS broadcast reads and the
 //                    FP64 VALU work (2-3 instructions per dimension and pair).
-//                    k > 16 and the queries the filter below cannot certify
+//                    k > 32 and the queries the filter below cannot certify
 //                    take this kernel.
-//   k_knn_filter(3), the same k-NN for k <= 16 (float32 or float64 points):
+//   k_knn_filter(3), the same k-NN for k <= 32 (float32 or float64 points):
 //   k_knn_rerank     matrix-core distances with rigorous error bounds keep a
 //                    short candidate list per query, and FP64 distances of
 //                    the certified candidates (k_knn's arithmetic) rank them;
@@ -258,7 +258,7 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 // 16-query tiles per wavefront: 4, or 2 above 32 coordinates (registers)
 template <int KS> constexpr int filt_tiles() { return KS > 8 ? 2 : 4; }
-constexpr int kFiltK = 16;                     // largest k of the filtered path
+constexpr int kFiltK = 32;                     // largest k of the filtered path
 constexpr int kFiltL = 16;                     // kept candidates per lane and query
 constexpr int kFiltTile = 256;                 // candidates per LDS tile
 constexpr int kFiltGroups = 4;                 // lanes sharing one query (one per quarter)

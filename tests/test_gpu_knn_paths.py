@@ -80,7 +80,7 @@ def test_forced_direct_requeries_every_query(ctx):
     np.testing.assert_array_equal(d2.view(np.uint32), d1.view(np.uint32))
 
 
-@pytest.mark.parametrize("k", [1, 8, 16, 17])
+@pytest.mark.parametrize("k", [1, 8, 16, 17, 30, 32, 33])
 def test_k_boundary_matches_oracle(ctx, k):
     print(f"case knn paths k={k}", flush=True)
     x = gmm(3000, 13, 6, 100 + k)
