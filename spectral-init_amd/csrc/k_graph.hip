@@ -557,10 +557,13 @@ __global__ __launch_bounds__(256) void k_knn_filter3(
     constexpr int DB = 32 * KB;
     constexpr int DS = DB + 8;       // LDS row stride (bf16): spreads the 16-byte reads over the banks
     // candidates per LDS tile, a multiple of the 16-row sub-tile (every
-    // sub-tile read stays inside the tile): 512, 256, 160 rows, <= 80 KB of LDS
-    constexpr int kT = (512 / KB) / 16 * 16;
+    // sub-tile read stays inside the tile): 256, 128, 80 rows, <= 40 KB of LDS,
+    // so LDS does not cap the waves per SIMD
+    constexpr int kT = (256 / KB) / 16 * 16;
     static_assert(kT % 16 == 0, "tile rows must be whole sub-tiles");
-    constexpr int R = KB == 1 ? 4 : 2;  // 16-query tiles per wavefront (registers)
+    // 16-query tiles per wavefront: two keep the registers low enough for three
+    // waves per SIMD (four tiles and two waves measured 12 % slower)
+    constexpr int R = 2;
     constexpr float kEmpty = 1e30f;  // list sentinel, above every real bound
     __shared__ __align__(16) uint16_t th[kT * DS], tl[kT * DS];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1100,7 +1103,7 @@ bool knn_filtered(const CT* xp, int dx, int64_t n, int d, int k, int32_t* oi, fl
     const int db = 32 * kb;
     const double alpha3 = 4.02 * std::ldexp(1.0, -16) + (6.0 * db + 10.0) * u;
     const double eabs = 96.0 * 1e-30;
-    const int qpb = bf16x3 ? 4 * 16 * (kb == 1 ? 4 : 2) : filt_queries<KS>();
+    const int qpb = bf16x3 ? 4 * 16 * 2 : filt_queries<KS>();
     // candidate splits: about 2048 workgroups, at most 16 splits
     const int64_t qblocks = ceil_div(n, (int64_t)qpb);
     const int splits =
