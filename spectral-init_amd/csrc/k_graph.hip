@@ -313,20 +313,34 @@ __global__ void k_center_norms(const CT* __restrict__ x, int64_t n, int dx, int 
         double s = 0.0;
 #pragma unroll
         for (int t = 0; t < DP; ++t) {
-            const float c = t < d ? (float)((double)x[i * dx + t] - mean[t]) : 0.0f;
+            // float32 rows are stored at the filter width (dx == DP) with zero
+            // padding and zero means there, so no per-coordinate test is needed
+            // and the row load vectorizes; float64 rows may be narrower (dx = 2)
+            float c;
+            if constexpr (sizeof(CT) == 4) c = (float)((double)x[i * DP + t] - mean[t]);
+            else c = t < d ? (float)((double)x[i * dx + t] - mean[t]) : 0.0f;
             xc[i * DP + t] = c;
             s += (double)c * (double)c;
         }
         nv = (float)s;
         nrm[i] = nv;
     }
+    // wavefront max, then the workgroup's (256 threads) in LDS: one atomic per
+    // workgroup keeps the contention on *nmax low
+    __shared__ unsigned red[4];
     unsigned b = __float_as_uint(nv);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned o = __shfl_xor(b, off);
         b = o > b ? o : b;
     }
-    if ((threadIdx.x & 63) == 0) atomicMax(nmax, b);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned m = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = red[w] > m ? red[w] : m;
+        atomicMax(nmax, m);
+    }
 }
 
 // Stage 1.  A workgroup owns filt_queries() queries of [q0, q1) (16 per
@@ -1068,9 +1082,10 @@ bool knn_filtered(const CT* xp, int dx, int64_t n, int d, int k, int32_t* oi, fl
                   hipStream_t s) {
     constexpr int DP = 4 * KS;
     DevBuf<double> part((size_t)kColParts * DP), mean(DP);
-    hipLaunchKernelGGL((k_col_partials<CT>), dim3(kColParts), dim3(64), 0, s, xp, n, dx, d,
+    // means over every stored column (the padding's are 0)
+    hipLaunchKernelGGL((k_col_partials<CT>), dim3(kColParts), dim3(64), 0, s, xp, n, dx, dx,
                        part.get());
-    hipLaunchKernelGGL(k_col_mean, dim3(1), dim3(1024), 0, s, part.get(), kColParts, n, d,
+    hipLaunchKernelGGL(k_col_mean, dim3(1), dim3(1024), 0, s, part.get(), kColParts, n, dx,
                        mean.get());
     SI_LAUNCH_CHECK();
     DevBuf<float> xc((size_t)n * DP), nrm(n);
