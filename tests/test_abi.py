@@ -67,3 +67,21 @@ def test_product_does_not_link_the_oracle(library):
     out = subprocess.run(["ldd", si.library_path()], capture_output=True, text=True).stdout
     assert "oracle" not in out
     assert not hasattr(library, "or_spmm")
+
+
+def test_knn_path_seam_without_gpu(library):
+    """The k-NN path selector is host-only: modes -1..3 are accepted (and the
+    mode persists until reset), anything else is SI_ERR_INVALID_ARGUMENT."""
+    out = ctypes.c_int64(-7)
+    fn = library.si_test_knn_path
+    fn.argtypes = [ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]
+    fn.restype = ctypes.c_int
+    try:
+        for mode in (0, 1, 2, 3, -1):
+            assert fn(mode, ctypes.byref(out)) == si.SI_OK
+            assert out.value >= 0
+        assert fn(4, ctypes.byref(out)) == si.SI_ERR_INVALID_ARGUMENT
+        assert fn(-2, None) == si.SI_ERR_INVALID_ARGUMENT
+        assert fn(-1, None) == si.SI_OK
+    finally:
+        fn(0, None)
