@@ -41,6 +41,7 @@ int or_get_threads(void);
 
 /* ---- counter-based normal RNG (Philox4x32-10 + Box-Muller), shared by the
  *      oracle and the device path so both start from the same blocks ---- */
+void or_philox4x32_10(const uint32_t ctr[4], uint32_t key0, uint32_t key1, uint32_t out[4]);
 double or_normal(uint64_t seed, uint32_t stream, uint64_t index);
 void or_fill_normal(uint64_t seed, uint32_t stream, size_t count, double* out);
 

@@ -18,6 +18,7 @@ _i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 
 
 class LobpcgStats(C.Structure):
@@ -41,6 +42,7 @@ def lib():
             build()
         L = C.CDLL(LIB_PATH)
         L.or_set_threads.argtypes = [C.c_int]
+        L.or_philox4x32_10.argtypes = [_u32p, C.c_uint32, C.c_uint32, _u32p]
         L.or_normal.restype = C.c_double
         L.or_normal.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64]
         L.or_fill_normal.argtypes = [C.c_uint64, C.c_uint32, C.c_size_t, _f64p]
@@ -129,6 +131,18 @@ def _csr_l(indptr, indices, data):
     return (np.ascontiguousarray(indptr, dtype=np.int64),
             np.ascontiguousarray(indices, dtype=np.int32),
             np.ascontiguousarray(data, dtype=np.float64))
+
+
+def philox4x32_10(ctr, key0: int, key1: int) -> np.ndarray:
+    """The raw Philox4x32-10 output block for one 4-word counter."""
+    c = np.ascontiguousarray(ctr, np.uint32)
+    out = np.empty(4, np.uint32)
+    lib().or_philox4x32_10(c, key0, key1, out)
+    return out
+
+
+def normal_at(seed: int, stream: int, index: int) -> float:
+    return lib().or_normal(seed, stream, index)
 
 
 def normal_block(seed: int, stream: int, count: int) -> np.ndarray:

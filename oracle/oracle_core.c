@@ -39,6 +39,14 @@ static void philox4x32_10(uint32_t ctr[4], uint32_t key0, uint32_t key1) {
     }
 }
 
+/* The raw generator on one counter block (the known-answer vectors of the
+ * published Random123 suite check it). */
+void or_philox4x32_10(const uint32_t ctr[4], uint32_t key0, uint32_t key1, uint32_t out[4]) {
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    philox4x32_10(c, key0, key1);
+    for (int i = 0; i < 4; ++i) out[i] = c[i];
+}
+
 double or_normal(uint64_t seed, uint32_t stream, uint64_t index) {
     uint32_t c[4] = {(uint32_t)index, (uint32_t)(index >> 32), stream, 0x5eedu};
     philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));

@@ -558,9 +558,7 @@ si_status si_max_eigenpair_residual(si_context* ctx, const si_laplacian* lap,
         si::DevBuf<double> vi((size_t)lap->L.n * k);
         si::rows_to_internal(lap->L, evecs, k, vi.get(), k, k, ctx->c.stream);
         si::fp64_residuals(lap->L, vi, k, evals, res.data(), ctx->c.stream);
-        double q = 0.0;
-        for (double r : res) q = (std::isnan(q) || std::isnan(r)) ? NAN : std::max(q, r);
-        *out = q;
+        *out = si::max_residual_of(res.data(), k);
     });
 }
 
@@ -569,6 +567,8 @@ si_status si_lobpcg_solve(si_context* ctx, const si_laplacian* lap, int32_t n_co
                           double* evecs, si_report* rep) {
     if (!ctx || !lap || !evals || !evecs) return SI_ERR_INVALID_ARGUMENT;
     return guard(ctx, [&] {
+        SI_CHECK(n_components >= 0 && n_components + 1 <= 64, si::kErrUnsupported,
+                 "n_components above 63 is not supported");
         si::SolveStats st;
         const int k = n_components + 1;
         double* vi = si::ws_get<double>(si::kWsChainVec, (size_t)lap->L.n * k);
@@ -579,7 +579,7 @@ si_status si_lobpcg_solve(si_context* ctx, const si_laplacian* lap, int32_t n_co
         if (ok) {
             std::vector<double> res(k);
             si::fp64_residuals(lap->L, vi, k, evals, res.data(), ctx->c.stream);
-            st.max_residual = *std::max_element(res.begin(), res.end());
+            st.max_residual = si::max_residual_of(res.data(), k);
             si::rows_to_external(lap->L, vi, k, evecs, k, k, ctx->c.stream);
         }
         const double ms = t.stop_ms();
@@ -604,7 +604,7 @@ si_status si_lobpcg_sinv_solve(si_context* ctx, const si_laplacian* lap, int32_t
         if (ok) {
             std::vector<double> res(k);
             si::fp64_residuals(lap->L, vi, k, evals, res.data(), ctx->c.stream);
-            st.max_residual = *std::max_element(res.begin(), res.end());
+            st.max_residual = si::max_residual_of(res.data(), k);
             si::rows_to_external(lap->L, vi, k, evecs, k, k, ctx->c.stream);
         }
         const double ms = t.stop_ms();
@@ -658,7 +658,7 @@ si_status si_rsvd_solve(si_context* ctx, const si_laplacian* lap, int32_t n_comp
         const double ms = t.stop_ms();
         std::vector<double> res(k);
         si::fp64_residuals(lap->L, vi, k, evals, res.data(), ctx->c.stream);
-        st.max_residual = *std::max_element(res.begin(), res.end());
+        st.max_residual = si::max_residual_of(res.data(), k);
         si::rows_to_external(lap->L, vi.get(), k, evecs, k, k, ctx->c.stream);
         SI_HIP(hipStreamSynchronize(ctx->c.stream));
         st.level = 4;
@@ -844,7 +844,9 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
                 for (int64_t node : mem) lookup[node] = -1;
                 csolved[c] = 1;
             };
-            if (batch) {
+            // the batched segment solver carries k = d + 1 <= 8 columns; wider
+            // embeddings take the per-component loop
+            if (batch && d <= 7) {
                 solve_components_batched(ctx->c, n, indptr, indices, data, labels, members, d, seed,
                                          mode, opt, cev, cvec, csolved, last, &lap_ms, &solve_ms);
             }

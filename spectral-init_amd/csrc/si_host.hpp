@@ -124,8 +124,18 @@ struct FastOptions {
     const double* warm_start = nullptr;  // device [n x k]
 };
 
-// Per-vector FP64 residuals ||L v - lambda v|| / max(||v||, eps) of device
-// vectors (lobpcg.rs:193-223, metrics.rs:57-92).
+// sqrt(rr) / max(sqrt(vv), 1e-300) from the squared norms rr = ||L v - lambda v||^2
+// and vv = ||v||^2 (metrics.rs:57-68); NaN when either is NaN.
+double residual_from_sums(double rr, double vv);
+// NaN-propagating maximum (max_eigenpair_residual's fold, metrics.rs:80-91).
+inline double max_residual_of(const double* r, int k) {
+    double q = 0.0;
+    for (int c = 0; c < k; ++c) q = (q != q || r[c] != r[c]) ? NAN : (r[c] > q ? r[c] : q);
+    return q;
+}
+
+// Per-vector FP64 residuals ||L v - lambda v|| / max(||v||, 1e-300) of device
+// vectors (metrics.rs:57-92; lobpcg.rs:193-223 batches the same sums).
 void fp64_residuals(const DevLaplacian& L, const double* v, int k, const double* evals_host,
                     double* res_host, hipStream_t s);
 

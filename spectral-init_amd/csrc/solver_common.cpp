@@ -53,8 +53,21 @@ void workspace_release() {
     }
 }
 
-// ||L v_j - lambda_j v_j|| / max(||v_j||, eps): one block SpMM, one residual
-// block, two FP64 Grams (per_vector_residuals, lobpcg.rs:193-223).
+// eigenpair_residual (metrics.rs:57-68) of column c from the FP64 Gram
+// diagonals rr = ||L v - lambda v||^2 and vv = ||v||^2: the norm is floored at
+// 1e-300 as there, and a NaN in either sum (a NaN entry of v, L v or lambda)
+// gives a NaN residual, which max_eigenpair_residual (metrics.rs:73-92)
+// propagates.  rr and vv are sums of squares, never negative unless NaN.
+double residual_from_sums(double rr, double vv) {
+    if (std::isnan(rr) || std::isnan(vv)) return NAN;
+    double vn = std::sqrt(vv);
+    if (vn < 1e-300) vn = 1e-300;
+    return std::sqrt(rr) / vn;
+}
+
+// ||L v_j - lambda_j v_j|| / max(||v_j||, 1e-300): one block SpMM, one residual
+// block, two FP64 Grams (metrics.rs:57-68, batched over the k columns as
+// per_vector_residuals does, lobpcg.rs:193-223).
 void fp64_residuals(const DevLaplacian& L, const double* v, int k, const double* evals_host,
                     double* res_host, hipStream_t s) {
     const int64_t n = L.n;
@@ -72,11 +85,7 @@ void fp64_residuals(const DevLaplacian& L, const double* v, int k, const double*
     std::vector<double> hg((size_t)k * k), hx((size_t)k * k);
     to_host(hg.data(), g, hg.size(), s);
     to_host(hx.data(), gx, hx.size(), s);
-    for (int c = 0; c < k; ++c) {
-        double vn = std::sqrt(std::fmax(hx[c * k + c], 0.0));
-        if (vn < 2.220446049250313e-16) vn = 2.220446049250313e-16;
-        res_host[c] = std::sqrt(std::fmax(hg[c * k + c], 0.0)) / vn;
-    }
+    for (int c = 0; c < k; ++c) res_host[c] = residual_from_sums(hg[c * k + c], hx[c * k + c]);
 }
 
 // rayleigh_ritz_refine (lobpcg.rs:158-187): AX = L X, G = X^T AX, eigh (lower

@@ -209,22 +209,41 @@ inline void rr_update_t(int64_t n, const double* y, const double* ly, int ld, in
     rr_update_f64(n, y, ly, ld, m, c, th, k, x, ax, norms2, ws, s);
 }
 
+// residual norms of the first k columns: ||ax - x theta|| / max(||x||, 1e-300)
+// with theta = sm.w on the device (r: an n x k scratch block); the squared
+// norms come back as Gram diagonals and NaN propagates (metrics.rs:57-92)
+template <typename T>
+void residual_norms(int64_t n, const T* ax, const T* x, int ld, int k, T* r, Small& sm,
+                    double* ws, double* res_host, hipStream_t s) {
+    residual_t(n, ax, x, ld, k, sm.w, r, k, s);
+    gram_t(n, r, k, k, r, k, k, sm.g, ws, s);
+    gram_t(n, x, ld, k, x, ld, k, sm.g2, ws, s);
+    std::vector<double> hg((size_t)k * k), hx((size_t)k * k);
+    to_host(hg.data(), sm.g, hg.size(), s);
+    to_host(hx.data(), sm.g2, hx.size(), s);
+    for (int c = 0; c < k; ++c) res_host[c] = residual_from_sums(hg[c * k + c], hx[c * k + c]);
+}
+
 // Fused Rayleigh-Ritz of the first w columns of y (leading dimension ld):
 //   ly = L y;  G = y^T [y | ly] (one pass);  (y^T ly) c = theta (y^T y) c with
 //   diagonal scaling (small, on the device);  x = y C, ax = ly C and the
-//   wanted residual norms in one pass;  one host synchronisation.
-// Returns false when y^T y is not numerically positive definite.
+//   wanted residual norms in one pass;  one host synchronisation.  The update
+//   kernel accumulates the residual sums of up to 8 columns; for k > 8 the
+//   norms take one extra residual pass through r (an n x k scratch block).
+// Returns false when y^T y is not numerically positive definite or 2w > 64.
 template <typename T>
 bool fused_rr(const DevLaplacian& L, int64_t n, T* y, T* ly, int w, int ld, int k, T* x, T* ax,
               Small& sm, double* gws, double* rr_ws, double* theta_host, double* res_host,
-              int64_t& spmm_calls, hipStream_t s) {
-    if (2 * w > 64 || k > 8) return false;  // the one-pass Gram covers up to 64 columns
+              int64_t& spmm_calls, hipStream_t s, T* r = nullptr) {
+    if (2 * w > 64 || k > w) return false;  // the one-pass Gram covers up to 64 columns
+    if (k > 8 && r == nullptr) return false;
+    const int kn = std::min(k, 8);
     spmm_t_ld(L, y, ly, w, ld, s);
     ++spmm_calls;
     double* g = sm.g;  // w x 2w, row stride 2w (fits: g, g2 are contiguous m*m blocks)
     gram2_t(n, y, ld, w, ly, g, gws, s);
     small_gen_eig_scaled(w, g + w, 2 * w, g, 2 * w, sm.w, sm.v, sm.status, s);
-    rr_update_t(n, y, ly, ld, w, sm.v, sm.w, k, x, ax, sm.res, rr_ws, s);
+    rr_update_t(n, y, ly, ld, w, sm.v, sm.w, kn, x, ax, sm.res, rr_ws, s);
     // one download into page-locked memory: Ritz values, norms and status
     double* h = static_cast<double*>(pinned_stage(sm.tail_bytes()));
     SI_HIP(hipMemcpyAsync(h, sm.w, sm.tail_bytes(), hipMemcpyDeviceToHost, s));
@@ -233,24 +252,13 @@ bool fused_rr(const DevLaplacian& L, int64_t n, T* y, T* ly, int w, int ld, int 
     std::memcpy(&st, h + 4 * sm.m + 16, sizeof(int));
     if (st != 0) return false;
     for (int j = 0; j < w; ++j) theta_host[j] = h[j];
-    const double* nrm = h + 2 * sm.m;  // sm.res
-    for (int j = 0; j < k; ++j) {
-        double xn = std::sqrt(std::fmax(nrm[k + j], 0.0));
-        if (xn < 2.220446049250313e-16) xn = 2.220446049250313e-16;
-        res_host[j] = std::sqrt(std::fmax(nrm[j], 0.0)) / xn;
+    if (k <= 8) {
+        const double* nrm = h + 2 * sm.m;  // sm.res: k residual sums, then k norm sums
+        for (int j = 0; j < k; ++j) res_host[j] = residual_from_sums(nrm[j], nrm[k + j]);
+    } else {
+        residual_norms<T>(n, ax, x, ld, k, r, sm, gws, res_host, s);
     }
     return true;
-}
-
-// residual norms of the first k columns: ||ax - x theta|| / ||x||
-template <typename T>
-void residual_norms(int64_t n, const T* ax, const T* x, int m, int k, T* r, Small& sm,
-                    double* ws, double* res_host, hipStream_t s) {
-    residual_t(n, ax, x, m, k, sm.w, r, k, s);
-    gram_t(n, r, k, k, r, k, k, sm.g, ws, s);
-    gram_t(n, x, m, k, x, m, k, sm.g2, ws, s);
-    small_residual_norms(k, sm.g, k, sm.g2, k, sm.res, s);
-    to_host(res_host, sm.res, k, s);
 }
 
 }  // namespace
@@ -263,6 +271,9 @@ bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const Fa
     if (n_components <= 0 || k >= n) return false;
     const int m = choose_block(n, k, opt.block_size);
     if (m >= n) return false;
+    // the FP64 finish runs the one-pass Rayleigh-Ritz on mf = k + 5 columns
+    // (at most 32); wider problems go on to the next escalation level
+    if (std::min(m, std::max(8, k + 5)) > 32) return false;
     st.block_size = m;
     const int degree = opt.cheb_degree > 0 ? opt.cheb_degree : 24;
     const size_t nm = (size_t)n * m;
@@ -312,7 +323,7 @@ bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const Fa
         // one fused Rayleigh-Ritz on the filtered block; the explicit
         // orthonormalisation path is the fallback when Y^T Y is too ill-conditioned
         if (!fused_rr<float>(L, n, y, ly, m, m, k, x32, ax32, sm, g2w, rr_ws, theta.data(),
-                             res.data(), st.spmm_calls, s)) {
+                             res.data(), st.spmm_calls, s, r32)) {
             if (!orthonormalise<float>(n, y, m, sm, ws, s)) {
                 gram_t(n, y, m, m, y, m, m, sm.g, ws, s);
                 small_inv_sqrt_diag(m, sm.g, sm.sc, s);
@@ -323,7 +334,8 @@ bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const Fa
             residual_norms<float>(n, ax32, x32, m, k, r32, sm, ws, res.data(), s);
         }
         ++st.iterations;
-        const double worst = *std::max_element(res.begin(), res.end());
+        const double worst = max_residual_of(res.data(), k);
+        if (std::isnan(worst)) break;  // the FP64 finish reports it; the chain escalates
         if (trace_on())
             std::fprintf(stderr, "[fast] fp32 round %d m=%d a=%.4g deg=%d worst=%.3e theta_k=%.8g theta_m=%.6g spmm=%lld\n",
                          round, m, a, degree_for(a), worst, theta[k - 1], theta[m - 1],
@@ -343,16 +355,16 @@ bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const Fa
     // guards suffice.
     const int mf = std::min(m, std::max(8, k + 5));
     double* ax64 = ws_get<double>(kWsFastAx64, nm);
-    (void)r32;
+    double* r64 = k > 8 ? ws_get<double>(kWsResR, (size_t)n * k) : nullptr;
     FilterBufs<double> fb64;
     fb64.b0 = ws_get<double>(kWsFastF64a, nm);
     fb64.b1 = ws_get<double>(kWsFastF64b, nm);
     fb64.b2 = ws_get<double>(kWsFastF64c, nm);
     convert_f32_to_f64(x32, fb64.b0, (int64_t)nm, s);
     if (!fused_rr<double>(L, n, fb64.b0, fb64.b1, mf, m, k, x64, ax64, sm, g2w, rr_ws,
-                          theta.data(), res.data(), st.spmm_calls, s))
+                          theta.data(), res.data(), st.spmm_calls, s, r64))
         return false;
-    double worst = *std::max_element(res.begin(), res.end());
+    double worst = max_residual_of(res.data(), k);
     if (trace_on())
         std::fprintf(stderr, "[fast] fp64 rr mf=%d worst=%.3e spmm=%lld\n", mf, worst,
                      (long long)st.spmm_calls);
@@ -362,10 +374,10 @@ bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const Fa
                                         ax64);
         double* ly = (y == fb64.b0) ? fb64.b1 : fb64.b0;
         if (!fused_rr<double>(L, n, y, ly, mf, m, k, x64, ax64, sm, g2w, rr_ws, theta.data(),
-                              res.data(), st.spmm_calls, s))
+                              res.data(), st.spmm_calls, s, r64))
             break;
         ++st.iterations;
-        worst = *std::max_element(res.begin(), res.end());
+        worst = max_residual_of(res.data(), k);
         if (trace_on())
             std::fprintf(stderr, "[fast] fp64 round %d mf=%d worst=%.3e spmm=%lld\n", extra, mf,
                          worst, (long long)st.spmm_calls);

@@ -73,6 +73,11 @@ bool lobpcg_core(int64_t n, const BlockOperator& op, double* x, int m, double to
                  double* lam_h, double* rnorm_h, int64_t& iters, int64_t& op_calls,
                  hipStream_t s, bool robust) {
     const int s3 = 3 * m;
+    // the Rayleigh-Ritz problems on span[X, R] (and P) are solved in one
+    // workgroup, at most kSmallMax wide: blocks wider than half of that are
+    // not run here (the escalation chain moves on), and P is left out of an
+    // iteration whose span[X, R, P] would not fit
+    if (m > kSmallMax / 2) return false;
     PlainWork W(n, m);
     upload_eye(W.eye, m, s);
     double* X = W.S;
@@ -167,7 +172,7 @@ bool lobpcg_core(int64_t n, const BlockOperator& op, double* x, int m, double to
         block_mul_f64(n, Ra, s3, na, W.t1, na, nullptr, 0, 0, nullptr, Ra, s3, s);
         apply(Ra, ARa, na);
         bool use_p = false;
-        if (has_p) {
+        if (has_p && m + 2 * na <= kSmallMax) {
             double* Pa = X + m + na;
             double* APa = AX + m + na;
             block_mul_f64(n, W.P, m, m, W.sel, na, nullptr, 0, 0, nullptr, Pa, s3, s);
