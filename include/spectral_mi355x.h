@@ -137,6 +137,18 @@ si_status si_max_eigenpair_residual(si_context* ctx, const si_laplacian* lap,
                                     const double* evals, const double* evecs, int32_t k,
                                     double* out);
 
+/* The same residual as si_max_eigenpair_residual, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  It has no counterpart in the reference.
+ * The call makes no hipMalloc / hipFree: its partial sums live in a static
+ * device array of 1 MiB + 1 KiB per GPU, held until the process ends and
+ * shared by every call on that GPU, so calls that overlap on one GPU race.
+ * evals (host, k), evecs (device, n x k row-major, the caller's node order),
+ * out (host); k <= 64, larger k is rejected with SI_ERR_INVALID_ARGUMENT. */
+si_status si_max_eigenpair_residual_noalloc(si_context* ctx, const si_laplacian* lap,
+                                            const double* evals, const double* evecs, int32_t k,
+                                            double* out);
+
 /* lobpcg_solve (src/solvers/lobpcg.rs:227-465) on the device, FP64, with the
  * reference constants; tol as LOBPCG_ACCEPT_TOL.  evals (host, k =
  * n_components+1), evecs (device, n x k).  Returns SI_ERR_CONVERGENCE for

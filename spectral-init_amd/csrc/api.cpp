@@ -352,6 +352,8 @@ si_status si_context_create(int device, si_context** out) {
         hipDeviceProp_t prop;
         SI_HIP(hipGetDeviceProperties(&prop, device));
         ctx->c.num_cus = prop.multiProcessorCount;
+        // the allocation-free residual's static scratch: its address on this device
+        si::residual_noalloc_setup(device);
     });
     if (st != SI_OK) {
         delete ctx;

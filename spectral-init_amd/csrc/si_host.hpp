@@ -134,6 +134,10 @@ inline double max_residual_of(const double* r, int k) {
     return q;
 }
 
+// Looks up, once per device and with that device current, the device address
+// of the static scratch array of si_max_eigenpair_residual_noalloc.
+void residual_noalloc_setup(int device);
+
 // Per-vector FP64 residuals ||L v - lambda v|| / max(||v||, 1e-300) of device
 // vectors (metrics.rs:57-92; lobpcg.rs:193-223 batches the same sums).
 void fp64_residuals(const DevLaplacian& L, const double* v, int k, const double* evals_host,

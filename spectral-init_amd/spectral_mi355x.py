@@ -36,6 +36,7 @@ EXPORTED = [
     "si_last_error", "si_laplacian_build", "si_laplacian_build_ex", "si_laplacian_destroy",
     "si_laplacian_info",
     "si_laplacian_export", "si_spmm", "si_chebyshev_filter", "si_max_eigenpair_residual",
+    "si_max_eigenpair_residual_noalloc",
     "si_lobpcg_solve", "si_lobpcg_sinv_solve", "si_solve_eigenproblem", "si_rsvd_solve",
     "si_dense_evd",
     "si_spectral_init",
@@ -187,6 +188,7 @@ def lib():
         L.si_spmm.argtypes = [vp, vp, i32, vp, vp, i32, i32]
         L.si_chebyshev_filter.argtypes = [vp, vp, vp, i32, dbl, dbl, i32, vp]
         L.si_max_eigenpair_residual.argtypes = [vp, vp, vp, vp, i32, C.POINTER(dbl)]
+        L.si_max_eigenpair_residual_noalloc.argtypes = [vp, vp, vp, vp, i32, C.POINTER(dbl)]
         L.si_lobpcg_solve.argtypes = [vp, vp, i32, u64, i32, dbl, vp, vp, C.POINTER(_Report)]
         L.si_lobpcg_sinv_solve.argtypes = [vp, vp, i32, u64, vp, vp, C.POINTER(_Report)]
         L.si_solve_eigenproblem.argtypes = [vp, vp, i32, u64, C.POINTER(_Config), vp, vp,
@@ -355,6 +357,17 @@ def max_eigenpair_residual(lap: Laplacian, evals, evecs) -> float:
     out = C.c_double()
     lap.ctx._check(lib().si_max_eigenpair_residual(lap.ctx.handle, lap.handle, _ptr(ev),
                                                    _ptr(evecs), evecs.shape[1], C.byref(out)))
+    return out.value
+
+
+def max_eigenpair_residual_noalloc(lap: Laplacian, evals, evecs, k: Optional[int] = None) -> float:
+    """The same residual through the entry point that allocates no device
+    memory (its scratch is a static device array; overlapping calls race)."""
+    ev = np.ascontiguousarray(evals, np.float64)
+    out = C.c_double()
+    kk = evecs.shape[1] if k is None else k
+    lap.ctx._check(lib().si_max_eigenpair_residual_noalloc(lap.ctx.handle, lap.handle, _ptr(ev),
+                                                           _ptr(evecs), kk, C.byref(out)))
     return out.value
 
 
