@@ -245,7 +245,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--degree", type=int, default=0, help="max filter degree per round, 0 = automatic")
-    ap.add_argument("--plain-runs", type=int, default=1)
+    ap.add_argument("--plain-runs", type=int, default=10,
+                    help="timed plain GPU LOBPCG runs (the median is the speed-up's denominator)")
+    ap.add_argument("--plain-warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmm-only", action="store_true",
@@ -398,6 +400,11 @@ def main():
 
     plain = []
     plain_iters = None
+    for i in range(args.plain_warmup if args.plain_runs else 0):
+        try:
+            si.lobpcg_solve(lap, args.n_components, 42, tol=args.tol)
+        except si.ConvergenceFailure:
+            pass
     for i in range(args.plain_runs):
         torch.cuda.synchronize()
         t = time.perf_counter()
@@ -460,6 +467,21 @@ def main():
         log(f"CPU oracle on {ncores} cores: {cpu_all['value']} {cpu_all['unit']}")
 
     traffic = read_traffic(args.traffic, args.workload, last.block_size)
+    # the unit of the roofline: one application of L to an n x b block, FP32
+    # value + int32 column per stored entry, int32 row pointers, the block read
+    # once and written once (diagonal folded in the epilogue: stored = nnz(W));
+    # the Chebyshev recurrence's epilogue also reads y_{j-1} (a third n x b
+    # stream), reported beside it
+    b = last.block_size
+    unit_bytes = nnz * 8.0 + 4.0 * (n + 1) + 2.0 * 4.0 * n * b
+    unit_gbs = unit_bytes / (spmm_avg_us * 1e-6) / 1e9 if sp["launches_f32"] else None
+    # the ceiling of the gathers on this graph: the SpMM's 64-byte row gathers
+    # alone, along its column stream and at random (one 128-byte line each)
+    probe = None
+    if b == 16:
+        probe = {m: round(si.gather_probe(lap, i, 20), 2)
+                 for i, m in enumerate(("column_stream_us", "random_rows_us", "random_rows_2MiB_us"))}
+        log(f"gather probe: {probe}")
 
     gbuild = None
     if not args.no_graph_builder and args.workload in ("gmm1m", "gmm100k", "blobs2k"):
@@ -490,13 +512,19 @@ def main():
                         "step_ms": [round(t, 3) for t in times]},
         "laplacian_build_ms": round(lap_ms, 2),
         "roofline": {"bound": "hbm", "kernel": "k_spmm_wide<float> (FP32 Chebyshev sweeps)",
-                     "achieved": round(spmm_gbs, 1) if spmm_gbs else None,
+                     "achieved": round(unit_gbs, 1) if unit_gbs else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(spmm_gbs / HBM_PEAK_GBS, 4) if spmm_gbs else None,
+                     "frac": round(unit_gbs / HBM_PEAK_GBS, 4) if unit_gbs else None,
                      "traffic": traffic,
-                     "algorithmic_bytes_per_launch": round(bytes_per_launch),
+                     "algorithmic_bytes_per_launch": round(unit_bytes),
+                     "bytes_formula": "nnz*(4+4) + 4*(n+1) + 2*4*n*b (b = %d)" % b,
                      "avg_launch_us": round(spmm_avg_us, 2),
-                     "launches": sp["launches_f32"]},
+                     "launches": sp["launches_f32"],
+                     "with_recurrence_stream": {
+                         "bytes_per_launch": round(bytes_per_launch),
+                         "achieved": round(spmm_gbs, 1) if spmm_gbs else None,
+                         "frac": round(spmm_gbs / HBM_PEAK_GBS, 4) if spmm_gbs else None},
+                     "gather_probe": probe},
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all if not args.no_cpu else None,
         "graph_builder": gbuild,

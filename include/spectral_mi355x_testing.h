@@ -69,6 +69,16 @@ typedef struct {
 si_status si_profile_spmm_begin(si_context* ctx);
 si_status si_profile_spmm_end(si_context* ctx, si_spmm_profile* out);
 
+/* The SpMM's gather stream alone, as a measure of the rate the memory system
+ * gives it: one 64-byte FP32 row (16 columns) read per stored entry of the
+ * Laplacian, no values, row pointers or output rows.  mode 0 reads the rows
+ * the column array names (the SpMM's own gathers), mode 1 random rows over
+ * the same n rows, mode 2 random rows among the first 32768 (2 MiB, resident
+ * in every L2).  *us receives the average microseconds per launch over reps
+ * launches. */
+si_status si_profile_gather_probe(si_context* ctx, const si_laplacian* lap, int32_t mode,
+                                  int32_t reps, double* us);
+
 /* The locality order of si_laplacian_build (reverse Cuthill-McKee; perm[new] =
  * old) of a host CSR pattern, on `threads` host threads (0: as many as the
  * hardware allows, at most 16).  Host-only: needs no device. */

@@ -197,6 +197,12 @@ si_status si_profile_spmm_end(si_context* ctx, si_spmm_profile* out) {
     });
 }
 
+si_status si_profile_gather_probe(si_context* ctx, const si_laplacian* lap, int32_t mode,
+                                  int32_t reps, double* us) {
+    if (!ctx || !lap || !us) return SI_ERR_INVALID_ARGUMENT;
+    return tguard(ctx, [&] { *us = si::gather_probe_us(lap->L, mode, reps, ctx->c.stream); });
+}
+
 si_status si_test_batch_graphs(int32_t enable, int64_t* captured, int64_t* launches) {
     if (captured) *captured = si::g_last_batch.graphs_captured;
     if (launches) *launches = si::g_last_batch.graph_launches;

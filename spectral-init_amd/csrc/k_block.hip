@@ -282,11 +282,56 @@ __global__ __launch_bounds__(256) void k_block_mul(int64_t n, const T* __restric
     }
 }
 
+// Out[n x 16] = A[n x 16] C[16 x 16] in FP64 on v_mfma_f64_16x16x4f64.  Each
+// wavefront owns 16-row tiles (grid stride).  Lane (li = lane % 16, lk =
+// lane / 16) feeds A[r0 + li][4s + lk] and C[4s + lk][li] for the k-steps s =
+// 0..3; D[i][j] comes back in lane j + 16 (i % 4), element i / 4, so each
+// store instruction writes 4 whole 128-byte rows.  A wavefront reads all of
+// its tile before it writes it, and no other wavefront touches those rows:
+// the product may overwrite A (out == a).
+__global__ __launch_bounds__(256) void k_block_mul16_mfma(int64_t n, const double* a, int lda,
+                                                          const double* __restrict__ c,
+                                                          double* out, int ldo) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    double cb[4];
+#pragma unroll
+    for (int st = 0; st < 4; ++st) cb[st] = c[(4 * st + lk) * 16 + li];
+    const int64_t tiles = (n + 15) / 16;
+    for (int64_t t = (int64_t)blockIdx.x * 4 + wave; t < tiles; t += (int64_t)gridDim.x * 4) {
+        const int64_t r0 = t * 16;
+        const int64_t ra = r0 + li;
+        double av[4];
+#pragma unroll
+        for (int st = 0; st < 4; ++st) av[st] = ra < n ? a[ra * lda + 4 * st + lk] : 0.0;
+        v4d acc = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int st = 0; st < 4; ++st)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[st], cb[st], acc, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t row = r0 + lk + 4 * v;
+            if (row < n) out[row * ldo + li] = acc[v];
+        }
+    }
+}
+
 template <typename T>
 void block_mul_impl(int64_t n, const T* a, int lda, int p, const double* c, int q, const T* b,
                     int ldb, int p2, const double* d, T* out, int ldo, hipStream_t s) {
     SI_CHECK(p <= 64 && q <= 64 && p2 <= 64, kErrInvalidArgument, "block_mul: width > 64");
     SI_CHECK(out != b || b == nullptr, kErrInvalidArgument, "block_mul: out aliases the second term");
+    if constexpr (sizeof(T) == 8) {
+        if (p == 16 && q == 16 && b == nullptr) {  // the 16-wide TS-GEMM on the FP64 matrix cores
+            int64_t blocks = ceil_div(ceil_div(n, (int64_t)16), (int64_t)4);
+            if (blocks > 2048) blocks = 2048;
+            if (blocks < 1) blocks = 1;
+            hipLaunchKernelGGL(k_block_mul16_mfma, dim3(blocks), dim3(256), 0, s, n,
+                               (const double*)a, lda, c, (double*)out, ldo);
+            SI_LAUNCH_CHECK();
+            return;
+        }
+    }
     const int64_t blocks = ceil_div(n, sizeof(T) == 8 ? 32 : 64);
     if (q <= 4)
         hipLaunchKernelGGL((k_block_mul<T, 4>), dim3(blocks), dim3(256), 0, s, n, a, lda, p, c, q, b, ldb, p2, d, out, ldo);

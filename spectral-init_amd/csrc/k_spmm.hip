@@ -412,7 +412,101 @@ void spmm_impl(const DevLaplacian& L, const T* val, const T* x, T* out, int k, i
     }
 }
 
+// -------------------------------------------------------------- gather probe
+// The SpMM's gather stream alone, to measure the rate the memory system gives
+// it: a quad of lanes per stored entry p reads the 64-byte FP32 row c of a
+// 16-column block (16 B per lane), 4 entries per quad in flight, grid-stride
+// over the entries; each lane sums what it reads and writes one float4.
+//   mode 0: c = col[p], the Laplacian's own column stream (no values, row
+//           pointers or output rows);
+//   mode 1: c = a hash of (p, col[p]) over all n rows (no locality, the same
+//           footprint);
+//   mode 2: the same hash over the first min(n, 32768) rows (a 2 MiB footprint
+//           that every XCD's L2 holds).
+// Every c is in [0, n); out holds gridDim.x * 256 float4.
+__device__ inline uint32_t probe_hash(uint32_t v) {
+    v ^= v >> 16; v *= 0x7feb352dU; v ^= v >> 15; v *= 0x846ca68bU; v ^= v >> 16;
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_gather_probe(const int32_t* __restrict__ col, int64_t count,
+                                                      int64_t n, int mode, const float* __restrict__ x,
+                                                      float4* __restrict__ out) {
+    constexpr int U = 4;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int q = threadIdx.x & 3;
+    const int64_t g = t >> 2;
+    const int64_t stride = ((int64_t)gridDim.x * 256) >> 2;
+    const uint32_t span = mode == 2 ? (uint32_t)(n < 32768 ? n : 32768) : (uint32_t)n;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int64_t p0 = g; p0 < count; p0 += stride * U) {
+        int64_t cc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t p = p0 + u * stride;
+            int64_t c = -1;
+            if (p < count) {
+                c = col[p];
+                if (mode != 0) c = probe_hash((uint32_t)p * 2654435761u ^ (uint32_t)c) % span;
+            }
+            cc[u] = c;
+        }
+        float4 xs[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            xs[u] = cc[u] >= 0 ? *reinterpret_cast<const float4*>(x + cc[u] * 16 + q * 4)
+                               : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            acc.x += xs[u].x; acc.y += xs[u].y; acc.z += xs[u].z; acc.w += xs[u].w;
+        }
+    }
+    out[t] = acc;
+}
+
 }  // namespace
+
+double gather_probe_us(const DevLaplacian& L, int mode, int reps, hipStream_t s) {
+    SI_CHECK(mode >= 0 && mode <= 2, kErrInvalidArgument, "gather probe: mode 0, 1 or 2");
+    SI_CHECK(reps >= 1, kErrInvalidArgument, "gather probe: reps >= 1");
+    SI_CHECK(L.n > 0 && L.nnz > 0, kErrInvalidArgument, "gather probe: empty Laplacian");
+    const unsigned grid = 1024;
+    float* x = nullptr;
+    float4* out = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double us = 0.0;
+    try {
+        SI_HIP(hipMalloc(&x, sizeof(float) * 16 * (size_t)(L.n > 0 ? L.n : 1)));
+        SI_HIP(hipMalloc(&out, sizeof(float4) * grid * 256));
+        SI_HIP(hipMemsetAsync(x, 0, sizeof(float) * 16 * (size_t)L.n, s));
+        SI_HIP(hipEventCreate(&e0));
+        SI_HIP(hipEventCreate(&e1));
+        hipLaunchKernelGGL(k_gather_probe, dim3(grid), dim3(256), 0, s, (const int32_t*)L.col, L.nnz,
+                           L.n, mode, (const float*)x, out);  // warm
+        SI_LAUNCH_CHECK();
+        SI_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(k_gather_probe, dim3(grid), dim3(256), 0, s, (const int32_t*)L.col,
+                               L.nnz, L.n, mode, (const float*)x, out);
+        SI_LAUNCH_CHECK();
+        SI_HIP(hipEventRecord(e1, s));
+        SI_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SI_HIP(hipEventElapsedTime(&ms, e0, e1));
+        us = 1e3 * ms / reps;
+    } catch (...) {
+        if (e0) hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+        if (x) hipFree(x);
+        if (out) hipFree(out);
+        throw;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(x);
+    hipFree(out);
+    return us;
+}
 
 void spmm_profile_attach(SpmmProfile* p) { g_profile = p; }
 

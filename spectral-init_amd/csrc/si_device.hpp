@@ -95,6 +95,10 @@ bool spmm_profiling();
 // candidate on it (hipEvents, a few launches each); sets L.shape16/32, L.tuned.
 void spmm_autotune(DevLaplacian& L, hipStream_t s);
 int spmm_shape_count();  // candidates per width
+// Average microseconds per launch of the gather probe (the SpMM's 64-byte row
+// gathers alone; mode 0 along the column stream, 1 random rows, 2 random rows
+// in a 2 MiB footprint), over `reps` launches; see k_spmm.hip.
+double gather_probe_us(const DevLaplacian& L, int mode, int reps, hipStream_t s);
 
 // Optional live timing of every SpMM launch (hipEvents on the launching
 // stream), used by the benchmark to measure the dominant kernel inside a

@@ -1,96 +1,10 @@
-// solver_misc.cpp -- randomized SVD, small dense EVD and the escalation chain.
+// solver_misc.cpp -- small dense EVD and the escalation chain.
 #include <algorithm>
 #include <cmath>
 
 #include "si_host.hpp"
 
 namespace si {
-
-namespace {
-
-// Q <- orthonormal basis of span(Y) by SVQB, applied twice (FP64).
-void svqb_orth(int64_t n, double* y, int k, DevBuf<double>& g, DevBuf<double>& w,
-               DevBuf<double>& u, DevBuf<double>& t, double* ws, hipStream_t s) {
-    for (int pass = 0; pass < 2; ++pass) {
-        gram_f64(n, y, k, k, y, k, k, g, ws, s);
-        small_sym_eig(k, g, k, w, u, s);
-        small_svqb(k, w, u, t, 1e-28, s);
-        block_mul_f64(n, y, k, k, t, k, nullptr, 0, 0, nullptr, y, k, s);
-    }
-}
-
-}  // namespace
-
-// rsvd_solve (src/solvers/rsvd.rs:135-213).  M = 2I - L is applied through the
-// SpMM epilogue (s_l = -1, s_x = 2), so no second matrix is stored.  The thin
-// Q factors come from SVQB (the reference uses Householder QR; both give an
-// orthonormal basis of the same span, which is all Steps C-G use).
-void rsvd_solve_device(const DevLaplacian& L, int n_components, uint64_t seed, int oversampling,
-                       int nbiter, double* evals_host, double* evecs_dev, SolveStats& st,
-                       hipStream_t s) {
-    const int64_t n = L.n;
-    const int rank = n_components + 1;
-    SI_CHECK(n_components > 0 && n_components < n, kErrInvalidArgument,
-             "rsvd_solve: need 0 < n_components < n");
-    int64_t ksub;
-    if (oversampling < 0) {  // rsvd_k_sub (rsvd.rs:84-87)
-        int64_t os = std::max<int64_t>(n / 10, std::max(rank, 5));
-        os = std::min<int64_t>(os, n > rank ? n - rank : 0);
-        ksub = rank + os;
-    } else {
-        ksub = rank + oversampling;
-    }
-    ksub = std::min<int64_t>(ksub, n);
-    // device envelope: the sketch width is capped at 64 columns
-    ksub = std::min<int64_t>(ksub, kSmallMax);
-    SI_CHECK(ksub >= rank, kErrUnsupported, "rsvd_solve: sketch narrower than the rank");
-    const int k = (int)ksub;
-    st = SolveStats{};
-    st.block_size = k;
-    const size_t nk = (size_t)n * k;
-    DevBuf<double> om(nk), y(nk), g((size_t)k * k), w(k), u((size_t)k * k), t((size_t)k * k),
-        ws(gram_workspace_doubles(n, k, k)), sel((size_t)k * rank);
-    auto apply_m = [&](const double* in, double* out) {
-        spmm_f64(L, in, out, k, k, -1.0, 2.0, 0.0, nullptr, s);  // (2I - L) x
-        ++st.spmm_calls;
-    };
-    fill_normal_f64(om, n, k, k, seed, kStreamRsvdOmega, s, L.perm);
-    for (int it = 0; it < nbiter; ++it) {
-        apply_m(om, y);
-        svqb_orth(n, y, k, g, w, u, t, ws, s);
-        apply_m(y, om);
-        svqb_orth(n, om, k, g, w, u, t, ws, s);
-    }
-    apply_m(om, y);
-    svqb_orth(n, y, k, g, w, u, t, ws, s);   // Q
-    apply_m(y, om);                           // M Q
-    gram_f64(n, y, k, k, om, k, k, g, ws, s); // B = Q^T M Q
-    small_sym_eig(k, g, k, w, u, s);          // ascending
-    std::vector<double> hw(k);
-    to_host(hw.data(), w.get(), k, s);
-    // trivial pair = largest theta, then the next n_components (rsvd.rs:190-212)
-    std::vector<double> hs((size_t)k * rank, 0.0);
-    std::vector<int> cols(rank);
-    cols[0] = k - 1;
-    for (int c = 0; c < n_components; ++c) cols[c + 1] = std::max(0, k - 2 - c);
-    for (int c = 0; c < rank; ++c) {
-        hs[(size_t)cols[c] * rank + c] = 1.0;
-        evals_host[c] = 2.0 - hw[cols[c]];
-    }
-    to_device(sel.get(), hs.data(), hs.size(), s);
-    // V = Q (U S): combine the eigenvector rotation and the column selection
-    DevBuf<double> us((size_t)k * rank);
-    {
-        std::vector<double> hu((size_t)k * k), hus((size_t)k * rank, 0.0);
-        to_host(hu.data(), u.get(), hu.size(), s);
-        for (int r = 0; r < k; ++r)
-            for (int c = 0; c < rank; ++c) hus[(size_t)r * rank + c] = hu[(size_t)r * k + cols[c]];
-        to_device(us.get(), hus.data(), hus.size(), s);
-    }
-    block_mul_f64(n, y, k, k, us, rank, nullptr, 0, 0, nullptr, evecs_dev, rank, s);
-    SI_HIP(hipStreamSynchronize(s));
-    st.iterations = nbiter;
-}
 
 // dense_evd (src/solvers/dense.rs:9-46) for n <= 64: densify, Jacobi, keep the
 // k smallest.

@@ -43,7 +43,7 @@ EXPORTED = [
     "si_test_gram", "si_test_block_mul", "si_test_small_sym_eig", "si_test_small_gen_eig",
     "si_test_fill_normal", "si_test_fast_solve", "si_test_batch_graphs", "si_test_spmm_shapes",
     "si_test_band_solve", "si_test_rcm_order", "si_test_knn_path",
-    "si_profile_spmm_begin", "si_profile_spmm_end",
+    "si_profile_spmm_begin", "si_profile_spmm_end", "si_profile_gather_probe",
     "si_mpnn_param_count", "si_mpnn_create", "si_mpnn_destroy", "si_mpnn_get_params",
     "si_mpnn_set_params", "si_mpnn_features", "si_mpnn_forward", "si_mpnn_gradient",
     "si_mpnn_train_step", "si_mpnn_warm_start",
@@ -209,6 +209,7 @@ def lib():
         L.si_test_knn_path.argtypes = [i32, C.POINTER(i64)]
         L.si_profile_spmm_begin.argtypes = [vp]
         L.si_profile_spmm_end.argtypes = [vp, C.POINTER(_SpmmProfile)]
+        L.si_profile_gather_probe.argtypes = [vp, vp, i32, i32, C.POINTER(dbl)]
         for name in EXPORTED:
             if name not in ("si_version", "si_last_error", "si_config_default",
                             "si_context_destroy", "si_laplacian_destroy", "si_mpnn_destroy",
@@ -543,6 +544,16 @@ class SpmmProfiler:
         self.result = dict(launches=p.launches, bytes=p.bytes, ms=p.ms,
                            launches_f32=p.launches_f32, bytes_f32=p.bytes_f32, ms_f32=p.ms_f32)
         return False
+
+
+def gather_probe(lap: "Laplacian", mode: int, reps: int = 20) -> float:
+    """Microseconds per launch of the gather probe (mode 0: the SpMM's own
+    64-byte row gathers along the column stream; 1: random rows; 2: random rows
+    in a 2 MiB footprint)."""
+    us = C.c_double()
+    lap.ctx._check(lib().si_profile_gather_probe(lap.ctx.handle, lap.handle, int(mode), int(reps),
+                                                 C.byref(us)))
+    return us.value
 
 
 def batch_graphs(enable: Optional[bool] = None):

@@ -165,6 +165,60 @@ def test_rsvd_matches_oracle_same_sketch(ctx):
     assert ang[0] < 1e-6
 
 
+@pytest.mark.parametrize("name", ["blobs_connected_200", "blobs_connected_2000"])
+def test_rsvd_oversampling10_4iter_matches_oracle(ctx, name):
+    # the benchmark's comparison settings (oversampling 10 -> a 13-column sketch
+    # stored 16 wide, 4 power iterations) against the oracle with the same sketch
+    print(f"case rsvd p=10 q=4 {name}", flush=True)
+    ip, ix, w, z = _graph(name)
+    lap = si.Laplacian(ctx, ip, ix, w)
+    L, _, _ = O.laplacian_from_graph(ip, ix, w)
+    ev, vec, rep = si.rsvd_solve(lap, 2, 42, oversampling=10, nbiter=4)
+    oev, ovec = O.rsvd_solve(L, 2, 42, oversampling=10, nbiter=4)
+    assert rep.block_size == 13
+    np.testing.assert_allclose(ev, oev, atol=1e-9)
+    assert principal_angles(vec.cpu().numpy(), ovec)[0] < 1e-6
+    assert rep.max_residual == pytest.approx(O.max_eigenpair_residual(L, oev, ovec), rel=1e-6)
+
+
+def test_rsvd_reference_sketch_beyond_64_columns(ctx):
+    # rsvd_k_sub (rsvd.rs:84-87): rank + max(n/10, rank, 5) = 203 columns at
+    # n = 2000, with the reference's 2 power iterations (the chain's level 4)
+    print("case rsvd k_sub 203", flush=True)
+    ip, ix, w, z = _graph("blobs_connected_2000")
+    lap = si.Laplacian(ctx, ip, ix, w)
+    L, _, _ = O.laplacian_from_graph(ip, ix, w)
+    ev, vec, rep = si.rsvd_solve(lap, 2, 42)
+    assert rep.block_size == 203
+    oev, ovec = O.rsvd_solve(L, 2, 42)
+    np.testing.assert_allclose(ev, oev, atol=1e-9)
+    assert principal_angles(vec.cpu().numpy(), ovec)[0] < 1e-6
+
+
+def test_rsvd_eigenvector_distinctness(ctx):
+    # test_solver_invariants.rs:234-254: the returned eigenvectors are not collinear
+    ip, ix, w, z = _graph("blobs_5000")
+    lap = si.Laplacian(ctx, ip, ix, w)
+    ev, vec, rep = si.rsvd_solve(lap, 2, 42)
+    assert rep.block_size == 503
+    v = vec.cpu().numpy()
+    cos = abs(v[:, 0] @ v[:, 1]) / max(np.linalg.norm(v[:, 0]) * np.linalg.norm(v[:, 1]), 1e-300)
+    assert cos < 0.99
+    assert np.all(np.diff(ev) >= -1e-12) and np.isfinite(ev).all()
+
+
+def test_rsvd_one_spmm_launch_per_application(ctx):
+    # a 13-column sketch is applied as one 16-wide launch (not 8 + 4 + 1)
+    ip, ix, w, z = _graph("blobs_5000")
+    lap = si.Laplacian(ctx, ip, ix, w)
+    with si.SpmmProfiler(ctx) as p:
+        ev, vec, rep = si.rsvd_solve(lap, 2, 42, oversampling=10, nbiter=4)
+    # 4 iterations x 2 applications + Q + M Q inside the solve, then the
+    # FP64 residual's own application
+    assert rep.spmm_calls == 10
+    assert p.result["launches"] == rep.spmm_calls + 1
+
+
 def test_dense_evd_small_matches_numpy(ctx):
     ip, ix, w, z = _graph("blobs_50")
     lap = si.Laplacian(ctx, ip, ix, w)
