@@ -82,6 +82,14 @@ typedef struct {
     double eigenvalues[8];     /* first min(k, 8) eigenvalues */
     double solve_ms;           /* device time of the eigensolve */
     double laplacian_ms;       /* device time of the Laplacian build */
+    /* fields below are written only when struct_size covers them */
+    int32_t passes;            /* LOBPCG passes run: 1 + warm restarts of either kind (from
+                                  partial eigenvectors or after a failed residual check,
+                                  lobpcg.rs:315-450); 0 for the other solvers */
+    int32_t reserved;
+    double level_ms[6];        /* wall time of each escalation level tried, L0..L5, 0 for
+                                  levels not reached: the reference's [timing:level_N] lines
+                                  (solvers/mod.rs:76-223; also printed with SI_TRACE=1) */
 } si_report;
 
 typedef struct si_context si_context;

@@ -39,6 +39,11 @@ void fill_report(si_report* rep, const si::SolveStats& st, const double* ev, int
     rep->max_residual = st.max_residual;
     for (int i = 0; i < 8; ++i) rep->eigenvalues[i] = i < k ? ev[i] : 0.0;
     rep->solve_ms = ms;
+    if (rep->struct_size >= offsetof(si_report, level_ms) + sizeof(rep->level_ms)) {
+        rep->passes = st.passes;
+        rep->reserved = 0;
+        for (int l = 0; l < 6; ++l) rep->level_ms[l] = st.level_ms[l];
+    }
 }
 
 si::ChainOptions chain_options(const si_config* cfg) {

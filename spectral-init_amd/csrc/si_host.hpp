@@ -114,6 +114,8 @@ struct SolveStats {
     double max_residual = 0.0;
     bool converged = false;
     bool residual_is_fp64 = false;  // max_residual is the FP64 residual of the returned vectors
+    int passes = 0;                 // LOBPCG passes (1 + warm restarts)
+    double level_ms[6] = {0, 0, 0, 0, 0, 0};  // wall time per escalation level
 };
 
 struct FastOptions {

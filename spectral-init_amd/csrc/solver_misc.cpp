@@ -1,6 +1,8 @@
 // solver_misc.cpp -- small dense EVD and the escalation chain.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 
 #include "si_host.hpp"
 
@@ -40,18 +42,32 @@ int solve_eigenproblem_device(const DevLaplacian& L, int n_components, uint64_t 
     std::vector<double> res(k);
     auto quality = [&](const double* ev) {
         fp64_residuals(L, evecs_dev, k, ev, res.data(), s);
-        double q = 0.0;
-        for (double r : res) {
-            if (std::isnan(r) || std::isnan(q)) q = NAN;
-            else q = std::max(q, r);
-        }
-        return q;
+        return max_residual_of(res.data(), k);
+    };
+    // wall time of each level tried (every level ends with a host
+    // synchronisation), the reference's [timing:level_N] (solvers/mod.rs:76-223)
+    double lms[6] = {0, 0, 0, 0, 0, 0};
+    using clk = std::chrono::steady_clock;
+    clk::time_point t0 = clk::now();
+    auto lap_time = [&](int level) {
+        const clk::time_point t1 = clk::now();
+        lms[level] += std::chrono::duration<double, std::milli>(t1 - t0).count();
+        if (trace_on())
+            std::fprintf(stderr, "[timing:level_%d] %.3f ms\n", level, lms[level]);
+        t0 = t1;
+    };
+    auto finish = [&](int level, double q) {
+        st.level = level;
+        st.max_residual = q;
+        for (int l = 0; l < 6; ++l) st.level_ms[l] = lms[l];
+        return level;
     };
     const double gate_l1 = opt.tol;  // LOBPCG_QUALITY_THRESHOLD tracks the solver tolerance
     if (n < opt.dense_threshold && n <= kDenseMax && k <= n) {
         dense_evd_device(L, k, evals_host, evecs_dev, s);
         const double q = quality(evals_host);
-        if (q < 1e-6) { st.level = 0; st.max_residual = q; return 0; }
+        lap_time(0);
+        if (q < 1e-6) return finish(0, q);
     }
     bool have = false;
     if (opt.solver == 0) {
@@ -66,33 +82,48 @@ int solve_eigenproblem_device(const DevLaplacian& L, int n_components, uint64_t 
         // the FP32-bulk solver already measured the FP64 residual of exactly
         // these vectors against the exact Laplacian (metrics.rs:57-92 semantics)
         const double q = st.residual_is_fp64 ? st.max_residual : quality(evals_host);
-        if (q < gate_l1) { st.level = 1; st.max_residual = q; return 1; }
+        lap_time(1);
+        if (q < gate_l1) return finish(1, q);
+    } else {
+        lap_time(1);
     }
     SolveStats st2;
     if (lobpcg_sinv_solve_device(L, n_components, seed, evals_host, evecs_dev, st2, s)) {
         const double q = quality(evals_host);
         st.iterations += st2.iterations;
-        if (q < kSinvQualityThreshold) { st.level = 2; st.max_residual = q; return 2; }
+        st.passes += st2.passes;
+        lap_time(2);
+        if (q < kSinvQualityThreshold) return finish(2, q);
+    } else {
+        lap_time(2);
     }
     SolveStats st3;
     if (lobpcg_solve_plain(L, n_components, seed, true, opt.tol, evals_host, evecs_dev, st3, s)) {
         const double q = quality(evals_host);
         st.iterations += st3.iterations;
         st.spmm_calls += st3.spmm_calls;
-        if (q < gate_l1) { st.level = 3; st.max_residual = q; return 3; }
+        st.passes += st3.passes;
+        st.restarts += st3.restarts;
+        lap_time(3);
+        if (q < gate_l1) return finish(3, q);
+    } else {
+        lap_time(3);
     }
     SolveStats st4;
     rsvd_solve_device(L, n_components, seed, -1, 2, evals_host, evecs_dev, st4, s);
     {
         const double q = quality(evals_host);
-        if (q < 1e-2) { st.level = 4; st.max_residual = q; return 4; }
+        st.spmm_calls += st4.spmm_calls;
+        lap_time(4);
+        if (q < 1e-2) return finish(4, q);
     }
     if (n <= kDenseMax) {
         dense_evd_device(L, k, evals_host, evecs_dev, s);
-        st.level = 5;
-        st.max_residual = quality(evals_host);
-        return 5;
+        const double q = quality(evals_host);
+        lap_time(5);
+        return finish(5, q);
     }
+    for (int l = 0; l < 6; ++l) st.level_ms[l] = lms[l];
     throw Error(kErrConvergence, "eigensolver failed to converge after full escalation chain");
 }
 

@@ -279,6 +279,7 @@ bool lobpcg_solve_plain(const DevLaplacian& L, int n_components, uint64_t seed, 
     for (int restart = 0; restart <= 3; ++restart) {
         const int64_t cap = restart == 0 ? 300 : 1000;  // lobpcg.rs:89-97
         const int maxiter = (int)((5 * n) < cap ? 5 * n : cap);
+        ++st.passes;
         const bool conv = lobpcg_core_plain(L, shift, x, k, tol, maxiter, lam.data(), rn.data(),
                                             st.iterations, st.spmm_calls, s);
         bool accept = conv;

@@ -79,7 +79,8 @@ class _Report(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("level", C.c_int32), ("restarts", C.c_int32),
                 ("block_size", C.c_int32), ("iterations", C.c_int64), ("spmm_calls", C.c_int64),
                 ("max_residual", C.c_double), ("eigenvalues", C.c_double * 8),
-                ("solve_ms", C.c_double), ("laplacian_ms", C.c_double)]
+                ("solve_ms", C.c_double), ("laplacian_ms", C.c_double),
+                ("passes", C.c_int32), ("reserved", C.c_int32), ("level_ms", C.c_double * 6)]
 
 
 @dataclass
@@ -121,11 +122,14 @@ class Report:
     eigenvalues: list = field(default_factory=list)
     solve_ms: float = 0.0
     laplacian_ms: float = 0.0
+    passes: int = 0
+    level_ms: list = field(default_factory=list)
 
     @staticmethod
     def _from(r: _Report, k: int = 8) -> "Report":
         return Report(r.level, r.restarts, r.block_size, r.iterations, r.spmm_calls,
-                      r.max_residual, list(r.eigenvalues)[:k], r.solve_ms, r.laplacian_ms)
+                      r.max_residual, list(r.eigenvalues)[:k], r.solve_ms, r.laplacian_ms,
+                      r.passes, list(r.level_ms))
 
 
 class SpectralError(Exception):
