@@ -138,6 +138,59 @@ def bench_graph_builder(args, torch, graphs, si, ctx):
                 cpu_baseline=cpu)
 
 
+def bench_warm_start(args, torch, si, ctx, lap, cfg, fast_ms, plain):
+    """The committed warm-start network's block (spectral-init_amd/warmstart_mpnn.npy)
+    fed to the FP32-bulk solver and to the plain LOBPCG, against their random
+    starts; the network's own cost is timed separately."""
+    path = os.path.join(ROOT, "spectral-init_amd", "warmstart_mpnn.npy")
+    net = si.WarmStartNet(ctx, params=np.load(path))
+    net.warm_start(lap, seed=42)
+    ts = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        blk, used, rn, rr = net.warm_start(lap, seed=42)
+        torch.cuda.synchronize()
+        ts.append(1e3 * (time.perf_counter() - t))
+    net_ms = float(np.median(ts))
+    fcfg = si.SpectralInitConfig(tol=cfg.tol, block_size=cfg.block_size, cheb_degree=cfg.cheb_degree,
+                                 solver=si.Solver.Fast, warm_start=blk.data_ptr())
+    si.solve_eigenproblem(lap, 2, 42, fcfg)
+    ft, frep = [], None
+    for _ in range(5):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _, _, frep = si.solve_eigenproblem(lap, 2, 42, fcfg)
+        torch.cuda.synchronize()
+        ft.append(1e3 * (time.perf_counter() - t))
+    pcfg = si.SpectralInitConfig(tol=cfg.tol, solver=si.Solver.PlainLobpcg, warm_start=blk.data_ptr())
+    si.solve_eigenproblem(lap, 2, 42, pcfg)
+    pt, prep = [], None
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _, _, prep = si.solve_eigenproblem(lap, 2, 42, pcfg)
+        torch.cuda.synchronize()
+        pt.append(1e3 * (time.perf_counter() - t))
+    fast_w, plain_w = float(np.median(ft)), float(np.median(pt))
+    plain_r = float(np.median([p["ms"] for p in plain])) if plain else None
+    out = dict(network_ms=round(net_ms, 2), network_block_used=bool(used),
+               network_block_residual=rn, random_block_residual=rr,
+               fast_ms_with_block=round(fast_w, 3), fast_rounds_with_block=frep.iterations,
+               fast_ms_with_block_incl_network=round(fast_w + net_ms, 3),
+               fast_ms_random_start=round(fast_ms, 3),
+               plain_ms_with_block=round(plain_w, 2),
+               plain_ms_with_block_incl_network=round(plain_w + net_ms, 2),
+               plain_iterations_with_block=prep.iterations,
+               plain_ms_random_start=plain_r,
+               plain_iterations_random_start=plain[0]["iterations"] if plain else None,
+               max_residual_with_block=max(frep.max_residual, prep.max_residual))
+    log(f"warm start: network {net_ms:.1f} ms (used {used}); fast {fast_w:.2f} ms "
+        f"({frep.iterations} rounds) vs {fast_ms:.2f} random; plain {plain_w:.1f} ms "
+        f"({prep.iterations} iterations) vs {plain_r} random")
+    return out
+
+
 def bench_components(args, torch, graphs, si):
     """64 disconnected components (sizes log-uniform in [50, 200 000], each an
     independent 30-d GMM 15-NN graph): spectral_init with all components in one
@@ -268,6 +321,8 @@ def main():
                     help="time the device fuzzy k-NN graph builder alone, print its record and exit")
     ap.add_argument("--no-graph-builder", action="store_true",
                     help="skip timing the device fuzzy k-NN graph builder on the workload's points")
+    ap.add_argument("--no-warm-start", action="store_true",
+                    help="skip the learned warm start leg (network block fed to both solvers)")
     ap.add_argument("--spmm-shape16", type=int, default=-1,
                     help="pin the FP32 16-column SpMM shape (0-3) instead of autotuning; "
                          "counter runs pin the shape the autotuner picks unprofiled")
@@ -424,6 +479,10 @@ def main():
             f"res={plain[-1]['max_residual']}")
     plain_ms = float(np.median([p["ms"] for p in plain])) if plain else None
 
+    warm = None
+    if not args.no_warm_start and args.n_components == 2:
+        warm = bench_warm_start(args, torch, si, ctx, lap, cfg, fast_ms, plain)
+
     rsvd = []
     for i in range(args.rsvd_runs):
         torch.cuda.synchronize()
@@ -525,6 +584,7 @@ def main():
                          "achieved": round(spmm_gbs, 1) if spmm_gbs else None,
                          "frac": round(spmm_gbs / HBM_PEAK_GBS, 4) if spmm_gbs else None},
                      "gather_probe": probe},
+        "warm_start": warm,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all if not args.no_cpu else None,
         "graph_builder": gbuild,

@@ -147,39 +147,57 @@ void mpnn_init(Mpnn& M, uint64_t seed, hipStream_t s) {
     M.packed = false;
 }
 
-// Node features [n x 16] in the Laplacian's row order (see oracle_mpnn.c).
-void mpnn_features(const DevLaplacian& L, uint64_t seed, float* f, hipStream_t s) {
+// Node features [n x 16] in the Laplacian's row order (see oracle_mpnn.c) of
+// `sets` probe seeds seed, seed + 1, ... at once, into f = [sets][n x 16].  The
+// random-walk probes of all sets advance together, one `sets`-column FP64 SpMM
+// per step; each column's arithmetic is that of a walk of its own, so set j's
+// features are those of a single-set call with seed + j.
+void mpnn_features_sets(const DevLaplacian& L, uint64_t seed, int sets, float* f, hipStream_t s) {
     const int64_t n = L.n;
+    const size_t nf = (size_t)n * 16;
     std::vector<double> sq(n);
     to_host(sq.data(), L.sqrt_deg, n, s);
     double sqmax = 0.0, sqn2 = 0.0;
     for (double v : sq) { sqmax = std::max(sqmax, v); sqn2 += v * v; }
     mpnn_rowfeat(L, sqmax > 0.0 ? 1.0 / sqmax : 0.0, f, s);
+    for (int j = 1; j < sets; ++j)  // the degree and weight columns do not depend on the seed
+        SI_HIP(hipMemcpyAsync(f + j * nf, f, sizeof(float) * nf, hipMemcpyDeviceToDevice, s));
     // the walk runs in FP64: A_hat never damps the trivial direction, so FP32
     // rounding noise along it would grow relative to the decaying probe
-    DevBuf<double> p(n), q(n), g(1), ws(gram_workspace_doubles(n, 1, 1));
+    DevBuf<double> p((size_t)n * sets), q((size_t)n * sets), one(n), g((size_t)sets * sets),
+        ws(gram_workspace_doubles(n, sets, sets));
     DevBuf<float> pf(n);
-    fill_normal_f64(p, n, 1, 1, seed, kStreamProbe, s, L.perm);
-    gram_f64(n, p, 1, 1, L.sqrt_deg, 1, 1, g, ws, s);
-    double dot = 0.0;
-    to_host(&dot, g.get(), 1, s);
-    mpnn_deflate(n, p, L.sqrt_deg, sqn2 > 0.0 ? dot / sqn2 : 0.0, pf, s);
+    for (int j = 0; j < sets; ++j) {
+        fill_normal_f64(one, n, 1, 1, seed + (uint64_t)j, kStreamProbe, s, L.perm);
+        gram_f64(n, one, 1, 1, L.sqrt_deg, 1, 1, g, ws, s);
+        double dot = 0.0;
+        to_host(&dot, g.get(), 1, s);
+        mpnn_deflate(n, one, L.sqrt_deg, sqn2 > 0.0 ? dot / sqn2 : 0.0, pf, s);
+        copy_cols_f64(one, 1, p.get() + j, sets, n, 1, s);
+    }
     double* cur = p.get();
     double* nxt = q.get();
     int step = 0;
+    std::vector<double> hg((size_t)sets * sets);
     for (int t = 0; t < 8; ++t) {
         for (; step < kProbeSteps[t]; ++step) {
-            spmm_f64(L, cur, nxt, 1, 1, -1.0, 1.0, 0.0, nullptr, s);  // A_hat = I - L
+            spmm_f64(L, cur, nxt, sets, sets, -1.0, 1.0, 0.0, nullptr, s);  // A_hat = I - L
             std::swap(cur, nxt);
         }
-        gram_f64(n, cur, 1, 1, cur, 1, 1, g, ws, s);
-        double s2 = 0.0;
-        to_host(&s2, g.get(), 1, s);
-        const double rms = std::sqrt(s2 / (double)n);
-        convert_f64_to_f32(cur, pf, n, s);
-        mpnn_setcol(n, 6 + t, pf, rms > 0.0 ? (float)(1.0 / rms) : 0.f, f, s);
+        gram_f64(n, cur, sets, sets, cur, sets, sets, g, ws, s);
+        to_host(hg.data(), g.get(), hg.size(), s);
+        for (int j = 0; j < sets; ++j) {
+            const double rms = std::sqrt(hg[(size_t)j * sets + j] / (double)n);
+            copy_cols_f64(cur + j, sets, one, 1, n, 1, s);
+            convert_f64_to_f32(one, pf, n, s);
+            mpnn_setcol(n, 6 + t, pf, rms > 0.0 ? (float)(1.0 / rms) : 0.f, f + j * nf, s);
+        }
     }
     SI_HIP(hipStreamSynchronize(s));
+}
+
+void mpnn_features(const DevLaplacian& L, uint64_t seed, float* f, hipStream_t s) {
+    mpnn_features_sets(L, seed, 1, f, s);
 }
 
 // X = net(F) [n x 8], activations kept for the backward pass
@@ -437,17 +455,19 @@ si_status si_mpnn_warm_start(si_context* ctx, si_mpnn* m, const si_laplacian* la
         const int64_t n = L.n;
         const int k = 3;
         const int W = 1 + si::kMpnnOut * si::kProbeSets;
+        const int WP = 32;  // the candidate block stored 32 wide: one SpMM launch
         hipStream_t s = ctx->c.stream;
         // candidate subspaces: [sqrt_deg | X(probe set 0) | X(set 1) | X(set 2)] and
         // the solver's own random start [sqrt_deg | N(0,1)]
-        si::DevBuf<float> fi((size_t)n * 16);
-        si::DevBuf<double> bn((size_t)n * W), br((size_t)n * k);
-        si::set_column_f64(bn, W, n, 0, L.sqrt_deg, 1.0, s);
+        si::DevBuf<float> fi((size_t)n * 16 * si::kProbeSets);
+        si::DevBuf<double> bn((size_t)n * WP), br((size_t)n * k);
+        SI_HIP(hipMemsetAsync(bn.get(), 0, sizeof(double) * (size_t)n * WP, s));
+        si::set_column_f64(bn, WP, n, 0, L.sqrt_deg, 1.0, s);
+        si::mpnn_features_sets(L, seed, si::kProbeSets, fi, s);
         for (int set = 0; set < si::kProbeSets; ++set) {
-            si::mpnn_features(L, seed + (uint64_t)set, fi, s);
-            si::mpnn_forward(m->m, L, fi, s);
+            si::mpnn_forward(m->m, L, fi.get() + (size_t)set * n * 16, s);
             si::convert_f32_to_f64(m->m.x, m->m.xd, n * si::kMpnnOut, s);
-            si::copy_cols_f64(m->m.xd, si::kMpnnOut, bn.get() + 1 + set * si::kMpnnOut, W, n,
+            si::copy_cols_f64(m->m.xd, si::kMpnnOut, bn.get() + 1 + set * si::kMpnnOut, WP, n,
                               si::kMpnnOut, s);
         }
         si::fill_normal_f64(br, n, k, k, seed, si::kStreamLobpcgInit, s, L.perm);
@@ -456,26 +476,27 @@ si_status si_mpnn_warm_start(si_context* ctx, si_mpnn* m, const si_laplacian* la
         // Rayleigh-Ritz of a subspace basis that need not be orthonormal:
         // (B^T L B) c = theta (B^T B) c; the k lowest Ritz vectors go to out,
         // judged by their FP64 residuals
-        si::DevBuf<double> lb((size_t)n * W), gg((size_t)W * 2 * W), th(W), cm((size_t)W * W),
+        si::DevBuf<double> lb((size_t)n * WP), gg((size_t)W * 2 * W), th(W), cm((size_t)W * W),
             csub((size_t)W * k), gws(si::gram_workspace_doubles(n, W, 2 * W)), on((size_t)n * k),
             orr((size_t)n * k);
         si::DevBuf<int> stt(1);
-        auto judge = [&](const double* b, int w, double* out) -> double {
-            si::spmm_f64(L, b, lb, w, w, 1.0, 0.0, 0.0, nullptr, s);
-            si::gram2_f64(n, b, w, w, b, w, w, lb, w, w, gg, gws, s);
+        // b: w basis columns stored ld wide (columns past w are zero)
+        auto judge = [&](const double* b, int w, int ld, double* out) -> double {
+            si::spmm_f64(L, b, lb, ld, ld, 1.0, 0.0, 0.0, nullptr, s);
+            si::gram2_f64(n, b, ld, w, b, ld, w, lb, ld, w, gg, gws, s);
             si::small_gen_eig_scaled(w, gg.get() + w, 2 * w, gg, 2 * w, th, cm, stt, s);
             int st = 0;
             si::to_host(&st, stt.get(), 1, s);
             if (st != 0) return INFINITY;
             si::small_copy_sub(w, k, cm, w, csub, k, 1.0, s);
-            si::block_mul_f64(n, b, w, w, csub, k, nullptr, 0, 0, nullptr, out, k, s);
+            si::block_mul_f64(n, b, ld, w, csub, k, nullptr, 0, 0, nullptr, out, k, s);
             si::to_host(ev, th.get(), k, s);
             si::fp64_residuals(L, out, k, ev, rs, s);
             double r = 0.0;
             for (double x : rs) r = std::isnan(x) ? INFINITY : std::max(r, x);
             return r;
         };
-        const double rn = judge(bn, W, on), rr = judge(br, k, orr);
+        const double rn = judge(bn, W, WP, on), rr = judge(br, k, k, orr);
         const bool net = rn < rr;
         si::rows_to_external(L, net ? on.get() : orr.get(), k, block, k, k, s);
         SI_HIP(hipStreamSynchronize(s));
