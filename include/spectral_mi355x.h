@@ -134,6 +134,25 @@ si_status si_laplacian_export(si_context* ctx, const si_laplacian* lap, double* 
 si_status si_spmm(si_context* ctx, const si_laplacian* lap, int32_t precision, const void* x,
                   void* y, int32_t k, int32_t ld);
 
+/* LinearOperator::apply (src/operator.rs:6-12) on host arrays: y = L x for
+ * row-major host blocks [n x k] (FP64, the caller's node order), computed on
+ * the device with the exact FP64 Laplacian.  The entry a Rust
+ * `impl LinearOperator` forwards to. */
+si_status si_operator_apply_host(si_context* ctx, const si_laplacian* lap, const double* x,
+                                 double* y, int32_t k);
+
+/* spmv_csr (src/operator.rs:16-57), the fixed raw-slice "SIMD replacement
+ * point": y[i] = sum_k data[k] * x[indices[k]] for a general CSR matrix given
+ * as host slices (usize row pointers and column indices, f64 values), computed
+ * on the device with the scalar loop's rounding (each product rounded, then
+ * added in storage order), so y equals the reference's bit for bit.
+ * indptr has n_rows + 1 entries, indices and data at least indptr[n_rows];
+ * x has x_len entries, y n_rows.  The reference's precondition panics
+ * (indptr length, column bounds, data length) return SI_ERR_INVALID_ARGUMENT. */
+si_status si_spmv_csr(si_context* ctx, int64_t n_rows, const uint64_t* indptr,
+                      const uint64_t* indices, const double* data, int64_t data_len,
+                      const double* x, int64_t x_len, double* y);
+
 /* chebyshev_precond (src/solvers/lobpcg.rs:113-150): out = T_d((L - cI)/e) r
  * with c = (a+b)/2, e = (b-a)/2; FP64 device blocks [n x k]. */
 si_status si_chebyshev_filter(si_context* ctx, const si_laplacian* lap, const double* r,

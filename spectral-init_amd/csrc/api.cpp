@@ -555,6 +555,66 @@ si_status si_chebyshev_filter(si_context* ctx, const si_laplacian* lap, const do
     });
 }
 
+si_status si_operator_apply_host(si_context* ctx, const si_laplacian* lap, const double* x,
+                                 double* y, int32_t k) {
+    if (!ctx || !lap || !x || !y) return SI_ERR_INVALID_ARGUMENT;
+    return guard(ctx, [&] {
+        SI_CHECK(si::spmm_width_supported(k), si::kErrInvalidArgument,
+                 "operator apply: need 1 <= k <= 4096");
+        const si::DevLaplacian& L = lap->L;
+        hipStream_t s = ctx->c.stream;
+        const size_t nk = (size_t)L.n * k;
+        si::DevBuf<double> xe(nk), xi(nk), yi(nk);
+        si::to_device(xe.get(), x, nk, s);
+        si::rows_to_internal(L, xe.get(), k, xi.get(), k, k, s);
+        si::spmm_f64(L, xi, yi, k, k, 1.0, 0.0, 0.0, nullptr, s);
+        si::rows_to_external(L, yi.get(), k, xe.get(), k, k, s);
+        si::to_host(y, xe.get(), nk, s);
+    });
+}
+
+si_status si_spmv_csr(si_context* ctx, int64_t n_rows, const uint64_t* indptr,
+                      const uint64_t* indices, const double* data, int64_t data_len,
+                      const double* x, int64_t x_len, double* y) {
+    if (!ctx || n_rows < 0 || x_len < 0 || data_len < 0 || !indptr) return SI_ERR_INVALID_ARGUMENT;
+    return guard(ctx, [&] {
+        const uint64_t nnz = indptr[n_rows];
+        // the reference's preconditions (operator.rs:33-48), as errors
+        SI_CHECK(data_len >= 0 && (uint64_t)data_len >= nnz, si::kErrInvalidArgument,
+                 "CSR invariant violated: data.len() < nnz");
+        SI_CHECK(nnz == 0 || (indices && data), si::kErrInvalidArgument, "CSR: missing arrays");
+        SI_CHECK(n_rows == 0 || y, si::kErrInvalidArgument, "CSR: missing y");
+        SI_CHECK(nnz < (1ull << 31) && x_len < (1ll << 31), si::kErrUnsupported,
+                 "spmv_csr: more than 2^31 entries or columns");
+        std::vector<int64_t> ip(n_rows + 1);
+        for (int64_t i = 0; i <= n_rows; ++i) {
+            SI_CHECK(indptr[i] <= nnz && (i == 0 || indptr[i] >= indptr[i - 1]),
+                     si::kErrInvalidArgument, "CSR invariant violated: indptr not monotone");
+            ip[i] = (int64_t)indptr[i];
+        }
+        std::vector<int32_t> ix(nnz);
+        for (uint64_t p = 0; p < nnz; ++p) {
+            SI_CHECK(indices[p] < (uint64_t)x_len, si::kErrInvalidArgument,
+                     "CSR invariant violated: column index out of bounds (x.len()=" +
+                         std::to_string(x_len) + ")");
+            ix[p] = (int32_t)indices[p];
+        }
+        if (n_rows == 0) return;
+        hipStream_t s = ctx->c.stream;
+        si::DevBuf<int64_t> dip(n_rows + 1);
+        si::DevBuf<int32_t> dix(nnz);
+        si::DevBuf<double> dd(nnz), dx(x_len), dy(n_rows);
+        si::to_device(dip.get(), ip.data(), ip.size(), s);
+        if (nnz) {
+            si::to_device(dix.get(), ix.data(), nnz, s);
+            si::to_device(dd.get(), data, nnz, s);
+        }
+        if (x_len) si::to_device(dx.get(), x, (size_t)x_len, s);
+        si::csr_spmv_exact_f64(n_rows, dip, dix, dd, dx, dy, s);
+        si::to_host(y, dy.get(), (size_t)n_rows, s);
+    });
+}
+
 si_status si_max_eigenpair_residual(si_context* ctx, const si_laplacian* lap,
                                     const double* evals, const double* evecs, int32_t k,
                                     double* out) {

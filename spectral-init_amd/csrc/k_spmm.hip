@@ -464,7 +464,39 @@ __global__ __launch_bounds__(256) void k_gather_probe(const int32_t* __restrict_
     out[t] = acc;
 }
 
+// y[i] = sum_k data[k] x[indices[k]] over row i's entries in storage order,
+// each product rounded, then added (contraction into fused multiply-adds is
+// switched off for this kernel): the arithmetic of
+// the reference's scalar spmv_csr (src/operator.rs:24-57), bit for bit.  One
+// thread per row, grid stride.
+__global__ __launch_bounds__(256) void k_csr_spmv_exact(int64_t n, const int64_t* __restrict__ ip,
+                                                        const int32_t* __restrict__ ix,
+                                                        const double* __restrict__ data,
+                                                        const double* __restrict__ x,
+                                                        double* __restrict__ y) {
+#pragma clang fp contract(off)
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256) {
+        double acc = 0.0;
+        for (int64_t k = ip[i]; k < ip[i + 1]; ++k) {
+            const double prod = data[k] * x[ix[k]];
+            acc = acc + prod;
+        }
+        y[i] = acc;
+    }
+}
+
 }  // namespace
+
+void csr_spmv_exact_f64(int64_t n, const int64_t* ip, const int32_t* ix, const double* data,
+                        const double* x, double* y, hipStream_t s) {
+    int64_t blocks = ceil_div(n, (int64_t)256);
+    if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_csr_spmv_exact, dim3((unsigned)blocks), dim3(256), 0, s, n, ip, ix, data,
+                       x, y);
+    SI_LAUNCH_CHECK();
+}
 
 double gather_probe_us(const DevLaplacian& L, int mode, int reps, hipStream_t s) {
     SI_CHECK(mode >= 0 && mode <= 2, kErrInvalidArgument, "gather probe: mode 0, 1 or 2");

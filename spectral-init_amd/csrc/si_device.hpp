@@ -89,6 +89,11 @@ void spmm_seg_f32(const DevLaplacian& L, const float* x, float* out, int k, int 
 void spmm_seg_f64(const DevLaplacian& L, const double* x, double* out, int k, int ld,
                   const double* z, const int32_t* rseg, const double* coef, hipStream_t s);
 bool spmm_width_supported(int k);
+// y = A x for a general CSR matrix (FP64 values, any stored entries) with the
+// products rounded before each sum in storage order (the scalar spmv_csr's
+// arithmetic, bit for bit); all arrays on the device.
+void csr_spmv_exact_f64(int64_t n, const int64_t* ip, const int32_t* ix, const double* data,
+                        const double* x, double* y, hipStream_t s);
 // true while an SpmmProfile is collecting (SpMM calls then record events)
 bool spmm_profiling();
 // Pick the FP32 16- and 32-column SpMM shapes for this graph by timing every

@@ -36,7 +36,7 @@ EXPORTED = [
     "si_last_error", "si_laplacian_build", "si_laplacian_build_ex", "si_laplacian_destroy",
     "si_laplacian_info",
     "si_laplacian_export", "si_spmm", "si_chebyshev_filter", "si_max_eigenpair_residual",
-    "si_max_eigenpair_residual_noalloc",
+    "si_max_eigenpair_residual_noalloc", "si_operator_apply_host", "si_spmv_csr",
     "si_lobpcg_solve", "si_lobpcg_sinv_solve", "si_solve_eigenproblem", "si_rsvd_solve",
     "si_dense_evd",
     "si_spectral_init",
@@ -193,6 +193,8 @@ def lib():
         L.si_chebyshev_filter.argtypes = [vp, vp, vp, i32, dbl, dbl, i32, vp]
         L.si_max_eigenpair_residual.argtypes = [vp, vp, vp, vp, i32, C.POINTER(dbl)]
         L.si_max_eigenpair_residual_noalloc.argtypes = [vp, vp, vp, vp, i32, C.POINTER(dbl)]
+        L.si_operator_apply_host.argtypes = [vp, vp, vp, vp, i32]
+        L.si_spmv_csr.argtypes = [vp, i64, vp, vp, vp, i64, vp, i64, vp]
         L.si_lobpcg_solve.argtypes = [vp, vp, i32, u64, i32, dbl, vp, vp, C.POINTER(_Report)]
         L.si_lobpcg_sinv_solve.argtypes = [vp, vp, i32, u64, vp, vp, C.POINTER(_Report)]
         L.si_solve_eigenproblem.argtypes = [vp, vp, i32, u64, C.POINTER(_Config), vp, vp,
@@ -347,6 +349,29 @@ class Laplacian:
         ld = x.stride(0) if x.dim() == 2 else 1
         self.ctx._check(lib().si_spmm(self.ctx.handle, self._h, precision, _ptr(x), _ptr(y), k, ld))
         return y
+
+
+def operator_apply_host(lap: Laplacian, x: np.ndarray) -> np.ndarray:
+    """LinearOperator::apply on host arrays (src/operator.rs:6-12): y = L x, FP64."""
+    x = np.ascontiguousarray(x, np.float64)
+    xx = x[:, None] if x.ndim == 1 else x
+    y = np.empty_like(xx)
+    lap.ctx._check(lib().si_operator_apply_host(lap.ctx.handle, lap.handle, _ptr(xx), _ptr(y),
+                                                xx.shape[1]))
+    return y[:, 0] if x.ndim == 1 else y
+
+
+def spmv_csr(ctx: Context, indptr, indices, data, x) -> np.ndarray:
+    """spmv_csr (src/operator.rs:16-57) on the device, bit-identical to the
+    reference's scalar loop: y = A x for a general CSR matrix."""
+    ip = np.ascontiguousarray(indptr, np.uint64)
+    ix = np.ascontiguousarray(indices, np.uint64)
+    d = np.ascontiguousarray(data, np.float64)
+    xv = np.ascontiguousarray(x, np.float64)
+    y = np.empty(ip.size - 1, np.float64)
+    ctx._check(lib().si_spmv_csr(ctx.handle, ip.size - 1, _ptr(ip), _ptr(ix), _ptr(d), d.size,
+                                 _ptr(xv), xv.size, _ptr(y)))
+    return y
 
 
 def chebyshev_filter(lap: Laplacian, r, a=0.1, b=2.0, degree=8, out=None):

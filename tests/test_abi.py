@@ -85,3 +85,25 @@ def test_knn_path_seam_without_gpu(library):
         assert fn(-1, None) == si.SI_OK
     finally:
         fn(0, None)
+
+
+def _rust_struct_fields(text, name):
+    body = re.search(r"pub struct " + name + r" \{(.*?)\n\}", text, flags=re.S).group(1)
+    return re.findall(r"pub (\w+):\s*([^,]+),", body)
+
+
+def test_rust_shim_matches_the_c_abi():
+    """rust/src/mi355x.rs (compile-unverified: no cargo here) declares only
+    entry points the public header has, and its repr(C) structs list the C
+    structs' fields in the same order with the same widths."""
+    text = open(os.path.join(ROOT, "rust", "src", "mi355x.rs")).read()
+    used = set(re.findall(r"pub fn (si_[a-z0-9_]+)\(", text))
+    assert used and used <= _declared(), used - _declared()
+    rust_size = {"u32": 4, "i32": 4, "i64": 8, "f64": 8, "*const f64": 8}
+    for rname, cstruct in (("SiConfig", si._Config), ("SiReport", si._Report)):
+        fields = _rust_struct_fields(text, rname)
+        assert [f for f, _ in fields] == [f for f, _ in cstruct._fields_], rname
+        for (f, ty), (_, cty) in zip(fields, cstruct._fields_):
+            m = re.match(r"\[f64; (\d+)\]", ty.strip())
+            size = 8 * int(m.group(1)) if m else rust_size[ty.strip()]
+            assert size == ctypes.sizeof(cty), (rname, f)

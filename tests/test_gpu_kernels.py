@@ -297,3 +297,51 @@ def test_reordered_spmm_matches_oracle(ctx, k):
     yd = torch.empty_like(xd)
     lap.spmm(xd, yd, si.SI_F64)
     np.testing.assert_allclose(yd.cpu().numpy(), O.spmm(L, x), rtol=1e-12, atol=1e-12)
+
+
+# ------------------------------------------------------------------ host operator seams
+def test_spmv_csr_is_bit_identical_to_the_scalar_loop(ctx):
+    # spmv_csr (operator.rs:16-57): y[i] = sum data[k] * x[indices[k]] in
+    # storage order, each product rounded before the sum -- a general matrix
+    # (unsymmetric, unsorted rows, repeated columns, explicit zeros, empty rows)
+    rng = np.random.default_rng(11)
+    n, m = 3000, 2500
+    lens = rng.integers(0, 40, n)
+    lens[::97] = 0
+    ip = np.zeros(n + 1, np.uint64)
+    ip[1:] = np.cumsum(lens)
+    nnz = int(ip[-1])
+    ix = rng.integers(0, m, nnz).astype(np.uint64)
+    d = rng.standard_normal(nnz) * rng.choice([0.0, 1.0, 1e-8, 1e8], nnz)
+    x = rng.standard_normal(m)
+    y = si.spmv_csr(ctx, ip, ix, d, x)
+    want = np.zeros(n)
+    for i in range(n):
+        acc = 0.0
+        for k in range(int(ip[i]), int(ip[i + 1])):
+            acc = acc + float(d[k]) * float(x[int(ix[k])])
+        want[i] = acc
+    np.testing.assert_array_equal(y, want)
+
+
+def test_spmv_csr_preconditions_are_errors(ctx):
+    ip = np.array([0, 2, 3], np.uint64)
+    ix = np.array([0, 1, 2], np.uint64)
+    d = np.ones(3)
+    with pytest.raises(ValueError, match="out of bounds"):
+        si.spmv_csr(ctx, ip, ix, d, np.ones(2))          # column 2 >= x.len() = 2
+    with pytest.raises(ValueError, match="data.len"):
+        si.spmv_csr(ctx, ip, ix, d[:2], np.ones(3))      # data shorter than nnz
+    assert si.spmv_csr(ctx, ip, ix, d, np.arange(3.0)).tolist() == [1.0, 2.0]
+
+
+@pytest.mark.parametrize("name,k", [("blobs_connected_2000", 1), ("blobs_5000", 5)])
+def test_operator_apply_on_host_arrays(ctx, name, k):
+    # LinearOperator::apply (operator.rs:6-12) for host blocks; blobs_5000 is
+    # stored in the locality order, which stays invisible to the caller
+    z = load_fixture(name)
+    lap = si.Laplacian(ctx, z["w_indptr"], z["w_indices"], z["w_data"])
+    L, _, _ = O.laplacian_from_graph(z["w_indptr"], z["w_indices"], z["w_data"])
+    x = np.random.default_rng(k).standard_normal((lap.n, k))
+    y = si.operator_apply_host(lap, x)
+    np.testing.assert_allclose(y, O.spmm(L, x), rtol=1e-12, atol=1e-13)
