@@ -401,8 +401,13 @@ def main():
                                   gbs=round(gbs, 1)))
                 del x, y
         if args.spmm_only:
+            probe = {m: round(si.gather_probe(lap, i, 10), 2)
+                     for i, m in enumerate(("column_stream_us", "random_rows_us",
+                                            "random_rows_2MiB_us"))}
+            log(f"gather probe: {probe}")
             print(json.dumps({"workload": args.workload, "n": n, "nnz": nnz,
-                              "hub_rows": lap.n_long, "spmm": sweep}), flush=True)
+                              "hub_rows": lap.n_long, "spmm": sweep, "gather_probe": probe}),
+                  flush=True)
             return
 
     cfg = si.SpectralInitConfig(tol=args.tol, block_size=args.block, cheb_degree=args.degree,
