@@ -104,6 +104,51 @@ __global__ __launch_bounds__(256) void k_gram(int64_t n, const T* __restrict__ a
     for (int e = threadIdx.x; e < p * q; e += 256) out[e] = red[e / q][e % q];
 }
 
+// Narrow Gram (p, q <= 4): one thread per row (grid stride) keeps the p x q
+// running sums in registers (products and sums in FP64); a workgroup reduces
+// them with wave butterflies and a fixed-order sum of its 4 waves into one
+// partial, laid out as k_gram's (p x q row-major per workgroup).
+template <typename T>
+__global__ __launch_bounds__(256) void k_gram_narrow(int64_t n, const T* __restrict__ a, int lda,
+                                                     int p, const T* __restrict__ b, int ldb,
+                                                     int q1, const T* __restrict__ b2, int ldb2,
+                                                     int q, double* __restrict__ partial) {
+    double acc[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256) {
+        double av[4], bv[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            av[c] = c < p ? (double)a[i * lda + c] : 0.0;
+            bv[c] = c < q1 ? (double)b[i * ldb + c]
+                           : (c < q ? (double)b2[i * ldb2 + (c - q1)] : 0.0);
+        }
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) acc[x * 4 + y] += av[x] * bv[y];
+    }
+    __shared__ double red[4][16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[e] += __shfl_xor(acc[e], off, 64);
+    if (lane == 0)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) red[wave][e] = acc[e];
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < p * q) {
+        const int x = t / q, y = t % q;
+        const int e = x * 4 + y;
+        partial[(int64_t)blockIdx.x * p * q + t] =
+            ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
+    }
+}
+
 // Sum the per-workgroup partials, g[e] = sum_b partial[b][e], in a fixed order
 // (deterministic).  A workgroup owns 8 outputs; 32 groups of 8 lanes stride the
 // partials, each lane keeping 8 independent loads in flight, and the group sums
@@ -205,6 +250,13 @@ void gram_impl(int64_t n, const T* a, int lda, int p, const T* b, int ldb, int q
         rpb = ceil_div(ceil_div(n, nblocks), 16) * 16;
         nblocks = ceil_div(n, rpb);
     }
+    if (!seg && p <= 4 && q <= 4) {  // narrow blocks: no 16-wide MFMA tiles of padding
+        hipLaunchKernelGGL((k_gram_narrow<T>), dim3(nblocks), dim3(256), 0, s, n, a, lda, p, b,
+                           ldb, q1, b2, ldb2, q, ws);
+        SI_LAUNCH_CHECK();
+        reduce_partials(ws, (int)nblocks, p * q, g, s);
+        return;
+    }
     const int pt = (p + 15) / 16, qt = (q + 15) / 16;
 #define SI_GRAM_CASE(PT_, QT_)                                                                   \
     if (pt == PT_ && qt == QT_) {                                                                \
@@ -282,6 +334,47 @@ __global__ __launch_bounds__(256) void k_block_mul(int64_t n, const T* __restric
     }
 }
 
+// Narrow block product (p, p2, q <= 4): one thread per row (grid stride), the
+// small matrices in registers in the data precision, the same term and k order
+// as k_block_mul.  A thread reads its row before writing it: out may alias a.
+template <typename T>
+__global__ __launch_bounds__(256) void k_block_mul_narrow(int64_t n, const T* a, int lda, int p,
+                                                          const double* __restrict__ c, int q,
+                                                          const T* __restrict__ b, int ldb, int p2,
+                                                          const double* __restrict__ d, T* out,
+                                                          int ldo) {
+    T cm[16], dm[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int k = e / 4, j = e % 4;
+        cm[e] = (k < p && j < q) ? (T)c[k * q + j] : T(0);
+        dm[e] = (b && k < p2 && j < q) ? (T)d[k * q + j] : T(0);
+    }
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256) {
+        T av[4], bv[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            av[k] = k < p ? a[i * lda + k] : T(0);
+            bv[k] = (b && k < p2) ? b[i * ldb + k] : T(0);
+        }
+        T acc[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[j] += av[k] * cm[k * 4 + j];
+        if (b) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[j] += bv[k] * dm[k * 4 + j];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (j < q) out[i * ldo + j] = acc[j];
+    }
+}
+
 // Out[n x 16] = A[n x 16] C[16 x 16] in FP64 on v_mfma_f64_16x16x4f64.  Each
 // wavefront owns 16-row tiles (grid stride).  Lane (li = lane % 16, lk =
 // lane / 16) feeds A[r0 + li][4s + lk] and C[4s + lk][li] for the k-steps s =
@@ -331,6 +424,15 @@ void block_mul_impl(int64_t n, const T* a, int lda, int p, const double* c, int 
             SI_LAUNCH_CHECK();
             return;
         }
+    }
+    if (p <= 4 && p2 <= 4 && q <= 4) {  // narrow blocks: a thread per row, no LDS staging
+        int64_t nb = ceil_div(n, (int64_t)256);
+        if (nb > 8192) nb = 8192;
+        if (nb < 1) nb = 1;
+        hipLaunchKernelGGL((k_block_mul_narrow<T>), dim3(nb), dim3(256), 0, s, n, a, lda, p, c, q,
+                           b, ldb, p2, d, out, ldo);
+        SI_LAUNCH_CHECK();
+        return;
     }
     const int64_t blocks = ceil_div(n, sizeof(T) == 8 ? 32 : 64);
     if (q <= 4)
