@@ -945,9 +945,19 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
         if (!(mx > 0.0))
             throw si::Error(si::kErrInvalidGraph, "invalid graph: scale_coords: embedding is all-zero; cannot scale to max=10");
         const double ex = 10.0 / mx;
-        for (size_t i = 0; i < coords.size(); ++i) {
-            const float noise = (float)(1e-4 * si::normal_at(seed, si::kStreamNoise, (uint64_t)i));
-            out[i] = (float)(coords[i] * ex) + noise;
+        // the noise draws (Philox stream kStreamNoise, index = the flat output
+        // position) are generated on the device and downloaded
+        {
+            hipStream_t s = ctx->c.stream;
+            const size_t cnt = coords.size();
+            double* nz = si::ws_get<double>(si::kWsAbiOut, cnt);
+            si::fill_normal_f64(nz, (int64_t)(cnt / d), d, d, seed, si::kStreamNoise, s);
+            std::vector<double> hn(cnt);
+            si::to_host(hn.data(), nz, cnt, s);
+            for (size_t i = 0; i < cnt; ++i) {
+                const float noise = (float)(1e-4 * hn[i]);
+                out[i] = (float)(coords[i] * ex) + noise;
+            }
         }
         if (rep) {
             fill_report(rep, last, ev.empty() ? nullptr : ev.data(), ev.empty() ? 0 : (int)ev.size(), solve_ms);

@@ -241,6 +241,22 @@ si_status si_test_rcm_order(int64_t n, const int64_t* indptr, const int32_t* ind
     }
 }
 
+si_status si_test_laplacian_storage(si_context* ctx, const si_laplacian* lap, int32_t* rowptr,
+                                    int32_t* col, float* a32, int32_t* perm) {
+    if (!ctx || !lap) return SI_ERR_INVALID_ARGUMENT;
+    return tguard(ctx, [&] {
+        hipStream_t s = ctx->c.stream;
+        const si::DevLaplacian& L = lap->L;
+        if (rowptr) si::to_host(rowptr, L.rowptr, L.n + 1, s);
+        if (col && L.nnz) si::to_host(col, L.col, L.nnz, s);
+        if (a32 && L.nnz) si::to_host(a32, L.a32, L.nnz, s);
+        if (perm) {
+            if (L.perm) si::to_host(perm, L.perm, L.n, s);
+            else for (int64_t i = 0; i < L.n; ++i) perm[i] = (int32_t)i;
+        }
+    });
+}
+
 si_status si_test_knn_path(int32_t mode, int64_t* last_direct) {
     if (mode < -1 || mode > 3) return SI_ERR_INVALID_ARGUMENT;
     if (mode >= 0) si::knn_set_path(mode);

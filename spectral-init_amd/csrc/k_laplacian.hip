@@ -16,7 +16,6 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
-#include <thread>
 #include <utility>
 #include <vector>
 
@@ -262,44 +261,15 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
         phase("rcm order");
         std::vector<int32_t> iperm(n);
         for (int64_t i = 0; i < n; ++i) iperm[perm[i]] = (int32_t)i;
-        std::vector<int32_t> rp2(n + 1), c2(nnz);
-        std::vector<float> w2(nnz);
-        rp2[0] = 0;
-        for (int64_t i = 0; i < n; ++i)
-            rp2[i + 1] = rp2[i] + (int32_t)(hp[perm[i] + 1] - hp[perm[i]]);
-        // rows are independent: renumber and sort them on host threads
-        auto fill = [&](int64_t lo, int64_t hi) {
-            std::vector<std::pair<int32_t, float>> row;
-            for (int64_t i = lo; i < hi; ++i) {
-                const int64_t v = perm[i];
-                row.clear();
-                for (int64_t p = hp[v]; p < hp[v + 1]; ++p) row.emplace_back(iperm[hx[p]], hw[p]);
-                std::sort(row.begin(), row.end(),
-                          [](const std::pair<int32_t, float>& x, const std::pair<int32_t, float>& y) {
-                              return x.first < y.first;
-                          });
-                int32_t o = rp2[i];
-                for (const auto& e : row) { c2[o] = e.first; w2[o] = e.second; ++o; }
-            }
-        };
-        const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(
-            std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency())), n / 4096));
-        std::vector<std::thread> pool;
-        for (int t = 1; t < nth; ++t)
-            pool.emplace_back(fill, n * t / nth, n * (t + 1) / nth);
-        fill(0, n / nth);
-        for (auto& th : pool) th.join();
-        phase("permuted csr (host)");
         SI_HIP(hipMalloc(&d_w2, sizeof(float) * (nnz > 0 ? nnz : 1)));
         SI_HIP(hipMalloc(&L.perm, sizeof(int32_t) * n));
         SI_HIP(hipMalloc(&L.iperm, sizeof(int32_t) * n));
-        SI_HIP(hipMemcpyAsync(L.rowptr, rp2.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, s));
-        if (nnz) {
-            SI_HIP(hipMemcpyAsync(L.col, c2.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s));
-            SI_HIP(hipMemcpyAsync(d_w2, w2.data(), sizeof(float) * nnz, hipMemcpyHostToDevice, s));
-        }
         SI_HIP(hipMemcpyAsync(L.perm, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
         SI_HIP(hipMemcpyAsync(L.iperm, iperm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+        // rows renumbered and sorted on the device, from the input CSR
+        permute_csr_device(n, nnz, d_indptr, d_indices, d_w, L.perm, L.iperm, L.rowptr, L.col,
+                           d_w2, s);
+        phase("permuted csr");
         // node vectors follow the permutation (computed in input order above)
         double* tmp = nullptr;
         SI_HIP(hipMalloc(&tmp, sizeof(double) * n));

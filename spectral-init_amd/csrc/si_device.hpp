@@ -68,6 +68,12 @@ void laplacian_free(DevLaplacian& L);
 // `threads` host threads (0: as the hardware allows, at most 16); the order
 // does not depend on the thread count.
 std::vector<int32_t> rcm_order(int64_t n, const int64_t* ip, const int32_t* ix, int threads = 0);
+// The permuted CSR P W P^T on the device: new row i is original row perm[i]
+// with its columns renumbered through iperm and sorted (stable, so equal
+// columns keep their input order).  rp2 has n + 1 entries; c2, w2 nnz.
+void permute_csr_device(int64_t n, int64_t nnz, const int64_t* rp, const int32_t* col,
+                        const float* w, const int32_t* perm, const int32_t* iperm, int32_t* rp2,
+                        int32_t* c2, float* w2, hipStream_t s);
 // dst[i, :k] = src[idx[i], :k] (row gather); idx == nullptr copies.
 void gather_rows_f64(const double* src, int lds, double* dst, int ldd, int64_t n, int k,
                      const int32_t* idx, hipStream_t s);

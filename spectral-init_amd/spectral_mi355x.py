@@ -43,6 +43,7 @@ EXPORTED = [
     "si_test_gram", "si_test_block_mul", "si_test_small_sym_eig", "si_test_small_gen_eig",
     "si_test_fill_normal", "si_test_fast_solve", "si_test_batch_graphs", "si_test_spmm_shapes",
     "si_test_band_solve", "si_test_rcm_order", "si_test_knn_path",
+    "si_test_laplacian_storage",
     "si_profile_spmm_begin", "si_profile_spmm_end", "si_profile_gather_probe",
     "si_mpnn_param_count", "si_mpnn_create", "si_mpnn_destroy", "si_mpnn_get_params",
     "si_mpnn_set_params", "si_mpnn_features", "si_mpnn_forward", "si_mpnn_gradient",
@@ -212,6 +213,7 @@ def lib():
                                          C.POINTER(_Report)]
         L.si_test_band_solve.argtypes = [vp, vp, dbl, vp, vp, i32, C.POINTER(i32)]
         L.si_test_rcm_order.argtypes = [i64, vp, vp, i32, vp]
+        L.si_test_laplacian_storage.argtypes = [vp, vp, vp, vp, vp, vp]
         L.si_test_knn_path.argtypes = [i32, C.POINTER(i64)]
         L.si_profile_spmm_begin.argtypes = [vp]
         L.si_profile_spmm_end.argtypes = [vp, C.POINTER(_SpmmProfile)]
@@ -787,6 +789,20 @@ def fuzzy_graph(ctx: Context, x, k: int = 15, n_epochs: int = -1) -> Graph:
     ctx._check(lib().si_fuzzy_graph_build(ctx.handle, x.ctypes.data, prec, n, d, k,
                                           int(n_epochs), C.byref(h)))
     return Graph(ctx, h)
+
+
+def laplacian_storage(lap: "Laplacian"):
+    """Testing seam: the operator as stored on the device -- (rowptr, col, a32,
+    perm) in the storage node order, perm[new] = old."""
+    n, nnz = lap.n, lap.nnz
+    rp = np.empty(n + 1, np.int32)
+    col = np.empty(nnz, np.int32)
+    a32 = np.empty(nnz, np.float32)
+    perm = np.empty(n, np.int32)
+    lap.ctx._check(lib().si_test_laplacian_storage(lap.ctx.handle, lap._h, rp.ctypes.data,
+                                                   col.ctypes.data, a32.ctypes.data,
+                                                   perm.ctypes.data))
+    return rp, col, a32, perm
 
 
 def rcm_order(indptr, indices, threads: int = 0) -> np.ndarray:

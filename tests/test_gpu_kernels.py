@@ -287,6 +287,33 @@ def test_reordered_laplacian_exports_reference_layout(ctx, name):
     np.testing.assert_array_equal(d, od)
 
 
+@pytest.mark.parametrize("name", ["hubs_3000", "blobs_5000", "disconnected_200"])
+def test_reordered_storage_is_the_permuted_csr(ctx, name):
+    # the device permutation (rows renumbered, columns relabelled and sorted)
+    # equals P W P^T computed by scipy, and the FP32 sweep values are those of
+    # the unpermuted build carried along bit for bit
+    import scipy.sparse as sp
+    if name == "hubs_3000":
+        ip, ix, w = _hub_graph()
+    else:
+        z = load_fixture(name)
+        ip, ix, w = z["w_indptr"], z["w_indices"], z["w_data"]
+    n = ip.size - 1
+    lap = si.Laplacian(ctx, ip, ix, w, reorder=True)
+    rp, col, a32, perm = si.laplacian_storage(lap)
+    np.testing.assert_array_equal(perm, si.rcm_order(ip, ix))
+    plain = si.Laplacian(ctx, ip, ix, w, reorder=False)
+    prp, pcol, pa32, pperm = si.laplacian_storage(plain)
+    np.testing.assert_array_equal(pperm, np.arange(n))
+    np.testing.assert_array_equal(prp, ip)
+    np.testing.assert_array_equal(pcol, ix)
+    a = sp.csr_matrix((pa32, ix, ip), shape=(n, n))[perm][:, perm].tocsr()
+    a.sort_indices()
+    np.testing.assert_array_equal(rp, a.indptr)
+    np.testing.assert_array_equal(col, a.indices)
+    np.testing.assert_array_equal(a32, a.data)
+
+
 @pytest.mark.parametrize("k", [3, 32])
 def test_reordered_spmm_matches_oracle(ctx, k):
     ip, ix, w = _hub_graph()
