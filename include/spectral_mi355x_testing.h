@@ -85,6 +85,14 @@ si_status si_profile_gather_probe(si_context* ctx, const si_laplacian* lap, int3
 si_status si_test_rcm_order(int64_t n, const int64_t* indptr, const int32_t* indices,
                             int32_t threads, int32_t* perm);
 
+/* The device counterparts of the two host steps above, for a symmetric host
+ * CSR pattern (uploaded here): component labels (numbered by smallest node, as
+ * find_components) with their count, and the locality order perm[new] = old. */
+si_status si_test_components_device(si_context* ctx, int64_t n, const int64_t* indptr,
+                                    const int32_t* indices, int32_t* labels, int64_t* n_comp);
+si_status si_test_rcm_order_device(si_context* ctx, int64_t n, const int64_t* indptr,
+                                   const int32_t* indices, int32_t* perm);
+
 /* The operator as stored on the device: the CSR of W in the storage node order
  * (rowptr[n+1], col[nnz], the FP32 sweep values a32[nnz] = +W_ij / sqrt(d_i d_j))
  * and perm[n] (perm[new] = old; the identity when no locality order was

@@ -1,7 +1,9 @@
 // api.cpp -- the C ABI (include/spectral_mi355x.h) over the device solvers,
 // plus the spectral_init pipeline around them (src/lib.rs:198-258).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstddef>
 #include <memory>
 #include <numeric>
@@ -146,7 +148,11 @@ int solve_graph(si::Context& c, int64_t n, const int64_t* ip, const int32_t* ix,
                 double* lap_ms, double* solve_ms) {
     hipStream_t s = c.stream;
     DeviceGraph g;
+    const auto t0 = std::chrono::steady_clock::now();
     upload_graph(g, n, ip, ix, w, s);
+    if (si::trace_on())
+        std::fprintf(stderr, "[spectral_init] %-22s %8.1f ms\n", "graph upload",
+                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
     si::DevLaplacian L;
     {
         si::EventTimer t(s);
@@ -771,8 +777,17 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
                            cfg->batch_components != 0;
         const int d = n_components;
         const int k = d + 1;
+        auto tp = std::chrono::steady_clock::now();
+        auto phase = [&](const char* what) {  // SI_TRACE=1: pipeline phase timings
+            if (!si::trace_on()) return;
+            const auto now = std::chrono::steady_clock::now();
+            std::fprintf(stderr, "[spectral_init] %-22s %8.1f ms\n", what,
+                         std::chrono::duration<double, std::milli>(now - tp).count());
+            tp = now;
+        };
         std::vector<int64_t> labels;
         const int64_t nc = find_components(n, indptr, indices, labels);
+        phase("components");
         std::vector<double> coords((size_t)n * d, 0.0);
         si::SolveStats last;
         double lap_ms = 0.0, solve_ms = 0.0;
@@ -780,6 +795,7 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
         if (nc == 1) {
             last.level = solve_graph(ctx->c, n, indptr, indices, data, d, seed, mode, opt, ev, vec,
                                      last, &lap_ms, &solve_ms);
+            phase("solve_graph");
             select_and_sign(ev, vec, n, k, d, coords);
         } else {
             // embed_disconnected (src/multi_component.rs:16-74)
@@ -939,6 +955,7 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
             for (int64_t c = 0; c < nc; ++c)
                 if (!cev[c].empty()) ev = cev[c];
         }
+        phase("select and place");
         // scale_and_add_noise (src/scaling.rs:78-89): max-abs 10, N(0, 1e-4) f32 noise
         double mx = 0.0;
         for (double v : coords) mx = std::max(mx, std::fabs(v));
@@ -959,6 +976,7 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
                 out[i] = (float)(coords[i] * ex) + noise;
             }
         }
+        phase("scale and noise");
         if (rep) {
             fill_report(rep, last, ev.empty() ? nullptr : ev.data(), ev.empty() ? 0 : (int)ev.size(), solve_ms);
             rep->laplacian_ms = lap_ms;

@@ -241,6 +241,44 @@ si_status si_test_rcm_order(int64_t n, const int64_t* indptr, const int32_t* ind
     }
 }
 
+namespace {
+struct DevPattern {
+    si::DevBuf<int64_t> ip;
+    si::DevBuf<int32_t> ix;
+    DevPattern(int64_t n, const int64_t* indptr, const int32_t* indices, hipStream_t s)
+        : ip(n + 1), ix(indptr[n]) {
+        si::to_device(ip.get(), indptr, n + 1, s);
+        if (indptr[n] > 0) si::to_device(ix.get(), indices, indptr[n], s);
+    }
+};
+}  // namespace
+
+si_status si_test_components_device(si_context* ctx, int64_t n, const int64_t* indptr,
+                                    const int32_t* indices, int32_t* labels, int64_t* n_comp) {
+    if (!ctx || n < 1 || !indptr || !labels || !n_comp || (indptr[n] > 0 && !indices))
+        return SI_ERR_INVALID_ARGUMENT;
+    return tguard(ctx, [&] {
+        hipStream_t s = ctx->c.stream;
+        DevPattern g(n, indptr, indices, s);
+        si::DevBuf<int32_t> lab(n);
+        *n_comp = si::components_device(n, g.ip.get(), g.ix.get(), lab.get(), nullptr, s);
+        si::to_host(labels, lab.get(), n, s);
+    });
+}
+
+si_status si_test_rcm_order_device(si_context* ctx, int64_t n, const int64_t* indptr,
+                                   const int32_t* indices, int32_t* perm) {
+    if (!ctx || n < 1 || !indptr || !perm || (indptr[n] > 0 && !indices))
+        return SI_ERR_INVALID_ARGUMENT;
+    return tguard(ctx, [&] {
+        hipStream_t s = ctx->c.stream;
+        DevPattern g(n, indptr, indices, s);
+        si::DevBuf<int32_t> p(n), ip(n);
+        si::rcm_order_device(n, g.ip.get(), g.ix.get(), p.get(), ip.get(), s);
+        si::to_host(perm, p.get(), n, s);
+    });
+}
+
 si_status si_test_laplacian_storage(si_context* ctx, const si_laplacian* lap, int32_t* rowptr,
                                     int32_t* col, float* a32, int32_t* perm) {
     if (!ctx || !lap) return SI_ERR_INVALID_ARGUMENT;

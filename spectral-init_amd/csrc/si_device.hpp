@@ -74,6 +74,16 @@ std::vector<int32_t> rcm_order(int64_t n, const int64_t* ip, const int32_t* ix, 
 void permute_csr_device(int64_t n, int64_t nnz, const int64_t* rp, const int32_t* col,
                         const float* w, const int32_t* perm, const int32_t* iperm, int32_t* rp2,
                         int32_t* c2, float* w2, hipStream_t s);
+// Connected components of a symmetric CSR pattern on the device: lab[v] =
+// component id, numbered by smallest node (the labels of a BFS sweep in node
+// order); root[c] (optional, n entries) = the smallest node of component c.
+// Returns the number of components.
+int64_t components_device(int64_t n, const int64_t* rp, const int32_t* ix, int32_t* lab,
+                          int32_t* root, hipStream_t s);
+// The order of rcm_order for a symmetric pattern, computed on the device:
+// perm[new] = old and iperm[old] = new (device arrays of n).
+void rcm_order_device(int64_t n, const int64_t* rp, const int32_t* ix, int32_t* perm,
+                      int32_t* iperm, hipStream_t s);
 // dst[i, :k] = src[idx[i], :k] (row gather); idx == nullptr copies.
 void gather_rows_f64(const double* src, int lds, double* dst, int ldd, int64_t n, int k,
                      const int32_t* idx, hipStream_t s);

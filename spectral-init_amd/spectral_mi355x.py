@@ -43,7 +43,7 @@ EXPORTED = [
     "si_test_gram", "si_test_block_mul", "si_test_small_sym_eig", "si_test_small_gen_eig",
     "si_test_fill_normal", "si_test_fast_solve", "si_test_batch_graphs", "si_test_spmm_shapes",
     "si_test_band_solve", "si_test_rcm_order", "si_test_knn_path",
-    "si_test_laplacian_storage",
+    "si_test_laplacian_storage", "si_test_components_device", "si_test_rcm_order_device",
     "si_profile_spmm_begin", "si_profile_spmm_end", "si_profile_gather_probe",
     "si_mpnn_param_count", "si_mpnn_create", "si_mpnn_destroy", "si_mpnn_get_params",
     "si_mpnn_set_params", "si_mpnn_features", "si_mpnn_forward", "si_mpnn_gradient",
@@ -214,6 +214,8 @@ def lib():
         L.si_test_band_solve.argtypes = [vp, vp, dbl, vp, vp, i32, C.POINTER(i32)]
         L.si_test_rcm_order.argtypes = [i64, vp, vp, i32, vp]
         L.si_test_laplacian_storage.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.si_test_components_device.argtypes = [vp, i64, vp, vp, vp, vp]
+        L.si_test_rcm_order_device.argtypes = [vp, i64, vp, vp, vp]
         L.si_test_knn_path.argtypes = [i32, C.POINTER(i64)]
         L.si_profile_spmm_begin.argtypes = [vp]
         L.si_profile_spmm_end.argtypes = [vp, C.POINTER(_SpmmProfile)]
@@ -803,6 +805,32 @@ def laplacian_storage(lap: "Laplacian"):
                                                    col.ctypes.data, a32.ctypes.data,
                                                    perm.ctypes.data))
     return rp, col, a32, perm
+
+
+def components_device(ctx: Context, indptr, indices):
+    """Testing seam: (labels, count) of a symmetric pattern, computed on the device."""
+    ip = np.ascontiguousarray(indptr, np.int64)
+    ix = np.ascontiguousarray(indices, np.int32)
+    n = ip.size - 1
+    lab = np.empty(n, np.int32)
+    nc = C.c_int64()
+    ctx._check(lib().si_test_components_device(ctx.handle, n, ip.ctypes.data,
+                                               ix.ctypes.data if ix.size else None,
+                                               lab.ctypes.data, C.byref(nc)))
+    return lab, nc.value
+
+
+def rcm_order_device(ctx: Context, indptr, indices) -> np.ndarray:
+    """Testing seam: the locality order of a symmetric pattern computed on the
+    device, perm[new] = old."""
+    ip = np.ascontiguousarray(indptr, np.int64)
+    ix = np.ascontiguousarray(indices, np.int32)
+    n = ip.size - 1
+    perm = np.empty(n, np.int32)
+    ctx._check(lib().si_test_rcm_order_device(ctx.handle, n, ip.ctypes.data,
+                                              ix.ctypes.data if ix.size else None,
+                                              perm.ctypes.data))
+    return perm
 
 
 def rcm_order(indptr, indices, threads: int = 0) -> np.ndarray:
