@@ -141,11 +141,11 @@ void check_host_csr(int64_t n, const int64_t* ip) {
         SI_CHECK(ip[i + 1] >= ip[i], si::kErrInvalidGraph, "invalid graph: indptr must be non-decreasing");
 }
 
-// Solve one connected graph: build L, run the chain, return host eigenpairs.
-int solve_graph(si::Context& c, int64_t n, const int64_t* ip, const int32_t* ix, const float* w,
-                int n_components, uint64_t seed, int mode, const si::ChainOptions& opt,
-                std::vector<double>& ev, std::vector<double>& vec, si::SolveStats& st,
-                double* lap_ms, double* solve_ms) {
+// Upload a host graph and build its operator on the device (the build and
+// the SpMM autotuning count as Laplacian time).
+void build_operator(si::Context& c, si::DevLaplacian& L, int64_t n, const int64_t* ip,
+                    const int32_t* ix, const float* w, int mode, bool reorder, bool tune,
+                    double* lap_ms) {
     hipStream_t s = c.stream;
     DeviceGraph g;
     const auto t0 = std::chrono::steady_clock::now();
@@ -153,33 +153,48 @@ int solve_graph(si::Context& c, int64_t n, const int64_t* ip, const int32_t* ix,
     if (si::trace_on())
         std::fprintf(stderr, "[spectral_init] %-22s %8.1f ms\n", "graph upload",
                      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
-    si::DevLaplacian L;
-    {
-        si::EventTimer t(s);
-        si::laplacian_build_device(L, n, g.ip, g.ix, g.w, ip[n], mode, s, /*reorder=*/n >= 4096,
-                                   ip, ix, w);
-        if (opt.solver == SI_SOLVER_FAST) tune_once(L, s);
-        const double ms = t.stop_ms();
-        if (lap_ms) *lap_ms += ms;
-    }
+    si::EventTimer t(s);
+    si::laplacian_build_device(L, n, g.ip, g.ix, g.w, ip[n], mode, s, reorder, ip, ix);
+    if (tune) tune_once(L, s);
+    const double ms = t.stop_ms();
+    if (lap_ms) *lap_ms += ms;
+}
+
+// Run the chain on a built operator of a connected graph; host eigenpairs in
+// the caller's node order.
+int solve_operator(si::Context& c, const si::DevLaplacian& L, int n_components, uint64_t seed,
+                   const si::ChainOptions& opt, std::vector<double>& ev, std::vector<double>& vec,
+                   si::SolveStats& st, double* solve_ms) {
+    hipStream_t s = c.stream;
+    const int64_t n = L.n;
     const int k = n_components + 1;
     ev.assign(k, 0.0);
     si::DevBuf<double> dv((size_t)n * k), dx((size_t)n * k);
-    int level = -1;
-    try {
-        si::EventTimer t(s);
-        level = si::solve_eigenproblem_device(L, n_components, seed, opt, ev.data(), dv, st, s);
-        si::rows_to_external(L, dv, k, dx, k, k, s);
-        const double ms = t.stop_ms();
-        if (solve_ms) *solve_ms += ms;
-    } catch (...) {
-        si::laplacian_free(L);
-        throw;
-    }
+    si::EventTimer t(s);
+    const int level = si::solve_eigenproblem_device(L, n_components, seed, opt, ev.data(), dv, st, s);
+    si::rows_to_external(L, dv, k, dx, k, k, s);
+    const double ms = t.stop_ms();
+    if (solve_ms) *solve_ms += ms;
     vec.resize((size_t)n * k);
     si::to_host(vec.data(), dx.get(), vec.size(), s);
-    si::laplacian_free(L);
     return level;
+}
+
+// Frees a device operator when the scope ends.
+struct OperatorScope {
+    si::DevLaplacian L;
+    ~OperatorScope() { si::laplacian_free(L); }
+};
+
+// Solve one connected graph: build L, run the chain, return host eigenpairs.
+int solve_graph(si::Context& c, int64_t n, const int64_t* ip, const int32_t* ix, const float* w,
+                int n_components, uint64_t seed, int mode, const si::ChainOptions& opt,
+                std::vector<double>& ev, std::vector<double>& vec, si::SolveStats& st,
+                double* lap_ms, double* solve_ms) {
+    OperatorScope op;
+    build_operator(c, op.L, n, ip, ix, w, mode, /*reorder=*/n >= 4096,
+                   /*tune=*/opt.solver == SI_SOLVER_FAST, lap_ms);
+    return solve_operator(c, op.L, n_components, seed, opt, ev, vec, st, solve_ms);
 }
 
 // Solve a Laplacian given by FP64 off-diagonal values and degrees (the
@@ -222,26 +237,18 @@ namespace {
 // (one workgroup each, the reference's level 0), the others by the batched
 // FP32-bulk solver.  A component the batch leaves above tol stays unsolved here
 // and goes through the per-component escalation chain.
-void solve_components_batched(si::Context& cx, int64_t n, const int64_t* ip, const int32_t* ix,
-                              const float* w, const std::vector<int64_t>& labels,
+void solve_components_batched(si::Context& cx, const si::DevLaplacian& L,
+                              const std::vector<int64_t>& labels,
                               const std::vector<std::vector<int64_t>>& members, int d,
-                              uint64_t seed, int mode, const si::ChainOptions& opt,
+                              uint64_t seed, const si::ChainOptions& opt,
                               std::vector<std::vector<double>>& cev,
                               std::vector<std::vector<double>>& cvec, std::vector<int>& csolved,
-                              si::SolveStats& last, double* lap_ms, double* solve_ms) {
+                              si::SolveStats& last, double* solve_ms) {
     hipStream_t s = cx.stream;
     const int k = d + 1;
+    const int64_t n = L.n;
     const int64_t nc = (int64_t)members.size();
-    DeviceGraph g;
-    upload_graph(g, n, ip, ix, w, s);
-    si::DevLaplacian L;
     {
-        si::EventTimer t(s);
-        si::laplacian_build_device(L, n, g.ip, g.ix, g.w, ip[n], mode, s, /*reorder=*/true, ip, ix, w);
-        tune_once(L, s);
-        *lap_ms += t.stop_ms();
-    }
-    try {
         std::vector<int32_t> perm(n), iperm(n);
         if (L.perm) si::to_host(perm.data(), L.perm, n, s);
         else std::iota(perm.begin(), perm.end(), 0);
@@ -316,11 +323,7 @@ void solve_components_batched(si::Context& cx, int64_t n, const int64_t* ip, con
         }
         last.iterations += bst.rounds;
         last.spmm_calls += bst.spmm_calls;
-    } catch (...) {
-        si::laplacian_free(L);
-        throw;
     }
-    si::laplacian_free(L);
 }
 
 }  // namespace
@@ -411,7 +414,7 @@ si_status si_laplacian_build_ex(si_context* ctx, int64_t n, const int64_t* indpt
             DeviceGraph g;
             upload_graph(g, n, indptr, indices, data, s);
             si::laplacian_build_device(lap->L, n, g.ip, g.ix, g.w, indptr[n], compute_mode, s,
-                                       reorder, indptr, indices, data);
+                                       reorder, indptr, indices);
         }
         *out = lap.release();
     });
@@ -785,17 +788,44 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
                          std::chrono::duration<double, std::milli>(now - tp).count());
             tp = now;
         };
+        // The operator of the whole graph serves the single-component solve and
+        // the batched solve of every component.  For n >= 4096 it is built
+        // first, in locality order, whose device form also labels the
+        // components (symmetric graphs); otherwise the host sweep labels them.
+        OperatorScope whole;
+        double lap_ms = 0.0, solve_ms = 0.0;
+        const bool large = n >= 4096;
+        const bool fast = opt.solver == SI_SOLVER_FAST;
+        if (large) {
+            build_operator(ctx->c, whole.L, n, indptr, indices, data, mode, /*reorder=*/true,
+                           /*tune=*/false, &lap_ms);
+            phase("operator");
+        }
         std::vector<int64_t> labels;
-        const int64_t nc = find_components(n, indptr, indices, labels);
+        int64_t nc = 0;
+        if (whole.L.n_comp > 0) {
+            nc = whole.L.n_comp;
+            std::vector<int32_t> lab32(n);
+            si::to_host(lab32.data(), whole.L.comp, n, ctx->c.stream);
+            labels.assign(lab32.begin(), lab32.end());
+        } else {
+            nc = find_components(n, indptr, indices, labels);
+        }
         phase("components");
         std::vector<double> coords((size_t)n * d, 0.0);
         si::SolveStats last;
-        double lap_ms = 0.0, solve_ms = 0.0;
         std::vector<double> ev, vec, sel;
         if (nc == 1) {
-            last.level = solve_graph(ctx->c, n, indptr, indices, data, d, seed, mode, opt, ev, vec,
-                                     last, &lap_ms, &solve_ms);
-            phase("solve_graph");
+            if (!large)
+                build_operator(ctx->c, whole.L, n, indptr, indices, data, mode, /*reorder=*/false,
+                               /*tune=*/false, &lap_ms);
+            if (fast) {
+                si::EventTimer t(ctx->c.stream);
+                tune_once(whole.L, ctx->c.stream);
+                lap_ms += t.stop_ms();
+            }
+            last.level = solve_operator(ctx->c, whole.L, d, seed, opt, ev, vec, last, &solve_ms);
+            phase("solve");
             select_and_sign(ev, vec, n, k, d, coords);
         } else {
             // embed_disconnected (src/multi_component.rs:16-74)
@@ -929,9 +959,21 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
             };
             // the batched segment solver carries k = d + 1 <= 8 columns; wider
             // embeddings take the per-component loop
+            if (!(batch && d <= 7)) si::laplacian_free(whole.L);
             if (batch && d <= 7) {
-                solve_components_batched(ctx->c, n, indptr, indices, data, labels, members, d, seed,
-                                         mode, opt, cev, cvec, csolved, last, &lap_ms, &solve_ms);
+                // the batch needs every component as a contiguous row range:
+                // the locality order
+                if (!whole.L.perm)
+                    build_operator(ctx->c, whole.L, n, indptr, indices, data, mode,
+                                   /*reorder=*/true, /*tune=*/false, &lap_ms);
+                {
+                    si::EventTimer t(ctx->c.stream);
+                    tune_once(whole.L, ctx->c.stream);
+                    lap_ms += t.stop_ms();
+                }
+                solve_components_batched(ctx->c, whole.L, labels, members, d, seed, opt, cev, cvec,
+                                         csolved, last, &solve_ms);
+                phase("batched solve");
             }
             for (int64_t c = 0; c < nc; ++c)
                 if (!csolved[c] && (int64_t)members[c].size() >= 2 * d) solve_one(c);

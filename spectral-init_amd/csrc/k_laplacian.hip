@@ -138,14 +138,14 @@ __global__ void k_values(const int32_t* __restrict__ rp, const int32_t* __restri
 void laplacian_free(DevLaplacian& L) {
     hipFree(L.rowptr); hipFree(L.col); hipFree(L.a32); hipFree(L.l64);
     hipFree(L.deg); hipFree(L.sqrt_deg); hipFree(L.isd); hipFree(L.long_rows);
-    hipFree(L.perm); hipFree(L.iperm);
+    hipFree(L.perm); hipFree(L.iperm); hipFree(L.comp);
     L = DevLaplacian{};
 }
 
 void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
                             const int32_t* d_indices, const float* d_w, int64_t nnz, int mode,
                             hipStream_t s, bool reorder, const int64_t* h_indptr,
-                            const int32_t* h_indices, const float* h_w) {
+                            const int32_t* h_indices) {
     SI_CHECK(n > 0, kErrInvalidGraph, "laplacian: n must be > 0");
     SI_CHECK(nnz < (int64_t)INT32_MAX && n < (int64_t)INT32_MAX, kErrUnsupported,
              "laplacian: more than 2^31-1 stored entries or nodes is not supported");
@@ -237,35 +237,39 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
     const float* w_vals = d_w;
     float* d_w2 = nullptr;
     if (reorder && n > 1) {
-        // host copy of the pattern (downloaded only if the caller has none)
-        std::vector<int64_t> hp_buf;
-        std::vector<int32_t> hx_buf;
-        std::vector<float> hw_buf;
-        const int64_t* hp = h_indptr;
-        const int32_t* hx = h_indices;
-        const float* hw = h_w;
-        if (!hp || !hx || !hw) {
-            hp_buf.resize(n + 1);
-            hx_buf.resize(nnz);
-            hw_buf.resize(nnz);
-            SI_HIP(hipMemcpyAsync(hp_buf.data(), d_indptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, s));
-            if (nnz) {
-                SI_HIP(hipMemcpyAsync(hx_buf.data(), d_indices, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost, s));
-                SI_HIP(hipMemcpyAsync(hw_buf.data(), d_w, sizeof(float) * nnz, hipMemcpyDeviceToHost, s));
-            }
-            SI_HIP(hipStreamSynchronize(s));
-            hp = hp_buf.data(); hx = hx_buf.data(); hw = hw_buf.data();
-        }
-        phase("host copy");
-        std::vector<int32_t> perm = rcm_order(n, hp, hx);
-        phase("rcm order");
-        std::vector<int32_t> iperm(n);
-        for (int64_t i = 0; i < n; ++i) iperm[perm[i]] = (int32_t)i;
         SI_HIP(hipMalloc(&d_w2, sizeof(float) * (nnz > 0 ? nnz : 1)));
         SI_HIP(hipMalloc(&L.perm, sizeof(int32_t) * n));
         SI_HIP(hipMalloc(&L.iperm, sizeof(int32_t) * n));
-        SI_HIP(hipMemcpyAsync(L.perm, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
-        SI_HIP(hipMemcpyAsync(L.iperm, iperm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+        if (L.symmetric) {
+            SI_HIP(hipMalloc(&L.comp, sizeof(int32_t) * n));
+            L.n_comp = rcm_order_device(n, d_indptr, d_indices, L.perm, L.iperm, s, L.comp);
+            phase("rcm order (device)");
+        } else {
+            // an asymmetric pattern: the host order, whose sweeps follow the
+            // stored (out-)edges; pattern downloaded if the caller has no copy
+            std::vector<int64_t> hp_buf;
+            std::vector<int32_t> hx_buf;
+            const int64_t* hp = h_indptr;
+            const int32_t* hx = h_indices;
+            if (!hp || !hx) {
+                hp_buf.resize(n + 1);
+                hx_buf.resize(nnz);
+                SI_HIP(hipMemcpyAsync(hp_buf.data(), d_indptr, sizeof(int64_t) * (n + 1),
+                                      hipMemcpyDeviceToHost, s));
+                if (nnz)
+                    SI_HIP(hipMemcpyAsync(hx_buf.data(), d_indices, sizeof(int32_t) * nnz,
+                                          hipMemcpyDeviceToHost, s));
+                SI_HIP(hipStreamSynchronize(s));
+                hp = hp_buf.data();
+                hx = hx_buf.data();
+            }
+            std::vector<int32_t> perm = rcm_order(n, hp, hx);
+            std::vector<int32_t> iperm(n);
+            for (int64_t i = 0; i < n; ++i) iperm[perm[i]] = (int32_t)i;
+            SI_HIP(hipMemcpyAsync(L.perm, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+            SI_HIP(hipMemcpyAsync(L.iperm, iperm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+            phase("rcm order (host)");
+        }
         // rows renumbered and sorted on the device, from the input CSR
         permute_csr_device(n, nnz, d_indptr, d_indices, d_w, L.perm, L.iperm, L.rowptr, L.col,
                            d_w2, s);

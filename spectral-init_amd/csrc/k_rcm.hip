@@ -311,11 +311,12 @@ int64_t components_device(int64_t n, const int64_t* rp, const int32_t* ix, int32
     return nc;
 }
 
-void rcm_order_device(int64_t n, const int64_t* rp, const int32_t* ix, int32_t* perm,
-                      int32_t* iperm, hipStream_t s) {
+int64_t rcm_order_device(int64_t n, const int64_t* rp, const int32_t* ix, int32_t* perm,
+                         int32_t* iperm, hipStream_t s, int32_t* lab_out) {
     SI_CHECK(n >= 1 && n < (int64_t)INT32_MAX, kErrUnsupported, "rcm: n out of range");
-    DevBuf<int32_t> lab(n), start(n);
-    const int64_t nc = components_device(n, rp, ix, lab.get(), start.get(), s);
+    DevBuf<int32_t> lab_own(lab_out ? 0 : n), start(n);
+    int32_t* lab = lab_out ? lab_out : lab_own.get();
+    const int64_t nc = components_device(n, rp, ix, lab, start.get(), s);
     DevBuf<int32_t> queue(n), mark(n), lvl(n), cnt(n + 1), off(n + 1), maxlev(nc);
     DevBuf<unsigned long long> key(n), skey(n), skey2(n), best(nc);
     DevBuf<int> big(1);
@@ -355,7 +356,7 @@ void rcm_order_device(int64_t n, const int64_t* rp, const int32_t* ix, int32_t* 
             SI_HIP(hipMemsetAsync(big.get(), 0, sizeof(int), s));
             hipLaunchKernelGGL(k_rcm_emit, grid_of(m * 64), dim3(256), 0, s, ls, le, m, rp, ix,
                                queue.get(), mark.get(), epoch, key.get(), off.get(), lvl.get(),
-                               level, lab.get(), maxlev.get(), sorted, skey.get(), big.get());
+                               level, lab, maxlev.get(), sorted, skey.get(), big.get());
             SI_LAUNCH_CHECK();
             SI_HIP(hipMemcpyAsync(&rb.total, off.get() + m, sizeof(int32_t), hipMemcpyDeviceToHost, s));
             SI_HIP(hipMemcpyAsync(&rb.big, big.get(), sizeof(int), hipMemcpyDeviceToHost, s));
@@ -383,7 +384,7 @@ void rcm_order_device(int64_t n, const int64_t* rp, const int32_t* ix, int32_t* 
             hipLaunchKernelGGL(k_fill_u64, grid_of(nc), dim3(256), 0, s, best.get(), nc,
                                (unsigned long long)kNoClaim);
             hipLaunchKernelGGL(k_rcm_last_level, grid_of(n), dim3(256), 0, s, n, rp, queue.get(),
-                               lvl.get(), lab.get(), maxlev.get(), best.get());
+                               lvl.get(), lab, maxlev.get(), best.get());
             hipLaunchKernelGGL(k_rcm_next_start, grid_of(nc), dim3(256), 0, s, (int32_t)nc,
                                best.get(), queue.get(), start.get());
             SI_LAUNCH_CHECK();
@@ -398,7 +399,7 @@ void rcm_order_device(int64_t n, const int64_t* rp, const int32_t* ix, int32_t* 
         ql.alloc(n);
         ql2.alloc(n);
         sorted_q.alloc(n);
-        hipLaunchKernelGGL(k_queue_labels, grid_of(n), dim3(256), 0, s, n, queue.get(), lab.get(),
+        hipLaunchKernelGGL(k_queue_labels, grid_of(n), dim3(256), 0, s, n, queue.get(), lab,
                            ql.get());
         int bits = 1;
         while (bits < 31 && ((int64_t)1 << bits) < nc) ++bits;
@@ -414,6 +415,7 @@ void rcm_order_device(int64_t n, const int64_t* rp, const int32_t* ix, int32_t* 
     hipLaunchKernelGGL(k_rcm_reverse, grid_of(n), dim3(256), 0, s, n, order, perm, iperm);
     SI_LAUNCH_CHECK();
     SI_HIP(hipStreamSynchronize(s));
+    return nc;
 }
 
 }  // namespace si

@@ -39,6 +39,11 @@ struct DevLaplacian {
     int degree_exact = 1;    // PythonCompat degrees reproduce the f32 column-sum order
     int32_t* perm = nullptr;   // perm[new] = old node id (nullptr: identity order)
     int32_t* iperm = nullptr;  // iperm[old] = new node id
+    // connected components, when the device locality order computed them
+    // (n_comp = 0: not computed): comp[old node id] = label, numbered by
+    // smallest node
+    int64_t n_comp = 0;
+    int32_t* comp = nullptr;
     // FP32 wide-SpMM shapes for 16 and 32 columns (index into the candidate
     // list of k_spmm.hip; 0 = default), set by spmm_autotune
     int shape16 = 0;
@@ -52,12 +57,12 @@ struct DevLaplacian {
 // malformed input.
 // reorder = true renumbers the nodes in reverse Cuthill-McKee order for gather
 // locality (degrees are computed in the input order first, so PythonCompat
-// stays bit-exact); h_* optionally give the same CSR on the host.
+// stays bit-exact); the order is computed on the device for a symmetric
+// graph, on the host otherwise, where h_* optionally give the same pattern.
 void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
                             const int32_t* d_indices, const float* d_w, int64_t nnz, int mode,
                             hipStream_t s, bool reorder = false,
-                            const int64_t* h_indptr = nullptr, const int32_t* h_indices = nullptr,
-                            const float* h_w = nullptr);
+                            const int64_t* h_indptr = nullptr, const int32_t* h_indices = nullptr);
 // Device Laplacian from host FP64 off-diagonal values (CSR without the
 // diagonal; unit diagonal implied) and FP64 degrees; input order is kept.
 void laplacian_from_values_device(DevLaplacian& L, int64_t n, const int64_t* h_ip,
@@ -82,8 +87,10 @@ int64_t components_device(int64_t n, const int64_t* rp, const int32_t* ix, int32
                           int32_t* root, hipStream_t s);
 // The order of rcm_order for a symmetric pattern, computed on the device:
 // perm[new] = old and iperm[old] = new (device arrays of n).
-void rcm_order_device(int64_t n, const int64_t* rp, const int32_t* ix, int32_t* perm,
-                      int32_t* iperm, hipStream_t s);
+// lab (optional, device, n entries) receives the component labels; returns
+// the number of components.
+int64_t rcm_order_device(int64_t n, const int64_t* rp, const int32_t* ix, int32_t* perm,
+                         int32_t* iperm, hipStream_t s, int32_t* lab = nullptr);
 // dst[i, :k] = src[idx[i], :k] (row gather); idx == nullptr copies.
 void gather_rows_f64(const double* src, int lds, double* dst, int ldd, int64_t n, int k,
                      const int32_t* idx, hipStream_t s);
