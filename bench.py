@@ -302,6 +302,8 @@ def main():
                     help="timed plain GPU LOBPCG runs (the median is the speed-up's denominator)")
     ap.add_argument("--plain-warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--e2e-runs", type=int, default=3,
+                    help="timed si_spectral_init runs on the host arrays (after one warm-up)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmm-only", action="store_true",
                     help="time the SpMM alone for widths 1..64 and exit")
@@ -500,6 +502,27 @@ def main():
         log(f"rSVD run {i}: {dt:.1f} ms residual={rres:.3e}")
         del rvec
 
+    # the drop-in end to end: si_spectral_init on host CSR arrays (upload,
+    # operator build with the device locality order, components, solve,
+    # selection, scaling and noise, coordinates back on the host)
+    e2e = []
+    ecfg = si.SpectralInitConfig(tol=args.tol)
+    for i in range(1 + args.e2e_runs):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _, erep = si.spectral_init((ip, ix, wt), args.n_components, 42, config=ecfg, ctx=ctx,
+                                   return_report=True)
+        dt = 1e3 * (time.perf_counter() - t)
+        if i > 0:
+            e2e.append(dict(ms=round(dt, 2), laplacian_ms=round(erep.laplacian_ms, 2),
+                            solve_ms=round(erep.solve_ms, 2), max_residual=erep.max_residual))
+        log(f"spectral_init end to end run {i}: {dt:.1f} ms (operator {erep.laplacian_ms:.1f} ms, "
+            f"solve {erep.solve_ms:.1f} ms)")
+    end_to_end = None
+    if e2e:
+        end_to_end = {"ms": float(np.median([r["ms"] for r in e2e])),
+                      "input": "host CSR arrays in, float32 coordinates out", "runs": e2e}
+
     dense = None
     if n <= 4096:
         # level 0/5 of the chain: the k smallest pairs of the dense Laplacian
@@ -575,6 +598,7 @@ def main():
                         "max_residual": last.max_residual, "eigenvalues": last.eigenvalues[:k],
                         "step_ms": [round(t, 3) for t in times]},
         "laplacian_build_ms": round(lap_ms, 2),
+        "spectral_init_end_to_end": end_to_end,
         "roofline": {"bound": "hbm", "kernel": "k_spmm_wide<float> (FP32 Chebyshev sweeps)",
                      "achieved": round(unit_gbs, 1) if unit_gbs else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",

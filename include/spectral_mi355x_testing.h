@@ -85,6 +85,11 @@ si_status si_profile_gather_probe(si_context* ctx, const si_laplacian* lap, int3
 si_status si_test_rcm_order(int64_t n, const int64_t* indptr, const int32_t* indices,
                             int32_t threads, int32_t* perm);
 
+/* Where si_spectral_init forms its output in later calls of this process
+ * (selection, signs, component placement, scaling, noise): 1 = on the device
+ * (the default), 0 = on the host.  Both give the same bits. */
+si_status si_test_output_stage(int32_t device);
+
 /* The device counterparts of the two host steps above, for a symmetric host
  * CSR pattern (uploaded here): component labels (numbered by smallest node, as
  * find_components) with their count, and the locality order perm[new] = old. */

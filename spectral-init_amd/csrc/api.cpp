@@ -92,6 +92,8 @@ int64_t find_components(int64_t n, const int64_t* ip, const int32_t* ix, std::ve
 // (src/scaling.rs:62-74) on host copies of the eigenvectors.
 void select_and_sign(const std::vector<double>& ev, const std::vector<double>& vec, int64_t n,
                      int k, int d, std::vector<double>& out) {
+    SI_CHECK((int)ev.size() == k && k > d && vec.size() == (size_t)n * k, si::kErrInternal,
+             "select_and_sign: eigenpairs missing");
     std::vector<int> idx(k);
     std::iota(idx.begin(), idx.end(), 0);
     std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return ev[a] < ev[b]; });
@@ -130,8 +132,9 @@ void upload_graph(DeviceGraph& g, int64_t n, const int64_t* ip, const int32_t* i
 }
 
 // SpMM shape autotuning for graphs large enough to repay it (~10 launches)
-void tune_once(si::DevLaplacian& L, hipStream_t s) {
-    if (!L.tuned && L.nnz >= (int64_t)1 << 20) si::spmm_autotune(L, s);
+void tune_once(si::DevLaplacian& L, hipStream_t s, int widths) {
+    widths &= ~L.tuned;
+    if (widths && L.nnz >= (int64_t)1 << 20) si::spmm_autotune(L, s, widths);
 }
 
 void check_host_csr(int64_t n, const int64_t* ip) {
@@ -144,7 +147,7 @@ void check_host_csr(int64_t n, const int64_t* ip) {
 // Upload a host graph and build its operator on the device (the build and
 // the SpMM autotuning count as Laplacian time).
 void build_operator(si::Context& c, si::DevLaplacian& L, int64_t n, const int64_t* ip,
-                    const int32_t* ix, const float* w, int mode, bool reorder, bool tune,
+                    const int32_t* ix, const float* w, int mode, bool reorder, int tune_widths,
                     double* lap_ms) {
     hipStream_t s = c.stream;
     DeviceGraph g;
@@ -155,16 +158,18 @@ void build_operator(si::Context& c, si::DevLaplacian& L, int64_t n, const int64_
                      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
     si::EventTimer t(s);
     si::laplacian_build_device(L, n, g.ip, g.ix, g.w, ip[n], mode, s, reorder, ip, ix);
-    if (tune) tune_once(L, s);
+    tune_once(L, s, tune_widths);
     const double ms = t.stop_ms();
     if (lap_ms) *lap_ms += ms;
 }
 
 // Run the chain on a built operator of a connected graph; host eigenpairs in
 // the caller's node order.
+// With `keep`, the eigenvector block stays on the device (in keep) and vec is
+// left empty.
 int solve_operator(si::Context& c, const si::DevLaplacian& L, int n_components, uint64_t seed,
                    const si::ChainOptions& opt, std::vector<double>& ev, std::vector<double>& vec,
-                   si::SolveStats& st, double* solve_ms) {
+                   si::SolveStats& st, double* solve_ms, si::DevBuf<double>* keep = nullptr) {
     hipStream_t s = c.stream;
     const int64_t n = L.n;
     const int k = n_components + 1;
@@ -175,9 +180,48 @@ int solve_operator(si::Context& c, const si::DevLaplacian& L, int n_components, 
     si::rows_to_external(L, dv, k, dx, k, k, s);
     const double ms = t.stop_ms();
     if (solve_ms) *solve_ms += ms;
-    vec.resize((size_t)n * k);
-    si::to_host(vec.data(), dx.get(), vec.size(), s);
+    if (keep) {
+        *keep = std::move(dx);
+        vec.clear();
+    } else {
+        vec.resize((size_t)n * k);
+        si::to_host(vec.data(), dx.get(), vec.size(), s);
+    }
     return level;
+}
+
+// The columns of an eigenvector block in selection order: ascending
+// eigenvalue (stable), the trivial first one skipped (selection.rs:11-29).
+std::vector<int32_t> selection_columns(const std::vector<double>& ev, int d) {
+    std::vector<int> idx(ev.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return ev[a] < ev[b]; });
+    std::vector<int32_t> cols(d);
+    for (int c = 0; c < d; ++c) cols[c] = idx[1 + c];
+    return cols;
+}
+
+// scale_and_add_noise (src/scaling.rs:78-89) of host coordinates: max-abs
+// 10, plus f32 N(0, 1e-4) noise (Philox stream kStreamNoise, index = the
+// flat output position, drawn on the device).
+void scale_and_add_noise(si::Context& c, const std::vector<double>& coords, int d, uint64_t seed,
+                         float* out) {
+    double mx = 0.0;
+    for (double v : coords) mx = std::max(mx, std::fabs(v));
+    if (!(mx > 0.0))
+        throw si::Error(si::kErrInvalidGraph,
+                        "invalid graph: scale_coords: embedding is all-zero; cannot scale to max=10");
+    const double ex = 10.0 / mx;
+    hipStream_t s = c.stream;
+    const size_t cnt = coords.size();
+    double* nz = si::ws_get<double>(si::kWsAbiOut, cnt);
+    si::fill_normal_f64(nz, (int64_t)(cnt / d), d, d, seed, si::kStreamNoise, s);
+    std::vector<double> hn(cnt);
+    si::to_host(hn.data(), nz, cnt, s);
+    for (size_t i = 0; i < cnt; ++i) {
+        const float noise = (float)(1e-4 * hn[i]);
+        out[i] = (float)(coords[i] * ex) + noise;
+    }
 }
 
 // Frees a device operator when the scope ends.
@@ -193,7 +237,10 @@ int solve_graph(si::Context& c, int64_t n, const int64_t* ip, const int32_t* ix,
                 double* lap_ms, double* solve_ms) {
     OperatorScope op;
     build_operator(c, op.L, n, ip, ix, w, mode, /*reorder=*/n >= 4096,
-                   /*tune=*/opt.solver == SI_SOLVER_FAST, lap_ms);
+                   opt.solver == SI_SOLVER_FAST
+                       ? si::fast_tune_widths(n, n_components + 1, opt.fast.block_size)
+                       : 0,
+                   lap_ms);
     return solve_operator(c, op.L, n_components, seed, opt, ev, vec, st, solve_ms);
 }
 
@@ -227,6 +274,9 @@ namespace si {
 // turn it off) and the statistics of the last batched solve
 bool g_batch_graphs = true;
 BatchStats g_last_batch;
+// spectral_init's output stage on the device (a test seam can select the host
+// form, which computes the same values)
+bool g_device_output = true;
 }  // namespace si
 
 namespace {
@@ -241,13 +291,20 @@ void solve_components_batched(si::Context& cx, const si::DevLaplacian& L,
                               const std::vector<int64_t>& labels,
                               const std::vector<std::vector<int64_t>>& members, int d,
                               uint64_t seed, const si::ChainOptions& opt,
-                              std::vector<std::vector<double>>& cev,
-                              std::vector<std::vector<double>>& cvec, std::vector<int>& csolved,
-                              si::SolveStats& last, double* solve_ms) {
+                              std::vector<std::vector<double>>& cev, si::DevBuf<double>& blk,
+                              std::vector<int>& csolved, si::SolveStats& last, double* solve_ms) {
     hipStream_t s = cx.stream;
     const int k = d + 1;
     const int64_t n = L.n;
     const int64_t nc = (int64_t)members.size();
+    auto tp = std::chrono::steady_clock::now();
+    auto phase = [&](const char* what) {  // SI_TRACE=1: phase timings
+        if (!si::trace_on()) return;
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[batched] %-26s %8.1f ms\n", what,
+                     std::chrono::duration<double, std::milli>(now - tp).count());
+        tp = now;
+    };
     {
         std::vector<int32_t> perm(n), iperm(n);
         if (L.perm) si::to_host(perm.data(), L.perm, n, s);
@@ -274,6 +331,7 @@ void solve_components_batched(si::Context& cx, const si::DevLaplacian& L,
                 icomp.push_back(c); irow0.push_back(first[c]); in.push_back(sz);
             }
         }
+        phase("segment ranges");
         si::DevBuf<double> block((size_t)n * k);
         std::vector<double> dev_h(dcomp.size() * k), iev(icomp.size() * k), resid;
         std::vector<int> ok;
@@ -299,16 +357,12 @@ void solve_components_batched(si::Context& cx, const si::DevLaplacian& L,
             }
             *solve_ms += t.stop_ms();
         }
-        std::vector<double> hb((size_t)n * k);
-        si::to_host(hb.data(), block.get(), hb.size(), s);
+        phase("device solve");
+        // the eigenvector blocks in the caller's node order
+        blk.alloc((size_t)n * k);
+        si::rows_to_external(L, block, k, blk, k, k, s);
         auto take = [&](int64_t c, const double* evc) {
-            const std::vector<int64_t>& mem = members[c];
             cev[c].assign(evc, evc + k);
-            cvec[c].resize(mem.size() * k);
-            for (size_t q = 0; q < mem.size(); ++q) {
-                const int64_t row = iperm[mem[q]];
-                for (int j = 0; j < k; ++j) cvec[c][q * k + j] = hb[(size_t)row * k + j];
-            }
             csolved[c] = 1;
         };
         for (size_t i = 0; i < dcomp.size(); ++i) {
@@ -323,6 +377,8 @@ void solve_components_batched(si::Context& cx, const si::DevLaplacian& L,
         }
         last.iterations += bst.rounds;
         last.spmm_calls += bst.spmm_calls;
+        SI_HIP(hipStreamSynchronize(s));
+        phase("eigenvector block");
     }
 }
 
@@ -704,7 +760,9 @@ si_status si_solve_eigenproblem(si_context* ctx, const si_laplacian* lap, int32_
         hipStream_t s = ctx->c.stream;
         // the FP32 sweeps pick their SpMM shape for this graph once (a cache
         // held by the Laplacian)
-        if (o.solver == SI_SOLVER_FAST) tune_once(const_cast<si::DevLaplacian&>(lap->L), s);
+        if (o.solver == SI_SOLVER_FAST)
+            tune_once(const_cast<si::DevLaplacian&>(lap->L), s,
+                      si::fast_tune_widths(lap->L.n, n_components + 1, o.fast.block_size));
         const si::DevLaplacian& L = lap->L;
         double* vi = si::ws_get<double>(si::kWsChainVec, (size_t)L.n * k);
         if (o.fast.warm_start) {
@@ -798,7 +856,7 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
         const bool fast = opt.solver == SI_SOLVER_FAST;
         if (large) {
             build_operator(ctx->c, whole.L, n, indptr, indices, data, mode, /*reorder=*/true,
-                           /*tune=*/false, &lap_ms);
+                           /*tune_widths=*/0, &lap_ms);
             phase("operator");
         }
         std::vector<int64_t> labels;
@@ -812,21 +870,38 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
             nc = find_components(n, indptr, indices, labels);
         }
         phase("components");
-        std::vector<double> coords((size_t)n * d, 0.0);
+        std::vector<double> coords;  // host coordinates (paths without the device output stage)
+        bool output_done = false;
         si::SolveStats last;
         std::vector<double> ev, vec, sel;
         if (nc == 1) {
             if (!large)
                 build_operator(ctx->c, whole.L, n, indptr, indices, data, mode, /*reorder=*/false,
-                               /*tune=*/false, &lap_ms);
+                               /*tune_widths=*/0, &lap_ms);
             if (fast) {
                 si::EventTimer t(ctx->c.stream);
-                tune_once(whole.L, ctx->c.stream);
+                tune_once(whole.L, ctx->c.stream, si::fast_tune_widths(n, k, opt.fast.block_size));
                 lap_ms += t.stop_ms();
             }
-            last.level = solve_operator(ctx->c, whole.L, d, seed, opt, ev, vec, last, &solve_ms);
+            si::DevBuf<double> blk;
+            last.level = solve_operator(ctx->c, whole.L, d, seed, opt, ev, vec, last, &solve_ms,
+                                        si::g_device_output ? &blk : nullptr);
             phase("solve");
-            select_and_sign(ev, vec, n, k, d, coords);
+            if (!si::g_device_output) {
+                select_and_sign(ev, vec, n, k, d, coords);
+            } else {
+                si::EmbedPlan plan;
+                plan.n = n;
+                plan.d = d;
+                plan.b = blk.get();
+                plan.ldb = k;
+                plan.kind = {1};
+                plan.cols = selection_columns(ev, d);
+                plan.seed = seed;
+                plan.out = out;
+                si::embed_output_device(plan, ctx->c.stream);
+                output_done = true;
+            }
         } else {
             // embed_disconnected (src/multi_component.rs:16-74)
             std::vector<std::vector<int64_t>> members(nc);
@@ -931,6 +1006,7 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
             // all at once on the device (batched), or one solve per component.
             std::vector<std::vector<double>> cev(nc), cvec(nc);
             std::vector<int> csolved(nc, 0);
+            si::DevBuf<double> blk;  // batch-solved eigenvectors, caller's node order
             std::vector<int64_t> lookup(n, -1);
             auto solve_one = [&](int64_t c) {
                 const std::vector<int64_t>& mem = members[c];
@@ -965,20 +1041,76 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
                 // the locality order
                 if (!whole.L.perm)
                     build_operator(ctx->c, whole.L, n, indptr, indices, data, mode,
-                                   /*reorder=*/true, /*tune=*/false, &lap_ms);
+                                   /*reorder=*/true, /*tune_widths=*/0, &lap_ms);
                 {
+                    // the batched solver's block is 16 columns wide
                     si::EventTimer t(ctx->c.stream);
-                    tune_once(whole.L, ctx->c.stream);
+                    tune_once(whole.L, ctx->c.stream, si::kTuned16);
                     lap_ms += t.stop_ms();
                 }
-                solve_components_batched(ctx->c, whole.L, labels, members, d, seed, opt, cev, cvec,
+                phase("spmm autotune");
+                solve_components_batched(ctx->c, whole.L, labels, members, d, seed, opt, cev, blk,
                                          csolved, last, &solve_ms);
                 phase("batched solve");
             }
+            // components left to the per-component chain (a batch failure, or
+            // no batch) make the placement a host step
+            bool all_batched = blk.get() != nullptr && si::g_device_output;
             for (int64_t c = 0; c < nc; ++c)
-                if (!csolved[c] && (int64_t)members[c].size() >= 2 * d) solve_one(c);
+                if (!csolved[c] && (int64_t)members[c].size() >= 2 * d) {
+                    all_batched = false;
+                    solve_one(c);
+                }
+            for (int64_t c = 0; c < nc; ++c)
+                if (!cev[c].empty()) ev = cev[c];
+            if (all_batched) {
+                // every component's placement, the scaling and the noise on the
+                // device, from the batch's block
+                si::DevBuf<int32_t> dlab;
+                const int32_t* lab = whole.L.comp;
+                if (!whole.L.n_comp) {
+                    std::vector<int32_t> l32(labels.begin(), labels.end());
+                    dlab.alloc(n);
+                    si::to_device(dlab.get(), l32.data(), n, ctx->c.stream);
+                    lab = dlab.get();
+                }
+                si::EmbedPlan plan;
+                plan.n = n;
+                plan.d = d;
+                plan.b = blk.get();
+                plan.ldb = k;
+                plan.lab = lab;
+                plan.nseg = (int)nc;
+                plan.kind.assign(nc, 0);
+                plan.cols.assign((size_t)nc * d, 0);
+                for (int64_t c = 0; c < nc; ++c) {
+                    if ((int64_t)members[c].size() < 2 * d) continue;
+                    plan.kind[c] = 1;
+                    const std::vector<int32_t> cc = selection_columns(cev[c], d);
+                    std::copy(cc.begin(), cc.end(), plan.cols.begin() + c * d);
+                }
+                plan.affine = true;
+                plan.range = range;
+                plan.meta = meta;
+                plan.seed = seed;
+                plan.out = out;
+                si::embed_output_device(plan, ctx->c.stream);
+                output_done = true;
+            } else if (blk.get()) {
+                // batch-solved components in member order, for the host placement
+                std::vector<double> hb((size_t)n * k);
+                si::to_host(hb.data(), blk.get(), hb.size(), ctx->c.stream);
+                for (int64_t c = 0; c < nc; ++c) {
+                    if (!csolved[c] || !cvec[c].empty()) continue;
+                    const std::vector<int64_t>& mem = members[c];
+                    cvec[c].resize(mem.size() * k);
+                    for (size_t q = 0; q < mem.size(); ++q)
+                        for (int j = 0; j < k; ++j) cvec[c][q * k + j] = hb[(size_t)mem[q] * k + j];
+                }
+            }
             // embed_single_component (multi_component.rs:261-312): place each component
-            for (int64_t c = 0; c < nc; ++c) {
+            if (!output_done) coords.assign((size_t)n * d, 0.0);
+            for (int64_t c = 0; c < nc && !output_done; ++c) {
                 const std::vector<int64_t>& mem = members[c];
                 const int64_t sz = (int64_t)mem.size();
                 if (sz < 2 * d) {
@@ -994,30 +1126,9 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
                     for (int e = 0; e < d; ++e)
                         coords[mem[q] * d + e] = sel[(size_t)q * d + e] * f + meta[c * d + e];
             }
-            for (int64_t c = 0; c < nc; ++c)
-                if (!cev[c].empty()) ev = cev[c];
         }
         phase("select and place");
-        // scale_and_add_noise (src/scaling.rs:78-89): max-abs 10, N(0, 1e-4) f32 noise
-        double mx = 0.0;
-        for (double v : coords) mx = std::max(mx, std::fabs(v));
-        if (!(mx > 0.0))
-            throw si::Error(si::kErrInvalidGraph, "invalid graph: scale_coords: embedding is all-zero; cannot scale to max=10");
-        const double ex = 10.0 / mx;
-        // the noise draws (Philox stream kStreamNoise, index = the flat output
-        // position) are generated on the device and downloaded
-        {
-            hipStream_t s = ctx->c.stream;
-            const size_t cnt = coords.size();
-            double* nz = si::ws_get<double>(si::kWsAbiOut, cnt);
-            si::fill_normal_f64(nz, (int64_t)(cnt / d), d, d, seed, si::kStreamNoise, s);
-            std::vector<double> hn(cnt);
-            si::to_host(hn.data(), nz, cnt, s);
-            for (size_t i = 0; i < cnt; ++i) {
-                const float noise = (float)(1e-4 * hn[i]);
-                out[i] = (float)(coords[i] * ex) + noise;
-            }
-        }
+        if (!output_done) scale_and_add_noise(ctx->c, coords, d, seed, out);
         phase("scale and noise");
         if (rep) {
             fill_report(rep, last, ev.empty() ? nullptr : ev.data(), ev.empty() ? 0 : (int)ev.size(), solve_ms);

@@ -28,6 +28,7 @@ si_status tguard(si_context* ctx, F&& f) {
 
 namespace si {
 extern bool g_batch_graphs;
+extern bool g_device_output;
 extern BatchStats g_last_batch;
 }  // namespace si
 
@@ -121,7 +122,8 @@ si_status si_test_fast_solve(si_context* ctx, const si_laplacian* lap, int32_t n
         const int k = n_components + 1;
         hipStream_t s = ctx->c.stream;
         si::DevLaplacian& ml = const_cast<si::DevLaplacian&>(lap->L);
-        if (!ml.tuned && ml.nnz >= (int64_t)1 << 20) si::spmm_autotune(ml, s);
+        const int tw = si::fast_tune_widths(ml.n, k, o.block_size) & ~ml.tuned;
+        if (tw && ml.nnz >= (int64_t)1 << 20) si::spmm_autotune(ml, s, tw);
         si::DevBuf<double> vi((size_t)lap->L.n * k), wsi;
         if (o.warm_start) {
             wsi.alloc((size_t)lap->L.n * k);
@@ -222,7 +224,7 @@ si_status si_test_spmm_shapes(si_context* ctx, si_laplacian* lap, int32_t shape1
                      si::kErrInvalidArgument, "spmm shape out of range");
             lap->L.shape16 = shape16;
             lap->L.shape32 = shape32;
-            lap->L.tuned = 1;
+            lap->L.tuned = si::kTuned16 | si::kTuned32;
         }
         if (out16) *out16 = lap->L.shape16;
         if (out32) *out32 = lap->L.shape32;
@@ -293,6 +295,12 @@ si_status si_test_laplacian_storage(si_context* ctx, const si_laplacian* lap, in
             else for (int64_t i = 0; i < L.n; ++i) perm[i] = (int32_t)i;
         }
     });
+}
+
+si_status si_test_output_stage(int32_t device) {
+    if (device != 0 && device != 1) return SI_ERR_INVALID_ARGUMENT;
+    si::g_device_output = device != 0;
+    return SI_OK;
 }
 
 si_status si_test_knn_path(int32_t mode, int64_t* last_direct) {

@@ -19,7 +19,7 @@
 // of rows (its L2 then holds the x rows that neighbouring rows gather).
 #include <hip/hip_ext.h>
 
-#include "si_device.hpp"
+#include "si_host.hpp"
 
 namespace si {
 
@@ -575,22 +575,24 @@ bool spmm_width_supported(int k) { return k >= 1 && k <= 4096; }
 
 int spmm_shape_count() { return 4; }
 
-void spmm_autotune(DevLaplacian& L, hipStream_t s) {
+void spmm_autotune(DevLaplacian& L, hipStream_t s, int widths) {
     const int64_t n = L.n;
-    const int reps = 3;
-    float* x = nullptr;
-    float* y = nullptr;
+    const int reps = 2;
+    const int wmax = (widths & kTuned32) ? 32 : 16;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     SpmmProfile* saved = g_profile;
     g_profile = nullptr;  // tuning launches are not part of a profiled region
     try {
-        SI_HIP(hipMalloc(&x, sizeof(float) * n * 32));
-        SI_HIP(hipMalloc(&y, sizeof(float) * n * 32));
+        // the solvers' FP32 block buffers (grow-only workspace), refilled by
+        // every solve
+        float* x = ws_get<float>(kWsFastX32, (size_t)n * wmax);
+        float* y = ws_get<float>(kWsFastAx32, (size_t)n * wmax);
         // any finite block: the x row values do not change the access pattern
-        SI_HIP(hipMemsetAsync(x, 0, sizeof(float) * n * 32, s));
+        SI_HIP(hipMemsetAsync(x, 0, sizeof(float) * n * wmax, s));
         SI_HIP(hipEventCreate(&e0));
         SI_HIP(hipEventCreate(&e1));
         for (int width : {16, 32}) {
+            if (!(widths & (width == 16 ? kTuned16 : kTuned32))) continue;
             float best = 0.f;
             int pick = 0;
             for (int c = 0; c < spmm_shape_count(); ++c) {
@@ -605,25 +607,22 @@ void spmm_autotune(DevLaplacian& L, hipStream_t s) {
                 if (c == 0 || ms < best) { best = ms; pick = c; }
             }
             (width == 16 ? L.shape16 : L.shape32) = pick;
+            L.tuned |= width == 16 ? kTuned16 : kTuned32;
             if (trace_on())
                 std::fprintf(stderr, "[autotune] FP32 width %d: shape %d (%.1f us/launch)\n", width,
                              pick, 1e3 * best / reps);
         }
-        L.tuned = 1;
     } catch (...) {
         g_profile = saved;
         if (e0) hipEventDestroy(e0);
         if (e1) hipEventDestroy(e1);
-        if (x) hipFree(x);
-        if (y) hipFree(y);
-        L.shape16 = L.shape32 = 0;
+        if (widths & kTuned16) L.shape16 = 0;
+        if (widths & kTuned32) L.shape32 = 0;
         throw;
     }
     g_profile = saved;
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    hipFree(x);
-    hipFree(y);
 }
 
 void spmm_f32(const DevLaplacian& L, const float* x, float* out, int k, int ld, float s_l,

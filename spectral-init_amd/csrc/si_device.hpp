@@ -45,7 +45,8 @@ struct DevLaplacian {
     int64_t n_comp = 0;
     int32_t* comp = nullptr;
     // FP32 wide-SpMM shapes for 16 and 32 columns (index into the candidate
-    // list of k_spmm.hip; 0 = default), set by spmm_autotune
+    // list of k_spmm.hip; 0 = default), set by spmm_autotune; tuned holds the
+    // widths already tuned (kTuned16 | kTuned32)
     int shape16 = 0;
     int shape32 = 0;
     int tuned = 0;
@@ -121,7 +122,10 @@ void csr_spmv_exact_f64(int64_t n, const int64_t* ip, const int32_t* ix, const d
 bool spmm_profiling();
 // Pick the FP32 16- and 32-column SpMM shapes for this graph by timing every
 // candidate on it (hipEvents, a few launches each); sets L.shape16/32, L.tuned.
-void spmm_autotune(DevLaplacian& L, hipStream_t s);
+constexpr int kTuned16 = 1, kTuned32 = 2;
+// Times every candidate shape of the FP32 16- and/or 32-column SpMM (widths:
+// kTuned16 | kTuned32) on this graph and keeps the fastest.
+void spmm_autotune(DevLaplacian& L, hipStream_t s, int widths = kTuned16 | kTuned32);
 int spmm_shape_count();  // candidates per width
 // Average microseconds per launch of the gather probe (the SpMM's 64-byte row
 // gathers alone; mode 0 along the column stream, 1 random rows, 2 random rows

@@ -154,6 +154,35 @@ bool lobpcg_solve_plain(const DevLaplacian& L, int n_components, uint64_t seed, 
                         hipStream_t s,
                         const double* warm_start = nullptr);
 
+// The output stage of spectral_init on the device (select_eigenvectors,
+// normalize_signs, embed_single_component's placement, scale_and_add_noise):
+// b holds the eigenvector block of every segment with its own embedding, in
+// the caller's node order.
+struct EmbedPlan {
+    int64_t n = 0;
+    int d = 0;                     // embedding dimension
+    const double* b = nullptr;     // device, n x ldb
+    int ldb = 0;
+    const int32_t* lab = nullptr;  // device segment label per node (nullptr: one segment)
+    int nseg = 1;
+    std::vector<int32_t> kind;     // per segment: 1 = own eigenvectors, 0 = meta only
+    std::vector<int32_t> cols;     // per segment: d columns of b in selection order
+    bool affine = false;           // coordinates sel * (range / max|sel|) + meta
+    std::vector<double> range;     // per segment
+    std::vector<double> meta;      // per segment, d values
+    uint64_t seed = 0;
+    float* out = nullptr;          // host, n x d
+};
+void embed_output_device(const EmbedPlan& p, hipStream_t s);
+
+// Block width of fast_solve for n nodes, k = n_components + 1 wanted pairs and
+// a requested width (0: automatic), and the SpMM widths to tune for it.
+int fast_block_width(int64_t n, int k, int requested);
+inline int fast_tune_widths(int64_t n, int k, int requested) {
+    const int m = fast_block_width(n, k, requested);
+    return m == 16 ? kTuned16 : (m == 32 ? kTuned32 : 0);
+}
+
 // FP32-bulk Chebyshev-filtered block solver; FP64 Rayleigh-Ritz and residual.
 bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const FastOptions& opt,
                 double* evals_host, double* evecs_dev, SolveStats& st, hipStream_t s);

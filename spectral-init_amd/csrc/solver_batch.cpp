@@ -21,7 +21,10 @@
 // The FP64 finish (Rayleigh-Ritz on the lowest mf Ritz vectors and the FP64
 // residual against the exact Laplacian) is segmented the same way.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstring>
 #include <map>
 
 #include "si_host.hpp"
@@ -219,12 +222,28 @@ void fast_solve_segments(const DevLaplacian& L, const std::vector<int64_t>& row0
         for (auto& kv : graphs) hipGraphExecDestroy(kv.second);
         graphs.clear();
     };
+    // a round's host side: the coefficient table goes up through page-locked
+    // memory without a wait, and the readback below is the round's only
+    // synchronisation (SI_TRACE=1 reports the time spent waiting for the
+    // device and the time spent in host decisions)
+    using clk = std::chrono::steady_clock;
+    double wait_ms = 0.0, host_ms = 0.0;
+    clk::time_point th0 = clk::now();
+    auto upload_async = [&](void* dst, const void* src, size_t bytes) {
+        void* h = pinned_stage(bytes);
+        std::memcpy(h, src, bytes);
+        SI_HIP(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, s));
+    };
     auto read_round = [&](int w) {
+        const clk::time_point t0 = clk::now();
+        host_ms += std::chrono::duration<double, std::milli>(t0 - th0).count();
         SI_HIP(hipMemcpyAsync(hstat.data(), status, sizeof(int) * nseg, hipMemcpyDeviceToHost, s));
         SI_HIP(hipMemcpyAsync(th.data(), theta, sizeof(double) * nseg * w, hipMemcpyDeviceToHost, s));
         SI_HIP(hipMemcpyAsync(nrm.data(), norms2, sizeof(double) * nseg * 2 * k,
                               hipMemcpyDeviceToHost, s));
         SI_HIP(hipStreamSynchronize(s));
+        th0 = clk::now();
+        wait_ms += std::chrono::duration<double, std::milli>(th0 - t0).count();
     };
     auto seg_worst = [&](int c) {
         double wv = 0.0;
@@ -238,6 +257,8 @@ void fast_solve_segments(const DevLaplacian& L, const std::vector<int64_t>& row0
     };
 
     try {
+        th0 = clk::now();
+        double capture_ms = 0.0;
         // ---------------- FP32 rounds
         for (int round = 0; round < opt.max_outer; ++round) {
             int steps = 0;
@@ -249,7 +270,7 @@ void fast_solve_segments(const DevLaplacian& L, const std::vector<int64_t>& row0
             }
             if (!any) break;
             coef_table<float>(a, deg, b, steps, tab);
-            to_device(coef32, tab.data(), tab.size(), s);
+            upload_async(coef32, tab.data(), sizeof(float) * tab.size());
             auto body = [&] {
                 float* y = enqueue_filter<float>(L, r32, x32, m, m, steps, rseg, coef32, nseg,
                                                  st.spmm_calls, s);
@@ -260,6 +281,7 @@ void fast_solve_segments(const DevLaplacian& L, const std::vector<int64_t>& row0
             if (graphs_ok) {
                 auto it = graphs.find(steps);
                 if (it == graphs.end()) {
+                    const clk::time_point tc0 = clk::now();
                     hipGraph_t gr = nullptr;
                     hipGraphExec_t ex = nullptr;
                     const int64_t calls0 = st.spmm_calls;
@@ -277,6 +299,7 @@ void fast_solve_segments(const DevLaplacian& L, const std::vector<int64_t>& row0
                     it = graphs.emplace(steps, ex).first;
                     ++st.graphs_captured;
                     st.spmm_calls = calls0;  // counted when launched
+                    capture_ms += std::chrono::duration<double, std::milli>(clk::now() - tc0).count();
                 }
                 SI_HIP(hipGraphLaunch(it->second, s));
                 ++st.graph_launches;
@@ -331,7 +354,7 @@ void fast_solve_segments(const DevLaplacian& L, const std::vector<int64_t>& row0
             }
             if (steps == 0) break;
             coef_table<double>(a64, deg, b, steps, tab64);
-            to_device(coef64, tab64.data(), tab64.size(), s);
+            upload_async(coef64, tab64.data(), sizeof(double) * tab64.size());
             double* y = enqueue_filter<double>(L, r64, x64, mf, m, steps, rseg, coef64, nseg,
                                                st.spmm_calls, s);
             double* ly = (y == r64.b[0]) ? r64.b[1] : r64.b[0];
@@ -352,6 +375,10 @@ void fast_solve_segments(const DevLaplacian& L, const std::vector<int64_t>& row0
         }
         copy_cols_f64(x64, m, evecs_dev, k, n, k, s);
         SI_HIP(hipStreamSynchronize(s));
+        if (trace_on())
+            std::fprintf(stderr, "[batch] %d segments, %lld rounds: %.2f ms waiting for the device, "
+                         "%.2f ms on the host between rounds (%.2f ms of it graph capture)\n",
+                         nseg, (long long)st.rounds, wait_ms, host_ms, capture_ms);
         for (int c = 0; c < nseg; ++c) {
             for (int j = 0; j < k; ++j) evals_host[(size_t)c * k + j] = thk[(size_t)c * mf + j];
             resid[c] = worst[c];

@@ -263,6 +263,8 @@ bool fused_rr(const DevLaplacian& L, int64_t n, T* y, T* ly, int w, int ld, int 
 
 }  // namespace
 
+int fast_block_width(int64_t n, int k, int requested) { return choose_block(n, k, requested); }
+
 bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const FastOptions& opt,
                 double* evals_host, double* evecs_dev, SolveStats& st, hipStream_t s) {
     const int64_t n = L.n;

@@ -44,6 +44,7 @@ EXPORTED = [
     "si_test_fill_normal", "si_test_fast_solve", "si_test_batch_graphs", "si_test_spmm_shapes",
     "si_test_band_solve", "si_test_rcm_order", "si_test_knn_path",
     "si_test_laplacian_storage", "si_test_components_device", "si_test_rcm_order_device",
+    "si_test_output_stage",
     "si_profile_spmm_begin", "si_profile_spmm_end", "si_profile_gather_probe",
     "si_mpnn_param_count", "si_mpnn_create", "si_mpnn_destroy", "si_mpnn_get_params",
     "si_mpnn_set_params", "si_mpnn_features", "si_mpnn_forward", "si_mpnn_gradient",
@@ -216,6 +217,7 @@ def lib():
         L.si_test_laplacian_storage.argtypes = [vp, vp, vp, vp, vp, vp]
         L.si_test_components_device.argtypes = [vp, i64, vp, vp, vp, vp]
         L.si_test_rcm_order_device.argtypes = [vp, i64, vp, vp, vp]
+        L.si_test_output_stage.argtypes = [i32]
         L.si_test_knn_path.argtypes = [i32, C.POINTER(i64)]
         L.si_profile_spmm_begin.argtypes = [vp]
         L.si_profile_spmm_end.argtypes = [vp, C.POINTER(_SpmmProfile)]
@@ -805,6 +807,14 @@ def laplacian_storage(lap: "Laplacian"):
                                                    col.ctypes.data, a32.ctypes.data,
                                                    perm.ctypes.data))
     return rp, col, a32, perm
+
+
+def set_output_stage(device: bool) -> None:
+    """Testing seam: form spectral_init's output on the device (default) or on
+    the host (the same values)."""
+    st = lib().si_test_output_stage(1 if device else 0)
+    if st != SI_OK:
+        raise _ERRORS.get(st, SpectralError)(f"status {st}")
 
 
 def components_device(ctx: Context, indptr, indices):
