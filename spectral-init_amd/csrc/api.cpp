@@ -292,7 +292,8 @@ void solve_components_batched(si::Context& cx, const si::DevLaplacian& L,
                               const std::vector<std::vector<int64_t>>& members, int d,
                               uint64_t seed, const si::ChainOptions& opt,
                               std::vector<std::vector<double>>& cev, si::DevBuf<double>& blk,
-                              std::vector<int>& csolved, si::SolveStats& last, double* solve_ms) {
+                              std::vector<int64_t>& seg_row0, std::vector<int>& csolved,
+                              si::SolveStats& last, double* solve_ms) {
     hipStream_t s = cx.stream;
     const int k = d + 1;
     const int64_t n = L.n;
@@ -306,10 +307,9 @@ void solve_components_batched(si::Context& cx, const si::DevLaplacian& L,
         tp = now;
     };
     {
-        std::vector<int32_t> perm(n), iperm(n);
+        std::vector<int32_t> perm(n);
         if (L.perm) si::to_host(perm.data(), L.perm, n, s);
         else std::iota(perm.begin(), perm.end(), 0);
-        for (int64_t r = 0; r < n; ++r) iperm[perm[r]] = (int32_t)r;
         // row range of every component (the locality order keeps them contiguous)
         std::vector<int64_t> first(nc, -1), last_row(nc, -1);
         for (int64_t r = 0; r < n; ++r) {
@@ -358,9 +358,9 @@ void solve_components_batched(si::Context& cx, const si::DevLaplacian& L,
             *solve_ms += t.stop_ms();
         }
         phase("device solve");
-        // the eigenvector blocks in the caller's node order
-        blk.alloc((size_t)n * k);
-        si::rows_to_external(L, block, k, blk, k, k, s);
+        // the eigenvector blocks stay in the operator's row order, where every
+        // component is the row range starting at seg_row0[c]
+        seg_row0 = first;
         auto take = [&](int64_t c, const double* evc) {
             cev[c].assign(evc, evc + k);
             csolved[c] = 1;
@@ -377,7 +377,7 @@ void solve_components_batched(si::Context& cx, const si::DevLaplacian& L,
         }
         last.iterations += bst.rounds;
         last.spmm_calls += bst.spmm_calls;
-        SI_HIP(hipStreamSynchronize(s));
+        blk = std::move(block);
         phase("eigenvector block");
     }
 }
@@ -896,6 +896,8 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
                 plan.b = blk.get();
                 plan.ldb = k;
                 plan.kind = {1};
+                plan.row0 = {0};
+                plan.rows = {n};
                 plan.cols = selection_columns(ev, d);
                 plan.seed = seed;
                 plan.out = out;
@@ -1006,7 +1008,8 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
             // all at once on the device (batched), or one solve per component.
             std::vector<std::vector<double>> cev(nc), cvec(nc);
             std::vector<int> csolved(nc, 0);
-            si::DevBuf<double> blk;  // batch-solved eigenvectors, caller's node order
+            si::DevBuf<double> blk;        // batch-solved eigenvectors, operator row order
+            std::vector<int64_t> seg_row0;  // each component's first row in blk
             std::vector<int64_t> lookup(n, -1);
             auto solve_one = [&](int64_t c) {
                 const std::vector<int64_t>& mem = members[c];
@@ -1050,7 +1053,7 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
                 }
                 phase("spmm autotune");
                 solve_components_batched(ctx->c, whole.L, labels, members, d, seed, opt, cev, blk,
-                                         csolved, last, &solve_ms);
+                                         seg_row0, csolved, last, &solve_ms);
                 phase("batched solve");
             }
             // components left to the per-component chain (a batch failure, or
@@ -1079,13 +1082,19 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
                 plan.d = d;
                 plan.b = blk.get();
                 plan.ldb = k;
+                plan.rowmap = whole.L.iperm;
+                plan.rownode = whole.L.perm;
                 plan.lab = lab;
                 plan.nseg = (int)nc;
                 plan.kind.assign(nc, 0);
                 plan.cols.assign((size_t)nc * d, 0);
+                plan.row0.assign(nc, 0);
+                plan.rows.assign(nc, 0);
                 for (int64_t c = 0; c < nc; ++c) {
                     if ((int64_t)members[c].size() < 2 * d) continue;
                     plan.kind[c] = 1;
+                    plan.row0[c] = seg_row0[c];
+                    plan.rows[c] = (int64_t)members[c].size();
                     const std::vector<int32_t> cc = selection_columns(cev[c], d);
                     std::copy(cc.begin(), cc.end(), plan.cols.begin() + c * d);
                 }
@@ -1099,13 +1108,17 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
             } else if (blk.get()) {
                 // batch-solved components in member order, for the host placement
                 std::vector<double> hb((size_t)n * k);
+                std::vector<int32_t> iperm(n);
                 si::to_host(hb.data(), blk.get(), hb.size(), ctx->c.stream);
+                si::to_host(iperm.data(), whole.L.iperm, n, ctx->c.stream);
                 for (int64_t c = 0; c < nc; ++c) {
                     if (!csolved[c] || !cvec[c].empty()) continue;
                     const std::vector<int64_t>& mem = members[c];
                     cvec[c].resize(mem.size() * k);
-                    for (size_t q = 0; q < mem.size(); ++q)
-                        for (int j = 0; j < k; ++j) cvec[c][q * k + j] = hb[(size_t)mem[q] * k + j];
+                    for (size_t q = 0; q < mem.size(); ++q) {
+                        const size_t row = (size_t)iperm[mem[q]];
+                        for (int j = 0; j < k; ++j) cvec[c][q * k + j] = hb[row * k + j];
+                    }
                 }
             }
             // embed_single_component (multi_component.rs:261-312): place each component

@@ -156,15 +156,19 @@ bool lobpcg_solve_plain(const DevLaplacian& L, int n_components, uint64_t seed, 
 
 // The output stage of spectral_init on the device (select_eigenvectors,
 // normalize_signs, embed_single_component's placement, scale_and_add_noise):
-// b holds the eigenvector block of every segment with its own embedding, in
-// the caller's node order.
+// b holds the eigenvector block of every segment with its own embedding, each
+// segment a contiguous row range [row0, row0 + rows) of it; rowmap / rownode
+// map the caller's node ids to rows of b and back (nullptr: the same order).
 struct EmbedPlan {
     int64_t n = 0;
-    int d = 0;                     // embedding dimension
-    const double* b = nullptr;     // device, n x ldb
+    int d = 0;                         // embedding dimension
+    const double* b = nullptr;         // device, n x ldb
     int ldb = 0;
-    const int32_t* lab = nullptr;  // device segment label per node (nullptr: one segment)
+    const int32_t* rowmap = nullptr;   // device, node -> row of b
+    const int32_t* rownode = nullptr;  // device, row of b -> node
+    const int32_t* lab = nullptr;      // device segment label per node (nullptr: one segment)
     int nseg = 1;
+    std::vector<int64_t> row0, rows;   // per segment (rows = 0: no block of its own)
     std::vector<int32_t> kind;     // per segment: 1 = own eigenvectors, 0 = meta only
     std::vector<int32_t> cols;     // per segment: d columns of b in selection order
     bool affine = false;           // coordinates sel * (range / max|sel|) + meta
