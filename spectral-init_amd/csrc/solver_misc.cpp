@@ -110,9 +110,18 @@ int solve_eigenproblem_device(const DevLaplacian& L, int n_components, uint64_t 
         lap_time(3);
     }
     SolveStats st4;
-    rsvd_solve_device(L, n_components, seed, -1, 2, evals_host, evecs_dev, st4, s);
+    bool rsvd_ran = true;
+    try {
+        rsvd_solve_device(L, n_components, seed, -1, 2, evals_host, evecs_dev, st4, s);
+    } catch (const Error& e) {
+        // a sketch that does not fit in device memory (n / 10 columns of a very
+        // large graph) fails the level, as an allocation failure would
+        if (e.code != kErrUnsupported) throw;
+        rsvd_ran = false;
+        if (trace_on()) std::fprintf(stderr, "[si] level 4 skipped: %s\n", e.what());
+    }
     {
-        const double q = quality(evals_host);
+        const double q = rsvd_ran ? quality(evals_host) : INFINITY;
         st.spmm_calls += st4.spmm_calls;
         lap_time(4);
         if (q < 1e-2) return finish(4, q);
