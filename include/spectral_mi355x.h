@@ -104,13 +104,18 @@ const char* si_last_error(const si_context* ctx);
 /* compute_degrees + build_normalized_laplacian (src/laplacian.rs:29-88),
  * producing a device-resident operator.  input_on_device = 0: indptr /
  * indices / data are host arrays; 1: device arrays.  Graphs with n >= 4096
- * are stored in a locality (reverse Cuthill-McKee) node order; every entry
- * point below takes and returns vectors in the caller's node order. */
+ * are stored in a locality node order (see SI_BUILD_LOCALITY_ORDER); every
+ * entry point below takes and returns vectors in the caller's node order. */
 si_status si_laplacian_build(si_context* ctx, int64_t n, const int64_t* indptr,
                              const int32_t* indices, const float* data, int input_on_device,
                              int32_t compute_mode, si_laplacian** out);
-/* Same, with explicit build flags (SI_BUILD_LOCALITY_ORDER or 0). */
+/* Same, with explicit build flags: SI_BUILD_LOCALITY_ORDER stores the graph in
+ * a locality order (a connected symmetric graph of >= 65536 nodes: rows grouped
+ * by k-means clusters of a short random-walk embedding; otherwise reverse
+ * Cuthill-McKee), SI_BUILD_RCM_ORDER in reverse Cuthill-McKee order always,
+ * 0 in the input order. */
 #define SI_BUILD_LOCALITY_ORDER 1u
+#define SI_BUILD_RCM_ORDER 2u
 si_status si_laplacian_build_ex(si_context* ctx, int64_t n, const int64_t* indptr,
                                 const int32_t* indices, const float* data, int input_on_device,
                                 int32_t compute_mode, uint32_t flags, si_laplacian** out);

@@ -99,11 +99,12 @@ si_status si_test_rcm_order_device(si_context* ctx, int64_t n, const int64_t* in
                                    const int32_t* indices, int32_t* perm);
 
 /* The operator as stored on the device: the CSR of W in the storage node order
- * (rowptr[n+1], col[nnz], the FP32 sweep values a32[nnz] = +W_ij / sqrt(d_i d_j))
- * and perm[n] (perm[new] = old; the identity when no locality order was
- * applied).  Any output may be NULL. */
+ * (rowptr[n+1], col[nnz], the FP32 sweep values a32[nnz] = +W_ij / sqrt(d_i d_j)),
+ * perm[n] (perm[new] = old; the identity when no locality order was applied)
+ * and which order it is (0 input, 1 reverse Cuthill-McKee, 2 clustering).  Any
+ * output may be NULL. */
 si_status si_test_laplacian_storage(si_context* ctx, const si_laplacian* lap, int32_t* rowptr,
-                                    int32_t* col, float* a32, int32_t* perm);
+                                    int32_t* col, float* a32, int32_t* perm, int32_t* order);
 
 /* The arithmetic of the exact k-NN step (si_knn_exact, si_fuzzy_graph_build)
  * in later calls of this process: mode 0 = automatic (points with k <= 32

@@ -456,7 +456,8 @@ si_status si_laplacian_build_ex(si_context* ctx, int64_t n, const int64_t* indpt
     return guard(ctx, [&] {
         SI_CHECK(n > 0, si::kErrInvalidGraph, "invalid graph: n must be > 0");
         SI_CHECK(compute_mode == 0 || compute_mode == 1, si::kErrInvalidArgument, "unknown compute mode");
-        const bool reorder = (flags & SI_BUILD_LOCALITY_ORDER) != 0;
+        const bool reorder = (flags & (SI_BUILD_LOCALITY_ORDER | SI_BUILD_RCM_ORDER)) != 0;
+        const bool prefer_rcm = (flags & SI_BUILD_RCM_ORDER) != 0;
         hipStream_t s = ctx->c.stream;
         std::unique_ptr<si_laplacian> lap(new si_laplacian());
         lap->device = ctx->c.device;
@@ -464,13 +465,14 @@ si_status si_laplacian_build_ex(si_context* ctx, int64_t n, const int64_t* indpt
             int64_t nnz = 0;
             SI_HIP(hipMemcpyAsync(&nnz, indptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
             SI_HIP(hipStreamSynchronize(s));
-            si::laplacian_build_device(lap->L, n, indptr, indices, data, nnz, compute_mode, s, reorder);
+            si::laplacian_build_device(lap->L, n, indptr, indices, data, nnz, compute_mode, s, reorder,
+                                       nullptr, nullptr, prefer_rcm);
         } else {
             check_host_csr(n, indptr);
             DeviceGraph g;
             upload_graph(g, n, indptr, indices, data, s);
             si::laplacian_build_device(lap->L, n, g.ip, g.ix, g.w, indptr[n], compute_mode, s,
-                                       reorder, indptr, indices);
+                                       reorder, indptr, indices, prefer_rcm);
         }
         *out = lap.release();
     });

@@ -282,7 +282,7 @@ si_status si_test_rcm_order_device(si_context* ctx, int64_t n, const int64_t* in
 }
 
 si_status si_test_laplacian_storage(si_context* ctx, const si_laplacian* lap, int32_t* rowptr,
-                                    int32_t* col, float* a32, int32_t* perm) {
+                                    int32_t* col, float* a32, int32_t* perm, int32_t* order) {
     if (!ctx || !lap) return SI_ERR_INVALID_ARGUMENT;
     return tguard(ctx, [&] {
         hipStream_t s = ctx->c.stream;
@@ -294,6 +294,7 @@ si_status si_test_laplacian_storage(si_context* ctx, const si_laplacian* lap, in
             if (L.perm) si::to_host(perm, L.perm, L.n, s);
             else for (int64_t i = 0; i < L.n; ++i) perm[i] = (int32_t)i;
         }
+        if (order) *order = L.order;
     });
 }
 

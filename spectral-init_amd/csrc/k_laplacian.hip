@@ -19,7 +19,7 @@
 #include <utility>
 #include <vector>
 
-#include "si_device.hpp"
+#include "si_host.hpp"
 
 namespace si {
 
@@ -145,7 +145,7 @@ void laplacian_free(DevLaplacian& L) {
 void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
                             const int32_t* d_indices, const float* d_w, int64_t nnz, int mode,
                             hipStream_t s, bool reorder, const int64_t* h_indptr,
-                            const int32_t* h_indices) {
+                            const int32_t* h_indices, bool prefer_rcm) {
     SI_CHECK(n > 0, kErrInvalidGraph, "laplacian: n must be > 0");
     SI_CHECK(nnz < (int64_t)INT32_MAX && n < (int64_t)INT32_MAX, kErrUnsupported,
              "laplacian: more than 2^31-1 stored entries or nodes is not supported");
@@ -242,8 +242,29 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
         SI_HIP(hipMalloc(&L.iperm, sizeof(int32_t) * n));
         if (L.symmetric) {
             SI_HIP(hipMalloc(&L.comp, sizeof(int32_t) * n));
-            L.n_comp = rcm_order_device(n, d_indptr, d_indices, L.perm, L.iperm, s, L.comp);
-            phase("rcm order (device)");
+            DevBuf<int32_t> root(n);
+            L.n_comp = components_device(n, d_indptr, d_indices, L.comp, root.get(), s);
+            phase("components (device)");
+            if (L.n_comp == 1 && n >= kClusterOrderMin && !prefer_rcm) {
+                // the clustering order walks on the operator in the input order:
+                // its values first (rebuilt below in the new order)
+                hipLaunchKernelGGL(k_values, dim3(gn), dim3(T), 0, s, L.rowptr, L.col, d_w, L.isd,
+                                   n, L.l64, L.a32, L.long_rows, d_nlong);
+                SI_LAUNCH_CHECK();
+                int nl = 0;
+                SI_HIP(hipMemcpyAsync(&nl, d_nlong, sizeof(int), hipMemcpyDeviceToHost, s));
+                SI_HIP(hipStreamSynchronize(s));
+                SI_HIP(hipMemsetAsync(d_nlong, 0, sizeof(int), s));
+                L.n_long = nl;
+                cluster_order_device(L, L.perm, L.iperm, s);
+                L.order = 2;
+                phase("cluster order (device)");
+            } else {
+                rcm_order_labeled(n, d_indptr, d_indices, L.comp, root.get(), L.n_comp, L.perm,
+                                  L.iperm, s);
+                L.order = 1;
+                phase("rcm order (device)");
+            }
         } else {
             // an asymmetric pattern: the host order, whose sweeps follow the
             // stored (out-)edges; pattern downloaded if the caller has no copy
@@ -268,6 +289,7 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
             for (int64_t i = 0; i < n; ++i) iperm[perm[i]] = (int32_t)i;
             SI_HIP(hipMemcpyAsync(L.perm, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
             SI_HIP(hipMemcpyAsync(L.iperm, iperm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+            L.order = 1;
             phase("rcm order (host)");
         }
         // rows renumbered and sorted on the device, from the input CSR

@@ -314,9 +314,17 @@ int64_t components_device(int64_t n, const int64_t* rp, const int32_t* ix, int32
 int64_t rcm_order_device(int64_t n, const int64_t* rp, const int32_t* ix, int32_t* perm,
                          int32_t* iperm, hipStream_t s, int32_t* lab_out) {
     SI_CHECK(n >= 1 && n < (int64_t)INT32_MAX, kErrUnsupported, "rcm: n out of range");
-    DevBuf<int32_t> lab_own(lab_out ? 0 : n), start(n);
+    DevBuf<int32_t> lab_own(lab_out ? 0 : n), root(n);
     int32_t* lab = lab_out ? lab_out : lab_own.get();
-    const int64_t nc = components_device(n, rp, ix, lab, start.get(), s);
+    const int64_t nc = components_device(n, rp, ix, lab, root.get(), s);
+    rcm_order_labeled(n, rp, ix, lab, root.get(), nc, perm, iperm, s);
+    return nc;
+}
+
+void rcm_order_labeled(int64_t n, const int64_t* rp, const int32_t* ix, const int32_t* lab,
+                       int32_t* start, int64_t nc, int32_t* perm, int32_t* iperm, hipStream_t s) {
+    SI_CHECK(n >= 1 && n < (int64_t)INT32_MAX && nc >= 1 && nc <= n, kErrUnsupported,
+             "rcm: n out of range");
     DevBuf<int32_t> queue(n), mark(n), lvl(n), cnt(n + 1), off(n + 1), maxlev(nc);
     DevBuf<unsigned long long> key(n), skey(n), skey2(n), best(nc);
     DevBuf<int> big(1);
@@ -337,7 +345,7 @@ int64_t rcm_order_device(int64_t n, const int64_t* rp, const int32_t* ix, int32_
         const int32_t epoch = sweep + 1;
         const int sorted = sweep == 2 ? 1 : 0;
         hipLaunchKernelGGL(k_fill_i32, grid_of(nc), dim3(256), 0, s, maxlev.get(), nc, 0);
-        hipLaunchKernelGGL(k_rcm_start, grid_of(nc), dim3(256), 0, s, (int32_t)nc, start.get(),
+        hipLaunchKernelGGL(k_rcm_start, grid_of(nc), dim3(256), 0, s, (int32_t)nc, start,
                            queue.get(), mark.get(), lvl.get(), epoch);
         SI_LAUNCH_CHECK();
         int64_t ls = 0, le = nc;
@@ -386,7 +394,7 @@ int64_t rcm_order_device(int64_t n, const int64_t* rp, const int32_t* ix, int32_
             hipLaunchKernelGGL(k_rcm_last_level, grid_of(n), dim3(256), 0, s, n, rp, queue.get(),
                                lvl.get(), lab, maxlev.get(), best.get());
             hipLaunchKernelGGL(k_rcm_next_start, grid_of(nc), dim3(256), 0, s, (int32_t)nc,
-                               best.get(), queue.get(), start.get());
+                               best.get(), queue.get(), start);
             SI_LAUNCH_CHECK();
         }
     }
@@ -415,7 +423,6 @@ int64_t rcm_order_device(int64_t n, const int64_t* rp, const int32_t* ix, int32_
     hipLaunchKernelGGL(k_rcm_reverse, grid_of(n), dim3(256), 0, s, n, order, perm, iperm);
     SI_LAUNCH_CHECK();
     SI_HIP(hipStreamSynchronize(s));
-    return nc;
 }
 
 }  // namespace si

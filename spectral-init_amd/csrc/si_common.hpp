@@ -80,6 +80,7 @@ constexpr uint32_t kStreamNoise = 7u;
 constexpr uint32_t kStreamProbe = 9u;     // warm-start network: random-walk probes
 constexpr uint32_t kStreamNetInit = 11u;  // warm-start network: initial weights
 constexpr uint32_t kStreamDenseInvit = 13u;  // dense eigensolver: inverse-iteration starts
+constexpr uint32_t kStreamOrder = 15u;       // clustering locality order: walk start
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 

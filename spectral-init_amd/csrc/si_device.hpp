@@ -39,6 +39,7 @@ struct DevLaplacian {
     int degree_exact = 1;    // PythonCompat degrees reproduce the f32 column-sum order
     int32_t* perm = nullptr;   // perm[new] = old node id (nullptr: identity order)
     int32_t* iperm = nullptr;  // iperm[old] = new node id
+    int order = 0;             // storage order: 0 input, 1 reverse Cuthill-McKee, 2 clustering
     // connected components, when the device locality order computed them
     // (n_comp = 0: not computed): comp[old node id] = label, numbered by
     // smallest node
@@ -56,14 +57,17 @@ struct DevLaplacian {
 // Builds L on the device from a CSR graph already resident on the device
 // (int64 row pointers, int32 columns, f32 weights).  Throws si::Error on
 // malformed input.
-// reorder = true renumbers the nodes in reverse Cuthill-McKee order for gather
-// locality (degrees are computed in the input order first, so PythonCompat
-// stays bit-exact); the order is computed on the device for a symmetric
-// graph, on the host otherwise, where h_* optionally give the same pattern.
+// reorder = true renumbers the nodes for gather locality (degrees are computed
+// in the input order first, so PythonCompat stays bit-exact): a connected
+// symmetric graph of at least kClusterOrderMin nodes takes the clustering
+// order (unless prefer_rcm), other symmetric graphs reverse Cuthill-McKee on
+// the device, an asymmetric one reverse Cuthill-McKee on the host, where h_*
+// optionally give the same pattern.
 void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
                             const int32_t* d_indices, const float* d_w, int64_t nnz, int mode,
                             hipStream_t s, bool reorder = false,
-                            const int64_t* h_indptr = nullptr, const int32_t* h_indices = nullptr);
+                            const int64_t* h_indptr = nullptr, const int32_t* h_indices = nullptr,
+                            bool prefer_rcm = false);
 // Device Laplacian from host FP64 off-diagonal values (CSR without the
 // diagonal; unit diagonal implied) and FP64 degrees; input order is kept.
 void laplacian_from_values_device(DevLaplacian& L, int64_t n, const int64_t* h_ip,
@@ -92,6 +96,16 @@ int64_t components_device(int64_t n, const int64_t* rp, const int32_t* ix, int32
 // the number of components.
 int64_t rcm_order_device(int64_t n, const int64_t* rp, const int32_t* ix, int32_t* perm,
                          int32_t* iperm, hipStream_t s, int32_t* lab = nullptr);
+// The same order from known components: lab (n) and the nc smallest nodes in
+// start (device, overwritten).
+void rcm_order_labeled(int64_t n, const int64_t* rp, const int32_t* ix, const int32_t* lab,
+                       int32_t* start, int64_t nc, int32_t* perm, int32_t* iperm, hipStream_t s);
+// The clustering order of a connected graph (rows grouped by k-means clusters
+// of a 16-column random-walk embedding, node order inside each cluster); L is
+// the operator in the input node order with its values built; n >= 256.
+void cluster_order_device(const DevLaplacian& L, int32_t* perm, int32_t* iperm, hipStream_t s);
+// Connected graphs of at least this many nodes take the clustering order.
+constexpr int64_t kClusterOrderMin = 65536;
 // dst[i, :k] = src[idx[i], :k] (row gather); idx == nullptr copies.
 void gather_rows_f64(const double* src, int lds, double* dst, int ldd, int64_t n, int k,
                      const int32_t* idx, hipStream_t s);

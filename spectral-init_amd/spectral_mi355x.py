@@ -29,6 +29,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libspectral_mi355x.so")
 SI_OK, SI_ERR_CONVERGENCE, SI_ERR_INVALID_GRAPH, SI_ERR_TOO_FEW_NODES = 0, 1, 2, 3
 SI_ERR_HIP, SI_ERR_INTERNAL, SI_ERR_INVALID_ARGUMENT, SI_ERR_UNSUPPORTED = 4, 5, 6, 7
 SI_F32, SI_F64 = 0, 1
+SI_BUILD_LOCALITY_ORDER, SI_BUILD_RCM_ORDER = 1, 2
 
 # every symbol the public and testing headers declare
 EXPORTED = [
@@ -214,7 +215,7 @@ def lib():
                                          C.POINTER(_Report)]
         L.si_test_band_solve.argtypes = [vp, vp, dbl, vp, vp, i32, C.POINTER(i32)]
         L.si_test_rcm_order.argtypes = [i64, vp, vp, i32, vp]
-        L.si_test_laplacian_storage.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.si_test_laplacian_storage.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.si_test_components_device.argtypes = [vp, i64, vp, vp, vp, vp]
         L.si_test_rcm_order_device.argtypes = [vp, i64, vp, vp, vp]
         L.si_test_output_stage.argtypes = [i32]
@@ -296,8 +297,11 @@ class Laplacian:
 
     def __init__(self, ctx: Context, indptr, indices, data, mode: int = 0, on_device=False,
                  reorder: Optional[bool] = None):
-        """reorder: store in locality (reverse Cuthill-McKee) order; None = the
-        library default (on for n >= 4096).  Vectors always use the caller's order."""
+        """reorder: True stores the operator in a locality order (the clustering
+        order for a connected graph of >= 65536 nodes, reverse Cuthill-McKee
+        otherwise), "rcm" in reverse Cuthill-McKee order, False in the input
+        order; None = the library default (a locality order for n >= 4096).
+        Vectors always use the caller's order."""
         self.ctx = ctx
         self._h = C.c_void_p()
         if on_device:
@@ -314,9 +318,10 @@ class Laplacian:
                                                 _ptr(data), int(on_device), int(mode),
                                                 C.byref(self._h)))
         else:
+            flags = SI_BUILD_RCM_ORDER if reorder == "rcm" else (SI_BUILD_LOCALITY_ORDER if reorder else 0)
             ctx._check(lib().si_laplacian_build_ex(ctx.handle, n, _ptr(indptr), _ptr(indices),
                                                    _ptr(data), int(on_device), int(mode),
-                                                   1 if reorder else 0, C.byref(self._h)))
+                                                   flags, C.byref(self._h)))
         n_, nnz, nl, ex = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
         lib().si_laplacian_info(self._h, C.byref(n_), C.byref(nnz), C.byref(nl), C.byref(ex))
         self.n, self.nnz, self.n_long, self.degree_exact = n_.value, nnz.value, nl.value, bool(ex.value)
@@ -795,18 +800,23 @@ def fuzzy_graph(ctx: Context, x, k: int = 15, n_epochs: int = -1) -> Graph:
     return Graph(ctx, h)
 
 
+ORDER_INPUT, ORDER_RCM, ORDER_CLUSTER = 0, 1, 2
+
+
 def laplacian_storage(lap: "Laplacian"):
     """Testing seam: the operator as stored on the device -- (rowptr, col, a32,
-    perm) in the storage node order, perm[new] = old."""
+    perm, order) in the storage node order, perm[new] = old, order one of
+    ORDER_INPUT, ORDER_RCM, ORDER_CLUSTER."""
     n, nnz = lap.n, lap.nnz
     rp = np.empty(n + 1, np.int32)
     col = np.empty(nnz, np.int32)
     a32 = np.empty(nnz, np.float32)
     perm = np.empty(n, np.int32)
+    order = C.c_int32()
     lap.ctx._check(lib().si_test_laplacian_storage(lap.ctx.handle, lap._h, rp.ctypes.data,
                                                    col.ctypes.data, a32.ctypes.data,
-                                                   perm.ctypes.data))
-    return rp, col, a32, perm
+                                                   perm.ctypes.data, C.byref(order)))
+    return rp, col, a32, perm, order.value
 
 
 def set_output_stage(device: bool) -> None:

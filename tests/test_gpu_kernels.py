@@ -300,10 +300,12 @@ def test_reordered_storage_is_the_permuted_csr(ctx, name):
         ip, ix, w = z["w_indptr"], z["w_indices"], z["w_data"]
     n = ip.size - 1
     lap = si.Laplacian(ctx, ip, ix, w, reorder=True)
-    rp, col, a32, perm = si.laplacian_storage(lap)
+    rp, col, a32, perm, order = si.laplacian_storage(lap)
+    assert order == si.ORDER_RCM  # below the clustering order's size
     np.testing.assert_array_equal(perm, si.rcm_order(ip, ix))
     plain = si.Laplacian(ctx, ip, ix, w, reorder=False)
-    prp, pcol, pa32, pperm = si.laplacian_storage(plain)
+    prp, pcol, pa32, pperm, porder = si.laplacian_storage(plain)
+    assert porder == si.ORDER_INPUT
     np.testing.assert_array_equal(pperm, np.arange(n))
     np.testing.assert_array_equal(prp, ip)
     np.testing.assert_array_equal(pcol, ix)

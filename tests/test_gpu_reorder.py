@@ -9,6 +9,12 @@ unsorted rows and repeated columns, several components with isolated nodes
 and self loops, hubs whose new-node groups exceed one wavefront (the
 segmented sort of the last sweep), a star, a complete graph, a ring, many
 small components with shuffled ids, and the fixture graphs.
+
+Connected graphs of at least 65536 nodes are stored in the clustering order
+(k-means clusters of a random-walk embedding): a permutation, the same on
+every build, undone by every entry point -- the exported Laplacian, the SpMM
+and the eigenpairs are those of the input order -- while disconnected graphs
+and the explicit RCM flag keep reverse Cuthill-McKee.
 """
 import numpy as np
 import pytest
@@ -144,3 +150,71 @@ def test_device_rcm_repeated_calls_agree(ctx):
     p1 = si.rcm_order_device(ctx, ip, ix)
     p2 = si.rcm_order_device(ctx, ip, ix)
     np.testing.assert_array_equal(p1, p2)
+
+
+def _storage_matches(ctx, lap, ip, ix, w):
+    """The stored CSR is P W P^T of the input and its FP32 values are the input
+    order's, carried along bit for bit."""
+    n = ip.size - 1
+    rp, col, a32, perm, order = si.laplacian_storage(lap)
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    plain = si.Laplacian(ctx, ip, ix, w, reorder=False)
+    _, _, pa32, _, _ = si.laplacian_storage(plain)
+    a = sp.csr_matrix((pa32, ix, ip), shape=(n, n))[perm][:, perm].tocsr()
+    a.sort_indices()
+    np.testing.assert_array_equal(rp, a.indptr)
+    np.testing.assert_array_equal(col, a.indices)
+    np.testing.assert_array_equal(a32, a.data)
+    return perm, order
+
+
+def test_cluster_order_for_large_connected_graphs(ctx):
+    print("case cluster order random_large", flush=True)
+    ip, ix = random_graph(120_000, 4, 11, False)
+    w = np.random.default_rng(3).uniform(0.1, 1.0, ix.size).astype(np.float32)
+    a = sp.csr_matrix((w, ix, ip))
+    a = ((a + a.T) * 0.5).tocsr()
+    a.sort_indices()
+    ip, ix, w = a.indptr.astype(np.int64), a.indices.astype(np.int32), a.data.astype(np.float32)
+    lap = si.Laplacian(ctx, ip, ix, w)
+    perm, order = _storage_matches(ctx, lap, ip, ix, w)
+    assert order == si.ORDER_CLUSTER
+    assert not np.array_equal(perm, si.rcm_order(ip, ix))
+    # deterministic: a second build stores the same order
+    perm2, _ = _storage_matches(ctx, si.Laplacian(ctx, ip, ix, w), ip, ix, w)
+    np.testing.assert_array_equal(perm, perm2)
+    # the explicit RCM flag keeps reverse Cuthill-McKee
+    lap_r = si.Laplacian(ctx, ip, ix, w, reorder="rcm")
+    perm_r, order_r = _storage_matches(ctx, lap_r, ip, ix, w)
+    assert order_r == si.ORDER_RCM
+    np.testing.assert_array_equal(perm_r, si.rcm_order(ip, ix))
+    # the order is invisible: the exported L and the SpMM are the input order's
+    deg, sq, (lip, lix, ld) = lap.export()
+    (oip, oix, od), odeg, _ = O.laplacian_from_graph(ip, ix, w)
+    np.testing.assert_array_equal(lip, oip)
+    np.testing.assert_array_equal(lix, oix)
+    np.testing.assert_array_equal(ld, od)
+    x = np.random.default_rng(5).standard_normal((ip.size - 1, 3))
+    xd = torch.from_numpy(x).cuda()
+    y1, y2 = torch.empty_like(xd), torch.empty_like(xd)
+    lap.spmm(xd, y1, si.SI_F64)
+    lap_r.spmm(xd, y2, si.SI_F64)
+    np.testing.assert_allclose(y1.cpu().numpy(), y2.cpu().numpy(), rtol=1e-13, atol=1e-13)
+    # and the eigenpairs agree with the RCM-stored operator's
+    ev1, v1, r1 = si.solve_eigenproblem(lap, 2, 42, si.SpectralInitConfig(tol=1e-8))
+    ev2, v2, r2 = si.solve_eigenproblem(lap_r, 2, 42, si.SpectralInitConfig(tol=1e-8))
+    assert r1.max_residual <= 1e-8 and r2.max_residual <= 1e-8
+    np.testing.assert_allclose(ev1, ev2, rtol=1e-9, atol=1e-12)
+
+
+def test_disconnected_large_graphs_keep_rcm(ctx):
+    print("case cluster order not for several components", flush=True)
+    ip, ix = random_graph(70_000, 4, 12, False)
+    ip2, ix2 = random_graph(5000, 3, 13, False)
+    ip = np.concatenate([ip, ip[-1] + ip2[1:]])
+    ix = np.concatenate([ix, ix2 + 70_000]).astype(np.int32)
+    w = np.ones(ix.size, np.float32)
+    lap = si.Laplacian(ctx, ip, ix, w)
+    perm, order = _storage_matches(ctx, lap, ip, ix, w)
+    assert order == si.ORDER_RCM
+    np.testing.assert_array_equal(perm, si.rcm_order(ip, ix))
