@@ -265,10 +265,11 @@ def bench_components(args, torch, graphs, si):
     print(json.dumps(out), flush=True)
 
 
-def read_traffic(path, workload, block):
+def read_traffic(path, workload, block, order):
     """HBM bytes per launch of the dominant SpMM, from a committed rocprofv3
-    --pmc summary (lines "workload: ...", "block: ...", "hbm_bytes_per_launch: ...").
-    None when the summary is absent or measured another workload or width."""
+    --pmc summary (lines "workload: ...", "block: ...", "order: ...",
+    "hbm_bytes_per_launch: ...").  None when the summary is absent or measured
+    another workload, width or node order."""
     if not path or not os.path.exists(path):
         return None
     kv = {}
@@ -277,7 +278,7 @@ def read_traffic(path, workload, block):
             if ":" in line and not line.startswith(("#", " ", "|")):
                 key, _, val = line.partition(":")
                 kv[key.strip()] = val.strip()
-    if kv.get("workload") != workload or kv.get("block") != str(block):
+    if kv.get("workload") != workload or kv.get("block") != str(block) or kv.get("order") != order:
         return None
     try:
         return int(float(kv["hbm_bytes_per_launch"]))
@@ -309,6 +310,8 @@ def main():
                     help="time the SpMM alone for widths 1..64 and exit")
     ap.add_argument("--spmm-sweep", action="store_true",
                     help="also time the SpMM alone for widths 1..64 (FP32 and FP64)")
+    ap.add_argument("--rcm", action="store_true",
+                    help="store the operator in reverse Cuthill-McKee order (not the clustering order)")
     ap.add_argument("--no-reorder", action="store_true",
                     help="keep the input node order (no locality reordering)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "spmm_wide_f32_gmm1m_traffic.md"),
@@ -373,7 +376,7 @@ def main():
 
     ctx = si.Context(0)
     t0 = time.perf_counter()
-    lap = si.Laplacian(ctx, ip, ix, wt, reorder=not args.no_reorder)
+    lap = si.Laplacian(ctx, ip, ix, wt, reorder="rcm" if args.rcm else not args.no_reorder)
     torch.cuda.synchronize()
     lap_ms = 1e3 * (time.perf_counter() - t0)
     log(f"laplacian on device: nnz={lap.nnz} hub rows={lap.n_long} ({lap_ms:.1f} ms incl. upload)")
@@ -553,7 +556,8 @@ def main():
                                threads=ncores)
         log(f"CPU oracle on {ncores} cores: {cpu_all['value']} {cpu_all['unit']}")
 
-    traffic = read_traffic(args.traffic, args.workload, last.block_size)
+    order = {0: "input", 1: "rcm", 2: "cluster"}[si.laplacian_order(lap)]
+    traffic = read_traffic(args.traffic, args.workload, last.block_size, order)
     # the unit of the roofline: one application of L to an n x b block, FP32
     # value + int32 column per stored entry, int32 row pointers, the block read
     # once and written once (diagonal folded in the epilogue: stored = nnz(W));
@@ -589,7 +593,7 @@ def main():
         "data": "synthetic (seeded Gaussian-mixture exact 15-NN fuzzy graph)",
         "config": {"workload": args.workload, "n": n, "nnz": nnz, "n_components": args.n_components,
                    "block": last.block_size, "cheb_degree": args.degree or "auto", "tol": args.tol,
-                   "components_generated": ncomp},
+                   "components_generated": ncomp, "node_order": order},
         "plain_gpu_lobpcg": {"ms": plain_ms, "runs": plain},
         "rsvd_oversampling10_4iter": rsvd or None,
         "dense_evd": dense,

@@ -853,6 +853,15 @@ def rcm_order_device(ctx: Context, indptr, indices) -> np.ndarray:
     return perm
 
 
+def laplacian_order(lap: "Laplacian") -> int:
+    """Testing seam: which node order the operator is stored in (ORDER_INPUT,
+    ORDER_RCM or ORDER_CLUSTER)."""
+    order = C.c_int32()
+    lap.ctx._check(lib().si_test_laplacian_storage(lap.ctx.handle, lap._h, None, None, None, None,
+                                                   C.byref(order)))
+    return order.value
+
+
 def rcm_order(indptr, indices, threads: int = 0) -> np.ndarray:
     """Testing seam (host only): the locality order of si_laplacian_build,
     perm[new] = old."""
