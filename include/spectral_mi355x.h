@@ -110,10 +110,10 @@ si_status si_laplacian_build(si_context* ctx, int64_t n, const int64_t* indptr,
                              const int32_t* indices, const float* data, int input_on_device,
                              int32_t compute_mode, si_laplacian** out);
 /* Same, with explicit build flags: SI_BUILD_LOCALITY_ORDER stores the graph in
- * a locality order (a connected symmetric graph of >= 65536 nodes: rows grouped
- * by k-means clusters of a short random-walk embedding; otherwise reverse
- * Cuthill-McKee), SI_BUILD_RCM_ORDER in reverse Cuthill-McKee order always,
- * 0 in the input order. */
+ * a locality order (a connected symmetric graph of >= 65536 nodes with >= 8
+ * stored entries per node: rows grouped by k-means clusters of a short
+ * random-walk embedding; otherwise reverse Cuthill-McKee), SI_BUILD_RCM_ORDER
+ * in reverse Cuthill-McKee order always, 0 in the input order. */
 #define SI_BUILD_LOCALITY_ORDER 1u
 #define SI_BUILD_RCM_ORDER 2u
 si_status si_laplacian_build_ex(si_context* ctx, int64_t n, const int64_t* indptr,

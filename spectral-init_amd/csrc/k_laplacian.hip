@@ -245,7 +245,11 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
             DevBuf<int32_t> root(n);
             L.n_comp = components_device(n, d_indptr, d_indices, L.comp, root.get(), s);
             phase("components (device)");
-            if (L.n_comp == 1 && n >= kClusterOrderMin && !prefer_rcm) {
+            // k-NN-like graphs only: a mesh-like graph (average degree below
+            // kClusterOrderMinDegree) gets a narrow band from RCM, which both
+            // the SpMM and the banded Cholesky of level 2 use
+            if (L.n_comp == 1 && n >= kClusterOrderMin && nnz >= kClusterOrderMinDegree * n &&
+                !prefer_rcm) {
                 // the clustering order walks on the operator in the input order:
                 // its values first (rebuilt below in the new order)
                 hipLaunchKernelGGL(k_values, dim3(gn), dim3(T), 0, s, L.rowptr, L.col, d_w, L.isd,

@@ -59,8 +59,9 @@ struct DevLaplacian {
 // malformed input.
 // reorder = true renumbers the nodes for gather locality (degrees are computed
 // in the input order first, so PythonCompat stays bit-exact): a connected
-// symmetric graph of at least kClusterOrderMin nodes takes the clustering
-// order (unless prefer_rcm), other symmetric graphs reverse Cuthill-McKee on
+// symmetric graph of at least kClusterOrderMin nodes and average degree
+// kClusterOrderMinDegree takes the clustering order (unless prefer_rcm), other
+// symmetric graphs reverse Cuthill-McKee on
 // the device, an asymmetric one reverse Cuthill-McKee on the host, where h_*
 // optionally give the same pattern.
 void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
@@ -104,8 +105,10 @@ void rcm_order_labeled(int64_t n, const int64_t* rp, const int32_t* ix, const in
 // of a 16-column random-walk embedding, node order inside each cluster); L is
 // the operator in the input node order with its values built; n >= 256.
 void cluster_order_device(const DevLaplacian& L, int32_t* perm, int32_t* iperm, hipStream_t s);
-// Connected graphs of at least this many nodes take the clustering order.
+// Connected graphs of at least this many nodes and this average degree (stored
+// entries per node) take the clustering order.
 constexpr int64_t kClusterOrderMin = 65536;
+constexpr int64_t kClusterOrderMinDegree = 8;
 // dst[i, :k] = src[idx[i], :k] (row gather); idx == nullptr copies.
 void gather_rows_f64(const double* src, int lds, double* dst, int ldd, int64_t n, int k,
                      const int32_t* idx, hipStream_t s);

@@ -10,11 +10,12 @@ and self loops, hubs whose new-node groups exceed one wavefront (the
 segmented sort of the last sweep), a star, a complete graph, a ring, many
 small components with shuffled ids, and the fixture graphs.
 
-Connected graphs of at least 65536 nodes are stored in the clustering order
+Connected graphs of at least 65536 nodes and 8 entries per node are stored in
+the clustering order
 (k-means clusters of a random-walk embedding): a permutation, the same on
 every build, undone by every entry point -- the exported Laplacian, the SpMM
-and the eigenpairs are those of the input order -- while disconnected graphs
-and the explicit RCM flag keep reverse Cuthill-McKee.
+and the eigenpairs are those of the input order -- while disconnected graphs,
+mesh-like graphs and the explicit RCM flag keep reverse Cuthill-McKee.
 """
 import numpy as np
 import pytest
@@ -170,7 +171,7 @@ def _storage_matches(ctx, lap, ip, ix, w):
 
 def test_cluster_order_for_large_connected_graphs(ctx):
     print("case cluster order random_large", flush=True)
-    ip, ix = random_graph(120_000, 4, 11, False)
+    ip, ix = random_graph(120_000, 6, 11, False)
     w = np.random.default_rng(3).uniform(0.1, 1.0, ix.size).astype(np.float32)
     a = sp.csr_matrix((w, ix, ip))
     a = ((a + a.T) * 0.5).tocsr()
@@ -218,3 +219,12 @@ def test_disconnected_large_graphs_keep_rcm(ctx):
     perm, order = _storage_matches(ctx, lap, ip, ix, w)
     assert order == si.ORDER_RCM
     np.testing.assert_array_equal(perm, si.rcm_order(ip, ix))
+
+
+def test_mesh_like_large_graphs_keep_rcm(ctx):
+    # average degree 2: RCM's band (2 for a ring) serves the SpMM and level 2
+    print("case cluster order not for a ring", flush=True)
+    ip, ix = ring(70_000)
+    w = np.ones(ix.size, np.float32)
+    lap = si.Laplacian(ctx, ip, ix, w)
+    assert si.laplacian_order(lap) == si.ORDER_RCM
