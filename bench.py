@@ -289,8 +289,8 @@ def read_traffic(path, workload, block, order):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="gmm1m",
                     choices=["gmm1m", "gmm100k", "blobs2k", "band1m", "comp64"])
     ap.add_argument("--n", type=int, default=None, help="override the node count")
@@ -574,6 +574,22 @@ def main():
                  for i, m in enumerate(("column_stream_us", "random_rows_us", "random_rows_2MiB_us"))}
         log(f"gather probe: {probe}")
 
+    # the measured device-copy bandwidth beside the 8 TB/s peak: a 1 GiB
+    # device-to-device copy, read + write bytes over the kernel time
+    a = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    bcp = torch.empty_like(a)
+    bcp.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        bcp.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 5 * 2 * a.numel() * 4 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, bcp
+    log(f"device copy: {copy_gbs:.0f} GB/s (1 GiB, read + write)")
+
     gbuild = None
     if not args.no_graph_builder and args.workload in ("gmm1m", "gmm100k", "blobs2k"):
         gbuild = bench_graph_builder(args, torch, graphs, si, ctx)
@@ -616,7 +632,8 @@ def main():
                          "bytes_per_launch": round(bytes_per_launch),
                          "achieved": round(spmm_gbs, 1) if spmm_gbs else None,
                          "frac": round(spmm_gbs / HBM_PEAK_GBS, 4) if spmm_gbs else None},
-                     "gather_probe": probe},
+                     "gather_probe": probe,
+                     "measured_device_copy_gbs": round(copy_gbs, 1)},
         "warm_start": warm,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all if not args.no_cpu else None,
