@@ -131,10 +131,22 @@ void upload_graph(DeviceGraph& g, int64_t n, const int64_t* ip, const int32_t* i
     SI_HIP(hipStreamSynchronize(s));
 }
 
-// SpMM shape autotuning for graphs large enough to repay it (~10 launches)
-void tune_once(si::DevLaplacian& L, hipStream_t s, int widths) {
+// SpMM shape autotuning for graphs large enough to repay it (~10 launches per
+// width), once per graph shape and context
+void tune_once(si::Context& c, si::DevLaplacian& L, int widths) {
     widths &= ~L.tuned;
-    if (widths && L.nnz >= (int64_t)1 << 20) si::spmm_autotune(L, s, widths);
+    if (!widths || L.nnz < (int64_t)1 << 20) return;
+    const si::TuneKey key{L.n, L.nnz, L.n_long, L.max_row_len, L.order};
+    si::TunedShapes& t = c.tune_cache[key];
+    if (t.tuned & widths & si::kTuned16) L.shape16 = t.shape16;
+    if (t.tuned & widths & si::kTuned32) L.shape32 = t.shape32;
+    L.tuned |= t.tuned & widths;
+    widths &= ~t.tuned;
+    if (!widths) return;
+    si::spmm_autotune(L, c.stream, widths);
+    if (widths & si::kTuned16) t.shape16 = L.shape16;
+    if (widths & si::kTuned32) t.shape32 = L.shape32;
+    t.tuned |= widths;
 }
 
 void check_host_csr(int64_t n, const int64_t* ip) {
@@ -158,7 +170,7 @@ void build_operator(si::Context& c, si::DevLaplacian& L, int64_t n, const int64_
                      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
     si::EventTimer t(s);
     si::laplacian_build_device(L, n, g.ip, g.ix, g.w, ip[n], mode, s, reorder, ip, ix);
-    tune_once(L, s, tune_widths);
+    tune_once(c, L, tune_widths);
     const double ms = t.stop_ms();
     if (lap_ms) *lap_ms += ms;
 }
@@ -763,7 +775,7 @@ si_status si_solve_eigenproblem(si_context* ctx, const si_laplacian* lap, int32_
         // the FP32 sweeps pick their SpMM shape for this graph once (a cache
         // held by the Laplacian)
         if (o.solver == SI_SOLVER_FAST)
-            tune_once(const_cast<si::DevLaplacian&>(lap->L), s,
+            tune_once(ctx->c, const_cast<si::DevLaplacian&>(lap->L),
                       si::fast_tune_widths(lap->L.n, n_components + 1, o.fast.block_size));
         const si::DevLaplacian& L = lap->L;
         double* vi = si::ws_get<double>(si::kWsChainVec, (size_t)L.n * k);
@@ -882,7 +894,7 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
                                /*tune_widths=*/0, &lap_ms);
             if (fast) {
                 si::EventTimer t(ctx->c.stream);
-                tune_once(whole.L, ctx->c.stream, si::fast_tune_widths(n, k, opt.fast.block_size));
+                tune_once(ctx->c, whole.L, si::fast_tune_widths(n, k, opt.fast.block_size));
                 lap_ms += t.stop_ms();
             }
             si::DevBuf<double> blk;
@@ -1050,7 +1062,7 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
                 {
                     // the batched solver's block is 16 columns wide
                     si::EventTimer t(ctx->c.stream);
-                    tune_once(whole.L, ctx->c.stream, si::kTuned16);
+                    tune_once(ctx->c, whole.L, si::kTuned16);
                     lap_ms += t.stop_ms();
                 }
                 phase("spmm autotune");

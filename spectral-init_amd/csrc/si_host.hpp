@@ -5,6 +5,7 @@
 
 #include <cstring>
 #include <functional>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -63,12 +64,31 @@ inline T* ws_get(int slot, size_t count) {
     return static_cast<T*>(workspace_get(slot, sizeof(T) * (count ? count : 1)));
 }
 
+// Tuned FP32 SpMM shapes remembered per graph shape (n, stored entries, hub
+// rows, widest row, node order): operators of the same shape reuse them.  The
+// shapes are performance choices only; every shape computes the same product.
+struct TuneKey {
+    int64_t n, nnz, n_long, max_row_len;
+    int order;
+    bool operator<(const TuneKey& o) const {
+        if (n != o.n) return n < o.n;
+        if (nnz != o.nnz) return nnz < o.nnz;
+        if (n_long != o.n_long) return n_long < o.n_long;
+        if (max_row_len != o.max_row_len) return max_row_len < o.max_row_len;
+        return order < o.order;
+    }
+};
+struct TunedShapes {
+    int shape16 = 0, shape32 = 0, tuned = 0;
+};
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string last_error;
     int num_cus = 0;
     SpmmProfile profile;
+    std::map<TuneKey, TunedShapes> tune_cache;
 };
 
 // Stream-ordered timer (hipEvents on the context's stream).
