@@ -1,12 +1,14 @@
 // api.cpp -- the C ABI (include/spectral_mi355x.h) over the device solvers,
 // plus the spectral_init pipeline around them (src/lib.rs:198-258).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstddef>
 #include <memory>
 #include <numeric>
+#include <thread>
 
 #include "si_abi.hpp"
 
@@ -921,7 +923,12 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
         } else {
             // embed_disconnected (src/multi_component.rs:16-74)
             std::vector<std::vector<int64_t>> members(nc);
-            for (int64_t i = 0; i < n; ++i) members[labels[i]].push_back(i);
+            {
+                std::vector<int64_t> cnt(nc, 0);
+                for (int64_t i = 0; i < n; ++i) ++cnt[labels[i]];
+                for (int64_t c = 0; c < nc; ++c) members[c].reserve(cnt[c]);
+                for (int64_t i = 0; i < n; ++i) members[labels[i]].push_back(i);
+            }
             std::vector<double> meta((size_t)nc * d, 0.0);
             if (nc > 2 * d) {
                 if (!raw_data)
@@ -929,14 +936,26 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
                                     "invalid graph: data is required for spectral meta-embedding when "
                                     "n_conn_components > 2 * n_embedding_dims");
                 // spectral_meta_embedding (multi_component.rs:143-220)
+                // each component's sums in member order (components on host threads)
                 std::vector<double> cen((size_t)nc * n_features, 0.0);
-                for (int64_t c = 0; c < nc; ++c) {
+                auto centroid = [&](int64_t c) {
+                    double* cc = cen.data() + c * n_features;
                     for (int64_t node : members[c])
                         for (int64_t f = 0; f < n_features; ++f)
-                            cen[c * n_features + f] += (double)raw_data[node * n_features + f];
+                            cc[f] += (double)raw_data[node * n_features + f];
                     const double inv = 1.0 / (double)members[c].size();
-                    for (int64_t f = 0; f < n_features; ++f) cen[c * n_features + f] *= inv;
-                }
+                    for (int64_t f = 0; f < n_features; ++f) cc[f] *= inv;
+                };
+                const int nth = (int)std::min<int64_t>(
+                    nc, std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency())));
+                std::atomic<int64_t> next{0};
+                auto worker = [&] {
+                    for (int64_t c = next++; c < nc; c = next++) centroid(c);
+                };
+                std::vector<std::thread> pool;
+                for (int t = 1; t < nth; ++t) pool.emplace_back(worker);
+                worker();
+                for (auto& th : pool) th.join();
                 // Gaussian affinity exp(-||c_i - c_j||^2) in FP64
                 std::vector<double> aff((size_t)nc * nc);
                 for (int64_t i = 0; i < nc; ++i)
@@ -1018,6 +1037,7 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
                 }
                 range[c] = std::isfinite(md) ? md / 2.0 : 1.0;
             }
+            phase("meta-embedding, ranges");
             // Eigenpairs of every component that gets its own embedding (size >= 2d):
             // all at once on the device (batched), or one solve per component.
             std::vector<std::vector<double>> cev(nc), cvec(nc);
