@@ -1,0 +1,193 @@
+// k_batch.hip -- the round planner of the batched component solver, on the
+// device.
+//
+// The batched solver (solver_batch.cpp) runs rounds of "Chebyshev filter of
+// per-segment degree, then segmented Rayleigh-Ritz" over every connected
+// component of a block-diagonal Laplacian.  The decisions between rounds --
+// which segments have converged, stalled or failed, each segment's damped
+// interval [a_c, 2] and filter degree, the per-step recurrence scalars of every
+// segment -- are made here, by one workgroup that reads the round's Ritz
+// values, residual norms and Cholesky status where the Rayleigh-Ritz kernels
+// left them.  It writes:
+//   * the coefficient table of the next round's filter steps,
+//   * the gate ctl[0]: the number of filter steps the next round runs (0 = no
+//     round: every segment is done or the round limit is reached).
+// Every launch of a round reads the gate when it starts: filter step j runs
+// only while j <= gate, the Rayleigh-Ritz launches only while gate > 0.  So a
+// captured graph of several rounds replays without the host in between, and
+// rounds after the last useful one cost only their empty launches.
+//
+// The filter steps take their buffers from a three-buffer ring by the gate
+// (spmm_seg_ring_*), so the last step that runs always writes the buffer the
+// captured Rayleigh-Ritz reads.
+#include "si_device.hpp"
+
+namespace si {
+
+namespace {
+
+constexpr int kPlanThreads = 256;
+
+// Largest degree d <= cap with T_d(c/e) <= e^9.2 on [lo, b] (the
+// amplification cap of the single-graph solver).
+__device__ int plan_degree(double lo, double b, int cap) {
+    const double c = (lo + b) / 2.0, e = (b - lo) / 2.0;
+    const double x0 = c / e;
+    const int dmax = (int)floor(9.2 / acosh(x0 > 1.0 + 1e-12 ? x0 : 1.0 + 1e-12));
+    const int d = cap < dmax ? cap : dmax;
+    return d > 2 ? d : 2;
+}
+
+// max_j ||L v_j - theta_j v_j|| / max(||v_j||, eps) of segment c from the
+// squared norms of the Rayleigh-Ritz update; a NaN anywhere gives +inf.
+__device__ double plan_worst(const double* __restrict__ norms2, int c, int k) {
+    double wv = 0.0;
+    for (int j = 0; j < k; ++j) {
+        const double r2 = norms2[(int64_t)c * 2 * k + j];
+        const double x2 = norms2[(int64_t)c * 2 * k + k + j];
+        if (r2 != r2 || x2 != x2) return INFINITY;
+        double xn = sqrt(x2 > 0.0 ? x2 : 0.0);
+        if (xn < 2.220446049250313e-16) xn = 2.220446049250313e-16;
+        const double r = sqrt(r2 > 0.0 ? r2 : 0.0) / xn;
+        wv = r > wv ? r : wv;
+    }
+    return wv;
+}
+
+// mode 0: FP32 rounds (a segment stops at tol, at a 1e-4 stall floor, or on a
+//         failed Cholesky; a_c from its latest Ritz values);
+// mode 1: FP64 rounds (a segment stops at tol or on a failed Cholesky; its
+//         kept Ritz values thk and a_c follow every round it is re-projected).
+template <typename T>
+__global__ __launch_bounds__(kPlanThreads) void k_batch_plan(BatchPlan p, int mode, int update,
+                                                             T* __restrict__ coef) {
+    __shared__ int s_dmax, s_active;
+    const int t = threadIdx.x;
+    const int prev_gate = p.ctl[0];  // read by every thread before thread 0 rewrites it
+    if (t == 0) { s_dmax = 0; s_active = 0; }
+    __syncthreads();
+    // update 1: the round just planned by the gate ran if the gate was open;
+    // update 2: results of a round run without a gate (always read)
+    const bool ran = update == 2 || (update == 1 && prev_gate > 0);
+    const int k = p.k, m = p.m;
+    for (int c = t; c < p.nseg; c += kPlanThreads) {
+        int done = p.done[c];
+        if (ran) {
+            if (mode == 0) {
+                if (done == 0) {
+                    if (p.status[c] != 0) {
+                        done = 2;  // Y^T Y not positive definite
+                    } else {
+                        const double wv = plan_worst(p.norms2, c, k);
+                        p.worst[c] = wv;
+                        if (wv <= p.tol) {
+                            done = 1;
+                        } else if (wv < 0.7 * p.best[c]) {
+                            p.best[c] = wv;
+                            p.stall[c] = 0;
+                        } else if (++p.stall[c] >= 3 && wv < 1e-4) {
+                            done = 1;  // FP32 floor
+                        }
+                        if (done == 0) {
+                            const double* tc = p.theta + (int64_t)c * m;
+                            const double lo = tc[k - 1] * 1.5 + 1e-12;
+                            const double hi = tc[m - 1] > lo ? tc[m - 1] : lo;
+                            p.a[c] = hi < 0.6 ? hi : 0.6;
+                        }
+                    }
+                }
+            } else if (done != 2) {
+                if (p.status[c] != 0) {
+                    done = 2;
+                    p.worst[c] = INFINITY;
+                } else {
+                    for (int j = 0; j < m; ++j)
+                        p.thk[(int64_t)c * m + j] = p.theta[(int64_t)c * m + j];
+                    const double wv = plan_worst(p.norms2, c, k);
+                    p.worst[c] = wv;
+                    if (wv <= p.tol) done = 1;
+                }
+            }
+            p.done[c] = done;
+        }
+        if (mode == 1 && done == 0) {
+            const double* tc = p.thk + (int64_t)c * m;
+            const double lo = tc[k - 1] * 1.5 + 1e-12;
+            const double hi = tc[m - 1] > lo ? tc[m - 1] : lo;
+            p.a[c] = hi < 0.6 ? hi : 0.6;
+        }
+        const int deg = done ? 0 : plan_degree(p.a[c], p.b, p.cap < p.steps ? p.cap : p.steps);
+        p.deg[c] = deg;
+        if (deg > 0) {
+            atomicMax(&s_dmax, deg);
+            atomicAdd(&s_active, 1);
+        }
+    }
+    __syncthreads();
+    int gate = s_dmax;
+    if (p.ctl[1] >= p.max_rounds) gate = 0;
+    // the recurrence scalars of steps 1..steps (row nseg: rows of no segment)
+    const int rows = p.nseg + 1;
+    for (int e = t; e < p.steps * rows; e += kPlanThreads) {
+        const int j = e / rows + 1, c = e % rows;
+        T* q = coef + (int64_t)e * 3;
+        const int deg = c < p.nseg ? p.deg[c] : 0;
+        if (j <= deg) {
+            const double lo = p.a[c];
+            const double cc = (lo + p.b) / 2.0, ee = (p.b - lo) / 2.0;
+            if (j == 1) { q[0] = (T)(1.0 / ee); q[1] = (T)(-cc / ee); q[2] = T(0); }
+            else { q[0] = (T)(2.0 / ee); q[1] = (T)(-2.0 * cc / ee); q[2] = T(-1); }
+        } else {
+            q[0] = T(0); q[1] = T(1); q[2] = T(0);  // copy through
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        p.ctl[0] = gate;
+        if (gate > 0) {
+            p.ctl[1] += 1;
+            p.ctl[2] += gate + 1;  // filter steps + the Rayleigh-Ritz product
+        }
+        p.ctl[3] = s_active;
+    }
+}
+
+__global__ void k_batch_init(BatchPlan p) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < p.nseg) {
+        p.a[c] = 0.1;
+        p.best[c] = INFINITY;
+        p.worst[c] = INFINITY;
+        p.stall[c] = 0;
+        p.done[c] = 0;
+        p.deg[c] = 0;
+        if (p.thk)
+            for (int j = 0; j < p.m; ++j) p.thk[(int64_t)c * p.m + j] = 0.0;
+    }
+    if (c < 4) p.ctl[c] = 0;
+}
+
+}  // namespace
+
+void batch_plan_init(const BatchPlan& p, hipStream_t s) {
+    SI_CHECK(p.nseg >= 1, kErrInvalidArgument, "batch plan: no segments");
+    hipLaunchKernelGGL(k_batch_init, dim3((unsigned)ceil_div((int64_t)p.nseg + 4, (int64_t)256)),
+                       dim3(256), 0, s, p);
+    SI_LAUNCH_CHECK();
+}
+
+void batch_plan_f32(const BatchPlan& p, int update, float* coef, hipStream_t s) {
+    SI_CHECK(p.steps >= 2 && p.k >= 1 && p.k <= p.m, kErrInvalidArgument, "batch plan: bad sizes");
+    hipLaunchKernelGGL((k_batch_plan<float>), dim3(1), dim3(kPlanThreads), 0, s, p, 0, update, coef);
+    SI_LAUNCH_CHECK();
+}
+
+void batch_plan_f64(const BatchPlan& p, int update, double* coef, hipStream_t s) {
+    SI_CHECK(p.steps >= 2 && p.k >= 1 && p.k <= p.m && p.thk, kErrInvalidArgument,
+             "batch plan: bad sizes");
+    hipLaunchKernelGGL((k_batch_plan<double>), dim3(1), dim3(kPlanThreads), 0, s, p, 1, update,
+                       coef);
+    SI_LAUNCH_CHECK();
+}
+
+}  // namespace si

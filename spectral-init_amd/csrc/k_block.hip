@@ -27,7 +27,9 @@ __global__ __launch_bounds__(256) void k_gram(int64_t n, const T* __restrict__ a
                                               const T* __restrict__ b2, int ldb2, int q,
                                               double* __restrict__ partial,
                                               int64_t rows_per_block,
-                                              const int64_t* __restrict__ blk_rows) {
+                                              const int64_t* __restrict__ blk_rows,
+                                              const int32_t* __restrict__ gate) {
+    if (gate != nullptr && *gate == 0) return;
     __shared__ double red[PT * 16][QT * 16 + 1];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
@@ -188,7 +190,9 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const double* __restric
 // segment c = blockIdx.y; same fixed order as above.
 __global__ __launch_bounds__(256) void k_reduce_partials_seg(const double* __restrict__ partial,
                                                              const int32_t* __restrict__ seg_blk,
-                                                             int pq, double* __restrict__ g) {
+                                                             int pq, double* __restrict__ g,
+                                                             const int32_t* __restrict__ gate) {
+    if (gate != nullptr && *gate == 0) return;
     constexpr int E = 8, G = 256 / E, D = 8;
     __shared__ double red[G][E + 1];
     const int lane = threadIdx.x % E, grp = threadIdx.x / E;
@@ -219,9 +223,9 @@ __global__ __launch_bounds__(256) void k_reduce_partials_seg(const double* __res
 }
 
 void reduce_partials_seg(const double* partial, const int32_t* seg_blk, int nseg, int pq,
-                         double* g, hipStream_t s) {
+                         double* g, hipStream_t s, const int32_t* gate = nullptr) {
     hipLaunchKernelGGL(k_reduce_partials_seg, dim3(ceil_div(pq, 8), nseg), dim3(256), 0, s,
-                       partial, seg_blk, pq, g);
+                       partial, seg_blk, pq, g, gate);
     SI_LAUNCH_CHECK();
 }
 
@@ -234,7 +238,7 @@ void reduce_partials(const double* partial, int nparts, int pq, double* g, hipSt
 template <typename T>
 void gram_impl(int64_t n, const T* a, int lda, int p, const T* b, int ldb, int q1, const T* b2,
                int ldb2, int q, double* g, double* ws, hipStream_t s,
-               const SegWork* seg = nullptr) {
+               const SegWork* seg = nullptr, const int32_t* gate = nullptr) {
     SI_CHECK(p >= 1 && q >= 1 && p <= 64 && q <= 64 && q1 >= 1 && q1 <= q, kErrInvalidArgument,
              "gram: width out of range");
     SI_CHECK(q1 == q || b2 != nullptr, kErrInvalidArgument, "gram: second block missing");
@@ -261,7 +265,7 @@ void gram_impl(int64_t n, const T* a, int lda, int p, const T* b, int ldb, int q
 #define SI_GRAM_CASE(PT_, QT_)                                                                   \
     if (pt == PT_ && qt == QT_) {                                                                \
         hipLaunchKernelGGL((k_gram<T, PT_, QT_>), dim3(nblocks), dim3(256), 0, s, n, a, lda, p, b, \
-                           ldb, q1, b2, ldb2, q, ws, rpb, blk_rows);                              \
+                           ldb, q1, b2, ldb2, q, ws, rpb, blk_rows, gate);                        \
     } else
     SI_GRAM_CASE(1, 1) SI_GRAM_CASE(1, 2) SI_GRAM_CASE(1, 3) SI_GRAM_CASE(1, 4)
     SI_GRAM_CASE(2, 1) SI_GRAM_CASE(2, 2) SI_GRAM_CASE(2, 3) SI_GRAM_CASE(2, 4)
@@ -271,7 +275,7 @@ void gram_impl(int64_t n, const T* a, int lda, int p, const T* b, int ldb, int q
 #undef SI_GRAM_CASE
     SI_LAUNCH_CHECK();
     if (seg)
-        reduce_partials_seg(ws, seg->seg_blk, seg->nseg, p * q, g, s);
+        reduce_partials_seg(ws, seg->seg_blk, seg->nseg, p * q, g, s, gate);
     else
         reduce_partials(ws, (int)nblocks, p * q, g, s);
 }
@@ -585,7 +589,8 @@ __global__ __launch_bounds__(256) void k_rr_update_seg(const int64_t* __restrict
                                                        const double* __restrict__ c,
                                                        const double* __restrict__ theta, int k,
                                                        T* __restrict__ x, T* __restrict__ ax,
-                                                       double* __restrict__ partial) {
+                                                       double* __restrict__ partial,
+                                                       const int32_t* __restrict__ gate) {
     constexpr int R = sizeof(T) == 8 ? 32 : 64;
     constexpr int TPR = 256 / R;
     constexpr int CPT = QMAX / TPR;
@@ -594,6 +599,7 @@ __global__ __launch_bounds__(256) void k_rr_update_seg(const int64_t* __restrict
     __shared__ T cm[64 * QMAX];
     __shared__ T tile[R * LDT];
     __shared__ double red[R][2 * 8 + 1];
+    if (gate != nullptr && *gate == 0) return;
     const int t = threadIdx.x;
     const int g = t % TPR;
     const int sg = blk_seg[blockIdx.x];
@@ -636,18 +642,18 @@ void rr_update_impl(int64_t n, const T* y, const T* ly, int ld, int m, const dou
 template <typename T>
 void rr_update_seg_impl(const SegWork& tiles, const T* y, const T* ly, int ld, int m,
                         const double* c, const double* theta, int k, T* x, T* ax,
-                        double* norms2, double* ws, hipStream_t s) {
+                        double* norms2, double* ws, hipStream_t s, const int32_t* gate) {
     SI_CHECK(m >= 1 && m <= 64 && k >= 1 && k <= 8 && k <= m, kErrInvalidArgument,
              "rr_update: need k <= 8, k <= m <= 64");
     const dim3 grid((unsigned)tiles.nblk);
     if (m <= 16)
-        hipLaunchKernelGGL((k_rr_update_seg<T, 16>), grid, dim3(256), 0, s, tiles.blk_rows, tiles.blk_seg, y, ly, ld, m, c, theta, k, x, ax, ws);
+        hipLaunchKernelGGL((k_rr_update_seg<T, 16>), grid, dim3(256), 0, s, tiles.blk_rows, tiles.blk_seg, y, ly, ld, m, c, theta, k, x, ax, ws, gate);
     else if (m <= 32)
-        hipLaunchKernelGGL((k_rr_update_seg<T, 32>), grid, dim3(256), 0, s, tiles.blk_rows, tiles.blk_seg, y, ly, ld, m, c, theta, k, x, ax, ws);
+        hipLaunchKernelGGL((k_rr_update_seg<T, 32>), grid, dim3(256), 0, s, tiles.blk_rows, tiles.blk_seg, y, ly, ld, m, c, theta, k, x, ax, ws, gate);
     else
-        hipLaunchKernelGGL((k_rr_update_seg<T, 64>), grid, dim3(256), 0, s, tiles.blk_rows, tiles.blk_seg, y, ly, ld, m, c, theta, k, x, ax, ws);
+        hipLaunchKernelGGL((k_rr_update_seg<T, 64>), grid, dim3(256), 0, s, tiles.blk_rows, tiles.blk_seg, y, ly, ld, m, c, theta, k, x, ax, ws, gate);
     SI_LAUNCH_CHECK();
-    reduce_partials_seg(ws, tiles.seg_blk, tiles.nseg, 2 * k, norms2, s);
+    reduce_partials_seg(ws, tiles.seg_blk, tiles.nseg, 2 * k, norms2, s, gate);
 }
 
 // ------------------------------------------------------------------ elementwise
@@ -770,22 +776,22 @@ size_t rr_update_workspace_doubles(int64_t n, int k) {
 }
 
 void gram2_seg_f32(const SegWork& blocks, const float* a, int ld, int p, const float* b2, int ld2,
-                   double* g, double* ws, hipStream_t s) {
-    gram_impl<float>(0, a, ld, p, a, ld, p, b2, ld2, 2 * p, g, ws, s, &blocks);
+                   double* g, double* ws, hipStream_t s, const int32_t* gate) {
+    gram_impl<float>(0, a, ld, p, a, ld, p, b2, ld2, 2 * p, g, ws, s, &blocks, gate);
 }
 void gram2_seg_f64(const SegWork& blocks, const double* a, int ld, int p, const double* b2,
-                   int ld2, double* g, double* ws, hipStream_t s) {
-    gram_impl<double>(0, a, ld, p, a, ld, p, b2, ld2, 2 * p, g, ws, s, &blocks);
+                   int ld2, double* g, double* ws, hipStream_t s, const int32_t* gate) {
+    gram_impl<double>(0, a, ld, p, a, ld, p, b2, ld2, 2 * p, g, ws, s, &blocks, gate);
 }
 void rr_update_seg_f32(const SegWork& tiles, const float* y, const float* ly, int ld, int m,
                        const double* c, const double* theta, int k, float* x, float* ax,
-                       double* norms2, double* ws, hipStream_t s) {
-    rr_update_seg_impl<float>(tiles, y, ly, ld, m, c, theta, k, x, ax, norms2, ws, s);
+                       double* norms2, double* ws, hipStream_t s, const int32_t* gate) {
+    rr_update_seg_impl<float>(tiles, y, ly, ld, m, c, theta, k, x, ax, norms2, ws, s, gate);
 }
 void rr_update_seg_f64(const SegWork& tiles, const double* y, const double* ly, int ld, int m,
                        const double* c, const double* theta, int k, double* x, double* ax,
-                       double* norms2, double* ws, hipStream_t s) {
-    rr_update_seg_impl<double>(tiles, y, ly, ld, m, c, theta, k, x, ax, norms2, ws, s);
+                       double* norms2, double* ws, hipStream_t s, const int32_t* gate) {
+    rr_update_seg_impl<double>(tiles, y, ly, ld, m, c, theta, k, x, ax, norms2, ws, s, gate);
 }
 void gram2_f32(int64_t n, const float* a, int lda, int p, const float* b, int ldb, int q1,
                const float* b2, int ldb2, int q2, double* g, double* ws, hipStream_t s) {

@@ -200,7 +200,8 @@ __global__ __launch_bounds__(kThreads) void k_gen_eig(int n, const double* __res
                                                        double* __restrict__ w,
                                                        double* __restrict__ cout,
                                                        int* __restrict__ status, int64_t sa,
-                                                       int64_t sb) {
+                                                       int64_t sb, const int32_t* __restrict__ gate) {
+    if (gate != nullptr && *gate == 0) return;  // device-planned round with nothing to do
     ain += blockIdx.x * sa;
     bin += blockIdx.x * sb;
     w += (int64_t)blockIdx.x * n;
@@ -421,7 +422,7 @@ void small_gen_eig(int n, const double* a, const double* b, double* w, double* c
                    hipStream_t s) {
     SI_CHECK(n >= 1 && n <= kSmallMax, kErrInvalidArgument, "small_gen_eig: n out of range");
     hipLaunchKernelGGL(k_gen_eig, dim3(1), dim3(small_threads(n)), 0, s, n, a, n, b, n, 0, w, c, status,
-                       (int64_t)0, (int64_t)0);
+                       (int64_t)0, (int64_t)0, (const int32_t*)nullptr);
     SI_LAUNCH_CHECK();
 }
 
@@ -429,16 +430,16 @@ void small_gen_eig_scaled(int n, const double* a, int lda, const double* b, int 
                           double* c, int* status, hipStream_t s) {
     SI_CHECK(n >= 1 && n <= kSmallMax, kErrInvalidArgument, "small_gen_eig: n out of range");
     hipLaunchKernelGGL(k_gen_eig, dim3(1), dim3(small_threads(n)), 0, s, n, a, lda, b, ldb, 1, w, c, status,
-                       (int64_t)0, (int64_t)0);
+                       (int64_t)0, (int64_t)0, (const int32_t*)nullptr);
     SI_LAUNCH_CHECK();
 }
 void small_gen_eig_scaled_batched(int count, int n, const double* a, int lda, int64_t sa,
                                   const double* b, int ldb, int64_t sb, double* w, double* c,
-                                  int* status, hipStream_t s) {
+                                  int* status, hipStream_t s, const int32_t* gate) {
     SI_CHECK(n >= 1 && n <= kSmallMax && count >= 1, kErrInvalidArgument,
              "small_gen_eig_batched: size out of range");
     hipLaunchKernelGGL(k_gen_eig, dim3(count), dim3(small_threads(n)), 0, s, n, a, lda, b, ldb, 1, w, c,
-                       status, sa, sb);
+                       status, sa, sb, gate);
     SI_LAUNCH_CHECK();
 }
 void dense_seg_eig(const DevLaplacian& L, int count, const int64_t* seg_row0, const int32_t* seg_n,

@@ -16,8 +16,14 @@
 //                     wave butterflies and a fixed-order sum of the 4 waves.
 // Every reduction has a fixed order, so results are deterministic.
 // Blocks are remapped so that each of the 8 XCDs streams a contiguous range
-// of rows (its L2 then holds the x rows that neighbouring rows gather).
+// of rows (its L2 then holds the x rows that neighbouring rows gather).  A
+// launch gated by a device-planned round (the batched solver) uses only the
+// resident workgroups, each walking its XCD's blocks, so that a launch the gate
+// keeps idle costs a few microseconds instead of dispatching every block.
 #include <hip/hip_ext.h>
+
+#include <map>
+#include <mutex>
 
 #include "si_host.hpp"
 
@@ -70,6 +76,25 @@ __device__ inline void seg_scalars(const int32_t* __restrict__ rseg, const T* __
     }
 }
 
+// Three-buffer ring of a device-planned filter: step j of a round whose gate
+// (last step) is g reads x = the previous step's output (the round's input x0
+// for j = 1), z = the output before that (x0 for j = 2) and writes
+// ring[(g - j) mod 3], so the round's last step always writes ring[0].  On
+// entry x is x0 and z null.
+template <typename T>
+__device__ inline void ring_buffers(int g, int j, T* r0, T* r1, T* r2, const T*& x, T*& out,
+                                    const T*& z) {
+    auto pick = [&](int d) -> T* {
+        const int i = ((d % 3) + 3) % 3;
+        return i == 0 ? r0 : (i == 1 ? r1 : r2);
+    };
+    const T* x0 = x;
+    out = pick(g - j);
+    if (j >= 2) x = pick(g - j + 1);
+    if (j == 2) z = x0;
+    else if (j >= 3) z = pick(g - j + 2);
+}
+
 // -------------------------------------------------------------- long rows
 // A row with more than kLongRow entries (a degree hub) gets a whole 256-thread
 // workgroup, appended after the row blocks of the same launch: threads stride
@@ -118,18 +143,13 @@ __device__ void long_row(int64_t i, const int32_t* __restrict__ rp, const int32_
     }
 }
 
+// One block of rows (blockDim.x / G of them) of the narrow schedule.
 template <typename T, int K, int G>
-__global__ __launch_bounds__(256) void k_spmm_narrow(
-    const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const T* __restrict__ val,
-    const T* __restrict__ x, T* __restrict__ out, const T* __restrict__ z, int ld, int64_t n,
-    T s_l, T s_x, T s_z, const int32_t* __restrict__ rseg, const T* __restrict__ coef,
-    const int32_t* __restrict__ long_rows, int64_t nb_main) {
-    if ((int64_t)blockIdx.x >= nb_main) {  // appended hub-row workgroups
-        long_row<T>(long_rows[blockIdx.x - nb_main], rp, col, val, x, out, z, ld, K, s_l, s_x,
-                    s_z, rseg, coef);
-        return;
-    }
-    const int64_t blk = xcd_block(blockIdx.x, nb_main);
+__device__ __forceinline__ void narrow_rows(
+    int64_t blk, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+    const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ out,
+    const T* __restrict__ z, int ld, int64_t n, T s_l, T s_x, T s_z,
+    const int32_t* __restrict__ rseg, const T* __restrict__ coef) {
     const int lane = threadIdx.x % G;
     const int64_t i = (blk * blockDim.x + threadIdx.x) / G;
     if (i >= n) return;
@@ -185,24 +205,73 @@ __global__ __launch_bounds__(256) void k_spmm_narrow(
     }
 }
 
-// -------------------------------------------------------------- wide
-template <typename T, int K, int SLOTS, int U>
-__global__ __launch_bounds__(256) void k_spmm_wide(
+
+// One workgroup per row block (blocks remapped per XCD); workgroups from
+// nb_main on take one hub row each.
+template <typename T, int K, int G>
+__global__ __launch_bounds__(256) void k_spmm_narrow(
     const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const T* __restrict__ val,
     const T* __restrict__ x, T* __restrict__ out, const T* __restrict__ z, int ld, int64_t n,
     T s_l, T s_x, T s_z, const int32_t* __restrict__ rseg, const T* __restrict__ coef,
     const int32_t* __restrict__ long_rows, int64_t nb_main) {
-    using V = typename Vec4<T>::type;
-    constexpr int VEC = Vec4<T>::n;
-    constexpr int C = K / VEC;                 // lanes across one x row
-    constexpr int RPW = 64 / (C * SLOTS);      // rows per wavefront
-    static_assert(C * SLOTS * RPW == 64, "bad wide SpMM shape");
     if ((int64_t)blockIdx.x >= nb_main) {  // appended hub-row workgroups
         long_row<T>(long_rows[blockIdx.x - nb_main], rp, col, val, x, out, z, ld, K, s_l, s_x,
                     s_z, rseg, coef);
         return;
     }
-    const int64_t blk = xcd_block(blockIdx.x, nb_main);
+    narrow_rows<T, K, G>(xcd_block(blockIdx.x, nb_main), rp, col, val, x, out, z, ld, n, s_l,
+                         s_x, s_z, rseg, coef);
+}
+
+// The gated form (launches of a device-planned round): the launch does
+// nothing when gate_step > *gate; with a ring (r0 set) its buffers come from
+// ring_buffers.  LOOP = false: one workgroup per row block, as above.  LOOP =
+// true: workgroups [0, grid_main) walk the nb_main row blocks with stride
+// grid_main (a multiple of 8, so every block stays on the XCD that xcd_block
+// assigns it) -- an idle launch then dispatches only grid_main workgroups, a
+// busy one runs a few percent slower.  Workgroups from grid_main on take one
+// hub row each.
+template <typename T, int K, int G, bool LOOP>
+__global__ __launch_bounds__(256) void k_spmm_narrow_gated(
+    const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const T* __restrict__ val,
+    const T* x_arg, T* out_arg, const T* z_arg, int ld, int64_t n,
+    T s_l, T s_x, T s_z, const int32_t* __restrict__ rseg, const T* __restrict__ coef,
+    const int32_t* __restrict__ long_rows, int64_t nb_main, int64_t grid_main,
+    const int32_t* __restrict__ gate, int gate_step, T* r0, T* r1, T* r2) {
+    const T* x = x_arg;
+    T* out = out_arg;
+    const T* z = z_arg;
+    const int g = *gate;
+    if (gate_step > g) return;  // step past the round's filter degree
+    if (r0 != nullptr) ring_buffers<T>(g, gate_step, r0, r1, r2, x, out, z);
+    if ((int64_t)blockIdx.x >= grid_main) {  // appended hub-row workgroups
+        long_row<T>(long_rows[blockIdx.x - grid_main], rp, col, val, x, out, z, ld, K, s_l, s_x,
+                    s_z, rseg, coef);
+        return;
+    }
+    if constexpr (LOOP) {
+        for (int64_t bb = blockIdx.x; bb < nb_main; bb += grid_main)
+            narrow_rows<T, K, G>(xcd_block(bb, nb_main), rp, col, val, x, out, z, ld, n, s_l,
+                                 s_x, s_z, rseg, coef);
+    } else {
+        narrow_rows<T, K, G>(xcd_block(blockIdx.x, nb_main), rp, col, val, x, out, z, ld, n,
+                             s_l, s_x, s_z, rseg, coef);
+    }
+}
+
+// -------------------------------------------------------------- wide
+// One block of rows (4 wavefronts x RPW rows) of the wide schedule.
+template <typename T, int K, int SLOTS, int U>
+__device__ __forceinline__ void wide_rows(
+    int64_t blk, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+    const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ out,
+    const T* __restrict__ z, int ld, int64_t n, T s_l, T s_x, T s_z,
+    const int32_t* __restrict__ rseg, const T* __restrict__ coef) {
+    using V = typename Vec4<T>::type;
+    constexpr int VEC = Vec4<T>::n;
+    constexpr int C = K / VEC;                 // lanes across one x row
+    constexpr int RPW = 64 / (C * SLOTS);      // rows per wavefront
+    static_assert(C * SLOTS * RPW == 64, "bad wide SpMM shape");
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int chunk = lane % C;
@@ -274,12 +343,65 @@ __global__ __launch_bounds__(256) void k_spmm_wide(
     }
 }
 
+
+// Grid as k_spmm_narrow's.
+template <typename T, int K, int SLOTS, int U>
+__global__ __launch_bounds__(256) void k_spmm_wide(
+    const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const T* __restrict__ val,
+    const T* __restrict__ x, T* __restrict__ out, const T* __restrict__ z, int ld, int64_t n,
+    T s_l, T s_x, T s_z, const int32_t* __restrict__ rseg, const T* __restrict__ coef,
+    const int32_t* __restrict__ long_rows, int64_t nb_main) {
+    if ((int64_t)blockIdx.x >= nb_main) {  // appended hub-row workgroups
+        long_row<T>(long_rows[blockIdx.x - nb_main], rp, col, val, x, out, z, ld, K, s_l, s_x,
+                    s_z, rseg, coef);
+        return;
+    }
+    wide_rows<T, K, SLOTS, U>(xcd_block(blockIdx.x, nb_main), rp, col, val, x, out, z, ld, n,
+                              s_l, s_x, s_z, rseg, coef);
+}
+
+// Gated form, as k_spmm_narrow_gated.
+template <typename T, int K, int SLOTS, int U, bool LOOP>
+__global__ __launch_bounds__(256) void k_spmm_wide_gated(
+    const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const T* __restrict__ val,
+    const T* x_arg, T* out_arg, const T* z_arg, int ld, int64_t n,
+    T s_l, T s_x, T s_z, const int32_t* __restrict__ rseg, const T* __restrict__ coef,
+    const int32_t* __restrict__ long_rows, int64_t nb_main, int64_t grid_main,
+    const int32_t* __restrict__ gate, int gate_step, T* r0, T* r1, T* r2) {
+    const T* x = x_arg;
+    T* out = out_arg;
+    const T* z = z_arg;
+    const int g = *gate;
+    if (gate_step > g) return;  // step past the round's filter degree
+    if (r0 != nullptr) ring_buffers<T>(g, gate_step, r0, r1, r2, x, out, z);
+    if ((int64_t)blockIdx.x >= grid_main) {  // appended hub-row workgroups
+        long_row<T>(long_rows[blockIdx.x - grid_main], rp, col, val, x, out, z, ld, K, s_l, s_x,
+                    s_z, rseg, coef);
+        return;
+    }
+    if constexpr (LOOP) {
+        for (int64_t bb = blockIdx.x; bb < nb_main; bb += grid_main)
+            wide_rows<T, K, SLOTS, U>(xcd_block(bb, nb_main), rp, col, val, x, out, z, ld, n,
+                                      s_l, s_x, s_z, rseg, coef);
+    } else {
+        wide_rows<T, K, SLOTS, U>(xcd_block(blockIdx.x, nb_main), rp, col, val, x, out, z, ld, n,
+                                  s_l, s_x, s_z, rseg, coef);
+    }
+}
+
 template <typename T>
 struct Epi {
     T s_l, s_x, s_z;
     const T* z;
     const int32_t* rseg;
     const T* coef;
+    // device-side gate: the launch does nothing when gate_step > *gate
+    const int32_t* gate = nullptr;
+    int gate_step = 0;
+    // ring mode (with a gate): buffers from ring_buffers, x is the round's x0
+    T* ring[3] = {nullptr, nullptr, nullptr};
+    // resident workgroups walking the row blocks (cheap when the gate idles it)
+    bool resident = false;
     // profiled launches: events recorded by the kernel's own dispatch
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -295,14 +417,56 @@ void launch(F kernel, dim3 grid, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, 
         hipLaunchKernelGGL(kernel, grid, dim3(256), 0, s, args...);
 }
 
+// Workgroups of `kernel` (256 threads) resident on the whole device at once,
+// rounded down to a multiple of 8 (one per XCD slot); looked up once per
+// kernel.
+template <typename F>
+int64_t resident_workgroups(F kernel) {
+    static std::mutex mu;
+    static std::map<const void*, int64_t> known;
+    std::lock_guard<std::mutex> lock(mu);
+    const void* key = reinterpret_cast<const void*>(kernel);
+    auto it = known.find(key);
+    if (it != known.end()) return it->second;
+    int dev = 0, cus = 0, per_cu = 0;
+    SI_HIP(hipGetDevice(&dev));
+    SI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    SI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0));
+    int64_t g = (int64_t)cus * (per_cu > 0 ? per_cu : 1);
+    g = g / 8 * 8;
+    if (g < 8) g = 8;
+    known.emplace(key, g);
+    return g;
+}
+
+// Row-block grid of a launch: one workgroup per block, or -- for a launch
+// that a device gate may keep idle, where the empty launch must cost little --
+// the resident workgroups walking the blocks.
+template <typename F>
+int64_t main_grid(F kernel, int64_t blocks, bool resident) {
+    if (!resident) return blocks;
+    const int64_t g = resident_workgroups(kernel);
+    return blocks < g ? blocks : g;
+}
+
 template <typename T, int K, int G>
 void launch_narrow(const DevLaplacian& L, const T* val, const T* x, T* out, int ld,
                    const Epi<T>& ep, hipStream_t s) {
     const int64_t threads = L.n * G;
     const int64_t blocks = ceil_div(threads, 256);
-    launch(k_spmm_narrow<T, K, G>, dim3(blocks + L.n_long), s, ep.ev0, ep.ev1,
+    if (ep.gate == nullptr) {
+        launch(k_spmm_narrow<T, K, G>, dim3(blocks + L.n_long), s, ep.ev0, ep.ev1,
+               (const int32_t*)L.rowptr, (const int32_t*)L.col, val, x, out, ep.z, ld, L.n,
+               ep.s_l, ep.s_x, ep.s_z, ep.rseg, ep.coef, (const int32_t*)L.long_rows, blocks);
+        return;
+    }
+    auto kern = ep.resident ? k_spmm_narrow_gated<T, K, G, true>
+                            : k_spmm_narrow_gated<T, K, G, false>;
+    const int64_t grid = main_grid(kern, blocks, ep.resident);
+    launch(kern, dim3(grid + L.n_long), s, ep.ev0, ep.ev1,
            (const int32_t*)L.rowptr, (const int32_t*)L.col, val, x, out, ep.z, ld, L.n, ep.s_l,
-           ep.s_x, ep.s_z, ep.rseg, ep.coef, (const int32_t*)L.long_rows, blocks);
+           ep.s_x, ep.s_z, ep.rseg, ep.coef, (const int32_t*)L.long_rows, blocks, grid, ep.gate,
+           ep.gate_step, ep.ring[0], ep.ring[1], ep.ring[2]);
 }
 
 template <typename T, int K, int SLOTS, int U>
@@ -313,9 +477,19 @@ void launch_wide(const DevLaplacian& L, const T* val, const T* x, T* out, int ld
     constexpr int RPW = 64 / (C * SLOTS);
     const int64_t waves = ceil_div(L.n, RPW);
     const int64_t blocks = ceil_div(waves, 4);
-    launch(k_spmm_wide<T, K, SLOTS, U>, dim3(blocks + L.n_long), s, ep.ev0, ep.ev1,
+    if (ep.gate == nullptr) {
+        launch(k_spmm_wide<T, K, SLOTS, U>, dim3(blocks + L.n_long), s, ep.ev0, ep.ev1,
+               (const int32_t*)L.rowptr, (const int32_t*)L.col, val, x, out, ep.z, ld, L.n,
+               ep.s_l, ep.s_x, ep.s_z, ep.rseg, ep.coef, (const int32_t*)L.long_rows, blocks);
+        return;
+    }
+    auto kern = ep.resident ? k_spmm_wide_gated<T, K, SLOTS, U, true>
+                            : k_spmm_wide_gated<T, K, SLOTS, U, false>;
+    const int64_t grid = main_grid(kern, blocks, ep.resident);
+    launch(kern, dim3(grid + L.n_long), s, ep.ev0, ep.ev1,
            (const int32_t*)L.rowptr, (const int32_t*)L.col, val, x, out, ep.z, ld, L.n, ep.s_l,
-           ep.s_x, ep.s_z, ep.rseg, ep.coef, (const int32_t*)L.long_rows, blocks);
+           ep.s_x, ep.s_z, ep.rseg, ep.coef, (const int32_t*)L.long_rows, blocks, grid, ep.gate,
+           ep.gate_step, ep.ring[0], ep.ring[1], ep.ring[2]);
 }
 
 // One launch of a directly supported width (1, 2, 3, 4, 8, 16, 32, 64).
@@ -375,7 +549,7 @@ void spmm_chunk(const DevLaplacian& L, const T* val, const T* x, T* out, int k, 
 template <typename T>
 void spmm_impl(const DevLaplacian& L, const T* val, const T* x, T* out, int k, int ld, T s_l,
                T s_x, T s_z, const T* z, hipStream_t s, const int32_t* rseg = nullptr,
-               const T* coef = nullptr) {
+               const T* coef = nullptr, const int32_t* gate = nullptr, int gate_step = 0) {
     SI_CHECK(k >= 1 && ld >= k, kErrInvalidArgument, "spmm: need 1 <= k <= ld");
     SI_CHECK(x != out && (z == nullptr || z != out), kErrInvalidArgument,
              "spmm: output must not alias an input");
@@ -397,7 +571,7 @@ void spmm_impl(const DevLaplacian& L, const T* val, const T* x, T* out, int k, i
             for (int c : widths_narrow)
                 if (c <= rem) { w = c; break; }
         }
-        Epi<T> ep{s_l, s_x, s_z, z ? z + off : nullptr, rseg, coef};
+        Epi<T> ep{s_l, s_x, s_z, z ? z + off : nullptr, rseg, coef, gate, gate_step};
         if (prof) {
             ep.ev0 = pooled_event(g_profile);
             ep.ev1 = pooled_event(g_profile);
@@ -626,25 +800,67 @@ void spmm_autotune(DevLaplacian& L, hipStream_t s, int widths) {
 }
 
 void spmm_f32(const DevLaplacian& L, const float* x, float* out, int k, int ld, float s_l,
-              float s_x, float s_z, const float* z, hipStream_t s) {
-    spmm_impl<float>(L, L.a32, x, out, k, ld, s_l, s_x, s_z, z, s);
+              float s_x, float s_z, const float* z, hipStream_t s, const int32_t* gate,
+              int gate_step) {
+    spmm_impl<float>(L, L.a32, x, out, k, ld, s_l, s_x, s_z, z, s, nullptr, nullptr, gate,
+                     gate_step);
 }
 
 void spmm_f64(const DevLaplacian& L, const double* x, double* out, int k, int ld, double s_l,
-              double s_x, double s_z, const double* z, hipStream_t s) {
-    spmm_impl<double>(L, L.l64, x, out, k, ld, s_l, s_x, s_z, z, s);
+              double s_x, double s_z, const double* z, hipStream_t s, const int32_t* gate,
+              int gate_step) {
+    spmm_impl<double>(L, L.l64, x, out, k, ld, s_l, s_x, s_z, z, s, nullptr, nullptr, gate,
+                      gate_step);
+}
+
+namespace {
+
+template <typename T>
+void spmm_ring_impl(const DevLaplacian& L, const T* val, const T* x0, T* const ring[3], int k,
+                    int ld, const int32_t* rseg, const T* coef, const int32_t* gate, int step,
+                    bool resident, hipStream_t s) {
+    SI_CHECK(rseg && coef && gate && ring[0] && ring[1] && ring[2] && x0 && step >= 1,
+             kErrInvalidArgument, "spmm ring step: buffers, segment scalars and gate required");
+    SI_CHECK(ring[0] != ring[1] && ring[1] != ring[2] && ring[0] != ring[2] && x0 != ring[0] &&
+                 x0 != ring[1] && x0 != ring[2],
+             kErrInvalidArgument, "spmm ring step: the ring and x0 must be four distinct buffers");
+    bool aligned = (ld % Vec4<T>::n) == 0 && (reinterpret_cast<uintptr_t>(x0) % 16) == 0;
+    for (int i = 0; i < 3; ++i) aligned = aligned && (reinterpret_cast<uintptr_t>(ring[i]) % 16) == 0;
+    const bool single = k == 1 || k == 2 || k == 3 || k == 4 || k == 8 ||
+                        ((k == 16 || k == 32 || k == 64) && aligned);
+    SI_CHECK(single && ld >= k, kErrInvalidArgument, "spmm ring step: width must be one launch");
+    Epi<T> ep{T(1), T(0), T(0), nullptr, rseg, coef, gate, step};
+    for (int i = 0; i < 3; ++i) ep.ring[i] = ring[i];
+    ep.resident = resident;
+    spmm_chunk<T>(L, val, x0, ring[0], k, ld, ep, s);
+}
+
+}  // namespace
+
+void spmm_seg_ring_f32(const DevLaplacian& L, const float* x0, float* const ring[3], int k, int ld,
+                       const int32_t* rseg, const float* coef, const int32_t* gate, int step,
+                       bool resident, hipStream_t s) {
+    spmm_ring_impl<float>(L, L.a32, x0, ring, k, ld, rseg, coef, gate, step, resident, s);
+}
+
+void spmm_seg_ring_f64(const DevLaplacian& L, const double* x0, double* const ring[3], int k,
+                       int ld, const int32_t* rseg, const double* coef, const int32_t* gate,
+                       int step, bool resident, hipStream_t s) {
+    spmm_ring_impl<double>(L, L.l64, x0, ring, k, ld, rseg, coef, gate, step, resident, s);
 }
 
 void spmm_seg_f32(const DevLaplacian& L, const float* x, float* out, int k, int ld,
-                  const float* z, const int32_t* rseg, const float* coef, hipStream_t s) {
+                  const float* z, const int32_t* rseg, const float* coef, hipStream_t s,
+                  const int32_t* gate, int gate_step) {
     SI_CHECK(rseg && coef, kErrInvalidArgument, "spmm_seg: segment map and scalars required");
-    spmm_impl<float>(L, L.a32, x, out, k, ld, 1.f, 0.f, 0.f, z, s, rseg, coef);
+    spmm_impl<float>(L, L.a32, x, out, k, ld, 1.f, 0.f, 0.f, z, s, rseg, coef, gate, gate_step);
 }
 
 void spmm_seg_f64(const DevLaplacian& L, const double* x, double* out, int k, int ld,
-                  const double* z, const int32_t* rseg, const double* coef, hipStream_t s) {
+                  const double* z, const int32_t* rseg, const double* coef, hipStream_t s,
+                  const int32_t* gate, int gate_step) {
     SI_CHECK(rseg && coef, kErrInvalidArgument, "spmm_seg: segment map and scalars required");
-    spmm_impl<double>(L, L.l64, x, out, k, ld, 1.0, 0.0, 0.0, z, s, rseg, coef);
+    spmm_impl<double>(L, L.l64, x, out, k, ld, 1.0, 0.0, 0.0, z, s, rseg, coef, gate, gate_step);
 }
 
 }  // namespace si

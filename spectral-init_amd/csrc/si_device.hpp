@@ -118,17 +118,37 @@ void gather_rows_f32(const float* src, int lds, float* dst, int ldd, int64_t n, 
 // ----------------------------------------------------------------- SpMM
 // out = s_l * (L x) + s_x * x + s_z * z   for row-major blocks of width k.
 // x, z, out have leading dimension ld (>= k).  z may be null when s_z == 0.
+// With gate set (a device int), the launch does nothing when gate_step > *gate
+// (read when the kernel starts: work of a device-planned round).
 void spmm_f32(const DevLaplacian& L, const float* x, float* out, int k, int ld, float s_l,
-              float s_x, float s_z, const float* z, hipStream_t s);
+              float s_x, float s_z, const float* z, hipStream_t s,
+              const int32_t* gate = nullptr, int gate_step = 0);
 void spmm_f64(const DevLaplacian& L, const double* x, double* out, int k, int ld, double s_l,
-              double s_x, double s_z, const double* z, hipStream_t s);
+              double s_x, double s_z, const double* z, hipStream_t s,
+              const int32_t* gate = nullptr, int gate_step = 0);
 // Segmented form (a block-diagonal L whose diagonal blocks are row segments):
 // row i uses (s_l, s_x, s_z) = coef[3 * rseg[i] + {0, 1, 2}] (device arrays);
 // a segment with s_l == 0 copies s_x * x + s_z * z without gathering.
 void spmm_seg_f32(const DevLaplacian& L, const float* x, float* out, int k, int ld,
-                  const float* z, const int32_t* rseg, const float* coef, hipStream_t s);
+                  const float* z, const int32_t* rseg, const float* coef, hipStream_t s,
+                  const int32_t* gate = nullptr, int gate_step = 0);
 void spmm_seg_f64(const DevLaplacian& L, const double* x, double* out, int k, int ld,
-                  const double* z, const int32_t* rseg, const double* coef, hipStream_t s);
+                  const double* z, const int32_t* rseg, const double* coef, hipStream_t s,
+                  const int32_t* gate = nullptr, int gate_step = 0);
+// Step `step` of a device-planned segmented filter (one launch): with
+// g = *gate the step runs only while step <= g; it reads the previous step's
+// output (x0 for step 1) and the one before (x0 for step 2, none for step 1)
+// from the three-buffer ring and writes ring[(g - step) mod 3], so the round's
+// last step always leaves its result in ring[0].  coef is this step's
+// [(nseg + 1) x 3] scalar row.  resident: launch only the resident workgroups
+// (an idle launch then costs a few microseconds; a busy one is a little
+// slower than one workgroup per row block).  k must be a single-launch width.
+void spmm_seg_ring_f32(const DevLaplacian& L, const float* x0, float* const ring[3], int k, int ld,
+                       const int32_t* rseg, const float* coef, const int32_t* gate, int step,
+                       bool resident, hipStream_t s);
+void spmm_seg_ring_f64(const DevLaplacian& L, const double* x0, double* const ring[3], int k,
+                       int ld, const int32_t* rseg, const double* coef, const int32_t* gate,
+                       int step, bool resident, hipStream_t s);
 bool spmm_width_supported(int k);
 // y = A x for a general CSR matrix (FP64 values, any stored entries) with the
 // products rounded before each sum in storage order (the scalar spmv_csr's
@@ -204,20 +224,52 @@ struct SegWork {
     const int32_t* blk_seg = nullptr;
     const int32_t* seg_blk = nullptr;
 };
+// Round planner of the batched component solver (k_batch.hip): one workgroup
+// reads a round's Ritz values, residual norms and Cholesky status, updates
+// every segment's state and writes the next round's per-step, per-segment
+// recurrence scalars and the gate ctl[0] (filter steps of the next round, 0 =
+// no round).  ctl[1] counts planned rounds, ctl[2] planned SpMM launches,
+// ctl[3] the segments still iterating.  Device arrays; theta and thk rows are
+// m wide, norms2 rows 2k.
+struct BatchPlan {
+    int nseg = 0, k = 0, m = 0, cap = 24, steps = 0, max_rounds = 0;
+    double tol = 1e-6, b = 2.0;
+    double* a = nullptr;      // [nseg] lower end of each segment's damped interval
+    double* best = nullptr;   // [nseg] best residual so far (FP32 rounds)
+    double* worst = nullptr;  // [nseg] latest residual
+    int* stall = nullptr;     // [nseg]
+    int* done = nullptr;      // [nseg] 0 iterating, 1 converged (or at the FP32 floor), 2 failed
+    int* deg = nullptr;       // [nseg] filter degree of the next round
+    double* thk = nullptr;    // [nseg x m] kept Ritz values (FP64 rounds)
+    const int* status = nullptr;
+    const double* theta = nullptr;
+    const double* norms2 = nullptr;
+    int32_t* ctl = nullptr;   // [4]
+};
+// a = 0.1, best = worst = inf, stall = done = deg = 0, thk = 0, ctl = 0
+void batch_plan_init(const BatchPlan& p, hipStream_t s);
+// update = 0: plan from the current state only (the first round); 1: read the
+// results of the round the gate planned (skipped when that gate was 0); 2: read
+// the results of a round launched without a gate
+void batch_plan_f32(const BatchPlan& p, int update, float* coef, hipStream_t s);
+void batch_plan_f64(const BatchPlan& p, int update, double* coef, hipStream_t s);
+
 // per segment c: G_c = Y_c^T [Y_c | B2_c]  (p x 2p each, g = [nseg x p x 2p]);
-// workspace: nblk * 2 p^2 doubles
+// workspace: nblk * 2 p^2 doubles.  The segmented Gram, update and batched
+// small-eigenproblem launches take an optional device gate: with *gate == 0
+// (a device-planned round with nothing to do) they return at once.
 void gram2_seg_f32(const SegWork& blocks, const float* a, int ld, int p, const float* b2, int ld2,
-                   double* g, double* ws, hipStream_t s);
+                   double* g, double* ws, hipStream_t s, const int32_t* gate = nullptr);
 void gram2_seg_f64(const SegWork& blocks, const double* a, int ld, int p, const double* b2,
-                   int ld2, double* g, double* ws, hipStream_t s);
+                   int ld2, double* g, double* ws, hipStream_t s, const int32_t* gate = nullptr);
 // per segment c: x_c = y_c C_c, ax_c = ly_c C_c (C_c = c + c*m*m, theta + c*m),
 // norms2 [nseg x 2k]; workspace: nblk * 2k doubles
 void rr_update_seg_f32(const SegWork& tiles, const float* y, const float* ly, int ld, int m,
                        const double* c, const double* theta, int k, float* x, float* ax,
-                       double* norms2, double* ws, hipStream_t s);
+                       double* norms2, double* ws, hipStream_t s, const int32_t* gate = nullptr);
 void rr_update_seg_f64(const SegWork& tiles, const double* y, const double* ly, int ld, int m,
                        const double* c, const double* theta, int k, double* x, double* ax,
-                       double* norms2, double* ws, hipStream_t s);
+                       double* norms2, double* ws, hipStream_t s, const int32_t* gate = nullptr);
 // G = A^T [B1 | B2]  (p x (q1+q2), q1 + q2 <= 64) in one pass over the rows.
 void gram2_f32(int64_t n, const float* a, int lda, int p, const float* b, int ldb, int q1,
                const float* b2, int ldb2, int q2, double* g, double* ws, hipStream_t s);
@@ -306,7 +358,7 @@ void small_gen_eig_scaled(int n, const double* a, int lda, const double* b, int 
 // c [count x n x n], status [count]
 void small_gen_eig_scaled_batched(int count, int n, const double* a, int lda, int64_t sa,
                                   const double* b, int ldb, int64_t sb, double* w, double* c,
-                                  int* status, hipStream_t s);
+                                  int* status, hipStream_t s, const int32_t* gate = nullptr);
 // dense EVD of count diagonal blocks (<= 64 rows each) of L: device arrays
 // seg_row0 [count], seg_n [count]; evals [count x k]; vectors into rows of evecs
 void dense_seg_eig(const DevLaplacian& L, int count, const int64_t* seg_row0, const int32_t* seg_n,

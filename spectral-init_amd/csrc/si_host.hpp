@@ -51,7 +51,7 @@ enum WsSlot : int {
     kWsSmall0, kWsSmall1, kWsSmall2, kWsSmall3, kWsSmall4, kWsSmall5, kWsSmall6, kWsSmall7,
     kWsPlainS, kWsPlainAS, kWsPlainR, kWsPlainP, kWsPlainAP, kWsPlainPn, kWsPlainAPn,
     kWsPlainX, kWsPlainT, kWsPlainXr, kWsPlainBest, kWsStatus,
-    kWsBatchSmall, kWsBatchCoef, kWsBatchRr, kWsDenseA, kWsDenseAux, kWsDensePiv,
+    kWsBatchSmall, kWsBatchCoef, kWsBatchRr, kWsBatchPlan, kWsDenseA, kWsDenseAux, kWsDensePiv,
     kWsNumSlots
 };
 void* workspace_get(int slot, size_t bytes);

@@ -14,18 +14,20 @@
 //   * the Rayleigh-Ritz Gram matrices, the m x m generalized eigenproblems and
 //     the X = Y C / AX = LY C update are segmented kernels (one workgroup per
 //     segment for the small problems);
-//   * each FP32 round (filter + Rayleigh-Ritz) is captured once per filter
-//     degree as a hipGraph and replayed, so a round is one graph launch plus one
-//     small upload and one small download.
+//   * the decisions between rounds (convergence, stall and failure flags, each
+//     segment's interval and degree, the next round's recurrence scalars) are
+//     made on the device by the round planner (k_batch.hip), whose gate every
+//     launch of a round reads, so several rounds are captured as one hipGraph
+//     and replayed with no host work in between; the host only reads the gate
+//     after each replay to see whether another is needed.
 //
 // The FP64 finish (Rayleigh-Ritz on the lowest mf Ritz vectors and the FP64
-// residual against the exact Laplacian) is segmented the same way.
+// residual against the exact Laplacian) is segmented and planned the same way.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <map>
 
 #include "si_host.hpp"
 
@@ -67,92 +69,60 @@ void build_worklist(const std::vector<int64_t>& row0, const std::vector<int64_t>
     wl.w.seg_blk = wl.segblk.get();
 }
 
-// Largest degree d <= cap with T_d(c/e) <= e^9.2 on [lo, b] (the
-// amplification cap of the single-graph solver).
-int capped_degree(double lo, double b, int cap) {
-    const double c = (lo + b) / 2.0, e = (b - lo) / 2.0;
-    const double x0 = c / e;
-    const int dmax = (int)std::floor(9.2 / std::acosh(std::max(x0, 1.0 + 1e-12)));
-    return std::max(2, std::min(cap, dmax));
-}
-
-// Per-step, per-segment epilogue scalars of a filter of per-segment degree
-// deg[c] (0 = frozen) on [a[c], b]; row `nseg` of each step is the frozen
-// segment of rows that belong to no segment.
-template <typename T>
-void coef_table(const std::vector<double>& a, const std::vector<int>& deg, double b, int steps,
-                std::vector<T>& tab) {
-    const int nseg = (int)a.size();
-    tab.assign((size_t)steps * (nseg + 1) * 3, T(0));
-    for (int j = 1; j <= steps; ++j) {
-        T* row = tab.data() + (size_t)(j - 1) * (nseg + 1) * 3;
-        for (int c = 0; c <= nseg; ++c) {
-            T* q = row + 3 * c;
-            if (c < nseg && j <= deg[c]) {
-                const double cc = (a[c] + b) / 2.0, e = (b - a[c]) / 2.0;
-                if (j == 1) { q[0] = (T)(1.0 / e); q[1] = (T)(-cc / e); q[2] = T(0); }
-                else { q[0] = (T)(2.0 / e); q[1] = (T)(-2.0 * cc / e); q[2] = T(-1); }
-            } else {
-                q[0] = T(0); q[1] = T(1); q[2] = T(0);  // copy through
-            }
-        }
-    }
-}
-
 template <typename T>
 struct Ring {
     T* b[3];
 };
 
-// Enqueue a segmented filter of `steps` steps on w columns (leading dimension
-// ld) starting from x; returns the buffer holding the result.
+// Enqueue a segmented filter of up to `steps` steps on w columns (leading
+// dimension ld) starting from x; step j runs only while j <= *gate, and the
+// ring puts the last step's result in r.b[0], which is returned.  Steps up to
+// `always` run in every planned round (one workgroup per row block); later
+// steps may idle, so they launch only the resident workgroups.
 template <typename T>
 T* enqueue_filter(const DevLaplacian& L, Ring<T>& r, const T* x, int w, int ld, int steps,
-                  const int32_t* rseg, const T* coef, int nseg, int64_t& spmm_calls,
+                  int always, const int32_t* rseg, const T* coef, int nseg, const int32_t* gate,
                   hipStream_t s) {
-    T* yprev = r.b[0];
-    T* y = r.b[1];
-    T* ynew = r.b[2];
     const size_t stride = (size_t)(nseg + 1) * 3;
-    if constexpr (sizeof(T) == 4) spmm_seg_f32(L, x, y, w, ld, nullptr, rseg, coef, s);
-    else spmm_seg_f64(L, x, y, w, ld, nullptr, rseg, coef, s);
-    ++spmm_calls;
-    const T* prev = x;
-    for (int j = 2; j <= steps; ++j) {
+    for (int j = 1; j <= steps; ++j) {
         const T* cj = coef + (size_t)(j - 1) * stride;
-        if constexpr (sizeof(T) == 4) spmm_seg_f32(L, y, ynew, w, ld, prev, rseg, cj, s);
-        else spmm_seg_f64(L, y, ynew, w, ld, prev, rseg, cj, s);
-        ++spmm_calls;
-        T* t = yprev; yprev = y; y = ynew; ynew = t;
-        prev = yprev;
+        if constexpr (sizeof(T) == 4)
+            spmm_seg_ring_f32(L, x, r.b, w, ld, rseg, cj, gate, j, j > always, s);
+        else
+            spmm_seg_ring_f64(L, x, r.b, w, ld, rseg, cj, gate, j, j > always, s);
     }
-    return y;
+    return r.b[0];
 }
 
 // Segmented fused Rayleigh-Ritz of the first w columns of y (ld): ly = L y,
 // G_c = y_c^T [y_c | ly_c], scaled generalized eig per segment, x/ax update and
-// wanted-column norms.  Enqueue only (no synchronisation).
+// wanted-column norms.  Enqueue only (no synchronisation); with a gate, every
+// launch returns at once when *gate == 0.
 template <typename T>
 void enqueue_rr(const DevLaplacian& L, T* y, T* ly, int w, int ld, int k, T* x, T* ax,
                 const WorkList& gram, const WorkList& tiles, double* g, double* theta,
                 double* cmat, int* status, double* norms2, double* gws, double* rws,
-                int64_t& spmm_calls, hipStream_t s) {
+                const int32_t* gate, hipStream_t s) {
     const int nseg = gram.w.nseg;
     if constexpr (sizeof(T) == 4) {
-        spmm_f32(L, y, ly, w, ld, 1.f, 0.f, 0.f, nullptr, s);
-        gram2_seg_f32(gram.w, y, ld, w, ly, ld, g, gws, s);
+        spmm_f32(L, y, ly, w, ld, 1.f, 0.f, 0.f, nullptr, s, gate, 1);
+        gram2_seg_f32(gram.w, y, ld, w, ly, ld, g, gws, s, gate);
     } else {
-        spmm_f64(L, y, ly, w, ld, 1.0, 0.0, 0.0, nullptr, s);
-        gram2_seg_f64(gram.w, y, ld, w, ly, ld, g, gws, s);
+        spmm_f64(L, y, ly, w, ld, 1.0, 0.0, 0.0, nullptr, s, gate, 1);
+        gram2_seg_f64(gram.w, y, ld, w, ly, ld, g, gws, s, gate);
     }
-    ++spmm_calls;
     const int64_t gs = (int64_t)w * 2 * w;
-    small_gen_eig_scaled_batched(nseg, w, g + w, 2 * w, gs, g, 2 * w, gs, theta, cmat, status, s);
+    small_gen_eig_scaled_batched(nseg, w, g + w, 2 * w, gs, g, 2 * w, gs, theta, cmat, status, s,
+                                 gate);
     if constexpr (sizeof(T) == 4)
-        rr_update_seg_f32(tiles.w, y, ly, ld, w, cmat, theta, k, x, ax, norms2, rws, s);
+        rr_update_seg_f32(tiles.w, y, ly, ld, w, cmat, theta, k, x, ax, norms2, rws, s, gate);
     else
-        rr_update_seg_f64(tiles.w, y, ly, ld, w, cmat, theta, k, x, ax, norms2, rws, s);
+        rr_update_seg_f64(tiles.w, y, ly, ld, w, cmat, theta, k, x, ax, norms2, rws, s, gate);
 }
+
+// Rounds captured per hipGraph: after the last useful round the gate closes
+// and the remaining captured rounds cost only their empty launches.
+constexpr int kRoundsPerGraph = 4;
 
 }  // namespace
 
@@ -173,9 +143,23 @@ void fast_solve_segments(const DevLaplacian& L, const std::vector<int64_t>& row0
         SI_CHECK(rown[c] > 64 && row0[c] >= 0 && row0[c] + rown[c] <= n, kErrInvalidArgument,
                  "fast_solve_segments: segments must lie in the operator and hold > 64 rows");
     SI_CHECK(k <= 8 && k + 5 <= m, kErrUnsupported, "fast_solve_segments: n_components above 7");
+    SI_CHECK(opt.max_outer >= 1, kErrInvalidArgument, "fast_solve_segments: max_outer >= 1");
     const int mf = std::min(m, std::max(8, k + 5));
     const int cap = opt.cheb_degree > 0 ? opt.cheb_degree : 24;
-    const double b = 2.0;
+    // every round captures `steps` filter launches; the planner's degrees never exceed cap
+    const int steps = std::max(cap, 2);
+    // the planner's degree on [lo, 2] (as plan_degree in k_batch.hip)
+    auto degree_at = [&](double lo) {
+        const double x0 = (lo + 2.0) / (2.0 - lo);
+        return std::max(2, std::min(steps, (int)std::floor(9.2 / std::acosh(x0))));
+    };
+    // the planner's smallest degree (its interval's lower end is at most 0.6):
+    // the filter steps every planned round runs
+    const int always = degree_at(0.6);
+    // the first FP32 round: every segment starts at lo = 0.1, so every step
+    // up to this degree runs
+    const int first_steps = degree_at(0.1);
+    const int max_extra64 = 20;
 
     // row -> segment map (rows outside every segment use the frozen row nseg)
     DevBuf<int32_t> rseg(n);
@@ -204,188 +188,178 @@ void fast_solve_segments(const DevLaplacian& L, const std::vector<int64_t>& row0
     double* theta = cmat + csz;
     double* norms2 = theta + (size_t)nseg * m;
     int* status = ws_get<int>(kWsStatus, nseg);
-    const int max_steps = cap;
-    float* coef32 = ws_get<float>(kWsBatchCoef, (size_t)max_steps * (nseg + 1) * 3);
+    // coefficient tables: FP32 rounds use the first steps*(nseg+1)*3 floats,
+    // FP64 rounds the whole buffer as doubles
+    void* coef_buf = ws_get<double>(kWsBatchCoef, (size_t)steps * (nseg + 1) * 3);
+    float* coef32 = static_cast<float*>(coef_buf);
+    double* coef64 = static_cast<double*>(coef_buf);
+    // planner state: a, best, worst [nseg] and thk [nseg x mf] doubles, then
+    // stall, done, deg [nseg] and ctl [4] ints
+    const size_t plan_doubles = (size_t)nseg * (3 + mf);
+    const size_t plan_ints = (size_t)nseg * 3 + 4;
+    double* pd = ws_get<double>(kWsBatchPlan, plan_doubles + (plan_ints + 1) / 2);
+    int32_t* pi = reinterpret_cast<int32_t*>(pd + plan_doubles);
+    BatchPlan P;
+    P.nseg = nseg;
+    P.k = k;
+    P.cap = cap;
+    P.steps = steps;
+    P.tol = opt.tol;
+    P.b = 2.0;
+    P.a = pd;
+    P.best = pd + nseg;
+    P.worst = pd + 2 * (size_t)nseg;
+    P.stall = pi;
+    P.done = pi + nseg;
+    P.deg = pi + 2 * (size_t)nseg;
+    P.ctl = pi + 3 * (size_t)nseg;
+    P.status = status;
+    P.theta = theta;
+    P.norms2 = norms2;
 
     // start block: N(0,1) (index = original node id) with D^{1/2} 1 in column 0
     fill_normal_f32(x32, n, m, m, seed, kStreamLobpcgInit, s, L.perm);
     set_column_f32(x32, m, n, 0, L.sqrt_deg, 1.0, s);
 
-    std::vector<double> a(nseg, 0.1), best(nseg, INFINITY), worst(nseg, INFINITY);
-    std::vector<int> stall(nseg, 0), done(nseg, 0), deg(nseg, 0);
-    std::vector<double> th((size_t)nseg * m), nrm((size_t)nseg * 2 * k);
-    std::vector<int> hstat(nseg);
-    std::vector<float> tab;
-    std::map<int, hipGraphExec_t> graphs;
     const bool graphs_ok = use_graphs && !spmm_profiling();
-    auto release_graphs = [&] {
-        for (auto& kv : graphs) hipGraphExecDestroy(kv.second);
-        graphs.clear();
-    };
-    // a round's host side: the coefficient table goes up through page-locked
-    // memory without a wait, and the readback below is the round's only
-    // synchronisation (SI_TRACE=1 reports the time spent waiting for the
-    // device and the time spent in host decisions)
+    int32_t* ctl_host = static_cast<int32_t*>(pinned_stage(4 * sizeof(int32_t)));
     using clk = std::chrono::steady_clock;
-    double wait_ms = 0.0, host_ms = 0.0;
-    clk::time_point th0 = clk::now();
-    auto upload_async = [&](void* dst, const void* src, size_t bytes) {
-        void* h = pinned_stage(bytes);
-        std::memcpy(h, src, bytes);
-        SI_HIP(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, s));
-    };
-    auto read_round = [&](int w) {
+    double wait_ms = 0.0, capture_ms = 0.0;
+    int host_reads = 0;
+    auto read_ctl = [&] {
         const clk::time_point t0 = clk::now();
-        host_ms += std::chrono::duration<double, std::milli>(t0 - th0).count();
-        SI_HIP(hipMemcpyAsync(hstat.data(), status, sizeof(int) * nseg, hipMemcpyDeviceToHost, s));
-        SI_HIP(hipMemcpyAsync(th.data(), theta, sizeof(double) * nseg * w, hipMemcpyDeviceToHost, s));
-        SI_HIP(hipMemcpyAsync(nrm.data(), norms2, sizeof(double) * nseg * 2 * k,
-                              hipMemcpyDeviceToHost, s));
+        SI_HIP(hipMemcpyAsync(ctl_host, P.ctl, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
         SI_HIP(hipStreamSynchronize(s));
-        th0 = clk::now();
-        wait_ms += std::chrono::duration<double, std::milli>(th0 - t0).count();
+        wait_ms += std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+        ++host_reads;
     };
-    auto seg_worst = [&](int c) {
-        double wv = 0.0;
-        for (int j = 0; j < k; ++j) {
-            double xn = std::sqrt(std::fmax(nrm[(size_t)c * 2 * k + k + j], 0.0));
-            if (xn < 2.220446049250313e-16) xn = 2.220446049250313e-16;
-            const double r = std::sqrt(std::fmax(nrm[(size_t)c * 2 * k + j], 0.0)) / xn;
-            wv = std::isnan(r) ? INFINITY : std::max(wv, r);
+    hipGraphExec_t exec = nullptr;
+    auto release = [&] {
+        if (exec) hipGraphExecDestroy(exec);
+        exec = nullptr;
+    };
+    // Captures `count` rounds (the first one by `first`) as one graph.
+    auto capture = [&](hipGraphExec_t& ex, const auto& first, const auto& round, int count) {
+        const clk::time_point tc0 = clk::now();
+        hipGraph_t gr = nullptr;
+        SI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            first();
+            for (int r = 1; r < count; ++r) round();
+        } catch (...) {
+            hipStreamEndCapture(s, &gr);
+            if (gr) hipGraphDestroy(gr);
+            throw;
         }
-        return wv;
+        SI_HIP(hipStreamEndCapture(s, &gr));
+        const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        hipGraphDestroy(gr);
+        SI_HIP(e);
+        ++st.graphs_captured;
+        capture_ms += std::chrono::duration<double, std::milli>(clk::now() - tc0).count();
+    };
+    // Replays rounds until the planner closes the gate: kRoundsPerGraph rounds
+    // per graph launch (the first launch starts with `first`, later ones with
+    // `round`), or one eagerly enqueued round per host read.  The planner
+    // closes the gate after max_rounds planned rounds, so the loop ends; the
+    // host bound is a second guard.
+    auto run_rounds = [&](const auto& first, const auto& round, int max_rounds) {
+        read_ctl();
+        if (ctl_host[0] == 0) return;
+        const int per = graphs_ok ? kRoundsPerGraph : 1;
+        const int max_launches = max_rounds / per + 2;
+        for (int launch = 0; launch < max_launches; ++launch) {
+            if (graphs_ok) {
+                if (launch == 0) {
+                    capture(exec, first, round, per);
+                } else if (launch == 1) {
+                    release();
+                    capture(exec, round, round, per);
+                }
+                SI_HIP(hipGraphLaunch(exec, s));
+                ++st.graph_launches;
+            } else if (launch == 0) {
+                first();
+            } else {
+                round();
+            }
+            read_ctl();
+            if (ctl_host[0] == 0) break;
+        }
+        release();
+        SI_CHECK(ctl_host[0] == 0, kErrInternal, "batched solve: the round planner did not stop");
     };
 
     try {
-        th0 = clk::now();
-        double capture_ms = 0.0;
         // ---------------- FP32 rounds
-        for (int round = 0; round < opt.max_outer; ++round) {
-            int steps = 0;
-            bool any = false;
-            for (int c = 0; c < nseg; ++c) {
-                deg[c] = done[c] ? 0 : capped_degree(a[c], b, cap);
-                steps = std::max(steps, deg[c]);
-                any = any || !done[c];
-            }
-            if (!any) break;
-            coef_table<float>(a, deg, b, steps, tab);
-            upload_async(coef32, tab.data(), sizeof(float) * tab.size());
-            auto body = [&] {
-                float* y = enqueue_filter<float>(L, r32, x32, m, m, steps, rseg, coef32, nseg,
-                                                 st.spmm_calls, s);
-                float* ly = (y == r32.b[0]) ? r32.b[1] : r32.b[0];
-                enqueue_rr<float>(L, y, ly, m, m, k, x32, ax32, gram, t32, g, theta, cmat, status,
-                                  norms2, gws, rws, st.spmm_calls, s);
-            };
-            if (graphs_ok) {
-                auto it = graphs.find(steps);
-                if (it == graphs.end()) {
-                    const clk::time_point tc0 = clk::now();
-                    hipGraph_t gr = nullptr;
-                    hipGraphExec_t ex = nullptr;
-                    const int64_t calls0 = st.spmm_calls;
-                    SI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                    try {
-                        body();
-                    } catch (...) {
-                        hipStreamEndCapture(s, &gr);
-                        if (gr) hipGraphDestroy(gr);
-                        throw;
-                    }
-                    SI_HIP(hipStreamEndCapture(s, &gr));
-                    SI_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
-                    hipGraphDestroy(gr);
-                    it = graphs.emplace(steps, ex).first;
-                    ++st.graphs_captured;
-                    st.spmm_calls = calls0;  // counted when launched
-                    capture_ms += std::chrono::duration<double, std::milli>(clk::now() - tc0).count();
-                }
-                SI_HIP(hipGraphLaunch(it->second, s));
-                ++st.graph_launches;
-                st.spmm_calls += steps + 1;
-            } else {
-                body();
-            }
-            read_round(m);
-            ++st.rounds;
-            for (int c = 0; c < nseg; ++c) {
-                if (done[c]) continue;
-                if (hstat[c] != 0) { done[c] = 2; continue; }  // Y^T Y not positive definite
-                const double wv = seg_worst(c);
-                worst[c] = wv;
-                if (wv <= opt.tol) { done[c] = 1; continue; }
-                if (wv < 0.7 * best[c]) { best[c] = wv; stall[c] = 0; }
-                else if (++stall[c] >= 3 && wv < 1e-4) { done[c] = 1; continue; }  // FP32 floor
-                const double* tc = th.data() + (size_t)c * m;
-                a[c] = std::min(std::max(tc[m - 1], tc[k - 1] * 1.5 + 1e-12), 0.6);
-            }
+        P.m = m;
+        P.thk = nullptr;
+        P.max_rounds = opt.max_outer;
+        batch_plan_init(P, s);
+        {
+            BatchPlan p1 = P;  // the first round runs first_steps steps
+            p1.steps = first_steps;
+            batch_plan_f32(p1, 0, coef32, s);
         }
-        release_graphs();
+        auto round32 = [&](int nsteps, int full) {
+            float* y = enqueue_filter<float>(L, r32, x32, m, m, nsteps, full, rseg, coef32, nseg,
+                                             P.ctl, s);
+            float* ly = r32.b[1];
+            enqueue_rr<float>(L, y, ly, m, m, k, x32, ax32, gram, t32, g, theta, cmat, status,
+                              norms2, gws, rws, P.ctl, s);
+            batch_plan_f32(P, 1, coef32, s);  // plans the next round over `steps` steps
+        };
+        run_rounds([&] { round32(first_steps, first_steps); }, [&] { round32(steps, always); },
+                   P.max_rounds);
+        st.rounds += ctl_host[1];
+        st.spmm_calls += ctl_host[2];
 
         // ---------------- FP64 finish on the lowest mf Ritz vectors
         double* x64 = ws_get<double>(kWsFastX64, nm);
         double* ax64 = ws_get<double>(kWsFastAx64, nm);
         Ring<double> r64{{ws_get<double>(kWsFastF64a, nm), ws_get<double>(kWsFastF64b, nm),
                           ws_get<double>(kWsFastF64c, nm)}};
-        double* coef64 = ws_get<double>(kWsBatchCoef, (size_t)max_steps * (nseg + 1) * 3 * 2);
         convert_f32_to_f64(x32, r64.b[0], (int64_t)nm, s);
         enqueue_rr<double>(L, r64.b[0], r64.b[1], mf, m, k, x64, ax64, gram, t64, g, theta, cmat,
-                           status, norms2, gws, rws, st.spmm_calls, s);
-        read_round(mf);
-        std::vector<int> fail(nseg, 0);
-        for (int c = 0; c < nseg; ++c) {
-            fail[c] = hstat[c] != 0;
-            worst[c] = fail[c] ? INFINITY : seg_worst(c);
-            done[c] = fail[c] || worst[c] <= opt.tol;
-        }
-        std::vector<double> a64(nseg, 0.1), thk((size_t)nseg * mf);
-        std::copy(th.begin(), th.begin() + (size_t)nseg * mf, thk.begin());
-        std::vector<double> tab64;
-        for (int extra = 0; extra < 20; ++extra) {
-            int steps = 0;
-            for (int c = 0; c < nseg; ++c) {
-                if (!done[c]) {
-                    const double* tc = thk.data() + (size_t)c * mf;
-                    a64[c] = std::min(std::max(tc[mf - 1], tc[k - 1] * 1.5 + 1e-12), 0.6);
-                }
-                deg[c] = done[c] ? 0 : capped_degree(a64[c], b, cap);
-                steps = std::max(steps, deg[c]);
-            }
-            if (steps == 0) break;
-            coef_table<double>(a64, deg, b, steps, tab64);
-            upload_async(coef64, tab64.data(), sizeof(double) * tab64.size());
-            double* y = enqueue_filter<double>(L, r64, x64, mf, m, steps, rseg, coef64, nseg,
-                                               st.spmm_calls, s);
-            double* ly = (y == r64.b[0]) ? r64.b[1] : r64.b[0];
-            // the update rewrites x64/ax64 of every segment; converged segments
-            // were copied through the filter, so they are re-projected unchanged
+                           status, norms2, gws, rws, nullptr, s);
+        st.spmm_calls += 1;
+        P.m = mf;
+        P.thk = pd + 3 * (size_t)nseg;
+        P.max_rounds = max_extra64;
+        batch_plan_init(P, s);
+        // the update rewrites x64/ax64 of every segment; converged segments
+        // were copied through the filter, so they are re-projected unchanged
+        batch_plan_f64(P, 2, coef64, s);
+        auto round64 = [&] {
+            double* y = enqueue_filter<double>(L, r64, x64, mf, m, steps, always, rseg, coef64,
+                                               nseg, P.ctl, s);
+            double* ly = r64.b[1];
             enqueue_rr<double>(L, y, ly, mf, m, k, x64, ax64, gram, t64, g, theta, cmat, status,
-                               norms2, gws, rws, st.spmm_calls, s);
-            read_round(mf);
-            ++st.rounds;
-            for (int c = 0; c < nseg; ++c) {
-                if (fail[c]) continue;
-                if (hstat[c] != 0) { fail[c] = 1; done[c] = 1; worst[c] = INFINITY; continue; }
-                std::copy(th.begin() + (size_t)c * mf, th.begin() + (size_t)(c + 1) * mf,
-                          thk.begin() + (size_t)c * mf);
-                worst[c] = seg_worst(c);
-                if (worst[c] <= opt.tol) done[c] = 1;
-            }
-        }
+                               norms2, gws, rws, P.ctl, s);
+            batch_plan_f64(P, 1, coef64, s);
+        };
+        run_rounds(round64, round64, P.max_rounds);
+        st.rounds += ctl_host[1];
+        st.spmm_calls += ctl_host[2];
         copy_cols_f64(x64, m, evecs_dev, k, n, k, s);
-        SI_HIP(hipStreamSynchronize(s));
+
+        std::vector<double> thk((size_t)nseg * mf), worst(nseg);
+        std::vector<int32_t> done(nseg);
+        to_host(thk.data(), P.thk, thk.size(), s);
+        to_host(worst.data(), P.worst, worst.size(), s);
+        to_host(done.data(), P.done, done.size(), s);  // synchronises the stream
         if (trace_on())
-            std::fprintf(stderr, "[batch] %d segments, %lld rounds: %.2f ms waiting for the device, "
-                         "%.2f ms on the host between rounds (%.2f ms of it graph capture)\n",
-                         nseg, (long long)st.rounds, wait_ms, host_ms, capture_ms);
+            std::fprintf(stderr, "[batch] %d segments, %d rounds in %d graph launches (%d host "
+                         "reads): %.2f ms waiting for the device, %.2f ms graph capture\n",
+                         nseg, st.rounds, st.graph_launches, host_reads, wait_ms, capture_ms);
         for (int c = 0; c < nseg; ++c) {
             for (int j = 0; j < k; ++j) evals_host[(size_t)c * k + j] = thk[(size_t)c * mf + j];
-            resid[c] = worst[c];
-            ok[c] = !fail[c] && worst[c] <= opt.tol;
+            const bool fail = done[c] == 2;
+            resid[c] = fail ? INFINITY : worst[c];
+            ok[c] = !fail && worst[c] <= opt.tol;
         }
     } catch (...) {
-        release_graphs();
+        release();
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive) {
             hipGraph_t gr = nullptr;
