@@ -191,6 +191,32 @@ def bench_warm_start(args, torch, si, ctx, lap, cfg, fast_ms, plain):
     return out
 
 
+def locality_report(si, lap, workload):
+    """Where the columns of the stored operator's entries lie relative to their
+    rows: the share of entries with |col - row| below a distance, the share
+    inside a row block of R rows widened by H rows each side (the x rows one
+    workgroup could stage in LDS: 64 bytes per 16-column FP32 row), and the
+    share inside the row's own eighth of the rows (one XCD's range)."""
+    rp, col, _, _, order = si.laplacian_storage(lap)
+    n = rp.size - 1
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp.astype(np.int64)))
+    c = col.astype(np.int64)
+    d = np.abs(c - rows)
+    out = {"workload": workload, "n": n, "stored_entries": int(col.size),
+           "order": {si.ORDER_INPUT: "input", si.ORDER_RCM: "rcm",
+                     si.ORDER_CLUSTER: "cluster"}.get(order, str(order)),
+           "within_distance": {}, "within_row_block_window": {}}
+    for w in (256, 1024, 2048, 4096, 16384, 65536, 131072):
+        out["within_distance"][str(w)] = round(float(np.mean(d < w)), 4)
+    for r, h in ((256, 896), (1024, 512), (1024, 1024), (4096, 0)):
+        b0 = (rows // r) * r
+        key = f"block {r} rows, halo {h} ({(r + 2 * h) * 64 // 1024} KB of x)"
+        out["within_row_block_window"][key] = round(float(np.mean((c >= b0 - h) & (c < b0 + r + h))), 4)
+    per = n // 8
+    out["within_xcd_range"] = round(float(np.mean((rows // per) == (c // per))), 4)
+    return out
+
+
 def bench_components(args, torch, graphs, si):
     """64 disconnected components (sizes log-uniform in [50, 200 000], each an
     independent 30-d GMM 15-NN graph): spectral_init with all components in one
@@ -328,6 +354,9 @@ def main():
                     help="skip timing the device fuzzy k-NN graph builder on the workload's points")
     ap.add_argument("--no-warm-start", action="store_true",
                     help="skip the learned warm start leg (network block fed to both solvers)")
+    ap.add_argument("--locality", action="store_true",
+                    help="print how close each stored entry's column lies to its row in the "
+                         "operator's stored node order (the reach of an LDS-staged x block) and exit")
     ap.add_argument("--spmm-shape16", type=int, default=-1,
                     help="pin the FP32 16-column SpMM shape (0-3) instead of autotuning; "
                          "counter runs pin the shape the autotuner picks unprofiled")
@@ -382,6 +411,10 @@ def main():
     log(f"laplacian on device: nnz={lap.nnz} hub rows={lap.n_long} ({lap_ms:.1f} ms incl. upload)")
     if args.spmm_shape16 >= 0:
         log(f"FP32 16-column SpMM shape pinned to {si.spmm_shapes(lap, args.spmm_shape16, 0)[0]}")
+
+    if args.locality:
+        print(json.dumps(locality_report(si, lap, args.workload)), flush=True)
+        return
 
     sweep = []
     if args.spmm_sweep or args.spmm_only:
