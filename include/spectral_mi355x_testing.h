@@ -49,6 +49,22 @@ si_status si_test_band_solve(si_context* ctx, const si_laplacian* lap, double sh
  * the last batched solve. */
 si_status si_test_batch_graphs(int32_t enable, int64_t* captured, int64_t* launches);
 
+/* One call of the batched solver's device round planner (k_batch.hip) on
+ * given round results, all arrays on the host.  mode 0: the FP32 rule, 1: the
+ * FP64 rule; update 0 plans from the state only, 1 reads the round the gate
+ * ctl[0] planned (skipped when it is 0), 2 always reads it.  In / out: the
+ * segment state a, best, worst [nseg] (doubles), stall, done [nseg], thk
+ * [nseg x m] (mode 1 only; may be NULL in mode 0), ctl [4] (gate, planned
+ * rounds, planned SpMM launches, iterating segments).  In: status [nseg],
+ * theta [nseg x m], norms2 [nseg x 2k].  Out: deg [nseg] and the recurrence
+ * scalars coef [steps x (nseg + 1) x 3], widened to double in mode 0. */
+si_status si_test_batch_plan(si_context* ctx, int32_t mode, int32_t update, int32_t nseg,
+                             int32_t k, int32_t m, int32_t cap, int32_t steps,
+                             int32_t max_rounds, double tol, double* a, double* best,
+                             double* worst, int32_t* stall, int32_t* done, double* thk,
+                             const int32_t* status, const double* theta, const double* norms2,
+                             int32_t* ctl, int32_t* deg, double* coef);
+
 /* FP32 SpMM shape for 16 / 32 columns (0 .. 3, see k_spmm.hip): autotune != 0
  * times every candidate on this graph and keeps the fastest, otherwise the
  * given shapes are set; out16 / out32 (may be NULL) receive the shapes used. */

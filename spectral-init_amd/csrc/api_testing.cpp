@@ -4,6 +4,7 @@
 // single device kernels against the CPU oracle.
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "si_abi.hpp"
 #include "si_graph.hpp"
@@ -203,6 +204,66 @@ si_status si_profile_gather_probe(si_context* ctx, const si_laplacian* lap, int3
                                   int32_t reps, double* us) {
     if (!ctx || !lap || !us) return SI_ERR_INVALID_ARGUMENT;
     return tguard(ctx, [&] { *us = si::gather_probe_us(lap->L, mode, reps, ctx->c.stream); });
+}
+
+si_status si_test_batch_plan(si_context* ctx, int32_t mode, int32_t update, int32_t nseg,
+                             int32_t k, int32_t m, int32_t cap, int32_t steps,
+                             int32_t max_rounds, double tol, double* a, double* best,
+                             double* worst, int32_t* stall, int32_t* done, double* thk,
+                             const int32_t* status, const double* theta, const double* norms2,
+                             int32_t* ctl, int32_t* deg, double* coef) {
+    if (!ctx || !a || !best || !worst || !stall || !done || !status || !theta || !norms2 || !ctl ||
+        !deg || !coef || (mode == 1 && !thk))
+        return SI_ERR_INVALID_ARGUMENT;
+    if ((mode != 0 && mode != 1) || update < 0 || update > 2 || nseg < 1 || k < 1 || m < k ||
+        m > 64 || steps < 2 || cap < 1)
+        return SI_ERR_INVALID_ARGUMENT;
+    return tguard(ctx, [&] {
+        hipStream_t s = ctx->c.stream;
+        const size_t ns = (size_t)nseg;
+        si::DevBuf<double> dd(ns * 3 + ns * m + ns * m + ns * 2 * k);
+        si::DevBuf<int32_t> di(ns * 4 + 4);
+        const size_t ncoef = (size_t)steps * (ns + 1) * 3;
+        si::DevBuf<double> dcoef(ncoef);
+        si::BatchPlan p;
+        p.nseg = nseg; p.k = k; p.m = m; p.cap = cap; p.steps = steps;
+        p.max_rounds = max_rounds; p.tol = tol; p.b = 2.0;
+        p.a = dd.get(); p.best = p.a + ns; p.worst = p.best + ns;
+        p.thk = mode == 1 ? p.worst + ns : nullptr;
+        double* th = p.worst + ns + ns * m;
+        double* nr = th + ns * m;
+        p.stall = di.get(); p.done = p.stall + ns; p.deg = p.done + ns; p.ctl = p.deg + ns;
+        int32_t* st = p.ctl + 4;
+        p.status = st; p.theta = th; p.norms2 = nr;
+        si::to_device(p.a, a, ns, s);
+        si::to_device(p.best, best, ns, s);
+        si::to_device(p.worst, worst, ns, s);
+        if (p.thk) si::to_device(p.thk, thk, ns * m, s);
+        si::to_device(th, theta, ns * m, s);
+        si::to_device(nr, norms2, ns * 2 * k, s);
+        si::to_device(p.stall, stall, ns, s);
+        si::to_device(p.done, done, ns, s);
+        si::to_device(p.ctl, ctl, 4, s);
+        si::to_device(st, status, ns, s);
+        if (mode == 0) {
+            float* c32 = reinterpret_cast<float*>(dcoef.get());
+            si::batch_plan_f32(p, update, c32, s);
+            std::vector<float> h(ncoef);
+            si::to_host(h.data(), c32, ncoef, s);
+            for (size_t i = 0; i < ncoef; ++i) coef[i] = h[i];
+        } else {
+            si::batch_plan_f64(p, update, dcoef.get(), s);
+            si::to_host(coef, dcoef.get(), ncoef, s);
+        }
+        si::to_host(a, p.a, ns, s);
+        si::to_host(best, p.best, ns, s);
+        si::to_host(worst, p.worst, ns, s);
+        if (p.thk) si::to_host(thk, p.thk, ns * m, s);
+        si::to_host(stall, p.stall, ns, s);
+        si::to_host(done, p.done, ns, s);
+        si::to_host(deg, p.deg, ns, s);
+        si::to_host(ctl, p.ctl, 4, s);
+    });
 }
 
 si_status si_test_batch_graphs(int32_t enable, int64_t* captured, int64_t* launches) {

@@ -42,7 +42,8 @@ EXPORTED = [
     "si_dense_evd",
     "si_spectral_init",
     "si_test_gram", "si_test_block_mul", "si_test_small_sym_eig", "si_test_small_gen_eig",
-    "si_test_fill_normal", "si_test_fast_solve", "si_test_batch_graphs", "si_test_spmm_shapes",
+    "si_test_fill_normal", "si_test_fast_solve", "si_test_batch_graphs", "si_test_batch_plan",
+    "si_test_spmm_shapes",
     "si_test_band_solve", "si_test_rcm_order", "si_test_knn_path",
     "si_test_laplacian_storage", "si_test_components_device", "si_test_rcm_order_device",
     "si_test_output_stage",
@@ -604,6 +605,41 @@ def batch_graphs(enable: Optional[bool] = None):
     lib().si_test_batch_graphs(-1 if enable is None else int(bool(enable)), C.byref(cap),
                                C.byref(lau))
     return cap.value, lau.value
+
+
+def batch_plan(ctx: Context, mode: int, update: int, state: dict, status, theta, norms2, k: int,
+               m: int, cap: int, steps: int, max_rounds: int, tol: float):
+    """Testing seam: one call of the batched solver's device round planner.
+    state holds a, best, worst (float64 [nseg]), stall, done (int32 [nseg]),
+    ctl (int32 [4]) and, for mode 1, thk (float64 [nseg, m]); it is updated in
+    place.  Returns (deg [nseg], coef [steps, nseg + 1, 3])."""
+    a = np.ascontiguousarray(state["a"], np.float64)
+    nseg = a.size
+    arrs = {"a": a}
+    for key, dt in (("best", np.float64), ("worst", np.float64), ("stall", np.int32),
+                    ("done", np.int32), ("ctl", np.int32)):
+        arrs[key] = np.ascontiguousarray(state[key], dt).copy()
+    arrs["a"] = a.copy()
+    thk = None
+    if mode == 1:
+        thk = np.ascontiguousarray(state["thk"], np.float64).copy()
+    status = np.ascontiguousarray(status, np.int32)
+    theta = np.ascontiguousarray(theta, np.float64)
+    norms2 = np.ascontiguousarray(norms2, np.float64)
+    assert theta.size == nseg * m and norms2.size == nseg * 2 * k and status.size == nseg
+    deg = np.zeros(nseg, np.int32)
+    coef = np.zeros((steps, nseg + 1, 3))
+    P = lambda x: C.c_void_p(x.ctypes.data) if x is not None else C.c_void_p(0)
+    ctx._check(lib().si_test_batch_plan(
+        ctx.handle, C.c_int32(mode), C.c_int32(update), C.c_int32(nseg), C.c_int32(k),
+        C.c_int32(m), C.c_int32(cap), C.c_int32(steps), C.c_int32(max_rounds), C.c_double(tol),
+        P(arrs["a"]), P(arrs["best"]), P(arrs["worst"]), P(arrs["stall"]), P(arrs["done"]),
+        P(thk), P(status), P(theta), P(norms2), P(arrs["ctl"]), P(deg), P(coef)))
+    for key in arrs:
+        state[key] = arrs[key]
+    if thk is not None:
+        state["thk"] = thk
+    return deg, coef
 
 
 def spmm_shapes(lap: "Laplacian", shape16: int = 0, shape32: int = 0, autotune: bool = False):
