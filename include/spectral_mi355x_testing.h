@@ -43,6 +43,12 @@ si_status si_test_fast_solve(si_context* ctx, const si_laplacian* lap, int32_t n
 si_status si_test_band_solve(si_context* ctx, const si_laplacian* lap, double shift,
                              const double* x, double* y, int32_t k, int32_t* bandwidth);
 
+/* Shift-and-invert level: enable != 0 factorises M = L + shift I as a dense
+ * matrix (rocSOLVER Cholesky) whatever its band, so the dense fallback used for
+ * graphs with no narrow band can be compared with the banded factor; 0 restores
+ * the automatic choice (band first, dense when the band is too large). */
+si_status si_test_sinv_dense(int32_t enable);
+
 /* Batched component solves (si_spectral_init with batch_components): enable
  * >= 0 switches hipGraph capture of the FP32 rounds on (1) or off (0), -1
  * leaves it; captured / launches (may be NULL) receive the graph counts of

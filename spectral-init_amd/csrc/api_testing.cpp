@@ -266,6 +266,11 @@ si_status si_test_batch_plan(si_context* ctx, int32_t mode, int32_t update, int3
     });
 }
 
+si_status si_test_sinv_dense(int32_t enable) {
+    si::sinv_force_dense(enable != 0);
+    return SI_OK;
+}
+
 si_status si_test_batch_graphs(int32_t enable, int64_t* captured, int64_t* launches) {
     if (captured) *captured = si::g_last_batch.graphs_captured;
     if (launches) *launches = si::g_last_batch.graph_launches;

@@ -18,13 +18,47 @@
 // Solves: one workgroup per right-hand side sweeps the panels; the part of a
 // row outside its panel is a wave-reduced dot product, the triangle inside
 // the panel is solved by one wavefront with lane r holding row r.
+//
+// Dense fallback: when the band exceeds kBandMaxEntries / kBandMaxFlops (graphs
+// with no narrow ordering, e.g. k-NN graphs) and n <= kSinvDenseMax, M is
+// assembled as a full n x n matrix and factorised by rocSOLVER DPOTRF (a plain
+// library Cholesky on the FP64 matrix cores).  A row-major n x k block Y is the
+// column-major k x n matrix Y^T, so Y = M^-1 X is Y^T = X^T L^-T L^-1: two
+// right-side rocBLAS DTRSM on the block in place, no transposes.
+#include <rocsolver/rocsolver.h>
+
 #include "si_host.hpp"
+#include "si_rocblas.hpp"
 
 namespace si {
 
 namespace {
 
 constexpr int kP = 64;  // panel width
+bool g_force_dense = false;
+
+// lower triangle of M = L + shift I, column-major n x n (zeroed beforehand)
+__global__ void k_dense_fill(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                             const double* __restrict__ l64, int n, double shift,
+                             double* __restrict__ A) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    A[(int64_t)i * n + i] = 1.0 + shift;
+    for (int32_t p = rp[i]; p < rp[i + 1]; ++p) {
+        const int j = col[p];
+        if (j < i) A[(int64_t)j * n + i] = l64[p];
+    }
+}
+
+// y = scale * x for an n x k row-major block
+__global__ void k_dense_copy(const double* __restrict__ x, int ldx, double* __restrict__ y,
+                             int ldy, int64_t n, int k, double scale) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * k) return;
+    const int64_t i = e / k;
+    const int c = (int)(e % k);
+    y[i * ldy + c] = scale * x[i * ldx + c];
+}
 
 __device__ inline double wave_sum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -253,14 +287,46 @@ int band_width(const DevLaplacian& L, hipStream_t s) {
     return h;
 }
 
+namespace {
+
+bool dense_factor(const DevLaplacian& L, double shift, BandFactor& F, hipStream_t s) {
+    const int64_t n = L.n;
+    if (n < 1 || n > kSinvDenseMax) return false;
+    const size_t bytes = sizeof(double) * (size_t)n * (size_t)n;
+    size_t free_b = 0, total_b = 0;
+    SI_HIP(hipMemGetInfo(&free_b, &total_b));
+    if ((double)bytes > kSinvDenseMemFrac * (double)free_b) return false;
+    F.n = (int)n;
+    F.b = (int)n - 1;
+    F.dense = true;
+    F.B.ensure((size_t)n * (size_t)n);
+    SI_HIP(hipMemsetAsync(F.B.get(), 0, bytes, s));
+    hipLaunchKernelGGL(k_dense_fill, dim3(ceil_div(n, (int64_t)256)), dim3(256), 0, s, L.rowptr,
+                       L.col, L.l64, (int)n, shift, F.B.get());
+    SI_LAUNCH_CHECK();
+    DevBuf<rocblas_int> info(1);
+    rb_check(rocsolver_dpotrf(rb_handle(s), rocblas_fill_lower, (rocblas_int)n, F.B.get(),
+                              (rocblas_int)n, info.get()),
+             "rocsolver_dpotrf");
+    rocblas_int h = 0;
+    to_host(&h, info.get(), 1, s);
+    return h == 0;
+}
+
+}  // namespace
+
+void sinv_force_dense(bool on) { g_force_dense = on; }
+
 bool band_factor(const DevLaplacian& L, double shift, BandFactor& F, hipStream_t s) {
     const int64_t n64 = L.n;
     if (n64 < 1 || n64 > INT32_MAX) return false;
     const int n = (int)n64;
+    F.dense = false;
+    if (g_force_dense) return dense_factor(L, shift, F, s);
     const int b = band_width(L, s);
     const double entries = (double)n * (double)(b + 1);
     const double flops = (double)n * (double)b * (double)b;
-    if (entries > kBandMaxEntries || flops > kBandMaxFlops) return false;
+    if (entries > kBandMaxEntries || flops > kBandMaxFlops) return dense_factor(L, shift, F, s);
     F.n = n;
     F.b = b;
     F.B.ensure((size_t)n * (size_t)(b + 1));
@@ -294,6 +360,23 @@ void band_solve(const BandFactor& F, const double* x, int ldx, double* y, int ld
                 double scale, hipStream_t s) {
     SI_CHECK(scale == 1.0 || scale == -1.0, kErrInvalidArgument, "band_solve: scale must be +-1");
     SI_CHECK(k >= 1, kErrInvalidArgument, "band_solve: k < 1");
+    if (F.dense) {
+        SI_CHECK(ldy >= k && x != y, kErrInvalidArgument, "band_solve: bad dense solve operands");
+        const int64_t nk = (int64_t)F.n * k;
+        hipLaunchKernelGGL(k_dense_copy, dim3(ceil_div(nk, (int64_t)256)), dim3(256), 0, s, x,
+                           ldx, y, ldy, (int64_t)F.n, k, scale);
+        SI_LAUNCH_CHECK();
+        const double one = 1.0;
+        rocblas_handle h = rb_handle(s);
+        // Y^T <- Y^T L^-T, then Y^T <- Y^T L^-1 (Y^T: column-major k x n, ld ldy)
+        rb_check(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                               rocblas_diagonal_non_unit, k, F.n, &one, F.B.get(), F.n, y, ldy),
+                 "rocblas_dtrsm (L^T)");
+        rb_check(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_none,
+                               rocblas_diagonal_non_unit, k, F.n, &one, F.B.get(), F.n, y, ldy),
+                 "rocblas_dtrsm (L)");
+        return;
+    }
     hipLaunchKernelGGL(k_band_fwd, dim3(k), dim3(256), 0, s, (const double*)F.B.get(), F.n, F.b,
                        x, ldx, y, ldy);
     hipLaunchKernelGGL(k_band_bwd, dim3(k), dim3(256), 0, s, (const double*)F.B.get(), F.n, F.b,

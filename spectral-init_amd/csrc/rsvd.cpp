@@ -27,26 +27,11 @@
 #include <cmath>
 
 #include "si_host.hpp"
+#include "si_rocblas.hpp"
 
 namespace si {
 
 namespace {
-
-void rb_check(rocblas_status st, const char* what) {
-    if (st != rocblas_status_success)
-        throw Error(kErrHip, std::string(what) + ": rocBLAS status " + std::to_string((int)st));
-}
-
-// one rocBLAS handle per host thread (a context is used by one thread), kept
-// until the process ends (destroying it in a thread-exit destructor could run
-// after the HIP runtime has shut down)
-thread_local rocblas_handle g_rb = nullptr;
-
-rocblas_handle rb_handle(hipStream_t s) {
-    if (!g_rb) rb_check(rocblas_create_handle(&g_rb), "rocblas_create_handle");
-    rb_check(rocblas_set_stream(g_rb, s), "rocblas_set_stream");
-    return g_rb;
-}
 
 __global__ void k_svqb_scale_cm(int k, const double* __restrict__ w, double rel, double* u) {
     // U[:, j] *= w_j^-1/2 (0 below rel * w_max); U column-major k x k

@@ -269,14 +269,24 @@ bool lobpcg_core(int64_t n, const BlockOperator& op, double* x, int m, double to
 // kBandMaxFlops for the factorisation; larger bands are not factorised.
 constexpr double kBandMaxEntries = 134217728.0;  // 1 GiB of FP64
 constexpr double kBandMaxFlops = 4e12;
+// When the band exceeds those limits, M is factorised as a dense matrix
+// instead (rocSOLVER DPOTRF on the lower triangle, column-major n x n) if
+// n <= kSinvDenseMax and n^2 doubles take at most kSinvDenseMemFrac of the free
+// device memory; the solves are two rocBLAS DTRSM on the row-major block.
+constexpr int64_t kSinvDenseMax = 65536;
+constexpr double kSinvDenseMemFrac = 0.6;
 struct BandFactor {
     int n = 0;
-    int b = 0;  // bandwidth: M[i][j] == 0 for i - j > b
+    int b = 0;           // bandwidth: M[i][j] == 0 for i - j > b (n - 1 when dense)
+    bool dense = false;  // B holds the dense Cholesky factor (lower, column-major)
     DevBuf<double> B;
 };
 int band_width(const DevLaplacian& L, hipStream_t s);
-// false when the band is too large or M is not positive definite
+// false when neither a band nor a dense factor is available (too large) or M
+// is not positive definite
 bool band_factor(const DevLaplacian& L, double shift, BandFactor& F, hipStream_t s);
+// testing: factorise densely whatever the band (the dense fallback's seam)
+void sinv_force_dense(bool on);
 // y = scale * M^-1 x (scale +1 or -1) for k columns
 void band_solve(const BandFactor& F, const double* x, int ldx, double* y, int ldy, int k,
                 double scale, hipStream_t s);

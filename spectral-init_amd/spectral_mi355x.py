@@ -43,6 +43,7 @@ EXPORTED = [
     "si_spectral_init",
     "si_test_gram", "si_test_block_mul", "si_test_small_sym_eig", "si_test_small_gen_eig",
     "si_test_fill_normal", "si_test_fast_solve", "si_test_batch_graphs", "si_test_batch_plan",
+    "si_test_sinv_dense",
     "si_test_spmm_shapes",
     "si_test_band_solve", "si_test_rcm_order", "si_test_knn_path",
     "si_test_laplacian_storage", "si_test_components_device", "si_test_rcm_order_device",
@@ -605,6 +606,12 @@ def batch_graphs(enable: Optional[bool] = None):
     lib().si_test_batch_graphs(-1 if enable is None else int(bool(enable)), C.byref(cap),
                                C.byref(lau))
     return cap.value, lau.value
+
+
+def sinv_dense(enable: bool) -> None:
+    """Testing seam: factorise the shift-and-invert operator densely whatever
+    its band (True), or choose automatically (False)."""
+    lib().si_test_sinv_dense(C.c_int32(1 if enable else 0))
 
 
 def batch_plan(ctx: Context, mode: int, update: int, state: dict, status, theta, norms2, k: int,

@@ -93,3 +93,30 @@ def max_subspace_angle(a, b):
 def residuals_dense(lmat, evals, evecs):
     r = lmat @ evecs - evecs * evals[None, :]
     return np.linalg.norm(r, axis=0) / np.maximum(np.linalg.norm(evecs, axis=0), 1e-300)
+
+
+def chained_random_graphs(nb: int, blocks: int, seed: int, links: int = 20, link_w: float = 0.5):
+    """`blocks` random graphs of nb nodes (random_symmetric_graph, average degree
+    8) joined in a chain by `links` edges of weight link_w between neighbouring
+    blocks: no narrow band under any ordering (each block is an expander), and
+    blocks - 1 small eigenvalues well below the rest of the spectrum."""
+    import scipy.sparse as sp
+    mats = []
+    for b in range(blocks):
+        ip, ix, w = random_symmetric_graph(nb, 8, seed + b)
+        mats.append(sp.csr_matrix((w, ix, ip), shape=(nb, nb)))
+    a = sp.block_diag(mats).tocoo()
+    rng = np.random.default_rng(seed)
+    r, c, v = [a.row], [a.col], [a.data]
+    for b in range(blocks - 1):
+        i = b * nb + rng.integers(0, nb, links)
+        j = (b + 1) * nb + rng.integers(0, nb, links)
+        r += [i, j]
+        c += [j, i]
+        v += [np.full(links, link_w, np.float32)] * 2
+    n = nb * blocks
+    m = sp.coo_matrix((np.concatenate(v), (np.concatenate(r), np.concatenate(c))), shape=(n, n))
+    m = m.tocsr()
+    m.sum_duplicates()
+    m.sort_indices()
+    return m.indptr.astype(np.int64), m.indices.astype(np.int32), m.data.astype(np.float32)

@@ -8,14 +8,17 @@ there and here.  Ring C_2500 must give non-zero eigenvectors
 (test_solver_invariants.rs:255-270); rings and paths are also checked against
 their closed-form spectra, including one stored in reverse Cuthill-McKee order
 (n >= 4096, bandwidth 2).  The chain test forces level 1 to fail (one filter
-round) and checks that level 2 produces the result.
+round) and checks that level 2 produces the result.  Graphs with no narrow band
+(a random graph) take the dense Cholesky fallback up to 65536 nodes, checked
+against scipy's direct solve and the oracle's LOBPCG.
 """
 import numpy as np
 import pytest
 
 import oracle as O
 from conftest import load_fixture
-from helpers import path_graph, random_symmetric_graph, ring_eigenvalues, ring_graph
+from helpers import (chained_random_graphs, path_graph, random_symmetric_graph, ring_eigenvalues,
+                     ring_graph)
 
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
@@ -83,10 +86,60 @@ def test_sinv_locality_ordered_band(ctx, kind, n):
 def test_sinv_returns_none_when_band_too_large(ctx):
     # a random graph keeps a wide band under any ordering: sinv.rs returns None
     # when its factorisation is unavailable, here when the band exceeds the cap
-    ip, ix, w = random_symmetric_graph(60000, 8, seed=11)
+    # and n is above the dense fallback's limit (65536)
+    ip, ix, w = random_symmetric_graph(70000, 8, seed=11)
     lap = si.Laplacian(ctx, ip, ix, w)
     with pytest.raises(si.ConvergenceFailure):
         si.lobpcg_sinv_solve(lap, 2, 42)
+
+
+@pytest.mark.parametrize("kind", ["ring", "fixture"])
+def test_dense_factor_solve_matches_scipy(ctx, kind):
+    # the dense fallback (rocSOLVER Cholesky, two right-side TRSMs on the
+    # row-major block), forced on graphs that also have a narrow band
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    if kind == "fixture":
+        z = load_fixture("blobs_connected_2000")
+        ip, ix, w = z["w_indptr"], z["w_indices"], z["w_data"]
+    else:
+        ip, ix, w = ring_graph(2500)
+    lap = si.Laplacian(ctx, ip, ix, w)
+    L, _, _ = O.laplacian_from_graph(ip, ix, w)
+    nn = lap.n
+    m = sp.csr_matrix((L[2], L[1], L[0]), shape=(nn, nn)) + 1e-4 * sp.eye(nn)
+    x = torch.from_numpy(np.random.default_rng(nn + 1).standard_normal((nn, 3))).cuda()
+    si.sinv_dense(True)
+    try:
+        y, bw = si.band_solve(lap, x, 1e-4)
+    finally:
+        si.sinv_dense(False)
+    assert bw == nn - 1  # the dense factor is the full lower triangle
+    ref = sla.spsolve(m.tocsc(), x.cpu().numpy())
+    err = np.abs(y.cpu().numpy() - ref).max() / np.abs(ref).max()
+    assert err < 1e-10, err
+    y_band, _ = si.band_solve(lap, x, 1e-4)  # the banded factor (the whole triangle below 4096 nodes)
+    assert np.abs(y.cpu().numpy() - y_band.cpu().numpy()).max() / np.abs(ref).max() < 1e-10
+
+
+def test_sinv_dense_fallback_on_wide_band_graph(ctx):
+    # three random graphs of 12000 nodes in a weak chain: the band exceeds the
+    # banded limits, so level 2 factorises densely (10.4 GB) instead of
+    # returning None; eigenpairs against the oracle's LOBPCG and the
+    # reference's 1e-8 residual bar
+    ip, ix, w = chained_random_graphs(12000, 3, seed=5)
+    lap = si.Laplacian(ctx, ip, ix, w)
+    x = torch.from_numpy(np.random.default_rng(9).standard_normal((lap.n, 2))).cuda()
+    _, bw = si.band_solve(lap, x, 1e-4)
+    assert bw == lap.n - 1, bw  # the automatic choice took the dense factor
+    print(f"sinv dense fallback n={lap.n}", flush=True)
+    ev, vec, rep = si.lobpcg_sinv_solve(lap, 2, 42)
+    assert rep.level == 2
+    L, _, sq = O.laplacian_from_graph(ip, ix, w)
+    ref = O.lobpcg_solve(L, 2, 42, sq, tol=1e-10)
+    assert ref is not None and ref["converged"]
+    np.testing.assert_allclose(ev, ref["evals"], rtol=1e-6, atol=1e-12)
+    assert _residual(ip, ix, w, ev, vec) < 1e-8
 
 
 def test_sinv_deterministic(ctx):
