@@ -32,6 +32,11 @@ si_status si_test_small_gen_eig(si_context* ctx, int32_t n, const double* a_host
 /* Philox4x32-10 / Box-Muller normals into a device block [n x k]. */
 si_status si_test_fill_normal(si_context* ctx, double* x, int64_t n, int32_t k, uint64_t seed,
                               uint32_t stream);
+/* the FP32 start-block stream of the FP32-bulk solvers into a device float
+ * block [n x k]: row i, column pair p holds pair i * ceil(k/2) + p (cosine in
+ * column 2p, sine in column 2p + 1). */
+si_status si_test_fill_normal_f32(si_context* ctx, float* x, int64_t n, int32_t k, uint64_t seed,
+                                  uint32_t stream);
 /* the FP32-bulk Chebyshev-filtered block solver alone (escalation level 1). */
 si_status si_test_fast_solve(si_context* ctx, const si_laplacian* lap, int32_t n_components,
                              uint64_t seed, const si_config* cfg, double* evals, double* evecs,

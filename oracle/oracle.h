@@ -44,6 +44,8 @@ int or_get_threads(void);
 void or_philox4x32_10(const uint32_t ctr[4], uint32_t key0, uint32_t key1, uint32_t out[4]);
 double or_normal(uint64_t seed, uint32_t stream, uint64_t index);
 void or_fill_normal(uint64_t seed, uint32_t stream, size_t count, double* out);
+/* pairs p = 0 .. npairs-1 of the FP32 stream: out[2p] (cosine), out[2p+1] (sine) */
+void or_normal_pairs_f32(uint64_t seed, uint32_t stream, size_t npairs, float* out);
 
 /* ---- laplacian.rs ---- */
 /* mode 0 = PythonCompat (f32 column-sum scatter, laplacian.rs:8-16,37-40),

@@ -46,6 +46,7 @@ def lib():
         L.or_normal.restype = C.c_double
         L.or_normal.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64]
         L.or_fill_normal.argtypes = [C.c_uint64, C.c_uint32, C.c_size_t, _f64p]
+        L.or_normal_pairs_f32.argtypes = [C.c_uint64, C.c_uint32, C.c_size_t, _f32p]
         L.or_compute_degrees.argtypes = [C.c_int64, _i64p, _i32p, _f32p, C.c_int, _f64p, _f64p]
         L.or_laplacian_nnz.restype = C.c_int64
         L.or_laplacian_nnz.argtypes = [C.c_int64, _i64p, _i32p]
@@ -149,6 +150,14 @@ def normal_block(seed: int, stream: int, count: int) -> np.ndarray:
     out = np.empty(count, np.float64)
     lib().or_fill_normal(seed, stream, count, out)
     return out
+
+
+def normal_pairs_f32(seed: int, stream: int, npairs: int) -> np.ndarray:
+    """Pairs 0 .. npairs-1 of the FP32 start-block stream, float32 [npairs, 2]
+    (cosine and sine output of each pair's Box-Muller transform)."""
+    out = np.empty(2 * npairs, np.float32)
+    lib().or_normal_pairs_f32(seed, stream, npairs, out)
+    return out.reshape(npairs, 2)
 
 
 def compute_degrees(indptr, indices, data, mode: int = 0):

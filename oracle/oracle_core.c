@@ -64,6 +64,23 @@ void or_fill_normal(uint64_t seed, uint32_t stream, size_t count, double* out) {
     for (size_t i = 0; i < count; ++i) out[i] = or_normal(seed, stream, (uint64_t)i);
 }
 
+/* The FP32 start-block stream of the device's FP32-bulk solvers: pair p of
+ * the counter space gives the cosine and sine outputs of one Box-Muller
+ * transform in float, from 24-bit uniforms of the first two Philox words. */
+void or_normal_pairs_f32(uint64_t seed, uint32_t stream, size_t npairs, float* out) {
+    const float inv24 = 1.0f / 16777216.0f; /* 2^-24 */
+    for (size_t p = 0; p < npairs; ++p) {
+        uint32_t c[4] = {(uint32_t)p, (uint32_t)((uint64_t)p >> 32), stream, 0x9a1fu};
+        philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+        const float u1 = ((float)(c[0] >> 8) + 1.0f) * inv24; /* (0, 1] */
+        const float u2 = (float)(c[1] >> 8) * inv24;          /* [0, 1) */
+        const float r = sqrtf(-2.0f * logf(u1));
+        const float t = 6.2831853071795865f * u2;
+        out[2 * p] = r * cosf(t);
+        out[2 * p + 1] = r * sinf(t);
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* compute_degrees (src/laplacian.rs:29-49).                                  */
 void or_compute_degrees(int64_t n, const int64_t* indptr, const int32_t* indices,

@@ -105,6 +105,15 @@ si_status si_test_fill_normal(si_context* ctx, double* x, int64_t n, int32_t k, 
     });
 }
 
+si_status si_test_fill_normal_f32(si_context* ctx, float* x, int64_t n, int32_t k, uint64_t seed,
+                                  uint32_t stream) {
+    if (!ctx || !x || n < 0 || k < 1) return SI_ERR_INVALID_ARGUMENT;
+    return tguard(ctx, [&] {
+        si::fill_normal_f32(x, n, k, k, seed, stream, ctx->c.stream);
+        SI_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
 si_status si_test_fast_solve(si_context* ctx, const si_laplacian* lap, int32_t n_components,
                              uint64_t seed, const si_config* cfg, double* evals, double* evecs,
                              si_report* rep) {

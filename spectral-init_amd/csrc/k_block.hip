@@ -702,6 +702,23 @@ __global__ void k_gather_rows(const T* __restrict__ src, int lds, T* __restrict_
     dst[i * ldd + c] = src[r * lds + c];
 }
 
+// FP32 start blocks: thread per (row, column pair); row i draws the pairs of
+// original node perm[i] (counter orig * ceil(k/2) + pair).
+__global__ void k_fill_normal_pairs(float* __restrict__ x, int64_t n, int k, int ld, uint64_t seed,
+                                    uint32_t stream, const int32_t* __restrict__ perm) {
+    const int kp = (k + 1) / 2;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * kp) return;
+    const int64_t i = e / kp;
+    const int p = (int)(e % kp);
+    const int64_t orig = perm ? (int64_t)perm[i] : i;
+    float z0, z1;
+    normal_pair_f32(seed, stream, (uint64_t)(orig * kp + p), z0, z1);
+    float* row = x + i * ld + 2 * p;
+    row[0] = z0;
+    if (2 * p + 1 < k) row[1] = z1;
+}
+
 template <typename A, typename B>
 __global__ void k_convert(const A* __restrict__ a, B* __restrict__ b, int64_t count) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -841,9 +858,16 @@ void fill_normal_f64(double* x, int64_t n, int k, int ld, uint64_t seed, uint32_
                        stream, perm);
     SI_LAUNCH_CHECK();
 }
+void fill_normal_rounded_f32(float* x, int64_t n, int k, int ld, uint64_t seed, uint32_t stream,
+                             hipStream_t s, const int32_t* perm) {
+    hipLaunchKernelGGL((k_fill_normal<float>), grid1(n * k), dim3(256), 0, s, x, n, k, ld, seed,
+                       stream, perm);
+    SI_LAUNCH_CHECK();
+}
 void fill_normal_f32(float* x, int64_t n, int k, int ld, uint64_t seed, uint32_t stream,
                      hipStream_t s, const int32_t* perm) {
-    hipLaunchKernelGGL((k_fill_normal<float>), grid1(n * k), dim3(256), 0, s, x, n, k, ld, seed,
+    const int64_t pairs = n * ((k + 1) / 2);
+    hipLaunchKernelGGL(k_fill_normal_pairs, grid1(pairs), dim3(256), 0, s, x, n, k, ld, seed,
                        stream, perm);
     SI_LAUNCH_CHECK();
 }

@@ -188,7 +188,7 @@ void cluster_order_device(const DevLaplacian& L, int32_t* perm, int32_t* iperm, 
         block_mul_f32(n, x, kE, kE, c, kE, nullptr, 0, 0, nullptr, y, kE, s);
         std::swap(x, y);
     };
-    fill_normal_f32(x, n, kE, kE, kOrderSeed, kStreamOrder, s);
+    fill_normal_rounded_f32(x, n, kE, kE, kOrderSeed, kStreamOrder, s);
     orthonormalise();
     for (int step = 1; step <= kSteps; ++step) {
         spmm_f32(L, x, y, kE, kE, -1.0f, 1.0f, 0.0f, nullptr, s);  // y = (I - L) x

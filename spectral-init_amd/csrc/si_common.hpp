@@ -74,6 +74,23 @@ __host__ __device__ inline double normal_at(uint64_t seed, uint32_t stream, uint
     return sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925286766559 * u2);
 }
 
+// The FP32 start blocks' stream: one Philox4x32-10 call per (row, column
+// pair) gives two normals, the cosine and sine outputs of one Box-Muller
+// transform in FP32 (24-bit uniforms): half the Philox calls of normal_at and
+// no FP64 transcendentals.  (counter = pair index; z0 -> column 2p, z1 -> 2p+1.)
+__host__ __device__ inline void normal_pair_f32(uint64_t seed, uint32_t stream, uint64_t index,
+                                                float& z0, float& z1) {
+    uint32_t c[4] = {(uint32_t)index, (uint32_t)(index >> 32), stream, 0x9a1fu};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const float inv24 = 1.0f / 16777216.0f;
+    const float u1 = ((float)(c[0] >> 8) + 1.0f) * inv24;  // (0, 1]
+    const float u2 = (float)(c[1] >> 8) * inv24;           // [0, 1)
+    const float r = sqrtf(-2.0f * logf(u1));
+    const float t = 6.2831853071795865f * u2;
+    z0 = r * cosf(t);
+    z1 = r * sinf(t);
+}
+
 constexpr uint32_t kStreamLobpcgInit = 1u;
 constexpr uint32_t kStreamRsvdOmega = 2u;
 constexpr uint32_t kStreamNoise = 7u;

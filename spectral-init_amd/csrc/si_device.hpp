@@ -292,9 +292,13 @@ void residual_f64(int64_t n, const double* ax, const double* x, int ld, int k,
 
 void fill_normal_f64(double* x, int64_t n, int k, int ld, uint64_t seed, uint32_t stream,
                      hipStream_t s, const int32_t* perm = nullptr);
-// the same FP64 draws, rounded to FP32
+// FP32 start blocks of the FP32-bulk solvers: the paired FP32 stream
+// (normal_pair_f32: one Philox call per row and column pair).
 void fill_normal_f32(float* x, int64_t n, int k, int ld, uint64_t seed, uint32_t stream,
                      hipStream_t s, const int32_t* perm = nullptr);
+// FP32 block holding normal_at's FP64 draws rounded (the FP64 stream's values).
+void fill_normal_rounded_f32(float* x, int64_t n, int k, int ld, uint64_t seed, uint32_t stream,
+                             hipStream_t s, const int32_t* perm = nullptr);
 // out = alpha a + beta b on columns [0,w) (all three with leading dimension ld)
 void axpby_cols_f32(int64_t n, int w, int ld, float alpha, const float* a, float beta,
                     const float* b, float* out, hipStream_t s);

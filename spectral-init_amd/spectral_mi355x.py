@@ -42,7 +42,7 @@ EXPORTED = [
     "si_dense_evd",
     "si_spectral_init",
     "si_test_gram", "si_test_block_mul", "si_test_small_sym_eig", "si_test_small_gen_eig",
-    "si_test_fill_normal", "si_test_fast_solve", "si_test_batch_graphs", "si_test_batch_plan",
+    "si_test_fill_normal", "si_test_fill_normal_f32", "si_test_fast_solve", "si_test_batch_graphs", "si_test_batch_plan",
     "si_test_sinv_dense",
     "si_test_spmm_shapes",
     "si_test_band_solve", "si_test_rcm_order", "si_test_knn_path",
@@ -213,6 +213,13 @@ def lib():
         L.si_test_small_sym_eig.argtypes = [vp, i32, vp, vp, vp]
         L.si_test_small_gen_eig.argtypes = [vp, i32, vp, vp, vp, vp]
         L.si_test_fill_normal.argtypes = [vp, vp, i64, i32, u64, C.c_uint32]
+        L.si_test_fill_normal_f32.argtypes = [vp, vp, i64, i32, u64, C.c_uint32]
+        L.si_test_batch_plan.argtypes = [vp] + [i32] * 8 + [dbl] + [vp] * 12
+        L.si_test_sinv_dense.argtypes = [i32]
+        L.si_test_batch_graphs.argtypes = [i32, C.POINTER(i64), C.POINTER(i64)]
+        L.si_test_spmm_shapes.argtypes = [vp, vp, i32, i32, i32, C.POINTER(i32), C.POINTER(i32)]
+        L.si_version.argtypes = []
+        L.si_mpnn_param_count.argtypes = []
         L.si_test_fast_solve.argtypes = [vp, vp, i32, u64, C.POINTER(_Config), vp, vp,
                                          C.POINTER(_Report)]
         L.si_test_band_solve.argtypes = [vp, vp, dbl, vp, vp, i32, C.POINTER(i32)]
@@ -566,6 +573,12 @@ def small_gen_eig(ctx: Context, a: np.ndarray, b: np.ndarray):
 
 def fill_normal(ctx: Context, x, seed: int, stream: int):
     ctx._check(lib().si_test_fill_normal(ctx.handle, _ptr(x), x.shape[0], x.shape[1], seed, stream))
+
+
+def fill_normal_f32(ctx: Context, x, seed: int, stream: int):
+    """Testing seam: the FP32 start-block stream into a device float32 [n, k] block."""
+    ctx._check(lib().si_test_fill_normal_f32(ctx.handle, _ptr(x), x.shape[0], x.shape[1], seed,
+                                             stream))
     return x
 
 

@@ -40,6 +40,14 @@ def test_library_exports_every_declared_symbol(library):
     assert not missing, missing
 
 
+def test_every_binding_declares_its_argument_types():
+    # without argtypes ctypes passes a Python int as a 32-bit C int, which
+    # truncates a device address handed over as an integer
+    lib = si.lib()
+    undeclared = [n for n in si.EXPORTED if getattr(lib, n).argtypes is None]
+    assert not undeclared, undeclared
+
+
 def test_version_and_default_config_without_gpu(library):
     lib = si.lib()
     assert b"gfx950" in lib.si_version()

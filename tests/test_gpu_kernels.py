@@ -272,6 +272,18 @@ def test_fill_normal_matches_oracle_rng(ctx):
     np.testing.assert_allclose(x.cpu().numpy(), ref, rtol=1e-13, atol=1e-13)
 
 
+@pytest.mark.parametrize("k", [16, 3])
+def test_fill_normal_f32_pairs_match_oracle(ctx, k):
+    # the FP32 start blocks: row i, column pair p holds stream pair
+    # i * ceil(k/2) + p (an odd k drops the last sine); device logf/cosf/sinf
+    # against the host's, to a few float32 ulps
+    n, kp = 1000, (k + 1) // 2
+    x = torch.empty((n, k), dtype=torch.float32, device="cuda")
+    si.fill_normal_f32(ctx, x, 42, 1)
+    pairs = O.normal_pairs_f32(42, 1, n * kp).reshape(n, 2 * kp)[:, :k]
+    np.testing.assert_allclose(x.cpu().numpy(), pairs, rtol=4e-6, atol=4e-6)
+
+
 # ------------------------------------------------------------------ locality order
 @pytest.mark.parametrize("name", ["blobs_connected_2000", "blobs_5000", "disconnected_200"])
 def test_reordered_laplacian_exports_reference_layout(ctx, name):

@@ -207,3 +207,33 @@ def test_normals_follow_the_standard_normal_law():
     assert scipy.stats.kstest(y, "norm").pvalue > 1e-3
     assert abs(np.corrcoef(x, y)[0, 1]) < 0.01
     assert np.isfinite(x).all() and np.abs(x).max() < 7.0
+
+
+def test_fp32_pair_stream_from_the_pinned_philox():
+    # the FP32 start-block stream: pair p takes the first two Philox words of
+    # counter (p, 0, stream, 0x9a1f) under the seed key (the Philox itself is
+    # pinned by the known answers above) as 24-bit uniforms u1 in (0, 1] and
+    # u2 in [0, 1); its two normals have squared length -2 ln u1 and angle
+    # 2 pi u2 (the Box-Muller identities), each to float32 rounding
+    seed, stream, npairs = 42, 1, 64
+    z = O.normal_pairs_f32(seed, stream, npairs).astype(np.float64)
+    for p in range(npairs):
+        c = O.philox4x32_10((p, 0, stream, 0x9A1F), seed & 0xFFFFFFFF, seed >> 32)
+        u1 = ((int(c[0]) >> 8) + 1) / 2.0 ** 24
+        u2 = (int(c[1]) >> 8) / 2.0 ** 24
+        r2 = z[p, 0] ** 2 + z[p, 1] ** 2
+        assert abs(r2 - (-2.0 * np.log(u1))) <= 1e-5 * max(r2, 1.0), (p, r2, u1)
+        ang = np.arctan2(z[p, 1], z[p, 0]) % (2 * np.pi)
+        d = abs(ang - 2 * np.pi * u2)
+        assert min(d, 2 * np.pi - d) < 1e-5 * max(1.0, 1.0 / np.sqrt(r2)), (p, ang, u2)
+
+
+def test_fp32_pair_stream_follows_the_standard_normal_law():
+    z = O.normal_pairs_f32(42, 1, 100000)
+    assert z.dtype == np.float32
+    for col in (z[:, 0], z[:, 1]):
+        assert scipy.stats.kstest(col.astype(np.float64), "norm").pvalue > 1e-3
+    # the two outputs of a pair are uncorrelated, and so are two streams
+    assert abs(np.corrcoef(z[:, 0], z[:, 1])[0, 1]) < 0.01
+    w = O.normal_pairs_f32(42, 2, 100000)
+    assert abs(np.corrcoef(z[:, 0], w[:, 0])[0, 1]) < 0.01
