@@ -347,28 +347,52 @@ void solve_components_batched(si::Context& cx, const si::DevLaplacian& L,
         }
         phase("segment ranges");
         si::DevBuf<double> block((size_t)n * k);
+        // rows of components too small to solve (< 2d nodes) are not written
+        // by either solver
+        SI_HIP(hipMemsetAsync(block.get(), 0, sizeof(double) * (size_t)n * k, s));
         std::vector<double> dev_h(dcomp.size() * k), iev(icomp.size() * k), resid;
         std::vector<int> ok;
         si::BatchStats bst;
         {
             si::EventTimer t(s);
-            // the iterative solve writes every row of the block, so it goes first
-            if (!icomp.empty()) {
-                si::FastOptions fo = opt.fast;
-                fo.tol = opt.tol;
-                si::fast_solve_segments(L, irow0, in, d, seed, fo, si::g_batch_graphs, iev.data(),
-                                        block, resid, ok, bst, s);
-                si::g_last_batch = bst;
+            // The dense EVD of the small components runs on the auxiliary
+            // stream, overlapping the iterative solve (each writes only the
+            // rows of its own components); the main stream waits for it.
+            si::DevBuf<int64_t> r0(std::max<size_t>(drow0.size(), 1));
+            si::DevBuf<int32_t> nn(std::max<size_t>(dn.size(), 1));
+            si::DevBuf<double> ev(std::max<size_t>(dcomp.size() * k, 1));
+            hipEvent_t ready = nullptr, dense_done = nullptr;
+            try {
+                if (!dcomp.empty()) {
+                    hipStream_t a = si::aux_stream(cx);
+                    SI_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+                    SI_HIP(hipEventCreateWithFlags(&dense_done, hipEventDisableTiming));
+                    SI_HIP(hipEventRecord(ready, s));  // the operator and the block exist
+                    SI_HIP(hipStreamWaitEvent(a, ready, 0));
+                    si::to_device(r0.get(), drow0.data(), drow0.size(), a);
+                    si::to_device(nn.get(), dn.data(), dn.size(), a);
+                    si::dense_seg_eig(L, (int)dcomp.size(), r0, nn, k, ev, block, k, a);
+                    SI_HIP(hipEventRecord(dense_done, a));
+                }
+                if (!icomp.empty()) {
+                    si::FastOptions fo = opt.fast;
+                    fo.tol = opt.tol;
+                    si::fast_solve_segments(L, irow0, in, d, seed, fo, si::g_batch_graphs,
+                                            iev.data(), block, resid, ok, bst, s);
+                    si::g_last_batch = bst;
+                }
+                if (dense_done) {
+                    SI_HIP(hipStreamWaitEvent(s, dense_done, 0));
+                    si::to_host(dev_h.data(), ev.get(), dev_h.size(), s);
+                }
+            } catch (...) {
+                if (dense_done) hipStreamSynchronize(si::aux_stream(cx));
+                if (ready) hipEventDestroy(ready);
+                if (dense_done) hipEventDestroy(dense_done);
+                throw;
             }
-            if (!dcomp.empty()) {
-                si::DevBuf<int64_t> r0(drow0.size());
-                si::DevBuf<int32_t> nn(dn.size());
-                si::DevBuf<double> ev(dcomp.size() * k);
-                si::to_device(r0.get(), drow0.data(), drow0.size(), s);
-                si::to_device(nn.get(), dn.data(), dn.size(), s);
-                si::dense_seg_eig(L, (int)dcomp.size(), r0, nn, k, ev, block, k, s);
-                si::to_host(dev_h.data(), ev.get(), dev_h.size(), s);
-            }
+            if (ready) hipEventDestroy(ready);
+            if (dense_done) hipEventDestroy(dense_done);
             *solve_ms += t.stop_ms();
         }
         phase("device solve");
@@ -449,8 +473,12 @@ si_status si_context_create(int device, si_context** out) {
 
 void si_context_destroy(si_context* ctx) {
     if (!ctx) return;
+    if (ctx->c.stream || ctx->c.aux) hipSetDevice(ctx->c.device);
+    if (ctx->c.aux) {
+        hipStreamSynchronize(ctx->c.aux);
+        hipStreamDestroy(ctx->c.aux);
+    }
     if (ctx->c.stream) {
-        hipSetDevice(ctx->c.device);
         hipStreamSynchronize(ctx->c.stream);
         hipStreamDestroy(ctx->c.stream);
     }

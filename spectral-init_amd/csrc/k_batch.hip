@@ -167,7 +167,53 @@ __global__ void k_batch_init(BatchPlan p) {
     if (c < 4) p.ctl[c] = 0;
 }
 
+// thread per row: the last segment starting at or before the row (binary
+// search over the sorted starts) holds it if the row is before its end
+__global__ void k_segment_map(int32_t* __restrict__ rseg, int64_t n, int nseg,
+                              const int64_t* __restrict__ starts, const int64_t* __restrict__ ends,
+                              const int32_t* __restrict__ ids) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int lo = 0, hi = nseg;  // first segment with starts > i is at hi
+    while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (starts[mid] <= i) lo = mid + 1;
+        else hi = mid;
+    }
+    const int j = lo - 1;
+    rseg[i] = (j >= 0 && i < ends[j]) ? ids[j] : nseg;
+}
+
+// workgroup per work item: its rows [blk_rows[2b], blk_rows[2b+1]), k columns
+__global__ void k_copy_item_rows(const int64_t* __restrict__ blk_rows,
+                                 const double* __restrict__ src, int lds, double* __restrict__ dst,
+                                 int ldd, int k) {
+    const int64_t r0 = blk_rows[2 * blockIdx.x], r1 = blk_rows[2 * blockIdx.x + 1];
+    const int64_t cnt = (r1 - r0) * k;
+    for (int64_t e = threadIdx.x; e < cnt; e += blockDim.x) {
+        const int64_t r = r0 + e / k;
+        const int c = (int)(e % k);
+        dst[r * ldd + c] = src[r * lds + c];
+    }
+}
+
 }  // namespace
+
+void batch_copy_rows(const SegWork& items, const double* src, int lds, double* dst, int ldd,
+                     int k, hipStream_t s) {
+    if (items.nblk <= 0) return;
+    hipLaunchKernelGGL(k_copy_item_rows, dim3((unsigned)items.nblk), dim3(256), 0, s,
+                       items.blk_rows, src, lds, dst, ldd, k);
+    SI_LAUNCH_CHECK();
+}
+
+void batch_segment_map(int32_t* rseg, int64_t n, int nseg, const int64_t* starts,
+                       const int64_t* ends, const int32_t* ids, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_segment_map, dim3((unsigned)ceil_div(n, (int64_t)256)), dim3(256), 0, s,
+                       rseg, n, nseg, starts, ends, ids);
+    SI_LAUNCH_CHECK();
+}
 
 void batch_plan_init(const BatchPlan& p, hipStream_t s) {
     SI_CHECK(p.nseg >= 1, kErrInvalidArgument, "batch plan: no segments");

@@ -246,6 +246,14 @@ struct BatchPlan {
     const double* norms2 = nullptr;
     int32_t* ctl = nullptr;   // [4]
 };
+// rseg[i] = the segment holding row i, nseg for rows of no segment; the
+// segments' row ranges [starts[j], ends[j]) sorted by start, ids[j] their
+// segment numbers (device arrays)
+void batch_segment_map(int32_t* rseg, int64_t n, int nseg, const int64_t* starts,
+                       const int64_t* ends, const int32_t* ids, hipStream_t s);
+// dst[r, :k] = src[r, :k] for the rows of every work item (a segment's rows)
+void batch_copy_rows(const SegWork& items, const double* src, int lds, double* dst, int ldd,
+                     int k, hipStream_t s);
 // a = 0.1, best = worst = inf, stall = done = deg = 0, thk = 0, ctl = 0
 void batch_plan_init(const BatchPlan& p, hipStream_t s);
 // update = 0: plan from the current state only (the first round); 1: read the

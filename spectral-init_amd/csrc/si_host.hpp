@@ -51,7 +51,8 @@ enum WsSlot : int {
     kWsSmall0, kWsSmall1, kWsSmall2, kWsSmall3, kWsSmall4, kWsSmall5, kWsSmall6, kWsSmall7,
     kWsPlainS, kWsPlainAS, kWsPlainR, kWsPlainP, kWsPlainAP, kWsPlainPn, kWsPlainAPn,
     kWsPlainX, kWsPlainT, kWsPlainXr, kWsPlainBest, kWsStatus,
-    kWsBatchSmall, kWsBatchCoef, kWsBatchRr, kWsBatchPlan, kWsDenseA, kWsDenseAux, kWsDensePiv,
+    kWsBatchSmall, kWsBatchCoef, kWsBatchRr, kWsBatchPlan, kWsBatchWork, kWsBatchMap,
+    kWsDenseA, kWsDenseAux, kWsDensePiv,
     kWsNumSlots
 };
 void* workspace_get(int slot, size_t bytes);
@@ -85,11 +86,17 @@ struct TunedShapes {
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
+    // a second stream (default priority) for small independent work that
+    // overlaps the main stream's, joined back through events; made on first use
+    hipStream_t aux = nullptr;
     std::string last_error;
     int num_cus = 0;
     SpmmProfile profile;
     std::map<TuneKey, TunedShapes> tune_cache;
 };
+
+// The context's auxiliary stream (created on first use).
+hipStream_t aux_stream(Context& cx);
 
 // Stream-ordered timer (hipEvents on the context's stream).
 struct EventTimer {
@@ -217,7 +224,7 @@ bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const Fa
 // filter step / Gram / Rayleigh-Ritz update serves every segment; each segment
 // has its own filter interval and degree.  evals_host [nseg x k]; evecs_dev
 // [L.n x k] receives each segment's vectors in its rows (rows outside every
-// segment are overwritten with unspecified values); resid [nseg] the FP64
+// segment are left untouched); resid [nseg] the FP64
 // residual of each segment's pairs; ok[c] = 0 if segment c did not reach tol.
 struct BatchStats {
     int rounds = 0;

@@ -36,37 +36,29 @@ namespace si {
 namespace {
 
 struct WorkList {
-    DevBuf<int64_t> rows;
-    DevBuf<int32_t> seg;
-    DevBuf<int32_t> segblk;
     SegWork w;
 };
 
-// Split every segment into items of at most `chunk` rows.
-void build_worklist(const std::vector<int64_t>& row0, const std::vector<int64_t>& rown,
-                    int64_t chunk, WorkList& wl, hipStream_t s) {
-    std::vector<int64_t> hr;
-    std::vector<int32_t> hs, hb(row0.size() + 1, 0);
+// Work items of at most `chunk` rows per segment, built on the host: rows
+// (begin, end) pairs, the segment of each item, each segment's first item.
+struct HostWork {
+    std::vector<int64_t> rows;
+    std::vector<int32_t> seg, segblk;
+};
+HostWork work_items(const std::vector<int64_t>& row0, const std::vector<int64_t>& rown,
+                    int64_t chunk) {
+    HostWork h;
+    h.segblk.assign(row0.size() + 1, 0);
     for (size_t c = 0; c < row0.size(); ++c) {
-        hb[c] = (int32_t)hs.size();
+        h.segblk[c] = (int32_t)h.seg.size();
         for (int64_t r = 0; r < rown[c]; r += chunk) {
-            hr.push_back(row0[c] + r);
-            hr.push_back(row0[c] + std::min(rown[c], r + chunk));
-            hs.push_back((int32_t)c);
+            h.rows.push_back(row0[c] + r);
+            h.rows.push_back(row0[c] + std::min(rown[c], r + chunk));
+            h.seg.push_back((int32_t)c);
         }
     }
-    hb[row0.size()] = (int32_t)hs.size();
-    wl.rows.alloc(hr.size());
-    wl.seg.alloc(hs.size());
-    wl.segblk.alloc(hb.size());
-    to_device(wl.rows.get(), hr.data(), hr.size(), s);
-    to_device(wl.seg.get(), hs.data(), hs.size(), s);
-    to_device(wl.segblk.get(), hb.data(), hb.size(), s);
-    wl.w.nseg = (int)row0.size();
-    wl.w.nblk = (int64_t)hs.size();
-    wl.w.blk_rows = wl.rows.get();
-    wl.w.blk_seg = wl.seg.get();
-    wl.w.seg_blk = wl.segblk.get();
+    h.segblk[row0.size()] = (int32_t)h.seg.size();
+    return h;
 }
 
 template <typename T>
@@ -161,18 +153,65 @@ void fast_solve_segments(const DevLaplacian& L, const std::vector<int64_t>& row0
     const int first_steps = degree_at(0.1);
     const int max_extra64 = 20;
 
-    // row -> segment map (rows outside every segment use the frozen row nseg)
-    DevBuf<int32_t> rseg(n);
-    {
-        std::vector<int32_t> h(n, nseg);
-        for (int c = 0; c < nseg; ++c)
-            for (int64_t r = 0; r < rown[c]; ++r) h[row0[c] + r] = c;
-        to_device(rseg.get(), h.data(), h.size(), s);
+    // The work lists (Gram items of 1024 rows, FP32 / FP64 update tiles of 64 /
+    // 32 rows) and the segments sorted by first row go up in one copy from
+    // page-locked memory, into workspace (no allocation, no wait); the
+    // row -> segment map is then built on the device.
+    const HostWork hw[3] = {work_items(row0, rown, 1024), work_items(row0, rown, 64),
+                            work_items(row0, rown, 32)};
+    std::vector<int32_t> by_start(nseg);
+    for (int c = 0; c < nseg; ++c) by_start[c] = c;
+    std::sort(by_start.begin(), by_start.end(),
+              [&](int32_t u, int32_t v) { return row0[u] < row0[v]; });
+    size_t n64 = 2 * (size_t)nseg, n32 = (size_t)nseg;
+    for (const HostWork& h : hw) {
+        n64 += h.rows.size();
+        n32 += h.seg.size() + h.segblk.size();
     }
-    WorkList gram, t32, t64;
-    build_worklist(row0, rown, 1024, gram, s);
-    build_worklist(row0, rown, 64, t32, s);   // FP32 update tiles (64 rows)
-    build_worklist(row0, rown, 32, t64, s);   // FP64 update tiles (32 rows)
+    const size_t payload = 8 * n64 + 4 * n32;
+    constexpr size_t kCtlBytes = 64;  // ctl readback area at the front of the stage
+    char* stage = static_cast<char*>(pinned_stage(kCtlBytes + payload));
+    char* dwork = ws_get<char>(kWsBatchWork, payload);
+    WorkList wl[3];
+    {
+        int64_t* h64 = reinterpret_cast<int64_t*>(stage + kCtlBytes);
+        int32_t* h32 = reinterpret_cast<int32_t*>(h64 + n64);
+        const int64_t* d64 = reinterpret_cast<const int64_t*>(dwork);
+        const int32_t* d32 = reinterpret_cast<const int32_t*>(d64 + n64);
+        size_t o64 = 0, o32 = 0;
+        auto put64 = [&](const int64_t* src, size_t cnt) {
+            std::memcpy(h64 + o64, src, 8 * cnt);
+            o64 += cnt;
+            return d64 + o64 - cnt;
+        };
+        auto put32 = [&](const int32_t* src, size_t cnt) {
+            std::memcpy(h32 + o32, src, 4 * cnt);
+            o32 += cnt;
+            return d32 + o32 - cnt;
+        };
+        for (int i = 0; i < 3; ++i) {
+            wl[i].w.nseg = nseg;
+            wl[i].w.nblk = (int64_t)hw[i].seg.size();
+            wl[i].w.blk_rows = put64(hw[i].rows.data(), hw[i].rows.size());
+            wl[i].w.blk_seg = put32(hw[i].seg.data(), hw[i].seg.size());
+            wl[i].w.seg_blk = put32(hw[i].segblk.data(), hw[i].segblk.size());
+        }
+        std::vector<int64_t> starts(nseg), ends(nseg);
+        for (int c = 0; c < nseg; ++c) {
+            starts[c] = row0[by_start[c]];
+            ends[c] = row0[by_start[c]] + rown[by_start[c]];
+        }
+        const int64_t* d_starts = put64(starts.data(), nseg);
+        const int64_t* d_ends = put64(ends.data(), nseg);
+        const int32_t* d_ids = put32(by_start.data(), nseg);
+        SI_HIP(hipMemcpyAsync(dwork, stage + kCtlBytes, payload, hipMemcpyHostToDevice, s));
+        batch_segment_map(ws_get<int32_t>(kWsBatchMap, (size_t)n), n, nseg, d_starts, d_ends,
+                          d_ids, s);
+    }
+    const int32_t* rseg = ws_get<int32_t>(kWsBatchMap, (size_t)n);
+    const WorkList& gram = wl[0];
+    const WorkList& t32 = wl[1];
+    const WorkList& t64 = wl[2];
 
     const size_t nm = (size_t)n * m;
     float* x32 = ws_get<float>(kWsFastX32, nm);
@@ -222,7 +261,9 @@ void fast_solve_segments(const DevLaplacian& L, const std::vector<int64_t>& row0
     set_column_f32(x32, m, n, 0, L.sqrt_deg, 1.0, s);
 
     const bool graphs_ok = use_graphs && !spmm_profiling();
-    int32_t* ctl_host = static_cast<int32_t*>(pinned_stage(4 * sizeof(int32_t)));
+    // the front of the page-locked stage (its work-list part is read by the
+    // copy above, which the stream completes before any readback)
+    int32_t* ctl_host = reinterpret_cast<int32_t*>(stage);
     using clk = std::chrono::steady_clock;
     double wait_ms = 0.0, capture_ms = 0.0;
     int host_reads = 0;
@@ -341,7 +382,9 @@ void fast_solve_segments(const DevLaplacian& L, const std::vector<int64_t>& row0
         run_rounds(round64, round64, P.max_rounds);
         st.rounds += ctl_host[1];
         st.spmm_calls += ctl_host[2];
-        copy_cols_f64(x64, m, evecs_dev, k, n, k, s);
+        // only the segments' rows: rows of other components belong to other
+        // solvers (the dense EVD of the small ones may be writing them)
+        batch_copy_rows(t64.w, x64, m, evecs_dev, k, k, s);
 
         std::vector<double> thk((size_t)nseg * mf), worst(nseg);
         std::vector<int32_t> done(nseg);

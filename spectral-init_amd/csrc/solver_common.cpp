@@ -33,6 +33,11 @@ void* pinned_stage(size_t bytes) {
     return g_pin->p;
 }
 
+hipStream_t aux_stream(Context& cx) {
+    if (!cx.aux) SI_HIP(hipStreamCreateWithFlags(&cx.aux, hipStreamNonBlocking));
+    return cx.aux;
+}
+
 void* workspace_get(int slot, size_t bytes) {
     if (!g_ws) g_ws = new Workspace();
     DevBuf<char>& b = g_ws->slots[slot];
