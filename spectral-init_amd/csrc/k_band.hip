@@ -299,7 +299,13 @@ bool dense_factor(const DevLaplacian& L, double shift, BandFactor& F, hipStream_
     F.n = (int)n;
     F.b = (int)n - 1;
     F.dense = true;
-    F.B.ensure((size_t)n * (size_t)n);
+    try {
+        F.B.ensure((size_t)n * (size_t)n);
+    } catch (const Error&) {  // out of device memory: no factor, as a failed Cholesky
+        (void)hipGetLastError();
+        F.dense = false;
+        return false;
+    }
     SI_HIP(hipMemsetAsync(F.B.get(), 0, bytes, s));
     hipLaunchKernelGGL(k_dense_fill, dim3(ceil_div(n, (int64_t)256)), dim3(256), 0, s, L.rowptr,
                        L.col, L.l64, (int)n, shift, F.B.get());

@@ -24,6 +24,7 @@
 
 #include <map>
 #include <mutex>
+#include <utility>
 
 #include "si_host.hpp"
 
@@ -417,19 +418,20 @@ void launch(F kernel, dim3 grid, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, 
         hipLaunchKernelGGL(kernel, grid, dim3(256), 0, s, args...);
 }
 
-// Workgroups of `kernel` (256 threads) resident on the whole device at once,
-// rounded down to a multiple of 8 (one per XCD slot); looked up once per
-// kernel.
+// Workgroups of `kernel` (256 threads) resident on the current device at
+// once, rounded down to a multiple of 8 (one per XCD slot); looked up once per
+// kernel and device.
 template <typename F>
 int64_t resident_workgroups(F kernel) {
     static std::mutex mu;
-    static std::map<const void*, int64_t> known;
+    static std::map<std::pair<const void*, int>, int64_t> known;
+    int dev = 0;
+    SI_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(mu);
-    const void* key = reinterpret_cast<const void*>(kernel);
+    const auto key = std::make_pair(reinterpret_cast<const void*>(kernel), dev);
     auto it = known.find(key);
     if (it != known.end()) return it->second;
-    int dev = 0, cus = 0, per_cu = 0;
-    SI_HIP(hipGetDevice(&dev));
+    int cus = 0, per_cu = 0;
     SI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     SI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0));
     int64_t g = (int64_t)cus * (per_cu > 0 ? per_cu : 1);
