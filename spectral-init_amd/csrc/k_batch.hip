@@ -145,8 +145,10 @@ __global__ __launch_bounds__(kPlanThreads) void k_batch_plan(BatchPlan p, int mo
     if (t == 0) {
         p.ctl[0] = gate;
         if (gate > 0) {
+            // filter steps + the Rayleigh-Ritz product; an FP32 round after the
+            // first starts from the known L X (no SpMM for its first step)
+            p.ctl[2] += gate + 1 - ((mode == 0 && p.ctl[1] > 0) ? 1 : 0);
             p.ctl[1] += 1;
-            p.ctl[2] += gate + 1;  // filter steps + the Rayleigh-Ritz product
         }
         p.ctl[3] = s_active;
     }
@@ -184,6 +186,37 @@ __global__ void k_segment_map(int32_t* __restrict__ rseg, int64_t n, int nseg,
     rseg[i] = (j >= 0 && i < ends[j]) ? ids[j] : nseg;
 }
 
+// First filter step of a round from the known L X (ax, from the previous
+// Rayleigh-Ritz update): out = s_l * ax + s_x * x per row with the segment's
+// step-1 scalars (s_l = 0: a copy-through segment, ax not read), written to the
+// ring buffer that spmm_seg_ring's step 1 would write (ring[(g - 1) mod 3]).
+// Thread per 4 columns; returns at once when the gate is 0.
+__global__ void k_axpby_seg_ring(int64_t n, int w4, int ld, const int32_t* __restrict__ rseg,
+                                 const float* __restrict__ coef, const float* __restrict__ ax,
+                                 const float* __restrict__ x, float* r0, float* r1, float* r2,
+                                 const int32_t* __restrict__ gate) {
+    const int g = *gate;
+    if (g < 1) return;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * w4) return;
+    const int i0 = ((g - 1) % 3 + 3) % 3;
+    float* out = i0 == 0 ? r0 : (i0 == 1 ? r1 : r2);
+    const int64_t i = e / w4;
+    const int64_t o = i * ld + 4 * (e % w4);
+    const int c = rseg[i];
+    const float sl = coef[3 * c], sx = coef[3 * c + 1];
+    const float4 xv = *reinterpret_cast<const float4*>(x + o);
+    float4 r;
+    if (sl == 0.f) {
+        r = make_float4(sx * xv.x, sx * xv.y, sx * xv.z, sx * xv.w);
+    } else {
+        const float4 av = *reinterpret_cast<const float4*>(ax + o);
+        r = make_float4(sl * av.x + sx * xv.x, sl * av.y + sx * xv.y, sl * av.z + sx * xv.z,
+                        sl * av.w + sx * xv.w);
+    }
+    *reinterpret_cast<float4*>(out + o) = r;
+}
+
 // workgroup per work item: its rows [blk_rows[2b], blk_rows[2b+1]), k columns
 __global__ void k_copy_item_rows(const int64_t* __restrict__ blk_rows,
                                  const double* __restrict__ src, int lds, double* __restrict__ dst,
@@ -198,6 +231,22 @@ __global__ void k_copy_item_rows(const int64_t* __restrict__ blk_rows,
 }
 
 }  // namespace
+
+void batch_axpby_ring_f32(int64_t n, int w, int ld, const int32_t* rseg, const float* coef,
+                          const float* ax, const float* x, float* const ring[3],
+                          const int32_t* gate, hipStream_t s) {
+    SI_CHECK(w % 4 == 0 && ld % 4 == 0 && ld >= w, kErrInvalidArgument,
+             "batch axpby: widths must be multiples of 4");
+    SI_CHECK((reinterpret_cast<uintptr_t>(ax) | reinterpret_cast<uintptr_t>(x) |
+              reinterpret_cast<uintptr_t>(ring[0]) | reinterpret_cast<uintptr_t>(ring[1]) |
+              reinterpret_cast<uintptr_t>(ring[2])) % 16 == 0,
+             kErrInvalidArgument, "batch axpby: 16-byte aligned blocks required");
+    const int64_t threads = n * (w / 4);
+    if (threads <= 0) return;
+    hipLaunchKernelGGL(k_axpby_seg_ring, dim3((unsigned)ceil_div(threads, (int64_t)256)), dim3(256),
+                       0, s, n, w / 4, ld, rseg, coef, ax, x, ring[0], ring[1], ring[2], gate);
+    SI_LAUNCH_CHECK();
+}
 
 void batch_copy_rows(const SegWork& items, const double* src, int lds, double* dst, int ldd,
                      int k, hipStream_t s) {

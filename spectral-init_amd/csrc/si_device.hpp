@@ -255,6 +255,12 @@ struct BatchPlan {
 // segment numbers (device arrays)
 void batch_segment_map(int32_t* rseg, int64_t n, int nseg, const int64_t* starts,
                        const int64_t* ends, const int32_t* ids, hipStream_t s);
+// The first filter step of a planned round from the known L X: per row,
+// s_l ax + s_x x with its segment's step-1 scalars (coef row 0), into the ring
+// buffer a step-1 spmm_seg_ring launch would write; idle when *gate == 0.
+void batch_axpby_ring_f32(int64_t n, int w, int ld, const int32_t* rseg, const float* coef,
+                          const float* ax, const float* x, float* const ring[3],
+                          const int32_t* gate, hipStream_t s);
 // dst[r, :k] = src[r, :k] for the rows of every work item (a segment's rows)
 void batch_copy_rows(const SegWork& items, const double* src, int lds, double* dst, int ldd,
                      int k, hipStream_t s);

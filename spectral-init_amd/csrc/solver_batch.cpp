@@ -74,14 +74,18 @@ struct Ring {
 template <typename T>
 T* enqueue_filter(const DevLaplacian& L, Ring<T>& r, const T* x, int w, int ld, int steps,
                   int always, const int32_t* rseg, const T* coef, int nseg, const int32_t* gate,
-                  hipStream_t s) {
+                  hipStream_t s, const T* lx = nullptr) {
     const size_t stride = (size_t)(nseg + 1) * 3;
     for (int j = 1; j <= steps; ++j) {
         const T* cj = coef + (size_t)(j - 1) * stride;
-        if constexpr (sizeof(T) == 4)
-            spmm_seg_ring_f32(L, x, r.b, w, ld, rseg, cj, gate, j, j > always, s);
-        else
+        if constexpr (sizeof(T) == 4) {
+            if (j == 1 && lx)  // L x known from the previous update: no SpMM
+                batch_axpby_ring_f32(L.n, w, ld, rseg, cj, lx, x, r.b, gate, s);
+            else
+                spmm_seg_ring_f32(L, x, r.b, w, ld, rseg, cj, gate, j, j > always, s);
+        } else {
             spmm_seg_ring_f64(L, x, r.b, w, ld, rseg, cj, gate, j, j > always, s);
+        }
     }
     return r.b[0];
 }
@@ -342,16 +346,18 @@ void fast_solve_segments(const DevLaplacian& L, const std::vector<int64_t>& row0
             p1.steps = first_steps;
             batch_plan_f32(p1, 0, coef32, s);
         }
-        auto round32 = [&](int nsteps, int full) {
+        // the first round filters the start block; later ones start from the
+        // L X that the previous round's update left in ax32
+        auto round32 = [&](int nsteps, int full, bool lx) {
             float* y = enqueue_filter<float>(L, r32, x32, m, m, nsteps, full, rseg, coef32, nseg,
-                                             P.ctl, s);
+                                             P.ctl, s, lx ? ax32 : nullptr);
             float* ly = r32.b[1];
             enqueue_rr<float>(L, y, ly, m, m, k, x32, ax32, gram, t32, g, theta, cmat, status,
                               norms2, gws, rws, P.ctl, s);
             batch_plan_f32(P, 1, coef32, s);  // plans the next round over `steps` steps
         };
-        run_rounds([&] { round32(first_steps, first_steps); }, [&] { round32(steps, always); },
-                   P.max_rounds);
+        run_rounds([&] { round32(first_steps, first_steps, false); },
+                   [&] { round32(steps, always, true); }, P.max_rounds);
         st.rounds += ctl_host[1];
         st.spmm_calls += ctl_host[2];
 

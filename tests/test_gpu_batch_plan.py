@@ -110,8 +110,9 @@ def host_plan(mode, update, st, status, theta, norms2, k, m, cap, steps, max_rou
     active = int(np.count_nonzero(deg))
     ctl[0] = gate
     if gate > 0:
+        # an FP32 round after the first starts from the known L X: one SpMM fewer
+        ctl[2] += gate + 1 - (1 if mode == 0 and ctl[1] > 0 else 0)
         ctl[1] += 1
-        ctl[2] += gate + 1
     ctl[3] = active
     return deg, coef
 
