@@ -54,10 +54,10 @@ si_status si_test_band_solve(si_context* ctx, const si_laplacian* lap, double sh
  * the automatic choice (band first, dense when the band is too large). */
 si_status si_test_sinv_dense(int32_t enable);
 
-/* FP32 rounds as replayed hipGraphs, in the batched component solve and in
- * the single-graph FP32-bulk solver: enable >= 0 switches capture on (1) or
- * off (0, rounds enqueued eagerly), -1 leaves it; captured / launches (may be
- * NULL) receive the graph counts of the last batched solve. */
+/* Batched component solves (si_spectral_init with batch_components): enable
+ * >= 0 switches hipGraph capture of the FP32 rounds on (1) or off (0), -1
+ * leaves it; captured / launches (may be NULL) receive the graph counts of
+ * the last batched solve. */
 si_status si_test_batch_graphs(int32_t enable, int64_t* captured, int64_t* launches);
 
 /* One call of the batched solver's device round planner (k_batch.hip) on

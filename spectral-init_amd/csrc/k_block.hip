@@ -677,16 +677,6 @@ __global__ void k_axpby(int64_t n, int w, int ld, T alpha, const T* __restrict__
     out[o] = alpha * a[o] + beta * b[o];
 }
 
-template <typename T>
-__global__ void k_axpby_dev(int64_t n, int w, int ld, const T* __restrict__ coef,
-                            const T* __restrict__ a, const T* __restrict__ b, T* __restrict__ out) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n * w) return;
-    const T alpha = coef[0], beta = coef[1];
-    const int64_t o = (e / w) * ld + e % w;
-    out[o] = alpha * a[o] + beta * b[o];
-}
-
 // Row i draws the normals of original node perm[i], so a reordered solve
 // starts from the same block as an unordered one.  The draws are FP64; an FP32
 // block holds them rounded.
@@ -885,12 +875,6 @@ void axpby_cols_f32(int64_t n, int w, int ld, float alpha, const float* a, float
                     const float* b, float* out, hipStream_t s) {
     hipLaunchKernelGGL((k_axpby<float>), grid1(n * w), dim3(256), 0, s, n, w, ld, alpha, a, beta,
                        b, out);
-    SI_LAUNCH_CHECK();
-}
-void axpby_cols_dev_f32(int64_t n, int w, int ld, const float* coef, const float* a,
-                        const float* b, float* out, hipStream_t s) {
-    hipLaunchKernelGGL((k_axpby_dev<float>), grid1(n * w), dim3(256), 0, s, n, w, ld, coef, a, b,
-                       out);
     SI_LAUNCH_CHECK();
 }
 void axpby_cols_f64(int64_t n, int w, int ld, double alpha, const double* a, double beta,

@@ -96,19 +96,6 @@ __device__ inline void ring_buffers(int g, int j, T* r0, T* r1, T* r2, const T*&
     else if (j >= 3) z = pick(g - j + 2);
 }
 
-// Scalars for every row from a device table (coef set, rseg null): read once
-// per workgroup; the rows then use them as if passed by value (seg_scalars
-// reads coef only with rseg set).
-template <typename T>
-__device__ inline void uniform_scalars(const int32_t* rseg, const T* coef, T& s_l, T& s_x,
-                                       T& s_z) {
-    if (rseg == nullptr && coef != nullptr) {
-        s_l = coef[0];
-        s_x = coef[1];
-        s_z = coef[2];
-    }
-}
-
 // -------------------------------------------------------------- long rows
 // A row with more than kLongRow entries (a degree hub) gets a whole 256-thread
 // workgroup, appended after the row blocks of the same launch: threads stride
@@ -228,7 +215,6 @@ __global__ __launch_bounds__(256) void k_spmm_narrow(
     const T* __restrict__ x, T* __restrict__ out, const T* __restrict__ z, int ld, int64_t n,
     T s_l, T s_x, T s_z, const int32_t* __restrict__ rseg, const T* __restrict__ coef,
     const int32_t* __restrict__ long_rows, int64_t nb_main) {
-    uniform_scalars<T>(rseg, coef, s_l, s_x, s_z);
     if ((int64_t)blockIdx.x >= nb_main) {  // appended hub-row workgroups
         long_row<T>(long_rows[blockIdx.x - nb_main], rp, col, val, x, out, z, ld, K, s_l, s_x,
                     s_z, rseg, coef);
@@ -366,7 +352,6 @@ __global__ __launch_bounds__(256) void k_spmm_wide(
     const T* __restrict__ x, T* __restrict__ out, const T* __restrict__ z, int ld, int64_t n,
     T s_l, T s_x, T s_z, const int32_t* __restrict__ rseg, const T* __restrict__ coef,
     const int32_t* __restrict__ long_rows, int64_t nb_main) {
-    uniform_scalars<T>(rseg, coef, s_l, s_x, s_z);
     if ((int64_t)blockIdx.x >= nb_main) {  // appended hub-row workgroups
         long_row<T>(long_rows[blockIdx.x - nb_main], rp, col, val, x, out, z, ld, K, s_l, s_x,
                     s_z, rseg, coef);
@@ -828,12 +813,6 @@ void spmm_f64(const DevLaplacian& L, const double* x, double* out, int k, int ld
               int gate_step) {
     spmm_impl<double>(L, L.l64, x, out, k, ld, s_l, s_x, s_z, z, s, nullptr, nullptr, gate,
                       gate_step);
-}
-
-void spmm_dev_f32(const DevLaplacian& L, const float* x, float* out, int k, int ld,
-                  const float* z, const float* coef, hipStream_t s) {
-    SI_CHECK(coef != nullptr, kErrInvalidArgument, "spmm_dev: scalar table required");
-    spmm_impl<float>(L, L.a32, x, out, k, ld, 0.f, 0.f, 0.f, z, s, nullptr, coef);
 }
 
 namespace {

@@ -149,10 +149,6 @@ void spmm_seg_ring_f32(const DevLaplacian& L, const float* x0, float* const ring
 void spmm_seg_ring_f64(const DevLaplacian& L, const double* x0, double* const ring[3], int k,
                        int ld, const int32_t* rseg, const double* coef, const int32_t* gate,
                        int step, bool resident, hipStream_t s);
-// out = s_l * (L x) + s_x * x + s_z * z with (s_l, s_x, s_z) = coef[0..2] read on
-// the device (a launch whose scalars change between graph replays)
-void spmm_dev_f32(const DevLaplacian& L, const float* x, float* out, int k, int ld,
-                  const float* z, const float* coef, hipStream_t s);
 bool spmm_width_supported(int k);
 // y = A x for a general CSR matrix (FP64 values, any stored entries) with the
 // products rounded before each sum in storage order (the scalar spmv_csr's
@@ -318,9 +314,6 @@ void fill_normal_f32(float* x, int64_t n, int k, int ld, uint64_t seed, uint32_t
 void fill_normal_rounded_f32(float* x, int64_t n, int k, int ld, uint64_t seed, uint32_t stream,
                              hipStream_t s, const int32_t* perm = nullptr);
 // out = alpha a + beta b on columns [0,w) (all three with leading dimension ld)
-// out = coef[0] a + coef[1] b on columns [0,w) (scalars read on the device)
-void axpby_cols_dev_f32(int64_t n, int w, int ld, const float* coef, const float* a,
-                        const float* b, float* out, hipStream_t s);
 void axpby_cols_f32(int64_t n, int w, int ld, float alpha, const float* a, float beta,
                     const float* b, float* out, hipStream_t s);
 void axpby_cols_f64(int64_t n, int w, int ld, double alpha, const double* a, double beta,

@@ -51,7 +51,7 @@ enum WsSlot : int {
     kWsSmall0, kWsSmall1, kWsSmall2, kWsSmall3, kWsSmall4, kWsSmall5, kWsSmall6, kWsSmall7,
     kWsPlainS, kWsPlainAS, kWsPlainR, kWsPlainP, kWsPlainAP, kWsPlainPn, kWsPlainAPn,
     kWsPlainX, kWsPlainT, kWsPlainXr, kWsPlainBest, kWsStatus,
-    kWsBatchSmall, kWsBatchCoef, kWsBatchRr, kWsBatchPlan, kWsBatchWork, kWsBatchMap, kWsFastCoef,
+    kWsBatchSmall, kWsBatchCoef, kWsBatchRr, kWsBatchPlan, kWsBatchWork, kWsBatchMap,
     kWsDenseA, kWsDenseAux, kWsDensePiv,
     kWsNumSlots
 };
@@ -144,10 +144,6 @@ struct SolveStats {
     int passes = 0;                 // LOBPCG passes (1 + warm restarts)
     double level_ms[6] = {0, 0, 0, 0, 0, 0};  // wall time per escalation level
 };
-
-// FP32 rounds of the fast and batched solvers run as replayed hipGraphs (a
-// test seam switches them off to compare with eagerly enqueued rounds).
-extern bool g_batch_graphs;
 
 struct FastOptions {
     int block_size = 0;     // 0 = automatic

@@ -261,77 +261,6 @@ bool fused_rr(const DevLaplacian& L, int64_t n, T* y, T* ly, int w, int ld, int 
     return true;
 }
 
-// ---------------------------------------------------------------- round graphs
-// An FP32 round (Chebyshev filter + fused Rayleigh-Ritz) recorded as a hipGraph
-// and replayed: the launches of a round then run back to back with no host
-// launch between them.  The filter's scalars come from a small device table
-// that the host fills before each replay, so a graph depends only on the
-// round's degree, on whether its first step is an SpMM (round 0) or the
-// update from the known L X, and on the buffers and sizes its launches use.
-// Instantiated graphs are kept per host thread across solves; the key holds
-// every such value (the operator's arrays and tuned SpMM shapes, the
-// workspace buffers, the widths), so a hit replays exactly the launches a new
-// capture would record.
-struct GraphKey {
-    std::vector<uintptr_t> v;
-    bool operator==(const GraphKey& o) const { return v == o.v; }
-};
-
-class RoundGraphs {
-public:
-    hipGraphExec_t find(const GraphKey& k) {
-        for (size_t i = 0; i < items_.size(); ++i)
-            if (items_[i].first == k) return items_[i].second;
-        return nullptr;
-    }
-    void put(const GraphKey& k, hipGraphExec_t e) {
-        if (items_.size() >= kMax) {
-            hipGraphExecDestroy(items_.front().second);
-            items_.erase(items_.begin());
-        }
-        items_.emplace_back(k, e);
-    }
-
-private:
-    static constexpr size_t kMax = 24;
-    std::vector<std::pair<GraphKey, hipGraphExec_t>> items_;
-};
-
-// kept until the process ends (a thread-exit destructor could run after the
-// HIP runtime has shut down)
-RoundGraphs& round_graphs() {
-    static thread_local RoundGraphs* g = nullptr;
-    if (!g) g = new RoundGraphs();
-    return *g;
-}
-
-// Enqueue one FP32 round on w = m columns: the filter of `degree` steps on
-// [a, b] with scalars from coef (coef[0..2]: first step, coef[3..5]: later
-// steps), its first step an SpMM of x (first_spmm) or coef[0] lx + coef[1] x,
-// then ly = L y, G = y^T [y | ly], the scaled m x m generalized problem and
-// x = y C, ax = ly C with the wanted columns' residual sums.  Returns y.
-float* enqueue_round32(const DevLaplacian& L, int64_t n, FilterBufs<float>& fb, float* x, float* ax,
-                       int m, int k, int degree, bool first_spmm, const float* coef, Small& sm,
-                       double* gws, double* rr_ws, hipStream_t s) {
-    float* yprev = fb.b0;
-    float* y = fb.b1;
-    float* ynew = fb.b2;
-    if (first_spmm) spmm_dev_f32(L, x, y, m, m, nullptr, coef, s);
-    else axpby_cols_dev_f32(n, m, m, coef, ax, x, y, s);
-    const float* prev = x;
-    for (int d = 2; d <= degree; ++d) {
-        spmm_dev_f32(L, y, ynew, m, m, prev, coef + 3, s);
-        float* t = yprev; yprev = y; y = ynew; ynew = t;
-        prev = yprev;
-    }
-    float* ly = (y == fb.b0) ? fb.b1 : fb.b0;
-    spmm_t_ld(L, y, ly, m, m, s);
-    gram2_t(n, y, m, m, ly, sm.g, gws, s);
-    small_gen_eig_scaled(m, sm.g + m, 2 * m, sm.g, 2 * m, sm.w, sm.v, sm.status, s);
-    rr_update_t(n, y, ly, m, m, sm.v, sm.w, std::min(k, 8), x, ax, sm.res, rr_ws, s);
-    return y;
-}
-
 }  // namespace
 
 int fast_block_width(int64_t n, int k, int requested) { return choose_block(n, k, requested); }
@@ -389,74 +318,14 @@ bool fast_solve(const DevLaplacian& L, int n_components, uint64_t seed, const Fa
         const int dmax = (int)std::floor(9.2 / std::acosh(std::max(x0, 1.0 + 1e-12)));
         return std::max(2, std::min(degree, dmax));
     };
-    // rounds as replayed graphs (the fused Rayleigh-Ritz covers 2m <= 64, k <= 8;
-    // profiled SpMMs record events per launch, so those run eagerly)
-    const bool round_graphs_ok = 2 * m <= 64 && k <= 8 && !spmm_profiling() && g_batch_graphs;
-    float* coef_dev = ws_get<float>(kWsFastCoef, 8);
     for (; round < opt.max_outer; ++round) {
-        const int deg = degree_for(a);
-        float* y = nullptr;
-        bool fused = false;
-        if (round_graphs_ok) {
-            const double c = (a + b) / 2.0, e = (b - a) / 2.0;
-            const float tab[6] = {(float)(1.0 / e), (float)(-c / e), 0.f,
-                                  (float)(2.0 / e), (float)(-2.0 * c / e), -1.f};
-            // page-locked stage: the table at the front, the readback after it
-            char* stage = static_cast<char*>(pinned_stage(64 + sm.tail_bytes()));
-            std::memcpy(stage, tab, sizeof(tab));
-            SI_HIP(hipMemcpyAsync(coef_dev, stage, sizeof(tab), hipMemcpyHostToDevice, s));
-            const bool first_spmm = round == 0;
-            GraphKey key{{(uintptr_t)L.rowptr, (uintptr_t)L.col, (uintptr_t)L.a32,
-                          (uintptr_t)L.long_rows, (uintptr_t)L.n, (uintptr_t)L.nnz,
-                          (uintptr_t)L.n_long, (uintptr_t)L.shape16, (uintptr_t)L.shape32,
-                          (uintptr_t)x32, (uintptr_t)ax32, (uintptr_t)fb32.b0, (uintptr_t)fb32.b1,
-                          (uintptr_t)fb32.b2, (uintptr_t)g2w, (uintptr_t)rr_ws, (uintptr_t)sm.g,
-                          (uintptr_t)coef_dev, (uintptr_t)m, (uintptr_t)k, (uintptr_t)deg,
-                          (uintptr_t)first_spmm}};
-            hipGraphExec_t ex = round_graphs().find(key);
-            if (!ex) {
-                hipGraph_t gr = nullptr;
-                SI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                try {
-                    enqueue_round32(L, n, fb32, x32, ax32, m, k, deg, first_spmm, coef_dev, sm, g2w,
-                                    rr_ws, s);
-                } catch (...) {
-                    hipStreamEndCapture(s, &gr);
-                    if (gr) hipGraphDestroy(gr);
-                    throw;
-                }
-                SI_HIP(hipStreamEndCapture(s, &gr));
-                const hipError_t ie = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
-                hipGraphDestroy(gr);
-                SI_HIP(ie);
-                round_graphs().put(key, ex);
-            }
-            SI_HIP(hipGraphLaunch(ex, s));
-            // the buffer the filter ends in (as enqueue_round32 rotates them)
-            y = deg % 3 == 1 ? fb32.b1 : (deg % 3 == 2 ? fb32.b2 : fb32.b0);
-            st.spmm_calls += (first_spmm ? deg : deg - 1) + 1;
-            double* h = reinterpret_cast<double*>(stage + 64);
-            SI_HIP(hipMemcpyAsync(h, sm.w, sm.tail_bytes(), hipMemcpyDeviceToHost, s));
-            SI_HIP(hipStreamSynchronize(s));
-            int stv = 0;
-            std::memcpy(&stv, h + 4 * sm.m + 16, sizeof(int));
-            if (stv == 0) {
-                for (int j = 0; j < m; ++j) theta[j] = h[j];
-                const double* nrm = h + 2 * sm.m;
-                for (int j = 0; j < k; ++j) res[j] = residual_from_sums(nrm[j], nrm[k + j]);
-                fused = true;
-            }
-        } else {
-            y = cheb_filter<float>(L, fb32, x32, m, m, a, b, deg, st.spmm_calls, s,
-                                   round > 0 ? ax32 : nullptr);
-            float* ly0 = (y == fb32.b0) ? fb32.b1 : fb32.b0;
-            fused = fused_rr<float>(L, n, y, ly0, m, m, k, x32, ax32, sm, g2w, rr_ws, theta.data(),
-                                    res.data(), st.spmm_calls, s, r32);
-        }
+        float* y = cheb_filter<float>(L, fb32, x32, m, m, a, b, degree_for(a), st.spmm_calls, s,
+                                      round > 0 ? ax32 : nullptr);
         float* ly = (y == fb32.b0) ? fb32.b1 : fb32.b0;
         // one fused Rayleigh-Ritz on the filtered block; the explicit
         // orthonormalisation path is the fallback when Y^T Y is too ill-conditioned
-        if (!fused) {
+        if (!fused_rr<float>(L, n, y, ly, m, m, k, x32, ax32, sm, g2w, rr_ws, theta.data(),
+                             res.data(), st.spmm_calls, s, r32)) {
             if (!orthonormalise<float>(n, y, m, sm, ws, s)) {
                 gram_t(n, y, m, m, y, m, m, sm.g, ws, s);
                 small_inv_sqrt_diag(m, sm.g, sm.sc, s);

@@ -145,30 +145,6 @@ def test_fast_solver_deterministic(ctx):
     assert torch.equal(v1, v2)
 
 
-@pytest.mark.parametrize("block", [8, 16, 32])
-def test_fast_solver_round_graphs_match_eager_rounds(ctx, gmm20k, block):
-    # FP32 rounds replayed as hipGraphs (scalars from a device table, graphs
-    # kept across solves) against the same rounds enqueued one launch at a time:
-    # the same kernels in the same order, so bit for bit; the third solve
-    # replays the graphs the second one captured
-    w, L, ref = gmm20k
-    print(f"case fast round graphs block {block}", flush=True)
-    lap = si.Laplacian(ctx, w["indptr"], w["indices"], w["data"])
-    cfg = si.SpectralInitConfig(tol=1e-6, block_size=block)
-    si.batch_graphs(False)
-    try:
-        ev_e, v_e, rep_e = si.fast_solve(lap, 2, 42, cfg)
-    finally:
-        si.batch_graphs(True)
-    for _ in range(2):
-        ev_g, v_g, rep_g = si.fast_solve(lap, 2, 42, cfg)
-        np.testing.assert_array_equal(ev_g, ev_e)
-        assert torch.equal(v_g, v_e)
-        assert rep_g.spmm_calls == rep_e.spmm_calls and rep_g.iterations == rep_e.iterations
-        torch.cuda.synchronize()
-    _check_pairs(ev_g, v_g.cpu().numpy(), ref["evals"], ref["evecs"], L, 1e-6, 3)
-
-
 # ------------------------------------------------------------------ rSVD / dense / chain
 def test_rsvd_ring6_closed_form(ctx):
     ip, ix, w = ring_graph(6)
