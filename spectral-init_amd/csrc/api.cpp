@@ -321,15 +321,28 @@ void solve_components_batched(si::Context& cx, const si::DevLaplacian& L,
         tp = now;
     };
     {
-        std::vector<int32_t> perm(n);
-        if (L.perm) si::to_host(perm.data(), L.perm, n, s);
-        else std::iota(perm.begin(), perm.end(), 0);
         // row range of every component (the locality order keeps them contiguous)
         std::vector<int64_t> first(nc, -1), last_row(nc, -1);
-        for (int64_t r = 0; r < n; ++r) {
-            const int64_t c = labels[perm[r]];
-            if (first[c] < 0) first[c] = r;
-            last_row[c] = r;
+        if (L.perm && L.comp && L.n_comp == nc) {
+            // device labels of the operator's build: one pass over the rows there
+            si::DevBuf<int32_t> rr(2 * nc);
+            si::component_rows(n, L.perm, L.comp, nc, rr.get(), s);
+            std::vector<int32_t> h(2 * nc);
+            si::to_host(h.data(), rr.get(), 2 * nc, s);
+            for (int64_t c = 0; c < nc; ++c)
+                if (h[nc + c] >= 0) {
+                    first[c] = h[c];
+                    last_row[c] = h[nc + c];
+                }
+        } else {
+            std::vector<int32_t> perm(n);
+            if (L.perm) si::to_host(perm.data(), L.perm, n, s);
+            else std::iota(perm.begin(), perm.end(), 0);
+            for (int64_t r = 0; r < n; ++r) {
+                const int64_t c = labels[perm[r]];
+                if (first[c] < 0) first[c] = r;
+                last_row[c] = r;
+            }
         }
         for (int64_t c = 0; c < nc; ++c)
             SI_CHECK(last_row[c] - first[c] + 1 == (int64_t)members[c].size(), si::kErrInternal,

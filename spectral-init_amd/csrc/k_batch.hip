@@ -186,6 +186,26 @@ __global__ void k_segment_map(int32_t* __restrict__ rseg, int64_t n, int nseg,
     rseg[i] = (j >= 0 && i < ends[j]) ? ids[j] : nseg;
 }
 
+// Row range of every component in the operator's row order: rows [rr[c],
+// rr[nc + c]] carry component c (perm[row] = input node, comp = its label).
+// Only the ends of runs write (a component in one contiguous run writes each
+// end once); rr starts at (INT32_MAX, -1).
+__global__ void k_component_rows(int64_t n, const int32_t* __restrict__ perm,
+                                 const int32_t* __restrict__ comp, int32_t nc,
+                                 int32_t* __restrict__ rr) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int32_t c = comp[perm[r]];
+    if (r == 0 || comp[perm[r - 1]] != c) atomicMin(&rr[c], (int32_t)r);
+    if (r == n - 1 || comp[perm[r + 1]] != c) atomicMax(&rr[nc + c], (int32_t)r);
+}
+
+__global__ void k_component_rows_init(int32_t nc, int32_t* __restrict__ rr) {
+    const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < nc) rr[i] = INT32_MAX;
+    else if (i < 2 * nc) rr[i] = -1;
+}
+
 // First filter step of a round from the known L X (ax, from the previous
 // Rayleigh-Ritz update): out = s_l * ax + s_x * x per row with the segment's
 // step-1 scalars (s_l = 0: a copy-through segment, ax not read), written to the
@@ -261,6 +281,17 @@ void batch_segment_map(int32_t* rseg, int64_t n, int nseg, const int64_t* starts
     if (n <= 0) return;
     hipLaunchKernelGGL(k_segment_map, dim3((unsigned)ceil_div(n, (int64_t)256)), dim3(256), 0, s,
                        rseg, n, nseg, starts, ends, ids);
+    SI_LAUNCH_CHECK();
+}
+
+void component_rows(int64_t n, const int32_t* perm, const int32_t* comp, int64_t nc, int32_t* rr,
+                    hipStream_t s) {
+    SI_CHECK(n >= 1 && nc >= 1 && nc <= n && n < (int64_t)INT32_MAX, kErrInvalidArgument,
+             "component rows: bad sizes");
+    hipLaunchKernelGGL(k_component_rows_init, dim3((unsigned)ceil_div(2 * nc, (int64_t)256)),
+                       dim3(256), 0, s, (int32_t)nc, rr);
+    hipLaunchKernelGGL(k_component_rows, dim3((unsigned)ceil_div(n, (int64_t)256)), dim3(256), 0, s,
+                       n, perm, comp, (int32_t)nc, rr);
     SI_LAUNCH_CHECK();
 }
 

@@ -249,6 +249,11 @@ struct BatchPlan {
 // rseg[i] = the segment holding row i, nseg for rows of no segment; the
 // segments' row ranges [starts[j], ends[j]) sorted by start, ids[j] their
 // segment numbers (device arrays)
+// First and last row of every component in the operator's row order: rr[c]
+// and rr[nc + c] (device, 2 nc entries; INT32_MAX and -1 for a component
+// with no row); perm[row] = input node, comp[node] = component label.
+void component_rows(int64_t n, const int32_t* perm, const int32_t* comp, int64_t nc, int32_t* rr,
+                    hipStream_t s);
 void batch_segment_map(int32_t* rseg, int64_t n, int nseg, const int64_t* starts,
                        const int64_t* ends, const int32_t* ids, hipStream_t s);
 // The first filter step of a planned round from the known L X: per row,
