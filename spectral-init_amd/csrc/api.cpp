@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstddef>
+#include <exception>
 #include <memory>
 #include <numeric>
 #include <thread>
@@ -112,6 +113,41 @@ void select_and_sign(const std::vector<double>& ev, const std::vector<double>& v
             for (int64_t r = 0; r < n; ++r) out[(size_t)r * d + c] = -out[(size_t)r * d + c];
     }
 }
+
+// The centroid of every component's raw data rows (spectral_meta_embedding,
+// src/multi_component.rs:143-165): FP64 sums in member order, one component
+// per task, on host threads that run beside the caller until join().  The
+// caller keeps raw and members alive and unchanged until then.
+struct CentroidJob {
+    std::vector<double> cen;  // nc x n_features
+    std::vector<std::thread> pool;
+    std::atomic<int64_t> next{0};
+
+    void start(const float* raw, int64_t nf, const std::vector<std::vector<int64_t>>& members) {
+        const int64_t nc = (int64_t)members.size();
+        cen.assign((size_t)nc * nf, 0.0);
+        auto centroid = [this, raw, nf, &members](int64_t c) {
+            double* cc = cen.data() + c * nf;
+            for (int64_t node : members[c])
+                for (int64_t f = 0; f < nf; ++f) cc[f] += (double)raw[node * nf + f];
+            const double inv = 1.0 / (double)members[c].size();
+            for (int64_t f = 0; f < nf; ++f) cc[f] *= inv;
+        };
+        // the caller's thread keeps a core for the device work it drives
+        const unsigned hw = std::max(2u, std::thread::hardware_concurrency());
+        const int nth = (int)std::min<int64_t>(nc, std::min(8u, hw - 1));
+        for (int t = 0; t < nth; ++t)
+            pool.emplace_back([this, nc, centroid] {
+                for (int64_t c = next++; c < nc; c = next++) centroid(c);
+            });
+    }
+    void join() {
+        for (auto& th : pool)
+            if (th.joinable()) th.join();
+        pool.clear();
+    }
+    ~CentroidJob() { join(); }
+};
 
 struct DeviceGraph {
     si::DevBuf<int64_t> ip;
@@ -971,32 +1007,88 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
                 for (int64_t i = 0; i < n; ++i) members[labels[i]].push_back(i);
             }
             std::vector<double> meta((size_t)nc * d, 0.0);
-            if (nc > 2 * d) {
-                if (!raw_data)
-                    throw si::Error(si::kErrInvalidGraph,
-                                    "invalid graph: data is required for spectral meta-embedding when "
-                                    "n_conn_components > 2 * n_embedding_dims");
-                // spectral_meta_embedding (multi_component.rs:143-220)
-                // each component's sums in member order (components on host threads)
-                std::vector<double> cen((size_t)nc * n_features, 0.0);
-                auto centroid = [&](int64_t c) {
-                    double* cc = cen.data() + c * n_features;
-                    for (int64_t node : members[c])
-                        for (int64_t f = 0; f < n_features; ++f)
-                            cc[f] += (double)raw_data[node * n_features + f];
-                    const double inv = 1.0 / (double)members[c].size();
-                    for (int64_t f = 0; f < n_features; ++f) cc[f] *= inv;
-                };
-                const int nth = (int)std::min<int64_t>(
-                    nc, std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency())));
-                std::atomic<int64_t> next{0};
-                auto worker = [&] {
-                    for (int64_t c = next++; c < nc; c = next++) centroid(c);
-                };
-                std::vector<std::thread> pool;
-                for (int t = 1; t < nth; ++t) pool.emplace_back(worker);
-                worker();
-                for (auto& th : pool) th.join();
+            const bool spectral_meta = nc > 2 * d;
+            if (spectral_meta && !raw_data)
+                throw si::Error(si::kErrInvalidGraph,
+                                "invalid graph: data is required for spectral meta-embedding when "
+                                "n_conn_components > 2 * n_embedding_dims");
+            // spectral_meta_embedding (multi_component.rs:143-220) starts with the
+            // components' centroids of the raw data, summed in member order: on
+            // host threads that run while the device solves the components
+            CentroidJob centroids;
+            if (spectral_meta) centroids.start(raw_data, n_features, members);
+            // Eigenpairs of every component that gets its own embedding (size >= 2d):
+            // all at once on the device (batched), or one solve per component.
+            std::vector<std::vector<double>> cev(nc), cvec(nc);
+            std::vector<int> csolved(nc, 0);
+            si::DevBuf<double> blk;        // batch-solved eigenvectors, operator row order
+            std::vector<int64_t> seg_row0;  // each component's first row in blk
+            std::vector<int64_t> lookup(n, -1);
+            auto solve_one = [&](int64_t c) {
+                const std::vector<int64_t>& mem = members[c];
+                const int64_t sz = (int64_t)mem.size();
+                for (int64_t q = 0; q < sz; ++q) lookup[mem[q]] = q;
+                std::vector<int64_t> sip(sz + 1, 0);
+                std::vector<int32_t> six;
+                std::vector<float> sw;
+                for (int64_t q = 0; q < sz; ++q) {
+                    const int64_t u = mem[q];
+                    for (int64_t p = indptr[u]; p < indptr[u + 1]; ++p)
+                        if (lookup[indices[p]] >= 0) {
+                            six.push_back((int32_t)lookup[indices[p]]);
+                            sw.push_back(data[p]);
+                        }
+                    sip[q + 1] = (int64_t)six.size();
+                }
+                si::SolveStats cst;
+                last.level = solve_graph(ctx->c, sz, sip.data(), six.data(), sw.data(), d, seed, mode,
+                                         opt, cev[c], cvec[c], cst, &lap_ms, &solve_ms);
+                last.iterations += cst.iterations;
+                last.spmm_calls += cst.spmm_calls;
+                last.max_residual = std::max(last.max_residual, cst.max_residual);
+                for (int64_t node : mem) lookup[node] = -1;
+                csolved[c] = 1;
+            };
+            // the batched segment solver carries k = d + 1 <= 8 columns; wider
+            // embeddings take the per-component loop
+            bool all_batched = false;
+            std::exception_ptr solve_error;
+            try {
+                if (!(batch && d <= 7)) si::laplacian_free(whole.L);
+                if (batch && d <= 7) {
+                    // the batch needs every component as a contiguous row range:
+                    // the locality order
+                    if (!whole.L.perm)
+                        build_operator(ctx->c, whole.L, n, indptr, indices, data, mode,
+                                       /*reorder=*/true, /*tune_widths=*/0, &lap_ms);
+                    {
+                        // the batched solver's block is 16 columns wide
+                        si::EventTimer t(ctx->c.stream);
+                        tune_once(ctx->c, whole.L, si::kTuned16);
+                        lap_ms += t.stop_ms();
+                    }
+                    phase("spmm autotune");
+                    solve_components_batched(ctx->c, whole.L, labels, members, d, seed, opt, cev, blk,
+                                             seg_row0, csolved, last, &solve_ms);
+                    phase("batched solve");
+                }
+                // components left to the per-component chain (a batch failure, or
+                // no batch) make the placement a host step
+                all_batched = blk.get() != nullptr && si::g_device_output;
+                for (int64_t c = 0; c < nc; ++c)
+                    if (!csolved[c] && (int64_t)members[c].size() >= 2 * d) {
+                        all_batched = false;
+                        solve_one(c);
+                    }
+            } catch (...) {
+                solve_error = std::current_exception();
+            }
+            centroids.join();
+            phase("component eigenpairs");
+            // the rest of the meta-embedding; its own errors are reported
+            // before a component solve's, in the reference's order of steps
+            if (spectral_meta) {
+                const std::vector<double>& cen = centroids.cen;
                 // Gaussian affinity exp(-||c_i - c_j||^2) in FP64
                 std::vector<double> aff((size_t)nc * nc);
                 for (int64_t i = 0; i < nc; ++i)
@@ -1079,66 +1171,7 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
                 range[c] = std::isfinite(md) ? md / 2.0 : 1.0;
             }
             phase("meta-embedding, ranges");
-            // Eigenpairs of every component that gets its own embedding (size >= 2d):
-            // all at once on the device (batched), or one solve per component.
-            std::vector<std::vector<double>> cev(nc), cvec(nc);
-            std::vector<int> csolved(nc, 0);
-            si::DevBuf<double> blk;        // batch-solved eigenvectors, operator row order
-            std::vector<int64_t> seg_row0;  // each component's first row in blk
-            std::vector<int64_t> lookup(n, -1);
-            auto solve_one = [&](int64_t c) {
-                const std::vector<int64_t>& mem = members[c];
-                const int64_t sz = (int64_t)mem.size();
-                for (int64_t q = 0; q < sz; ++q) lookup[mem[q]] = q;
-                std::vector<int64_t> sip(sz + 1, 0);
-                std::vector<int32_t> six;
-                std::vector<float> sw;
-                for (int64_t q = 0; q < sz; ++q) {
-                    const int64_t u = mem[q];
-                    for (int64_t p = indptr[u]; p < indptr[u + 1]; ++p)
-                        if (lookup[indices[p]] >= 0) {
-                            six.push_back((int32_t)lookup[indices[p]]);
-                            sw.push_back(data[p]);
-                        }
-                    sip[q + 1] = (int64_t)six.size();
-                }
-                si::SolveStats cst;
-                last.level = solve_graph(ctx->c, sz, sip.data(), six.data(), sw.data(), d, seed, mode,
-                                         opt, cev[c], cvec[c], cst, &lap_ms, &solve_ms);
-                last.iterations += cst.iterations;
-                last.spmm_calls += cst.spmm_calls;
-                last.max_residual = std::max(last.max_residual, cst.max_residual);
-                for (int64_t node : mem) lookup[node] = -1;
-                csolved[c] = 1;
-            };
-            // the batched segment solver carries k = d + 1 <= 8 columns; wider
-            // embeddings take the per-component loop
-            if (!(batch && d <= 7)) si::laplacian_free(whole.L);
-            if (batch && d <= 7) {
-                // the batch needs every component as a contiguous row range:
-                // the locality order
-                if (!whole.L.perm)
-                    build_operator(ctx->c, whole.L, n, indptr, indices, data, mode,
-                                   /*reorder=*/true, /*tune_widths=*/0, &lap_ms);
-                {
-                    // the batched solver's block is 16 columns wide
-                    si::EventTimer t(ctx->c.stream);
-                    tune_once(ctx->c, whole.L, si::kTuned16);
-                    lap_ms += t.stop_ms();
-                }
-                phase("spmm autotune");
-                solve_components_batched(ctx->c, whole.L, labels, members, d, seed, opt, cev, blk,
-                                         seg_row0, csolved, last, &solve_ms);
-                phase("batched solve");
-            }
-            // components left to the per-component chain (a batch failure, or
-            // no batch) make the placement a host step
-            bool all_batched = blk.get() != nullptr && si::g_device_output;
-            for (int64_t c = 0; c < nc; ++c)
-                if (!csolved[c] && (int64_t)members[c].size() >= 2 * d) {
-                    all_batched = false;
-                    solve_one(c);
-                }
+            if (solve_error) std::rethrow_exception(solve_error);
             for (int64_t c = 0; c < nc; ++c)
                 if (!cev[c].empty()) ev = cev[c];
             if (all_batched) {

@@ -54,21 +54,46 @@ __global__ void k_validate(const int32_t* __restrict__ rp, const int32_t* __rest
 }
 
 // One thread per row: each entry (i, j, w) must have a bitwise-equal (j, i, w).
-__global__ void k_check_symmetric(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                  const float* __restrict__ w, int64_t n, int* __restrict__ flags) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    for (int32_t p = rp[i]; p < rp[i + 1]; ++p) {
-        int32_t j = col[p];
-        int32_t lo = rp[j], hi = rp[j + 1] - 1;
-        bool found = false;
-        while (lo <= hi) {
-            int32_t mid = (lo + hi) >> 1;
-            int32_t c = col[mid];
-            if (c == i) { found = __float_as_uint(w[mid]) == __float_as_uint(w[p]); break; }
-            if (c < i) lo = mid + 1; else hi = mid - 1;
+// The row kernels below give kRowLanes lanes to a row; a workgroup takes
+// 256 / kRowLanes consecutive rows at a time, grid-strided over at most
+// kRowGrid workgroups.
+constexpr int kRowLanes = 16;
+constexpr int64_t kRowsPerBlock = 256 / kRowLanes;
+constexpr int64_t kRowGrid = 4096;
+
+inline dim3 row_grid(int64_t n) {
+    return dim3((unsigned)std::min<int64_t>(ceil_div(n, kRowsPerBlock), kRowGrid));
+}
+
+__global__ __launch_bounds__(256) void k_check_symmetric(const int32_t* __restrict__ rp,
+                                                         const int32_t* __restrict__ col,
+                                                         const float* __restrict__ w, int64_t n,
+                                                         int* __restrict__ flags) {
+    const int sub = threadIdx.x & (kRowLanes - 1);
+    for (int64_t base = (int64_t)blockIdx.x * kRowsPerBlock; base < n;
+         base += (int64_t)gridDim.x * kRowsPerBlock) {
+        const int64_t i = base + threadIdx.x / kRowLanes;
+        if (i >= n) continue;
+        const int32_t e = rp[i + 1];
+        for (int32_t p = rp[i] + sub; p < e; p += kRowLanes) {
+            const int32_t j = col[p];
+            int32_t lo = rp[j], hi = rp[j + 1] - 1;
+            bool found = false;
+            while (lo <= hi) {
+                const int32_t mid = (lo + hi) >> 1;
+                const int32_t c = col[mid];
+                if (c == i) {
+                    found = __float_as_uint(w[mid]) == __float_as_uint(w[p]);
+                    break;
+                }
+                if (c < i) lo = mid + 1;
+                else hi = mid - 1;
+            }
+            if (!found) {
+                atomicOr(&flags[kFlagAsym], 1);
+                return;
+            }
         }
-        if (!found) { atomicOr(&flags[kFlagAsym], 1); return; }
     }
 }
 
@@ -113,23 +138,30 @@ __global__ void k_isd(const double* __restrict__ deg, int64_t n, double* __restr
     isd[i] = s > 0.0 ? 1.0 / s : 0.0;  // lib.rs:233-236
 }
 
-__global__ void k_values(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                         const float* __restrict__ w, const double* __restrict__ isd, int64_t n,
-                         double* __restrict__ l64, float* __restrict__ a32,
-                         int32_t* __restrict__ long_rows, int* __restrict__ n_long) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double di = isd[i];
-    int32_t b = rp[i], e = rp[i + 1];
-    for (int32_t p = b; p < e; ++p) {
-        int32_t j = col[p];
-        double v = (j == i) ? 0.0 : (-di * (double)w[p]) * isd[j];
-        l64[p] = v;
-        a32[p] = (float)(-v);
-    }
-    if (e - b > kLongRow) {
-        int slot = atomicAdd(n_long, 1);
-        long_rows[slot] = (int32_t)i;
+__global__ __launch_bounds__(256) void k_values(const int32_t* __restrict__ rp,
+                                                const int32_t* __restrict__ col,
+                                                const float* __restrict__ w,
+                                                const double* __restrict__ isd, int64_t n,
+                                                double* __restrict__ l64, float* __restrict__ a32,
+                                                int32_t* __restrict__ long_rows,
+                                                int* __restrict__ n_long) {
+    const int sub = threadIdx.x & (kRowLanes - 1);
+    for (int64_t base = (int64_t)blockIdx.x * kRowsPerBlock; base < n;
+         base += (int64_t)gridDim.x * kRowsPerBlock) {
+        const int64_t i = base + threadIdx.x / kRowLanes;
+        if (i >= n) continue;
+        const double di = isd[i];
+        const int32_t b = rp[i], e = rp[i + 1];
+        for (int32_t p = b + sub; p < e; p += kRowLanes) {
+            const int32_t j = col[p];
+            const double v = (j == i) ? 0.0 : (-di * (double)w[p]) * isd[j];
+            l64[p] = v;
+            a32[p] = (float)(-v);
+        }
+        if (sub == 0 && e - b > kLongRow) {
+            const int slot = atomicAdd(n_long, 1);
+            long_rows[slot] = (int32_t)i;
+        }
     }
 }
 
@@ -202,7 +234,7 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
     phase("validate");
     int sorted = flags[kFlagUnsorted] == 0;
     if (sorted) {
-        hipLaunchKernelGGL(k_check_symmetric, dim3(gn), dim3(T), 0, s, L.rowptr, L.col, d_w, n,
+        hipLaunchKernelGGL(k_check_symmetric, row_grid(n), dim3(T), 0, s, L.rowptr, L.col, d_w, n,
                            d_flags);
         SI_LAUNCH_CHECK();
         SI_HIP(hipMemcpyAsync(flags, d_flags, sizeof(int) * kNumFlags, hipMemcpyDeviceToHost, s));
@@ -252,7 +284,7 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
                 !prefer_rcm) {
                 // the clustering order walks on the operator in the input order:
                 // its values first (rebuilt below in the new order)
-                hipLaunchKernelGGL(k_values, dim3(gn), dim3(T), 0, s, L.rowptr, L.col, d_w, L.isd,
+                hipLaunchKernelGGL(k_values, row_grid(n), dim3(T), 0, s, L.rowptr, L.col, d_w, L.isd,
                                    n, L.l64, L.a32, L.long_rows, d_nlong);
                 SI_LAUNCH_CHECK();
                 int nl = 0;
@@ -312,7 +344,7 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
         w_vals = d_w2;
         phase("upload permuted");
     }
-    hipLaunchKernelGGL(k_values, dim3(gn), dim3(T), 0, s, L.rowptr, L.col, w_vals, L.isd, n, L.l64,
+    hipLaunchKernelGGL(k_values, row_grid(n), dim3(T), 0, s, L.rowptr, L.col, w_vals, L.isd, n, L.l64,
                        L.a32, L.long_rows, d_nlong);
     SI_LAUNCH_CHECK();
     int nlong = 0;

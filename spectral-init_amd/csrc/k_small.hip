@@ -4,9 +4,12 @@
 // src/solvers/lobpcg.rs:158-187 and src/solvers/rsvd.rs:56-77).
 //
 // Eigenvalues use parallel cyclic Jacobi with a round-robin (circle) pairing:
-// the n/2 disjoint rotations of a round are applied together (rows, then
-// columns), which is exact for disjoint pairs.  Matrices live in LDS with a
-// padded row stride; n <= 64.
+// the n/2 disjoint rotations of a round are applied together, J^T A J one
+// 2 x 2 block (two rows of one pair, two columns of another) per task, which
+// is exact for disjoint pairs.  Matrices live in LDS with a padded row
+// stride; n <= 64.
+#include <type_traits>
+
 #include "si_device.hpp"
 
 namespace si {
@@ -16,12 +19,22 @@ namespace {
 constexpr int N = kSmallMax;
 constexpr int LDA = N + 1;
 constexpr int kThreads = 256;
+constexpr int kWide = 1024;
 
 // Block size of the one-workgroup solvers.  The device code loops over
-// blockDim.x, so any power of two up to kThreads works; four wavefronts
+// blockDim.x, so any power of two up to kWide works; four wavefronts
 // measured faster than one even at 16 rows (64 vs 72 us per generalized
-// problem on MI355X).
+// problem on MI355X), and the eigensolvers give sixteen to a matrix above 32
+// rows, whose Jacobi rounds update up to 64 x 32 pairs of entries.
 inline int small_threads(int /*n*/) { return kThreads; }
+
+// Launch f(std::integral_constant<int, T>) with the eigensolvers' block size
+// T for an n x n problem.
+template <typename F>
+void by_eig_threads(int n, F&& f) {
+    if (n > 32) f(std::integral_constant<int, kWide>{});
+    else f(std::integral_constant<int, kThreads>{});
+}
 
 // Pair (p, q) of round r for an even number of players m2 (circle method).
 __device__ inline void round_pair(int r, int i, int m2, int& p, int& q) {
@@ -88,27 +101,30 @@ __device__ void jacobi_lds(double (*a)[LDA], double (*v)[LDA], int n, double* cs
                 cs[t] = c; sn[t] = s; pp[t] = p; qq[t] = q;
             }
             __syncthreads();
-            // rows: A <- J^T A
-            for (int e = t; e < half * n; e += (int)blockDim.x) {
-                const int pi = e / n, j = e % n;
-                const int p = pp[pi], q = qq[pi];
-                if (p == q) continue;
-                const double c = cs[pi], s = sn[pi];
-                const double apj = a[p][j], aqj = a[q][j];
-                a[p][j] = c * apj - s * aqj;
-                a[q][j] = s * apj + c * aqj;
+            // A <- J^T A J one 2 x 2 block at a time: rows of pair P, columns
+            // of pair Q (the blocks partition A); the annihilated entries of a
+            // diagonal block are set to exactly zero.  A dummy pair (p == q)
+            // has c = 1, s = 0 and writes its entries back unchanged.
+            for (int e = t; e < half * half; e += (int)blockDim.x) {
+                const int P = e / half, Q = e % half;
+                const int p1 = pp[P], q1 = qq[P], p2 = pp[Q], q2 = qq[Q];
+                const double c1 = cs[P], s1 = sn[P], c2 = cs[Q], s2 = sn[Q];
+                const double a11 = a[p1][p2], a12 = a[p1][q2];
+                const double a21 = a[q1][p2], a22 = a[q1][q2];
+                const double u11 = c1 * a11 - s1 * a21, u12 = c1 * a12 - s1 * a22;
+                const double u21 = s1 * a11 + c1 * a21, u22 = s1 * a12 + c1 * a22;
+                const bool diag = P == Q && p1 != q1;
+                a[p1][p2] = c2 * u11 - s2 * u12;
+                a[p1][q2] = diag ? 0.0 : s2 * u11 + c2 * u12;
+                a[q1][p2] = diag ? 0.0 : c2 * u21 - s2 * u22;
+                a[q1][q2] = s2 * u21 + c2 * u22;
             }
-            __syncthreads();
-            // columns: A <- A J, V <- V J; the annihilated pair (p, q) is set
-            // to exactly zero by the lanes of rows p and q
+            // V <- V J
             for (int e = t; e < half * n; e += (int)blockDim.x) {
                 const int pi = e / n, j = e % n;
                 const int p = pp[pi], q = qq[pi];
                 if (p == q) continue;
                 const double c = cs[pi], s = sn[pi];
-                const double ajp = a[j][p], ajq = a[j][q];
-                a[j][p] = (j == q) ? 0.0 : c * ajp - s * ajq;
-                a[j][q] = (j == p) ? 0.0 : s * ajp + c * ajq;
                 const double vjp = v[j][p], vjq = v[j][q];
                 v[j][p] = c * vjp - s * vjq;
                 v[j][q] = s * vjp + c * vjq;
@@ -169,14 +185,15 @@ __device__ bool cholesky_lds(double (*l)[LDA], int n, int* flag) {
     return *flag == 0;
 }
 
-__global__ __launch_bounds__(kThreads) void k_sym_eig(int n, const double* __restrict__ ain,
+template <int T>
+__global__ __launch_bounds__(T) void k_sym_eig(int n, const double* __restrict__ ain,
                                                        int lda, double* __restrict__ w,
                                                        double* __restrict__ out) {
     __shared__ double a[N][LDA];
     __shared__ double v[N][LDA];
     __shared__ double cs[N / 2], sn[N / 2];
     __shared__ int pp[N / 2], qq[N / 2], order[N];
-    __shared__ double red[2 * kThreads];
+    __shared__ double red[2 * (T / 64)];
     const int t = threadIdx.x;
     for (int e = t; e < n * n; e += (int)blockDim.x) {
         const int i = e / n, j = e % n;
@@ -194,7 +211,8 @@ __global__ __launch_bounds__(kThreads) void k_sym_eig(int n, const double* __res
 // returned vectors are D c'.
 // Batched over blockIdx.x with strides (in doubles) between problems: problem
 // p reads ain + p*sa, bin + p*sb and writes w + p*n, cout + p*n*n, status[p].
-__global__ __launch_bounds__(kThreads) void k_gen_eig(int n, const double* __restrict__ ain,
+template <int T>
+__global__ __launch_bounds__(T) void k_gen_eig(int n, const double* __restrict__ ain,
                                                        int lda, const double* __restrict__ bin,
                                                        int ldb, int scale,
                                                        double* __restrict__ w,
@@ -212,7 +230,7 @@ __global__ __launch_bounds__(kThreads) void k_gen_eig(int n, const double* __res
     __shared__ double v[N][LDA];
     __shared__ double cs[N / 2], sn[N / 2];
     __shared__ int pp[N / 2], qq[N / 2], order[N];
-    __shared__ double red[2 * kThreads];
+    __shared__ double red[2 * (T / 64)];
     __shared__ double dsc[N];
     __shared__ int flag;
     const int t = threadIdx.x;
@@ -344,7 +362,8 @@ __global__ void k_svqb(int n, const double* __restrict__ w, const double* __rest
 // block: block b covers rows [seg_row0[b], seg_row0[b] + seg_n[b]) (<= 64 rows,
 // columns inside the same range).  The k smallest eigenvalues go to
 // evals[b*k + j], their vectors to rows of evecs (leading dimension ldv).
-__global__ __launch_bounds__(kThreads) void k_dense_seg_eig(
+template <int T>
+__global__ __launch_bounds__(T) void k_dense_seg_eig(
     const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
     const double* __restrict__ l64, const int64_t* __restrict__ seg_row0,
     const int32_t* __restrict__ seg_n, int k, double* __restrict__ evals,
@@ -353,7 +372,7 @@ __global__ __launch_bounds__(kThreads) void k_dense_seg_eig(
     __shared__ double v[N][LDA];
     __shared__ double cs[N / 2], sn[N / 2];
     __shared__ int pp[N / 2], qq[N / 2], order[N];
-    __shared__ double red[2 * kThreads];
+    __shared__ double red[2 * (T / 64)];
     const int t = threadIdx.x;
     const int64_t r0 = seg_row0[blockIdx.x];
     const int n = seg_n[blockIdx.x];
@@ -414,23 +433,32 @@ void small_copy_sub(int rows, int cols, const double* src, int lds, double* dst,
 
 void small_sym_eig(int n, const double* a, int lda, double* w, double* v, hipStream_t s) {
     SI_CHECK(n >= 1 && n <= kSmallMax, kErrInvalidArgument, "small_sym_eig: n out of range");
-    hipLaunchKernelGGL(k_sym_eig, dim3(1), dim3(small_threads(n)), 0, s, n, a, lda, w, v);
+    by_eig_threads(n, [&](auto t) {
+        constexpr int T = decltype(t)::value;
+        hipLaunchKernelGGL((k_sym_eig<T>), dim3(1), dim3(T), 0, s, n, a, lda, w, v);
+    });
     SI_LAUNCH_CHECK();
 }
 
 void small_gen_eig(int n, const double* a, const double* b, double* w, double* c, int* status,
                    hipStream_t s) {
     SI_CHECK(n >= 1 && n <= kSmallMax, kErrInvalidArgument, "small_gen_eig: n out of range");
-    hipLaunchKernelGGL(k_gen_eig, dim3(1), dim3(small_threads(n)), 0, s, n, a, n, b, n, 0, w, c, status,
-                       (int64_t)0, (int64_t)0, (const int32_t*)nullptr);
+    by_eig_threads(n, [&](auto t) {
+        constexpr int T = decltype(t)::value;
+        hipLaunchKernelGGL((k_gen_eig<T>), dim3(1), dim3(T), 0, s, n, a, n, b, n, 0, w, c, status,
+                           (int64_t)0, (int64_t)0, (const int32_t*)nullptr);
+    });
     SI_LAUNCH_CHECK();
 }
 
 void small_gen_eig_scaled(int n, const double* a, int lda, const double* b, int ldb, double* w,
                           double* c, int* status, hipStream_t s) {
     SI_CHECK(n >= 1 && n <= kSmallMax, kErrInvalidArgument, "small_gen_eig: n out of range");
-    hipLaunchKernelGGL(k_gen_eig, dim3(1), dim3(small_threads(n)), 0, s, n, a, lda, b, ldb, 1, w, c, status,
-                       (int64_t)0, (int64_t)0, (const int32_t*)nullptr);
+    by_eig_threads(n, [&](auto t) {
+        constexpr int T = decltype(t)::value;
+        hipLaunchKernelGGL((k_gen_eig<T>), dim3(1), dim3(T), 0, s, n, a, lda, b, ldb, 1, w, c, status,
+                           (int64_t)0, (int64_t)0, (const int32_t*)nullptr);
+    });
     SI_LAUNCH_CHECK();
 }
 void small_gen_eig_scaled_batched(int count, int n, const double* a, int lda, int64_t sa,
@@ -438,15 +466,19 @@ void small_gen_eig_scaled_batched(int count, int n, const double* a, int lda, in
                                   int* status, hipStream_t s, const int32_t* gate) {
     SI_CHECK(n >= 1 && n <= kSmallMax && count >= 1, kErrInvalidArgument,
              "small_gen_eig_batched: size out of range");
-    hipLaunchKernelGGL(k_gen_eig, dim3(count), dim3(small_threads(n)), 0, s, n, a, lda, b, ldb, 1, w, c,
-                       status, sa, sb, gate);
+    by_eig_threads(n, [&](auto t) {
+        constexpr int T = decltype(t)::value;
+        hipLaunchKernelGGL((k_gen_eig<T>), dim3(count), dim3(T), 0, s, n, a, lda, b, ldb, 1, w, c,
+                           status, sa, sb, gate);
+    });
     SI_LAUNCH_CHECK();
 }
 void dense_seg_eig(const DevLaplacian& L, int count, const int64_t* seg_row0, const int32_t* seg_n,
                    int k, double* evals, double* evecs, int ldv, hipStream_t s) {
     if (count <= 0) return;
-    hipLaunchKernelGGL(k_dense_seg_eig, dim3(count), dim3(kThreads), 0, s, L.rowptr, L.col, L.l64,
-                       seg_row0, seg_n, k, evals, evecs, ldv);
+    // segments of up to 64 rows: the wide block
+    hipLaunchKernelGGL((k_dense_seg_eig<kWide>), dim3(count), dim3(kWide), 0, s, L.rowptr, L.col,
+                       L.l64, seg_row0, seg_n, k, evals, evecs, ldv);
     SI_LAUNCH_CHECK();
 }
 
