@@ -34,27 +34,7 @@ __global__ void k_convert_rowptr(const int64_t* __restrict__ ip64, int32_t* __re
 }
 
 // One thread per row: bounds, finiteness / sign of weights, ascending columns.
-__global__ void k_validate(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                           const float* __restrict__ w, int64_t n, int* __restrict__ flags,
-                           int32_t* __restrict__ max_len) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int32_t b = rp[i], e = rp[i + 1];
-    if (e < b) { atomicOr(&flags[kFlagIndex], 1); return; }
-    atomicMax(max_len, e - b);
-    int32_t prev = -1;
-    for (int32_t p = b; p < e; ++p) {
-        int32_t j = col[p];
-        if (j < 0 || j >= n) { atomicOr(&flags[kFlagIndex], 1); return; }
-        float v = w[p];
-        if (!(v >= 0.0f) || isinf(v)) atomicOr(&flags[kFlagWeight], 1);
-        if (j <= prev) atomicOr(&flags[kFlagUnsorted], 1);
-        prev = j;
-    }
-}
-
-// One thread per row: each entry (i, j, w) must have a bitwise-equal (j, i, w).
-// The row kernels below give kRowLanes lanes to a row; a workgroup takes
+// The row kernels give kRowLanes lanes to a row; a workgroup takes
 // 256 / kRowLanes consecutive rows at a time, grid-strided over at most
 // kRowGrid workgroups.
 constexpr int kRowLanes = 16;
@@ -65,18 +45,75 @@ inline dim3 row_grid(int64_t n) {
     return dim3((unsigned)std::min<int64_t>(ceil_div(n, kRowsPerBlock), kRowGrid));
 }
 
+// Row pointers and column indices in range, weights finite and non-negative,
+// columns strictly increasing per row (else the unsorted flag); the longest
+// row goes to *max_len (one atomic per workgroup).
+__global__ __launch_bounds__(256) void k_validate(const int32_t* __restrict__ rp,
+                                                  const int32_t* __restrict__ col,
+                                                  const float* __restrict__ w, int64_t n,
+                                                  int* __restrict__ flags,
+                                                  int32_t* __restrict__ max_len) {
+    __shared__ int32_t wmax[4];
+    const int sub = threadIdx.x & (kRowLanes - 1);
+    int32_t longest = 0;
+    for (int64_t base = (int64_t)blockIdx.x * kRowsPerBlock; base < n;
+         base += (int64_t)gridDim.x * kRowsPerBlock) {
+        const int64_t i = base + threadIdx.x / kRowLanes;
+        if (i >= n) continue;
+        const int32_t b = rp[i], e = rp[i + 1];
+        if (e < b) {
+            if (sub == 0) atomicOr(&flags[kFlagIndex], 1);
+            continue;
+        }
+        longest = max(longest, e - b);
+        for (int32_t p = b + sub; p < e; p += kRowLanes) {
+            const int32_t j = col[p];
+            if (j < 0 || j >= n) {
+                atomicOr(&flags[kFlagIndex], 1);
+                break;
+            }
+            const float v = w[p];
+            if (!(v >= 0.0f) || isinf(v)) atomicOr(&flags[kFlagWeight], 1);
+            if (p > b && j <= col[p - 1]) atomicOr(&flags[kFlagUnsorted], 1);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) longest = max(longest, __shfl_xor(longest, o));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = longest;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        longest = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+        if (longest > 0) atomicMax(max_len, longest);
+    }
+}
+
+// Each entry (i, j, w) must have a bitwise-equal (j, i, w).  Rows are strictly
+// increasing here (no repeated columns), so it suffices that every entry
+// above the diagonal finds its mirror and that the entries above and below
+// the diagonal are equally many: the mirror map from the upper entries into
+// the lower ones is then one-to-one and onto.  tri[0] / tri[1] count the
+// upper / lower entries (one atomic each per workgroup).
 __global__ __launch_bounds__(256) void k_check_symmetric(const int32_t* __restrict__ rp,
                                                          const int32_t* __restrict__ col,
                                                          const float* __restrict__ w, int64_t n,
-                                                         int* __restrict__ flags) {
+                                                         int* __restrict__ flags,
+                                                         int* __restrict__ tri) {
+    __shared__ int wsum[2][4];
     const int sub = threadIdx.x & (kRowLanes - 1);
-    for (int64_t base = (int64_t)blockIdx.x * kRowsPerBlock; base < n;
+    int upper = 0, lower = 0;
+    bool bad = false;
+    for (int64_t base = (int64_t)blockIdx.x * kRowsPerBlock; base < n && !bad;
          base += (int64_t)gridDim.x * kRowsPerBlock) {
         const int64_t i = base + threadIdx.x / kRowLanes;
         if (i >= n) continue;
         const int32_t e = rp[i + 1];
         for (int32_t p = rp[i] + sub; p < e; p += kRowLanes) {
             const int32_t j = col[p];
+            if (j < i) {
+                ++lower;
+                continue;
+            }
+            if (j == i) continue;
+            ++upper;
             int32_t lo = rp[j], hi = rp[j + 1] - 1;
             bool found = false;
             while (lo <= hi) {
@@ -90,10 +127,24 @@ __global__ __launch_bounds__(256) void k_check_symmetric(const int32_t* __restri
                 else hi = mid - 1;
             }
             if (!found) {
-                atomicOr(&flags[kFlagAsym], 1);
-                return;
+                bad = true;
+                break;
             }
         }
+    }
+    if (bad) atomicOr(&flags[kFlagAsym], 1);
+    for (int o = 32; o > 0; o >>= 1) {
+        upper += __shfl_xor(upper, o);
+        lower += __shfl_xor(lower, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        wsum[0][threadIdx.x >> 6] = upper;
+        wsum[1][threadIdx.x >> 6] = lower;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(&tri[0], wsum[0][0] + wsum[0][1] + wsum[0][2] + wsum[0][3]);
+        atomicAdd(&tri[1], wsum[1][0] + wsum[1][1] + wsum[1][2] + wsum[1][3]);
     }
 }
 
@@ -205,19 +256,20 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
     SI_HIP(hipMalloc(&L.isd, sizeof(double) * n));
     SI_HIP(hipMalloc(&L.long_rows, sizeof(int32_t) * (n > 0 ? n : 1)));
     int* d_flags = nullptr;
-    SI_HIP(hipMalloc(&d_flags, sizeof(int) * (kNumFlags + 2)));
-    SI_HIP(hipMemsetAsync(d_flags, 0, sizeof(int) * (kNumFlags + 2), s));
+    SI_HIP(hipMalloc(&d_flags, sizeof(int) * (kNumFlags + 4)));
+    SI_HIP(hipMemsetAsync(d_flags, 0, sizeof(int) * (kNumFlags + 4), s));
     int32_t* d_maxlen = d_flags + kNumFlags;
     int* d_nlong = d_flags + kNumFlags + 1;
+    int* d_tri = d_flags + kNumFlags + 2;  // entries above / below the diagonal
 
     hipLaunchKernelGGL(k_convert_rowptr, dim3(ceil_div(n + 1, T)), dim3(T), 0, s, d_indptr,
                        L.rowptr, n + 1);
     if (nnz > 0) SI_HIP(hipMemcpyAsync(L.col, d_indices, sizeof(int32_t) * nnz,
                                        hipMemcpyDeviceToDevice, s));
-    hipLaunchKernelGGL(k_validate, dim3(gn), dim3(T), 0, s, L.rowptr, L.col, d_w, n, d_flags,
+    hipLaunchKernelGGL(k_validate, row_grid(n), dim3(T), 0, s, L.rowptr, L.col, d_w, n, d_flags,
                        d_maxlen);
     SI_LAUNCH_CHECK();
-    int flags[kNumFlags + 2];
+    int flags[kNumFlags + 4];
     SI_HIP(hipMemcpyAsync(flags, d_flags, sizeof(flags), hipMemcpyDeviceToHost, s));
     SI_HIP(hipStreamSynchronize(s));
     if (flags[kFlagIndex]) {
@@ -235,12 +287,12 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
     int sorted = flags[kFlagUnsorted] == 0;
     if (sorted) {
         hipLaunchKernelGGL(k_check_symmetric, row_grid(n), dim3(T), 0, s, L.rowptr, L.col, d_w, n,
-                           d_flags);
+                           d_flags, d_tri);
         SI_LAUNCH_CHECK();
-        SI_HIP(hipMemcpyAsync(flags, d_flags, sizeof(int) * kNumFlags, hipMemcpyDeviceToHost, s));
+        SI_HIP(hipMemcpyAsync(flags, d_flags, sizeof(flags), hipMemcpyDeviceToHost, s));
         SI_HIP(hipStreamSynchronize(s));
     }
-    L.symmetric = sorted && flags[kFlagAsym] == 0;
+    L.symmetric = sorted && flags[kFlagAsym] == 0 && flags[kNumFlags + 2] == flags[kNumFlags + 3];
     phase("symmetry check");
 
     if (mode == 0) {

@@ -19,8 +19,9 @@ __global__ void k_perm_lens(int64_t n, const int64_t* __restrict__ rp,
     lens[i] = (int32_t)(rp[v + 1] - rp[v]);
 }
 
-// new row i = original row perm[i], columns renumbered through iperm; one
-// wavefront per row, lanes over its entries (order kept; sorted afterwards)
+// new row i = original row perm[i], columns renumbered through iperm (order
+// kept; sorted afterwards); 16 lanes per row, 16 rows per workgroup at a time,
+// grid-strided
 __global__ __launch_bounds__(256) void k_perm_fill(int64_t n, const int64_t* __restrict__ rp,
                                                    const int32_t* __restrict__ col,
                                                    const float* __restrict__ w,
@@ -28,14 +29,17 @@ __global__ __launch_bounds__(256) void k_perm_fill(int64_t n, const int64_t* __r
                                                    const int32_t* __restrict__ iperm,
                                                    const int32_t* __restrict__ rp2,
                                                    int32_t* __restrict__ c2, float* __restrict__ w2) {
-    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int lane = threadIdx.x & 63;
-    if (i >= n) return;
-    const int32_t v = perm[i];
-    const int32_t b = (int32_t)rp[v], e = (int32_t)rp[v + 1], o = rp2[i];
-    for (int32_t p = b + lane; p < e; p += 64) {
-        c2[o + (p - b)] = iperm[col[p]];
-        w2[o + (p - b)] = w[p];
+    const int sub = threadIdx.x & 15;
+    for (int64_t base = (int64_t)blockIdx.x * 16; base < n; base += (int64_t)gridDim.x * 16) {
+        const int64_t i = base + (threadIdx.x >> 4);
+        if (i >= n) continue;
+        const int32_t v = perm[i];
+        const int64_t b = rp[v], e = rp[v + 1];
+        const int64_t o = rp2[i] - b;
+        for (int64_t p = b + sub; p < e; p += 16) {
+            c2[o + p] = iperm[col[p]];
+            w2[o + p] = w[p];
+        }
     }
 }
 
@@ -60,7 +64,8 @@ void permute_csr_device(int64_t n, int64_t nnz, const int64_t* rp, const int32_t
     }
     DevBuf<int32_t> ck(nnz);
     DevBuf<float> wk(nnz);
-    hipLaunchKernelGGL(k_perm_fill, dim3((unsigned)ceil_div(n * 64, 256)), dim3(256), 0, s, n, rp,
+    hipLaunchKernelGGL(k_perm_fill, dim3((unsigned)std::min<int64_t>(ceil_div(n, 16), 4096)),
+                       dim3(256), 0, s, n, rp,
                        col, w, perm, iperm, rp2, ck.get(), wk.get());
     SI_LAUNCH_CHECK();
     // sort every row by its new column numbers (segments = the new rows)

@@ -89,6 +89,37 @@ def test_laplacian_asymmetric_input_falls_back(ctx):
     assert not lap.degree_exact
 
 
+def _csr_of(n, entries):
+    import scipy.sparse as sp
+    r, c, v = (np.array(t) for t in zip(*entries))
+    a = sp.csr_matrix((v.astype(np.float32), (r, c)), shape=(n, n))
+    a.sort_indices()
+    return a.indptr.astype(np.int64), a.indices.astype(np.int32), a.data.astype(np.float32)
+
+
+def test_symmetry_check_needs_both_triangles_and_equal_weights(ctx):
+    """The device symmetry check searches the mirror of every entry above the
+    diagonal and compares the entry counts of the two triangles: an extra entry
+    below the diagonal, or a mirror with another weight, makes the pattern
+    asymmetric (column-sum degrees); the symmetric graph keeps row sums."""
+    print("case symmetry check triangles", flush=True)
+    base = [(0, 1, 0.5), (1, 0, 0.5), (1, 2, 0.25), (2, 1, 0.25), (2, 3, 1.0), (3, 2, 1.0),
+            (3, 3, 2.0)]
+    cases = {
+        "symmetric": (base, True),
+        "extra_lower": (base + [(3, 0, 0.75)], False),
+        "extra_upper": (base + [(0, 3, 0.75)], False),
+        "weight_mismatch": ([(0, 1, 0.5), (1, 0, 0.125)] + base[2:], False),
+    }
+    for name, (entries, sym) in cases.items():
+        ip, ix, w = _csr_of(4, entries)
+        lap = si.Laplacian(ctx, ip, ix, w)
+        deg, _, _ = lap.export()
+        odeg, _ = O.compute_degrees(ip, ix, w, 0)
+        np.testing.assert_allclose(deg, odeg, rtol=1e-7, err_msg=name)
+        assert bool(lap.degree_exact) == sym, name
+
+
 # ------------------------------------------------------------------ a3 / a4
 GRAPHS = {
     "blobs_connected_2000": lambda: tuple(load_fixture("blobs_connected_2000")[k]
