@@ -19,27 +19,74 @@ __global__ void k_perm_lens(int64_t n, const int64_t* __restrict__ rp,
     lens[i] = (int32_t)(rp[v + 1] - rp[v]);
 }
 
-// new row i = original row perm[i], columns renumbered through iperm (order
-// kept; sorted afterwards); 16 lanes per row, 16 rows per workgroup at a time,
-// grid-strided
-__global__ __launch_bounds__(256) void k_perm_fill(int64_t n, const int64_t* __restrict__ rp,
+// new row i = original row perm[i], columns renumbered through iperm and
+// sorted (stable: equal columns keep their order); one wavefront per row,
+// grid-strided.  A row of up to 64 entries is ranked in registers; a longer
+// row is copied unsorted and listed in longs[1 ..] (count in longs[0]) for
+// the segmented sort.
+__global__ __launch_bounds__(256) void k_perm_rows(int64_t n, const int64_t* __restrict__ rp,
                                                    const int32_t* __restrict__ col,
                                                    const float* __restrict__ w,
                                                    const int32_t* __restrict__ perm,
                                                    const int32_t* __restrict__ iperm,
                                                    const int32_t* __restrict__ rp2,
-                                                   int32_t* __restrict__ c2, float* __restrict__ w2) {
-    const int sub = threadIdx.x & 15;
-    for (int64_t base = (int64_t)blockIdx.x * 16; base < n; base += (int64_t)gridDim.x * 16) {
-        const int64_t i = base + (threadIdx.x >> 4);
-        if (i >= n) continue;
+                                                   int32_t* __restrict__ c2, float* __restrict__ w2,
+                                                   int32_t* __restrict__ longs) {
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < n; i += waves) {
         const int32_t v = perm[i];
-        const int64_t b = rp[v], e = rp[v + 1];
-        const int64_t o = rp2[i] - b;
-        for (int64_t p = b + sub; p < e; p += 16) {
-            c2[o + p] = iperm[col[p]];
-            w2[o + p] = w[p];
+        const int64_t b = rp[v];
+        const int32_t len = (int32_t)(rp[v + 1] - b);
+        const int64_t o = rp2[i];
+        if (len <= 64) {
+            int32_t key = INT32_MAX;
+            float val = 0.0f;
+            if (lane < len) {
+                key = iperm[col[b + lane]];
+                val = w[b + lane];
+            }
+            int32_t r = 0;
+            for (int t = 0; t < len; ++t) {
+                const int32_t kt = __shfl(key, t);
+                r += (kt < key || (kt == key && t < lane)) ? 1 : 0;
+            }
+            if (lane < len) {
+                c2[o + r] = key;
+                w2[o + r] = val;
+            }
+        } else {
+            for (int32_t p = lane; p < len; p += 64) {
+                c2[o + p] = iperm[col[b + p]];
+                w2[o + p] = w[b + p];
+            }
+            if (lane == 0) longs[1 + atomicAdd(&longs[0], 1)] = (int32_t)i;
         }
+    }
+}
+
+// bounds of the listed long rows in the new CSR
+__global__ void k_long_bounds(int32_t count, const int32_t* __restrict__ longs,
+                              const int32_t* __restrict__ rp2, int32_t* __restrict__ beg,
+                              int32_t* __restrict__ end) {
+    const int32_t j = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j >= count) return;
+    const int32_t i = longs[1 + j];
+    beg[j] = rp2[i];
+    end[j] = rp2[i + 1];
+}
+
+// sorted long rows back into the new CSR; one wavefront per row
+__global__ __launch_bounds__(256) void k_long_copy(int32_t count, const int32_t* __restrict__ beg,
+                                                   const int32_t* __restrict__ end,
+                                                   const int32_t* __restrict__ ck,
+                                                   const float* __restrict__ wk,
+                                                   int32_t* __restrict__ c2, float* __restrict__ w2) {
+    const int64_t j = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (j >= count) return;
+    for (int32_t p = beg[j] + (int32_t)(threadIdx.x & 63); p < end[j]; p += 64) {
+        c2[p] = ck[p];
+        w2[p] = wk[p];
     }
 }
 
@@ -62,19 +109,34 @@ void permute_csr_device(int64_t n, int64_t nnz, const int64_t* rp, const int32_t
         SI_HIP(hipStreamSynchronize(s));
         return;
     }
-    DevBuf<int32_t> ck(nnz);
-    DevBuf<float> wk(nnz);
-    hipLaunchKernelGGL(k_perm_fill, dim3((unsigned)std::min<int64_t>(ceil_div(n, 16), 4096)),
-                       dim3(256), 0, s, n, rp,
-                       col, w, perm, iperm, rp2, ck.get(), wk.get());
+    DevBuf<int32_t> longs(n + 1);
+    SI_HIP(hipMemsetAsync(longs.get(), 0, sizeof(int32_t), s));
+    hipLaunchKernelGGL(k_perm_rows, dim3((unsigned)std::min<int64_t>(ceil_div(n, 4), 4096)),
+                       dim3(256), 0, s, n, rp, col, w, perm, iperm, rp2, c2, w2, longs.get());
     SI_LAUNCH_CHECK();
-    // sort every row by its new column numbers (segments = the new rows)
-    size_t tb2 = 0;
-    SI_HIP(hipcub::DeviceSegmentedSort::StableSortPairs(nullptr, tb2, ck.get(), c2, wk.get(), w2,
-                                                        (int)nnz, (int)n, rp2, rp2 + 1, s));
-    DevBuf<char> tmp2(tb2);
-    SI_HIP(hipcub::DeviceSegmentedSort::StableSortPairs(tmp2.get(), tb2, ck.get(), c2, wk.get(), w2,
-                                                        (int)nnz, (int)n, rp2, rp2 + 1, s));
+    int32_t count = 0;
+    SI_HIP(hipMemcpyAsync(&count, longs.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    SI_HIP(hipStreamSynchronize(s));
+    SI_CHECK(count >= 0 && count <= n, kErrInternal, "permute_csr: bad long-row count");
+    if (count > 0) {
+        // rows above 64 entries: a segmented sort of just those rows
+        DevBuf<int32_t> beg(count), end(count), ck(nnz);
+        DevBuf<float> wk(nnz);
+        hipLaunchKernelGGL(k_long_bounds, dim3((unsigned)ceil_div(count, 256)), dim3(256), 0, s, count,
+                           longs.get(), rp2, beg.get(), end.get());
+        SI_LAUNCH_CHECK();
+        size_t tb2 = 0;
+        SI_HIP(hipcub::DeviceSegmentedSort::StableSortPairs(nullptr, tb2, c2, ck.get(), w2, wk.get(),
+                                                            (int)nnz, (int)count, beg.get(),
+                                                            end.get(), s));
+        DevBuf<char> tmp2(tb2);
+        SI_HIP(hipcub::DeviceSegmentedSort::StableSortPairs(tmp2.get(), tb2, c2, ck.get(), w2,
+                                                            wk.get(), (int)nnz, (int)count,
+                                                            beg.get(), end.get(), s));
+        hipLaunchKernelGGL(k_long_copy, dim3((unsigned)ceil_div((int64_t)count * 64, 256)), dim3(256),
+                           0, s, count, beg.get(), end.get(), ck.get(), wk.get(), c2, w2);
+        SI_LAUNCH_CHECK();
+    }
     SI_HIP(hipStreamSynchronize(s));
 }
 
