@@ -149,18 +149,19 @@ struct CentroidJob {
     ~CentroidJob() { join(); }
 };
 
+// the caller's CSR on the device for one build (stream-ordered scratch)
 struct DeviceGraph {
-    si::DevBuf<int64_t> ip;
-    si::DevBuf<int32_t> ix;
-    si::DevBuf<float> w;
+    si::StreamBuf<int64_t> ip;
+    si::StreamBuf<int32_t> ix;
+    si::StreamBuf<float> w;
 };
 
 void upload_graph(DeviceGraph& g, int64_t n, const int64_t* ip, const int32_t* ix, const float* w,
                   hipStream_t s) {
     const int64_t nnz = ip[n];
-    g.ip.alloc(n + 1);
-    g.ix.alloc(nnz);
-    g.w.alloc(nnz);
+    g.ip.alloc(n + 1, s);
+    g.ix.alloc(nnz, s);
+    g.w.alloc(nnz, s);
     SI_HIP(hipMemcpyAsync(g.ip.get(), ip, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, s));
     if (nnz > 0) {
         SI_HIP(hipMemcpyAsync(g.ix.get(), ix, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s));

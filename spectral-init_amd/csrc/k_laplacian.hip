@@ -237,12 +237,15 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
     L.nnz = nnz;
     L.degree_mode = mode;
     auto tp = std::chrono::steady_clock::now();
+    double alloc0 = devbuf_alloc_ms();
     auto phase = [&](const char* what) {  // SI_TRACE=1: build phase timings
         if (!trace_on()) return;
         SI_HIP(hipStreamSynchronize(s));
         const auto now = std::chrono::steady_clock::now();
-        std::fprintf(stderr, "[build] %-24s %8.1f ms\n", what,
-                     std::chrono::duration<double, std::milli>(now - tp).count());
+        std::fprintf(stderr, "[build] %-24s %8.1f ms (device allocations %.1f ms)\n", what,
+                     std::chrono::duration<double, std::milli>(now - tp).count(),
+                     devbuf_alloc_ms() - alloc0);
+        alloc0 = devbuf_alloc_ms();
         tp = now;
     };
     const int T = 256;
@@ -326,7 +329,7 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
         SI_HIP(hipMalloc(&L.iperm, sizeof(int32_t) * n));
         if (L.symmetric) {
             SI_HIP(hipMalloc(&L.comp, sizeof(int32_t) * n));
-            DevBuf<int32_t> root(n);
+            StreamBuf<int32_t> root(n, s);
             L.n_comp = components_device(n, d_indptr, d_indices, L.comp, root.get(), s);
             phase("components (device)");
             // k-NN-like graphs only: a mesh-like graph (average degree below

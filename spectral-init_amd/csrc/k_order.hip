@@ -155,9 +155,9 @@ void cluster_order_device(const DevLaplacian& L, int32_t* perm, int32_t* iperm, 
     const int64_t n = L.n;
     SI_CHECK(n >= 2 * kK && n < (int64_t)INT32_MAX, kErrInternal, "cluster order: size out of range");
     const size_t nm = (size_t)n * kE;
-    DevBuf<float> xa(nm), xb(nm), v0(n);
-    DevBuf<double> small(kE * kE * 2 + 64);
-    DevBuf<double> ws(gram_workspace_doubles(n, kE, kE));
+    StreamBuf<float> xa(nm, s), xb(nm, s), v0(n, s);
+    StreamBuf<double> small(kE * kE * 2 + 64, s);
+    StreamBuf<double> ws(gram_workspace_doubles(n, kE, kE), s);
     double* g = small.get();            // 16 x 16 Gram / transforms
     double* c = g + kE * kE;            // second small matrix
     std::vector<double> hg(kE * kE), ht, hid(kE * kE, 0.0);
@@ -200,9 +200,9 @@ void cluster_order_device(const DevLaplacian& L, int32_t* perm, int32_t* iperm, 
     SI_LAUNCH_CHECK();
     const float* pts = y;
     // k-means
-    DevBuf<float> cent(kK * kE);
-    DevBuf<unsigned long long> acc(kK * kE + kK);
-    DevBuf<int32_t> lab(n), ids(n), lab2(n);
+    StreamBuf<float> cent(kK * kE, s);
+    StreamBuf<unsigned long long> acc(kK * kE + kK, s);
+    StreamBuf<int32_t> lab(n, s), ids(n, s), lab2(n, s);
     hipLaunchKernelGGL(k_ord_seed, grid_of(kK * kE), dim3(256), 0, s, n, pts, cent.get());
     SI_LAUNCH_CHECK();
     const unsigned blocks = (unsigned)std::min<int64_t>(1024, ceil_div(n, 256));
@@ -233,7 +233,7 @@ void cluster_order_device(const DevLaplacian& L, int32_t* perm, int32_t* iperm, 
     size_t tb = 0;
     SI_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, lab.get(), lab2.get(), ids.get(), perm,
                                               (int)n, 0, 8, s));
-    DevBuf<char> tmp(tb);
+    StreamBuf<char> tmp(tb, s);
     SI_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.get(), tb, lab.get(), lab2.get(), ids.get(), perm,
                                               (int)n, 0, 8, s));
     hipLaunchKernelGGL(k_ord_inverse, grid_of(n), dim3(256), 0, s, n, perm, iperm);

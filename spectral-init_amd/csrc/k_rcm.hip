@@ -426,8 +426,8 @@ inline dim3 grid_of(int64_t threads) { return dim3((unsigned)ceil_div(threads, 2
 int64_t components_device(int64_t n, const int64_t* rp, const int32_t* ix, int32_t* lab,
                           int32_t* root, hipStream_t s) {
     SI_CHECK(n >= 1 && n < (int64_t)INT32_MAX, kErrUnsupported, "components: n out of range");
-    DevBuf<int32_t> f(n), rk(n + 1);
-    DevBuf<int> flag(1);
+    StreamBuf<int32_t> f(n, s), rk(n + 1, s);
+    StreamBuf<int> flag(1, s);
     hipLaunchKernelGGL(k_cc_init, grid_of(n), dim3(256), 0, s, n, f.get());
     SI_LAUNCH_CHECK();
     // every changing round lowers some pointer, so n rounds bound the loop; in
@@ -446,11 +446,11 @@ int64_t components_device(int64_t n, const int64_t* rp, const int32_t* ix, int32
         SI_HIP(hipStreamSynchronize(s));
         if (!ch) break;
     }
-    DevBuf<int32_t> isroot(n + 1);
+    StreamBuf<int32_t> isroot(n + 1, s);
     hipLaunchKernelGGL(k_cc_roots, grid_of(n + 1), dim3(256), 0, s, n, f.get(), isroot.get());
     size_t tb = 0;
     SI_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, isroot.get(), rk.get(), (int)(n + 1), s));
-    DevBuf<char> tmp(tb);
+    StreamBuf<char> tmp(tb, s);
     SI_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.get(), tb, isroot.get(), rk.get(), (int)(n + 1), s));
     hipLaunchKernelGGL(k_cc_label, grid_of(n), dim3(256), 0, s, n, f.get(), rk.get(), lab, root);
     SI_LAUNCH_CHECK();
@@ -463,7 +463,7 @@ int64_t components_device(int64_t n, const int64_t* rp, const int32_t* ix, int32
 int64_t rcm_order_device(int64_t n, const int64_t* rp, const int32_t* ix, int32_t* perm,
                          int32_t* iperm, hipStream_t s, int32_t* lab_out) {
     SI_CHECK(n >= 1 && n < (int64_t)INT32_MAX, kErrUnsupported, "rcm: n out of range");
-    DevBuf<int32_t> lab_own(lab_out ? 0 : n), root(n);
+    StreamBuf<int32_t> lab_own(lab_out ? 0 : n, s), root(n, s);
     int32_t* lab = lab_out ? lab_out : lab_own.get();
     const int64_t nc = components_device(n, rp, ix, lab, root.get(), s);
     rcm_order_labeled(n, rp, ix, lab, root.get(), nc, perm, iperm, s);
@@ -474,9 +474,9 @@ void rcm_order_labeled(int64_t n, const int64_t* rp, const int32_t* ix, const in
                        int32_t* start, int64_t nc, int32_t* perm, int32_t* iperm, hipStream_t s) {
     SI_CHECK(n >= 1 && n < (int64_t)INT32_MAX && nc >= 1 && nc <= n, kErrUnsupported,
              "rcm: n out of range");
-    DevBuf<int32_t> queue(n), mark(n), lvl(n), cnt(n), loc(n), part(kLevelGrid), bnd(n + 2),
-        maxlev(nc);
-    DevBuf<unsigned long long> key(n), skey(n), best(nc);
+    StreamBuf<int32_t> queue(n, s), mark(n, s), lvl(n, s), cnt(n, s), loc(n, s),
+        part(kLevelGrid, s), bnd(n + 2, s), maxlev(nc, s);
+    StreamBuf<unsigned long long> key(n, s), skey(n, s), best(nc, s);
     DevBuf<int> flags(2);  // [0] a group above kGroupLds, [1] the longest row
     hipLaunchKernelGGL(k_fill_u64, grid_of(n), dim3(256), 0, s, key.get(), n,
                        (unsigned long long)kNoClaim);
@@ -492,9 +492,9 @@ void rcm_order_labeled(int64_t n, const int64_t* rp, const int32_t* ix, const in
     // above kGroupLds can need the segmented sort, and its sorted sweep then
     // reads back every level
     const bool long_rows = max_row > kGroupLds;
-    DevBuf<int32_t> off(long_rows ? n + 1 : 0);
-    DevBuf<unsigned long long> skey2(long_rows ? n : 0);
-    DevBuf<char> tsort;
+    StreamBuf<int32_t> off(long_rows ? n + 1 : 0, s);
+    StreamBuf<unsigned long long> skey2(long_rows ? n : 0, s);
+    StreamBuf<char> tsort;
     Level A{rp,        ix,        queue.get(), mark.get(), key.get(),    bnd.get(),   cnt.get(),
             loc.get(), part.get(), lvl.get(),  lab,        maxlev.get(), skey.get(), flags.get(),
             0};
@@ -546,7 +546,7 @@ void rcm_order_labeled(int64_t n, const int64_t* rp, const int32_t* ix, const in
                 SI_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(
                     nullptr, tb, skey.get(), skey2.get(), (int)total, (int)m, off.get(),
                     off.get() + 1, 0, 64, s));
-                if (tb > tsort.n) tsort.alloc(tb);
+                if (tb > tsort.n) tsort.alloc(tb, s);
                 SI_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(
                     tsort.get(), tb, skey.get(), skey2.get(), (int)total, (int)m, off.get(),
                     off.get() + 1, 0, 64, s));
@@ -577,11 +577,11 @@ void rcm_order_labeled(int64_t n, const int64_t* rp, const int32_t* ix, const in
     // a stable sort by component label gives the per-component orders in
     // component order
     const int32_t* order = queue.get();
-    DevBuf<int32_t> ql, ql2, sorted_q;
+    StreamBuf<int32_t> ql, ql2, sorted_q;
     if (nc > 1) {
-        ql.alloc(n);
-        ql2.alloc(n);
-        sorted_q.alloc(n);
+        ql.alloc(n, s);
+        ql2.alloc(n, s);
+        sorted_q.alloc(n, s);
         hipLaunchKernelGGL(k_queue_labels, grid_of(n), dim3(256), 0, s, n, queue.get(), lab,
                            ql.get());
         int bits = 1;
@@ -589,7 +589,7 @@ void rcm_order_labeled(int64_t n, const int64_t* rp, const int32_t* ix, const in
         size_t tb = 0;
         SI_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, ql.get(), ql2.get(), queue.get(),
                                                   sorted_q.get(), (int)n, 0, bits, s));
-        DevBuf<char> t(tb);
+        StreamBuf<char> t(tb, s);
         SI_HIP(hipcub::DeviceRadixSort::SortPairs(t.get(), tb, ql.get(), ql2.get(), queue.get(),
                                                   sorted_q.get(), (int)n, 0, bits, s));
         SI_HIP(hipStreamSynchronize(s));

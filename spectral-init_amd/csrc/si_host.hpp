@@ -3,6 +3,7 @@
 // the C ABI.
 #pragma once
 
+#include <chrono>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -12,6 +13,25 @@
 #include "si_device.hpp"
 
 namespace si {
+
+// SI_TRACE=1: wall time this thread spent in DevBuf's hipMalloc / hipFree.
+inline double& devbuf_alloc_ms() {
+    thread_local double ms = 0.0;
+    return ms;
+}
+
+struct AllocTimer {
+    std::chrono::steady_clock::time_point t0;
+    bool on;
+    AllocTimer() : on(trace_on()) {
+        if (on) t0 = std::chrono::steady_clock::now();
+    }
+    ~AllocTimer() {
+        if (on)
+            devbuf_alloc_ms() += std::chrono::duration<double, std::milli>(
+                                     std::chrono::steady_clock::now() - t0).count();
+    }
+};
 
 // Owning device buffer (hipMalloc'd, freed on destruction).
 template <typename T>
@@ -31,11 +51,60 @@ struct DevBuf {
     void alloc(size_t count) {
         release();
         if (count == 0) count = 1;
+        AllocTimer at;
         SI_HIP(hipMalloc(&p, sizeof(T) * count));
         n = count;
     }
     void ensure(size_t count) { if (count > n || p == nullptr) alloc(count); }
-    void release() { if (p) hipFree(p); p = nullptr; n = 0; }
+    void release() {
+        if (p) {
+            AllocTimer at;
+            hipFree(p);
+        }
+        p = nullptr;
+        n = 0;
+    }
+    T* get() const { return p; }
+    operator T*() const { return p; }
+};
+
+// The device's default memory pool keeps what it is given (its release
+// threshold raised once per device), so stream-ordered scratch is reused
+// across calls instead of being mapped again.
+void stream_pool_setup();
+
+// Stream-ordered scratch buffer: hipMallocAsync on stream s, hipFreeAsync on
+// the same stream at destruction, so a block is handed out again only to work
+// ordered after its last use on s.  For scratch that only work on s touches.
+template <typename T>
+struct StreamBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipStream_t st = nullptr;
+    StreamBuf() = default;
+    StreamBuf(size_t count, hipStream_t s) { alloc(count, s); }
+    StreamBuf(const StreamBuf&) = delete;
+    StreamBuf& operator=(const StreamBuf&) = delete;
+    ~StreamBuf() { release(); }
+    void alloc(size_t count, hipStream_t s) {
+        release();
+        if (count == 0) count = 1;
+        stream_pool_setup();
+        AllocTimer at;
+        void* q = nullptr;
+        SI_HIP(hipMallocAsync(&q, sizeof(T) * count, s));
+        p = static_cast<T*>(q);
+        n = count;
+        st = s;
+    }
+    void release() {
+        if (p) {
+            AllocTimer at;
+            hipFreeAsync(p, st);
+        }
+        p = nullptr;
+        n = 0;
+    }
     T* get() const { return p; }
     operator T*() const { return p; }
 };

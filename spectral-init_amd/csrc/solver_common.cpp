@@ -1,6 +1,8 @@
 // solver_common.cpp -- FP64 building blocks shared by the solvers: residual
 // norms, Rayleigh-Ritz refinement and the Chebyshev filter.
 #include <cmath>
+#include <cstdint>
+#include <mutex>
 
 #include "si_host.hpp"
 
@@ -31,6 +33,19 @@ void* pinned_stage(size_t bytes) {
         g_pin->n = bytes;
     }
     return g_pin->p;
+}
+
+void stream_pool_setup() {
+    constexpr int kMaxDevices = 64;
+    static std::once_flag once[kMaxDevices];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return;
+    std::call_once(once[dev], [dev] {
+        hipMemPool_t pool = nullptr;
+        if (hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess || pool == nullptr) return;
+        uint64_t keep = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    });
 }
 
 hipStream_t aux_stream(Context& cx) {

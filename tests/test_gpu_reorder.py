@@ -169,6 +169,34 @@ def _storage_matches(ctx, lap, ip, ix, w):
     return perm, order
 
 
+def test_permuted_rows_of_every_length(ctx):
+    # rows of up to 64 entries are ranked by one register per lane, rows of
+    # 65..256 by four, longer rows by the segmented sort of just those rows
+    print("case permuted csr row lengths", flush=True)
+    n = 6000
+    ip, ix = random_graph(n, 4, 21, False)
+    rows = [np.repeat(np.arange(n), np.diff(ip))]
+    cols = [ix.astype(np.int64)]
+    rng = np.random.default_rng(22)
+    for h, deg in enumerate((63, 64, 65, 128, 200, 255, 256, 257, 700)):
+        nb = rng.choice(np.arange(100, n), deg, replace=False)
+        rows.append(np.full(deg, h * 11))
+        cols.append(nb)
+    r, c = np.concatenate(rows), np.concatenate(cols)
+    keep = r != c
+    ip, ix = _sym(r[keep], c[keep], n)
+    w = rng.uniform(0.1, 1.0, ix.size).astype(np.float32)
+    a = sp.csr_matrix((w, ix, ip), shape=(n, n))
+    a = ((a + a.T) * 0.5).tocsr()
+    a.sort_indices()
+    ip, ix, w = a.indptr.astype(np.int64), a.indices.astype(np.int32), a.data.astype(np.float32)
+    lens = np.diff(ip)
+    assert lens.max() > 256 and ((lens > 64) & (lens <= 256)).sum() >= 5
+    lap = si.Laplacian(ctx, ip, ix, w)
+    perm, order = _storage_matches(ctx, lap, ip, ix, w)
+    assert order == si.ORDER_RCM
+
+
 def test_cluster_order_for_large_connected_graphs(ctx):
     print("case cluster order random_large", flush=True)
     ip, ix = random_graph(120_000, 6, 11, False)
