@@ -150,6 +150,59 @@ struct CentroidJob {
 };
 
 // the caller's CSR on the device for one build (stream-ordered scratch)
+// members[c] = the nodes of component c in increasing order (a counting sort
+// by label; on host threads, each over a contiguous range of nodes, when the
+// per-thread counts stay small).
+void member_lists(const std::vector<int64_t>& labels, int64_t nc,
+                  std::vector<std::vector<int64_t>>& members) {
+    const int64_t n = (int64_t)labels.size();
+    members.assign(nc, {});
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nth = (int)std::min<int64_t>({8, (int64_t)hw, std::max<int64_t>(1, n / 65536)});
+    if (nth <= 1 || nc > 65536) {
+        std::vector<int64_t> cnt(nc, 0);
+        for (int64_t i = 0; i < n; ++i) ++cnt[labels[i]];
+        for (int64_t c = 0; c < nc; ++c) members[c].reserve(cnt[c]);
+        for (int64_t i = 0; i < n; ++i) members[labels[i]].push_back(i);
+        return;
+    }
+    // thread t takes nodes [t n / nth, (t + 1) n / nth); its nodes of component
+    // c go after those of threads 0..t-1, so every list stays increasing
+    std::vector<int64_t> pos((size_t)nth * nc, 0);
+    auto on_threads = [&](auto&& body) {
+        std::vector<std::thread> pool;
+        try {
+            for (int u = 0; u < nth; ++u)
+                pool.emplace_back([&, u] { body(u, n * u / nth, n * (u + 1) / nth); });
+        } catch (...) {
+            for (auto& th : pool) th.join();
+            throw;
+        }
+        for (auto& th : pool) th.join();
+    };
+    on_threads([&](int u, int64_t i0, int64_t i1) {
+        int64_t* cu = pos.data() + (size_t)u * nc;
+        for (int64_t i = i0; i < i1; ++i) ++cu[labels[i]];
+    });
+    for (int64_t c = 0; c < nc; ++c) {
+        int64_t run = 0;
+        for (int u = 0; u < nth; ++u) {
+            int64_t& p = pos[(size_t)u * nc + c];
+            const int64_t k = p;
+            p = run;
+            run += k;
+        }
+        members[c].resize(run);
+    }
+    on_threads([&](int u, int64_t i0, int64_t i1) {
+        int64_t* pu = pos.data() + (size_t)u * nc;
+        for (int64_t i = i0; i < i1; ++i) {
+            const int64_t c = labels[i];
+            members[c][pu[c]++] = i;
+        }
+    });
+}
+
 struct DeviceGraph {
     si::StreamBuf<int64_t> ip;
     si::StreamBuf<int32_t> ix;
@@ -1000,13 +1053,8 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
             }
         } else {
             // embed_disconnected (src/multi_component.rs:16-74)
-            std::vector<std::vector<int64_t>> members(nc);
-            {
-                std::vector<int64_t> cnt(nc, 0);
-                for (int64_t i = 0; i < n; ++i) ++cnt[labels[i]];
-                for (int64_t c = 0; c < nc; ++c) members[c].reserve(cnt[c]);
-                for (int64_t i = 0; i < n; ++i) members[labels[i]].push_back(i);
-            }
+            std::vector<std::vector<int64_t>> members;
+            member_lists(labels, nc, members);
             std::vector<double> meta((size_t)nc * d, 0.0);
             const bool spectral_meta = nc > 2 * d;
             if (spectral_meta && !raw_data)
