@@ -215,10 +215,10 @@ void upload_graph(DeviceGraph& g, int64_t n, const int64_t* ip, const int32_t* i
     g.ip.alloc(n + 1, s);
     g.ix.alloc(nnz, s);
     g.w.alloc(nnz, s);
-    SI_HIP(hipMemcpyAsync(g.ip.get(), ip, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, s));
+    si::upload_to_device(g.ip.get(), ip, sizeof(int64_t) * (n + 1), s);
     if (nnz > 0) {
-        SI_HIP(hipMemcpyAsync(g.ix.get(), ix, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s));
-        SI_HIP(hipMemcpyAsync(g.w.get(), w, sizeof(float) * nnz, hipMemcpyHostToDevice, s));
+        si::upload_to_device(g.ix.get(), ix, sizeof(int32_t) * nnz, s);
+        si::upload_to_device(g.w.get(), w, sizeof(float) * nnz, s);
     }
     SI_HIP(hipStreamSynchronize(s));
 }

@@ -129,6 +129,11 @@ void workspace_release();
 // Per-thread page-locked host buffer of at least `bytes` (grow-only; released
 // with the workspace).  Its contents are valid until the next call.
 void* pinned_stage(size_t bytes);
+// Copy `bytes` of pageable host memory to the device, ordered on stream s.
+// Large copies go through a per-thread ring of page-locked chunks (filled by
+// host threads while the previous chunk's DMA runs), about twice the rate of a
+// pageable copy; src may be reused as soon as the call returns.
+void upload_to_device(void* dst, const void* src, size_t bytes, hipStream_t s);
 template <typename T>
 inline T* ws_get(int slot, size_t count) {
     return static_cast<T*>(workspace_get(slot, sizeof(T) * (count ? count : 1)));

@@ -1,8 +1,10 @@
 // solver_common.cpp -- FP64 building blocks shared by the solvers: residual
 // norms, Rayleigh-Ritz refinement and the Chebyshev filter.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <mutex>
+#include <thread>
 
 #include "si_host.hpp"
 
@@ -21,7 +23,79 @@ struct PinnedStage {
     size_t n = 0;
 };
 thread_local PinnedStage* g_pin = nullptr;
+// page-locked chunks of the staged uploads, each with the event of its last DMA
+struct UploadRing {
+    static constexpr int kSlots = 4;
+    static constexpr size_t kChunk = (size_t)16 << 20;
+    char* slot[kSlots] = {};
+    hipEvent_t ev[kSlots] = {};
+    bool pending[kSlots] = {};
+    bool ok = false;
+    UploadRing() {
+        for (int i = 0; i < kSlots; ++i) {
+            if (hipHostMalloc(reinterpret_cast<void**>(&slot[i]), kChunk, hipHostMallocDefault) !=
+                    hipSuccess ||
+                hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess)
+                return;
+        }
+        ok = true;
+    }
+    ~UploadRing() {
+        for (int i = 0; i < kSlots; ++i) {
+            if (ev[i]) {
+                if (pending[i]) (void)hipEventSynchronize(ev[i]);
+                (void)hipEventDestroy(ev[i]);
+            }
+            if (slot[i]) (void)hipHostFree(slot[i]);
+        }
+    }
+};
+thread_local UploadRing* g_ring = nullptr;
 }  // namespace
+
+void upload_to_device(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    constexpr size_t kStagedMin = (size_t)8 << 20;
+    if (bytes < kStagedMin) {
+        SI_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+        return;
+    }
+    if (!g_ring) g_ring = new UploadRing();
+    if (!g_ring->ok) {  // no page-locked memory: the runtime's pageable path
+        SI_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+        return;
+    }
+    UploadRing& r = *g_ring;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nth = (int)std::min(4u, hw);
+    const char* from = static_cast<const char*>(src);
+    char* to = static_cast<char*>(dst);
+    for (size_t off = 0, k = 0; off < bytes; off += UploadRing::kChunk, ++k) {
+        const size_t len = std::min(UploadRing::kChunk, bytes - off);
+        const int sl = (int)(k % UploadRing::kSlots);
+        if (r.pending[sl]) {
+            SI_HIP(hipEventSynchronize(r.ev[sl]));
+            r.pending[sl] = false;
+        }
+        // the chunk's host copy, split over nth threads (this one included)
+        char* pin = r.slot[sl];
+        auto part = [&](int t) {
+            const size_t a = len * t / nth, b = len * (t + 1) / nth;
+            std::memcpy(pin + a, from + off + a, b - a);
+        };
+        std::vector<std::thread> pool;
+        try {
+            for (int t = 1; t < nth; ++t) pool.emplace_back(part, t);
+        } catch (...) {
+            for (auto& th : pool) th.join();
+            throw;
+        }
+        part(0);
+        for (auto& th : pool) th.join();
+        SI_HIP(hipMemcpyAsync(to + off, pin, len, hipMemcpyHostToDevice, s));
+        SI_HIP(hipEventRecord(r.ev[sl], s));
+        r.pending[sl] = true;
+    }
+}
 
 void* pinned_stage(size_t bytes) {
     if (!g_pin) g_pin = new PinnedStage();
@@ -66,6 +140,8 @@ void* workspace_get(int slot, size_t bytes) {
 void workspace_release() {
     delete g_ws;
     g_ws = nullptr;
+    delete g_ring;
+    g_ring = nullptr;
     if (g_pin) {
         if (g_pin->p) (void)hipHostFree(g_pin->p);
         delete g_pin;
