@@ -222,7 +222,7 @@ void embed_output_device(const EmbedPlan& p, hipStream_t s) {
     const size_t o_sign = o_first + 8 * (size_t)m, o_f = o_sign + 8 * (size_t)m;
     const size_t o_meta = o_f + 8 * (size_t)nseg, o_gmax = o_meta + 8 * (size_t)m;
     const size_t bytes = o_gmax + 8;
-    DevBuf<char> arena(bytes);
+    StreamBuf<char> arena(bytes, s);
     char* base = arena.get();
     const Item* ditems = reinterpret_cast<const Item*>(base + o_items);
     int32_t* kind = reinterpret_cast<int32_t*>(base + o_kind);
@@ -291,7 +291,7 @@ void embed_output_device(const EmbedPlan& p, hipStream_t s) {
     hipLaunchKernelGGL(k_emb_scale_noise, grid_of(n * d), dim3(256), 0, s, n * d, coords, ex, nz,
                        out);
     SI_LAUNCH_CHECK();
-    to_host(p.out, out, (size_t)n * d, s);
+    download_to_host(p.out, out, sizeof(float) * (size_t)n * d, s);
 }
 
 }  // namespace si

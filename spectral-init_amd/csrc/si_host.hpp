@@ -134,6 +134,10 @@ void* pinned_stage(size_t bytes);
 // host threads while the previous chunk's DMA runs), about twice the rate of a
 // pageable copy; src may be reused as soon as the call returns.
 void upload_to_device(void* dst, const void* src, size_t bytes, hipStream_t s);
+// Copy `bytes` from the device to pageable host memory after the work already
+// on stream s, through the same ring of page-locked chunks (each chunk's host
+// copy overlaps the next chunks' DMA).  Returns when dst holds the data.
+void download_to_host(void* dst, const void* src, size_t bytes, hipStream_t s);
 template <typename T>
 inline T* ws_get(int slot, size_t count) {
     return static_cast<T*>(workspace_get(slot, sizeof(T) * (count ? count : 1)));

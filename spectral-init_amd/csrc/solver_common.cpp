@@ -137,6 +137,56 @@ void* workspace_get(int slot, size_t bytes) {
     return b.p;
 }
 
+void download_to_host(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    constexpr size_t kStagedMin = (size_t)8 << 20;
+    if (bytes >= kStagedMin && !g_ring) g_ring = new UploadRing();
+    if (bytes < kStagedMin || !g_ring->ok) {
+        SI_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+        SI_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    UploadRing& r = *g_ring;
+    const size_t chunk = UploadRing::kChunk;
+    const size_t nchunks = (bytes + chunk - 1) / chunk;
+    const char* from = static_cast<const char*>(src);
+    char* to = static_cast<char*>(dst);
+    auto enqueue = [&](size_t k) {
+        const int sl = (int)(k % UploadRing::kSlots);
+        if (r.pending[sl]) {
+            SI_HIP(hipEventSynchronize(r.ev[sl]));
+            r.pending[sl] = false;
+        }
+        const size_t off = k * chunk, len = std::min(chunk, bytes - off);
+        SI_HIP(hipMemcpyAsync(r.slot[sl], from + off, len, hipMemcpyDeviceToHost, s));
+        SI_HIP(hipEventRecord(r.ev[sl], s));
+        r.pending[sl] = true;
+    };
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nth = (int)std::min(4u, hw);
+    for (size_t k = 0; k < std::min<size_t>(nchunks, UploadRing::kSlots); ++k) enqueue(k);
+    for (size_t k = 0; k < nchunks; ++k) {
+        const int sl = (int)(k % UploadRing::kSlots);
+        SI_HIP(hipEventSynchronize(r.ev[sl]));
+        r.pending[sl] = false;
+        const size_t off = k * chunk, len = std::min(chunk, bytes - off);
+        const char* pin = r.slot[sl];
+        auto part = [&](int t) {
+            const size_t a = len * t / nth, b = len * (t + 1) / nth;
+            std::memcpy(to + off + a, pin + a, b - a);
+        };
+        std::vector<std::thread> pool;
+        try {
+            for (int t = 1; t < nth; ++t) pool.emplace_back(part, t);
+        } catch (...) {
+            for (auto& th : pool) th.join();
+            throw;
+        }
+        part(0);
+        for (auto& th : pool) th.join();
+        if (k + UploadRing::kSlots < nchunks) enqueue(k + UploadRing::kSlots);
+    }
+}
+
 void workspace_release() {
     delete g_ws;
     g_ws = nullptr;
