@@ -37,6 +37,10 @@ si_status si_test_fill_normal(si_context* ctx, double* x, int64_t n, int32_t k, 
  * column 2p, sine in column 2p + 1). */
 si_status si_test_fill_normal_f32(si_context* ctx, float* x, int64_t n, int32_t k, uint64_t seed,
                                   uint32_t stream);
+/* host bytes to the device and back through the library's staged copies
+ * (page-locked chunk ring for >= 8 MiB, the runtime's pageable copy below):
+ * out = in, byte for byte, via a device buffer of `bytes`. */
+si_status si_test_staged_roundtrip(si_context* ctx, const void* in, void* out, int64_t bytes);
 /* the FP32-bulk Chebyshev-filtered block solver alone (escalation level 1). */
 si_status si_test_fast_solve(si_context* ctx, const si_laplacian* lap, int32_t n_components,
                              uint64_t seed, const si_config* cfg, double* evals, double* evecs,

@@ -114,6 +114,15 @@ si_status si_test_fill_normal_f32(si_context* ctx, float* x, int64_t n, int32_t 
     });
 }
 
+si_status si_test_staged_roundtrip(si_context* ctx, const void* in, void* out, int64_t bytes) {
+    if (!ctx || !in || !out || bytes < 1) return SI_ERR_INVALID_ARGUMENT;
+    return tguard(ctx, [&] {
+        si::StreamBuf<char> d((size_t)bytes, ctx->c.stream);
+        si::upload_to_device(d.get(), in, (size_t)bytes, ctx->c.stream);
+        si::download_to_host(out, d.get(), (size_t)bytes, ctx->c.stream);
+    });
+}
+
 si_status si_test_fast_solve(si_context* ctx, const si_laplacian* lap, int32_t n_components,
                              uint64_t seed, const si_config* cfg, double* evals, double* evecs,
                              si_report* rep) {

@@ -42,7 +42,7 @@ EXPORTED = [
     "si_dense_evd",
     "si_spectral_init",
     "si_test_gram", "si_test_block_mul", "si_test_small_sym_eig", "si_test_small_gen_eig",
-    "si_test_fill_normal", "si_test_fill_normal_f32", "si_test_fast_solve", "si_test_batch_graphs", "si_test_batch_plan",
+    "si_test_fill_normal", "si_test_fill_normal_f32", "si_test_staged_roundtrip", "si_test_fast_solve", "si_test_batch_graphs", "si_test_batch_plan",
     "si_test_sinv_dense",
     "si_test_spmm_shapes",
     "si_test_band_solve", "si_test_rcm_order", "si_test_knn_path",
@@ -214,6 +214,7 @@ def lib():
         L.si_test_small_gen_eig.argtypes = [vp, i32, vp, vp, vp, vp]
         L.si_test_fill_normal.argtypes = [vp, vp, i64, i32, u64, C.c_uint32]
         L.si_test_fill_normal_f32.argtypes = [vp, vp, i64, i32, u64, C.c_uint32]
+        L.si_test_staged_roundtrip.argtypes = [vp, vp, vp, i64]
         L.si_test_batch_plan.argtypes = [vp] + [i32] * 8 + [dbl] + [vp] * 12
         L.si_test_sinv_dense.argtypes = [i32]
         L.si_test_batch_graphs.argtypes = [i32, C.POINTER(i64), C.POINTER(i64)]
@@ -580,6 +581,15 @@ def fill_normal_f32(ctx: Context, x, seed: int, stream: int):
     ctx._check(lib().si_test_fill_normal_f32(ctx.handle, _ptr(x), x.shape[0], x.shape[1], seed,
                                              stream))
     return x
+
+
+def staged_roundtrip(ctx: Context, data: np.ndarray) -> np.ndarray:
+    """Testing seam: host bytes up to the device and back through the staged copies."""
+    src = np.ascontiguousarray(data)
+    out = np.empty_like(src)
+    ctx._check(lib().si_test_staged_roundtrip(ctx.handle, src.ctypes.data, out.ctypes.data,
+                                              src.nbytes))
+    return out
 
 
 class SpmmProfiler:

@@ -417,3 +417,14 @@ def test_operator_apply_on_host_arrays(ctx, name, k):
     x = np.random.default_rng(k).standard_normal((lap.n, k))
     y = si.operator_apply_host(lap, x)
     np.testing.assert_allclose(y, O.spmm(L, x), rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("nbytes", [1, 1000, (8 << 20) - 1, 8 << 20, (16 << 20) + 3,
+                                    (70 << 20) + 5])
+def test_staged_copies_round_trip(ctx, nbytes):
+    """Host bytes up to the device and back: below 8 MiB the runtime's pageable
+    copies, from 8 MiB a ring of four 16 MiB page-locked chunks (one chunk, a
+    ragged last chunk, and more chunks than the ring holds)."""
+    print(f"case staged round trip {nbytes} bytes", flush=True)
+    data = np.random.default_rng(nbytes % 1000).integers(0, 256, nbytes, dtype=np.uint8)
+    np.testing.assert_array_equal(si.staged_roundtrip(ctx, data), data)
