@@ -323,8 +323,10 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
     phase("degrees");
     const float* w_vals = d_w;
     float* d_w2 = nullptr;
+    StreamBuf<float> w2buf;  // the weights in the new order
     if (reorder && n > 1) {
-        SI_HIP(hipMalloc(&d_w2, sizeof(float) * (nnz > 0 ? nnz : 1)));
+        w2buf.alloc(nnz > 0 ? nnz : 1, s);
+        d_w2 = w2buf.get();
         SI_HIP(hipMalloc(&L.perm, sizeof(int32_t) * n));
         SI_HIP(hipMalloc(&L.iperm, sizeof(int32_t) * n));
         if (L.symmetric) {
@@ -388,14 +390,12 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
                            d_w2, s);
         phase("permuted csr");
         // node vectors follow the permutation (computed in input order above)
-        double* tmp = nullptr;
-        SI_HIP(hipMalloc(&tmp, sizeof(double) * n));
+        StreamBuf<double> tmp(n, s);
         for (double* vec : {L.deg, L.sqrt_deg, L.isd}) {
-            gather_rows_f64(vec, 1, tmp, 1, n, 1, L.perm, s);
-            SI_HIP(hipMemcpyAsync(vec, tmp, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            gather_rows_f64(vec, 1, tmp.get(), 1, n, 1, L.perm, s);
+            SI_HIP(hipMemcpyAsync(vec, tmp.get(), sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         }
         SI_HIP(hipStreamSynchronize(s));
-        hipFree(tmp);
         w_vals = d_w2;
         phase("upload permuted");
     }
@@ -408,7 +408,6 @@ void laplacian_build_device(DevLaplacian& L, int64_t n, const int64_t* d_indptr,
     L.n_long = nlong;
     phase("values");
     hipFree(d_flags);
-    if (d_w2) hipFree(d_w2);
 }
 
 // A Laplacian given by its FP64 off-diagonal values (unit diagonal implied),

@@ -477,7 +477,7 @@ void rcm_order_labeled(int64_t n, const int64_t* rp, const int32_t* ix, const in
     StreamBuf<int32_t> queue(n, s), mark(n, s), lvl(n, s), cnt(n, s), loc(n, s),
         part(kLevelGrid, s), bnd(n + 2, s), maxlev(nc, s);
     StreamBuf<unsigned long long> key(n, s), skey(n, s), best(nc, s);
-    DevBuf<int> flags(2);  // [0] a group above kGroupLds, [1] the longest row
+    StreamBuf<int> flags(2, s);  // [0] a group above kGroupLds, [1] the longest row
     hipLaunchKernelGGL(k_fill_u64, grid_of(n), dim3(256), 0, s, key.get(), n,
                        (unsigned long long)kNoClaim);
     hipLaunchKernelGGL(k_fill_i32, grid_of(n), dim3(256), 0, s, mark.get(), n, 0);
