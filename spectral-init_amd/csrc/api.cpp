@@ -153,7 +153,8 @@ struct CentroidJob {
 // members[c] = the nodes of component c in increasing order (a counting sort
 // by label; on host threads, each over a contiguous range of nodes, when the
 // per-thread counts stay small).
-void member_lists(const std::vector<int64_t>& labels, int64_t nc,
+template <typename Label>
+void member_lists(const std::vector<Label>& labels, int64_t nc,
                   std::vector<std::vector<int64_t>>& members) {
     const int64_t n = (int64_t)labels.size();
     members.assign(nc, {});
@@ -425,6 +426,9 @@ void solve_components_batched(si::Context& cx, const si::DevLaplacian& L,
                     last_row[c] = h[nc + c];
                 }
         } else {
+            // the host labels (a build without device labels)
+            SI_CHECK((int64_t)labels.size() == n, si::kErrInternal,
+                     "batched components: host labels missing");
             std::vector<int32_t> perm(n);
             if (L.perm) si::to_host(perm.data(), L.perm, n, s);
             else std::iota(perm.begin(), perm.end(), 0);
@@ -1006,13 +1010,15 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
                            /*tune_widths=*/0, &lap_ms);
             phase("operator");
         }
-        std::vector<int64_t> labels;
+        std::vector<int64_t> labels;  // the host sweep's labels
+        std::vector<int32_t> lab32;   // the build's device labels, read back for several components
         int64_t nc = 0;
         if (whole.L.n_comp > 0) {
             nc = whole.L.n_comp;
-            std::vector<int32_t> lab32(n);
-            si::to_host(lab32.data(), whole.L.comp, n, ctx->c.stream);
-            labels.assign(lab32.begin(), lab32.end());
+            if (nc > 1) {
+                lab32.resize(n);
+                si::to_host(lab32.data(), whole.L.comp, n, ctx->c.stream);
+            }
         } else {
             nc = find_components(n, indptr, indices, labels);
         }
@@ -1054,7 +1060,8 @@ si_status si_spectral_init(si_context* ctx, int64_t n, const int64_t* indptr,
         } else {
             // embed_disconnected (src/multi_component.rs:16-74)
             std::vector<std::vector<int64_t>> members;
-            member_lists(labels, nc, members);
+            if (lab32.empty()) member_lists(labels, nc, members);
+            else member_lists(lab32, nc, members);
             std::vector<double> meta((size_t)nc * d, 0.0);
             const bool spectral_meta = nc > 2 * d;
             if (spectral_meta && !raw_data)
