@@ -110,6 +110,20 @@ __global__ __launch_bounds__(256) void k_ord_assign(int64_t n, const float* __re
         if (lc[t]) atomicAdd(&counts[t], lc[t]);
 }
 
+// Lloyd update: each non-empty cluster's centroid = its fixed-point sum over
+// its count (FP64 division, rounded to FP32); an empty cluster keeps its
+// centroid.  One thread per centroid entry.
+__global__ void k_ord_update(const unsigned long long* __restrict__ sums,
+                             const unsigned long long* __restrict__ counts,
+                             float* __restrict__ cent) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= kK * kE) return;
+    const unsigned long long cnt = counts[t / kE];
+    if (!cnt) return;
+    const double sum = (double)(long long)sums[t];
+    cent[t] = (float)(sum / ((double)kFix * (double)cnt));
+}
+
 __global__ void k_ord_iota(int64_t n, int32_t* __restrict__ a) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = (int32_t)i;
@@ -206,24 +220,14 @@ void cluster_order_device(const DevLaplacian& L, int32_t* perm, int32_t* iperm, 
     hipLaunchKernelGGL(k_ord_seed, grid_of(kK * kE), dim3(256), 0, s, n, pts, cent.get());
     SI_LAUNCH_CHECK();
     const unsigned blocks = (unsigned)std::min<int64_t>(1024, ceil_div(n, 256));
-    std::vector<unsigned long long> hacc(kK * kE + kK);
-    std::vector<float> hc(kK * kE);
-    to_host(hc.data(), cent.get(), hc.size(), s);
+    // Lloyd iterations on the device, no host round trip in between
     for (int it = 0; it < kIters; ++it) {
         SI_HIP(hipMemsetAsync(acc.get(), 0, sizeof(unsigned long long) * acc.n, s));
         hipLaunchKernelGGL(k_ord_assign, dim3(blocks), dim3(256), 0, s, n, pts, cent.get(),
                            lab.get(), 1, acc.get(), acc.get() + kK * kE);
+        hipLaunchKernelGGL(k_ord_update, grid_of(kK * kE), dim3(256), 0, s, acc.get(),
+                           acc.get() + kK * kE, cent.get());
         SI_LAUNCH_CHECK();
-        to_host(hacc.data(), acc.get(), hacc.size(), s);
-        for (int k = 0; k < kK; ++k) {
-            const unsigned long long cnt = hacc[(size_t)kK * kE + k];
-            if (!cnt) continue;  // an empty cluster keeps its centroid
-            for (int j = 0; j < kE; ++j) {
-                const double sum = (double)(long long)hacc[(size_t)k * kE + j];
-                hc[(size_t)k * kE + j] = (float)(sum / ((double)kFix * (double)cnt));
-            }
-        }
-        to_device(cent.get(), hc.data(), hc.size(), s);
     }
     hipLaunchKernelGGL(k_ord_assign, dim3(blocks), dim3(256), 0, s, n, pts, cent.get(), lab.get(),
                        0, acc.get(), acc.get() + kK * kE);
