@@ -14,7 +14,8 @@
 
 namespace si {
 
-// SI_TRACE=1: wall time this thread spent in DevBuf's hipMalloc / hipFree.
+// SI_TRACE=1: wall time this thread spent allocating and freeing device
+// buffers (DevBuf's hipMalloc / hipFree, StreamBuf's stream-ordered pair).
 inline double& devbuf_alloc_ms() {
     thread_local double ms = 0.0;
     return ms;
